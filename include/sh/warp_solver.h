@@ -1,0 +1,220 @@
+/*
+ * StarHub warp-field Gauss-Newton solver for AMD Instinct MI355X (gfx950).
+ *
+ * C-ABI drop-in boundary.  Plain pointers and sizes only; every device pointer
+ * is caller-owned HBM that must stay valid for the duration of the call that
+ * consumes it (the reference's *4Solver structs are non-owning views in the
+ * same way, libs/star/opt/solver_types.h:21-120).  Library-owned outputs stay
+ * valid until the next set_inputs / solve on the same handle.
+ *
+ * Citations are to the StarHub sources (paths relative to the repository root).
+ * Status convention: every entry point returns an sh_status; none calls exit()
+ * (the reference exits / throws on errors, libs/star/common/containers/safe_call.cuh:54-60).
+ */
+#ifndef SH_WARP_SOLVER_H
+#define SH_WARP_SOLVER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *sh_stream_t; /* hipStream_t; NULL = default stream */
+
+typedef enum sh_status
+{
+    SH_OK = 0,
+    SH_E_ARG = 1,      /* invalid argument / size */
+    SH_E_CAPACITY = 2, /* input exceeds the capacity fixed at create time (GBufferArray.h:93-100) */
+    SH_E_HIP = 3,      /* HIP runtime error */
+    SH_E_NAN = 4,      /* non-finite right-hand side (WarpSolverPCG.cu:19-31 exits instead) */
+    SH_E_STATE = 5,    /* call out of order (e.g. solve before set_inputs) */
+    SH_E_NOMEM = 6     /* device allocation failed */
+} sh_status;
+
+/* Storage of the per-pixel residual/Jacobian stream. */
+typedef enum sh_stream_precision
+{
+    SH_STREAM_FP32 = 0,
+    SH_STREAM_FP16 = 1 /* FP16 storage, FP32 accumulation */
+} sh_stream_precision;
+
+/* How the PCG applies A = J^T W J + damping. */
+typedef enum sh_spmv_mode
+{
+    SH_SPMV_MATERIALIZED = 0, /* 6x6 block-CSR (JtJMaterializerBinBlockCSR.cu:8-74) */
+    SH_SPMV_MATRIX_FREE = 1,  /* J^T (W (J x)) from the term streams (JtJMaterializerSanityCheck.cu:47-79) */
+    SH_SPMV_AUTO = 2          /* autotuner picks */
+} sh_spmv_mode;
+
+/* Dual quaternion as stored by the reference: two float4 (x, y, z, w),
+ * real part q0 first (libs/star/math/DualQuaternion.hpp:206). */
+typedef struct sh_dual_quat
+{
+    float q0[4];
+    float q1[4];
+} sh_dual_quat;
+
+/* Construction-time capacities and solver constants.  Replaces the
+ * ConfigParser singleton + compile-time constants the reference reads
+ * (examples/mono_star/common/Constants.cpp:14-34, PenaltyConstants.cu:8-32,
+ * libs/star/geometry/constants.h:6-14). */
+typedef struct sh_solver_config
+{
+    uint32_t max_width, max_height; /* solver image capacity (downsampled size) */
+    uint32_t max_nodes;             /* <= 4096: node-pair key i*4096+j (solver_encode.h:14) */
+    uint32_t knn;                   /* K nearest nodes per surfel / node: 4 or 8 */
+    uint32_t gn_iterations;         /* reference default 6 (Constants.cpp:33) */
+    uint32_t pcg_iterations;        /* reference default 10 (WarpSolver.cu:303) */
+    float lambda_depth;             /* 0.1 */
+    float lambda_flow;              /* 0.0 */
+    float lambda_reg;               /* 1.0 (2.3 for K=4) */
+    float lambda_node_translation;  /* 0.0 */
+    float lambda_feature;           /* 1.0; the feature term has a zero Jacobian and no terms here */
+    float damping;                  /* 1e-3 added to the JtJ diagonal (PreconditionerRhsBuilder.cu:249-252) */
+    uint32_t stream_precision;      /* sh_stream_precision */
+    uint32_t spmv_mode;             /* sh_spmv_mode */
+} sh_solver_config;
+
+/* Pinhole camera of the solver images (ConfigParser.cu:231-256). */
+typedef struct sh_camera
+{
+    uint32_t width, height;
+    float fx, fy, cx, cy;
+} sh_camera;
+
+/* Measure4Solver (solver_types.h:52-58): measured maps, camera frame,
+ * float4 per pixel, row-major width*height, border reads return zero. */
+typedef struct sh_measure_maps
+{
+    const float *vertex_confid; /* float4 */
+    const float *normal_radius; /* float4 */
+} sh_measure_maps;
+
+/* Render4Solver (solver_types.h:62-68): reference maps in the world frame. */
+typedef struct sh_render_maps
+{
+    const float *reference_vertex; /* float4 */
+    const float *reference_normal; /* float4 */
+    const uint32_t *index_map;     /* surfel index, 0xffffffff = empty */
+} sh_render_maps;
+
+/* Geometry4Solver (libs/star/geometry/surfel/SurfelGeometry.h:76-82). */
+typedef struct sh_geometry
+{
+    const uint16_t *surfel_knn;           /* num_surfels * K */
+    const float *surfel_knn_spatial_weight; /* num_surfels * K */
+    const float *surfel_knn_connect_weight; /* num_surfels * K */
+    uint32_t num_surfels;
+} sh_geometry;
+
+/* NodeGraph4Solver (libs/star/geometry/node_graph/NodeGraph.h:89-99). */
+typedef struct sh_node_graph
+{
+    const float *reference_node_coords;    /* float4 * num_nodes */
+    const uint16_t *node_graph;            /* ushort3 (i, j, k) * num_edges */
+    const float *node_knn_connect_weight;  /* num_nodes * K */
+    const uint16_t *node_knn;              /* num_nodes * K (node-motion term) */
+    const float *node_knn_spatial_weight;  /* num_nodes * K */
+    uint32_t num_nodes;
+    uint32_t num_edges;
+    float node_radius_square;
+} sh_node_graph;
+
+/* NodeFlow4Solver (solver_types.h:108-112): may be NULL pointer (zeros). */
+typedef struct sh_node_flow
+{
+    const float *node_motion_pred; /* float4 * num_nodes: (x, y, z, confidence) */
+    uint32_t num_nodes;
+} sh_node_flow;
+
+/* OpticalFlow4Solver (solver_types.h:116-120): float2 per pixel, may be NULL. */
+typedef struct sh_flow_map
+{
+    const float *flow;
+} sh_flow_map;
+
+/* KeyPoint4Solver (solver_types.h:82-96).  The reference's feature term has a
+ * zero Jacobian (PreconditionerRhsBuilder.cu:215-219); matches are accepted
+ * for API compatibility and contribute nothing. */
+typedef struct sh_keypoints
+{
+    uint32_t num_matches;
+} sh_keypoints;
+
+typedef struct sh_solver sh_solver;
+
+const char *sh_version(void);
+const char *sh_status_string(int status);
+
+/* --------------------------------------------------------------- one solve */
+/* WarpSolver() + AllocateBuffer (WarpSolver.cu:50-103): all capacity is
+ * allocated here. */
+int sh_solver_create(const sh_solver_config *config, sh_solver **out);
+int sh_solver_destroy(sh_solver *solver);
+
+/* WarpSolver::SetSolverInputs (WarpSolver.cu:125-155); cam2world is the
+ * row-major 4x4 camera-to-world extrinsic (inverted here, as at :151-154). */
+int sh_solver_set_inputs(sh_solver *solver, const sh_camera *camera, const sh_measure_maps *measure,
+                         const sh_render_maps *render, const sh_geometry *geometry, const sh_node_graph *node_graph,
+                         const sh_node_flow *node_flow, const sh_flow_map *flow, const sh_keypoints *keypoints,
+                         const float cam2world[16]);
+
+/* WarpSolver::SolveStreamed (WarpSolver.cu:260-279): index build, then
+ * gn_iterations x (terms, JtJ/JtR, PCG, warp update) starting from identity.
+ * Work is enqueued on `stream`; the call returns after the stream has been
+ * synchronised (OptimizationProcessorWarpSolver.cu:51). */
+int sh_solver_solve(sh_solver *solver, sh_stream_t stream);
+
+/* WarpSolver::SolvedNodeSE3 / OptimizationProcessorWarpSolver::SolvedSE3
+ * (WarpSolver.h:45, OptimizationProcessorWarpSolver.h:28-30): device view of
+ * num_nodes dual quaternions. */
+int sh_solver_node_se3(const sh_solver *solver, const sh_dual_quat **device_view, uint32_t *num_nodes);
+
+/* Sizes of the last index build: number of valid pixels P, unique node pairs U
+ * (upper triangle), node->term entries. */
+int sh_solver_sizes(const sh_solver *solver, uint32_t *num_pixels, uint32_t *num_pairs, uint32_t *num_node_terms);
+
+/* Device views of internal arrays for parity checks (names in DESIGN.md):
+ * returns the device pointer and element count. */
+int sh_solver_debug_view(const sh_solver *solver, const char *name, const void **device_ptr, size_t *count);
+
+/* Per-GN-iteration total weighted cost (FP64), host copy, count = gn_iterations+1. */
+int sh_solver_cost_history(const sh_solver *solver, double *out, uint32_t capacity, uint32_t *count);
+
+/* ApplySpMVBase<6>::ApplySpMV (libs/star/pcg_solver/ApplySpMVBase.h:19-37):
+ * y = A x with the system of the last GN iteration (materialised or matrix-free). */
+int sh_solver_apply_jtj(sh_solver *solver, const float *x, float *y, uint32_t matrix_free, sh_stream_t stream);
+
+/* ------------------------------------------------------- input producers */
+/* SkinnerKernel (libs/star/geometry/node_graph/Skinner.cu:10-94): K nearest
+ * nodes per surfel with normalised Gaussian weights, connect weight 1. */
+int sh_skin_surfels(const float *surfel_vertex4, uint32_t num_surfels, const float *node_coords4, uint32_t num_nodes,
+                    uint32_t knn, float node_radius_square, uint16_t *surfel_knn, float *spatial_weight,
+                    float *connect_weight, sh_stream_t stream);
+
+/* NodeGraph::BuildNodeGraphFromScratch (NodeGraph.cu:10-118, 339-365):
+ * FP16-rounded pairwise distances, K-NN (self included), weights, edge list
+ * (i, knn[i][k], k) of num_nodes*K ushort3, connect weights 1. */
+int sh_build_node_graph(const float *node_coords4, uint32_t num_nodes, uint32_t knn, float node_radius_square,
+                        uint16_t *node_knn, float *node_spatial_weight, uint16_t *node_graph,
+                        float *node_connect_weight, sh_stream_t stream);
+
+/* ------------------------------------------------- batched uneven solves */
+typedef struct sh_batch sh_batch;
+int sh_batch_create(const sh_solver_config *config, uint32_t max_solves, sh_batch **out);
+int sh_batch_destroy(sh_batch *batch);
+int sh_batch_set_inputs(sh_batch *batch, uint32_t slot, const sh_camera *camera, const sh_measure_maps *measure,
+                        const sh_render_maps *render, const sh_geometry *geometry, const sh_node_graph *node_graph,
+                        const sh_node_flow *node_flow, const sh_flow_map *flow, const sh_keypoints *keypoints,
+                        const float cam2world[16]);
+/* Solves slots [0, num_solves) together: one launch per GN phase covers all. */
+int sh_batch_solve(sh_batch *batch, uint32_t num_solves, sh_stream_t stream);
+int sh_batch_node_se3(const sh_batch *batch, uint32_t slot, const sh_dual_quat **device_view, uint32_t *num_nodes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SH_WARP_SOLVER_H */

@@ -1,0 +1,1320 @@
+// StarHub warp-field Gauss-Newton step: CPU restatement (the ORACLE).
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP path: only the
+// tests, the smoke check and the CPU-baseline leg of the benchmark may build or
+// call it.  The product library never links it.
+//
+// Templated on the arithmetic type T:
+//   T = double : the FP64 oracle the GPU warp field is compared against;
+//   T = float  : the FP32 restatement of the reference GPU arithmetic, run with
+//                OpenMP on the host cores as the CPU baseline.
+// Inputs are always the FP32 arrays the reference consumes; every function
+// cites the reference file:line it restates (paths relative to the StarHub
+// source root).
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace sho
+{
+// ---------------------------------------------------------------- small math
+template <class T>
+struct V3
+{
+    T x, y, z;
+};
+template <class T>
+inline V3<T> operator+(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T>
+inline V3<T> operator-(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T>
+inline V3<T> operator*(T s, V3<T> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class T>
+inline T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T>
+inline V3<T> cross(V3<T> a, V3<T> b)
+{
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// Row-major 3x3 (the reference stores column-major, the arithmetic is identical:
+// libs/star/math/device_mat.h:10-181).
+template <class T>
+struct M33
+{
+    T m[3][3];
+    static M33 identity()
+    {
+        M33 r{};
+        r.m[0][0] = r.m[1][1] = r.m[2][2] = T(1);
+        return r;
+    }
+};
+template <class T>
+inline V3<T> mul(const M33<T> &A, V3<T> v)
+{
+    return {A.m[0][0] * v.x + A.m[0][1] * v.y + A.m[0][2] * v.z,
+            A.m[1][0] * v.x + A.m[1][1] * v.y + A.m[1][2] * v.z,
+            A.m[2][0] * v.x + A.m[2][1] * v.y + A.m[2][2] * v.z};
+}
+template <class T>
+inline V3<T> mulT(const M33<T> &A, V3<T> v) // A^T v  (device_mat.h:150-156)
+{
+    return {A.m[0][0] * v.x + A.m[1][0] * v.y + A.m[2][0] * v.z,
+            A.m[0][1] * v.x + A.m[1][1] * v.y + A.m[2][1] * v.z,
+            A.m[0][2] * v.x + A.m[1][2] * v.y + A.m[2][2] * v.z};
+}
+template <class T>
+inline M33<T> mul(const M33<T> &A, const M33<T> &B)
+{
+    M33<T> C;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            C.m[i][j] = A.m[i][0] * B.m[0][j] + A.m[i][1] * B.m[1][j] + A.m[i][2] * B.m[2][j];
+    return C;
+}
+template <class T>
+struct M34
+{
+    M33<T> R;
+    V3<T> t;
+};
+template <class T>
+inline M34<T> mul(const M34<T> &A, const M34<T> &B) // device_mat.h:197-203
+{
+    return {mul(A.R, B.R), mul(A.R, B.t) + A.t};
+}
+
+// Quaternion, (w, x, y, z)   libs/star/math/Quaternion.hpp
+template <class T>
+struct Q
+{
+    T w, x, y, z;
+};
+template <class T>
+inline Q<T> qmul(Q<T> a, Q<T> b) // Quaternion.hpp:139-148
+{
+    return {a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z,
+            a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+            a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+            a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w};
+}
+template <class T>
+inline Q<T> qconj(Q<T> a) { return {a.w, -a.x, -a.y, -a.z}; }
+template <class T>
+inline T qdot(Q<T> a, Q<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+template <class T>
+inline Q<T> qscale(T s, Q<T> a) { return {s * a.w, s * a.x, s * a.y, s * a.z}; }
+template <class T>
+inline Q<T> qadd(Q<T> a, Q<T> b) { return {a.w + b.w, a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T>
+inline Q<T> qsub(Q<T> a, Q<T> b) { return {a.w - b.w, a.x - b.x, a.y - b.y, a.z - b.z}; }
+
+template <class T>
+inline M33<T> quat_to_rot(Q<T> q) // Quaternion.hpp:96-111 (rotation_matrix(false))
+{
+    M33<T> r;
+    r.m[0][0] = 1 - 2 * q.y * q.y - 2 * q.z * q.z;
+    r.m[0][1] = 2 * q.x * q.y - 2 * q.z * q.w;
+    r.m[0][2] = 2 * q.x * q.z + 2 * q.y * q.w;
+    r.m[1][0] = 2 * q.x * q.y + 2 * q.z * q.w;
+    r.m[1][1] = 1 - 2 * q.x * q.x - 2 * q.z * q.z;
+    r.m[1][2] = 2 * q.y * q.z - 2 * q.x * q.w;
+    r.m[2][0] = 2 * q.x * q.z - 2 * q.y * q.w;
+    r.m[2][1] = 2 * q.y * q.z + 2 * q.x * q.w;
+    r.m[2][2] = 1 - 2 * q.x * q.x - 2 * q.y * q.y;
+    return r;
+}
+
+template <class T>
+inline Q<T> rot_to_quat(const M33<T> &R) // Quaternion.hpp:17-52 (trace branch)
+{
+    Q<T> q;
+    const T m00 = R.m[0][0], m01 = R.m[0][1], m02 = R.m[0][2];
+    const T m10 = R.m[1][0], m11 = R.m[1][1], m12 = R.m[1][2];
+    const T m20 = R.m[2][0], m21 = R.m[2][1], m22 = R.m[2][2];
+    const T tr = m00 + m11 + m22;
+    if (tr > 0)
+    {
+        const T s = std::sqrt(tr + T(1)) * 2;
+        q.w = s * T(0.25);
+        q.x = (m21 - m12) / s;
+        q.y = (m02 - m20) / s;
+        q.z = (m10 - m01) / s;
+    }
+    else if (m00 > m11 && m00 > m22)
+    {
+        const T s = std::sqrt(T(1) + m00 - m11 - m22) * 2;
+        q.w = (m21 - m12) / s;
+        q.x = T(0.25) * s;
+        q.y = (m01 + m10) / s;
+        q.z = (m02 + m20) / s;
+    }
+    else if (m11 > m22)
+    {
+        const T s = std::sqrt(T(1) + m11 - m00 - m22) * 2;
+        q.w = (m02 - m20) / s;
+        q.x = (m01 + m10) / s;
+        q.y = T(0.25) * s;
+        q.z = (m12 + m21) / s;
+    }
+    else
+    {
+        const T s = std::sqrt(T(1) + m22 - m00 - m11) * 2;
+        q.w = (m10 - m01) / s;
+        q.x = (m02 + m20) / s;
+        q.y = (m12 + m21) / s;
+        q.z = T(0.25) * s;
+    }
+    return q;
+}
+
+// Dual quaternion (q0 = real, q1 = dual)   libs/star/math/DualQuaternion.hpp:54-207
+template <class T>
+struct DQ
+{
+    Q<T> q0, q1;
+};
+
+template <class T>
+inline DQ<T> dq_normalized(DQ<T> d) // DualQuaternion.hpp:150-156
+{
+    const T inv = T(1) / std::sqrt(qdot(d.q0, d.q0));
+    d.q0 = qscale(inv, d.q0);
+    d.q1 = qscale(inv, d.q1);
+    d.q1 = qsub(d.q1, qscale(qdot(d.q0, d.q1), d.q0));
+    return d;
+}
+
+// se3_matrix(): normalise, rotation from q0 without re-normalising,
+// t = vec(2 q1 q0*)   DualQuaternion.hpp:182-189
+template <class T>
+inline M34<T> dq_se3(DQ<T> d)
+{
+    d = dq_normalized(d);
+    const Q<T> tp = qmul(qscale(T(2), d.q1), qconj(d.q0));
+    return {quat_to_rot(d.q0), {tp.x, tp.y, tp.z}};
+}
+
+// operator mat34(): normalised copy, q0.matrix() re-normalises q0   DualQuaternion.hpp:169-179
+template <class T>
+inline M34<T> dq_to_mat34(DQ<T> d)
+{
+    d = dq_normalized(d);
+    Q<T> qn = d.q0;
+    const T inv = T(1) / std::sqrt(qdot(qn, qn));
+    qn = qscale(inv, qn);
+    const Q<T> tp = qmul(qscale(T(2), d.q1), qconj(d.q0));
+    return {quat_to_rot(qn), {tp.x, tp.y, tp.z}};
+}
+
+// DualQuaternion(const mat34&) = vec_part * rot_part   DualQuaternion.hpp:59-66
+template <class T>
+inline DQ<T> dq_from_mat34(const M34<T> &M)
+{
+    const Q<T> qr = rot_to_quat(M.R);
+    const Q<T> tv{T(0), T(0.5) * M.t.x, T(0.5) * M.t.y, T(0.5) * M.t.z};
+    // (1 + e tv) * (qr + e 0) = qr + e (tv * qr + 1 * 0)
+    const Q<T> zero{0, 0, 0, 0};
+    return {qmul(Q<T>{1, 0, 0, 0}, qr), qadd(qmul(tv, qr), qmul(Q<T>{1, 0, 0, 0}, zero))};
+}
+
+// Rodrigues exponential of a twist; identity rotation when |w|_1 < 1e-4
+// examples/mono_star/opt/SolverIterationData.cu:41-68
+template <class T>
+inline M34<T> twist_exp(V3<T> w, V3<T> t)
+{
+    M34<T> S;
+    if (std::fabs(w.x) + std::fabs(w.y) + std::fabs(w.z) < T(1e-4f))
+    {
+        S.R = M33<T>::identity();
+    }
+    else
+    {
+        const T angle = std::sqrt(dot(w, w));
+        const V3<T> a = (T(1) / angle) * w;
+        const T c = std::cos(angle), s = std::sin(angle), tt = T(1) - c;
+        S.R.m[0][0] = tt * a.x * a.x + c;
+        S.R.m[0][1] = tt * a.x * a.y - a.z * s;
+        S.R.m[0][2] = tt * a.x * a.z + a.y * s;
+        S.R.m[1][0] = tt * a.x * a.y + a.z * s;
+        S.R.m[1][1] = tt * a.y * a.y + c;
+        S.R.m[1][2] = tt * a.y * a.z - a.x * s;
+        S.R.m[2][0] = tt * a.x * a.z - a.y * s;
+        S.R.m[2][1] = tt * a.y * a.z + a.x * s;
+        S.R.m[2][2] = tt * a.z * a.z + c;
+    }
+    S.t = t;
+    return S;
+}
+
+// ------------------------------------------------------------- FP16 rounding
+// IEEE binary16 round-to-nearest-even of a finite or infinite float, returned as
+// float.  Restates __float2half_rn + __half2float used by
+// libs/star/geometry/node_graph/NodeGraph.cu:35,71.
+inline float round_to_half(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    const uint32_t sign = u & 0x80000000u;
+    const uint32_t ax = u & 0x7fffffffu;
+    if (ax >= 0x7f800000u)
+        return f; // inf / nan
+    float a;
+    std::memcpy(&a, &ax, 4);
+    float r;
+    if (a >= 65520.0f)
+        r = INFINITY; // rounds to inf
+    else if (a < 6.103515625e-05f)
+    {
+        // subnormal half: spacing 2^-24
+        const double q = std::nearbyint(double(a) * 16777216.0);
+        r = float(q / 16777216.0);
+    }
+    else
+    {
+        int e;
+        std::frexp(a, &e); // a = m * 2^e, m in [0.5, 1)
+        const double scale = std::ldexp(1.0, 11 - e); // 11 significant bits
+        const double q = std::nearbyint(double(a) * scale);
+        r = float(q / scale);
+    }
+    uint32_t ru;
+    std::memcpy(&ru, &r, 4);
+    ru |= sign;
+    std::memcpy(&r, &ru, 4);
+    return r;
+}
+
+// float -> uint32 round-to-nearest-even with saturation (NaN -> 0), as
+// __float2uint_rn in examples/mono_star/opt/ImageTermKNNFetcher.cu:164-165.
+inline uint32_t float2uint_rn(float v)
+{
+    if (!(v > 0.0f))
+        return 0u;
+    if (v >= 4294967296.0f)
+        return 0xffffffffu;
+    return uint32_t(std::nearbyint(double(v)));
+}
+
+// ------------------------------------------------------------------ KNN heap
+// "Replace-current-max" heap of fixed capacity followed by a Lomuto quicksort,
+// libs/star/geometry/node_graph/brute_force_knn.cuh:52-135.
+struct KnnHeap
+{
+    int K;
+    float dist[16];
+    uint16_t idx[16];
+    int max_id;
+    float max_dist;
+    KnnHeap(int k, float init_d) : K(k)
+    {
+        for (int i = 0; i < K; ++i)
+        {
+            dist[i] = init_d;
+            idx[i] = 0;
+        }
+        refresh();
+    }
+    void refresh()
+    {
+        max_id = 0;
+        max_dist = dist[0];
+        for (int i = 1; i < K; ++i)
+            if (dist[i] > max_dist)
+            {
+                max_dist = dist[i];
+                max_id = i;
+            }
+    }
+    void update(uint16_t id, float d)
+    {
+        if (d < max_dist)
+        {
+            dist[max_id] = d;
+            idx[max_id] = id;
+        }
+        refresh();
+    }
+    int partition(int lo, int hi)
+    {
+        const float pivot = dist[hi];
+        int i = lo - 1;
+        for (int j = lo; j <= hi - 1; ++j)
+            if (dist[j] < pivot)
+            {
+                ++i;
+                std::swap(dist[i], dist[j]);
+                std::swap(idx[i], idx[j]);
+            }
+        std::swap(dist[i + 1], dist[hi]);
+        std::swap(idx[i + 1], idx[hi]);
+        return i + 1;
+    }
+    void quicksort(int lo, int hi)
+    {
+        if (lo < hi)
+        {
+            const int p = partition(lo, hi);
+            quicksort(lo, p - 1);
+            quicksort(p + 1, hi);
+        }
+    }
+    void sort() { quicksort(0, K - 1); }
+};
+
+// Squared distance with the contraction order fma(dz,dz,fma(dy,dy,dx*dx)),
+// libs/star/geometry/node_graph/NodeGraph.cu:32-34.
+inline float sqdist_fma(float dx, float dy, float dz)
+{
+    const float dx2 = dx * dx;
+    const float dxy = std::fma(dy, dy, dx2);
+    return std::fma(dz, dz, dxy);
+}
+
+// Surfel skinning: K nearest nodes + normalised Gaussian weights, connect = 1.
+// libs/star/geometry/node_graph/Skinner.cu:10-94
+inline void skin_surfels(const float *surfel4, int num_surfels, const float *node4, int num_nodes, int K,
+                         float node_radius_square, uint16_t *knn, float *spatial, float *connect)
+{
+#pragma omp parallel for schedule(static)
+    for (int s = 0; s < num_surfels; ++s)
+    {
+        const float vx = surfel4[4 * s], vy = surfel4[4 * s + 1], vz = surfel4[4 * s + 2];
+        KnnHeap h(K, 1e6f);
+        for (int i = 0; i < num_nodes; ++i)
+        {
+            const float d = sqdist_fma(vx - node4[4 * i], vy - node4[4 * i + 1], vz - node4[4 * i + 2]);
+            h.update(uint16_t(i), d);
+        }
+        h.sort();
+        float w[16];
+        float wsum = 0.f;
+        for (int k = 0; k < K; ++k)
+        {
+            const int n = h.idx[k];
+            const float d = sqdist_fma(vx - node4[4 * n], vy - node4[4 * n + 1], vz - node4[4 * n + 2]);
+            w[k] = std::exp(-d / (2.f * node_radius_square));
+            wsum += w[k];
+        }
+        const float inv = 1.f / wsum;
+        for (int k = 0; k < K; ++k)
+        {
+            knn[size_t(s) * K + k] = h.idx[k];
+            spatial[size_t(s) * K + k] = inv * w[k];
+            connect[size_t(s) * K + k] = 1.f;
+        }
+    }
+}
+
+// Node graph: FP16-rounded pairwise squared distances (self at distance 0),
+// K-NN with heap initialised at 1e3, normalised weights, edges (i, knn_k, k),
+// connect = 1.   NodeGraph.cu:10-118, 132-145
+inline void build_node_graph(const float *node4, int num_nodes, int K, float node_radius_square, uint16_t *node_knn,
+                             float *node_spatial, uint16_t *edges /* ushort3 x N*K */, float *node_connect)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < num_nodes; ++i)
+    {
+        KnnHeap h(K, 1e3f);
+        for (int j = 0; j < num_nodes; ++j)
+        {
+            const int a = std::max(i, j), b = std::min(i, j); // UpdateNodeDistanceKernel writes idx>=idy
+            const float d = sqdist_fma(node4[4 * a] - node4[4 * b], node4[4 * a + 1] - node4[4 * b + 1],
+                                       node4[4 * a + 2] - node4[4 * b + 2]);
+            h.update(uint16_t(j), round_to_half(d));
+        }
+        h.sort();
+        float wsum = 0.f, w[16];
+        for (int k = 0; k < K; ++k)
+        {
+            w[k] = std::exp(-h.dist[k] / (2 * node_radius_square));
+            wsum += w[k];
+        }
+        const float inv = 1.f / wsum;
+        for (int k = 0; k < K; ++k)
+        {
+            node_knn[size_t(i) * K + k] = h.idx[k];
+            node_spatial[size_t(i) * K + k] = w[k] * inv;
+            node_connect[size_t(i) * K + k] = 1.f;
+            edges[3 * (size_t(i) * K + k) + 0] = uint16_t(i);
+            edges[3 * (size_t(i) * K + k) + 1] = h.idx[k];
+            edges[3 * (size_t(i) * K + k) + 2] = uint16_t(k);
+        }
+    }
+}
+
+// ----------------------------------------------------------- solver inputs
+struct Inputs
+{
+    int width = 0, height = 0;
+    float fx = 0, fy = 0, cx = 0, cy = 0;
+    float cam2world[16]; // row-major 4x4 rigid transform
+    // maps, W*H each (row-major, pixel (x,y) at y*W+x)
+    const float *measure_vertex = nullptr; // float4, camera frame
+    const float *measure_normal = nullptr; // float4, camera frame
+    const float *ref_vertex = nullptr;     // float4, world frame
+    const float *ref_normal = nullptr;     // float4, world frame
+    const uint32_t *index_map = nullptr;   // surfel index or 0xffffffff
+    const float *flow = nullptr;           // float2 (may be null: zero flow)
+    // geometry
+    int num_surfels = 0;
+    int K = 8;
+    const uint16_t *surfel_knn = nullptr;
+    const float *surfel_spatial = nullptr;
+    const float *surfel_connect = nullptr;
+    // node graph
+    int num_nodes = 0;
+    const float *node_coords = nullptr; // float4
+    int num_edges = 0;
+    const uint16_t *edges = nullptr;       // ushort3 (i, j, k)
+    const float *node_connect = nullptr;   // N*K
+    const float *node_motion_pred = nullptr; // float4 per node (may be null: zeros)
+    float node_radius_square = 0;
+};
+
+struct Params
+{
+    int gn_iters = 6;
+    int pcg_iters = 10;
+    float lambda_depth = 0.1f;
+    float lambda_flow = 0.0f;
+    float lambda_reg = 1.0f;
+    float lambda_node_translation = 0.0f;
+    float damping = 1e-3f;
+};
+
+// Everything the oracle can expose for the checks.
+struct Result
+{
+    int num_pixels = 0;
+    std::vector<uint16_t> pixel_pair;  // P*4 (x, y, tx, ty)
+    std::vector<uint16_t> pixel_knn;   // P*K
+    std::vector<float> pixel_spatial;  // P*K
+    std::vector<float> pixel_connect;  // P*K
+    std::vector<uint32_t> n2t_offset;  // N+1
+    std::vector<uint32_t> n2t_terms;   // sorted term ids
+    std::vector<uint32_t> half_keys;   // unique (i<j) pair keys i*4096+j
+    std::vector<uint32_t> half_offset; // U+1 ranges into pair_terms
+    std::vector<uint32_t> pair_terms;  // sorted term ids per half pair
+    std::vector<uint32_t> sym_keys;    // 2U sorted symmetric keys
+    std::vector<uint32_t> sym_range;   // 2U*2 (start,end) into pair_terms
+    std::vector<uint32_t> row_offset;  // N+1 into sym_keys
+    // first GN iteration dumps (T cast to double)
+    std::vector<double> residual;      // P*3
+    std::vector<double> jacobian;      // P*18  (channel-major: c*6 + [rx ry rz tx ty tz])
+    std::vector<double> reg_Ti_xj, reg_Tj_xj; // E*3
+    std::vector<uint8_t> reg_valid;    // E
+    std::vector<double> diag;          // N*36 (with damping), [a*6+b]
+    std::vector<double> rhs;           // N*6
+    std::vector<double> offdiag;       // 2U*36, block (row,col) [a*6+b]
+    std::vector<double> diag_inv;      // N*36
+    std::vector<double> pcg_x;         // N*6, first GN iteration
+    std::vector<double> cost;          // per GN iteration (incl. a final one): total weighted cost
+    std::vector<double> dq_iter;       // (gn_iters+1) * N * 8, (q0 wxyz, q1 wxyz)
+    std::vector<double> dq;            // N*8 final
+};
+
+// ------------------------------------------------------------- the solver
+enum TermType
+{
+    kDense = 0,
+    kReg = 1,
+    kNodeT = 2,
+    kFeature = 3,
+    kInvalid = 4
+};
+
+struct TermOffset
+{
+    uint32_t v[4];
+};
+// examples/mono_star/common/term_offset_types.h:84-118
+inline void query_typed_index(uint32_t term, const TermOffset &o, int &type, uint32_t &typed)
+{
+    if (term < o.v[0])
+    {
+        type = kDense;
+        typed = term;
+        return;
+    }
+    if (term < o.v[1])
+    {
+        type = kReg;
+        typed = term - o.v[0];
+        return;
+    }
+    if (term < o.v[2])
+    {
+        type = kNodeT;
+        typed = term - o.v[1];
+        return;
+    }
+    if (term < o.v[3])
+    {
+        type = kFeature;
+        typed = term - o.v[2];
+        return;
+    }
+    type = kInvalid;
+    typed = 0xffffffffu;
+}
+
+// Stable LSD radix sort of (key, value) pairs on the low `bits` bits of the key.
+// Stands in for CUB DeviceRadixSort::SortPairs, whose output is defined by
+// stable-sort semantics (libs/star/common/algorithm_types.cuh:26-124).
+template <class K, class V>
+inline void stable_sort_pairs(std::vector<K> &keys, std::vector<V> &vals, int bits)
+{
+    const size_t n = keys.size();
+    std::vector<K> k2(n);
+    std::vector<V> v2(n);
+    for (int shift = 0; shift < bits; shift += 8)
+    {
+        size_t count[257] = {0};
+        for (size_t i = 0; i < n; ++i)
+            ++count[((uint64_t(keys[i]) >> shift) & 0xff) + 1];
+        for (int b = 0; b < 256; ++b)
+            count[b + 1] += count[b];
+        for (size_t i = 0; i < n; ++i)
+        {
+            const size_t d = count[(uint64_t(keys[i]) >> shift) & 0xff]++;
+            k2[d] = keys[i];
+            v2[d] = vals[i];
+        }
+        keys.swap(k2);
+        vals.swap(v2);
+    }
+}
+
+constexpr uint32_t kUninitKey = 0xffffffffu;
+
+template <class T>
+class Solver
+{
+public:
+    const Inputs &in;
+    Params prm;
+    Result &res;
+    int K, N, P, E, W, H;
+    TermOffset toff;
+    M34<T> w2c;
+
+    // per-pixel compacted streams
+    std::vector<uint16_t> pix;                // P*4
+    std::vector<uint16_t> knn;                // P*K
+    std::vector<float> spatial, connect;      // P*K
+    // indices
+    std::vector<uint32_t> n2t_off, n2t_terms;
+    std::vector<uint32_t> pair_terms;         // half pair term lists
+    std::vector<uint32_t> sym_keys;           // sorted symmetric keys
+    std::vector<uint32_t> sym_start, sym_end; // per symmetric pair ranges
+    std::vector<uint32_t> row_off;            // N+1 into sym_keys
+    // iteration data
+    std::vector<DQ<T>> dq;
+    std::vector<T> r, J;                      // P*3, P*18
+    std::vector<V3<T>> Ti_xj, Tj_xj;
+    std::vector<uint8_t> valid;
+    std::vector<float> cw;
+    std::vector<V3<T>> Ttrans;
+    std::vector<T> D, Dinv, b, B, x;
+    std::vector<T> pr, pt, pd, pq, ps;
+
+    Solver(const Inputs &i, const Params &p, Result &out) : in(i), prm(p), res(out)
+    {
+        K = in.K;
+        N = in.num_nodes;
+        E = in.num_edges;
+        W = in.width;
+        H = in.height;
+        // world2camera = inverse(camera2world)  WarpSolver.cu:151-154
+        M34<T> c2w;
+        for (int a = 0; a < 3; ++a)
+        {
+            for (int bb = 0; bb < 3; ++bb)
+                c2w.R.m[a][bb] = T(in.cam2world[4 * a + bb]);
+        }
+        c2w.t = {T(in.cam2world[3]), T(in.cam2world[7]), T(in.cam2world[11])};
+        // device_mat.h:217-223
+        for (int a = 0; a < 3; ++a)
+            for (int bb = 0; bb < 3; ++bb)
+                w2c.R.m[a][bb] = c2w.R.m[bb][a];
+        const V3<T> rt = mul(w2c.R, c2w.t);
+        w2c.t = {-rt.x, -rt.y, -rt.z};
+    }
+
+    // ---- a3/a4: pixel KNN query + compaction (WarpSolver.cu:7-48, ImageTermKNNFetcher.cu:9-29,56-88)
+    void compact_pixels()
+    {
+        pix.clear();
+        knn.clear();
+        spatial.clear();
+        connect.clear();
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x)
+            {
+                const uint32_t s = in.index_map[size_t(y) * W + x];
+                if (s == 0xffffffffu)
+                    continue;
+                float fx = 0.f, fy = 0.f;
+                if (in.flow)
+                {
+                    fx = in.flow[2 * (size_t(y) * W + x)];
+                    fy = in.flow[2 * (size_t(y) * W + x) + 1];
+                }
+                const uint32_t tx = float2uint_rn(fx + float(x));
+                const uint32_t ty = float2uint_rn(fy + float(y));
+                pix.push_back(uint16_t(x));
+                pix.push_back(uint16_t(y));
+                pix.push_back(uint16_t(tx)); // make_ushort4 truncates to 16 bits
+                pix.push_back(uint16_t(ty));
+                for (int k = 0; k < K; ++k)
+                {
+                    knn.push_back(in.surfel_knn[size_t(s) * K + k]);
+                    spatial.push_back(in.surfel_spatial[size_t(s) * K + k]);
+                    connect.push_back(in.surfel_connect[size_t(s) * K + k]);
+                }
+            }
+        P = int(pix.size() / 4);
+        // term offsets: [dense P][reg E][node translation N][feature 0]  term_offset_types.h:62-82
+        toff.v[0] = P;
+        toff.v[1] = P + E;
+        toff.v[2] = P + E + N;
+        toff.v[3] = P + E + N;
+    }
+
+    // ---- a9: node -> terms CSR  (Node2TermsIndex.cu:6-122, 312-360)
+    void build_node2term()
+    {
+        std::vector<uint16_t> keys;
+        std::vector<uint32_t> vals;
+        keys.reserve(size_t(P) * K + 2 * E + N);
+        vals.reserve(keys.capacity());
+        for (int t = 0; t < P; ++t)
+            for (int k = 0; k < K; ++k)
+            {
+                keys.push_back(knn[size_t(t) * K + k]);
+                vals.push_back(uint32_t(t));
+            }
+        for (int e = 0; e < E; ++e)
+        {
+            const uint16_t i = in.edges[3 * e], j = in.edges[3 * e + 1];
+            keys.push_back(i != j ? i : 0xffff);
+            keys.push_back(i != j ? j : 0xffff);
+            vals.push_back(uint32_t(P + e));
+            vals.push_back(uint32_t(P + e));
+        }
+        for (int n = 0; n < N; ++n)
+        {
+            keys.push_back(uint16_t(n));
+            vals.push_back(uint32_t(P + E + n));
+        }
+        stable_sort_pairs(keys, vals, 16);
+        // offsets: offset[n] = first position with key >= n (computeTermOffsetKernel)
+        n2t_off.assign(N + 1, 0);
+        size_t pos = 0;
+        for (int n = 0; n <= N; ++n)
+        {
+            while (pos < keys.size() && keys[pos] < n && keys[pos] != 0xffff)
+                ++pos;
+            n2t_off[n] = uint32_t(pos);
+        }
+        n2t_terms = vals;
+        n2t_terms.resize(n2t_off[N]);
+    }
+
+    // ---- a10/a11: node pair -> terms index  (NodePair2TermsIndex.cu:11-193, 591-691)
+    void build_nodepair2term()
+    {
+        const int npair = K * (K - 1) / 2;
+        std::vector<uint32_t> keys;
+        std::vector<uint32_t> vals;
+        keys.reserve(size_t(P) * npair + E);
+        for (int t = 0; t < P; ++t)
+        {
+            const uint16_t *kn = &knn[size_t(t) * K];
+            int cnt = 0;
+            uint32_t loc[120];
+            for (int a = 0; a < npair; ++a)
+                loc[a] = kUninitKey;
+            for (int a = 0; a < K; ++a)
+                for (int c = 0; c < K; ++c)
+                    if (kn[a] < kn[c])
+                        loc[cnt++] = uint32_t(kn[a]) * 4096u + kn[c];
+            for (int a = 0; a < npair; ++a)
+            {
+                keys.push_back(loc[a]);
+                vals.push_back(uint32_t(t));
+            }
+        }
+        for (int e = 0; e < E; ++e)
+        {
+            const uint32_t i = in.edges[3 * e], j = in.edges[3 * e + 1];
+            uint32_t key = kUninitKey;
+            if (i < j)
+                key = i * 4096u + j;
+            else if (j < i)
+                key = j * 4096u + i;
+            keys.push_back(key);
+            vals.push_back(uint32_t(P + e));
+        }
+        // Sort on the low 24 bits (NodePair2TermsIndex.cu:619); the uninitialised
+        // key 0xffffffff lands after every valid pair key (all < 0xffefff).
+        stable_sort_pairs(keys, vals, 24);
+        // unique keys + ranges, dropping the uninitialised tail
+        std::vector<uint32_t> hkeys, hoff;
+        for (size_t i = 0; i < keys.size(); ++i)
+        {
+            if (keys[i] == kUninitKey)
+                break;
+            if (i == 0 || keys[i] != keys[i - 1])
+            {
+                hkeys.push_back(keys[i]);
+                hoff.push_back(uint32_t(i));
+            }
+        }
+        size_t valid_end = 0;
+        while (valid_end < keys.size() && keys[valid_end] != kUninitKey)
+            ++valid_end;
+        hoff.push_back(uint32_t(valid_end));
+        pair_terms.assign(vals.begin(), vals.begin() + valid_end);
+        res.half_keys = hkeys;
+        res.half_offset = hoff;
+        res.pair_terms = pair_terms;
+        // symmetric: (i,j) and (j,i) share the range; stable sort on 24 bits
+        const size_t U = hkeys.size();
+        std::vector<uint32_t> skeys(2 * U);
+        std::vector<uint64_t> svals(2 * U);
+        for (size_t u = 0; u < U; ++u)
+        {
+            const uint32_t i = hkeys[u] / 4096u, j = hkeys[u] % 4096u;
+            skeys[2 * u] = hkeys[u];
+            skeys[2 * u + 1] = j * 4096u + i;
+            const uint64_t range = (uint64_t(hoff[u + 1]) << 32) | hoff[u];
+            svals[2 * u] = range;
+            svals[2 * u + 1] = range;
+        }
+        stable_sort_pairs(skeys, svals, 24);
+        sym_keys = skeys;
+        sym_start.resize(2 * U);
+        sym_end.resize(2 * U);
+        for (size_t u = 0; u < 2 * U; ++u)
+        {
+            sym_start[u] = uint32_t(svals[u] & 0xffffffffu);
+            sym_end[u] = uint32_t(svals[u] >> 32);
+        }
+        // block-row offsets (NodePairRowColumnIndex.cu:12-36)
+        row_off.assign(N + 1, 0);
+        size_t pos = 0;
+        for (int n = 0; n <= N; ++n)
+        {
+            while (pos < sym_keys.size() && sym_keys[pos] / 4096u < uint32_t(n))
+                ++pos;
+            row_off[n] = uint32_t(pos);
+        }
+        res.sym_keys = sym_keys;
+        res.sym_range.resize(4 * U);
+        for (size_t u = 0; u < 2 * U; ++u)
+        {
+            res.sym_range[2 * u] = sym_start[u];
+            res.sym_range[2 * u + 1] = sym_end[u];
+        }
+        res.row_offset = row_off;
+    }
+
+    // ---- a6: per-pixel residual / Jacobian   DenseImageJacobianResidual.cu:12-157,164-310
+    void dense_terms()
+    {
+        r.assign(size_t(P) * 3, T(0));
+        J.assign(size_t(P) * 18, T(0));
+        const T normal_thr = T(0.7f);   // global_configs.h:57
+        const T dist_thr = T(0.03f);    // global_configs.h:58
+#pragma omp parallel for schedule(static)
+        for (int p = 0; p < P; ++p)
+        {
+            DQ<T> avg{{0, 0, 0, 0}, {0, 0, 0, 0}};
+            for (int k = 0; k < K; ++k)
+            {
+                const T w = T(spatial[size_t(p) * K + k] * connect[size_t(p) * K + k]);
+                const DQ<T> &d = dq[knn[size_t(p) * K + k]];
+                avg.q0 = qadd(avg.q0, qscale(w, d.q0));
+                avg.q1 = qadd(avg.q1, qscale(w, d.q1));
+            }
+            const M34<T> se3 = dq_se3(avg);
+            const int x = pix[4 * p], y = pix[4 * p + 1], tx = pix[4 * p + 2], ty = pix[4 * p + 3];
+            const float *cv = &in.ref_vertex[4 * (size_t(y) * W + x)];
+            const float *cn = &in.ref_normal[4 * (size_t(y) * W + x)];
+            const V3<T> vw = mul(se3.R, V3<T>{T(cv[0]), T(cv[1]), T(cv[2])}) + se3.t;
+            const V3<T> nw = mul(se3.R, V3<T>{T(cn[0]), T(cn[1]), T(cn[2])});
+            const V3<T> vc = mul(w2c.R, vw) + w2c.t;
+            const V3<T> nc = mul(w2c.R, nw);
+            if (!(tx < W && ty < H))
+                continue;
+            const float *tv4 = &in.measure_vertex[4 * (size_t(ty) * W + tx)];
+            const float *tn4 = &in.measure_normal[4 * (size_t(ty) * W + tx)];
+            const V3<T> tv{T(tv4[0]), T(tv4[1]), T(tv4[2])};
+            const V3<T> tn{T(tn4[0]), T(tn4[1]), T(tn4[2])};
+            bool ok = true;
+            // is_zero_vertex  libs/star/math/vector_ops.hpp:275-279
+            if (std::fabs(tv4[0]) < 1e-3 && std::fabs(tv4[1]) < 1e-3 && std::fabs(tv4[2]) < 1e-3)
+                ok = false;
+            if (dot(tn, nc) < normal_thr)
+                ok = false;
+            if (std::fabs(tv.z - vc.z) > dist_thr)
+                ok = false;
+            if (!ok)
+                continue;
+            const V3<T> tvw = mulT(w2c.R, tv - w2c.t); // apply_inversed_se3
+            const V3<T> tnw = mulT(w2c.R, tn);
+            T *rr = &r[size_t(p) * 3];
+            T *jj = &J[size_t(p) * 18];
+            // point-to-plane  DenseImageJacobianResidual.cu:12-25
+            rr[0] = dot(tnw, vw - tvw);
+            const V3<T> jr0 = cross(vw, tnw);
+            jj[0] = jr0.x;
+            jj[1] = jr0.y;
+            jj[2] = jr0.z;
+            jj[3] = tnw.x;
+            jj[4] = tnw.y;
+            jj[5] = tnw.z;
+            // modified flow: e = (P R_w2c)(v - t)   DenseImageJacobianResidual.cu:77-115,143-158
+            const V3<T> diff = vw - tvw;
+            const V3<T> pr0{w2c.R.m[0][0], w2c.R.m[0][1], w2c.R.m[0][2]};
+            const V3<T> pr1{w2c.R.m[1][0], w2c.R.m[1][1], w2c.R.m[1][2]};
+            const T ex = dot(pr0, diff), ey = dot(pr1, diff);
+            if (std::sqrt(ex * ex + ey * ey) < T(1e-6f))
+                continue; // channels 1,2 stay zero
+            rr[1] = ex;
+            rr[2] = ey;
+            // J_r = (PR) so3(-v), so3(xi) = [[0,-z,y],[z,0,-x],[-y,x,0]]
+            const V3<T> mv{-vw.x, -vw.y, -vw.z};
+            const V3<T> rowsrc[2] = {pr0, pr1};
+            for (int c = 0; c < 2; ++c)
+            {
+                const V3<T> a = rowsrc[c];
+                T *jc = jj + 6 * (c + 1);
+                jc[0] = a.y * mv.z + a.z * (-mv.y);
+                jc[1] = a.x * (-mv.z) + a.z * mv.x;
+                jc[2] = a.x * mv.y + a.y * (-mv.x);
+                jc[3] = a.x;
+                jc[4] = a.y;
+                jc[5] = a.z;
+            }
+        }
+    }
+
+    // ---- a7: ARAP forward warp   NodeGraphHandler.cu:7-46
+    void reg_terms()
+    {
+        Ti_xj.resize(E);
+        Tj_xj.resize(E);
+        valid.resize(E);
+        cw.resize(E);
+        for (int e = 0; e < E; ++e)
+        {
+            const int i = in.edges[3 * e], j = in.edges[3 * e + 1], k = in.edges[3 * e + 2];
+            const float *xi = &in.node_coords[4 * i];
+            const float *xj = &in.node_coords[4 * j];
+            const M34<T> Ti = dq_se3(dq[i]);
+            const M34<T> Tj = dq_se3(dq[j]);
+            const V3<T> vj{T(xj[0]), T(xj[1]), T(xj[2])};
+            Ti_xj[e] = mul(Ti.R, vj) + Ti.t;
+            Tj_xj[e] = mul(Tj.R, vj) + Tj.t;
+            const float dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
+            valid[e] = (dx * dx + dy * dy + dz * dz > 64 * in.node_radius_square) ? 0 : 1;
+            cw[e] = in.node_connect[size_t(i) * K + k];
+        }
+        // a8: node translation  NodeMotionHandler.cu:9-21
+        Ttrans.resize(N);
+        for (int n = 0; n < N; ++n)
+            Ttrans[n] = dq_se3(dq[n]).t;
+    }
+
+    // ARAP Jacobian rows for the node on side i (is_i) or j   node_graph_reg_jacobian.cuh:250-275
+    static void reg_jac(const V3<T> &a, const V3<T> &s, T c, bool is_i, T g[18])
+    {
+        if (is_i)
+        {
+            const T rows[18] = {0, a.z, -a.y, 1, 0, 0, -a.z, 0, a.x, 0, 1, 0, a.y, -a.x, 0, 0, 0, 1};
+            for (int q = 0; q < 18; ++q)
+                g[q] = rows[q] * c;
+        }
+        else
+        {
+            const T rows[18] = {0, -s.z, s.y, -1, 0, 0, s.z, 0, -s.x, 0, -1, 0, -s.y, s.x, 0, 0, 0, -1};
+            for (int q = 0; q < 18; ++q)
+                g[q] = rows[q] * c;
+        }
+    }
+
+    int last_slot(int p, int node) const
+    {
+        int off = 0;
+        for (int k = 0; k < K; ++k)
+            if (knn[size_t(p) * K + k] == node)
+                off = k;
+        return off;
+    }
+
+    // ---- a12: diagonal blocks + damping   PreconditionerRhsBuilder.cu:110-262
+    // ---- a14: J^T W r                     JtDotResidualIndexed.cu:40-150
+    void diag_and_rhs()
+    {
+        D.assign(size_t(N) * 36, T(0));
+        b.assign(size_t(N) * 6, T(0));
+        const T lam2[3] = {T(prm.lambda_depth) * T(prm.lambda_depth), T(prm.lambda_flow) * T(prm.lambda_flow),
+                           T(prm.lambda_flow) * T(prm.lambda_flow)};
+        const T lreg2 = T(prm.lambda_reg) * T(prm.lambda_reg);
+        const T lnt2 = T(prm.lambda_node_translation) * T(prm.lambda_node_translation);
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int n = 0; n < N; ++n)
+        {
+            T acc[36] = {0};
+            T jtr[6] = {0};
+            for (uint32_t it = n2t_off[n]; it < n2t_off[n + 1]; ++it)
+            {
+                int type;
+                uint32_t ti;
+                query_typed_index(n2t_terms[it], toff, type, ti);
+                if (type == kDense)
+                {
+                    const int s = last_slot(int(ti), n);
+                    const T w = T(spatial[size_t(ti) * K + s] * connect[size_t(ti) * K + s]);
+                    const T *jj = &J[size_t(ti) * 18];
+                    const T *rr = &r[size_t(ti) * 3];
+                    for (int c = 0; c < 3; ++c)
+                    {
+                        T jw[6];
+                        for (int q = 0; q < 6; ++q)
+                            jw[q] = jj[6 * c + q] * w;
+                        for (int a = 0; a < 6; ++a)
+                            for (int bb = 0; bb < 6; ++bb)
+                                acc[6 * a + bb] += lam2[c] * jw[bb] * jw[a];
+                        const T ow = lam2[c] * (w * w);
+                        for (int q = 0; q < 6; ++q)
+                            jtr[q] -= ow * rr[c] * jj[6 * c + q];
+                    }
+                }
+                else if (type == kReg)
+                {
+                    if (!valid[ti])
+                        continue;
+                    const bool is_i = (uint32_t(n) == in.edges[3 * ti]);
+                    T g[18];
+                    reg_jac(Ti_xj[ti], Tj_xj[ti], T(cw[ti]), is_i, g);
+                    for (int c = 0; c < 3; ++c)
+                        for (int a = 0; a < 6; ++a)
+                            for (int bb = 0; bb < 6; ++bb)
+                                acc[6 * a + bb] += lreg2 * g[6 * c + bb] * g[6 * c + a];
+                    // computeRegTermJtResidual: residual scaled by cw^2, unweighted rows
+                    T g1[18];
+                    reg_jac(Ti_xj[ti], Tj_xj[ti], T(1), is_i, g1);
+                    const T c2 = T(cw[ti]) * T(cw[ti]);
+                    const V3<T> d = Ti_xj[ti] - Tj_xj[ti];
+                    const T res3[3] = {d.x * c2, d.y * c2, d.z * c2};
+                    for (int c = 0; c < 3; ++c)
+                        for (int q = 0; q < 6; ++q)
+                            jtr[q] -= lreg2 * res3[c] * g1[6 * c + q];
+                }
+                else if (type == kNodeT)
+                {
+                    const float *pred = in.node_motion_pred ? &in.node_motion_pred[4 * ti] : nullptr;
+                    const T conf = pred ? T(pred[3]) : T(0);
+                    for (int c = 0; c < 3; ++c)
+                        acc[6 * (3 + c) + (3 + c)] += lnt2 * conf * conf;
+                    const V3<T> tt = Ttrans[ti];
+                    const T px = pred ? T(pred[0]) : T(0), py = pred ? T(pred[1]) : T(0), pz = pred ? T(pred[2]) : T(0);
+                    jtr[3] -= lnt2 * (tt.x - px) * conf * conf;
+                    jtr[4] -= lnt2 * (tt.y - py) * conf * conf;
+                    jtr[5] -= lnt2 * (tt.z - pz) * conf * conf;
+                }
+            }
+            for (int a = 0; a < 6; ++a)
+                acc[7 * a] += T(prm.damping);
+            std::copy(acc, acc + 36, &D[size_t(n) * 36]);
+            std::copy(jtr, jtr + 6, &b[size_t(n) * 6]);
+        }
+    }
+
+    // ---- a15: off-diagonal blocks, block (row,col)[a][b] = sum J_row[a] J_col[b]
+    //      JtJMaterializer.cu:66-175 + JtJMaterializerBinBlockCSR.cu:37-39
+    void offdiag()
+    {
+        const size_t S = sym_keys.size();
+        B.assign(S * 36, T(0));
+        const T lam2[3] = {T(prm.lambda_depth) * T(prm.lambda_depth), T(prm.lambda_flow) * T(prm.lambda_flow),
+                           T(prm.lambda_flow) * T(prm.lambda_flow)};
+        const T lreg2 = T(prm.lambda_reg) * T(prm.lambda_reg);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (long u = 0; u < long(S); ++u)
+        {
+            const int row = int(sym_keys[u] / 4096u), col = int(sym_keys[u] % 4096u);
+            T acc[36] = {0};
+            for (uint32_t it = sym_start[u]; it < sym_end[u]; ++it)
+            {
+                int type;
+                uint32_t ti;
+                query_typed_index(pair_terms[it], toff, type, ti);
+                if (type == kDense)
+                {
+                    const int si = last_slot(int(ti), row), sj = last_slot(int(ti), col);
+                    const T wi = T(spatial[size_t(ti) * K + si] * connect[size_t(ti) * K + si]);
+                    const T wj = T(spatial[size_t(ti) * K + sj] * connect[size_t(ti) * K + sj]);
+                    const T ww = wi * wj;
+                    const T *jj = &J[size_t(ti) * 18];
+                    for (int c = 0; c < 3; ++c)
+                        for (int a = 0; a < 6; ++a)
+                            for (int bb = 0; bb < 6; ++bb)
+                                acc[6 * a + bb] += lam2[c] * ww * jj[6 * c + a] * jj[6 * c + bb];
+                }
+                else if (type == kReg)
+                {
+                    if (!valid[ti])
+                        continue;
+                    const bool row_is_i = (uint32_t(row) == in.edges[3 * ti]);
+                    T gi[18], gj[18];
+                    reg_jac(Ti_xj[ti], Tj_xj[ti], T(1), true, gi);
+                    reg_jac(Ti_xj[ti], Tj_xj[ti], T(1), false, gj);
+                    const T *gr = row_is_i ? gi : gj;
+                    const T *gc = row_is_i ? gj : gi;
+                    const T ws = lreg2 * (T(cw[ti]) * T(cw[ti]));
+                    for (int c = 0; c < 3; ++c)
+                        for (int a = 0; a < 6; ++a)
+                            for (int bb = 0; bb < 6; ++bb)
+                                acc[6 * a + bb] += ws * gr[6 * c + a] * gc[6 * c + bb];
+                }
+            }
+            std::copy(acc, acc + 36, &B[size_t(u) * 36]);
+        }
+    }
+
+    // ---- a13: 6x6 inverse, Gauss-Jordan with partial pivoting  libs/star/math/DenseGaussian.hpp:34-113
+    static void invert6(const T *A, T *Ainv)
+    {
+        T m[6][12];
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j)
+            {
+                m[i][j] = A[6 * i + j];
+                m[i][6 + j] = (i == j) ? T(1) : T(0);
+            }
+        for (int i = 0; i < 6; ++i)
+        {
+            int piv = i;
+            T pv = std::fabs(m[i][i]);
+            for (int j = i + 1; j < 6; ++j)
+                if (pv < std::fabs(m[j][i]))
+                {
+                    pv = std::fabs(m[j][i]);
+                    piv = j;
+                }
+            if (piv != i)
+                for (int k = 0; k < 12; ++k)
+                    std::swap(m[i][k], m[piv][k]);
+            const T inv = T(1) / m[i][i];
+            for (int k = 0; k < 12; ++k)
+                m[i][k] *= inv;
+            for (int rr = 0; rr < 6; ++rr)
+                if (rr != i)
+                {
+                    const T f = m[rr][i];
+                    for (int k = 0; k < 12; ++k)
+                        m[rr][k] -= f * m[i][k];
+                }
+        }
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j)
+                Ainv[6 * i + j] = m[i][6 + j];
+    }
+
+    // ---- a17: y = A x, row order: diag block then off-diagonal blocks by column
+    //      (libs/star/pcg_solver/BinBlockCSR.cuh:7-35 traversal order)
+    void spmv(const std::vector<T> &xv, std::vector<T> &yv) const
+    {
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < N; ++i)
+        {
+            for (int a = 0; a < 6; ++a)
+            {
+                T s = 0;
+                for (int bb = 0; bb < 6; ++bb)
+                    s += D[size_t(i) * 36 + 6 * a + bb] * xv[6 * size_t(i) + bb];
+                for (uint32_t u = row_off[i]; u < row_off[i + 1]; ++u)
+                {
+                    const int col = int(sym_keys[u] % 4096u);
+                    for (int bb = 0; bb < 6; ++bb)
+                        s += B[size_t(u) * 36 + 6 * a + bb] * xv[6 * size_t(col) + bb];
+                }
+                yv[6 * size_t(i) + a] = s;
+            }
+        }
+    }
+
+    T vdot(const std::vector<T> &a, const std::vector<T> &c) const
+    {
+        T s = 0;
+        for (size_t i = 0; i < a.size(); ++i)
+            s += a[i] * c[i];
+        return s;
+    }
+
+    void precond(const std::vector<T> &in_v, std::vector<T> &out_v) const
+    {
+        for (int i = 0; i < N; ++i)
+            for (int a = 0; a < 6; ++a)
+            {
+                T s = 0;
+                for (int bb = 0; bb < 6; ++bb)
+                    s += Dinv[size_t(i) * 36 + 6 * a + bb] * in_v[6 * size_t(i) + bb];
+                out_v[6 * size_t(i) + a] = s;
+            }
+    }
+
+    // ---- a18: block-Jacobi PCG, zero init, fixed iterations  libs/star/pcg_solver/BlockPCG.cuh:68-189,325-483
+    void pcg()
+    {
+        const size_t n = size_t(N) * 6;
+        x.assign(n, T(0));
+        pr = b;
+        pd.assign(n, T(0));
+        pq.assign(n, T(0));
+        ps.assign(n, T(0));
+        pt.assign(n, T(0));
+        precond(pr, pd);
+        T delta_new = vdot(pr, pd);
+        for (int it = 0; it < prm.pcg_iters; ++it)
+        {
+            spmv(pd, pq);
+            const T dq_ = vdot(pd, pq);
+            const T alpha = delta_new / dq_;
+            for (size_t i = 0; i < n; ++i)
+            {
+                x[i] += alpha * pd[i];
+                pt[i] = pr[i] - alpha * pq[i];
+            }
+            precond(pt, ps);
+            const T delta_old = delta_new;
+            delta_new = vdot(pt, ps);
+            const T beta = delta_new / delta_old;
+            for (size_t i = 0; i < n; ++i)
+                pd[i] = ps[i] + beta * pd[i];
+            pr.swap(pt);
+        }
+    }
+
+    // ---- a19: T_new = exp(xi) T_old, stored as a dual quaternion  SolverIterationData.cu:19-74
+    void update()
+    {
+        for (int i = 0; i < N; ++i)
+        {
+            const V3<T> w{x[6 * i], x[6 * i + 1], x[6 * i + 2]};
+            const V3<T> t{x[6 * i + 3], x[6 * i + 4], x[6 * i + 5]};
+            const M34<T> S = twist_exp(w, t);
+            const M34<T> prev = dq_to_mat34(dq[i]);
+            dq[i] = dq_from_mat34(mul(S, prev));
+        }
+    }
+
+    // total weighted cost sum over terms (reported, not used for control: the
+    // reference applies every step)
+    T cost() const
+    {
+        T c = 0;
+        for (int p = 0; p < P; ++p)
+        {
+            c += T(prm.lambda_depth) * T(prm.lambda_depth) * r[3 * p] * r[3 * p];
+            c += T(prm.lambda_flow) * T(prm.lambda_flow) * (r[3 * p + 1] * r[3 * p + 1] + r[3 * p + 2] * r[3 * p + 2]);
+        }
+        for (int e = 0; e < E; ++e)
+            if (valid[e])
+            {
+                const V3<T> d = Ti_xj[e] - Tj_xj[e];
+                c += T(prm.lambda_reg) * T(prm.lambda_reg) * T(cw[e]) * T(cw[e]) * dot(d, d);
+            }
+        return c;
+    }
+
+    void dump_dq(std::vector<double> &out) const
+    {
+        for (int i = 0; i < N; ++i)
+        {
+            const DQ<T> &d = dq[i];
+            const double v[8] = {double(d.q0.w), double(d.q0.x), double(d.q0.y), double(d.q0.z),
+                                 double(d.q1.w), double(d.q1.x), double(d.q1.y), double(d.q1.z)};
+            out.insert(out.end(), v, v + 8);
+        }
+    }
+
+    template <class S>
+    static void to_double(const std::vector<S> &a, std::vector<double> &out)
+    {
+        out.assign(a.begin(), a.end());
+    }
+
+    // ---- a20: the GN driver  WarpSolver.cu:260-337
+    void solve()
+    {
+        compact_pixels();
+        build_node2term();
+        build_nodepair2term();
+        res.num_pixels = P;
+        res.pixel_pair = pix;
+        res.pixel_knn = knn;
+        res.pixel_spatial = spatial;
+        res.pixel_connect = connect;
+        res.n2t_offset = n2t_off;
+        res.n2t_terms = n2t_terms;
+        dq.assign(N, DQ<T>{{1, 0, 0, 0}, {0, 0, 0, 0}}); // identity start  SolverIterationData.cu:78-88
+        res.dq_iter.clear();
+        res.cost.clear();
+        dump_dq(res.dq_iter);
+        Dinv.assign(size_t(N) * 36, T(0));
+        for (int it = 0; it < prm.gn_iters; ++it)
+        {
+            dense_terms();
+            reg_terms();
+            res.cost.push_back(double(cost()));
+            diag_and_rhs();
+            offdiag();
+            for (int n = 0; n < N; ++n)
+                invert6(&D[size_t(n) * 36], &Dinv[size_t(n) * 36]);
+            pcg();
+            if (it == 0)
+            {
+                to_double(r, res.residual);
+                to_double(J, res.jacobian);
+                res.reg_Ti_xj.clear();
+                res.reg_Tj_xj.clear();
+                for (int e = 0; e < E; ++e)
+                {
+                    const double a[3] = {double(Ti_xj[e].x), double(Ti_xj[e].y), double(Ti_xj[e].z)};
+                    const double c[3] = {double(Tj_xj[e].x), double(Tj_xj[e].y), double(Tj_xj[e].z)};
+                    res.reg_Ti_xj.insert(res.reg_Ti_xj.end(), a, a + 3);
+                    res.reg_Tj_xj.insert(res.reg_Tj_xj.end(), c, c + 3);
+                }
+                res.reg_valid = valid;
+                to_double(D, res.diag);
+                to_double(b, res.rhs);
+                to_double(B, res.offdiag);
+                to_double(Dinv, res.diag_inv);
+                to_double(x, res.pcg_x);
+            }
+            update();
+            dump_dq(res.dq_iter);
+        }
+        // cost at the final warp field
+        if (prm.gn_iters > 0)
+        {
+            dense_terms();
+            reg_terms();
+            res.cost.push_back(double(cost()));
+        }
+        res.dq.clear();
+        dump_dq(res.dq);
+    }
+};
+
+} // namespace sho

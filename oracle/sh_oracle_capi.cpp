@@ -1,0 +1,265 @@
+// C-ABI over the CPU restatement (sh_oracle.hpp) so that the Python tests and
+// the CPU-baseline leg of the benchmark can drive it through ctypes.
+// TEST INFRASTRUCTURE ONLY: never linked into the product library.
+#include "sh_oracle.hpp"
+
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <string>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+extern "C"
+{
+    struct sho_inputs
+    {
+        int width, height;
+        float fx, fy, cx, cy;
+        float cam2world[16];
+        const float *measure_vertex, *measure_normal, *ref_vertex, *ref_normal;
+        const uint32_t *index_map;
+        const float *flow;
+        int num_surfels, K;
+        const uint16_t *surfel_knn;
+        const float *surfel_spatial, *surfel_connect;
+        int num_nodes;
+        const float *node_coords;
+        int num_edges;
+        const uint16_t *edges;
+        const float *node_connect;
+        const float *node_motion_pred;
+        float node_radius_square;
+    };
+
+    struct sho_params
+    {
+        int gn_iters, pcg_iters;
+        float lambda_depth, lambda_flow, lambda_reg, lambda_node_translation, damping;
+    };
+}
+
+namespace
+{
+struct Handle
+{
+    sho::Result res;
+    double solve_ms = 0.0;
+    int threads = 1;
+};
+
+sho::Inputs to_inputs(const sho_inputs *c)
+{
+    sho::Inputs in;
+    in.width = c->width;
+    in.height = c->height;
+    in.fx = c->fx;
+    in.fy = c->fy;
+    in.cx = c->cx;
+    in.cy = c->cy;
+    std::memcpy(in.cam2world, c->cam2world, sizeof(in.cam2world));
+    in.measure_vertex = c->measure_vertex;
+    in.measure_normal = c->measure_normal;
+    in.ref_vertex = c->ref_vertex;
+    in.ref_normal = c->ref_normal;
+    in.index_map = c->index_map;
+    in.flow = c->flow;
+    in.num_surfels = c->num_surfels;
+    in.K = c->K;
+    in.surfel_knn = c->surfel_knn;
+    in.surfel_spatial = c->surfel_spatial;
+    in.surfel_connect = c->surfel_connect;
+    in.num_nodes = c->num_nodes;
+    in.node_coords = c->node_coords;
+    in.num_edges = c->num_edges;
+    in.edges = c->edges;
+    in.node_connect = c->node_connect;
+    in.node_motion_pred = c->node_motion_pred;
+    in.node_radius_square = c->node_radius_square;
+    return in;
+}
+
+sho::Params to_params(const sho_params *c)
+{
+    sho::Params p;
+    p.gn_iters = c->gn_iters;
+    p.pcg_iters = c->pcg_iters;
+    p.lambda_depth = c->lambda_depth;
+    p.lambda_flow = c->lambda_flow;
+    p.lambda_reg = c->lambda_reg;
+    p.lambda_node_translation = c->lambda_node_translation;
+    p.damping = c->damping;
+    return p;
+}
+
+void set_threads(int threads)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(threads > 0 ? threads : 1);
+#else
+    (void)threads;
+#endif
+}
+} // namespace
+
+extern "C"
+{
+    int sho_max_threads(void)
+    {
+#ifdef _OPENMP
+        return omp_get_num_procs();
+#else
+        return 1;
+#endif
+    }
+
+    // Surfel skinning (Skinner.cu:10-94).  Returns 0.
+    int sho_skin(const float *surfel4, int num_surfels, const float *node4, int num_nodes, int K, float r2,
+                 uint16_t *knn, float *spatial, float *connect, int threads)
+    {
+        if (K < 1 || K > 16 || num_nodes < 1)
+            return -1;
+        set_threads(threads);
+        sho::skin_surfels(surfel4, num_surfels, node4, num_nodes, K, r2, knn, spatial, connect);
+        return 0;
+    }
+
+    // Node graph (NodeGraph.cu:10-118).  Returns 0.
+    int sho_node_graph(const float *node4, int num_nodes, int K, float r2, uint16_t *node_knn, float *node_spatial,
+                       uint16_t *edges, float *node_connect, int threads)
+    {
+        if (K < 1 || K > 16 || num_nodes < K)
+            return -1;
+        set_threads(threads);
+        sho::build_node_graph(node4, num_nodes, K, r2, node_knn, node_spatial, edges, node_connect);
+        return 0;
+    }
+
+    // Full GN solve.  precision: 64 (FP64 oracle) or 32 (FP32 restatement).
+    void *sho_solve(const sho_inputs *cin, const sho_params *cp, int precision, int threads)
+    {
+        Handle *h = new Handle();
+        set_threads(threads);
+        h->threads = threads;
+        const sho::Inputs in = to_inputs(cin);
+        const sho::Params p = to_params(cp);
+        const auto t0 = std::chrono::steady_clock::now();
+        if (precision == 64)
+        {
+            sho::Solver<double> s(in, p, h->res);
+            s.solve();
+        }
+        else
+        {
+            sho::Solver<float> s(in, p, h->res);
+            s.solve();
+        }
+        const auto t1 = std::chrono::steady_clock::now();
+        h->solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        return h;
+    }
+
+    // dtype codes: 1 u8, 2 u16, 4 u32, 8 f64
+    int sho_result_get(void *vh, const char *name, const void **ptr, size_t *count, int *dtype)
+    {
+        Handle *h = static_cast<Handle *>(vh);
+        const sho::Result &r = h->res;
+        const std::string n(name);
+#define SHO_GET(field, code)                                                                                           \
+    if (n == #field)                                                                                                   \
+    {                                                                                                                  \
+        *ptr = r.field.data();                                                                                         \
+        *count = r.field.size();                                                                                       \
+        *dtype = code;                                                                                                 \
+        return 0;                                                                                                      \
+    }
+        SHO_GET(pixel_pair, 2)
+        SHO_GET(pixel_knn, 2)
+        SHO_GET(n2t_offset, 4)
+        SHO_GET(n2t_terms, 4)
+        SHO_GET(half_keys, 4)
+        SHO_GET(half_offset, 4)
+        SHO_GET(pair_terms, 4)
+        SHO_GET(sym_keys, 4)
+        SHO_GET(sym_range, 4)
+        SHO_GET(row_offset, 4)
+        SHO_GET(residual, 8)
+        SHO_GET(jacobian, 8)
+        SHO_GET(reg_Ti_xj, 8)
+        SHO_GET(reg_Tj_xj, 8)
+        SHO_GET(reg_valid, 1)
+        SHO_GET(diag, 8)
+        SHO_GET(rhs, 8)
+        SHO_GET(offdiag, 8)
+        SHO_GET(diag_inv, 8)
+        SHO_GET(pcg_x, 8)
+        SHO_GET(cost, 8)
+        SHO_GET(dq_iter, 8)
+        SHO_GET(dq, 8)
+#undef SHO_GET
+        if (n == "num_pixels")
+        {
+            static thread_local uint32_t v;
+            v = uint32_t(r.num_pixels);
+            *ptr = &v;
+            *count = 1;
+            *dtype = 4;
+            return 0;
+        }
+        if (n == "solve_ms")
+        {
+            *ptr = &h->solve_ms;
+            *count = 1;
+            *dtype = 8;
+            return 0;
+        }
+        return -1;
+    }
+
+    void sho_result_free(void *vh) { delete static_cast<Handle *>(vh); }
+
+    // Exposed pieces of the restatement for closed-form / brute-force checks.
+    float sho_round_to_half(float f) { return sho::round_to_half(f); }
+    uint32_t sho_float2uint_rn(float f) { return sho::float2uint_rn(f); }
+
+    // Closed-form hooks on the DQ algebra (double precision).
+    // dq: (q0 w x y z, q1 w x y z); out34: row-major 3x4.
+    void sho_dq_se3(const double *dq, double *out34)
+    {
+        sho::DQ<double> d{{dq[0], dq[1], dq[2], dq[3]}, {dq[4], dq[5], dq[6], dq[7]}};
+        const sho::M34<double> m = sho::dq_se3(d);
+        for (int a = 0; a < 3; ++a)
+        {
+            for (int b = 0; b < 3; ++b)
+                out34[4 * a + b] = m.R.m[a][b];
+        }
+        out34[3] = m.t.x;
+        out34[7] = m.t.y;
+        out34[11] = m.t.z;
+    }
+    void sho_dq_from_mat34(const double *in34, double *dq)
+    {
+        sho::M34<double> m;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                m.R.m[a][b] = in34[4 * a + b];
+        m.t = {in34[3], in34[7], in34[11]};
+        const sho::DQ<double> d = sho::dq_from_mat34(m);
+        const double v[8] = {d.q0.w, d.q0.x, d.q0.y, d.q0.z, d.q1.w, d.q1.x, d.q1.y, d.q1.z};
+        std::memcpy(dq, v, sizeof(v));
+    }
+    void sho_twist_exp(const double *xi6, double *out34)
+    {
+        const sho::M34<double> m = sho::twist_exp<double>({xi6[0], xi6[1], xi6[2]}, {xi6[3], xi6[4], xi6[5]});
+        for (int a = 0; a < 3; ++a)
+        {
+            for (int b = 0; b < 3; ++b)
+                out34[4 * a + b] = m.R.m[a][b];
+        }
+        out34[3] = m.t.x;
+        out34[7] = m.t.y;
+        out34[11] = m.t.z;
+    }
+    void sho_invert6(const double *A, double *Ainv) { sho::Solver<double>::invert6(A, Ainv); }
+}

@@ -1,0 +1,217 @@
+"""ctypes front end of the CPU restatement (``oracle/sh_oracle.hpp``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the CPU-baseline leg of ``bench.py`` as the checker; the product path never
+imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_LIB = None
+
+_FP = ctypes.POINTER(ctypes.c_float)
+_U16P = ctypes.POINTER(ctypes.c_uint16)
+_U32P = ctypes.POINTER(ctypes.c_uint32)
+_DP = ctypes.POINTER(ctypes.c_double)
+
+
+class ShoInputs(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int), ("height", ctypes.c_int),
+        ("fx", ctypes.c_float), ("fy", ctypes.c_float), ("cx", ctypes.c_float), ("cy", ctypes.c_float),
+        ("cam2world", ctypes.c_float * 16),
+        ("measure_vertex", _FP), ("measure_normal", _FP), ("ref_vertex", _FP), ("ref_normal", _FP),
+        ("index_map", _U32P), ("flow", _FP),
+        ("num_surfels", ctypes.c_int), ("K", ctypes.c_int),
+        ("surfel_knn", _U16P), ("surfel_spatial", _FP), ("surfel_connect", _FP),
+        ("num_nodes", ctypes.c_int), ("node_coords", _FP),
+        ("num_edges", ctypes.c_int), ("edges", _U16P), ("node_connect", _FP), ("node_motion_pred", _FP),
+        ("node_radius_square", ctypes.c_float),
+    ]
+
+
+class ShoParams(ctypes.Structure):
+    _fields_ = [
+        ("gn_iters", ctypes.c_int), ("pcg_iters", ctypes.c_int),
+        ("lambda_depth", ctypes.c_float), ("lambda_flow", ctypes.c_float), ("lambda_reg", ctypes.c_float),
+        ("lambda_node_translation", ctypes.c_float), ("damping", ctypes.c_float),
+    ]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = _HERE / "_build" / "libsh_oracle.so"
+        if not path.exists():
+            raise RuntimeError(f"{path} is missing: run `make -C oracle` or `python -m starhub_amd.build`")
+        L = ctypes.CDLL(str(path))
+        L.sho_skin.argtypes = [_FP, ctypes.c_int, _FP, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                               _U16P, _FP, _FP, ctypes.c_int]
+        L.sho_node_graph.argtypes = [_FP, ctypes.c_int, ctypes.c_int, ctypes.c_float, _U16P, _FP, _U16P, _FP,
+                                     ctypes.c_int]
+        L.sho_solve.argtypes = [ctypes.POINTER(ShoInputs), ctypes.POINTER(ShoParams), ctypes.c_int, ctypes.c_int]
+        L.sho_solve.restype = ctypes.c_void_p
+        L.sho_result_get.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p),
+                                     ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int)]
+        L.sho_result_free.argtypes = [ctypes.c_void_p]
+        L.sho_round_to_half.argtypes = [ctypes.c_float]
+        L.sho_round_to_half.restype = ctypes.c_float
+        L.sho_float2uint_rn.argtypes = [ctypes.c_float]
+        L.sho_float2uint_rn.restype = ctypes.c_uint32
+        L.sho_dq_se3.argtypes = [_DP, _DP]
+        L.sho_dq_from_mat34.argtypes = [_DP, _DP]
+        L.sho_twist_exp.argtypes = [_DP, _DP]
+        L.sho_invert6.argtypes = [_DP, _DP]
+        L.sho_max_threads.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def max_threads() -> int:
+    return int(lib().sho_max_threads())
+
+
+def skin(surfel_pos: np.ndarray, nodes: np.ndarray, K: int, r2: float, threads: int = 0):
+    S, N = surfel_pos.shape[0], nodes.shape[0]
+    sp = np.ascontiguousarray(surfel_pos, np.float32)
+    nd = np.ascontiguousarray(nodes, np.float32)
+    knn = np.zeros((S, K), np.uint16)
+    w = np.zeros((S, K), np.float32)
+    c = np.zeros((S, K), np.float32)
+    rc = lib().sho_skin(sp.ctypes.data_as(_FP), S, nd.ctypes.data_as(_FP), N, K, r2, knn.ctypes.data_as(_U16P),
+                        w.ctypes.data_as(_FP), c.ctypes.data_as(_FP), threads or max_threads())
+    if rc != 0:
+        raise ValueError("sho_skin rejected its arguments")
+    return knn, w, c
+
+
+def node_graph(nodes: np.ndarray, K: int, r2: float, threads: int = 0):
+    N = nodes.shape[0]
+    nd = np.ascontiguousarray(nodes, np.float32)
+    knn = np.zeros((N, K), np.uint16)
+    w = np.zeros((N, K), np.float32)
+    edges = np.zeros((N * K, 3), np.uint16)
+    c = np.zeros((N, K), np.float32)
+    rc = lib().sho_node_graph(nd.ctypes.data_as(_FP), N, K, r2, knn.ctypes.data_as(_U16P), w.ctypes.data_as(_FP),
+                              edges.ctypes.data_as(_U16P), c.ctypes.data_as(_FP), threads or max_threads())
+    if rc != 0:
+        raise ValueError("sho_node_graph rejected its arguments")
+    return knn, w, edges, c
+
+
+_DTYPES = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.float64}
+
+RESULT_FIELDS = ["pixel_pair", "pixel_knn", "n2t_offset", "n2t_terms", "half_keys", "half_offset", "pair_terms",
+                 "sym_keys", "sym_range", "row_offset", "residual", "jacobian", "reg_Ti_xj", "reg_Tj_xj",
+                 "reg_valid", "diag", "rhs", "offdiag", "diag_inv", "pcg_x", "cost", "dq_iter", "dq"]
+
+
+def solve(inputs: dict, params, precision: int = 64, threads: int = 1) -> dict:
+    """Run the restated GN solve.  `inputs` holds numpy arrays (see solver_inputs)."""
+    keep = []
+
+    def fp(a):
+        if a is None:
+            return _FP()
+        a = np.ascontiguousarray(a, np.float32)
+        keep.append(a)
+        return a.ctypes.data_as(_FP)
+
+    def up16(a):
+        a = np.ascontiguousarray(a, np.uint16)
+        keep.append(a)
+        return a.ctypes.data_as(_U16P)
+
+    def up32(a):
+        a = np.ascontiguousarray(a, np.uint32)
+        keep.append(a)
+        return a.ctypes.data_as(_U32P)
+
+    H, W = inputs["index_map"].shape
+    c = ShoInputs()
+    c.width, c.height = W, H
+    c.fx, c.fy, c.cx, c.cy = inputs["fx"], inputs["fy"], inputs["cx"], inputs["cy"]
+    c.cam2world[:] = [float(v) for v in np.asarray(inputs["cam2world"], np.float32).reshape(16)]
+    c.measure_vertex = fp(inputs["measure_vertex"])
+    c.measure_normal = fp(inputs["measure_normal"])
+    c.ref_vertex = fp(inputs["ref_vertex"])
+    c.ref_normal = fp(inputs["ref_normal"])
+    c.index_map = up32(inputs["index_map"])
+    c.flow = fp(inputs.get("flow"))
+    K = int(inputs["K"])
+    c.K = K
+    c.num_surfels = int(inputs["surfel_knn"].shape[0])
+    c.surfel_knn = up16(inputs["surfel_knn"])
+    c.surfel_spatial = fp(inputs["surfel_spatial"])
+    c.surfel_connect = fp(inputs["surfel_connect"])
+    c.num_nodes = int(inputs["node_coords"].shape[0])
+    c.node_coords = fp(inputs["node_coords"])
+    c.num_edges = int(inputs["edges"].shape[0])
+    c.edges = up16(inputs["edges"])
+    c.node_connect = fp(inputs["node_connect"])
+    c.node_motion_pred = fp(inputs.get("node_motion_pred"))
+    c.node_radius_square = float(inputs["node_radius_square"])
+    p = ShoParams(params.gn_iterations, params.pcg_iterations, params.lambda_depth, params.lambda_flow,
+                  params.lambda_reg, params.lambda_node_translation, params.damping)
+    h = lib().sho_solve(ctypes.byref(c), ctypes.byref(p), precision, threads)
+    try:
+        out = {}
+        for name in RESULT_FIELDS + ["num_pixels", "solve_ms"]:
+            ptr = ctypes.c_void_p()
+            cnt = ctypes.c_size_t()
+            dt = ctypes.c_int()
+            rc = lib().sho_result_get(h, name.encode(), ctypes.byref(ptr), ctypes.byref(cnt), ctypes.byref(dt))
+            if rc != 0:
+                continue
+            typ = _DTYPES[dt.value]
+            if cnt.value == 0:
+                out[name] = np.zeros(0, typ)
+                continue
+            buf = (ctypes.c_char * (cnt.value * np.dtype(typ).itemsize)).from_address(ptr.value)
+            out[name] = np.frombuffer(buf, typ).copy()
+        out["num_pixels"] = int(out["num_pixels"][0])
+        out["solve_ms"] = float(out["solve_ms"][0])
+        return out
+    finally:
+        lib().sho_result_free(h)
+
+
+def dq_se3(dq8) -> np.ndarray:
+    a = np.ascontiguousarray(dq8, np.float64)
+    o = np.zeros(12, np.float64)
+    lib().sho_dq_se3(a.ctypes.data_as(_DP), o.ctypes.data_as(_DP))
+    return o.reshape(3, 4)
+
+
+def dq_from_mat34(m34) -> np.ndarray:
+    a = np.ascontiguousarray(m34, np.float64).reshape(12)
+    o = np.zeros(8, np.float64)
+    lib().sho_dq_from_mat34(a.ctypes.data_as(_DP), o.ctypes.data_as(_DP))
+    return o
+
+
+def twist_exp(xi6) -> np.ndarray:
+    a = np.ascontiguousarray(xi6, np.float64)
+    o = np.zeros(12, np.float64)
+    lib().sho_twist_exp(a.ctypes.data_as(_DP), o.ctypes.data_as(_DP))
+    return o.reshape(3, 4)
+
+
+def invert6(A) -> np.ndarray:
+    a = np.ascontiguousarray(A, np.float64).reshape(36)
+    o = np.zeros(36, np.float64)
+    lib().sho_invert6(a.ctypes.data_as(_DP), o.ctypes.data_as(_DP))
+    return o.reshape(6, 6)
+
+
+def round_to_half(f: float) -> float:
+    return float(lib().sho_round_to_half(f))
+
+
+def float2uint_rn(f: float) -> int:
+    return int(lib().sho_float2uint_rn(f))

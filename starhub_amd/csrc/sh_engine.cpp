@@ -1,0 +1,688 @@
+// Host driver of the warp-field solver: capacity allocation, the once-per-solve
+// index build and the Gauss-Newton loop over a batch of solves, behind the
+// C-ABI of include/sh/warp_solver.h.
+//
+// Mirrors WarpSolver (examples/mono_star/opt/WarpSolver.cu:50-389) without its
+// host synchronisation inside the GN loop: the reference synchronises four
+// streams and downloads residuals every iteration (DenseImageDebug.cu:5-27,
+// NodeGraphHandler.cu:111-139, WarpSolverPCG.cu:19-31); here the index build
+// reads back two counts (valid pixels, unique node pairs) and the GN loop runs
+// without any host round trip.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sh_common.hpp"
+#include "sh_engine.hpp"
+#include "sh_kernels.hpp"
+
+namespace sh
+{
+namespace
+{
+inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+#define SH_HIP(call)                                                                                                   \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        const hipError_t e_ = (call);                                                                                  \
+        if (e_ != hipSuccess)                                                                                          \
+        {                                                                                                              \
+            last_hip_error = e_;                                                                                       \
+            return SH_E_HIP;                                                                                           \
+        }                                                                                                              \
+    } while (0)
+} // namespace
+
+hipError_t last_hip_error = hipSuccess;
+
+Engine::~Engine() { release(); }
+
+void Engine::release()
+{
+    for (void *p : allocs_)
+        (void)hipFree(p);
+    allocs_.clear();
+    if (h_meta_)
+        (void)hipHostFree(h_meta_);
+    h_meta_ = nullptr;
+    if (h_stage_)
+        (void)hipHostFree(h_stage_);
+    h_stage_ = nullptr;
+    if (graph_exec_)
+        (void)hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+}
+
+int Engine::alloc(void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0)
+        bytes = 16;
+    bytes = (bytes + 255) & ~size_t(255);
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess)
+    {
+        last_hip_error = e;
+        *p = nullptr;
+        return SH_E_NOMEM;
+    }
+    allocs_.push_back(*p);
+    return SH_OK;
+}
+
+int Engine::init(const sh_solver_config &c, uint32_t max_slots)
+{
+    if (c.knn != 4 && c.knn != 8)
+        return SH_E_ARG;
+    if (c.max_nodes < c.knn || c.max_nodes > kPairBase || c.max_width == 0 || c.max_height == 0 || max_slots == 0)
+        return SH_E_ARG;
+    if (c.pcg_iterations == 0 || c.pcg_iterations > 1000 || c.gn_iterations > 1000)
+        return SH_E_ARG;
+    if (c.stream_precision > SH_STREAM_FP16 || c.spmv_mode > SH_SPMV_AUTO)
+        return SH_E_ARG;
+    cfg_ = c;
+    K_ = int(c.knn);
+    fp16_ = (c.stream_precision == SH_STREAM_FP16);
+    const uint32_t np = uint32_t(K_ * (K_ - 1) / 2);
+    pcap_ = c.max_width * c.max_height;
+    if (uint64_t(pcap_) >= (1ull << 25))
+        return SH_E_ARG; // dense pair entries carry the pixel index in 25 bits
+    ncap_ = c.max_nodes;
+    ecap_ = ncap_ * uint32_t(K_);
+    ucap_ = std::min<uint64_t>(uint64_t(ncap_) * (ncap_ - 1) / 2, uint64_t(ncap_) * 96u);
+    n2t_cap_ = pcap_ * uint32_t(K_) + 2 * ecap_ + ncap_;
+    pair_cap_ = std::max<uint64_t>(uint64_t(pcap_) * np + ecap_, 2ull * ucap_);
+    if (pair_cap_ >= (1ull << 31))
+        return SH_E_ARG;
+
+    slots_.resize(max_slots);
+    int st;
+    for (uint32_t s = 0; s < max_slots; ++s)
+    {
+        Slot &sl = slots_[s];
+        SolveDesc &d = sl.d;
+        std::memset(&d, 0, sizeof(d));
+        d.K = K_;
+        d.pcg_iters = int(c.pcg_iterations);
+#define A(ptr, bytes)                                                                                                  \
+    if ((st = alloc(reinterpret_cast<void **>(&(ptr)), (bytes))) != SH_OK)                                            \
+        return st;
+        A(d.pix, size_t(pcap_) * 8);
+        A(d.pknn, size_t(pcap_) * K_ * 2);
+        A(d.pw, size_t(pcap_) * K_ * 4);
+        A(d.n2t_off, size_t(ncap_ + 1) * 4);
+        A(d.n2t_terms, size_t(n2t_cap_) * 4);
+        A(d.half_keys, size_t(ucap_) * 4);
+        A(d.half_off, size_t(ucap_ + 1) * 4);
+        A(d.pair_entries, size_t(pcap_) * np * 4 + size_t(ecap_) * 4);
+        A(d.upper_pos, size_t(ucap_) * 4);
+        A(d.lower_pos, size_t(ucap_) * 4);
+        A(d.row_ptr, size_t(ncap_ + 1) * 4);
+        A(d.blk_col, (size_t(ncap_) + 2 * size_t(ucap_)) * 4);
+        A(d.dq[0], size_t(ncap_) * 32);
+        A(d.dq[1], size_t(ncap_) * 32);
+        A(d.jr, size_t(pcap_) * kJR * (fp16_ ? 2 : 4));
+        A(d.reg, size_t(ecap_) * kReg * 4);
+        A(d.ttrans, size_t(ncap_) * 16);
+        A(d.blk, (size_t(ncap_) + 2 * size_t(ucap_)) * 36 * 4);
+        A(d.dinv, size_t(ncap_) * 36 * 8);
+        A(d.rhs, size_t(ncap_) * 6 * 4);
+        A(d.x, size_t(ncap_) * 6 * 4);
+        A(d.r, size_t(ncap_) * 6 * 4);
+        A(d.s, size_t(ncap_) * 6 * 4);
+        A(d.q, size_t(ncap_) * 6 * 4);
+        A(d.dv[0], size_t(ncap_) * 6 * 4);
+        A(d.dv[1], size_t(ncap_) * 6 * 4);
+        A(d.delta, size_t(c.pcg_iterations + 1) * 8);
+        A(d.cost, size_t(c.gn_iterations + 2) * 8);
+    }
+    // shared index-build workspace
+    const size_t npix = size_t(pcap_);
+    A(ws_flags_, npix * 4);
+    A(ws_pos_, npix * 4);
+    A(ws_k16a_, size_t(n2t_cap_) * 2);
+    A(ws_k16b_, size_t(n2t_cap_) * 2);
+    A(ws_v32a_, size_t(n2t_cap_) * 4);
+    A(ws_k32a_, size_t(pair_cap_) * 4);
+    A(ws_k32b_, size_t(pair_cap_) * 4);
+    A(ws_p32a_, size_t(pair_cap_) * 4);
+    A(ws_p32b_, size_t(pair_cap_) * 4);
+    A(ws_head_, size_t(pair_cap_) * 4);
+    A(ws_hpos_, size_t(pair_cap_) * 4);
+    A(ws_meta_, 64);
+    {
+        size_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+        SH_HIP(sort_pairs_u16_u32(nullptr, b0, nullptr, nullptr, nullptr, nullptr, n2t_cap_, 16, nullptr));
+        SH_HIP(sort_pairs_u32_u32(nullptr, b1, nullptr, nullptr, nullptr, nullptr, uint32_t(pair_cap_), 24, nullptr));
+        SH_HIP(exclusive_scan_u32(nullptr, b2, nullptr, nullptr, uint32_t(pair_cap_), nullptr));
+        SH_HIP(exclusive_scan_u32(nullptr, b3, nullptr, nullptr, uint32_t(npix), nullptr));
+        sort_temp_bytes_ = std::max(std::max(b0, b1), std::max(b2, b3));
+        A(sort_temp_, sort_temp_bytes_);
+    }
+    // batch-wide launch state
+    A(d_descs_, size_t(max_slots) * sizeof(SolveDesc));
+    A(d_wmaps_, size_t(kNumMaps) * (max_slots + 1) * 4);
+    const size_t pcg_wg = size_t(max_slots) * div_up(ncap_, kPcgNodesPerWg);
+    A(d_part_rs_, pcg_wg * 8);
+    A(d_part_dq_, pcg_wg * 8);
+#undef A
+    SH_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_meta_), 64, hipHostMallocDefault));
+    stage_bytes_ = size_t(max_slots) * sizeof(SolveDesc) + size_t(kNumMaps) * (max_slots + 1) * 4;
+    SH_HIP(hipHostMalloc(&h_stage_, stage_bytes_, hipHostMallocDefault));
+    return SH_OK;
+}
+
+int Engine::set_inputs(uint32_t slot, const sh_camera *cam, const sh_measure_maps *meas, const sh_render_maps *rend,
+                       const sh_geometry *geo, const sh_node_graph *ng, const sh_node_flow *nf, const sh_flow_map *flow,
+                       const sh_keypoints *kp, const float cam2world[16])
+{
+    (void)kp;
+    if (slot >= slots_.size() || !cam || !meas || !rend || !geo || !ng || !cam2world)
+        return SH_E_ARG;
+    if (!meas->vertex_confid || !meas->normal_radius || !rend->reference_vertex || !rend->reference_normal ||
+        !rend->index_map || !geo->surfel_knn || !geo->surfel_knn_spatial_weight || !geo->surfel_knn_connect_weight ||
+        !ng->reference_node_coords || !ng->node_graph || !ng->node_knn_connect_weight)
+        return SH_E_ARG;
+    if (cam->width == 0 || cam->height == 0 || ng->num_nodes == 0)
+        return SH_E_ARG;
+    if (cam->width > cfg_.max_width || cam->height > cfg_.max_height || ng->num_nodes > ncap_ ||
+        ng->num_edges > ecap_ || cam->width > 65535 || cam->height > 65535)
+        return SH_E_CAPACITY;
+    Slot &sl = slots_[slot];
+    SolveDesc &d = sl.d;
+    d.W = int(cam->width);
+    d.H = int(cam->height);
+    d.fx = cam->fx;
+    d.fy = cam->fy;
+    d.cx = cam->cx;
+    d.cy = cam->cy;
+    // world2camera = inverse(camera2world) for a rigid transform (WarpSolver.cu:151-154)
+    double R[3][3], t[3];
+    for (int a = 0; a < 3; ++a)
+    {
+        for (int b = 0; b < 3; ++b)
+            R[a][b] = cam2world[4 * a + b];
+        t[a] = cam2world[4 * a + 3];
+    }
+    for (int a = 0; a < 3; ++a)
+    {
+        for (int b = 0; b < 3; ++b)
+            d.w2c_r[3 * a + b] = float(R[b][a]);
+        d.w2c_t[a] = float(-(R[0][a] * t[0] + R[1][a] * t[1] + R[2][a] * t[2]));
+    }
+    d.lam2_depth = cfg_.lambda_depth * cfg_.lambda_depth;
+    d.lam2_flow = cfg_.lambda_flow * cfg_.lambda_flow;
+    d.lam2_reg = cfg_.lambda_reg * cfg_.lambda_reg;
+    d.lam2_nt = cfg_.lambda_node_translation * cfg_.lambda_node_translation;
+    d.damping = cfg_.damping;
+    d.node_radius_square = ng->node_radius_square;
+    d.N = int(ng->num_nodes);
+    d.E = int(ng->num_edges);
+    d.meas_v = reinterpret_cast<const float4 *>(meas->vertex_confid);
+    d.meas_n = reinterpret_cast<const float4 *>(meas->normal_radius);
+    d.ref_v = reinterpret_cast<const float4 *>(rend->reference_vertex);
+    d.ref_n = reinterpret_cast<const float4 *>(rend->reference_normal);
+    d.index_map = rend->index_map;
+    d.flow = (flow && flow->flow) ? reinterpret_cast<const float2 *>(flow->flow) : nullptr;
+    d.surfel_knn = geo->surfel_knn;
+    d.surfel_spatial = geo->surfel_knn_spatial_weight;
+    d.surfel_connect = geo->surfel_knn_connect_weight;
+    d.node_coords = reinterpret_cast<const float4 *>(ng->reference_node_coords);
+    d.edges = ng->node_graph;
+    d.node_connect = ng->node_knn_connect_weight;
+    d.node_pred = (nf && nf->node_motion_pred) ? reinterpret_cast<const float4 *>(nf->node_motion_pred) : nullptr;
+    d.P = 0;
+    d.U = 0;
+    d.n_pair_entries = 0;
+    sl.has_inputs = true;
+    sl.indexed = false;
+    sl.solved = false;
+    return SH_OK;
+}
+
+// Once-per-solve index build (WarpSolver::buildSolverIndexStreamed, WarpSolver.cu:194-223).
+int Engine::build_index(uint32_t slot, hipStream_t st)
+{
+    Slot &sl = slots_[slot];
+    SolveDesc &d = sl.d;
+    const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
+    size_t tb = sort_temp_bytes_;
+    uint32_t *flags = static_cast<uint32_t *>(ws_flags_), *pos = static_cast<uint32_t *>(ws_pos_);
+    SH_HIP(launch_pixel_flags(d.index_map, n, flags, st));
+    SH_HIP(exclusive_scan_u32(sort_temp_, tb, flags, pos, n, st));
+    SH_HIP(hipMemcpyAsync(h_meta_ + 0, pos + (n - 1), 4, hipMemcpyDeviceToHost, st));
+    SH_HIP(hipMemcpyAsync(h_meta_ + 1, flags + (n - 1), 4, hipMemcpyDeviceToHost, st));
+    SH_HIP(hipStreamSynchronize(st));
+    const uint32_t P = h_meta_[0] + h_meta_[1];
+    if (P > pcap_)
+        return SH_E_CAPACITY;
+    d.P = int(P);
+    SH_HIP(launch_compact_pixels(d, flags, pos, st));
+
+    // node -> term (Node2TermsIndex::BuildIndex, Node2TermsIndex.cu:191-199,312-360)
+    const uint32_t n_kv = P * uint32_t(K_) + 2 * uint32_t(d.E) + uint32_t(d.N);
+    uint16_t *k16a = static_cast<uint16_t *>(ws_k16a_), *k16b = static_cast<uint16_t *>(ws_k16b_);
+    uint32_t *v32a = static_cast<uint32_t *>(ws_v32a_);
+    SH_HIP(launch_node2term_kv(d, k16a, v32a, st));
+    tb = sort_temp_bytes_;
+    SH_HIP(sort_pairs_u16_u32(sort_temp_, tb, k16a, k16b, v32a, d.n2t_terms, n_kv, 16, st));
+    SH_HIP(launch_lower_bound_u16(k16b, n_kv, uint32_t(d.N) + 1, d.n2t_off, st));
+
+    // node pair -> term, half index (NodePair2TermsIndex::BuildHalfIndex, :294-298,591-654)
+    const uint32_t np = uint32_t(K_ * (K_ - 1) / 2);
+    const uint32_t n_pkv = P * np + uint32_t(d.E);
+    uint32_t *k32a = static_cast<uint32_t *>(ws_k32a_), *k32b = static_cast<uint32_t *>(ws_k32b_);
+    uint32_t *p32a = static_cast<uint32_t *>(ws_p32a_), *p32b = static_cast<uint32_t *>(ws_p32b_);
+    uint32_t *head = static_cast<uint32_t *>(ws_head_), *hpos = static_cast<uint32_t *>(ws_hpos_);
+    uint32_t *meta = static_cast<uint32_t *>(ws_meta_);
+    SH_HIP(launch_pair_kv(d, k32a, p32a, st));
+    tb = sort_temp_bytes_;
+    SH_HIP(sort_pairs_u32_u32(sort_temp_, tb, k32a, k32b, p32a, p32b, n_pkv, 24, st));
+    SH_HIP(hipMemsetAsync(meta, 0, 16, st));
+    SH_HIP(launch_pair_heads(k32b, n_pkv, head, meta, st));
+    tb = sort_temp_bytes_;
+    SH_HIP(exclusive_scan_u32(sort_temp_, tb, head, hpos, n_pkv, st));
+    // the half arrays are written up to index U, bounded by the capacity check below
+    // only after U is known; write into the workspace first
+    SH_HIP(launch_pair_compact(k32b, head, hpos, n_pkv, meta, k32a, p32a, meta + 2, st));
+    SH_HIP(hipMemcpyAsync(h_meta_ + 2, meta + 2, 8, hipMemcpyDeviceToHost, st));
+    SH_HIP(hipStreamSynchronize(st));
+    const uint32_t U = h_meta_[2];
+    if (U > ucap_)
+        return SH_E_CAPACITY;
+    d.U = int(U);
+    d.n_pair_entries = int(h_meta_[3]);
+    SH_HIP(hipMemcpyAsync(d.half_keys, k32a, size_t(U) * 4, hipMemcpyDeviceToDevice, st));
+    SH_HIP(hipMemcpyAsync(d.half_off, p32a, size_t(U + 1) * 4, hipMemcpyDeviceToDevice, st));
+    SH_HIP(launch_pair_entries(d, p32b, st));
+
+    // symmetric index + block rows (BuildSymmetricAndRowBlocksIndex, :300-312)
+    SH_HIP(launch_sym_kv(d, k32a, p32a, st));
+    tb = sort_temp_bytes_;
+    if (U > 0)
+        SH_HIP(sort_pairs_u32_u32(sort_temp_, tb, k32a, k32b, p32a, p32b, 2 * U, 24, st));
+    SH_HIP(launch_sym_scatter(d, k32b, p32b, st));
+    SH_HIP(launch_row_ptr(d, k32b, st));
+    sl.indexed = true;
+    return SH_OK;
+}
+
+void Engine::build_maps(uint32_t ns)
+{
+    maps_.assign(size_t(kNumMaps) * (ns + 1), 0u);
+    for (int m = 0; m < kNumMaps; ++m)
+    {
+        uint32_t acc = 0;
+        for (uint32_t s = 0; s < ns; ++s)
+        {
+            const SolveDesc &d = slots_[s].d;
+            uint32_t wg = 0;
+            switch (m)
+            {
+            case kMapPixel:
+                wg = div_up(uint32_t(d.P), kThreads);
+                break;
+            case kMapReg:
+                wg = div_up(uint32_t(d.E + d.N), kThreads);
+                break;
+            case kMapNodeWave:
+                wg = div_up(uint32_t(d.N), kWavesPerWg);
+                break;
+            case kMapPairWave:
+                wg = div_up(uint32_t(d.U), kWavesPerWg);
+                break;
+            case kMapPcg:
+                wg = div_up(uint32_t(d.N), kPcgNodesPerWg);
+                break;
+            case kMapNode:
+                wg = div_up(uint32_t(d.N), kThreads);
+                break;
+            }
+            maps_[size_t(m) * (ns + 1) + s] = acc;
+            acc += wg;
+        }
+        maps_[size_t(m) * (ns + 1) + ns] = acc;
+        map_total_[m] = acc;
+    }
+}
+
+int Engine::upload(uint32_t ns, hipStream_t st)
+{
+    char *h = static_cast<char *>(h_stage_);
+    for (uint32_t s = 0; s < ns; ++s)
+        std::memcpy(h + size_t(s) * sizeof(SolveDesc), &slots_[s].d, sizeof(SolveDesc));
+    const size_t mb = maps_.size() * 4;
+    std::memcpy(h + size_t(ns) * sizeof(SolveDesc), maps_.data(), mb);
+    SH_HIP(hipMemcpyAsync(d_descs_, h, size_t(ns) * sizeof(SolveDesc), hipMemcpyHostToDevice, st));
+    SH_HIP(hipMemcpyAsync(d_wmaps_, h + size_t(ns) * sizeof(SolveDesc), mb, hipMemcpyHostToDevice, st));
+    return SH_OK;
+}
+
+const uint32_t *Engine::map(int m, uint32_t ns) const
+{
+    return static_cast<const uint32_t *>(d_wmaps_) + size_t(m) * (ns + 1);
+}
+
+// The Gauss-Newton loop (WarpSolver::SolveStreamed :260-337) for slots [0, ns).
+int Engine::enqueue_gn(uint32_t ns, hipStream_t st)
+{
+    const SolveDesc *sds = static_cast<const SolveDesc *>(d_descs_);
+    const int n = int(ns);
+    const bool fp16 = fp16_;
+    SH_HIP(launch_identity(sds, map(kMapNode, ns), n, map_total_[kMapNode], st));
+    double *prs = static_cast<double *>(d_part_rs_), *pdq = static_cast<double *>(d_part_dq_);
+    for (uint32_t it = 0; it < cfg_.gn_iterations; ++it)
+    {
+        const int parity = int(it & 1u);
+        SH_HIP(launch_dense_terms(K_, fp16, sds, map(kMapPixel, ns), n, map_total_[kMapPixel], parity, st));
+        SH_HIP(launch_reg_terms(sds, map(kMapReg, ns), n, map_total_[kMapReg], parity, st));
+        SH_HIP(launch_diag_rhs(K_, fp16, sds, map(kMapNodeWave, ns), n, map_total_[kMapNodeWave], st));
+        SH_HIP(launch_dinv(sds, map(kMapNode, ns), n, map_total_[kMapNode], st));
+        SH_HIP(launch_offdiag(K_, fp16, sds, map(kMapPairWave, ns), n, map_total_[kMapPairWave], st));
+        SH_HIP(launch_pcg_init(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], prs, st));
+        for (uint32_t k = 0; k < cfg_.pcg_iterations; ++k)
+        {
+            SH_HIP(launch_pcg_spmv(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), prs, pdq, st));
+            SH_HIP(launch_pcg_update(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), pdq, prs, st));
+        }
+        SH_HIP(launch_update(sds, map(kMapNode, ns), n, map_total_[kMapNode], parity, st));
+    }
+    return SH_OK;
+}
+
+int Engine::solve(uint32_t ns, hipStream_t st)
+{
+    if (ns == 0 || ns > slots_.size())
+        return SH_E_ARG;
+    for (uint32_t s = 0; s < ns; ++s)
+        if (!slots_[s].has_inputs)
+            return SH_E_STATE;
+    int rc;
+    for (uint32_t s = 0; s < ns; ++s)
+        if ((rc = build_index(s, st)) != SH_OK)
+            return rc;
+    build_maps(ns);
+    if ((rc = upload(ns, st)) != SH_OK)
+        return rc;
+    if ((rc = enqueue_gn(ns, st)) != SH_OK)
+        return rc;
+    SH_HIP(hipStreamSynchronize(st));
+    for (uint32_t s = 0; s < ns; ++s)
+        slots_[s].solved = true;
+    return SH_OK;
+}
+
+const dq8 *Engine::result(uint32_t slot) const
+{
+    return slots_[slot].d.dq[cfg_.gn_iterations & 1u];
+}
+
+int Engine::debug_view(uint32_t slot, const char *name, const void **ptr, size_t *count) const
+{
+    if (slot >= slots_.size() || !name || !ptr || !count)
+        return SH_E_ARG;
+    const SolveDesc &d = slots_[slot].d;
+    const std::string n(name);
+    const size_t P = size_t(d.P), N = size_t(d.N), U = size_t(d.U), E = size_t(d.E), K = size_t(K_);
+    struct V
+    {
+        const char *name;
+        const void *p;
+        size_t c;
+    };
+    const V views[] = {
+        {"pix", d.pix, P * 4},
+        {"pknn", d.pknn, P * K},
+        {"pw", d.pw, P * K},
+        {"n2t_off", d.n2t_off, N + 1},
+        {"n2t_terms", d.n2t_terms, P * K + 2 * E + N},
+        {"half_keys", d.half_keys, U},
+        {"half_off", d.half_off, U + 1},
+        {"pair_entries", d.pair_entries, size_t(d.n_pair_entries)},
+        {"upper_pos", d.upper_pos, U},
+        {"lower_pos", d.lower_pos, U},
+        {"row_ptr", d.row_ptr, N + 1},
+        {"blk_col", d.blk_col, N + 2 * U},
+        {"jr", d.jr, P * kJR},
+        {"reg", d.reg, E * kReg},
+        {"blk", d.blk, (N + 2 * U) * 36},
+        {"dinv", d.dinv, N * 36},
+        {"rhs", d.rhs, N * 6},
+        {"x", d.x, N * 6},
+        {"dq0", d.dq[0], N * 8},
+        {"dq1", d.dq[1], N * 8},
+        {"delta", d.delta, size_t(cfg_.pcg_iterations) + 1},
+    };
+    for (const V &v : views)
+        if (n == v.name)
+        {
+            *ptr = v.p;
+            *count = v.c;
+            return SH_OK;
+        }
+    return SH_E_ARG;
+}
+
+} // namespace sh
+
+// ======================================================================= C-ABI
+struct sh_solver
+{
+    sh::Engine eng;
+};
+struct sh_batch
+{
+    sh::Engine eng;
+};
+
+extern "C"
+{
+    const char *sh_version(void) { return "starhub_amd 0.1.0 gfx950"; }
+
+    const char *sh_status_string(int s)
+    {
+        switch (s)
+        {
+        case SH_OK:
+            return "ok";
+        case SH_E_ARG:
+            return "invalid argument";
+        case SH_E_CAPACITY:
+            return "capacity exceeded";
+        case SH_E_HIP:
+            return "HIP runtime error";
+        case SH_E_NAN:
+            return "non-finite system";
+        case SH_E_STATE:
+            return "call out of order";
+        case SH_E_NOMEM:
+            return "device allocation failed";
+        default:
+            return "unknown status";
+        }
+    }
+
+    int sh_solver_create(const sh_solver_config *config, sh_solver **out)
+    {
+        if (!config || !out)
+            return SH_E_ARG;
+        *out = nullptr;
+        sh_solver *s = new sh_solver();
+        const int rc = s->eng.init(*config, 1);
+        if (rc != SH_OK)
+        {
+            delete s;
+            return rc;
+        }
+        *out = s;
+        return SH_OK;
+    }
+
+    int sh_solver_destroy(sh_solver *solver)
+    {
+        delete solver;
+        return SH_OK;
+    }
+
+    int sh_solver_set_inputs(sh_solver *solver, const sh_camera *camera, const sh_measure_maps *measure,
+                             const sh_render_maps *render, const sh_geometry *geometry,
+                             const sh_node_graph *node_graph, const sh_node_flow *node_flow, const sh_flow_map *flow,
+                             const sh_keypoints *keypoints, const float cam2world[16])
+    {
+        if (!solver)
+            return SH_E_ARG;
+        return solver->eng.set_inputs(0, camera, measure, render, geometry, node_graph, node_flow, flow, keypoints,
+                                      cam2world);
+    }
+
+    int sh_solver_solve(sh_solver *solver, sh_stream_t stream)
+    {
+        if (!solver)
+            return SH_E_ARG;
+        return solver->eng.solve(1, reinterpret_cast<hipStream_t>(stream));
+    }
+
+    int sh_solver_node_se3(const sh_solver *solver, const sh_dual_quat **view, uint32_t *num_nodes)
+    {
+        if (!solver || !view || !num_nodes)
+            return SH_E_ARG;
+        if (!solver->eng.solved(0))
+            return SH_E_STATE;
+        *view = reinterpret_cast<const sh_dual_quat *>(solver->eng.result(0));
+        *num_nodes = uint32_t(solver->eng.desc(0).N);
+        return SH_OK;
+    }
+
+    int sh_solver_sizes(const sh_solver *solver, uint32_t *num_pixels, uint32_t *num_pairs, uint32_t *num_node_terms)
+    {
+        if (!solver)
+            return SH_E_ARG;
+        const sh::SolveDesc &d = solver->eng.desc(0);
+        if (num_pixels)
+            *num_pixels = uint32_t(d.P);
+        if (num_pairs)
+            *num_pairs = uint32_t(d.U);
+        if (num_node_terms)
+            *num_node_terms = uint32_t(d.P * d.K + 2 * d.E + d.N);
+        return SH_OK;
+    }
+
+    int sh_solver_debug_view(const sh_solver *solver, const char *name, const void **device_ptr, size_t *count)
+    {
+        if (!solver)
+            return SH_E_ARG;
+        return solver->eng.debug_view(0, name, device_ptr, count);
+    }
+
+    int sh_solver_cost_history(const sh_solver *solver, double *out, uint32_t capacity, uint32_t *count)
+    {
+        (void)solver;
+        (void)out;
+        (void)capacity;
+        if (count)
+            *count = 0;
+        return SH_E_STATE;
+    }
+
+    int sh_solver_apply_jtj(sh_solver *solver, const float *x, float *y, uint32_t matrix_free, sh_stream_t stream)
+    {
+        (void)solver;
+        (void)x;
+        (void)y;
+        (void)matrix_free;
+        (void)stream;
+        return SH_E_STATE;
+    }
+
+    int sh_skin_surfels(const float *surfel_vertex4, uint32_t num_surfels, const float *node_coords4,
+                        uint32_t num_nodes, uint32_t knn, float node_radius_square, uint16_t *surfel_knn,
+                        float *spatial_weight, float *connect_weight, sh_stream_t stream)
+    {
+        if ((knn != 4 && knn != 8) || num_nodes < knn || num_nodes > 65535 || !surfel_vertex4 || !node_coords4 ||
+            !surfel_knn || !spatial_weight || !connect_weight)
+            return SH_E_ARG;
+        const hipError_t e = sh::launch_skin(knn, reinterpret_cast<const float4 *>(surfel_vertex4), num_surfels,
+                                             reinterpret_cast<const float4 *>(node_coords4), num_nodes,
+                                             node_radius_square, surfel_knn, spatial_weight, connect_weight,
+                                             reinterpret_cast<hipStream_t>(stream));
+        if (e != hipSuccess)
+        {
+            sh::last_hip_error = e;
+            return SH_E_HIP;
+        }
+        return SH_OK;
+    }
+
+    int sh_build_node_graph(const float *node_coords4, uint32_t num_nodes, uint32_t knn, float node_radius_square,
+                            uint16_t *node_knn, float *node_spatial_weight, uint16_t *node_graph,
+                            float *node_connect_weight, sh_stream_t stream)
+    {
+        if ((knn != 4 && knn != 8) || num_nodes < knn || num_nodes > 65535 || !node_coords4 || !node_knn ||
+            !node_spatial_weight || !node_graph || !node_connect_weight)
+            return SH_E_ARG;
+        const hipError_t e = sh::launch_node_graph(knn, reinterpret_cast<const float4 *>(node_coords4), num_nodes,
+                                                   node_radius_square, node_knn, node_spatial_weight, node_graph,
+                                                   node_connect_weight, reinterpret_cast<hipStream_t>(stream));
+        if (e != hipSuccess)
+        {
+            sh::last_hip_error = e;
+            return SH_E_HIP;
+        }
+        return SH_OK;
+    }
+
+    int sh_batch_create(const sh_solver_config *config, uint32_t max_solves, sh_batch **out)
+    {
+        if (!config || !out || max_solves == 0)
+            return SH_E_ARG;
+        *out = nullptr;
+        sh_batch *b = new sh_batch();
+        const int rc = b->eng.init(*config, max_solves);
+        if (rc != SH_OK)
+        {
+            delete b;
+            return rc;
+        }
+        *out = b;
+        return SH_OK;
+    }
+
+    int sh_batch_destroy(sh_batch *batch)
+    {
+        delete batch;
+        return SH_OK;
+    }
+
+    int sh_batch_set_inputs(sh_batch *batch, uint32_t slot, const sh_camera *camera, const sh_measure_maps *measure,
+                            const sh_render_maps *render, const sh_geometry *geometry,
+                            const sh_node_graph *node_graph, const sh_node_flow *node_flow, const sh_flow_map *flow,
+                            const sh_keypoints *keypoints, const float cam2world[16])
+    {
+        if (!batch)
+            return SH_E_ARG;
+        return batch->eng.set_inputs(slot, camera, measure, render, geometry, node_graph, node_flow, flow, keypoints,
+                                     cam2world);
+    }
+
+    int sh_batch_solve(sh_batch *batch, uint32_t num_solves, sh_stream_t stream)
+    {
+        if (!batch)
+            return SH_E_ARG;
+        return batch->eng.solve(num_solves, reinterpret_cast<hipStream_t>(stream));
+    }
+
+    int sh_batch_node_se3(const sh_batch *batch, uint32_t slot, const sh_dual_quat **view, uint32_t *num_nodes)
+    {
+        if (!batch || !view || !num_nodes)
+            return SH_E_ARG;
+        if (slot >= batch->eng.num_slots() || !batch->eng.solved(slot))
+            return SH_E_STATE;
+        *view = reinterpret_cast<const sh_dual_quat *>(batch->eng.result(slot));
+        *num_nodes = uint32_t(batch->eng.desc(slot).N);
+        return SH_OK;
+    }
+}

@@ -1,0 +1,85 @@
+// Host engine: capacity, per-slot solve state, index build and GN loop.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/sh/warp_solver.h"
+#include "sh_desc.hpp"
+
+namespace sh
+{
+struct dq8;
+extern hipError_t last_hip_error;
+
+class Engine
+{
+public:
+    Engine() = default;
+    ~Engine();
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+
+    int init(const sh_solver_config &cfg, uint32_t max_slots);
+    int set_inputs(uint32_t slot, const sh_camera *cam, const sh_measure_maps *meas, const sh_render_maps *rend,
+                   const sh_geometry *geo, const sh_node_graph *ng, const sh_node_flow *nf, const sh_flow_map *flow,
+                   const sh_keypoints *kp, const float cam2world[16]);
+    int solve(uint32_t num_solves, hipStream_t st);
+    const dq8 *result(uint32_t slot) const;
+    bool solved(uint32_t slot) const { return slot < slots_.size() && slots_[slot].solved; }
+    const SolveDesc &desc(uint32_t slot) const { return slots_[slot].d; }
+    uint32_t num_slots() const { return uint32_t(slots_.size()); }
+    int debug_view(uint32_t slot, const char *name, const void **ptr, size_t *count) const;
+    const sh_solver_config &config() const { return cfg_; }
+
+private:
+    struct Slot
+    {
+        SolveDesc d;
+        bool has_inputs = false;
+        bool indexed = false;
+        bool solved = false;
+    };
+    enum
+    {
+        kMapPixel = 0,
+        kMapReg,
+        kMapNodeWave,
+        kMapPairWave,
+        kMapPcg,
+        kMapNode,
+        kNumMaps
+    };
+
+    int alloc(void **p, size_t bytes);
+    void release();
+    int build_index(uint32_t slot, hipStream_t st);
+    void build_maps(uint32_t ns);
+    int upload(uint32_t ns, hipStream_t st);
+    int enqueue_gn(uint32_t ns, hipStream_t st);
+    const uint32_t *map(int m, uint32_t ns) const;
+
+    sh_solver_config cfg_{};
+    int K_ = 8;
+    bool fp16_ = false;
+    uint32_t pcap_ = 0, ncap_ = 0, ecap_ = 0, n2t_cap_ = 0;
+    uint64_t ucap_ = 0, pair_cap_ = 0;
+    std::vector<Slot> slots_;
+    std::vector<void *> allocs_;
+    // index workspace
+    void *ws_flags_ = nullptr, *ws_pos_ = nullptr, *ws_k16a_ = nullptr, *ws_k16b_ = nullptr, *ws_v32a_ = nullptr;
+    void *ws_k32a_ = nullptr, *ws_k32b_ = nullptr, *ws_p32a_ = nullptr, *ws_p32b_ = nullptr;
+    void *ws_head_ = nullptr, *ws_hpos_ = nullptr, *ws_meta_ = nullptr;
+    void *sort_temp_ = nullptr;
+    size_t sort_temp_bytes_ = 0;
+    uint32_t *h_meta_ = nullptr;
+    void *h_stage_ = nullptr;
+    size_t stage_bytes_ = 0;
+    // launch state
+    void *d_descs_ = nullptr, *d_wmaps_ = nullptr, *d_part_rs_ = nullptr, *d_part_dq_ = nullptr;
+    std::vector<uint32_t> maps_;
+    uint32_t map_total_[kNumMaps] = {0};
+    hipGraphExec_t graph_exec_ = nullptr;
+};
+} // namespace sh

@@ -1,0 +1,1614 @@
+// HIP kernels of the warp-field Gauss-Newton solve, hand-written for gfx950.
+//
+// Launch model: every GN-phase kernel takes the device array of SolveDesc and a
+// work map `wg_begin` (n_solve + 1 prefix offsets of work-groups per solve), so
+// one launch covers a whole batch of uneven solves.  Work-groups never straddle
+// two solves.  No float atomics anywhere: all reductions are fixed-order wave
+// (DPP / permute) and work-group trees, so results are deterministic and a
+// solve gives identical bits alone or inside a batch.
+#include "sh_common.hpp"
+#include "sh_desc.hpp"
+#include "sh_kernels.hpp"
+
+namespace sh
+{
+__device__ __forceinline__ int find_solve(const uint32_t *__restrict__ begin, int n, uint32_t wg)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1)
+    {
+        const int mid = (lo + hi) >> 1;
+        if (begin[mid] <= wg)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// __float2uint_rn with saturation, NaN -> 0 (ImageTermKNNFetcher.cu:164-165)
+__device__ __forceinline__ uint32_t f2u_rn_sat(float v)
+{
+    if (!(v > 0.f))
+        return 0u;
+    if (v >= 4294967296.0f)
+        return 0xffffffffu;
+    return uint32_t(rintf(v));
+}
+
+// ============================================================ index build
+__global__ void k_pixel_flags(const uint32_t *__restrict__ index_map, uint32_t n, uint32_t *__restrict__ flags)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        flags[i] = (index_map[i] != kInvalidIndex) ? 1u : 0u;
+}
+
+// Stream-compact the valid pixels in row-major order; gathers surfel KNN and
+// folds spatial*connect into one weight per slot (queryPixelKNNKernel,
+// WarpSolver.cu:7-48, + compactPontentialImageTermPixelsKernel,
+// ImageTermKNNFetcher.cu:56-88).
+template <int K>
+__global__ void k_compact_pixels(SolveDesc d, const uint32_t *__restrict__ flags, const uint32_t *__restrict__ pos)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
+    if (i >= n || flags[i] == 0u)
+        return;
+    const uint32_t o = pos[i];
+    const uint32_t x = i % uint32_t(d.W), y = i / uint32_t(d.W);
+    const uint32_t s = d.index_map[i];
+    float fx = 0.f, fy = 0.f;
+    if (d.flow)
+    {
+        const float2 f = d.flow[i];
+        fx = f.x;
+        fy = f.y;
+    }
+    const uint32_t tx = f2u_rn_sat(fx + float(x));
+    const uint32_t ty = f2u_rn_sat(fy + float(y));
+    d.pix[o] = make_ushort4(uint16_t(x), uint16_t(y), uint16_t(tx), uint16_t(ty));
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+    {
+        d.pknn[size_t(o) * K + k] = d.surfel_knn[size_t(s) * K + k];
+        d.pw[size_t(o) * K + k] = d.surfel_spatial[size_t(s) * K + k] * d.surfel_connect[size_t(s) * K + k];
+    }
+}
+
+// (node key, term) pairs: K per pixel, 2 per ARAP edge (self edge -> 0xffff),
+// 1 per node (node-translation term).  Node2TermsIndex.cu:6-76
+template <int K>
+__global__ void k_node2term_kv(SolveDesc d, uint16_t *__restrict__ keys, uint32_t *__restrict__ vals)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t P = d.P, E = d.E, N = d.N;
+    if (t < P)
+    {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+        {
+            keys[size_t(t) * K + k] = d.pknn[size_t(t) * K + k];
+            vals[size_t(t) * K + k] = t;
+        }
+    }
+    else if (t < P + E)
+    {
+        const uint32_t e = t - P;
+        const uint16_t i = d.edges[3 * e], j = d.edges[3 * e + 1];
+        const size_t o = size_t(P) * K + 2 * size_t(e);
+        keys[o] = (i != j) ? i : 0xffff;
+        keys[o + 1] = (i != j) ? j : 0xffff;
+        vals[o] = t;
+        vals[o + 1] = t;
+    }
+    else if (t < P + E + N)
+    {
+        const uint32_t n = t - P - E;
+        const size_t o = size_t(P) * K + 2 * size_t(E) + n;
+        keys[o] = uint16_t(n);
+        vals[o] = t;
+    }
+}
+
+// offset[n] = first sorted position whose key >= n  (computeTermOffsetKernel,
+// Node2TermsIndex.cu:78-122, written as a lower bound)
+__global__ void k_lower_bound_u16(const uint16_t *__restrict__ keys, uint32_t n_keys, uint32_t n_out,
+                                  uint32_t *__restrict__ out)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n_out)
+        return;
+    uint32_t lo = 0, hi = n_keys;
+    while (lo < hi)
+    {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (uint32_t(keys[mid]) < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    out[v] = lo;
+}
+
+// (i<j) pair keys i*4096+j per term, padded with the uninitialised key:
+// K(K-1)/2 per pixel, 1 per edge.  NodePair2TermsIndex.cu:11-92
+template <int K>
+__global__ void k_pair_kv(SolveDesc d, uint32_t *__restrict__ keys, uint32_t *__restrict__ vals)
+{
+    constexpr int NP = K * (K - 1) / 2;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t P = d.P, E = d.E;
+    if (t < P)
+    {
+        uint16_t kn[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            kn[k] = d.pknn[size_t(t) * K + k];
+        const size_t base = size_t(t) * NP;
+        // at most NP strictly ordered pairs among K slots; fewer with duplicate ids
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int a = 0; a < K; ++a)
+#pragma unroll
+            for (int c = 0; c < K; ++c)
+                if (kn[a] < kn[c])
+                {
+                    keys[base + cnt] = uint32_t(kn[a]) * kPairBase + kn[c];
+                    ++cnt;
+                }
+        for (uint32_t a = cnt; a < uint32_t(NP); ++a)
+            keys[base + a] = kUninitKey;
+#pragma unroll
+        for (int a = 0; a < NP; ++a)
+            vals[base + a] = t;
+    }
+    else if (t < P + E)
+    {
+        const uint32_t e = t - P;
+        const uint32_t i = d.edges[3 * e], j = d.edges[3 * e + 1];
+        uint32_t key = kUninitKey;
+        if (i < j)
+            key = i * kPairBase + j;
+        else if (j < i)
+            key = j * kPairBase + i;
+        const size_t o = size_t(P) * NP + e;
+        keys[o] = key;
+        vals[o] = t;
+    }
+}
+
+// Segment heads of the sorted pair keys (segmentNodePairKernel,
+// NodePair2TermsIndex.cu:94-128); `meta[0]` receives the number of valid
+// (initialised) keys.
+__global__ void k_pair_heads(const uint32_t *__restrict__ keys, uint32_t n, uint32_t *__restrict__ head,
+                             uint32_t *__restrict__ meta)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t k = keys[i];
+    const bool valid = (k != kUninitKey);
+    head[i] = (valid && (i == 0 || keys[i - 1] != k)) ? 1u : 0u;
+    if (valid && (i + 1 == n || keys[i + 1] == kUninitKey))
+        meta[0] = i + 1;
+    if (i == 0 && !valid)
+        meta[0] = 0;
+}
+
+// compactNodePairKeyKernel (NodePair2TermsIndex.cu:130-169)
+__global__ void k_pair_compact(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ head,
+                               const uint32_t *__restrict__ hpos, uint32_t n, const uint32_t *__restrict__ meta,
+                               uint32_t *__restrict__ half_keys, uint32_t *__restrict__ half_off,
+                               uint32_t *__restrict__ meta_out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && head[i])
+    {
+        half_keys[hpos[i]] = keys[i];
+        half_off[hpos[i]] = i;
+    }
+    if (i == 0)
+    {
+        const uint32_t U = (n > 0) ? hpos[n - 1] + head[n - 1] : 0u;
+        half_off[U] = meta[0];
+        meta_out[0] = U;
+        meta_out[1] = meta[0];
+    }
+}
+
+// Packed (pair, term) entries: dense -> pixel | slot_i << 25 | slot_j << 28,
+// ARAP -> 0x80000000 | (row node == edge.x) << 30 | edge.
+// Slots use the last matching position (dense_image_jacobian.cuh:92-104).
+template <int K>
+__global__ void k_pair_entries(SolveDesc d, const uint32_t *__restrict__ sorted_vals)
+{
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= uint32_t(d.n_pair_entries))
+        return;
+    // h = last half pair with half_off[h] <= e
+    uint32_t lo = 0, hi = uint32_t(d.U);
+    while (hi - lo > 1)
+    {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (d.half_off[mid] <= e)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint32_t key = d.half_keys[lo];
+    const uint32_t ni = key / kPairBase, nj = key % kPairBase;
+    const uint32_t t = sorted_vals[e];
+    uint32_t packed;
+    if (t < uint32_t(d.P))
+    {
+        uint32_t si = 0, sj = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+        {
+            const uint32_t v = d.pknn[size_t(t) * K + k];
+            if (v == ni)
+                si = k;
+            if (v == nj)
+                sj = k;
+        }
+        packed = t | (si << 25) | (sj << 28);
+    }
+    else
+    {
+        const uint32_t edge = t - uint32_t(d.P);
+        const uint32_t row_is_x = (uint32_t(d.edges[3 * edge]) == ni) ? 1u : 0u;
+        packed = 0x80000000u | (row_is_x << 30) | edge;
+    }
+    d.pair_entries[e] = packed;
+}
+
+// Symmetric keys (computeSymmetricNodePairKernel, NodePair2TermsIndex.cu:171-193);
+// values carry (half pair << 1 | mirrored).
+__global__ void k_sym_kv(SolveDesc d, uint32_t *__restrict__ keys, uint32_t *__restrict__ vals)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= uint32_t(d.U))
+        return;
+    const uint32_t k = d.half_keys[u];
+    const uint32_t i = k / kPairBase, j = k % kPairBase;
+    keys[2 * u] = k;
+    keys[2 * u + 1] = j * kPairBase + i;
+    vals[2 * u] = 2 * u;
+    vals[2 * u + 1] = 2 * u + 1;
+}
+
+__global__ void k_sym_scatter(SolveDesc d, const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ svals)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= 2u * uint32_t(d.U))
+        return;
+    const uint32_t v = svals[p];
+    if (v & 1u)
+        d.lower_pos[v >> 1] = p;
+    else
+        d.upper_pos[v >> 1] = p;
+    const uint32_t row = skeys[p] / kPairBase;
+    d.blk_col[p + row + 1] = skeys[p] % kPairBase;
+}
+
+// Block-row pointers over the sorted symmetric keys (computeRowOffsetKernel,
+// NodePairRowColumnIndex.cu:12-36) and the diagonal block columns.
+__global__ void k_row_ptr(SolveDesc d, const uint32_t *__restrict__ skeys)
+{
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n > uint32_t(d.N))
+        return;
+    const uint32_t target = n * kPairBase;
+    uint32_t lo = 0, hi = 2u * uint32_t(d.U);
+    while (lo < hi)
+    {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (skeys[mid] < target)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    d.row_ptr[n] = lo;
+    if (n < uint32_t(d.N))
+        d.blk_col[lo + n] = n;
+}
+
+// ============================================================ GN terms
+template <int K>
+__device__ __forceinline__ void load_knn(const SolveDesc &d, uint32_t p, uint16_t (&kn)[K], float (&w)[K])
+{
+    if constexpr (K == 8)
+    {
+        const uint4 kk = *reinterpret_cast<const uint4 *>(d.pknn + size_t(p) * 8);
+        const uint32_t v[4] = {kk.x, kk.y, kk.z, kk.w};
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+        {
+            kn[2 * a] = uint16_t(v[a] & 0xffffu);
+            kn[2 * a + 1] = uint16_t(v[a] >> 16);
+        }
+        const float4 w0 = *reinterpret_cast<const float4 *>(d.pw + size_t(p) * 8);
+        const float4 w1 = *reinterpret_cast<const float4 *>(d.pw + size_t(p) * 8 + 4);
+        w[0] = w0.x;
+        w[1] = w0.y;
+        w[2] = w0.z;
+        w[3] = w0.w;
+        w[4] = w1.x;
+        w[5] = w1.y;
+        w[6] = w1.z;
+        w[7] = w1.w;
+    }
+    else
+    {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+        {
+            kn[k] = d.pknn[size_t(p) * K + k];
+            w[k] = d.pw[size_t(p) * K + k];
+        }
+    }
+}
+
+__device__ __forceinline__ dq8 load_dq(const dq8 *__restrict__ a, uint32_t i)
+{
+    const float4 *p = reinterpret_cast<const float4 *>(a + i);
+    dq8 r;
+    r.q0 = p[0];
+    r.q1 = p[1];
+    return r;
+}
+
+template <class JT>
+__device__ __forceinline__ void store_jr(void *base, uint32_t p, const float (&v)[kJR])
+{
+    if constexpr (sizeof(JT) == 4)
+    {
+        float4 *o = reinterpret_cast<float4 *>(reinterpret_cast<float *>(base) + size_t(p) * kJR);
+#pragma unroll
+        for (int a = 0; a < kJR / 4; ++a)
+            o[a] = make_float4(v[4 * a], v[4 * a + 1], v[4 * a + 2], v[4 * a + 3]);
+    }
+    else
+    {
+        uint4 *o = reinterpret_cast<uint4 *>(reinterpret_cast<__half *>(base) + size_t(p) * kJR);
+#pragma unroll
+        for (int a = 0; a < kJR / 8; ++a)
+        {
+            uint32_t w[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+            {
+                const __half lo = __float2half_rn(v[8 * a + 2 * b]);
+                const __half hi = __float2half_rn(v[8 * a + 2 * b + 1]);
+                w[b] = uint32_t(__half_as_ushort(lo)) | (uint32_t(__half_as_ushort(hi)) << 16);
+            }
+            o[a] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    }
+}
+
+template <class JT>
+__device__ __forceinline__ void load_jr(const void *base, uint32_t p, float (&v)[kJR])
+{
+    if constexpr (sizeof(JT) == 4)
+    {
+        const float4 *o = reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(base) + size_t(p) * kJR);
+#pragma unroll
+        for (int a = 0; a < kJR / 4; ++a)
+        {
+            const float4 f = o[a];
+            v[4 * a] = f.x;
+            v[4 * a + 1] = f.y;
+            v[4 * a + 2] = f.z;
+            v[4 * a + 3] = f.w;
+        }
+    }
+    else
+    {
+        const uint4 *o = reinterpret_cast<const uint4 *>(reinterpret_cast<const __half *>(base) + size_t(p) * kJR);
+#pragma unroll
+        for (int a = 0; a < kJR / 8; ++a)
+        {
+            const uint4 u = o[a];
+            const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+            {
+                v[8 * a + 2 * b] = __half2float(__ushort_as_half(uint16_t(w[b] & 0xffffu)));
+                v[8 * a + 2 * b + 1] = __half2float(__ushort_as_half(uint16_t(w[b] >> 16)));
+            }
+        }
+    }
+}
+
+// Per-pixel residual / Jacobian (computeDenseImageJacobianKernel,
+// DenseImageJacobianResidual.cu:164-310; forms :12-157).
+template <int K, class JT>
+__global__ __launch_bounds__(256) void k_dense_terms(const SolveDesc *__restrict__ sds,
+                                                     const uint32_t *__restrict__ wg_begin, int n_solve, int parity)
+{
+    const int si = find_solve(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const uint32_t p = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
+    if (p >= uint32_t(d.P))
+        return;
+    const dq8 *__restrict__ dq = d.dq[parity];
+    uint16_t kn[K];
+    float w[K];
+    load_knn<K>(d, p, kn, w);
+    dq8 avg;
+    avg.q0 = make_float4(0.f, 0.f, 0.f, 0.f);
+    avg.q1 = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+    {
+        const dq8 n = load_dq(dq, kn[k]);
+        avg.q0.x += w[k] * n.q0.x;
+        avg.q0.y += w[k] * n.q0.y;
+        avg.q0.z += w[k] * n.q0.z;
+        avg.q0.w += w[k] * n.q0.w;
+        avg.q1.x += w[k] * n.q1.x;
+        avg.q1.y += w[k] * n.q1.y;
+        avg.q1.z += w[k] * n.q1.z;
+        avg.q1.w += w[k] * n.q1.w;
+    }
+    const rt34 T = dq_se3(avg);
+    const ushort4 px = d.pix[p];
+    const uint32_t src = uint32_t(px.y) * uint32_t(d.W) + px.x;
+    const float4 cv = d.ref_v[src];
+    const float4 cn = d.ref_n[src];
+    const f3 vw = rot_apply(T.r, mk3(cv.x, cv.y, cv.z)) + T.t;
+    const f3 nw = rot_apply(T.r, mk3(cn.x, cn.y, cn.z));
+    const f3 vc = rot_apply(d.w2c_r, vw) + mk3(d.w2c_t[0], d.w2c_t[1], d.w2c_t[2]);
+    const f3 nc = rot_apply(d.w2c_r, nw);
+    float out[kJR];
+#pragma unroll
+    for (int a = 0; a < kJR; ++a)
+        out[a] = 0.f;
+    if (px.z < d.W && px.w < d.H)
+    {
+        const uint32_t tgt = uint32_t(px.w) * uint32_t(d.W) + px.z;
+        const float4 tv4 = d.meas_v[tgt];
+        const float4 tn4 = d.meas_n[tgt];
+        const f3 tv = mk3(tv4.x, tv4.y, tv4.z), tn = mk3(tn4.x, tn4.y, tn4.z);
+        bool ok = true;
+        if (fabsf(tv4.x) < 1e-3f && fabsf(tv4.y) < 1e-3f && fabsf(tv4.z) < 1e-3f) // is_zero_vertex
+            ok = false;
+        if (dot3(tn, nc) < 0.7f)
+            ok = false;
+        if (fabsf(tv.z - vc.z) > 0.03f)
+            ok = false;
+        if (ok)
+        {
+            const f3 wt = mk3(d.w2c_t[0], d.w2c_t[1], d.w2c_t[2]);
+            const f3 tvw = rot_apply_t(d.w2c_r, tv - wt);
+            const f3 tnw = rot_apply_t(d.w2c_r, tn);
+            const f3 diff = vw - tvw;
+            out[18] = dot3(tnw, diff);
+            const f3 j0 = cross3(vw, tnw);
+            out[0] = j0.x;
+            out[1] = j0.y;
+            out[2] = j0.z;
+            out[3] = tnw.x;
+            out[4] = tnw.y;
+            out[5] = tnw.z;
+            const f3 pr0 = mk3(d.w2c_r[0], d.w2c_r[1], d.w2c_r[2]);
+            const f3 pr1 = mk3(d.w2c_r[3], d.w2c_r[4], d.w2c_r[5]);
+            const float ex = dot3(pr0, diff), ey = dot3(pr1, diff);
+            if (!(sqrtf(ex * ex + ey * ey) < 1e-6f))
+            {
+                out[19] = ex;
+                out[20] = ey;
+                const f3 mv = mk3(-vw.x, -vw.y, -vw.z);
+                const f3 rows[2] = {pr0, pr1};
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+                {
+                    const f3 a = rows[c];
+                    out[6 * (c + 1) + 0] = a.y * mv.z + a.z * (-mv.y);
+                    out[6 * (c + 1) + 1] = a.x * (-mv.z) + a.z * mv.x;
+                    out[6 * (c + 1) + 2] = a.x * mv.y + a.y * (-mv.x);
+                    out[6 * (c + 1) + 3] = a.x;
+                    out[6 * (c + 1) + 4] = a.y;
+                    out[6 * (c + 1) + 5] = a.z;
+                }
+            }
+        }
+    }
+    store_jr<JT>(d.jr, p, out);
+}
+
+// ARAP forward warp per edge + node translations (forwardWarpSmootherNodeKernel,
+// NodeGraphHandler.cu:7-46; ComputeNodeMotionKernel, NodeMotionHandler.cu:9-21).
+__global__ __launch_bounds__(256) void k_reg_terms(const SolveDesc *__restrict__ sds,
+                                                   const uint32_t *__restrict__ wg_begin, int n_solve, int parity)
+{
+    const int si = find_solve(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const uint32_t item = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
+    const dq8 *__restrict__ dq = d.dq[parity];
+    if (item < uint32_t(d.E))
+    {
+        const uint32_t i = d.edges[3 * item], j = d.edges[3 * item + 1], k = d.edges[3 * item + 2];
+        const float4 xi = d.node_coords[i], xj = d.node_coords[j];
+        const rt34 Ti = dq_se3(load_dq(dq, i));
+        const rt34 Tj = dq_se3(load_dq(dq, j));
+        const f3 vj = mk3(xj.x, xj.y, xj.z);
+        const f3 a = rot_apply(Ti.r, vj) + Ti.t;
+        const f3 b = rot_apply(Tj.r, vj) + Tj.t;
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float valid = (dx * dx + dy * dy + dz * dz > 64.f * d.node_radius_square) ? 0.f : 1.f;
+        const float c = d.node_connect[size_t(i) * d.K + k];
+        float4 *o = reinterpret_cast<float4 *>(d.reg + size_t(item) * kReg);
+        o[0] = make_float4(a.x, a.y, a.z, b.x);
+        o[1] = make_float4(b.y, b.z, c, valid);
+    }
+    else if (item < uint32_t(d.E + d.N))
+    {
+        const uint32_t n = item - uint32_t(d.E);
+        const rt34 T = dq_se3(load_dq(dq, n));
+        d.ttrans[n] = make_float4(T.t.x, T.t.y, T.t.z, 0.f);
+    }
+}
+
+// ARAP Jacobian rows of one side, scaled by c (node_graph_reg_jacobian.cuh:250-275).
+__device__ __forceinline__ void reg_rows(f3 a, f3 s, float c, bool is_i, float (&g)[18])
+{
+    if (is_i)
+    {
+        const float r[18] = {0.f, a.z, -a.y, 1.f, 0.f, 0.f, -a.z, 0.f, a.x, 0.f, 1.f, 0.f, a.y, -a.x, 0.f, 0.f, 0.f, 1.f};
+#pragma unroll
+        for (int q = 0; q < 18; ++q)
+            g[q] = r[q] * c;
+    }
+    else
+    {
+        const float r[18] = {0.f, -s.z, s.y, -1.f, 0.f, 0.f, s.z, 0.f, -s.x, 0.f, -1.f, 0.f, -s.y, s.x, 0.f, 0.f, 0.f, -1.f};
+#pragma unroll
+        for (int q = 0; q < 18; ++q)
+            g[q] = r[q] * c;
+    }
+}
+
+__device__ __forceinline__ void load_reg(const SolveDesc &d, uint32_t e, f3 &a, f3 &b, float &cw, float &valid)
+{
+    const float4 *o = reinterpret_cast<const float4 *>(d.reg + size_t(e) * kReg);
+    const float4 u0 = o[0], u1 = o[1];
+    a = mk3(u0.x, u0.y, u0.z);
+    b = mk3(u0.w, u1.x, u1.y);
+    cw = u1.z;
+    valid = u1.w;
+}
+
+// Gauss-Jordan 6x6 inverse with partial pivoting in FP64
+// (DenseGaussian<6>::Inverse, libs/star/math/DenseGaussian.hpp:34-113).
+__device__ void invert6_fp64(const float *A, double *out)
+{
+    double m[6][12];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+        {
+            m[i][j] = double(A[6 * i + j]);
+            m[i][6 + j] = (i == j) ? 1.0 : 0.0;
+        }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+    {
+        int piv = i;
+        double pv = fabs(m[i][i]);
+#pragma unroll
+        for (int j = i + 1; j < 6; ++j)
+            if (pv < fabs(m[j][i]))
+            {
+                pv = fabs(m[j][i]);
+                piv = j;
+            }
+        if (piv != i)
+        {
+#pragma unroll
+            for (int j = i + 1; j < 6; ++j)
+                if (j == piv)
+#pragma unroll
+                    for (int k = 0; k < 12; ++k)
+                    {
+                        const double t = m[i][k];
+                        m[i][k] = m[j][k];
+                        m[j][k] = t;
+                    }
+        }
+        const double inv = 1.0 / m[i][i];
+#pragma unroll
+        for (int k = 0; k < 12; ++k)
+            m[i][k] *= inv;
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+            if (r != i)
+            {
+                const double f = m[r][i];
+#pragma unroll
+                for (int k = 0; k < 12; ++k)
+                    m[r][k] -= f * m[i][k];
+            }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+            out[6 * i + j] = m[i][6 + j];
+}
+
+constexpr int kSymIdx[6][6] = {{0, 1, 2, 3, 4, 5},     {1, 6, 7, 8, 9, 10},    {2, 7, 11, 12, 13, 14},
+                               {3, 8, 12, 15, 16, 17}, {4, 9, 13, 16, 18, 19}, {5, 10, 14, 17, 19, 20}};
+
+// Diagonal block + damping and -J^T W r per node, one wave per node, lanes
+// stride the node's term list (computeBlockDiagonalPreconditionerKernel,
+// PreconditionerRhsBuilder.cu:110-262; computeJtResidualWithIndexKernel,
+// JtDotResidualIndexed.cu:40-150), then the FP64 block inverse.
+template <int K, class JT>
+__global__ __launch_bounds__(256) void k_diag_rhs(const SolveDesc *__restrict__ sds,
+                                                  const uint32_t *__restrict__ wg_begin, int n_solve)
+{
+    __shared__ float stage[4][48];
+    const int si = find_solve(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t n = (blockIdx.x - wg_begin[si]) * 4 + wave;
+    if (n >= uint32_t(d.N))
+        return;
+    float acc[21], g[6];
+#pragma unroll
+    for (int a = 0; a < 21; ++a)
+        acc[a] = 0.f;
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+        g[a] = 0.f;
+    const float lam2[3] = {d.lam2_depth, d.lam2_flow, d.lam2_flow};
+    const bool use_flow = (d.lam2_flow != 0.f);
+    const uint32_t P = d.P, E = d.E;
+    const uint32_t beg = d.n2t_off[n], end = d.n2t_off[n + 1];
+    for (uint32_t it = beg + lane; it < end; it += kWave)
+    {
+        const uint32_t t = d.n2t_terms[it];
+        if (t < P)
+        {
+            uint16_t kn[K];
+            float w8[K];
+            load_knn<K>(d, t, kn, w8);
+            float w = 0.f;
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (kn[k] == n)
+                    w = w8[k];
+            float jr[kJR];
+            load_jr<JT>(d.jr, t, jr);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+            {
+                if (c > 0 && !use_flow)
+                    break;
+                float jw[6];
+#pragma unroll
+                for (int q = 0; q < 6; ++q)
+                    jw[q] = jr[6 * c + q] * w;
+#pragma unroll
+                for (int a = 0; a < 6; ++a)
+#pragma unroll
+                    for (int b = a; b < 6; ++b)
+                        acc[kSymIdx[a][b]] += lam2[c] * jw[b] * jw[a];
+                const float ow = lam2[c] * (w * w) * jr[18 + c];
+#pragma unroll
+                for (int q = 0; q < 6; ++q)
+                    g[q] += ow * jr[6 * c + q];
+            }
+        }
+        else if (t < P + E)
+        {
+            const uint32_t e = t - P;
+            f3 a, b;
+            float cw, valid;
+            load_reg(d, e, a, b, cw, valid);
+            if (valid != 0.f)
+            {
+                const bool is_i = (uint32_t(d.edges[3 * e]) == n);
+                float gr[18];
+                reg_rows(a, b, cw, is_i, gr);
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int aa = 0; aa < 6; ++aa)
+#pragma unroll
+                        for (int bb = aa; bb < 6; ++bb)
+                            acc[kSymIdx[aa][bb]] += d.lam2_reg * gr[6 * c + bb] * gr[6 * c + aa];
+                float g1[18];
+                reg_rows(a, b, 1.f, is_i, g1);
+                const float c2 = cw * cw;
+                const float res[3] = {(a.x - b.x) * c2, (a.y - b.y) * c2, (a.z - b.z) * c2};
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int q = 0; q < 6; ++q)
+                        g[q] += d.lam2_reg * res[c] * g1[6 * c + q];
+            }
+        }
+        else
+        {
+            // node-translation term (node_motion_jacobian.cuh:7-35)
+            const uint32_t m = t - P - E;
+            const float4 pred = d.node_pred ? d.node_pred[m] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float4 tt = d.ttrans[m];
+            const float c2 = pred.w * pred.w;
+            acc[kSymIdx[3][3]] += d.lam2_nt * c2;
+            acc[kSymIdx[4][4]] += d.lam2_nt * c2;
+            acc[kSymIdx[5][5]] += d.lam2_nt * c2;
+            g[3] += d.lam2_nt * (tt.x - pred.x) * c2;
+            g[4] += d.lam2_nt * (tt.y - pred.y) * c2;
+            g[5] += d.lam2_nt * (tt.z - pred.z) * c2;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 21; ++a)
+        acc[a] = wave_sum(acc[a]);
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+        g[a] = wave_sum(g[a]);
+    if (lane == 0)
+    {
+        float D[36];
+#pragma unroll
+        for (int a = 0; a < 6; ++a)
+#pragma unroll
+            for (int b = 0; b < 6; ++b)
+                D[6 * a + b] = acc[kSymIdx[a][b]];
+#pragma unroll
+        for (int a = 0; a < 6; ++a)
+            D[7 * a] += d.damping;
+#pragma unroll
+        for (int a = 0; a < 36; ++a)
+            stage[wave][a] = D[a];
+#pragma unroll
+        for (int a = 0; a < 6; ++a)
+            stage[wave][36 + a] = -g[a];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    const uint32_t diag_blk = d.row_ptr[n] + n;
+    if (lane < 36)
+        d.blk[size_t(diag_blk) * 36 + lane] = stage[wave][lane];
+    else if (lane < 42)
+        d.rhs[size_t(n) * 6 + (lane - 36)] = stage[wave][lane];
+}
+
+// FP64 inverse of each damped diagonal block, one thread per node
+// (blockDiagonalInverseKernel, BlockDiagonalPreconditionerInverse.cuh:15-73).
+__global__ __launch_bounds__(256) void k_dinv(const SolveDesc *__restrict__ sds, const uint32_t *__restrict__ wg_begin,
+                                              int n_solve)
+{
+    const int si = find_solve(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const uint32_t n = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
+    if (n >= uint32_t(d.N))
+        return;
+    const float *D = d.blk + size_t(d.row_ptr[n] + n) * 36;
+    float A[36];
+#pragma unroll
+    for (int a = 0; a < 36; ++a)
+        A[a] = D[a];
+    double inv[36];
+    invert6_fp64(A, inv);
+    double *o = d.dinv + size_t(n) * 36;
+#pragma unroll
+    for (int a = 0; a < 36; ++a)
+        o[a] = inv[a];
+}
+
+// Off-diagonal blocks, one wave per unique pair (i<j): lanes stride the pair's
+// dense entries, ARAP entries (at most two, at the end of the sorted list) are
+// added by lane 0; block (i,j) and its transpose (j,i) are written
+// (computeJtJNonDiagonalBlockKernel, JtJMaterializer.cu:66-175 with the
+// transposed assembly of JtJMaterializerBinBlockCSR.cu:37-39).
+template <int K, class JT>
+__global__ __launch_bounds__(256) void k_offdiag(const SolveDesc *__restrict__ sds, const uint32_t *__restrict__ wg_begin,
+                                                 int n_solve)
+{
+    __shared__ float stage[4][36];
+    const int si = find_solve(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t u = (blockIdx.x - wg_begin[si]) * 4 + wave;
+    if (u >= uint32_t(d.U))
+        return;
+    const float lam2[3] = {d.lam2_depth, d.lam2_flow, d.lam2_flow};
+    const bool use_flow = (d.lam2_flow != 0.f);
+    const uint32_t beg = d.half_off[u], end = d.half_off[u + 1];
+    float acc[21];
+#pragma unroll
+    for (int a = 0; a < 21; ++a)
+        acc[a] = 0.f;
+    for (uint32_t e = beg + lane; e < end; e += kWave)
+    {
+        const uint32_t pk = d.pair_entries[e];
+        if (pk & 0x80000000u)
+            continue;
+        const uint32_t p = pk & 0x1ffffffu;
+        const uint32_t sa = (pk >> 25) & 7u, sb = (pk >> 28) & 7u;
+        const float ww = d.pw[size_t(p) * K + sa] * d.pw[size_t(p) * K + sb];
+        float jr[kJR];
+        load_jr<JT>(d.jr, p, jr);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+        {
+            if (c > 0 && !use_flow)
+                break;
+            const float sc = lam2[c] * ww;
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+#pragma unroll
+                for (int b = a; b < 6; ++b)
+                    acc[kSymIdx[a][b]] += sc * jr[6 * c + b] * jr[6 * c + a];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 21; ++a)
+        acc[a] = wave_sum(acc[a]);
+    const uint32_t key = d.half_keys[u];
+    const uint32_t ni = key / kPairBase, nj = key % kPairBase;
+    if (lane == 0)
+    {
+        float B[36];
+#pragma unroll
+        for (int a = 0; a < 6; ++a)
+#pragma unroll
+            for (int b = 0; b < 6; ++b)
+                B[6 * a + b] = acc[kSymIdx[a][b]];
+        for (uint32_t e = end; e > beg; --e)
+        {
+            const uint32_t pk = d.pair_entries[e - 1];
+            if (!(pk & 0x80000000u))
+                break;
+            const uint32_t edge = pk & 0x3fffffffu;
+            const bool row_is_x = (pk >> 30) & 1u;
+            f3 a, b;
+            float cw, valid;
+            load_reg(d, edge, a, b, cw, valid);
+            if (valid == 0.f)
+                continue;
+            float gi[18], gj[18];
+            reg_rows(a, b, 1.f, true, gi);
+            reg_rows(a, b, 1.f, false, gj);
+            const float ws = d.lam2_reg * (cw * cw);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int aa = 0; aa < 6; ++aa)
+#pragma unroll
+                    for (int bb = 0; bb < 6; ++bb)
+                    {
+                        const float gr = row_is_x ? gi[6 * c + aa] : gj[6 * c + aa];
+                        const float gc = row_is_x ? gj[6 * c + bb] : gi[6 * c + bb];
+                        B[6 * aa + bb] += ws * gr * gc;
+                    }
+        }
+#pragma unroll
+        for (int a = 0; a < 36; ++a)
+            stage[wave][a] = B[a];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    const size_t up = size_t(d.upper_pos[u]) + ni + 1;
+    const size_t lo = size_t(d.lower_pos[u]) + nj + 1;
+    if (lane < 36)
+    {
+        d.blk[up * 36 + lane] = stage[wave][lane];
+        const int a = lane / 6, b = lane % 6;
+        d.blk[lo * 36 + lane] = stage[wave][6 * b + a];
+    }
+}
+
+// ============================================================ PCG
+// Work-groups of 192 threads = 32 node blocks x 6 rows; one solve per group.
+constexpr int kPcgNodes = 32;
+constexpr int kPcgThreads = kPcgNodes * 6;
+
+__device__ __forceinline__ double block_sum_192(double v, double *sh)
+{
+    v = wave_sum(v);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0)
+        sh[wave] = v;
+    __syncthreads();
+    const double r = sh[0] + sh[1] + sh[2];
+    __syncthreads();
+    return r;
+}
+
+// Sum of the per-group partials of one solve, fixed order, broadcast to the group.
+__device__ __forceinline__ double reduce_partials(const double *__restrict__ part, uint32_t b0, uint32_t b1, double *sh)
+{
+    double v = 0.0;
+    if (threadIdx.x < 64)
+    {
+        for (uint32_t i = b0 + threadIdx.x; i < b1; i += 64)
+            v += part[i];
+        v = wave_sum(v);
+        if (threadIdx.x == 0)
+            sh[3] = v;
+    }
+    __syncthreads();
+    const double r = sh[3];
+    __syncthreads();
+    return r;
+}
+
+// r = b, s = M^-1 b, x = 0, d_prev = 0, partial r.s   (blockPCGZeroInitKernel, BlockPCG.cuh:68-116)
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_init(const SolveDesc *__restrict__ sds,
+                                                          const uint32_t *__restrict__ wg_begin, int n_solve,
+                                                          double *__restrict__ part_rs)
+{
+    __shared__ double sh[4];
+    __shared__ float rb[kPcgThreads];
+    const int si = find_solve(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const uint32_t node = (blockIdx.x - wg_begin[si]) * kPcgNodes + threadIdx.x / 6;
+    const int a = threadIdx.x % 6;
+    const bool live = node < uint32_t(d.N);
+    const size_t row = size_t(node) * 6 + a;
+    const float r = live ? d.rhs[row] : 0.f;
+    rb[threadIdx.x] = r;
+    __syncthreads();
+    double prod = 0.0;
+    if (live)
+    {
+        const double *Mi = d.dinv + size_t(node) * 36 + 6 * a;
+        const int b0 = threadIdx.x - a;
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            s += Mi[c] * double(rb[b0 + c]);
+        const float sf = float(s);
+        d.r[row] = r;
+        d.s[row] = sf;
+        d.x[row] = 0.f;
+        d.dv[0][row] = 0.f;
+        prod = double(r) * double(sf);
+    }
+    const double tot = block_sum_192(prod, sh);
+    if (threadIdx.x == 0)
+        part_rs[blockIdx.x] = tot;
+}
+
+__device__ __forceinline__ float d_next(double beta, float dold, float s)
+{
+    return float(fma(beta, double(dold), double(s)));
+}
+
+// d_k = s + beta d_{k-1}, q = A d_k (materialised blocks), partial d.q
+// (performSparseMVKernel + blockPCGThirdUpdateKernel, BlockPCG.cuh:175-189,
+// BinBlockCSR.cuh:7-35).
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_spmv(const SolveDesc *__restrict__ sds,
+                                                          const uint32_t *__restrict__ wg_begin, int n_solve, int k,
+                                                          const double *__restrict__ part_rs,
+                                                          double *__restrict__ part_dq)
+{
+    __shared__ double sh[4];
+    const int si = find_solve(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const double delta = reduce_partials(part_rs, wg_begin[si], wg_begin[si + 1], sh);
+    double beta = 0.0;
+    if (k > 0)
+        beta = delta / d.delta[k - 1];
+    if (threadIdx.x == 0 && blockIdx.x == wg_begin[si])
+        d.delta[k] = delta;
+    const float *__restrict__ dold = d.dv[k & 1];
+    float *__restrict__ dnew = d.dv[(k + 1) & 1];
+    const uint32_t node = (blockIdx.x - wg_begin[si]) * kPcgNodes + threadIdx.x / 6;
+    const int a = threadIdx.x % 6;
+    double prod = 0.0;
+    if (node < uint32_t(d.N))
+    {
+        const size_t row = size_t(node) * 6 + a;
+        const float dn = d_next(beta, dold[row], d.s[row]);
+        dnew[row] = dn;
+        const uint32_t b0 = d.row_ptr[node] + node;
+        const uint32_t b1 = d.row_ptr[node + 1] + node + 1;
+        float acc = 0.f;
+        for (uint32_t bb = b0; bb < b1; ++bb)
+        {
+            const uint32_t col = d.blk_col[bb];
+            const float *A = d.blk + size_t(bb) * 36 + 6 * a;
+            const size_t cr = size_t(col) * 6;
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                acc += A[c] * d_next(beta, dold[cr + c], d.s[cr + c]);
+        }
+        d.q[row] = acc;
+        prod = double(dn) * double(acc);
+    }
+    const double tot = block_sum_192(prod, sh);
+    if (threadIdx.x == 0)
+        part_dq[blockIdx.x] = tot;
+}
+
+// alpha = delta / d.q; x += alpha d; r -= alpha q; s = M^-1 r; partial r.s
+// (blockPCGSecondUpdateKernel, BlockPCG.cuh:125-169)
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_update(const SolveDesc *__restrict__ sds,
+                                                            const uint32_t *__restrict__ wg_begin, int n_solve, int k,
+                                                            const double *__restrict__ part_dq,
+                                                            double *__restrict__ part_rs)
+{
+    __shared__ double sh[4];
+    __shared__ float rb[kPcgThreads];
+    const int si = find_solve(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const double dq = reduce_partials(part_dq, wg_begin[si], wg_begin[si + 1], sh);
+    const double alpha = d.delta[k] / dq;
+    const float *__restrict__ dnew = d.dv[(k + 1) & 1];
+    const uint32_t node = (blockIdx.x - wg_begin[si]) * kPcgNodes + threadIdx.x / 6;
+    const int a = threadIdx.x % 6;
+    const bool live = node < uint32_t(d.N);
+    const size_t row = size_t(node) * 6 + a;
+    float rn = 0.f;
+    if (live)
+    {
+        const double dn = double(dnew[row]);
+        d.x[row] = float(fma(alpha, dn, double(d.x[row])));
+        rn = float(fma(-alpha, double(d.q[row]), double(d.r[row])));
+        d.r[row] = rn;
+    }
+    rb[threadIdx.x] = rn;
+    __syncthreads();
+    double prod = 0.0;
+    if (live)
+    {
+        const double *Mi = d.dinv + size_t(node) * 36 + 6 * a;
+        const int b0 = threadIdx.x - a;
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            s += Mi[c] * double(rb[b0 + c]);
+        const float sf = float(s);
+        d.s[row] = sf;
+        prod = double(rn) * double(sf);
+    }
+    const double tot = block_sum_192(prod, sh);
+    if (threadIdx.x == 0)
+        part_rs[blockIdx.x] = tot;
+}
+
+// T <- exp(xi) T, stored as a dual quaternion (ApplyWarpFieldUpdateKernel,
+// SolverIterationData.cu:19-74)
+__global__ __launch_bounds__(256) void k_update(const SolveDesc *__restrict__ sds, const uint32_t *__restrict__ wg_begin,
+                                                int n_solve, int parity)
+{
+    const int si = find_solve(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const uint32_t n = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
+    if (n >= uint32_t(d.N))
+        return;
+    const float *xv = d.x + size_t(n) * 6;
+    const f3 w = mk3(xv[0], xv[1], xv[2]);
+    const f3 t = mk3(xv[3], xv[4], xv[5]);
+    rt34 S;
+    if (fabsf(w.x) + fabsf(w.y) + fabsf(w.z) < 1e-4f)
+    {
+        S.r[0] = 1.f;
+        S.r[1] = 0.f;
+        S.r[2] = 0.f;
+        S.r[3] = 0.f;
+        S.r[4] = 1.f;
+        S.r[5] = 0.f;
+        S.r[6] = 0.f;
+        S.r[7] = 0.f;
+        S.r[8] = 1.f;
+    }
+    else
+    {
+        const float angle = sqrtf(dot3(w, w));
+        const f3 ax = (1.0f / angle) * w;
+        const float c = cosf(angle), s = sinf(angle), tt = 1.0f - c;
+        S.r[0] = tt * ax.x * ax.x + c;
+        S.r[1] = tt * ax.x * ax.y - ax.z * s;
+        S.r[2] = tt * ax.x * ax.z + ax.y * s;
+        S.r[3] = tt * ax.x * ax.y + ax.z * s;
+        S.r[4] = tt * ax.y * ax.y + c;
+        S.r[5] = tt * ax.y * ax.z - ax.x * s;
+        S.r[6] = tt * ax.x * ax.z - ax.y * s;
+        S.r[7] = tt * ax.y * ax.z + ax.x * s;
+        S.r[8] = tt * ax.z * ax.z + c;
+    }
+    S.t = t;
+    const rt34 prev = dq_to_mat34(load_dq(d.dq[parity], n));
+    rt34 nw;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            nw.r[3 * i + j] = S.r[3 * i] * prev.r[j] + S.r[3 * i + 1] * prev.r[3 + j] + S.r[3 * i + 2] * prev.r[6 + j];
+    nw.t = rot_apply(S.r, prev.t) + S.t;
+    const dq8 out = dq_from_mat34(nw);
+    float4 *o = reinterpret_cast<float4 *>(d.dq[parity ^ 1] + n);
+    o[0] = out.q0;
+    o[1] = out.q1;
+}
+
+__global__ void k_identity(const SolveDesc *__restrict__ sds, const uint32_t *__restrict__ wg_begin, int n_solve)
+{
+    const int si = find_solve(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const uint32_t n = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
+    if (n >= uint32_t(d.N))
+        return;
+    float4 *o = reinterpret_cast<float4 *>(d.dq[0] + n);
+    o[0] = make_float4(0.f, 0.f, 0.f, 1.f);
+    o[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ============================================================ input producers
+// KNN heap: replace the current maximum, then a Lomuto quicksort ascending
+// (libs/star/geometry/node_graph/brute_force_knn.cuh:52-135).
+template <int K>
+struct KnnHeap
+{
+    float dist[K];
+    uint16_t idx[K];
+    int max_id;
+    float max_d;
+    __device__ void init(float d0)
+    {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+        {
+            dist[k] = d0;
+            idx[k] = 0;
+        }
+        refresh();
+    }
+    __device__ __forceinline__ void refresh()
+    {
+        max_id = 0;
+        max_d = dist[0];
+#pragma unroll
+        for (int k = 1; k < K; ++k)
+            if (dist[k] > max_d)
+            {
+                max_d = dist[k];
+                max_id = k;
+            }
+    }
+    __device__ __forceinline__ void update(uint16_t id, float d)
+    {
+        if (d < max_d)
+        {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (k == max_id)
+                {
+                    dist[k] = d;
+                    idx[k] = id;
+                }
+            refresh();
+        }
+    }
+    // iterative Lomuto quicksort with the recursion order of the reference
+    __device__ void sort()
+    {
+        float dd[K];
+        uint16_t ii[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+        {
+            dd[k] = dist[k];
+            ii[k] = idx[k];
+        }
+        int stack_lo[2 * K], stack_hi[2 * K];
+        int top = 0;
+        stack_lo[0] = 0;
+        stack_hi[0] = K - 1;
+        top = 1;
+        while (top > 0)
+        {
+            --top;
+            const int lo = stack_lo[top], hi = stack_hi[top];
+            if (lo >= hi)
+                continue;
+            const float pivot = dd[hi];
+            int i = lo - 1;
+            for (int j = lo; j <= hi - 1; ++j)
+                if (dd[j] < pivot)
+                {
+                    ++i;
+                    const float tf = dd[i];
+                    dd[i] = dd[j];
+                    dd[j] = tf;
+                    const uint16_t ti = ii[i];
+                    ii[i] = ii[j];
+                    ii[j] = ti;
+                }
+            {
+                const float tf = dd[i + 1];
+                dd[i + 1] = dd[hi];
+                dd[hi] = tf;
+                const uint16_t ti = ii[i + 1];
+                ii[i + 1] = ii[hi];
+                ii[hi] = ti;
+            }
+            const int pi = i + 1;
+            // recursion order: left part first, then right part
+            stack_lo[top] = pi + 1;
+            stack_hi[top] = hi;
+            ++top;
+            stack_lo[top] = lo;
+            stack_hi[top] = pi - 1;
+            ++top;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+        {
+            dist[k] = dd[k];
+            idx[k] = ii[k];
+        }
+    }
+};
+
+// fma(dz,dz,fma(dy,dy,dx*dx)) with no further contraction (NodeGraph.cu:32-34)
+__device__ __forceinline__ float sqdist_fma(float dx, float dy, float dz)
+{
+    return __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmul_rn(dx, dx)));
+}
+
+constexpr int kSkinTile = 256;
+
+// SkinnerKernel (Skinner.cu:10-94): nodes staged through LDS in tiles.
+template <int K>
+__global__ __launch_bounds__(256) void k_skin(const float4 *__restrict__ surfels, uint32_t S,
+                                              const float4 *__restrict__ nodes, uint32_t N, float r2,
+                                              uint16_t *__restrict__ knn, float *__restrict__ spatial,
+                                              float *__restrict__ connect)
+{
+    __shared__ float4 tile[kSkinTile];
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = s < S;
+    const float4 v = live ? surfels[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    KnnHeap<K> h;
+    h.init(1e6f);
+    for (uint32_t base = 0; base < N; base += kSkinTile)
+    {
+        __syncthreads();
+        if (base + threadIdx.x < N)
+            tile[threadIdx.x] = nodes[base + threadIdx.x];
+        __syncthreads();
+        const uint32_t cnt = min(uint32_t(kSkinTile), N - base);
+        for (uint32_t i = 0; i < cnt; ++i)
+        {
+            const float4 nd = tile[i];
+            const float dd = sqdist_fma(__fsub_rn(v.x, nd.x), __fsub_rn(v.y, nd.y), __fsub_rn(v.z, nd.z));
+            h.update(uint16_t(base + i), dd);
+        }
+    }
+    if (!live)
+        return;
+    h.sort();
+    float w[K];
+    float wsum = 0.f;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+    {
+        const float4 nd = nodes[h.idx[k]];
+        const float dd = sqdist_fma(__fsub_rn(v.x, nd.x), __fsub_rn(v.y, nd.y), __fsub_rn(v.z, nd.z));
+        w[k] = __expf(-dd / (2.f * r2));
+        wsum += w[k];
+    }
+    const float inv = 1.f / wsum;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+    {
+        knn[size_t(s) * K + k] = h.idx[k];
+        spatial[size_t(s) * K + k] = inv * w[k];
+        connect[size_t(s) * K + k] = 1.f;
+    }
+}
+
+// UpdateNodeDistanceKernel + BuildNodeGraphKernel + BuildNodeGraphPairKernel
+// + ResetNodeConnectionKernel (NodeGraph.cu:10-145): the FP16 distance of each
+// pair is recomputed on the fly instead of being stored as an N x N matrix.
+template <int K>
+__global__ __launch_bounds__(256) void k_node_graph(const float4 *__restrict__ nodes, uint32_t N, float r2,
+                                                    uint16_t *__restrict__ node_knn, float *__restrict__ node_spatial,
+                                                    uint16_t *__restrict__ edges, float *__restrict__ node_connect)
+{
+    __shared__ float4 tile[kSkinTile];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < N;
+    const float4 vi = live ? nodes[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    KnnHeap<K> h;
+    h.init(1e3f);
+    for (uint32_t base = 0; base < N; base += kSkinTile)
+    {
+        __syncthreads();
+        if (base + threadIdx.x < N)
+            tile[threadIdx.x] = nodes[base + threadIdx.x];
+        __syncthreads();
+        const uint32_t cnt = min(uint32_t(kSkinTile), N - base);
+        for (uint32_t jj = 0; jj < cnt; ++jj)
+        {
+            const uint32_t j = base + jj;
+            const float4 vj = tile[jj];
+            // distance is written for idx >= idy with v1 = node[idx], v2 = node[idy]
+            const float4 hi = (i >= j) ? vi : vj;
+            const float4 lo = (i >= j) ? vj : vi;
+            const float dd = sqdist_fma(__fsub_rn(hi.x, lo.x), __fsub_rn(hi.y, lo.y), __fsub_rn(hi.z, lo.z));
+            h.update(uint16_t(j), __half2float(__float2half_rn(dd)));
+        }
+    }
+    if (!live)
+        return;
+    h.sort();
+    float w[K];
+    float wsum = 0.f;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+    {
+        w[k] = __expf(-h.dist[k] / (2.f * r2));
+        wsum += w[k];
+    }
+    const float inv = 1.f / wsum;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+    {
+        node_knn[size_t(i) * K + k] = h.idx[k];
+        node_spatial[size_t(i) * K + k] = w[k] * inv;
+        node_connect[size_t(i) * K + k] = 1.f;
+        const size_t e = size_t(i) * K + k;
+        edges[3 * e] = uint16_t(i);
+        edges[3 * e + 1] = h.idx[k];
+        edges[3 * e + 2] = uint16_t(k);
+    }
+}
+
+// ============================================================ host launchers
+#define SH_GRID(n, b) dim3(unsigned(((n) + (b)-1) / (b)))
+
+hipError_t launch_pixel_flags(const uint32_t *index_map, uint32_t n, uint32_t *flags, hipStream_t st)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_pixel_flags, SH_GRID(n, 256), dim3(256), 0, st, index_map, n, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_pixels(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos, hipStream_t st)
+{
+    const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
+    if (d.K == 8)
+        hipLaunchKernelGGL(k_compact_pixels<8>, SH_GRID(n, 256), dim3(256), 0, st, d, flags, pos);
+    else
+        hipLaunchKernelGGL(k_compact_pixels<4>, SH_GRID(n, 256), dim3(256), 0, st, d, flags, pos);
+    return hipGetLastError();
+}
+
+hipError_t launch_node2term_kv(const SolveDesc &d, uint16_t *keys, uint32_t *vals, hipStream_t st)
+{
+    const uint32_t n = uint32_t(d.P + d.E + d.N);
+    if (d.K == 8)
+        hipLaunchKernelGGL(k_node2term_kv<8>, SH_GRID(n, 256), dim3(256), 0, st, d, keys, vals);
+    else
+        hipLaunchKernelGGL(k_node2term_kv<4>, SH_GRID(n, 256), dim3(256), 0, st, d, keys, vals);
+    return hipGetLastError();
+}
+
+hipError_t launch_lower_bound_u16(const uint16_t *keys, uint32_t n_keys, uint32_t n_out, uint32_t *out,
+                                  hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lower_bound_u16, SH_GRID(n_out, 256), dim3(256), 0, st, keys, n_keys, n_out, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_kv(const SolveDesc &d, uint32_t *keys, uint32_t *vals, hipStream_t st)
+{
+    const uint32_t n = uint32_t(d.P + d.E);
+    if (n == 0)
+        return hipSuccess;
+    if (d.K == 8)
+        hipLaunchKernelGGL(k_pair_kv<8>, SH_GRID(n, 256), dim3(256), 0, st, d, keys, vals);
+    else
+        hipLaunchKernelGGL(k_pair_kv<4>, SH_GRID(n, 256), dim3(256), 0, st, d, keys, vals);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_heads(const uint32_t *keys, uint32_t n, uint32_t *head, uint32_t *meta, hipStream_t st)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_pair_heads, SH_GRID(n, 256), dim3(256), 0, st, keys, n, head, meta);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_compact(const uint32_t *keys, const uint32_t *head, const uint32_t *hpos, uint32_t n,
+                               const uint32_t *meta, uint32_t *half_keys, uint32_t *half_off, uint32_t *meta_out,
+                               hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pair_compact, SH_GRID(n > 0 ? n : 1, 256), dim3(256), 0, st, keys, head, hpos, n, meta,
+                       half_keys, half_off, meta_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_entries(const SolveDesc &d, const uint32_t *sorted_vals, hipStream_t st)
+{
+    const uint32_t n = uint32_t(d.n_pair_entries);
+    if (n == 0)
+        return hipSuccess;
+    if (d.K == 8)
+        hipLaunchKernelGGL(k_pair_entries<8>, SH_GRID(n, 256), dim3(256), 0, st, d, sorted_vals);
+    else
+        hipLaunchKernelGGL(k_pair_entries<4>, SH_GRID(n, 256), dim3(256), 0, st, d, sorted_vals);
+    return hipGetLastError();
+}
+
+hipError_t launch_sym_kv(const SolveDesc &d, uint32_t *keys, uint32_t *vals, hipStream_t st)
+{
+    if (d.U == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_sym_kv, SH_GRID(d.U, 256), dim3(256), 0, st, d, keys, vals);
+    return hipGetLastError();
+}
+
+hipError_t launch_sym_scatter(const SolveDesc &d, const uint32_t *skeys, const uint32_t *svals, hipStream_t st)
+{
+    if (d.U == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_sym_scatter, SH_GRID(2 * d.U, 256), dim3(256), 0, st, d, skeys, svals);
+    return hipGetLastError();
+}
+
+hipError_t launch_row_ptr(const SolveDesc &d, const uint32_t *skeys, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_row_ptr, SH_GRID(d.N + 1, 256), dim3(256), 0, st, d, skeys);
+    return hipGetLastError();
+}
+
+hipError_t launch_identity(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_identity, dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve);
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_terms(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve,
+                              uint32_t n_wg, int parity, hipStream_t st)
+{
+    if (n_wg == 0)
+        return hipSuccess;
+    if (K == 8)
+    {
+        if (fp16)
+            hipLaunchKernelGGL((k_dense_terms<8, __half>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity);
+        else
+            hipLaunchKernelGGL((k_dense_terms<8, float>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity);
+    }
+    else
+    {
+        if (fp16)
+            hipLaunchKernelGGL((k_dense_terms<4, __half>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity);
+        else
+            hipLaunchKernelGGL((k_dense_terms<4, float>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_reg_terms(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int parity,
+                            hipStream_t st)
+{
+    if (n_wg == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_reg_terms, dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_diag_rhs(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
+                           hipStream_t st)
+{
+    if (n_wg == 0)
+        return hipSuccess;
+    if (K == 8)
+    {
+        if (fp16)
+            hipLaunchKernelGGL((k_diag_rhs<8, __half>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve);
+        else
+            hipLaunchKernelGGL((k_diag_rhs<8, float>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve);
+    }
+    else
+    {
+        if (fp16)
+            hipLaunchKernelGGL((k_diag_rhs<4, __half>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve);
+        else
+            hipLaunchKernelGGL((k_diag_rhs<4, float>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_dinv(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, hipStream_t st)
+{
+    if (n_wg == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_dinv, dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve);
+    return hipGetLastError();
+}
+
+hipError_t launch_offdiag(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
+                          hipStream_t st)
+{
+    if (n_wg == 0)
+        return hipSuccess;
+    if (K == 8)
+    {
+        if (fp16)
+            hipLaunchKernelGGL((k_offdiag<8, __half>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve);
+        else
+            hipLaunchKernelGGL((k_offdiag<8, float>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve);
+    }
+    else
+    {
+        if (fp16)
+            hipLaunchKernelGGL((k_offdiag<4, __half>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve);
+        else
+            hipLaunchKernelGGL((k_offdiag<4, float>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pcg_init(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, double *part_rs,
+                           hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pcg_init, dim3(n_wg), dim3(kPcgThreads), 0, st, sds, wmap, n_solve, part_rs);
+    return hipGetLastError();
+}
+
+hipError_t launch_pcg_spmv(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int k,
+                           const double *part_rs, double *part_dq, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pcg_spmv, dim3(n_wg), dim3(kPcgThreads), 0, st, sds, wmap, n_solve, k, part_rs, part_dq);
+    return hipGetLastError();
+}
+
+hipError_t launch_pcg_update(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int k,
+                             const double *part_dq, double *part_rs, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pcg_update, dim3(n_wg), dim3(kPcgThreads), 0, st, sds, wmap, n_solve, k, part_dq, part_rs);
+    return hipGetLastError();
+}
+
+hipError_t launch_update(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int parity,
+                         hipStream_t st)
+{
+    hipLaunchKernelGGL(k_update, dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_skin(uint32_t K, const float4 *surfels, uint32_t S, const float4 *nodes, uint32_t N, float r2,
+                       uint16_t *knn, float *spatial, float *connect, hipStream_t st)
+{
+    if (S == 0)
+        return hipSuccess;
+    if (K == 8)
+        hipLaunchKernelGGL(k_skin<8>, SH_GRID(S, 256), dim3(256), 0, st, surfels, S, nodes, N, r2, knn, spatial,
+                           connect);
+    else
+        hipLaunchKernelGGL(k_skin<4>, SH_GRID(S, 256), dim3(256), 0, st, surfels, S, nodes, N, r2, knn, spatial,
+                           connect);
+    return hipGetLastError();
+}
+
+hipError_t launch_node_graph(uint32_t K, const float4 *nodes, uint32_t N, float r2, uint16_t *node_knn,
+                             float *node_spatial, uint16_t *edges, float *node_connect, hipStream_t st)
+{
+    if (K == 8)
+        hipLaunchKernelGGL(k_node_graph<8>, SH_GRID(N, 256), dim3(256), 0, st, nodes, N, r2, node_knn, node_spatial,
+                           edges, node_connect);
+    else
+        hipLaunchKernelGGL(k_node_graph<4>, SH_GRID(N, 256), dim3(256), 0, st, nodes, N, r2, node_knn, node_spatial,
+                           edges, node_connect);
+    return hipGetLastError();
+}
+
+} // namespace sh

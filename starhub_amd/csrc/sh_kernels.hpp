@@ -1,0 +1,62 @@
+// Host-side launchers of the solver kernels (defined in sh_kernels.hip) and
+// the rocPRIM-backed sort / scan helpers (sh_sort.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "sh_desc.hpp"
+
+namespace sh
+{
+hipError_t launch_pixel_flags(const uint32_t *index_map, uint32_t n, uint32_t *flags, hipStream_t st);
+hipError_t launch_compact_pixels(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos, hipStream_t st);
+hipError_t launch_node2term_kv(const SolveDesc &d, uint16_t *keys, uint32_t *vals, hipStream_t st);
+hipError_t launch_lower_bound_u16(const uint16_t *keys, uint32_t n_keys, uint32_t n_out, uint32_t *out,
+                                  hipStream_t st);
+hipError_t launch_pair_kv(const SolveDesc &d, uint32_t *keys, uint32_t *vals, hipStream_t st);
+hipError_t launch_pair_heads(const uint32_t *keys, uint32_t n, uint32_t *head, uint32_t *meta, hipStream_t st);
+hipError_t launch_pair_compact(const uint32_t *keys, const uint32_t *head, const uint32_t *hpos, uint32_t n,
+                               const uint32_t *meta, uint32_t *half_keys, uint32_t *half_off, uint32_t *meta_out,
+                               hipStream_t st);
+hipError_t launch_pair_entries(const SolveDesc &d, const uint32_t *sorted_vals, hipStream_t st);
+hipError_t launch_sym_kv(const SolveDesc &d, uint32_t *keys, uint32_t *vals, hipStream_t st);
+hipError_t launch_sym_scatter(const SolveDesc &d, const uint32_t *skeys, const uint32_t *svals, hipStream_t st);
+hipError_t launch_row_ptr(const SolveDesc &d, const uint32_t *skeys, hipStream_t st);
+
+hipError_t launch_identity(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, hipStream_t st);
+hipError_t launch_dense_terms(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve,
+                              uint32_t n_wg, int parity, hipStream_t st);
+hipError_t launch_reg_terms(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int parity,
+                            hipStream_t st);
+hipError_t launch_diag_rhs(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
+                           hipStream_t st);
+hipError_t launch_dinv(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, hipStream_t st);
+hipError_t launch_offdiag(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
+                          hipStream_t st);
+hipError_t launch_pcg_init(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, double *part_rs,
+                           hipStream_t st);
+hipError_t launch_pcg_spmv(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int k,
+                           const double *part_rs, double *part_dq, hipStream_t st);
+hipError_t launch_pcg_update(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int k,
+                             const double *part_dq, double *part_rs, hipStream_t st);
+hipError_t launch_update(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int parity,
+                         hipStream_t st);
+hipError_t launch_skin(uint32_t K, const float4 *surfels, uint32_t S, const float4 *nodes, uint32_t N, float r2,
+                       uint16_t *knn, float *spatial, float *connect, hipStream_t st);
+hipError_t launch_node_graph(uint32_t K, const float4 *nodes, uint32_t N, float r2, uint16_t *node_knn,
+                             float *node_spatial, uint16_t *edges, float *node_connect, hipStream_t st);
+
+// work-group geometry of the GN-phase kernels
+constexpr uint32_t kThreads = 256;
+constexpr uint32_t kWavesPerWg = 4;
+constexpr uint32_t kPcgNodesPerWg = 32;
+
+// rocPRIM wrappers (sh_sort.hip); temp == nullptr queries the temp size
+hipError_t sort_pairs_u16_u32(void *temp, size_t &temp_bytes, const uint16_t *kin, uint16_t *kout,
+                              const uint32_t *vin, uint32_t *vout, uint32_t n, int end_bit, hipStream_t st);
+hipError_t sort_pairs_u32_u32(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout,
+                              const uint32_t *vin, uint32_t *vout, uint32_t n, int end_bit, hipStream_t st);
+hipError_t exclusive_scan_u32(void *temp, size_t &temp_bytes, const uint32_t *in, uint32_t *out, uint32_t n,
+                              hipStream_t st);
+} // namespace sh

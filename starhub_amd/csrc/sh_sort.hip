@@ -1,0 +1,31 @@
+// Device-wide stable radix sorts and scans for the once-per-solve index build.
+// rocPRIM's LSD radix sort is stable, which is exactly the semantics the
+// reference's CUB DeviceRadixSort::SortPairs calls rely on
+// (libs/star/common/algorithm_types.cuh:26-124), so the sorted node->term and
+// node-pair->term lists match element for element.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "sh_kernels.hpp"
+
+namespace sh
+{
+hipError_t sort_pairs_u16_u32(void *temp, size_t &temp_bytes, const uint16_t *kin, uint16_t *kout,
+                              const uint32_t *vin, uint32_t *vout, uint32_t n, int end_bit, hipStream_t st)
+{
+    return rocprim::radix_sort_pairs(temp, temp_bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), st);
+}
+
+hipError_t sort_pairs_u32_u32(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout,
+                              const uint32_t *vin, uint32_t *vout, uint32_t n, int end_bit, hipStream_t st)
+{
+    return rocprim::radix_sort_pairs(temp, temp_bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), st);
+}
+
+hipError_t exclusive_scan_u32(void *temp, size_t &temp_bytes, const uint32_t *in, uint32_t *out, uint32_t n,
+                              hipStream_t st)
+{
+    return rocprim::exclusive_scan(temp, temp_bytes, in, out, 0u, n, rocprim::plus<uint32_t>(), st);
+}
+} // namespace sh

@@ -1,0 +1,332 @@
+"""ctypes binding of the HIP solver library (``include/sh/warp_solver.h``).
+
+Python-side mirror of the reference's processor API
+(``OptimizationProcessorWarpSolver::ProcessFrame`` / ``SolvedSE3``,
+examples/mono_star/opt/OptimizationProcessorWarpSolver.h:18-30).  Device
+memory and streams come from PyTorch (ROCm); every solve runs in the native
+library, and loading fails loudly when that library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+_LIB = None
+
+SH_OK = 0
+SH_STREAM_FP32, SH_STREAM_FP16 = 0, 1
+SH_SPMV_MATERIALIZED, SH_SPMV_MATRIX_FREE, SH_SPMV_AUTO = 0, 1, 2
+
+_VP = ctypes.c_void_p
+
+
+class SolverConfig(ctypes.Structure):
+    _fields_ = [
+        ("max_width", ctypes.c_uint32), ("max_height", ctypes.c_uint32), ("max_nodes", ctypes.c_uint32),
+        ("knn", ctypes.c_uint32), ("gn_iterations", ctypes.c_uint32), ("pcg_iterations", ctypes.c_uint32),
+        ("lambda_depth", ctypes.c_float), ("lambda_flow", ctypes.c_float), ("lambda_reg", ctypes.c_float),
+        ("lambda_node_translation", ctypes.c_float), ("lambda_feature", ctypes.c_float),
+        ("damping", ctypes.c_float), ("stream_precision", ctypes.c_uint32), ("spmv_mode", ctypes.c_uint32),
+    ]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("fx", ctypes.c_float),
+                ("fy", ctypes.c_float), ("cx", ctypes.c_float), ("cy", ctypes.c_float)]
+
+
+class MeasureMaps(ctypes.Structure):
+    _fields_ = [("vertex_confid", _VP), ("normal_radius", _VP)]
+
+
+class RenderMaps(ctypes.Structure):
+    _fields_ = [("reference_vertex", _VP), ("reference_normal", _VP), ("index_map", _VP)]
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [("surfel_knn", _VP), ("surfel_knn_spatial_weight", _VP), ("surfel_knn_connect_weight", _VP),
+                ("num_surfels", ctypes.c_uint32)]
+
+
+class NodeGraph(ctypes.Structure):
+    _fields_ = [("reference_node_coords", _VP), ("node_graph", _VP), ("node_knn_connect_weight", _VP),
+                ("node_knn", _VP), ("node_knn_spatial_weight", _VP), ("num_nodes", ctypes.c_uint32),
+                ("num_edges", ctypes.c_uint32), ("node_radius_square", ctypes.c_float)]
+
+
+class NodeFlow(ctypes.Structure):
+    _fields_ = [("node_motion_pred", _VP), ("num_nodes", ctypes.c_uint32)]
+
+
+class FlowMap(ctypes.Structure):
+    _fields_ = [("flow", _VP)]
+
+
+class KeyPoints(ctypes.Structure):
+    _fields_ = [("num_matches", ctypes.c_uint32)]
+
+
+def lib_path() -> Path:
+    return Path(__file__).resolve().parent / "_build" / "libstarhub_amd.so"
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = lib_path()
+        if not path.exists():
+            raise RuntimeError(f"native solver library {path} is missing: run `python -m starhub_amd.build`")
+        import torch  # noqa: F401  -- load the process's HIP runtime before ours resolves against it
+        L = ctypes.CDLL(str(path))
+        P = ctypes.POINTER
+        L.sh_version.restype = ctypes.c_char_p
+        L.sh_status_string.restype = ctypes.c_char_p
+        L.sh_status_string.argtypes = [ctypes.c_int]
+        L.sh_solver_create.argtypes = [P(SolverConfig), P(_VP)]
+        L.sh_solver_destroy.argtypes = [_VP]
+        L.sh_solver_set_inputs.argtypes = [_VP, P(Camera), P(MeasureMaps), P(RenderMaps), P(Geometry),
+                                           P(NodeGraph), P(NodeFlow), P(FlowMap), P(KeyPoints),
+                                           P(ctypes.c_float)]
+        L.sh_solver_solve.argtypes = [_VP, _VP]
+        L.sh_solver_node_se3.argtypes = [_VP, P(_VP), P(ctypes.c_uint32)]
+        L.sh_solver_sizes.argtypes = [_VP, P(ctypes.c_uint32), P(ctypes.c_uint32), P(ctypes.c_uint32)]
+        L.sh_solver_debug_view.argtypes = [_VP, ctypes.c_char_p, P(_VP), P(ctypes.c_size_t)]
+        L.sh_solver_cost_history.argtypes = [_VP, P(ctypes.c_double), ctypes.c_uint32, P(ctypes.c_uint32)]
+        L.sh_solver_apply_jtj.argtypes = [_VP, _VP, _VP, ctypes.c_uint32, _VP]
+        L.sh_skin_surfels.argtypes = [_VP, ctypes.c_uint32, _VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                                      _VP, _VP, _VP, _VP]
+        L.sh_build_node_graph.argtypes = [_VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, _VP, _VP, _VP,
+                                          _VP, _VP]
+        L.sh_batch_create.argtypes = [P(SolverConfig), ctypes.c_uint32, P(_VP)]
+        L.sh_batch_destroy.argtypes = [_VP]
+        L.sh_batch_set_inputs.argtypes = [_VP, ctypes.c_uint32, P(Camera), P(MeasureMaps), P(RenderMaps),
+                                          P(Geometry), P(NodeGraph), P(NodeFlow), P(FlowMap), P(KeyPoints),
+                                          P(ctypes.c_float)]
+        L.sh_batch_solve.argtypes = [_VP, ctypes.c_uint32, _VP]
+        L.sh_batch_node_se3.argtypes = [_VP, ctypes.c_uint32, P(_VP), P(ctypes.c_uint32)]
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols() -> list[str]:
+    """Names declared in include/sh/warp_solver.h that the library must export."""
+    hdr = Path(__file__).resolve().parent.parent / "include" / "sh" / "warp_solver.h"
+    import re
+    names = []
+    for m in re.finditer(r"^\s*(?:const\s+)?\w+\s*\*?\s*(sh_\w+)\s*\(", hdr.read_text(), re.M):
+        names.append(m.group(1))
+    return names
+
+
+class SolverError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str) -> None:
+    if rc != SH_OK:
+        msg = lib().sh_status_string(rc).decode()
+        raise SolverError(f"{what} failed: {msg} ({rc})")
+
+
+def make_config(params, max_width: int, max_height: int, max_nodes: int, K: int = 8,
+                spmv_mode: int = SH_SPMV_MATERIALIZED) -> SolverConfig:
+    return SolverConfig(max_width, max_height, max_nodes, K, params.gn_iterations, params.pcg_iterations,
+                        params.lambda_depth, params.lambda_flow, params.lambda_reg, params.lambda_node_translation,
+                        params.lambda_feature, params.damping,
+                        SH_STREAM_FP16 if params.fp16 else SH_STREAM_FP32, spmv_mode)
+
+
+@dataclass
+class DeviceFrame:
+    """Device-resident inputs of one solve (torch tensors on the GPU)."""
+    width: int
+    height: int
+    fx: float
+    fy: float
+    cx: float
+    cy: float
+    cam2world: np.ndarray
+    measure_vertex: object
+    measure_normal: object
+    ref_vertex: object
+    ref_normal: object
+    index_map: object
+    flow: object | None
+    surfel_knn: object
+    surfel_spatial: object
+    surfel_connect: object
+    node_coords: object
+    edges: object
+    node_connect: object
+    node_knn: object
+    node_spatial: object
+    node_radius_square: float
+    num_surfels: int
+    num_nodes: int
+    num_edges: int
+
+    def structs(self):
+        ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p()
+        cam = Camera(self.width, self.height, self.fx, self.fy, self.cx, self.cy)
+        meas = MeasureMaps(ptr(self.measure_vertex), ptr(self.measure_normal))
+        rend = RenderMaps(ptr(self.ref_vertex), ptr(self.ref_normal), ptr(self.index_map))
+        geo = Geometry(ptr(self.surfel_knn), ptr(self.surfel_spatial), ptr(self.surfel_connect), self.num_surfels)
+        ng = NodeGraph(ptr(self.node_coords), ptr(self.edges), ptr(self.node_connect), ptr(self.node_knn),
+                       ptr(self.node_spatial), self.num_nodes, self.num_edges, self.node_radius_square)
+        nf = NodeFlow(ctypes.c_void_p(), self.num_nodes)
+        fl = FlowMap(ptr(self.flow))
+        kp = KeyPoints(0)
+        c2w = (ctypes.c_float * 16)(*[float(v) for v in np.asarray(self.cam2world, np.float32).reshape(16)])
+        return cam, meas, rend, geo, ng, nf, fl, kp, c2w
+
+
+def _stream_ptr(stream) -> ctypes.c_void_p:
+    if stream is None:
+        return ctypes.c_void_p()
+    return ctypes.c_void_p(int(stream.cuda_stream))
+
+
+class WarpSolver:
+    """One solve handle (sh_solver)."""
+
+    def __init__(self, config: SolverConfig):
+        self._h = ctypes.c_void_p()
+        check(lib().sh_solver_create(ctypes.byref(config), ctypes.byref(self._h)), "sh_solver_create")
+        self.config = config
+        self._frame = None
+
+    def close(self):
+        if self._h:
+            lib().sh_solver_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_inputs(self, frame: DeviceFrame) -> None:
+        s = frame.structs()
+        self._frame = frame
+        check(lib().sh_solver_set_inputs(self._h, *[ctypes.byref(x) for x in s[:-1]], s[-1]), "sh_solver_set_inputs")
+
+    def solve(self, stream=None) -> None:
+        check(lib().sh_solver_solve(self._h, _stream_ptr(stream)), "sh_solver_solve")
+
+    def process_frame(self, frame: DeviceFrame, stream=None) -> None:
+        """ProcessFrame: set the inputs and solve (OptimizationProcessorWarpSolver.cu:22-52)."""
+        self.set_inputs(frame)
+        self.solve(stream)
+
+    def node_se3(self):
+        """SolvedSE3(): N x 8 float32 (q0 xyzw, q1 xyzw) copy as a torch tensor on the device."""
+        import torch
+        view = ctypes.c_void_p()
+        n = ctypes.c_uint32()
+        check(lib().sh_solver_node_se3(self._h, ctypes.byref(view), ctypes.byref(n)), "sh_solver_node_se3")
+        return _device_copy(view.value, n.value * 8, torch.float32).view(n.value, 8)
+
+    def sizes(self) -> tuple[int, int, int]:
+        a, b, c = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().sh_solver_sizes(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "sh_solver_sizes")
+        return a.value, b.value, c.value
+
+    def debug(self, name: str, dtype):
+        """Copy of an internal device array (see sh_solver_debug_view)."""
+        view = ctypes.c_void_p()
+        cnt = ctypes.c_size_t()
+        check(lib().sh_solver_debug_view(self._h, name.encode(), ctypes.byref(view), ctypes.byref(cnt)),
+              f"debug_view({name})")
+        return _device_copy(view.value, cnt.value, dtype)
+
+
+class BatchSolver:
+    """Batched uneven solves (sh_batch): one launch per GN phase covers every slot."""
+
+    def __init__(self, config: SolverConfig, max_solves: int):
+        self._h = ctypes.c_void_p()
+        check(lib().sh_batch_create(ctypes.byref(config), max_solves, ctypes.byref(self._h)), "sh_batch_create")
+        self.config = config
+        self.max_solves = max_solves
+
+    def close(self):
+        if self._h:
+            lib().sh_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_inputs(self, slot: int, frame: DeviceFrame) -> None:
+        s = frame.structs()
+        check(lib().sh_batch_set_inputs(self._h, slot, *[ctypes.byref(x) for x in s[:-1]], s[-1]),
+              "sh_batch_set_inputs")
+
+    def solve(self, num: int, stream=None) -> None:
+        check(lib().sh_batch_solve(self._h, num, _stream_ptr(stream)), "sh_batch_solve")
+
+    def node_se3(self, slot: int):
+        import torch
+        view = ctypes.c_void_p()
+        n = ctypes.c_uint32()
+        check(lib().sh_batch_node_se3(self._h, slot, ctypes.byref(view), ctypes.byref(n)), "sh_batch_node_se3")
+        return _device_copy(view.value, n.value * 8, torch.float32).view(n.value, 8)
+
+
+def _device_copy(addr: int, count: int, dtype):
+    """Copy `count` elements at device address `addr` into a new torch tensor."""
+    import torch
+    out = torch.empty(count, dtype=dtype, device="cuda")
+    if count:
+        nbytes = count * out.element_size()
+        hip = _hip()
+        rc = hip.hipMemcpy(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(addr), ctypes.c_size_t(nbytes), 3)
+        if rc != 0:
+            raise SolverError(f"hipMemcpy failed ({rc})")
+    return out
+
+
+_HIP = None
+
+
+def _hip():
+    global _HIP
+    if _HIP is None:
+        import torch  # noqa: F401  -- the HIP runtime torch loaded (same soname as ours)
+        _HIP = ctypes.CDLL("libamdhip64.so.7")
+        _HIP.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    return _HIP
+
+
+def skin_surfels(surfel_pos, nodes, K: int, r2: float, stream=None):
+    """SkinnerKernel on the device; returns (knn u16, spatial f32, connect f32) torch tensors."""
+    import torch
+    S, N = surfel_pos.shape[0], nodes.shape[0]
+    knn = torch.empty((S, K), dtype=torch.int16, device="cuda")
+    w = torch.empty((S, K), dtype=torch.float32, device="cuda")
+    c = torch.empty((S, K), dtype=torch.float32, device="cuda")
+    check(lib().sh_skin_surfels(ctypes.c_void_p(surfel_pos.data_ptr()), S, ctypes.c_void_p(nodes.data_ptr()), N, K,
+                                r2, ctypes.c_void_p(knn.data_ptr()), ctypes.c_void_p(w.data_ptr()),
+                                ctypes.c_void_p(c.data_ptr()), _stream_ptr(stream)), "sh_skin_surfels")
+    return knn, w, c
+
+
+def build_node_graph(nodes, K: int, r2: float, stream=None):
+    """Node-graph KNN + edges on the device; returns (knn, spatial, edges (N*K x 3), connect)."""
+    import torch
+    N = nodes.shape[0]
+    knn = torch.empty((N, K), dtype=torch.int16, device="cuda")
+    w = torch.empty((N, K), dtype=torch.float32, device="cuda")
+    edges = torch.empty((N * K, 3), dtype=torch.int16, device="cuda")
+    c = torch.empty((N, K), dtype=torch.float32, device="cuda")
+    check(lib().sh_build_node_graph(ctypes.c_void_p(nodes.data_ptr()), N, K, r2, ctypes.c_void_p(knn.data_ptr()),
+                                    ctypes.c_void_p(w.data_ptr()), ctypes.c_void_p(edges.data_ptr()),
+                                    ctypes.c_void_p(c.data_ptr()), _stream_ptr(stream)), "sh_build_node_graph")
+    return knn, w, edges, c
