@@ -312,7 +312,7 @@ struct KnnHeap
     float max_dist;
     KnnHeap(int k, float init_d) : K(k)
     {
-        for (int i = 0; i < K; ++i)
+        for (int i = 0; i < 16; ++i)
         {
             dist[i] = init_d;
             idx[i] = 0;
@@ -1189,7 +1189,9 @@ public:
         {
             spmv(pd, pq);
             const T dq_ = vdot(pd, pq);
-            const T alpha = delta_new / dq_;
+            // a zero system (e.g. no valid pixel, warp at rest) gives a zero step
+            // instead of the reference's 0/0
+            const T alpha = (dq_ != T(0)) ? delta_new / dq_ : T(0);
             for (size_t i = 0; i < n; ++i)
             {
                 x[i] += alpha * pd[i];
@@ -1198,7 +1200,7 @@ public:
             precond(pt, ps);
             const T delta_old = delta_new;
             delta_new = vdot(pt, ps);
-            const T beta = delta_new / delta_old;
+            const T beta = (delta_old != T(0)) ? delta_new / delta_old : T(0);
             for (size_t i = 0; i < n; ++i)
                 pd[i] = ps[i] + beta * pd[i];
             pr.swap(pt);

@@ -262,4 +262,38 @@ extern "C"
         out34[11] = m.t.z;
     }
     void sho_invert6(const double *A, double *Ainv) { sho::Solver<double>::invert6(A, Ainv); }
+
+    // Evaluate the per-pixel residual/Jacobian and the ARAP records at a given
+    // warp field (FP64): dq is N x 8 (q0 wxyz, q1 wxyz).  Outputs: r (P*3),
+    // J (P*18), ti/tj (E*3), valid (E).  Returns P, or -1 if r_cap is too small.
+    int sho_eval_terms(const sho_inputs *cin, const sho_params *cp, const double *dq, double *r, size_t r_cap,
+                       double *J, double *ti, double *tj, uint8_t *valid)
+    {
+        sho::Result res;
+        const sho::Inputs in = to_inputs(cin);
+        const sho::Params p = to_params(cp);
+        sho::Solver<double> s(in, p, res);
+        s.compact_pixels();
+        if (size_t(s.P) * 3 > r_cap)
+            return -1;
+        s.dq.resize(in.num_nodes);
+        for (int n = 0; n < in.num_nodes; ++n)
+            s.dq[n] = {{dq[8 * n], dq[8 * n + 1], dq[8 * n + 2], dq[8 * n + 3]},
+                       {dq[8 * n + 4], dq[8 * n + 5], dq[8 * n + 6], dq[8 * n + 7]}};
+        s.dense_terms();
+        s.reg_terms();
+        std::memcpy(r, s.r.data(), sizeof(double) * s.r.size());
+        std::memcpy(J, s.J.data(), sizeof(double) * s.J.size());
+        for (int e = 0; e < in.num_edges; ++e)
+        {
+            ti[3 * e] = s.Ti_xj[e].x;
+            ti[3 * e + 1] = s.Ti_xj[e].y;
+            ti[3 * e + 2] = s.Ti_xj[e].z;
+            tj[3 * e] = s.Tj_xj[e].x;
+            tj[3 * e + 1] = s.Tj_xj[e].y;
+            tj[3 * e + 2] = s.Tj_xj[e].z;
+            valid[e] = s.valid[e];
+        }
+        return s.P;
+    }
 }

@@ -111,8 +111,7 @@ RESULT_FIELDS = ["pixel_pair", "pixel_knn", "n2t_offset", "n2t_terms", "half_key
                  "reg_valid", "diag", "rhs", "offdiag", "diag_inv", "pcg_x", "cost", "dq_iter", "dq"]
 
 
-def solve(inputs: dict, params, precision: int = 64, threads: int = 1) -> dict:
-    """Run the restated GN solve.  `inputs` holds numpy arrays (see solver_inputs)."""
+def _inputs_struct(inputs: dict):
     keep = []
 
     def fp(a):
@@ -143,8 +142,7 @@ def solve(inputs: dict, params, precision: int = 64, threads: int = 1) -> dict:
     c.ref_normal = fp(inputs["ref_normal"])
     c.index_map = up32(inputs["index_map"])
     c.flow = fp(inputs.get("flow"))
-    K = int(inputs["K"])
-    c.K = K
+    c.K = int(inputs["K"])
     c.num_surfels = int(inputs["surfel_knn"].shape[0])
     c.surfel_knn = up16(inputs["surfel_knn"])
     c.surfel_spatial = fp(inputs["surfel_spatial"])
@@ -156,6 +154,12 @@ def solve(inputs: dict, params, precision: int = 64, threads: int = 1) -> dict:
     c.node_connect = fp(inputs["node_connect"])
     c.node_motion_pred = fp(inputs.get("node_motion_pred"))
     c.node_radius_square = float(inputs["node_radius_square"])
+    return c, keep
+
+
+def solve(inputs: dict, params, precision: int = 64, threads: int = 1) -> dict:
+    """Run the restated GN solve.  `inputs` holds numpy arrays (see starhub_amd.workload.host_inputs)."""
+    c, keep = _inputs_struct(inputs)
     p = ShoParams(params.gn_iterations, params.pcg_iterations, params.lambda_depth, params.lambda_flow,
                   params.lambda_reg, params.lambda_node_translation, params.damping)
     h = lib().sho_solve(ctypes.byref(c), ctypes.byref(p), precision, threads)
@@ -215,3 +219,30 @@ def round_to_half(f: float) -> float:
 
 def float2uint_rn(f: float) -> int:
     return int(lib().sho_float2uint_rn(f))
+
+
+def eval_terms(inputs: dict, params, dq_wxyz: np.ndarray):
+    """Residual/Jacobian per pixel and ARAP records at the warp field `dq_wxyz` (N x 8, FP64)."""
+    L = lib()
+    if not hasattr(L, "_eval_set"):
+        L.sho_eval_terms.argtypes = [ctypes.POINTER(ShoInputs), ctypes.POINTER(ShoParams), _DP, _DP, ctypes.c_size_t,
+                                     _DP, _DP, _DP, ctypes.POINTER(ctypes.c_uint8)]
+        L.sho_eval_terms.restype = ctypes.c_int
+        L._eval_set = True
+    c, keep = _inputs_struct(inputs)
+    p = ShoParams(params.gn_iterations, params.pcg_iterations, params.lambda_depth, params.lambda_flow,
+                  params.lambda_reg, params.lambda_node_translation, params.damping)
+    H, W = inputs["index_map"].shape
+    cap = H * W
+    E = inputs["edges"].shape[0]
+    r = np.zeros(cap * 3)
+    J = np.zeros(cap * 18)
+    ti = np.zeros(E * 3)
+    tj = np.zeros(E * 3)
+    va = np.zeros(E, np.uint8)
+    dq = np.ascontiguousarray(dq_wxyz, np.float64)
+    P = L.sho_eval_terms(ctypes.byref(c), ctypes.byref(p), dq.ctypes.data_as(_DP), r.ctypes.data_as(_DP), r.size,
+                         J.ctypes.data_as(_DP), ti.ctypes.data_as(_DP), tj.ctypes.data_as(_DP),
+                         va.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    assert P >= 0
+    return r[:3 * P].reshape(P, 3), J[:18 * P].reshape(P, 18), ti.reshape(E, 3), tj.reshape(E, 3), va

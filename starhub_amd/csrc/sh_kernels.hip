@@ -997,7 +997,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_spmv(const SolveDesc *__res
     const SolveDesc &d = sds[si];
     const double delta = reduce_partials(part_rs, wg_begin[si], wg_begin[si + 1], sh);
     double beta = 0.0;
-    if (k > 0)
+    if (k > 0 && d.delta[k - 1] != 0.0)
         beta = delta / d.delta[k - 1];
     if (threadIdx.x == 0 && blockIdx.x == wg_begin[si])
         d.delta[k] = delta;
@@ -1043,7 +1043,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(const SolveDesc *__r
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
     const SolveDesc &d = sds[si];
     const double dq = reduce_partials(part_dq, wg_begin[si], wg_begin[si + 1], sh);
-    const double alpha = d.delta[k] / dq;
+    const double alpha = (dq != 0.0) ? d.delta[k] / dq : 0.0;
     const float *__restrict__ dnew = d.dv[(k + 1) & 1];
     const uint32_t node = (blockIdx.x - wg_begin[si]) * kPcgNodes + threadIdx.x / 6;
     const int a = threadIdx.x % 6;

@@ -1,0 +1,244 @@
+"""GPU parity: the HIP path (through the C-ABI) against the oracle.
+
+Integer / index work must match bit for bit; floating-point stages within the
+tolerances written below; the solved warp field within 1e-4 relative
+(max over nodes of ||T_gpu - T_fp64||_F / ||T_fp64||_F) of the FP64 oracle.
+"""
+from dataclasses import replace
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+WARP_TOL = 1e-4
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available()
+    torch.cuda.set_device(0)
+    return torch
+
+
+def _idx(solver, name, torch):
+    return solver.debug(name, torch.int32).cpu().numpy().view(np.uint32)
+
+
+def _solve(sc, producers=None, spmv=0, params=None):
+    from starhub_amd import native, workload
+    if params is not None:
+        sc = replace(sc, cfg=replace(sc.cfg, solver=params))
+    frame, prod = workload.device_frame(sc, producers)
+    solver = workload.solver_for(sc, spmv)
+    solver.process_frame(frame)
+    return solver, frame, prod, sc
+
+
+@pytest.mark.parametrize("K", [8, 4])
+def test_producers_bit_exact(torch_dev, K):
+    import sho
+    from starhub_amd import scene, workload
+    cfg = replace(scene.config_c1(), K=K)
+    say("producers K", K)
+    sc = scene.generate(cfg)
+    _, prod = workload.device_frame(sc)
+    knn, w, c = sho.skin(sc.surfel_pos, sc.nodes, K, sc.node_radius_square)
+    nk, nw, edges, nc = sho.node_graph(sc.nodes, K, sc.node_radius_square)
+    assert np.array_equal(prod["surfel_knn"].reshape(knn.shape), knn)
+    assert np.array_equal(prod["node_knn"].reshape(nk.shape), nk)
+    assert np.array_equal(prod["edges"].reshape(edges.shape), edges)
+    # __expf vs exp: weights to FP32 tolerance
+    assert np.allclose(prod["surfel_spatial"].reshape(w.shape), w, rtol=2e-6, atol=1e-7)
+    assert np.allclose(prod["node_spatial"].reshape(nw.shape), nw, rtol=2e-6, atol=1e-7)
+    assert np.all(prod["surfel_connect"] == 1) and np.all(prod["node_connect"] == 1)
+
+
+def test_producer_tie_known_answers(torch_dev):
+    from starhub_amd import native
+    torch = torch_dev
+    corners = [[x, y, z, 1] for x in (0, 1) for y in (0, 1) for z in (0, 1)]
+    nodes = torch.tensor([[5, 5, 5, 1]] + corners, dtype=torch.float32).cuda()
+    surf = torch.tensor([[0.5, 0.5, 0.5, 1]], dtype=torch.float32).cuda()
+    knn, w, c = native.skin_surfels(surf, nodes, 8, 1.0)
+    torch.cuda.synchronize()
+    assert knn.cpu().numpy()[0].tolist() == [7, 8, 1, 2, 3, 4, 5, 6]
+    nodes = torch.tensor([[0, 0, 0, 1], [1, 0, 0, 1], [0, 1, 0, 1], [0, 0, 1, 1], [-1, 0, 0, 1], [0, -1, 0, 1]],
+                         dtype=torch.float32).cuda()
+    nk, nw, edges, nc = native.build_node_graph(nodes, 4, 1.0)
+    torch.cuda.synchronize()
+    assert nk.cpu().numpy()[0].tolist() == [0, 3, 1, 2]
+
+
+@pytest.fixture(scope="module")
+def c1_one_iter(torch_dev):
+    import sho
+    from starhub_amd import scene, workload
+    sc = scene.generate(scene.config_c1())
+    params = replace(sc.cfg.solver, gn_iterations=1)
+    solver, frame, prod, sc1 = _solve(sc, params=params)
+    ref = sho.solve(workload.host_inputs(sc1, prod), params, 64, sho.max_threads())
+    return solver, ref, sc1
+
+
+def test_index_bit_exact(c1_one_iter, torch_dev):
+    torch = torch_dev
+    solver, ref, sc = c1_one_iter
+    P, U, _ = solver.sizes()
+    N = sc.cfg.num_nodes
+    say("index P", P, "U", U)
+    assert P == ref["num_pixels"] and U == len(ref["half_keys"])
+    assert np.array_equal(solver.debug("pix", torch.int16).cpu().numpy().view(np.uint16), ref["pixel_pair"])
+    off = _idx(solver, "n2t_off", torch)
+    assert np.array_equal(off, ref["n2t_offset"])
+    assert np.array_equal(_idx(solver, "n2t_terms", torch)[:off[-1]], ref["n2t_terms"])
+    assert np.array_equal(_idx(solver, "half_keys", torch), ref["half_keys"])
+    assert np.array_equal(_idx(solver, "half_off", torch), ref["half_offset"])
+    pe = _idx(solver, "pair_entries", torch)
+    terms = np.where(pe & 0x80000000, P + (pe & 0x3FFFFFFF), pe & 0x1FFFFFF)
+    assert np.array_equal(terms, ref["pair_terms"])
+    rp = _idx(solver, "row_ptr", torch)
+    bc = _idx(solver, "blk_col", torch)
+    assert np.array_equal(rp, ref["row_offset"])
+    sym = np.concatenate([i * 4096 + bc[rp[i] + i + 1:rp[i + 1] + i + 1] for i in range(N)]).astype(np.uint32)
+    assert np.array_equal(sym, ref["sym_keys"])
+
+
+def test_stage_values(c1_one_iter, torch_dev):
+    torch = torch_dev
+    solver, ref, sc = c1_one_iter
+    P, U, _ = solver.sizes()
+    N = sc.cfg.num_nodes
+    jr = solver.debug("jr", torch.float32).cpu().numpy().reshape(P, 24).astype(np.float64)
+    J, r = ref["jacobian"].reshape(P, 18), ref["residual"].reshape(P, 3)
+    assert np.abs(jr[:, :18] - J).max() <= 1e-6 * np.abs(J).max()
+    assert np.abs(jr[:, 18:21] - r).max() <= 1e-6
+    rp = _idx(solver, "row_ptr", torch)
+    blk = solver.debug("blk", torch.float32).cpu().numpy().reshape(-1, 36).astype(np.float64)
+    D = np.stack([blk[rp[i] + i] for i in range(N)])
+    assert np.abs(D - ref["diag"].reshape(N, 36)).max() <= 1e-5 * np.abs(ref["diag"]).max()
+    B = np.concatenate([blk[rp[i] + i + 1:rp[i + 1] + i + 1] for i in range(N)])
+    assert np.abs(B - ref["offdiag"].reshape(-1, 36)).max() <= 1e-5 * np.abs(ref["offdiag"]).max()
+    rhs = solver.debug("rhs", torch.float32).cpu().numpy().astype(np.float64)
+    assert np.abs(rhs - ref["rhs"]).max() <= 1e-4 * np.abs(ref["rhs"]).max()
+    dinv = solver.debug("dinv", torch.float64).cpu().numpy()
+    assert np.abs(dinv - ref["diag_inv"]).max() <= 1e-4 * np.abs(ref["diag_inv"]).max()
+    x = solver.debug("x", torch.float32).cpu().numpy().astype(np.float64)
+    assert np.abs(x - ref["pcg_x"]).max() <= 1e-3 * np.abs(ref["pcg_x"]).max()
+
+
+@pytest.mark.parametrize("name", ["tiny_k8", "tiny_k4"])
+def test_golden_fixture(torch_dev, name):
+    import make_golden
+    from starhub_amd import workload
+    torch = torch_dev
+    g = np.load(GOLDEN / f"{name}.npz")
+    cfg = make_golden.case_config(name)
+    from starhub_amd import scene
+    sc = scene.generate(cfg)
+    solver, frame, prod, _ = _solve(sc)
+    assert np.array_equal(prod["surfel_knn"].reshape(g["prod_surfel_knn"].shape), g["prod_surfel_knn"])
+    assert np.array_equal(prod["edges"].reshape(g["prod_edges"].shape), g["prod_edges"])
+    off = _idx(solver, "n2t_off", torch)
+    assert np.array_equal(off, g["n2t_offset"])
+    assert np.array_equal(_idx(solver, "n2t_terms", torch)[:off[-1]], g["n2t_terms"])
+    assert np.array_equal(_idx(solver, "half_keys", torch), g["half_keys"])
+    gpu = workload.device_dq_to_wxyz(solver.node_se3())
+    assert workload.warp_error(gpu, g["dq"].reshape(-1, 8)) <= WARP_TOL
+
+
+@pytest.mark.parametrize("which", ["c1", "c2"])
+def test_full_solve_vs_fp64(torch_dev, which):
+    import sho
+    from starhub_amd import scene, workload
+    cfg = {"c1": scene.config_c1, "c2": scene.config_c2}[which]()
+    say("full solve", which)
+    sc = scene.generate(cfg)
+    solver, frame, prod, _ = _solve(sc)
+    gpu = workload.device_dq_to_wxyz(solver.node_se3())
+    ref = sho.solve(workload.host_inputs(sc, prod), cfg.solver, 64, sho.max_threads())
+    err = workload.warp_error(gpu, ref["dq"].reshape(-1, 8))
+    say(which, "warp error vs fp64", err)
+    assert err <= WARP_TOL
+    assert np.all(np.isfinite(gpu))
+
+
+def test_fp16_streams_vs_fp64(torch_dev):
+    import sho
+    from starhub_amd import scene, workload
+    cfg = scene.shrink(scene.config_c3(), 2)
+    say("fp16", cfg.num_surfels, cfg.num_nodes, cfg.width, cfg.height)
+    sc = scene.generate(cfg)
+    solver, frame, prod, _ = _solve(sc)
+    gpu = workload.device_dq_to_wxyz(solver.node_se3())
+    ref = sho.solve(workload.host_inputs(sc, prod), cfg.solver, 64, sho.max_threads())
+    err = workload.warp_error(gpu, ref["dq"].reshape(-1, 8))
+    say("fp16 warp error vs fp64", err)
+    assert err <= WARP_TOL
+
+
+def test_repeat_is_bit_identical(torch_dev):
+    from starhub_amd import scene
+    sc = scene.generate(scene.config_c1())
+    solver, frame, prod, _ = _solve(sc)
+    a = solver.node_se3().cpu().numpy()
+    solver.solve()
+    b = solver.node_se3().cpu().numpy()
+    assert np.array_equal(a, b)
+
+
+def test_batch_equals_solo(torch_dev):
+    """Uneven solves in one batch give the same bits as each solve alone."""
+    import sho
+    from starhub_amd import native, scene, workload
+    cfgs = [scene.config_batch_member(n, 900 + i) for i, n in enumerate([128, 300, 700, 200])]
+    scs = [scene.generate(c) for c in cfgs]
+    frames = [workload.device_frame(s) for s in scs]
+    W = max(c.width for c in cfgs)
+    H = max(c.height for c in cfgs)
+    N = max(c.num_nodes for c in cfgs)
+    bcfg = native.make_config(cfgs[0].solver, W, H, N, 8)
+    batch = native.BatchSolver(bcfg, len(cfgs))
+    for i, (f, _) in enumerate(frames):
+        batch.set_inputs(i, f)
+    batch.solve(len(cfgs))
+    for i, (s, (f, prod)) in enumerate(zip(scs, frames)):
+        solo = native.WarpSolver(bcfg)
+        solo.process_frame(f)
+        a = batch.node_se3(i).cpu().numpy()
+        b = solo.node_se3().cpu().numpy()
+        assert np.array_equal(a, b), i
+        ref = sho.solve(workload.host_inputs(s, prod), s.cfg.solver, 64, sho.max_threads())
+        assert workload.warp_error(workload.device_dq_to_wxyz(batch.node_se3(i)), ref["dq"].reshape(-1, 8)) <= WARP_TOL
+
+
+def test_empty_frame_gives_identity(torch_dev):
+    from starhub_amd import scene, workload
+    sc = scene.generate(scene.config_c1())
+    sc = replace(sc, index_map=np.full_like(sc.index_map, 0xFFFFFFFF))
+    solver, frame, prod, _ = _solve(sc)
+    P, U, _ = solver.sizes()
+    assert P == 0 and U > 0
+    dq = workload.device_dq_to_wxyz(solver.node_se3())
+    assert np.allclose(dq, np.tile([1, 0, 0, 0, 0, 0, 0, 0], (len(dq), 1)))
+
+
+def test_capacity_errors(torch_dev):
+    from starhub_amd import native, scene, workload
+    sc = scene.generate(scene.config_c1())
+    frame, prod = workload.device_frame(sc)
+    small = native.make_config(sc.cfg.solver, 160, 120, 512, 8)
+    s = native.WarpSolver(small)
+    with pytest.raises(native.SolverError, match="capacity"):
+        s.set_inputs(frame)
+    few = native.make_config(sc.cfg.solver, 320, 240, 128, 8)
+    s = native.WarpSolver(few)
+    with pytest.raises(native.SolverError, match="capacity"):
+        s.set_inputs(frame)
