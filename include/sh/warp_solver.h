@@ -49,6 +49,20 @@ typedef enum sh_spmv_mode
     SH_SPMV_AUTO = 2          /* autotuner picks */
 } sh_spmv_mode;
 
+/* Optional behaviour (sh_solver_config.flags). */
+typedef enum sh_solver_flags
+{
+    /* Also build the reference's node-pair -> term index (NodePair2TermsIndex.cu:294-312):
+     * unique (i<j) pairs, their sorted term lists and the symmetric keys.  Not needed by
+     * the default row-major materialisation; exposed through sh_solver_debug_view. */
+    SH_FLAG_REFERENCE_INDEX = 1u,
+    /* Materialise off-diagonal blocks pair-major from that index (the reference's
+     * traversal, JtJMaterializer.cu:66-175).  Implies SH_FLAG_REFERENCE_INDEX. */
+    SH_FLAG_PAIR_MAJOR = 2u,
+    /* Run the PCG as two launches per iteration instead of one persistent cooperative launch. */
+    SH_FLAG_NO_PERSISTENT_PCG = 4u
+} sh_solver_flags;
+
 /* Dual quaternion as stored by the reference: two float4 (x, y, z, w),
  * real part q0 first (libs/star/math/DualQuaternion.hpp:206). */
 typedef struct sh_dual_quat
@@ -76,6 +90,7 @@ typedef struct sh_solver_config
     float damping;                  /* 1e-3 added to the JtJ diagonal (PreconditionerRhsBuilder.cu:249-252) */
     uint32_t stream_precision;      /* sh_stream_precision */
     uint32_t spmv_mode;             /* sh_spmv_mode */
+    uint32_t flags;                 /* sh_solver_flags */
 } sh_solver_config;
 
 /* Pinhole camera of the solver images (ConfigParser.cu:231-256). */

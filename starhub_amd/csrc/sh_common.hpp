@@ -197,18 +197,127 @@ __device__ __forceinline__ void st_stream(float *p, size_t i, float v) { p[i] = 
 __device__ __forceinline__ void st_stream(__half *p, size_t i, float v) { p[i] = __float2half_rn(v); }
 
 // ------------------------------------------------------------ term layout
-// Per-pixel Jacobian/residual record (24 scalars): J[c][0..5] for c = 0..2
-// (rotation xyz, translation xyz) at 6c..6c+5, residual r[c] at 18+c, pad.
-constexpr int kJR = 24;
+// Per-pixel normal-equation record (28 scalars), written once per GN iteration
+// by the dense-term kernel: the 21 upper-triangle entries of
+// H = sum_c lambda_c^2 J_c^T J_c (row-major a <= b) followed by
+// g = sum_c lambda_c^2 r_c J_c (6) and one pad.  Node k of the pixel then
+// contributes w_i w_k H to block (i, k) and w_i^2 g to node i's J^T W r
+// (PreconditionerRhsBuilder.cu:38-108, JtDotResidualIndexed.cu:40-150,
+// JtJMaterializer.cu:31-63 with the linear-blend Jacobian w_k J).
+constexpr int kHG = 28;
+constexpr int kHGg = 21;
+
+// index of (a, b) in the packed upper triangle
+__host__ __device__ constexpr int sym_index(int a, int b)
+{
+    return (a <= b) ? (a * 6 - a * (a - 1) / 2 + (b - a)) : (b * 6 - b * (b - 1) / 2 + (a - b));
+}
 // Per-edge ARAP record (8 floats): Ti_xj (3), Tj_xj (3), connect weight, valid.
 constexpr int kReg = 8;
 
-// Packed (node pair, term) entry for the dense terms: pixel index in the low
-// 26 bits, slot of the row node in bits 26..28, slot of the column node in
-// bits 29..31.  ARAP entries carry the edge index with the marker bit pattern
-// in kRegEntry.
-constexpr uint32_t kSlotShiftI = 26;
-constexpr uint32_t kSlotShiftJ = 29;
-constexpr uint32_t kPixMask = (1u << 26) - 1u;
+
+// The FP16 record is block-scaled: the 27 values are stored as v * 2^-s, with
+// s chosen so that the largest magnitude stays below 2^15, and s itself (an
+// integer, exact in FP16) is kept in the pad slot.  The products of a
+// flow-weighted Jacobian (entries of order f / z) would otherwise overflow
+// FP16's range of 65504.
+__device__ __forceinline__ int hg_shift(const float (&v)[kHG])
+{
+    float m = 0.f;
+#pragma unroll
+    for (int e = 0; e < kHG - 1; ++e)
+        m = fmaxf(m, fabsf(v[e]));
+    if (!(m > 0.f) || !isfinite(m))
+        return 0;
+    int x;
+    (void)frexpf(m, &x); // m < 2^x
+    return x - 15;
+}
+
+template <class JT>
+__device__ __forceinline__ void store_hg(void *base, uint32_t p, const float (&v)[kHG])
+{
+    if constexpr (sizeof(JT) == 4)
+    {
+        float4 *o = reinterpret_cast<float4 *>(reinterpret_cast<float *>(base) + size_t(p) * kHG);
+#pragma unroll
+        for (int a = 0; a < kHG / 4; ++a)
+            o[a] = make_float4(v[4 * a], v[4 * a + 1], v[4 * a + 2], v[4 * a + 3]);
+    }
+    else
+    {
+        const int sh = hg_shift(v);
+        const float inv = ldexpf(1.f, -sh);
+        uint2 *o = reinterpret_cast<uint2 *>(reinterpret_cast<__half *>(base) + size_t(p) * kHG);
+#pragma unroll
+        for (int a = 0; a < kHG / 4; ++a)
+        {
+            uint32_t w[2];
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+            {
+                const int e0 = 4 * a + 2 * b, e1 = e0 + 1;
+                const __half lo = __float2half_rn(v[e0] * inv);
+                const __half hi = (e1 == kHG - 1) ? __float2half_rn(float(sh)) : __float2half_rn(v[e1] * inv);
+                w[b] = uint32_t(__half_as_ushort(lo)) | (uint32_t(__half_as_ushort(hi)) << 16);
+            }
+            o[a] = make_uint2(w[0], w[1]);
+        }
+    }
+}
+
+// Loads the 27 values (pad slot returned as 0).
+template <class JT>
+__device__ __forceinline__ void load_hg(const void *base, uint32_t p, float (&v)[kHG])
+{
+    if constexpr (sizeof(JT) == 4)
+    {
+        const float4 *o = reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(base) + size_t(p) * kHG);
+#pragma unroll
+        for (int a = 0; a < kHG / 4; ++a)
+        {
+            const float4 f = o[a];
+            v[4 * a] = f.x;
+            v[4 * a + 1] = f.y;
+            v[4 * a + 2] = f.z;
+            v[4 * a + 3] = f.w;
+        }
+    }
+    else
+    {
+        const uint2 *o = reinterpret_cast<const uint2 *>(reinterpret_cast<const __half *>(base) + size_t(p) * kHG);
+        float raw[kHG];
+#pragma unroll
+        for (int a = 0; a < kHG / 4; ++a)
+        {
+            const uint2 u = o[a];
+            const uint32_t w[2] = {u.x, u.y};
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+            {
+                raw[4 * a + 2 * b] = __half2float(__ushort_as_half(uint16_t(w[b] & 0xffffu)));
+                raw[4 * a + 2 * b + 1] = __half2float(__ushort_as_half(uint16_t(w[b] >> 16)));
+            }
+        }
+        const float scale = ldexpf(1.f, int(raw[kHG - 1]));
+#pragma unroll
+        for (int e = 0; e < kHG - 1; ++e)
+            v[e] = raw[e] * scale;
+        v[kHG - 1] = 0.f;
+    }
+}
+
+// One value of the record (e < kHG - 1).
+template <class JT>
+__device__ __forceinline__ float ld_hg(const void *base, uint32_t p, int e)
+{
+    if constexpr (sizeof(JT) == 4)
+        return ld_stream(reinterpret_cast<const JT *>(base), size_t(p) * kHG + e);
+    else
+    {
+        const __half *h = reinterpret_cast<const __half *>(base) + size_t(p) * kHG;
+        return __half2float(h[e]) * ldexpf(1.f, int(__half2float(h[kHG - 1])));
+    }
+}
 
 } // namespace sh

@@ -26,6 +26,7 @@ struct SolveDesc
     int U;               // unique upper node pairs
     int n_pair_entries;  // valid (pair, term) entries
     int pcg_iters;
+    uint32_t blk_cap;    // capacity of blk_col / blocks (N_max + 2 U_max)
 
     // caller-owned inputs
     const float4 *meas_v, *meas_n, *ref_v, *ref_n;
@@ -53,6 +54,7 @@ struct SolveDesc
     uint32_t *lower_pos;    // U: position of (j,i)
     uint32_t *row_ptr;      // N+1 into the sorted symmetric keys
     uint32_t *blk_col;      // N + 2U block columns, row-major, diagonal block first
+    uint32_t *dense_cnt;    // N: number of dense terms leading each node's term list
     // GN iteration state
     dq8 *dq[2];       // double buffer (SolverIterationData.h)
     void *jr;         // P*24 residual/Jacobian records, FP32 or FP16

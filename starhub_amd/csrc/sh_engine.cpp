@@ -82,9 +82,11 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         return SH_E_ARG;
     if (c.pcg_iterations == 0 || c.pcg_iterations > 1000 || c.gn_iterations > 1000)
         return SH_E_ARG;
-    if (c.stream_precision > SH_STREAM_FP16 || c.spmv_mode > SH_SPMV_AUTO)
+    if (c.stream_precision > SH_STREAM_FP16 || c.spmv_mode > SH_SPMV_AUTO || (c.flags & ~7u))
         return SH_E_ARG;
     cfg_ = c;
+    if (cfg_.flags & SH_FLAG_PAIR_MAJOR)
+        cfg_.flags |= SH_FLAG_REFERENCE_INDEX;
     K_ = int(c.knn);
     fp16_ = (c.stream_precision == SH_STREAM_FP16);
     const uint32_t np = uint32_t(K_ * (K_ - 1) / 2);
@@ -108,6 +110,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         std::memset(&d, 0, sizeof(d));
         d.K = K_;
         d.pcg_iters = int(c.pcg_iterations);
+        d.blk_cap = uint32_t(ncap_ + 2 * ucap_);
 #define A(ptr, bytes)                                                                                                  \
     if ((st = alloc(reinterpret_cast<void **>(&(ptr)), (bytes))) != SH_OK)                                            \
         return st;
@@ -125,7 +128,8 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         A(d.blk_col, (size_t(ncap_) + 2 * size_t(ucap_)) * 4);
         A(d.dq[0], size_t(ncap_) * 32);
         A(d.dq[1], size_t(ncap_) * 32);
-        A(d.jr, size_t(pcap_) * kJR * (fp16_ ? 2 : 4));
+        A(d.dense_cnt, size_t(ncap_) * 4);
+        A(d.jr, size_t(pcap_) * kHG * (fp16_ ? 2 : 4));
         A(d.reg, size_t(ecap_) * kReg * 4);
         A(d.ttrans, size_t(ncap_) * 16);
         A(d.blk, (size_t(ncap_) + 2 * size_t(ucap_)) * 36 * 4);
@@ -154,6 +158,8 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
     A(ws_head_, size_t(pair_cap_) * 4);
     A(ws_hpos_, size_t(pair_cap_) * 4);
     A(ws_meta_, 64);
+    A(ws_bitmap_, size_t(ncap_) * kBitmapWords * 4);
+    A(ws_rowlen_, size_t(ncap_ + 1) * 4);
     {
         size_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
         SH_HIP(sort_pairs_u16_u32(nullptr, b0, nullptr, nullptr, nullptr, nullptr, n2t_cap_, 16, nullptr));
@@ -171,6 +177,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
     A(d_part_dq_, pcg_wg * 8);
 #undef A
     SH_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_meta_), 64, hipHostMallocDefault));
+    coop_max_groups_ = pcg_coop_max_groups();
     stage_bytes_ = size_t(max_slots) * sizeof(SolveDesc) + size_t(kNumMaps) * (max_slots + 1) * 4;
     SH_HIP(hipHostMalloc(&h_stage_, stage_bytes_, hipHostMallocDefault));
     return SH_OK;
@@ -272,30 +279,52 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
     SH_HIP(sort_pairs_u16_u32(sort_temp_, tb, k16a, k16b, v32a, d.n2t_terms, n_kv, 16, st));
     SH_HIP(launch_lower_bound_u16(k16b, n_kv, uint32_t(d.N) + 1, d.n2t_off, st));
 
-    // node pair -> term, half index (NodePair2TermsIndex::BuildHalfIndex, :294-298,591-654)
+    uint32_t *meta = static_cast<uint32_t *>(ws_meta_);
+    uint32_t *rowlen = static_cast<uint32_t *>(ws_rowlen_);
+    uint32_t *bitmap = static_cast<uint32_t *>(ws_bitmap_);
+    SH_HIP(hipMemsetAsync(meta, 0, 64, st));
+    SH_HIP(hipMemsetAsync(rowlen + d.N, 0, 4, st));
+    // block-row structure from the node->term lists (dense-term counts, row lengths)
+    SH_HIP(launch_row_bitmap(d, bitmap, rowlen, meta, st));
+    if (!(cfg_.flags & SH_FLAG_REFERENCE_INDEX))
+    {
+        tb = sort_temp_bytes_;
+        SH_HIP(exclusive_scan_u32(sort_temp_, tb, rowlen, d.row_ptr, uint32_t(d.N) + 1, st));
+        SH_HIP(launch_row_cols(d, bitmap, meta, st));
+        SH_HIP(hipMemcpyAsync(h_meta_ + 4, meta, 8, hipMemcpyDeviceToHost, st));
+        SH_HIP(hipStreamSynchronize(st));
+        const uint32_t U = h_meta_[5];
+        if (U > ucap_)
+            return SH_E_CAPACITY;
+        d.U = int(U);
+        d.n_pair_entries = 0;
+        sl.max_row_len = h_meta_[4];
+        sl.indexed = true;
+        return SH_OK;
+    }
+
+    // reference node pair -> term index, half index (NodePair2TermsIndex::BuildHalfIndex, :294-298,591-654)
     const uint32_t np = uint32_t(K_ * (K_ - 1) / 2);
     const uint32_t n_pkv = P * np + uint32_t(d.E);
     uint32_t *k32a = static_cast<uint32_t *>(ws_k32a_), *k32b = static_cast<uint32_t *>(ws_k32b_);
     uint32_t *p32a = static_cast<uint32_t *>(ws_p32a_), *p32b = static_cast<uint32_t *>(ws_p32b_);
     uint32_t *head = static_cast<uint32_t *>(ws_head_), *hpos = static_cast<uint32_t *>(ws_hpos_);
-    uint32_t *meta = static_cast<uint32_t *>(ws_meta_);
     SH_HIP(launch_pair_kv(d, k32a, p32a, st));
     tb = sort_temp_bytes_;
     SH_HIP(sort_pairs_u32_u32(sort_temp_, tb, k32a, k32b, p32a, p32b, n_pkv, 24, st));
-    SH_HIP(hipMemsetAsync(meta, 0, 16, st));
-    SH_HIP(launch_pair_heads(k32b, n_pkv, head, meta, st));
+    SH_HIP(launch_pair_heads(k32b, n_pkv, head, meta + 8, st));
     tb = sort_temp_bytes_;
     SH_HIP(exclusive_scan_u32(sort_temp_, tb, head, hpos, n_pkv, st));
-    // the half arrays are written up to index U, bounded by the capacity check below
-    // only after U is known; write into the workspace first
-    SH_HIP(launch_pair_compact(k32b, head, hpos, n_pkv, meta, k32a, p32a, meta + 2, st));
-    SH_HIP(hipMemcpyAsync(h_meta_ + 2, meta + 2, 8, hipMemcpyDeviceToHost, st));
+    // the half arrays go to the workspace first: U is checked against the capacity below
+    SH_HIP(launch_pair_compact(k32b, head, hpos, n_pkv, meta + 8, k32a, p32a, meta + 2, st));
+    SH_HIP(hipMemcpyAsync(h_meta_, meta, 16, hipMemcpyDeviceToHost, st));
     SH_HIP(hipStreamSynchronize(st));
     const uint32_t U = h_meta_[2];
     if (U > ucap_)
         return SH_E_CAPACITY;
     d.U = int(U);
     d.n_pair_entries = int(h_meta_[3]);
+    sl.max_row_len = h_meta_[0];
     SH_HIP(hipMemcpyAsync(d.half_keys, k32a, size_t(U) * 4, hipMemcpyDeviceToDevice, st));
     SH_HIP(hipMemcpyAsync(d.half_off, p32a, size_t(U + 1) * 4, hipMemcpyDeviceToDevice, st));
     SH_HIP(launch_pair_entries(d, p32b, st));
@@ -341,6 +370,9 @@ void Engine::build_maps(uint32_t ns)
             case kMapNode:
                 wg = div_up(uint32_t(d.N), kThreads);
                 break;
+            case kMapRow:
+                wg = uint32_t(d.N);
+                break;
             }
             maps_[size_t(m) * (ns + 1) + s] = acc;
             acc += wg;
@@ -375,19 +407,45 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st)
     const bool fp16 = fp16_;
     SH_HIP(launch_identity(sds, map(kMapNode, ns), n, map_total_[kMapNode], st));
     double *prs = static_cast<double *>(d_part_rs_), *pdq = static_cast<double *>(d_part_dq_);
+    const bool use_coop = !(cfg_.flags & SH_FLAG_NO_PERSISTENT_PCG) && coop_max_groups_ > 0 &&
+                          map_total_[kMapPcg] <= uint32_t(coop_max_groups_);
+    uint32_t max_n = 0, max_l = 0;
+    for (uint32_t s = 0; s < ns; ++s)
+    {
+        max_n = std::max(max_n, uint32_t(slots_[s].d.N));
+        max_l = std::max(max_l, slots_[s].max_row_len);
+    }
+    rows_lds_ = rows_lds_bytes(max_n, max_l);
+    if (!(cfg_.flags & SH_FLAG_PAIR_MAJOR) && rows_lds_ > 160u * 1024u)
+        return SH_E_CAPACITY;
     for (uint32_t it = 0; it < cfg_.gn_iterations; ++it)
     {
         const int parity = int(it & 1u);
         SH_HIP(launch_dense_terms(K_, fp16, sds, map(kMapPixel, ns), n, map_total_[kMapPixel], parity, st));
         SH_HIP(launch_reg_terms(sds, map(kMapReg, ns), n, map_total_[kMapReg], parity, st));
-        SH_HIP(launch_diag_rhs(K_, fp16, sds, map(kMapNodeWave, ns), n, map_total_[kMapNodeWave], st));
-        SH_HIP(launch_dinv(sds, map(kMapNode, ns), n, map_total_[kMapNode], st));
-        SH_HIP(launch_offdiag(K_, fp16, sds, map(kMapPairWave, ns), n, map_total_[kMapPairWave], st));
-        SH_HIP(launch_pcg_init(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], prs, st));
-        for (uint32_t k = 0; k < cfg_.pcg_iterations; ++k)
+        if (cfg_.flags & SH_FLAG_PAIR_MAJOR)
         {
-            SH_HIP(launch_pcg_spmv(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), prs, pdq, st));
-            SH_HIP(launch_pcg_update(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), pdq, prs, st));
+            SH_HIP(launch_diag_rhs(K_, fp16, sds, map(kMapNodeWave, ns), n, map_total_[kMapNodeWave], st));
+            SH_HIP(launch_offdiag(K_, fp16, sds, map(kMapPairWave, ns), n, map_total_[kMapPairWave], st));
+        }
+        else
+        {
+            SH_HIP(launch_rows(K_, fp16, sds, map(kMapRow, ns), n, map_total_[kMapRow], rows_lds_, st));
+        }
+        SH_HIP(launch_dinv(sds, map(kMapNode, ns), n, map_total_[kMapNode], st));
+        if (use_coop)
+        {
+            SH_HIP(launch_pcg_coop(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(cfg_.pcg_iterations), prs, pdq,
+                                   st));
+        }
+        else
+        {
+            SH_HIP(launch_pcg_init(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], prs, st));
+            for (uint32_t k = 0; k < cfg_.pcg_iterations; ++k)
+            {
+                SH_HIP(launch_pcg_spmv(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), prs, pdq, st));
+                SH_HIP(launch_pcg_update(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), pdq, prs, st));
+            }
         }
         SH_HIP(launch_update(sds, map(kMapNode, ns), n, map_total_[kMapNode], parity, st));
     }
@@ -447,7 +505,8 @@ int Engine::debug_view(uint32_t slot, const char *name, const void **ptr, size_t
         {"lower_pos", d.lower_pos, U},
         {"row_ptr", d.row_ptr, N + 1},
         {"blk_col", d.blk_col, N + 2 * U},
-        {"jr", d.jr, P * kJR},
+        {"hg", d.jr, P * kHG},
+        {"dense_cnt", d.dense_cnt, N},
         {"reg", d.reg, E * kReg},
         {"blk", d.blk, (N + 2 * U) * 36},
         {"dinv", d.dinv, N * 36},

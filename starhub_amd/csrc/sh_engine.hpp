@@ -40,6 +40,7 @@ private:
         bool has_inputs = false;
         bool indexed = false;
         bool solved = false;
+        uint32_t max_row_len = 0;
     };
     enum
     {
@@ -49,6 +50,7 @@ private:
         kMapPairWave,
         kMapPcg,
         kMapNode,
+        kMapRow,
         kNumMaps
     };
 
@@ -71,6 +73,9 @@ private:
     void *ws_flags_ = nullptr, *ws_pos_ = nullptr, *ws_k16a_ = nullptr, *ws_k16b_ = nullptr, *ws_v32a_ = nullptr;
     void *ws_k32a_ = nullptr, *ws_k32b_ = nullptr, *ws_p32a_ = nullptr, *ws_p32b_ = nullptr;
     void *ws_head_ = nullptr, *ws_hpos_ = nullptr, *ws_meta_ = nullptr;
+    void *ws_bitmap_ = nullptr, *ws_rowlen_ = nullptr;
+    int coop_max_groups_ = 0;
+    size_t rows_lds_ = 0;
     void *sort_temp_ = nullptr;
     size_t sort_temp_bytes_ = 0;
     uint32_t *h_meta_ = nullptr;

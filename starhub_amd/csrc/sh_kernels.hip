@@ -359,69 +359,6 @@ __device__ __forceinline__ dq8 load_dq(const dq8 *__restrict__ a, uint32_t i)
     return r;
 }
 
-template <class JT>
-__device__ __forceinline__ void store_jr(void *base, uint32_t p, const float (&v)[kJR])
-{
-    if constexpr (sizeof(JT) == 4)
-    {
-        float4 *o = reinterpret_cast<float4 *>(reinterpret_cast<float *>(base) + size_t(p) * kJR);
-#pragma unroll
-        for (int a = 0; a < kJR / 4; ++a)
-            o[a] = make_float4(v[4 * a], v[4 * a + 1], v[4 * a + 2], v[4 * a + 3]);
-    }
-    else
-    {
-        uint4 *o = reinterpret_cast<uint4 *>(reinterpret_cast<__half *>(base) + size_t(p) * kJR);
-#pragma unroll
-        for (int a = 0; a < kJR / 8; ++a)
-        {
-            uint32_t w[4];
-#pragma unroll
-            for (int b = 0; b < 4; ++b)
-            {
-                const __half lo = __float2half_rn(v[8 * a + 2 * b]);
-                const __half hi = __float2half_rn(v[8 * a + 2 * b + 1]);
-                w[b] = uint32_t(__half_as_ushort(lo)) | (uint32_t(__half_as_ushort(hi)) << 16);
-            }
-            o[a] = make_uint4(w[0], w[1], w[2], w[3]);
-        }
-    }
-}
-
-template <class JT>
-__device__ __forceinline__ void load_jr(const void *base, uint32_t p, float (&v)[kJR])
-{
-    if constexpr (sizeof(JT) == 4)
-    {
-        const float4 *o = reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(base) + size_t(p) * kJR);
-#pragma unroll
-        for (int a = 0; a < kJR / 4; ++a)
-        {
-            const float4 f = o[a];
-            v[4 * a] = f.x;
-            v[4 * a + 1] = f.y;
-            v[4 * a + 2] = f.z;
-            v[4 * a + 3] = f.w;
-        }
-    }
-    else
-    {
-        const uint4 *o = reinterpret_cast<const uint4 *>(reinterpret_cast<const __half *>(base) + size_t(p) * kJR);
-#pragma unroll
-        for (int a = 0; a < kJR / 8; ++a)
-        {
-            const uint4 u = o[a];
-            const uint32_t w[4] = {u.x, u.y, u.z, u.w};
-#pragma unroll
-            for (int b = 0; b < 4; ++b)
-            {
-                v[8 * a + 2 * b] = __half2float(__ushort_as_half(uint16_t(w[b] & 0xffffu)));
-                v[8 * a + 2 * b + 1] = __half2float(__ushort_as_half(uint16_t(w[b] >> 16)));
-            }
-        }
-    }
-}
-
 // Per-pixel residual / Jacobian (computeDenseImageJacobianKernel,
 // DenseImageJacobianResidual.cu:164-310; forms :12-157).
 template <int K, class JT>
@@ -462,9 +399,9 @@ __global__ __launch_bounds__(256) void k_dense_terms(const SolveDesc *__restrict
     const f3 nw = rot_apply(T.r, mk3(cn.x, cn.y, cn.z));
     const f3 vc = rot_apply(d.w2c_r, vw) + mk3(d.w2c_t[0], d.w2c_t[1], d.w2c_t[2]);
     const f3 nc = rot_apply(d.w2c_r, nw);
-    float out[kJR];
+    float out[24]; // J[c][0..5] at 6c, r[c] at 18 + c
 #pragma unroll
-    for (int a = 0; a < kJR; ++a)
+    for (int a = 0; a < 24; ++a)
         out[a] = 0.f;
     if (px.z < d.W && px.w < d.H)
     {
@@ -516,7 +453,31 @@ __global__ __launch_bounds__(256) void k_dense_terms(const SolveDesc *__restrict
             }
         }
     }
-    store_jr<JT>(d.jr, p, out);
+    // fold the channels: H = sum_c lam_c^2 J_c^T J_c, g = sum_c lam_c^2 r_c J_c
+    const float lam2[3] = {d.lam2_depth, d.lam2_flow, d.lam2_flow};
+    float hg[kHG];
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+        for (int b = a; b < 6; ++b)
+        {
+            float h = 0.f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                h += lam2[c] * out[6 * c + a] * out[6 * c + b];
+            hg[sym_index(a, b)] = h;
+        }
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+    {
+        float g = 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            g += lam2[c] * out[18 + c] * out[6 * c + q];
+        hg[kHGg + q] = g;
+    }
+    hg[kHG - 1] = 0.f;
+    store_hg<JT>(d.jr, p, hg);
 }
 
 // ARAP forward warp per edge + node translations (forwardWarpSmootherNodeKernel,
@@ -665,8 +626,6 @@ __global__ __launch_bounds__(256) void k_diag_rhs(const SolveDesc *__restrict__ 
 #pragma unroll
     for (int a = 0; a < 6; ++a)
         g[a] = 0.f;
-    const float lam2[3] = {d.lam2_depth, d.lam2_flow, d.lam2_flow};
-    const bool use_flow = (d.lam2_flow != 0.f);
     const uint32_t P = d.P, E = d.E;
     const uint32_t beg = d.n2t_off[n], end = d.n2t_off[n + 1];
     for (uint32_t it = beg + lane; it < end; it += kWave)
@@ -682,27 +641,15 @@ __global__ __launch_bounds__(256) void k_diag_rhs(const SolveDesc *__restrict__ 
             for (int k = 0; k < K; ++k)
                 if (kn[k] == n)
                     w = w8[k];
-            float jr[kJR];
-            load_jr<JT>(d.jr, t, jr);
+            float hg[kHG];
+            load_hg<JT>(d.jr, t, hg);
+            const float w2 = w * w;
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-            {
-                if (c > 0 && !use_flow)
-                    break;
-                float jw[6];
+            for (int e = 0; e < 21; ++e)
+                acc[e] += w2 * hg[e];
 #pragma unroll
-                for (int q = 0; q < 6; ++q)
-                    jw[q] = jr[6 * c + q] * w;
-#pragma unroll
-                for (int a = 0; a < 6; ++a)
-#pragma unroll
-                    for (int b = a; b < 6; ++b)
-                        acc[kSymIdx[a][b]] += lam2[c] * jw[b] * jw[a];
-                const float ow = lam2[c] * (w * w) * jr[18 + c];
-#pragma unroll
-                for (int q = 0; q < 6; ++q)
-                    g[q] += ow * jr[6 * c + q];
-            }
+            for (int q = 0; q < 6; ++q)
+                g[q] += w2 * hg[kHGg + q];
         }
         else if (t < P + E)
         {
@@ -820,8 +767,6 @@ __global__ __launch_bounds__(256) void k_offdiag(const SolveDesc *__restrict__ s
     const uint32_t u = (blockIdx.x - wg_begin[si]) * 4 + wave;
     if (u >= uint32_t(d.U))
         return;
-    const float lam2[3] = {d.lam2_depth, d.lam2_flow, d.lam2_flow};
-    const bool use_flow = (d.lam2_flow != 0.f);
     const uint32_t beg = d.half_off[u], end = d.half_off[u + 1];
     float acc[21];
 #pragma unroll
@@ -835,20 +780,11 @@ __global__ __launch_bounds__(256) void k_offdiag(const SolveDesc *__restrict__ s
         const uint32_t p = pk & 0x1ffffffu;
         const uint32_t sa = (pk >> 25) & 7u, sb = (pk >> 28) & 7u;
         const float ww = d.pw[size_t(p) * K + sa] * d.pw[size_t(p) * K + sb];
-        float jr[kJR];
-        load_jr<JT>(d.jr, p, jr);
+        float hg[kHG];
+        load_hg<JT>(d.jr, p, hg);
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-        {
-            if (c > 0 && !use_flow)
-                break;
-            const float sc = lam2[c] * ww;
-#pragma unroll
-            for (int a = 0; a < 6; ++a)
-#pragma unroll
-                for (int b = a; b < 6; ++b)
-                    acc[kSymIdx[a][b]] += sc * jr[6 * c + b] * jr[6 * c + a];
-        }
+        for (int e = 0; e < 21; ++e)
+            acc[e] += ww * hg[e];
     }
 #pragma unroll
     for (int a = 0; a < 21; ++a)

@@ -47,6 +47,16 @@ hipError_t launch_skin(uint32_t K, const float4 *surfels, uint32_t S, const floa
 hipError_t launch_node_graph(uint32_t K, const float4 *nodes, uint32_t N, float r2, uint16_t *node_knn,
                              float *node_spatial, uint16_t *edges, float *node_connect, hipStream_t st);
 
+hipError_t launch_row_bitmap(const SolveDesc &d, uint32_t *bitmap, uint32_t *row_len, uint32_t *meta, hipStream_t st);
+hipError_t launch_row_cols(const SolveDesc &d, const uint32_t *bitmap, uint32_t *meta, hipStream_t st);
+size_t rows_lds_bytes(uint32_t max_nodes, uint32_t max_row_len);
+hipError_t launch_rows(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
+                       size_t lds, hipStream_t st);
+hipError_t launch_pcg_coop(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
+                           double *part_rs, double *part_dq, hipStream_t st);
+int pcg_coop_max_groups();
+constexpr uint32_t kBitmapWords = 128;
+
 // work-group geometry of the GN-phase kernels
 constexpr uint32_t kThreads = 256;
 constexpr uint32_t kWavesPerWg = 4;

@@ -19,6 +19,7 @@ _LIB = None
 SH_OK = 0
 SH_STREAM_FP32, SH_STREAM_FP16 = 0, 1
 SH_SPMV_MATERIALIZED, SH_SPMV_MATRIX_FREE, SH_SPMV_AUTO = 0, 1, 2
+SH_FLAG_REFERENCE_INDEX, SH_FLAG_PAIR_MAJOR, SH_FLAG_NO_PERSISTENT_PCG = 1, 2, 4
 
 _VP = ctypes.c_void_p
 
@@ -30,6 +31,7 @@ class SolverConfig(ctypes.Structure):
         ("lambda_depth", ctypes.c_float), ("lambda_flow", ctypes.c_float), ("lambda_reg", ctypes.c_float),
         ("lambda_node_translation", ctypes.c_float), ("lambda_feature", ctypes.c_float),
         ("damping", ctypes.c_float), ("stream_precision", ctypes.c_uint32), ("spmv_mode", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
     ]
 
 
@@ -132,11 +134,11 @@ def check(rc: int, what: str) -> None:
 
 
 def make_config(params, max_width: int, max_height: int, max_nodes: int, K: int = 8,
-                spmv_mode: int = SH_SPMV_MATERIALIZED) -> SolverConfig:
+                spmv_mode: int = SH_SPMV_MATERIALIZED, flags: int = 0) -> SolverConfig:
     return SolverConfig(max_width, max_height, max_nodes, K, params.gn_iterations, params.pcg_iterations,
                         params.lambda_depth, params.lambda_flow, params.lambda_reg, params.lambda_node_translation,
                         params.lambda_feature, params.damping,
-                        SH_STREAM_FP16 if params.fp16 else SH_STREAM_FP32, spmv_mode)
+                        SH_STREAM_FP16 if params.fp16 else SH_STREAM_FP32, spmv_mode, flags)
 
 
 @dataclass

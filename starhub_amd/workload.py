@@ -75,9 +75,9 @@ def host_inputs(scene: Scene, prod: dict) -> dict:
                 node_connect=prod["node_connect"], node_radius_square=scene.node_radius_square)
 
 
-def solver_for(scene: Scene, spmv_mode: int = native.SH_SPMV_MATERIALIZED) -> native.WarpSolver:
+def solver_for(scene: Scene, spmv_mode: int = native.SH_SPMV_MATERIALIZED, flags: int = 0) -> native.WarpSolver:
     cfg = scene.cfg
-    c = native.make_config(cfg.solver, cfg.width, cfg.height, cfg.num_nodes, cfg.K, spmv_mode)
+    c = native.make_config(cfg.solver, cfg.width, cfg.height, cfg.num_nodes, cfg.K, spmv_mode, flags)
     return native.WarpSolver(c)
 
 
