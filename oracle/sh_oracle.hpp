@@ -515,6 +515,7 @@ struct Result
     std::vector<double> pcg_x;         // N*6, first GN iteration
     std::vector<double> cost;          // per GN iteration (incl. a final one): total weighted cost
     std::vector<double> dq_iter;       // (gn_iters+1) * N * 8, (q0 wxyz, q1 wxyz)
+    std::vector<double> twist_l1;      // gn_iters * N: |w|_1 of each node's rotation increment
     std::vector<double> dq;            // N*8 final
 };
 
@@ -1271,6 +1272,7 @@ public:
         res.n2t_terms = n2t_terms;
         dq.assign(N, DQ<T>{{1, 0, 0, 0}, {0, 0, 0, 0}}); // identity start  SolverIterationData.cu:78-88
         res.dq_iter.clear();
+        res.twist_l1.clear();
         res.cost.clear();
         dump_dq(res.dq_iter);
         Dinv.assign(size_t(N) * 36, T(0));
@@ -1304,6 +1306,9 @@ public:
                 to_double(Dinv, res.diag_inv);
                 to_double(x, res.pcg_x);
             }
+            for (int n = 0; n < N; ++n)
+                res.twist_l1.push_back(std::fabs(double(x[6 * n])) + std::fabs(double(x[6 * n + 1])) +
+                                       std::fabs(double(x[6 * n + 2])));
             update();
             dump_dq(res.dq_iter);
         }

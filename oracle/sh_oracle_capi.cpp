@@ -196,6 +196,7 @@ extern "C"
         SHO_GET(pcg_x, 8)
         SHO_GET(cost, 8)
         SHO_GET(dq_iter, 8)
+        SHO_GET(twist_l1, 8)
         SHO_GET(dq, 8)
 #undef SHO_GET
         if (n == "num_pixels")

@@ -108,7 +108,7 @@ _DTYPES = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.float64}
 
 RESULT_FIELDS = ["pixel_pair", "pixel_knn", "n2t_offset", "n2t_terms", "half_keys", "half_offset", "pair_terms",
                  "sym_keys", "sym_range", "row_offset", "residual", "jacobian", "reg_Ti_xj", "reg_Tj_xj",
-                 "reg_valid", "diag", "rhs", "offdiag", "diag_inv", "pcg_x", "cost", "dq_iter", "dq"]
+                 "reg_valid", "diag", "rhs", "offdiag", "diag_inv", "pcg_x", "cost", "dq_iter", "twist_l1", "dq"]
 
 
 def _inputs_struct(inputs: dict):

@@ -55,6 +55,10 @@ struct SolveDesc
     uint32_t *row_ptr;      // N+1 into the sorted symmetric keys
     uint32_t *blk_col;      // N + 2U block columns, row-major, diagonal block first
     uint32_t *dense_cnt;    // N: number of dense terms leading each node's term list
+    uint32_t *seg_ptr;      // N+1: prefix of each node's dense-term segments (kSegTerms terms each)
+    int n_seg;              // dense-term segments in all
+    uint32_t part_stride;   // floats per segment partial: 8 (rhs) + 21 per block of the longest row
+    float *part;            // n_seg * part_stride partial block rows
     // GN iteration state
     dq8 *dq[2];       // double buffer (SolverIterationData.h)
     void *jr;         // P*24 residual/Jacobian records, FP32 or FP16

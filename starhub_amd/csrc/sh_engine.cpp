@@ -46,6 +46,13 @@ void Engine::release()
     for (void *p : allocs_)
         (void)hipFree(p);
     allocs_.clear();
+    for (Slot &sl : slots_)
+    {
+        if (sl.part_buf)
+            (void)hipFree(sl.part_buf);
+        sl.part_buf = nullptr;
+        sl.part_bytes = 0;
+    }
     if (h_meta_)
         (void)hipHostFree(h_meta_);
     h_meta_ = nullptr;
@@ -129,6 +136,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         A(d.dq[0], size_t(ncap_) * 32);
         A(d.dq[1], size_t(ncap_) * 32);
         A(d.dense_cnt, size_t(ncap_) * 4);
+        A(d.seg_ptr, size_t(ncap_ + 1) * 4);
         A(d.jr, size_t(pcap_) * kHG * (fp16_ ? 2 : 4));
         A(d.reg, size_t(ecap_) * kReg * 4);
         A(d.ttrans, size_t(ncap_) * 16);
@@ -160,6 +168,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
     A(ws_meta_, 64);
     A(ws_bitmap_, size_t(ncap_) * kBitmapWords * 4);
     A(ws_rowlen_, size_t(ncap_ + 1) * 4);
+    A(ws_segcnt_, size_t(ncap_ + 1) * 4);
     {
         size_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
         SH_HIP(sort_pairs_u16_u32(nullptr, b0, nullptr, nullptr, nullptr, nullptr, n2t_cap_, 16, nullptr));
@@ -172,7 +181,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
     // batch-wide launch state
     A(d_descs_, size_t(max_slots) * sizeof(SolveDesc));
     A(d_wmaps_, size_t(kNumMaps) * (max_slots + 1) * 4);
-    const size_t pcg_wg = size_t(max_slots) * div_up(ncap_, kPcgNodesPerWg);
+    const size_t pcg_wg = size_t(max_slots) * div_up(ncap_, std::min(kPcgNodesPerWg, kCoopNodesPerWg));
     A(d_part_rs_, pcg_wg * 8);
     A(d_part_dq_, pcg_wg * 8);
 #undef A
@@ -282,10 +291,16 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
     uint32_t *meta = static_cast<uint32_t *>(ws_meta_);
     uint32_t *rowlen = static_cast<uint32_t *>(ws_rowlen_);
     uint32_t *bitmap = static_cast<uint32_t *>(ws_bitmap_);
+    uint32_t *segcnt = static_cast<uint32_t *>(ws_segcnt_);
     SH_HIP(hipMemsetAsync(meta, 0, 64, st));
     SH_HIP(hipMemsetAsync(rowlen + d.N, 0, 4, st));
-    // block-row structure from the node->term lists (dense-term counts, row lengths)
-    SH_HIP(launch_row_bitmap(d, bitmap, rowlen, meta, st));
+    SH_HIP(hipMemsetAsync(segcnt + d.N, 0, 4, st));
+    // block-row structure from the node->term lists (dense-term counts and
+    // segments, row lengths)
+    SH_HIP(launch_row_bitmap(d, bitmap, rowlen, segcnt, meta, st));
+    tb = sort_temp_bytes_;
+    SH_HIP(exclusive_scan_u32(sort_temp_, tb, segcnt, d.seg_ptr, uint32_t(d.N) + 1, st));
+    SH_HIP(hipMemcpyAsync(h_meta_ + 6, d.seg_ptr + d.N, 4, hipMemcpyDeviceToHost, st));
     if (!(cfg_.flags & SH_FLAG_REFERENCE_INDEX))
     {
         tb = sort_temp_bytes_;
@@ -299,6 +314,10 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
         d.U = int(U);
         d.n_pair_entries = 0;
         sl.max_row_len = h_meta_[4];
+        d.n_seg = int(h_meta_[6]);
+        int rc = size_partials(sl);
+        if (rc != SH_OK)
+            return rc;
         sl.indexed = true;
         return SH_OK;
     }
@@ -325,6 +344,9 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
     d.U = int(U);
     d.n_pair_entries = int(h_meta_[3]);
     sl.max_row_len = h_meta_[0];
+    d.n_seg = int(h_meta_[6]);
+    if (int rc = size_partials(sl); rc != SH_OK)
+        return rc;
     SH_HIP(hipMemcpyAsync(d.half_keys, k32a, size_t(U) * 4, hipMemcpyDeviceToDevice, st));
     SH_HIP(hipMemcpyAsync(d.half_off, p32a, size_t(U + 1) * 4, hipMemcpyDeviceToDevice, st));
     SH_HIP(launch_pair_entries(d, p32b, st));
@@ -337,6 +359,33 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
     SH_HIP(launch_sym_scatter(d, k32b, p32b, st));
     SH_HIP(launch_row_ptr(d, k32b, st));
     sl.indexed = true;
+    return SH_OK;
+}
+
+// Segment partials: 8 floats of rhs + 21 per block of the longest row, per
+// segment.  Grown (never shrunk) outside the GN loop.
+int Engine::size_partials(Slot &sl)
+{
+    SolveDesc &d = sl.d;
+    d.part_stride = (sl.max_row_len + 1) * 21 + 8;
+    const size_t need = std::max<size_t>(size_t(d.n_seg) * d.part_stride * sizeof(float), 256);
+    if (need > sl.part_bytes)
+    {
+        if (sl.part_buf)
+            SH_HIP(hipFree(sl.part_buf));
+        sl.part_buf = nullptr;
+        sl.part_bytes = 0;
+        const size_t bytes = need + need / 4;
+        const hipError_t e = hipMalloc(&sl.part_buf, bytes);
+        if (e != hipSuccess)
+        {
+            last_hip_error = e;
+            sl.part_buf = nullptr;
+            return SH_E_NOMEM;
+        }
+        sl.part_bytes = bytes;
+    }
+    d.part = static_cast<float *>(sl.part_buf);
     return SH_OK;
 }
 
@@ -373,6 +422,12 @@ void Engine::build_maps(uint32_t ns)
             case kMapRow:
                 wg = uint32_t(d.N);
                 break;
+            case kMapCoop:
+                wg = div_up(uint32_t(d.N), kCoopNodesPerWg);
+                break;
+            case kMapSeg:
+                wg = uint32_t(d.n_seg);
+                break;
             }
             maps_[size_t(m) * (ns + 1) + s] = acc;
             acc += wg;
@@ -408,15 +463,15 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st)
     SH_HIP(launch_identity(sds, map(kMapNode, ns), n, map_total_[kMapNode], st));
     double *prs = static_cast<double *>(d_part_rs_), *pdq = static_cast<double *>(d_part_dq_);
     const bool use_coop = !(cfg_.flags & SH_FLAG_NO_PERSISTENT_PCG) && coop_max_groups_ > 0 &&
-                          map_total_[kMapPcg] <= uint32_t(coop_max_groups_);
+                          map_total_[kMapCoop] <= uint32_t(coop_max_groups_);
     uint32_t max_n = 0, max_l = 0;
     for (uint32_t s = 0; s < ns; ++s)
     {
         max_n = std::max(max_n, uint32_t(slots_[s].d.N));
         max_l = std::max(max_l, slots_[s].max_row_len);
     }
-    rows_lds_ = rows_lds_bytes(max_n, max_l);
-    if (!(cfg_.flags & SH_FLAG_PAIR_MAJOR) && rows_lds_ > 160u * 1024u)
+    const RowsPlan plan = rows_plan(max_n, max_l);
+    if (!(cfg_.flags & SH_FLAG_PAIR_MAJOR) && std::max(plan.lds, plan.lds_combine) > 160u * 1024u)
         return SH_E_CAPACITY;
     for (uint32_t it = 0; it < cfg_.gn_iterations; ++it)
     {
@@ -430,13 +485,14 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st)
         }
         else
         {
-            SH_HIP(launch_rows(K_, fp16, sds, map(kMapRow, ns), n, map_total_[kMapRow], rows_lds_, st));
+            SH_HIP(launch_rows(K_, fp16, sds, map(kMapSeg, ns), n, map_total_[kMapSeg], map(kMapRow, ns),
+                               map_total_[kMapRow], plan, st));
         }
         SH_HIP(launch_dinv(sds, map(kMapNode, ns), n, map_total_[kMapNode], st));
         if (use_coop)
         {
-            SH_HIP(launch_pcg_coop(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(cfg_.pcg_iterations), prs, pdq,
-                                   st));
+            SH_HIP(launch_pcg_coop(sds, map(kMapCoop, ns), n, map_total_[kMapCoop], int(cfg_.pcg_iterations), prs,
+                                   pdq, st));
         }
         else
         {
@@ -507,6 +563,7 @@ int Engine::debug_view(uint32_t slot, const char *name, const void **ptr, size_t
         {"blk_col", d.blk_col, N + 2 * U},
         {"hg", d.jr, P * kHG},
         {"dense_cnt", d.dense_cnt, N},
+        {"seg_ptr", d.seg_ptr, N + 1},
         {"reg", d.reg, E * kReg},
         {"blk", d.blk, (N + 2 * U) * 36},
         {"dinv", d.dinv, N * 36},

@@ -41,6 +41,8 @@ private:
         bool indexed = false;
         bool solved = false;
         uint32_t max_row_len = 0;
+        void *part_buf = nullptr; // segment partials, grown on demand
+        size_t part_bytes = 0;
     };
     enum
     {
@@ -51,12 +53,15 @@ private:
         kMapPcg,
         kMapNode,
         kMapRow,
+        kMapCoop,
+        kMapSeg,
         kNumMaps
     };
 
     int alloc(void **p, size_t bytes);
     void release();
     int build_index(uint32_t slot, hipStream_t st);
+    int size_partials(Slot &sl);
     void build_maps(uint32_t ns);
     int upload(uint32_t ns, hipStream_t st);
     int enqueue_gn(uint32_t ns, hipStream_t st);
@@ -73,9 +78,8 @@ private:
     void *ws_flags_ = nullptr, *ws_pos_ = nullptr, *ws_k16a_ = nullptr, *ws_k16b_ = nullptr, *ws_v32a_ = nullptr;
     void *ws_k32a_ = nullptr, *ws_k32b_ = nullptr, *ws_p32a_ = nullptr, *ws_p32b_ = nullptr;
     void *ws_head_ = nullptr, *ws_hpos_ = nullptr, *ws_meta_ = nullptr;
-    void *ws_bitmap_ = nullptr, *ws_rowlen_ = nullptr;
+    void *ws_bitmap_ = nullptr, *ws_rowlen_ = nullptr, *ws_segcnt_ = nullptr;
     int coop_max_groups_ = 0;
-    size_t rows_lds_ = 0;
     void *sort_temp_ = nullptr;
     size_t sort_temp_bytes_ = 0;
     uint32_t *h_meta_ = nullptr;

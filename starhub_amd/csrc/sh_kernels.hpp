@@ -47,11 +47,23 @@ hipError_t launch_skin(uint32_t K, const float4 *surfels, uint32_t S, const floa
 hipError_t launch_node_graph(uint32_t K, const float4 *nodes, uint32_t N, float r2, uint16_t *node_knn,
                              float *node_spatial, uint16_t *edges, float *node_connect, hipStream_t st);
 
-hipError_t launch_row_bitmap(const SolveDesc &d, uint32_t *bitmap, uint32_t *row_len, uint32_t *meta, hipStream_t st);
+hipError_t launch_row_bitmap(const SolveDesc &d, uint32_t *bitmap, uint32_t *row_len, uint32_t *seg_cnt,
+                             uint32_t *meta, hipStream_t st);
 hipError_t launch_row_cols(const SolveDesc &d, const uint32_t *bitmap, uint32_t *meta, hipStream_t st);
-size_t rows_lds_bytes(uint32_t max_nodes, uint32_t max_row_len);
-hipError_t launch_rows(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
-                       size_t lds, hipStream_t st);
+// Dense terms per segment of a node row (one work-group of the segment kernel).
+constexpr uint32_t kSegTerms = 256;
+struct RowsPlan
+{
+    uint32_t chunk = 0;      // dense terms staged per round of the segment kernel
+    size_t lds = 0;          // its dynamic LDS bytes per work-group
+    size_t lds_combine = 0;  // the combine kernel's
+};
+RowsPlan rows_plan(uint32_t max_nodes, uint32_t max_row_len);
+// Segment kernel over seg_map (one work-group per segment), then the combine
+// kernel over row_map (one work-group per node row).
+hipError_t launch_rows(int K, bool fp16, const SolveDesc *sds, const uint32_t *seg_map, int n_solve,
+                       uint32_t n_seg_wg, const uint32_t *row_map, uint32_t n_row_wg, const RowsPlan &plan,
+                       hipStream_t st);
 hipError_t launch_pcg_coop(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
                            double *part_rs, double *part_dq, hipStream_t st);
 int pcg_coop_max_groups();
@@ -60,7 +72,8 @@ constexpr uint32_t kBitmapWords = 128;
 // work-group geometry of the GN-phase kernels
 constexpr uint32_t kThreads = 256;
 constexpr uint32_t kWavesPerWg = 4;
-constexpr uint32_t kPcgNodesPerWg = 32;
+constexpr uint32_t kPcgNodesPerWg = 32;  // split PCG kernels
+constexpr uint32_t kCoopNodesPerWg = 4;  // persistent PCG kernel (one wave per node)
 
 // rocPRIM wrappers (sh_sort.hip); temp == nullptr queries the temp size
 hipError_t sort_pairs_u16_u32(void *temp, size_t &temp_bytes, const uint16_t *kin, uint16_t *kout,

@@ -13,6 +13,8 @@
 #include "sh_desc.hpp"
 #include "sh_kernels.hpp"
 
+#include <algorithm>
+
 namespace sh
 {
 __device__ __forceinline__ int find_solve_r(const uint32_t *__restrict__ begin, int n, uint32_t wg)
@@ -48,10 +50,12 @@ constexpr int kMaxWords = kPairBase / 32; // bitmap words for N <= 4096
 
 // One work-group per node: OR the nodes of every term of node i into an LDS
 // bitmap, drop i itself, count.  Also counts i's dense terms (they precede the
-// ARAP and node-translation terms in the sorted list).
+// ARAP and node-translation terms in the sorted list) and the segments of
+// kSegTerms dense terms they split into.
 template <int K>
 __global__ __launch_bounds__(kRowThreads) void k_row_bitmap(SolveDesc d, uint32_t *__restrict__ bitmap,
                                                             uint32_t *__restrict__ row_len,
+                                                            uint32_t *__restrict__ seg_cnt,
                                                             uint32_t *__restrict__ meta)
 {
     __shared__ uint32_t bits[kMaxWords];
@@ -105,6 +109,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row_bitmap(SolveDesc d, uint32_
     {
         row_len[i] = cnt[1];
         d.dense_cnt[i] = cnt[0];
+        seg_cnt[i] = (cnt[0] + kSegTerms - 1) / kSegTerms;
         atomicMax(&meta[0], cnt[1]);
     }
 }
@@ -165,95 +170,231 @@ __global__ void k_row_meta(SolveDesc d, uint32_t *__restrict__ meta)
         meta[1] = d.row_ptr[d.N] / 2;
 }
 
-// One work-group (4 waves) per node row.  Dense terms of the row are split in
-// four contiguous chunks, one per wave; per term the wave's 64 lanes cover
-// (target slot k, packed H entry e) so each lane adds w_i w_k H_e into a
-// private per-wave LDS block accumulator (distinct addresses within a term,
-// program order across terms).  Wave 0 then adds the ARAP and node-translation
-// terms.  The four wave accumulators are summed in a fixed order.
+// (a, b) of packed upper-triangle entry e of a symmetric 6x6 block.
+__device__ __forceinline__ void sym_pair(int e, int &a, int &b)
+{
+    a = (e >= 6) + (e >= 11) + (e >= 15) + (e >= 18) + (e >= 20);
+    b = e - (a * 6 - a * (a - 1) / 2) + a;
+}
+
+constexpr int kRowTasks = 4;  // (column, 4-entry group) tasks per thread and pass
+constexpr int kHs = 24;       // staged H stride (21 entries padded to 6 float4)
+
+// Row i of segment `seg`: the last row whose first segment is <= seg.
+__device__ __forceinline__ uint32_t seg_row(const uint32_t *__restrict__ seg_ptr, uint32_t n, uint32_t seg)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1)
+    {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (seg_ptr[mid] <= seg)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Dense part of the normal equations, one work-group per segment of up to
+// kSegTerms dense terms of one node row i (the terms of a node are very
+// uneven: a node facing the camera carries thousands, one behind it none).
+// The segment is processed in chunks: thread j stages term j -- its H record
+// (21 entries of sum_c lam_c^2 J_c^T J_c) into LDS and, for every slot k, the
+// coefficient w_i w_k into a dense chunk x L1 LDS matrix at the column of
+// node k (later slots overwrite earlier ones: the reference's "last matching
+// slot", dense_image_jacobian.cuh:17-23,92-104).  Every thread then owns
+// (column, group of 4 entries) accumulators in registers and runs through the
+// chunk in term order: partial block(i, c) = sum_t C[t][c] H_t, a small dense
+// product, no atomics.  The partial rhs sum_t w_i^2 g_t is summed by the
+// staging threads and reduced in a fixed tree.
 template <int K, class JT>
-__global__ __launch_bounds__(kRowThreads) void k_rows(const SolveDesc *__restrict__ sds,
-                                                      const uint32_t *__restrict__ wg_begin, int n_solve)
+__global__ __launch_bounds__(kRowThreads) void k_rows_seg(const SolveDesc *__restrict__ sds,
+                                                          const uint32_t *__restrict__ wg_begin, int n_solve,
+                                                          uint32_t chunk)
 {
     extern __shared__ float smem[];
-    constexpr int LPT = kWave / K;            // lanes per target slot
-    constexpr int EPL = (21 + LPT - 1) / LPT; // H entries per lane
+    const int si = find_solve_r(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const uint32_t seg = blockIdx.x - wg_begin[si];
+    if (seg >= uint32_t(d.n_seg))
+        return; // uniform over the group
+    const uint32_t tid = threadIdx.x;
+    const uint32_t i = seg_row(d.seg_ptr, uint32_t(d.N), seg);
+    const uint32_t beg = d.n2t_off[i] + (seg - d.seg_ptr[i]) * kSegTerms;
+    const uint32_t dend = min(beg + kSegTerms, d.n2t_off[i] + d.dense_cnt[i]);
+    const uint32_t base = d.row_ptr[i] + i;
+    const uint32_t L1 = d.row_ptr[i + 1] - d.row_ptr[i] + 1; // blocks in the row, diagonal first
+    // LDS: Hs[chunk][24] | Cm[chunk][L1] | colmap u16[N] | red[4][8]
+    float *Hs = smem;
+    float *Cm = Hs + size_t(chunk) * kHs;
+    uint16_t *colmap = reinterpret_cast<uint16_t *>(Cm + size_t(chunk) * L1);
+    float *red = reinterpret_cast<float *>(colmap + ((d.N + 1) & ~1));
+    for (uint32_t t = tid; t < L1; t += kRowThreads)
+        colmap[d.blk_col[base + t]] = uint16_t(t);
+    for (uint32_t t = tid; t < size_t(chunk) * L1; t += kRowThreads)
+        Cm[t] = 0.f;
+    __syncthreads();
+
+    float *out = d.part + size_t(seg) * d.part_stride;
+    const uint32_t n_tasks = L1 * 6;
+    float gsum[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (uint32_t tp = 0; tp < n_tasks; tp += kRowThreads * kRowTasks)
+    {
+        float acc[kRowTasks][4];
+#pragma unroll
+        for (int s = 0; s < kRowTasks; ++s)
+            acc[s][0] = acc[s][1] = acc[s][2] = acc[s][3] = 0.f;
+        for (uint32_t c0 = beg; c0 < dend; c0 += chunk)
+        {
+            const uint32_t n = min(chunk, dend - c0);
+            // ---- stage: thread j < n takes term c0 + j
+            uint16_t cols[K];
+            const bool stager = tid < n;
+            if (stager)
+            {
+                const uint32_t t = d.n2t_terms[c0 + tid];
+                uint16_t kn[K];
+                float w[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                {
+                    kn[k] = d.pknn[size_t(t) * K + k];
+                    w[k] = d.pw[size_t(t) * K + k];
+                }
+                float hg[kHG];
+                load_hg<JT>(d.jr, t, hg);
+                float wi = 0.f;
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                    if (kn[k] == i)
+                        wi = w[k];
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                {
+                    cols[k] = colmap[kn[k]];
+                    Cm[tid * L1 + cols[k]] = wi * w[k];
+                }
+                float4 *h4 = reinterpret_cast<float4 *>(Hs + tid * kHs);
+                h4[0] = make_float4(hg[0], hg[1], hg[2], hg[3]);
+                h4[1] = make_float4(hg[4], hg[5], hg[6], hg[7]);
+                h4[2] = make_float4(hg[8], hg[9], hg[10], hg[11]);
+                h4[3] = make_float4(hg[12], hg[13], hg[14], hg[15]);
+                h4[4] = make_float4(hg[16], hg[17], hg[18], hg[19]);
+                h4[5] = make_float4(hg[20], 0.f, 0.f, 0.f);
+                if (tp == 0)
+                {
+                    const float wi2 = wi * wi;
+#pragma unroll
+                    for (int q = 0; q < 6; ++q)
+                        gsum[q] += wi2 * hg[kHGg + q];
+                }
+            }
+            __syncthreads();
+            // ---- accumulate: thread owns tasks (column c, entries 4g..4g+3)
+#pragma unroll
+            for (int s = 0; s < kRowTasks; ++s)
+            {
+                const uint32_t task = tp + tid + s * kRowThreads;
+                if (task < n_tasks)
+                {
+                    const uint32_t c = task / 6, g = task % 6;
+                    const float *cp = Cm + c;
+                    const float4 *hp = reinterpret_cast<const float4 *>(Hs) + g;
+                    float a0 = acc[s][0], a1 = acc[s][1], a2 = acc[s][2], a3 = acc[s][3];
+                    for (uint32_t j = 0; j < n; ++j)
+                    {
+                        const float cv = cp[j * L1];
+                        const float4 h = hp[j * (kHs / 4)];
+                        const bool on = cv != 0.f;
+                        a0 = on ? fmaf(cv, h.x, a0) : a0;
+                        a1 = on ? fmaf(cv, h.y, a1) : a1;
+                        a2 = on ? fmaf(cv, h.z, a2) : a2;
+                        a3 = on ? fmaf(cv, h.w, a3) : a3;
+                    }
+                    acc[s][0] = a0;
+                    acc[s][1] = a1;
+                    acc[s][2] = a2;
+                    acc[s][3] = a3;
+                }
+            }
+            __syncthreads();
+            // ---- clear this term's coefficients for the next chunk
+            if (stager)
+            {
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                    Cm[tid * L1 + cols[k]] = 0.f;
+            }
+        }
+        // ---- the pass's partial blocks, packed upper triangles
+#pragma unroll
+        for (int s = 0; s < kRowTasks; ++s)
+        {
+            const uint32_t task = tp + tid + s * kRowThreads;
+            if (task < n_tasks)
+            {
+                const uint32_t c = task / 6, g = task % 6;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (int(g) * 4 + q < 21)
+                        out[8 + c * 21 + g * 4 + q] = acc[s][q];
+            }
+        }
+    }
+    // ---- partial rhs: fixed-order reduction of the staging threads' sums
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+    {
+        float v = gsum[q];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+            v += __shfl_xor(v, o, 64);
+        gsum[q] = v;
+    }
+    if ((tid & 63) == 0)
+    {
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            red[(tid >> 6) * 8 + q] = gsum[q];
+    }
+    __syncthreads();
+    if (tid < 6)
+        out[tid] = (red[tid] + red[8 + tid]) + (red[16 + tid] + red[24 + tid]);
+}
+
+// One work-group per node row: the row's segment partials summed in segment
+// order, plus the ARAP terms (node_graph_reg_jacobian.cuh:166-358) and the
+// node-translation term (node_motion_jacobian.cuh:7-35) of the row, added by
+// wave 0 in list order, plus the damping on the diagonal block.  Writes the
+// full 6x6 blocks of the row and rhs_i = -(sum of the partial g).
+__global__ __launch_bounds__(kRowThreads) void k_rows_combine(const SolveDesc *__restrict__ sds,
+                                                              const uint32_t *__restrict__ wg_begin, int n_solve)
+{
+    extern __shared__ float smem[];
     const int si = find_solve_r(wg_begin, n_solve, blockIdx.x);
     const SolveDesc &d = sds[si];
     const uint32_t i = blockIdx.x - wg_begin[si];
     if (i >= uint32_t(d.N))
         return; // uniform over the group
+    const uint32_t tid = threadIdx.x;
     const uint32_t base = d.row_ptr[i] + i;
-    const uint32_t L1 = d.row_ptr[i + 1] - d.row_ptr[i] + 1; // blocks in the row, diagonal included
-    uint16_t *colmap = reinterpret_cast<uint16_t *>(smem);
-    float *acc = smem + (d.N + 1) / 2;       // 4 x L1 x 21
-    float *regblk = acc + 4 * L1 * 21;       // L1 x 36
-    float *rhs_sh = regblk + L1 * 36;        // 4 x 8
-    for (uint32_t t = threadIdx.x; t < L1; t += blockDim.x)
+    const uint32_t L1 = d.row_ptr[i + 1] - d.row_ptr[i] + 1;
+    // LDS: regblk[L1][36] | colmap u16[N] | red[8]
+    float *regblk = smem;
+    uint16_t *colmap = reinterpret_cast<uint16_t *>(regblk + size_t(L1) * 36);
+    float *red = reinterpret_cast<float *>(colmap + ((d.N + 1) & ~1));
+    for (uint32_t t = tid; t < L1; t += kRowThreads)
         colmap[d.blk_col[base + t]] = uint16_t(t);
-    for (uint32_t t = threadIdx.x; t < 4 * L1 * 21 + L1 * 36; t += blockDim.x)
-        acc[t] = 0.f;
+    for (uint32_t t = tid; t < L1 * 36; t += kRowThreads)
+        regblk[t] = 0.f;
     __syncthreads();
 
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int k = lane / LPT, e0 = lane % LPT;
-    float *my = acc + wave * L1 * 21;
-    float g = 0.f; // lane q < 6 accumulates rhs component q
-    const uint32_t beg = d.n2t_off[i];
-    const uint32_t dend = beg + d.dense_cnt[i];
+    const uint32_t dend = d.n2t_off[i] + d.dense_cnt[i];
     const uint32_t end = d.n2t_off[i + 1];
-    const uint32_t nd = dend - beg;
-    const uint32_t chunk = (nd + 3) / 4;
-    const uint32_t cb = beg + min(nd, chunk * wave), ce = beg + min(nd, chunk * (wave + 1));
-    for (uint32_t it = cb; it < ce; ++it)
+    if (tid < 64)
     {
-        const uint32_t t = d.n2t_terms[it];
-        uint16_t kn[K];
-        float w[K];
-#pragma unroll
-        for (int kk = 0; kk < K; ++kk)
-        {
-            kn[kk] = d.pknn[size_t(t) * K + kk];
-            w[kk] = d.pw[size_t(t) * K + kk];
-        }
-        float wi = 0.f;
-        uint32_t mk = 0;
-        float mw = 0.f;
-#pragma unroll
-        for (int kk = 0; kk < K; ++kk)
-        {
-            if (kn[kk] == i)
-                wi = w[kk]; // last matching slot (dense_image_jacobian.cuh:17-23)
-            if (kk == k)
-            {
-                mk = kn[kk];
-                mw = w[kk];
-            }
-        }
-        bool last = true;
-#pragma unroll
-        for (int kk = 0; kk < K; ++kk)
-            if (kk > k && kn[kk] == mk)
-                last = false;
-        if (last)
-        {
-            const uint32_t c = colmap[mk];
-            const float coef = wi * mw;
-            float *dst = my + c * 21;
-#pragma unroll
-            for (int m = 0; m < EPL; ++m)
-            {
-                const int e = e0 + LPT * m;
-                if (e < 21)
-                    dst[e] += coef * ld_hg<JT>(d.jr, t, e);
-            }
-        }
-        if (lane < 6)
-            g += (wi * wi) * ld_hg<JT>(d.jr, t, kHGg + lane);
-    }
-    if (wave == 0)
-    {
-        // ARAP terms (node_graph_reg_jacobian.cuh:166-358), sequential in list order
+        const int lane = int(tid);
+        float gA = 0.f;
         const uint32_t P = d.P, E = d.E;
         for (uint32_t it = dend; it < end; ++it)
         {
@@ -271,10 +412,10 @@ __global__ __launch_bounds__(kRowThreads) void k_rows(const SolveDesc *__restric
                 const bool is_x = (ex == i);
                 const uint32_t other = is_x ? ey : ex;
                 const float ws = d.lam2_reg * (cw * cw);
-                // entries of the unweighted ARAP rows: x side [-[a]x, I], y side [[b]x, -I]
+                // unweighted ARAP rows: x side [-[a]x, I], y side [[b]x, -I]
                 // (node_graph_reg_jacobian.cuh:250-275)
-                const f3 pm = is_x ? a : b;  // this node's warped point
-                const f3 po = is_x ? b : a;  // the other node's
+                const f3 pm = is_x ? a : b; // this node's warped point
+                const f3 po = is_x ? b : a; // the other node's
                 const float sm = is_x ? 1.f : -1.f;
                 if (lane < 36)
                 {
@@ -289,8 +430,7 @@ __global__ __launch_bounds__(kRowThreads) void k_rows(const SolveDesc *__restric
                         vd += ga * gb;
                         vo += ga * ob;
                     }
-                    if (aa <= bb)
-                        my[sym_index(aa, bb)] += ws * vd;
+                    regblk[lane] += ws * vd;
                     if (other != i)
                         regblk[colmap[other] * 36 + lane] += ws * vo;
                 }
@@ -301,57 +441,63 @@ __global__ __launch_bounds__(kRowThreads) void k_rows(const SolveDesc *__restric
 #pragma unroll
                     for (int ch = 0; ch < 3; ++ch)
                         v += res[ch] * (sm * reg_elem(pm, ch, lane));
-                    g += ws * v;
+                    gA += ws * v;
                 }
             }
             else
             {
-                // node translation (node_motion_jacobian.cuh:7-35)
                 const uint32_t m = t - P - E;
                 const float4 pred = d.node_pred ? d.node_pred[m] : make_float4(0.f, 0.f, 0.f, 0.f);
                 const float4 tt = d.ttrans[m];
                 const float c2 = pred.w * pred.w;
                 if (lane >= 3 && lane < 6)
                 {
-                    my[sym_index(lane, lane)] += d.lam2_nt * c2;
+                    regblk[lane * 7] += d.lam2_nt * c2;
                     const float tv = (lane == 3) ? tt.x - pred.x : (lane == 4) ? tt.y - pred.y : tt.z - pred.z;
-                    g += d.lam2_nt * tv * c2;
+                    gA += d.lam2_nt * tv * c2;
                 }
             }
         }
+        if (lane < 6)
+            red[lane] = gA;
     }
-    if (lane < 6)
-        rhs_sh[wave * 8 + lane] = g;
     __syncthreads();
+    const uint32_t s0 = d.seg_ptr[i], s1 = d.seg_ptr[i + 1];
+    const float *part = d.part;
+    const uint32_t stride = d.part_stride;
     const size_t out0 = size_t(base) * 36;
-    for (uint32_t t = threadIdx.x; t < L1 * 36; t += blockDim.x)
+    for (uint32_t t = tid; t < L1 * 21; t += kRowThreads)
     {
-        const uint32_t c = t / 36, ab = t % 36;
-        const int a = int(ab / 6), b = int(ab % 6);
-        const int e = sym_index(a, b);
-        float v = acc[c * 21 + e];
-        v += acc[L1 * 21 + c * 21 + e];
-        v += acc[2 * L1 * 21 + c * 21 + e];
-        v += acc[3 * L1 * 21 + c * 21 + e];
-        v += regblk[c * 36 + ab];
+        const uint32_t c = t / 21;
+        const int e = int(t % 21);
+        float v = 0.f;
+        for (uint32_t sg = s0; sg < s1; ++sg)
+            v += part[size_t(sg) * stride + 8 + t];
+        int a, b;
+        sym_pair(e, a, b);
+        float u = v + regblk[c * 36 + a * 6 + b];
         if (c == 0 && a == b)
-            v += d.damping;
-        d.blk[out0 + t] = v;
+            u += d.damping;
+        d.blk[out0 + size_t(c) * 36 + a * 6 + b] = u;
+        if (a != b)
+            d.blk[out0 + size_t(c) * 36 + b * 6 + a] = v + regblk[c * 36 + b * 6 + a];
     }
-    if (threadIdx.x < 6)
+    if (tid < 6)
     {
-        const float s = rhs_sh[threadIdx.x] + rhs_sh[8 + threadIdx.x] + rhs_sh[16 + threadIdx.x] +
-                        rhs_sh[24 + threadIdx.x];
-        d.rhs[size_t(i) * 6 + threadIdx.x] = -s;
+        float g = 0.f;
+        for (uint32_t sg = s0; sg < s1; ++sg)
+            g += part[size_t(sg) * stride + tid];
+        d.rhs[size_t(i) * 6 + tid] = -(g + red[tid]);
     }
 }
 
-hipError_t launch_row_bitmap(const SolveDesc &d, uint32_t *bitmap, uint32_t *row_len, uint32_t *meta, hipStream_t st)
+hipError_t launch_row_bitmap(const SolveDesc &d, uint32_t *bitmap, uint32_t *row_len, uint32_t *seg_cnt,
+                             uint32_t *meta, hipStream_t st)
 {
     if (d.K == 8)
-        hipLaunchKernelGGL(k_row_bitmap<8>, dim3(d.N), dim3(kRowThreads), 0, st, d, bitmap, row_len, meta);
+        hipLaunchKernelGGL(k_row_bitmap<8>, dim3(d.N), dim3(kRowThreads), 0, st, d, bitmap, row_len, seg_cnt, meta);
     else
-        hipLaunchKernelGGL(k_row_bitmap<4>, dim3(d.N), dim3(kRowThreads), 0, st, d, bitmap, row_len, meta);
+        hipLaunchKernelGGL(k_row_bitmap<4>, dim3(d.N), dim3(kRowThreads), 0, st, d, bitmap, row_len, seg_cnt, meta);
     return hipGetLastError();
 }
 
@@ -362,10 +508,22 @@ hipError_t launch_row_cols(const SolveDesc &d, const uint32_t *bitmap, uint32_t 
     return hipGetLastError();
 }
 
-size_t rows_lds_bytes(uint32_t max_nodes, uint32_t max_row_len)
+RowsPlan rows_plan(uint32_t max_nodes, uint32_t max_row_len)
 {
+    // Segment kernel: the chunk of terms staged per round is as large as a
+    // 40 KB budget allows (several work-groups share a CU), 32..kSegTerms.
+    RowsPlan p;
     const size_t L1 = size_t(max_row_len) + 1;
-    return sizeof(float) * ((max_nodes + 1) / 2 + 4 * L1 * 21 + L1 * 36 + 32);
+    const size_t colmap = sizeof(uint16_t) * ((max_nodes + 1) & ~1u);
+    const size_t fixed = colmap + sizeof(float) * 32;
+    const size_t per_term = sizeof(float) * (kHs + L1);
+    const size_t budget = 40 * 1024;
+    size_t chunk = budget > fixed ? (budget - fixed) / per_term : 0;
+    chunk = std::min<size_t>(kSegTerms, std::max<size_t>(32, chunk & ~size_t(31)));
+    p.chunk = uint32_t(chunk);
+    p.lds = fixed + chunk * per_term;
+    p.lds_combine = sizeof(float) * (L1 * 36 + 8) + colmap;
+    return p;
 }
 
 template <int K, class JT>
@@ -373,33 +531,54 @@ static hipError_t allow_lds(size_t lds)
 {
     if (lds <= 65536)
         return hipSuccess;
-    return hipFuncSetAttribute(reinterpret_cast<const void *>(k_rows<K, JT>),
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(k_rows_seg<K, JT>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
 }
 
-hipError_t launch_rows(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
-                       size_t lds, hipStream_t st)
+hipError_t launch_rows(int K, bool fp16, const SolveDesc *sds, const uint32_t *seg_map, int n_solve,
+                       uint32_t n_seg_wg, const uint32_t *row_map, uint32_t n_row_wg, const RowsPlan &plan,
+                       hipStream_t st)
 {
-    if (n_wg == 0)
+    const size_t lds = plan.lds;
+    const uint32_t chunk = plan.chunk;
+    if (n_seg_wg > 0)
+    {
+        hipError_t e = (K == 8) ? (fp16 ? allow_lds<8, __half>(lds) : allow_lds<8, float>(lds))
+                                : (fp16 ? allow_lds<4, __half>(lds) : allow_lds<4, float>(lds));
+        if (e != hipSuccess)
+            return e;
+        if (K == 8)
+        {
+            if (fp16)
+                hipLaunchKernelGGL((k_rows_seg<8, __half>), dim3(n_seg_wg), dim3(kRowThreads), lds, st, sds, seg_map,
+                                   n_solve, chunk);
+            else
+                hipLaunchKernelGGL((k_rows_seg<8, float>), dim3(n_seg_wg), dim3(kRowThreads), lds, st, sds, seg_map,
+                                   n_solve, chunk);
+        }
+        else
+        {
+            if (fp16)
+                hipLaunchKernelGGL((k_rows_seg<4, __half>), dim3(n_seg_wg), dim3(kRowThreads), lds, st, sds, seg_map,
+                                   n_solve, chunk);
+            else
+                hipLaunchKernelGGL((k_rows_seg<4, float>), dim3(n_seg_wg), dim3(kRowThreads), lds, st, sds, seg_map,
+                                   n_solve, chunk);
+        }
+        if (hipError_t e2 = hipGetLastError(); e2 != hipSuccess)
+            return e2;
+    }
+    if (n_row_wg == 0)
         return hipSuccess;
-    hipError_t e = (K == 8) ? (fp16 ? allow_lds<8, __half>(lds) : allow_lds<8, float>(lds))
-                            : (fp16 ? allow_lds<4, __half>(lds) : allow_lds<4, float>(lds));
-    if (e != hipSuccess)
-        return e;
-    if (K == 8)
+    if (plan.lds_combine > 65536)
     {
-        if (fp16)
-            hipLaunchKernelGGL((k_rows<8, __half>), dim3(n_wg), dim3(kRowThreads), lds, st, sds, wmap, n_solve);
-        else
-            hipLaunchKernelGGL((k_rows<8, float>), dim3(n_wg), dim3(kRowThreads), lds, st, sds, wmap, n_solve);
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_rows_combine),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(plan.lds_combine));
+        if (e != hipSuccess)
+            return e;
     }
-    else
-    {
-        if (fp16)
-            hipLaunchKernelGGL((k_rows<4, __half>), dim3(n_wg), dim3(kRowThreads), lds, st, sds, wmap, n_solve);
-        else
-            hipLaunchKernelGGL((k_rows<4, float>), dim3(n_wg), dim3(kRowThreads), lds, st, sds, wmap, n_solve);
-    }
+    hipLaunchKernelGGL(k_rows_combine, dim3(n_row_wg), dim3(kRowThreads), plan.lds_combine, st, sds, row_map,
+                       n_solve);
     return hipGetLastError();
 }
 } // namespace sh
