@@ -60,7 +60,10 @@ typedef enum sh_solver_flags
      * traversal, JtJMaterializer.cu:66-175).  Implies SH_FLAG_REFERENCE_INDEX. */
     SH_FLAG_PAIR_MAJOR = 2u,
     /* Run the PCG as two launches per iteration instead of one persistent cooperative launch. */
-    SH_FLAG_NO_PERSISTENT_PCG = 4u
+    SH_FLAG_NO_PERSISTENT_PCG = 4u,
+    /* Keep the persistent PCG but stream the blocks from memory each iteration instead of
+     * holding them in registers (the variant used when a solve's rows do not fit). */
+    SH_FLAG_STREAMING_PCG = 8u
 } sh_solver_flags;
 
 /* Dual quaternion as stored by the reference: two float4 (x, y, z, w),

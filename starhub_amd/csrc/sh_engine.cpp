@@ -89,7 +89,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         return SH_E_ARG;
     if (c.pcg_iterations == 0 || c.pcg_iterations > 1000 || c.gn_iterations > 1000)
         return SH_E_ARG;
-    if (c.stream_precision > SH_STREAM_FP16 || c.spmv_mode > SH_SPMV_AUTO || (c.flags & ~7u))
+    if (c.stream_precision > SH_STREAM_FP16 || c.spmv_mode > SH_SPMV_AUTO || (c.flags & ~15u))
         return SH_E_ARG;
     cfg_ = c;
     if (cfg_.flags & SH_FLAG_PAIR_MAJOR)
@@ -181,12 +181,15 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
     // batch-wide launch state
     A(d_descs_, size_t(max_slots) * sizeof(SolveDesc));
     A(d_wmaps_, size_t(kNumMaps) * (max_slots + 1) * 4);
-    const size_t pcg_wg = size_t(max_slots) * div_up(ncap_, std::min(kPcgNodesPerWg, kCoopNodesPerWg));
+    const size_t pcg_wg = size_t(max_slots) * div_up(ncap_, kPcgNodesPerWg);
     A(d_part_rs_, pcg_wg * 8);
     A(d_part_dq_, pcg_wg * 8);
+    A(d_slots_, size_t(max_slots) * kPcgMaxGroups * sizeof(PcgSlot));
+    A(d_flags_, 64);
+    SH_HIP(hipMemset(d_slots_, 0, size_t(max_slots) * kPcgMaxGroups * sizeof(PcgSlot)));
+    SH_HIP(hipMemset(d_flags_, 0, 64));
 #undef A
     SH_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_meta_), 64, hipHostMallocDefault));
-    coop_max_groups_ = pcg_coop_max_groups();
     stage_bytes_ = size_t(max_slots) * sizeof(SolveDesc) + size_t(kNumMaps) * (max_slots + 1) * 4;
     SH_HIP(hipHostMalloc(&h_stage_, stage_bytes_, hipHostMallocDefault));
     return SH_OK;
@@ -389,9 +392,25 @@ int Engine::size_partials(Slot &sl)
     return SH_OK;
 }
 
+// Groups of the register-resident PCG for one solve: rows split evenly, at
+// most kPcgResRows rows and kPcgResBlocks blocks (rows x longest row) each;
+// 0 when that takes more than kPcgMaxGroups groups.
+static uint32_t resident_groups(int n, uint32_t max_row_len)
+{
+    const uint32_t rows = std::min(kPcgResRows, kPcgResBlocks / (max_row_len + 1));
+    if (rows == 0 || n <= 0)
+        return n <= 0 && rows > 0 ? 1u : 0u;
+    const uint32_t g = div_up(uint32_t(n), rows);
+    return g <= kPcgMaxGroups ? g : 0u;
+}
+
 void Engine::build_maps(uint32_t ns)
 {
     maps_.assign(size_t(kNumMaps) * (ns + 1), 0u);
+    coop_resident_ = !(cfg_.flags & SH_FLAG_STREAMING_PCG);
+    for (uint32_t s = 0; s < ns; ++s)
+        if (resident_groups(slots_[s].d.N, slots_[s].max_row_len) == 0)
+            coop_resident_ = false;
     for (int m = 0; m < kNumMaps; ++m)
     {
         uint32_t acc = 0;
@@ -423,7 +442,9 @@ void Engine::build_maps(uint32_t ns)
                 wg = uint32_t(d.N);
                 break;
             case kMapCoop:
-                wg = div_up(uint32_t(d.N), kCoopNodesPerWg);
+                wg = coop_resident_ ? resident_groups(d.N, slots_[s].max_row_len)
+                                    : std::min(kPcgMaxGroups, std::max(1u, div_up(uint32_t(d.N) + 2u * uint32_t(d.U),
+                                                                                  kPcgBlocksPerGroup)));
                 break;
             case kMapSeg:
                 wg = uint32_t(d.n_seg);
@@ -462,14 +483,31 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st)
     const bool fp16 = fp16_;
     SH_HIP(launch_identity(sds, map(kMapNode, ns), n, map_total_[kMapNode], st));
     double *prs = static_cast<double *>(d_part_rs_), *pdq = static_cast<double *>(d_part_dq_);
-    const bool use_coop = !(cfg_.flags & SH_FLAG_NO_PERSISTENT_PCG) && coop_max_groups_ > 0 &&
-                          map_total_[kMapCoop] <= uint32_t(coop_max_groups_);
-    uint32_t max_n = 0, max_l = 0;
+    uint32_t max_n = 0, max_l = 0, tile = 1;
+    const uint32_t *coop_begin = maps_.data() + size_t(kMapCoop) * (ns + 1);
     for (uint32_t s = 0; s < ns; ++s)
     {
-        max_n = std::max(max_n, uint32_t(slots_[s].d.N));
+        const SolveDesc &d = slots_[s].d;
+        max_n = std::max(max_n, uint32_t(d.N));
         max_l = std::max(max_l, slots_[s].max_row_len);
+        // blocks of one persistent-PCG group: its share plus one whole row
+        const uint32_t G = coop_begin[s + 1] - coop_begin[s];
+        const uint32_t nb = uint32_t(d.N) + 2u * uint32_t(d.U);
+        tile = std::max(tile, div_up(nb, G) + slots_[s].max_row_len + 2u);
     }
+    const size_t coop_lds = size_t(tile) * 6 * sizeof(float);
+    if (coop_lds != coop_lds_)
+    {
+        coop_lds_ = coop_lds;
+        coop_max_groups_ = coop_lds <= 160u * 1024u ? pcg_coop_max_groups(coop_lds) : 0;
+    }
+    if (res_max_groups_ < 0)
+        res_max_groups_ = pcg_resident_max_groups();
+    const bool resident = coop_resident_ && map_total_[kMapCoop] <= uint32_t(res_max_groups_);
+    const bool use_coop = !(cfg_.flags & SH_FLAG_NO_PERSISTENT_PCG) &&
+                          (resident || (!coop_resident_ && coop_max_groups_ > 0 &&
+                                        map_total_[kMapCoop] <= uint32_t(coop_max_groups_)));
+    uint32_t *flags = static_cast<uint32_t *>(d_flags_);
     const RowsPlan plan = rows_plan(max_n, max_l);
     if (!(cfg_.flags & SH_FLAG_PAIR_MAJOR) && std::max(plan.lds, plan.lds_combine) > 160u * 1024u)
         return SH_E_CAPACITY;
@@ -491,8 +529,13 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st)
         SH_HIP(launch_dinv(sds, map(kMapNode, ns), n, map_total_[kMapNode], st));
         if (use_coop)
         {
-            SH_HIP(launch_pcg_coop(sds, map(kMapCoop, ns), n, map_total_[kMapCoop], int(cfg_.pcg_iterations), prs,
-                                   pdq, st));
+            if (resident)
+                SH_HIP(launch_pcg_resident(sds, map(kMapCoop, ns), n, map_total_[kMapCoop],
+                                           int(cfg_.pcg_iterations), static_cast<PcgSlot *>(d_slots_), flags + 0,
+                                           flags + 1, st));
+            else
+                SH_HIP(launch_pcg_coop(sds, map(kMapCoop, ns), n, map_total_[kMapCoop], int(cfg_.pcg_iterations),
+                                       static_cast<PcgSlot *>(d_slots_), flags + 0, flags + 1, tile, st));
         }
         else
         {
@@ -524,7 +567,10 @@ int Engine::solve(uint32_t ns, hipStream_t st)
         return rc;
     if ((rc = enqueue_gn(ns, st)) != SH_OK)
         return rc;
+    SH_HIP(hipMemcpyAsync(h_meta_ + 8, static_cast<uint32_t *>(d_flags_) + 1, 4, hipMemcpyDeviceToHost, st));
     SH_HIP(hipStreamSynchronize(st));
+    if (h_meta_[8] != 0)
+        return SH_E_HIP; // a PCG barrier wait ran out (never expected)
     for (uint32_t s = 0; s < ns; ++s)
         slots_[s].solved = true;
     return SH_OK;

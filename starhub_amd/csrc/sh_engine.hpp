@@ -80,6 +80,11 @@ private:
     void *ws_head_ = nullptr, *ws_hpos_ = nullptr, *ws_meta_ = nullptr;
     void *ws_bitmap_ = nullptr, *ws_rowlen_ = nullptr, *ws_segcnt_ = nullptr;
     int coop_max_groups_ = 0;
+    size_t coop_lds_ = 0;
+    bool coop_resident_ = false; // every solve of the launch fits the register-resident PCG
+    int res_max_groups_ = -1;
+    void *d_slots_ = nullptr; // persistent-PCG barrier slots
+    void *d_flags_ = nullptr; // [0] PCG epoch base, [1] PCG barrier error
     void *sort_temp_ = nullptr;
     size_t sort_temp_bytes_ = 0;
     uint32_t *h_meta_ = nullptr;

@@ -64,16 +64,33 @@ RowsPlan rows_plan(uint32_t max_nodes, uint32_t max_row_len);
 hipError_t launch_rows(int K, bool fp16, const SolveDesc *sds, const uint32_t *seg_map, int n_solve,
                        uint32_t n_seg_wg, const uint32_t *row_map, uint32_t n_row_wg, const RowsPlan &plan,
                        hipStream_t st);
+// Barrier slot of one persistent-PCG work-group.
+struct PcgSlot
+{
+    double v[2];
+    uint32_t epoch;
+    uint32_t pad[3];
+};
+// Blocks of the normal matrix per persistent-PCG work-group (sets the
+// number of groups per solve, at most kPcgMaxGroups).
+constexpr uint32_t kPcgBlocksPerGroup = 1024;
+constexpr uint32_t kPcgMaxGroups = 64;
 hipError_t launch_pcg_coop(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
-                           double *part_rs, double *part_dq, hipStream_t st);
-int pcg_coop_max_groups();
+                           PcgSlot *slots, uint32_t *epoch_base, uint32_t *err, uint32_t tile, hipStream_t st);
+int pcg_coop_max_groups(size_t lds);
+// Register-resident persistent PCG: groups of at most kPcgResRows rows and
+// kPcgResBlocks blocks, split evenly by rows.
+constexpr uint32_t kPcgResRows = 85;
+constexpr uint32_t kPcgResBlocks = 1536;
+hipError_t launch_pcg_resident(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
+                               PcgSlot *slots, uint32_t *epoch_base, uint32_t *err, hipStream_t st);
+int pcg_resident_max_groups();
 constexpr uint32_t kBitmapWords = 128;
 
 // work-group geometry of the GN-phase kernels
 constexpr uint32_t kThreads = 256;
 constexpr uint32_t kWavesPerWg = 4;
 constexpr uint32_t kPcgNodesPerWg = 32;  // split PCG kernels
-constexpr uint32_t kCoopNodesPerWg = 4;  // persistent PCG kernel (one wave per node)
 
 // rocPRIM wrappers (sh_sort.hip); temp == nullptr queries the temp size
 hipError_t sort_pairs_u16_u32(void *temp, size_t &temp_bytes, const uint16_t *kin, uint16_t *kout,

@@ -2,15 +2,14 @@
 // BlockPCG<6>::SolveNoConvergeCheck (libs/star/pcg_solver/BlockPCG.cuh:325-483)
 // in one cooperative launch covering every solve of a batch.
 //
-// Per iteration two grid barriers replace the reference's 7 launches
-// (SpMV, 3 x cublasSdot, two update kernels, swap): phase A forms
-// d_k = s + beta d_{k-1} and q = A d_k for the group's own node rows (the
+// Per iteration two barriers among the groups of one solve replace the
+// reference's 7 launches (SpMV, 3 x cublasSdot, two update kernels, swap):
+// phase A forms d_k = s + beta d_{k-1} and q = A d_k for the group's rows (the
 // neighbours' d_k are recomputed from s and d_{k-1}, which no group writes
-// during phase A) and publishes the partial d.q; phase B updates x, r, s and
-// publishes the partial r.s.  x and r of a row stay in registers; FP64 for
-// every dot product, alpha and beta; fixed-order reductions.  All blocks of a
-// row are read at once (one lane per block), so a row's SpMV waits on one
-// round of loads.
+// during phase A) and sums d.q; phase B updates x, r, s and sums r.s.  FP64
+// for every dot product, alpha and beta; fixed-order reductions.  The
+// barriers are per solve (a batch's solves never wait for each other) and
+// carry the dot products, so a barrier costs one publish and one poll.
 #include <hip/hip_cooperative_groups.h>
 
 #include "sh_common.hpp"
@@ -21,13 +20,9 @@ namespace sh
 {
 namespace cg = cooperative_groups;
 
-// One wave per node row, 4 rows per work-group of 256 threads.  Lane b takes
-// block b of the row (and b + 64, ...): its 36 entries and the neighbour's 6
-// direction entries arrive in one round of loads, and the 6 row sums are
-// reduced across the wave in a fixed xor tree.  Lanes 0..5 own the row's
-// x, r, s and d entries (a = lane).
-constexpr int kCoopNodes = int(kCoopNodesPerWg);
-constexpr int kCoopThreads = 64 * kCoopNodes;
+constexpr int kCoopThreads = 512;
+constexpr int kCoopWaves = kCoopThreads / 64;
+constexpr unsigned kSpinCap = 1u << 22;
 
 __device__ __forceinline__ int find_solve_p(const uint32_t *__restrict__ begin, int n, uint32_t wg)
 {
@@ -43,6 +38,26 @@ __device__ __forceinline__ int find_solve_p(const uint32_t *__restrict__ begin, 
     return lo;
 }
 
+// First row whose first block is at or after block index t (rows are whole).
+__device__ __forceinline__ uint32_t row_at_block(const uint32_t *__restrict__ row_ptr, uint32_t n, uint32_t t)
+{
+    uint32_t lo = 0, hi = n; // answer in [0, n]
+    while (lo < hi)
+    {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (row_ptr[mid] + mid >= t)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ float next_dir(double beta, float dold, float s)
+{
+    return float(fma(beta, double(dold), double(s)));
+}
+
 // Sum over the work-group, fixed order: wave trees, then the waves in order.
 __device__ __forceinline__ double group_sum(double v, double *sh)
 {
@@ -54,186 +69,450 @@ __device__ __forceinline__ double group_sum(double v, double *sh)
     __syncthreads();
     double r = sh[0];
 #pragma unroll
-    for (int w = 1; w < kCoopNodes; ++w)
+    for (int w = 1; w < kCoopWaves; ++w)
         r += sh[w];
     __syncthreads();
     return r;
 }
 
-// Sum of the work-group partials of one solve, fixed order.
-__device__ __forceinline__ double partial_total(const double *__restrict__ part, uint32_t b0, uint32_t b1, double *sh)
+// Barrier of the G groups of one solve, fused with the sum of their partials.
+// Group g publishes (value, epoch) in its own slot; wave 0 of every group waits
+// for all G epochs and sums the G values in slot order, so every group gets
+// the same total, bit for bit.  Values alternate between two entries by
+// epoch parity: a group can only overwrite an entry two barriers later, after
+// every group has read it.  A wait that exceeds the spin cap sets *err and
+// ends every later wait at once, so each wave always exits.
+__device__ double solve_barrier_sum(double part, PcgSlot *__restrict__ slots, uint32_t b0, uint32_t G, uint32_t ep,
+                                    uint32_t *__restrict__ err, double *sh)
 {
-    double v = 0.0;
+    if (threadIdx.x == 0)
+    {
+        PcgSlot &me = slots[blockIdx.x];
+        me.v[ep & 1u] = part;
+        __hip_atomic_store(&me.epoch, ep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
     if (threadIdx.x < 64)
     {
-        for (uint32_t i = b0 + threadIdx.x; i < b1; i += 64)
-            v += part[i];
+        double v = 0.0;
+        for (uint32_t i = threadIdx.x; i < G; i += 64)
+        {
+            PcgSlot &o = slots[b0 + i];
+            unsigned spins = 0;
+            while (__hip_atomic_load(&o.epoch, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < ep)
+            {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kSpinCap || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                {
+                    __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+            v += __hip_atomic_load(&o.v[ep & 1u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1)
             v += __shfl_xor(v, o, 64);
         if (threadIdx.x == 0)
-            sh[kCoopNodes] = v;
+            sh[kCoopWaves] = v;
     }
     __syncthreads();
-    const double r = sh[kCoopNodes];
+    const double r = sh[kCoopWaves];
     __syncthreads();
     return r;
 }
 
-__device__ __forceinline__ float next_dir(double beta, float dold, float s)
-{
-    return float(fma(beta, double(dold), double(s)));
-}
-
+// Persistent PCG.  Each solve is split into G groups over contiguous node
+// rows holding about equal numbers of blocks.  Phase A: every thread takes
+// blocks of the group (2 in flight), forms d_k of the block's column
+// (s + beta d_{k-1}, from the previous iteration's vectors) and the 6 row
+// sums A_b d_k into LDS; one thread per row then adds its blocks in order,
+// writes d_k and q, and the groups sum d.q through the solve barrier.
+// Phase B: one thread per row updates x, r, s = M^-1 r and the groups sum r.s.
 __global__ __launch_bounds__(kCoopThreads) void k_pcg_coop(const SolveDesc *__restrict__ sds,
                                                            const uint32_t *__restrict__ wg_begin, int n_solve,
-                                                           int iters, double *__restrict__ part_rs,
-                                                           double *__restrict__ part_dq)
+                                                           int iters, PcgSlot *__restrict__ slots,
+                                                           uint32_t *__restrict__ epoch_base,
+                                                           uint32_t *__restrict__ err, uint32_t tile)
 {
-    cg::grid_group grid = cg::this_grid();
-    __shared__ double sh[kCoopNodes + 1];
+    extern __shared__ float qblk[]; // tile x 6
+    __shared__ double sh[kCoopWaves + 1];
     const int si = find_solve_p(wg_begin, n_solve, blockIdx.x);
     const SolveDesc &d = sds[si];
-    const uint32_t b0 = wg_begin[si], b1 = wg_begin[si + 1];
-    const int lane = threadIdx.x & 63;
-    const uint32_t node = (blockIdx.x - b0) * kCoopNodes + (threadIdx.x >> 6);
-    const bool live = node < uint32_t(d.N); // uniform over the wave
-    const bool owner = live && lane < 6;
-    const int a = owner ? lane : 0;
-    const size_t row = size_t(node) * 6 + a;
-    double M[6];
-#pragma unroll
-    for (int c = 0; c < 6; ++c)
-        M[c] = owner ? d.dinv[size_t(node) * 36 + 6 * a + c] : 0.0;
-    uint32_t rb0 = 0, rb1 = 0;
-    if (live)
-    {
-        rb0 = d.row_ptr[node] + node;
-        rb1 = d.row_ptr[node + 1] + node + 1;
-    }
+    const uint32_t b0 = wg_begin[si], G = wg_begin[si + 1] - b0, g = blockIdx.x - b0;
+    const uint32_t N = uint32_t(d.N);
+    const uint32_t NB = d.row_ptr[N] + N;
+    const uint32_t r0 = row_at_block(d.row_ptr, N, uint32_t((uint64_t(g) * NB + G - 1) / G));
+    const uint32_t r1 = (g + 1 == G) ? N : row_at_block(d.row_ptr, N, uint32_t((uint64_t(g + 1) * NB + G - 1) / G));
+    const uint32_t B0 = d.row_ptr[r0] + r0, B1 = d.row_ptr[r1] + r1;
+    const bool fits = (B1 - B0) <= tile; // true by construction of the partition
+    const uint32_t ep0 = __hip_atomic_load(epoch_base, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t ep = ep0;
+    const uint32_t tid = threadIdx.x;
+    if (!fits && tid == 0)
+        __hip_atomic_store(err, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
     // r = b, s = M^-1 b, x = 0, d_{-1} = 0  (blockPCGZeroInitKernel, BlockPCG.cuh:68-116)
-    float r = owner ? d.rhs[row] : 0.f;
-    float x = 0.f, dk = 0.f, s = 0.f;
+    double part = 0.0;
+    for (uint32_t r = r0 + tid; r < r1; r += kCoopThreads)
     {
-        double acc = 0.0;
+        float rv[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c)
-            acc += M[c] * double(__shfl(r, c, 64));
-        if (owner)
+        for (int a = 0; a < 6; ++a)
+            rv[a] = d.rhs[size_t(r) * 6 + a];
+        const double *M = d.dinv + size_t(r) * 36;
+#pragma unroll
+        for (int a = 0; a < 6; ++a)
         {
-            s = float(acc);
-            d.s[row] = s;
-            d.dv[0][row] = 0.f;
+            double acc = 0.0;
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                acc += M[6 * a + c] * double(rv[c]);
+            const float sv = float(acc);
+            d.r[size_t(r) * 6 + a] = rv[a];
+            d.s[size_t(r) * 6 + a] = sv;
+            d.x[size_t(r) * 6 + a] = 0.f;
+            d.dv[0][size_t(r) * 6 + a] = 0.f;
+            part += double(rv[a]) * double(sv);
         }
     }
-    double tot = group_sum(owner ? double(r) * double(s) : 0.0, sh);
-    if (threadIdx.x == 0)
-        part_rs[blockIdx.x] = tot;
-    grid.sync();
+    double delta = solve_barrier_sum(group_sum(part, sh), slots, b0, G, ++ep, err, sh);
 
     double delta_prev = 0.0;
     for (int k = 0; k < iters; ++k)
     {
         // ---- phase A: d_k = s + beta d_{k-1}, q = A d_k, partial d.q
-        const double delta = partial_total(part_rs, b0, b1, sh);
         const double beta = (k > 0 && delta_prev != 0.0) ? delta / delta_prev : 0.0;
-        if (threadIdx.x == 0 && blockIdx.x == b0)
+        if (tid == 0 && g == 0)
             d.delta[k] = delta;
         const float *__restrict__ dold = d.dv[k & 1];
         float *__restrict__ dnew = d.dv[(k + 1) & 1];
-        const float dn = next_dir(beta, dk, s);
-        if (owner)
-            dnew[row] = dn;
-        float qa[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        for (uint32_t bb = rb0 + lane; bb < rb1; bb += 64)
+        if (fits)
         {
-            const uint32_t col = d.blk_col[bb];
-            const float4 *A = reinterpret_cast<const float4 *>(d.blk + size_t(bb) * 36);
-            const float2 *po = reinterpret_cast<const float2 *>(dold + size_t(col) * 6);
-            const float2 *ps = reinterpret_cast<const float2 *>(d.s + size_t(col) * 6);
+            for (uint32_t bb0 = B0 + tid; bb0 < B1; bb0 += 2 * kCoopThreads)
+            {
+                uint32_t bbs[2];
+                float4 A[2][9];
+                float2 po[2][3], ps[2][3];
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                {
+                    bbs[j] = bb0 + j * kCoopThreads;
+                    const uint32_t bb = bbs[j] < B1 ? bbs[j] : B0;
+                    const uint32_t col = d.blk_col[bb];
+                    const float4 *Ab = reinterpret_cast<const float4 *>(d.blk + size_t(bb) * 36);
+#pragma unroll
+                    for (int v = 0; v < 9; ++v)
+                        A[j][v] = Ab[v];
+                    const float2 *pov = reinterpret_cast<const float2 *>(dold + size_t(col) * 6);
+                    const float2 *psv = reinterpret_cast<const float2 *>(d.s + size_t(col) * 6);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                    {
+                        po[j][c] = pov[c];
+                        ps[j][c] = psv[c];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                {
+                    if (bbs[j] >= B1)
+                        continue;
+                    float p[6];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                    {
+                        p[2 * c] = next_dir(beta, po[j][c].x, ps[j][c].x);
+                        p[2 * c + 1] = next_dir(beta, po[j][c].y, ps[j][c].y);
+                    }
+                    const float *Af = reinterpret_cast<const float *>(A[j]);
+                    float *out = qblk + size_t(bbs[j] - B0) * 6;
+#pragma unroll
+                    for (int a = 0; a < 6; ++a)
+                    {
+                        float q = 0.f;
+#pragma unroll
+                        for (int c = 0; c < 6; ++c)
+                            q = fmaf(Af[6 * a + c], p[c], q);
+                        out[a] = q;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        part = 0.0;
+        for (uint32_t r = r0 + tid; r < r1; r += kCoopThreads)
+        {
+            const uint32_t e0 = d.row_ptr[r] + r - B0, e1 = d.row_ptr[r + 1] + r + 1 - B0;
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+            {
+                const size_t i6 = size_t(r) * 6 + a;
+                const float dn = next_dir(beta, dold[i6], d.s[i6]);
+                float q = 0.f;
+                if (fits)
+                    for (uint32_t e = e0; e < e1; ++e)
+                        q += qblk[e * 6 + a];
+                dnew[i6] = dn;
+                d.q[i6] = q;
+                part += double(dn) * double(q);
+            }
+        }
+        const double dq = solve_barrier_sum(group_sum(part, sh), slots, b0, G, ++ep, err, sh);
+
+        // ---- phase B: alpha, x, r, s = M^-1 r, partial r.s
+        const double alpha = (dq != 0.0) ? delta / dq : 0.0;
+        part = 0.0;
+        for (uint32_t r = r0 + tid; r < r1; r += kCoopThreads)
+        {
+            float rv[6];
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+            {
+                const size_t i6 = size_t(r) * 6 + a;
+                const float dn = dnew[i6], q = d.q[i6];
+                d.x[i6] = float(fma(alpha, double(dn), double(d.x[i6])));
+                rv[a] = float(fma(-alpha, double(q), double(d.r[i6])));
+                d.r[i6] = rv[a];
+            }
+            const double *M = d.dinv + size_t(r) * 36;
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+            {
+                double acc = 0.0;
+#pragma unroll
+                for (int c = 0; c < 6; ++c)
+                    acc += M[6 * a + c] * double(rv[c]);
+                const float sv = float(acc);
+                d.s[size_t(r) * 6 + a] = sv;
+                part += double(rv[a]) * double(sv);
+            }
+        }
+        delta_prev = delta;
+        delta = solve_barrier_sum(group_sum(part, sh), slots, b0, G, ++ep, err, sh);
+    }
+    // next launch starts above every epoch used here; every group has read
+    // epoch_base before its first barrier, and this group has passed them all
+    if (blockIdx.x == 0 && tid == 0)
+        __hip_atomic_store(epoch_base, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Resident variant: the solve's rows are split evenly by count into G groups
+// of at most kResRows rows and kResThreads * kResBlocksPerThread blocks (the
+// host picks G so both hold).  For the whole launch every thread keeps its
+// blocks (column and 36 entries) and one owned vector entry (row, a): its
+// row of M^-1 and x, r, s, d, q in registers.  Per iteration the only global
+// traffic left is the neighbours' s and d_{k-1} of each block, the owned s
+// and d_k going out, and the two solve barriers.
+constexpr int kResThreads = 512;
+constexpr int kResBlocksPerThread = 3;
+constexpr int kResWaves = kResThreads / 64;
+
+__global__ __launch_bounds__(kResThreads) void k_pcg_resident(const SolveDesc *__restrict__ sds,
+                                                              const uint32_t *__restrict__ wg_begin, int n_solve,
+                                                              int iters, PcgSlot *__restrict__ slots,
+                                                              uint32_t *__restrict__ epoch_base,
+                                                              uint32_t *__restrict__ err)
+{
+    __shared__ float qblk[kResThreads * kResBlocksPerThread * 6];
+    __shared__ float rsh[kResThreads];
+    __shared__ double sh[kCoopWaves + 1];
+    const int si = find_solve_p(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const uint32_t b0 = wg_begin[si], G = wg_begin[si + 1] - b0, g = blockIdx.x - b0;
+    const uint32_t N = uint32_t(d.N);
+    const uint32_t r0 = uint32_t(uint64_t(g) * N / G), r1 = uint32_t(uint64_t(g + 1) * N / G);
+    const uint32_t B0 = d.row_ptr[r0] + r0, B1 = d.row_ptr[r1] + r1;
+    const uint32_t tid = threadIdx.x;
+    const bool fits = (r1 - r0) * 6 <= uint32_t(kResThreads) && (B1 - B0) <= uint32_t(kResThreads * kResBlocksPerThread);
+    if (!fits && tid == 0)
+        __hip_atomic_store(err, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t ep0 = __hip_atomic_load(epoch_base, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t ep = ep0;
+
+    // resident blocks
+    uint32_t col[kResBlocksPerThread];
+    float A[kResBlocksPerThread][36];
+#pragma unroll
+    for (int j = 0; j < kResBlocksPerThread; ++j)
+    {
+        const uint32_t bb = B0 + tid + j * kResThreads;
+        const bool on = fits && bb < B1;
+        col[j] = on ? d.blk_col[bb] : 0xffffffffu;
+        const float4 *Ab = reinterpret_cast<const float4 *>(d.blk + size_t(on ? bb : 0) * 36);
+#pragma unroll
+        for (int v = 0; v < 9; ++v)
+        {
+            const float4 f = on ? Ab[v] : make_float4(0.f, 0.f, 0.f, 0.f);
+            A[j][4 * v] = f.x;
+            A[j][4 * v + 1] = f.y;
+            A[j][4 * v + 2] = f.z;
+            A[j][4 * v + 3] = f.w;
+        }
+    }
+    // owned entry
+    const bool own = fits && tid < (r1 - r0) * 6;
+    const uint32_t row = r0 + tid / 6;
+    const int a = int(tid % 6);
+    const size_t i6 = size_t(row) * 6 + a;
+    const uint32_t lrow = (tid / 6) * 6; // first rsh entry of the row
+    double M[6];
+    uint32_t e0 = 0, e1 = 0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+        M[c] = own ? d.dinv[size_t(row) * 36 + 6 * a + c] : 0.0;
+    if (own)
+    {
+        e0 = d.row_ptr[row] + row - B0;
+        e1 = d.row_ptr[row + 1] + row + 1 - B0;
+    }
+
+    // r = b, s = M^-1 b, x = 0, d_{-1} = 0  (blockPCGZeroInitKernel, BlockPCG.cuh:68-116)
+    float r = own ? d.rhs[i6] : 0.f;
+    float x = 0.f, dk = 0.f, s = 0.f;
+    rsh[tid] = r;
+    __syncthreads();
+    if (own)
+    {
+        double acc = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            acc += M[c] * double(rsh[lrow + c]);
+        s = float(acc);
+        d.s[i6] = s;
+        d.dv[0][i6] = 0.f;
+    }
+    double delta = solve_barrier_sum(group_sum(own ? double(r) * double(s) : 0.0, sh), slots, b0, G, ++ep, err, sh);
+
+    double delta_prev = 0.0;
+    for (int k = 0; k < iters; ++k)
+    {
+        // ---- phase A: d_k = s + beta d_{k-1}, q = A d_k, partial d.q
+        const double beta = (k > 0 && delta_prev != 0.0) ? delta / delta_prev : 0.0;
+        if (tid == 0 && g == 0)
+            d.delta[k] = delta;
+        const float *__restrict__ dold = d.dv[k & 1];
+        float *__restrict__ dnew = d.dv[(k + 1) & 1];
+        float2 po[kResBlocksPerThread][3], ps[kResBlocksPerThread][3];
+#pragma unroll
+        for (int j = 0; j < kResBlocksPerThread; ++j)
+        {
+            const size_t c6 = size_t(col[j] != 0xffffffffu ? col[j] : 0) * 6;
+            const float2 *pov = reinterpret_cast<const float2 *>(dold + c6);
+            const float2 *psv = reinterpret_cast<const float2 *>(d.s + c6);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+            {
+                po[j][c] = pov[c];
+                ps[j][c] = psv[c];
+            }
+        }
+        const float dn = next_dir(beta, dk, s);
+        if (own)
+            dnew[i6] = dn;
+#pragma unroll
+        for (int j = 0; j < kResBlocksPerThread; ++j)
+        {
+            if (col[j] == 0xffffffffu)
+                continue;
             float p[6];
 #pragma unroll
             for (int c = 0; c < 3; ++c)
             {
-                const float2 ov = po[c], sv = ps[c];
-                p[2 * c] = next_dir(beta, ov.x, sv.x);
-                p[2 * c + 1] = next_dir(beta, ov.y, sv.y);
+                p[2 * c] = next_dir(beta, po[j][c].x, ps[j][c].x);
+                p[2 * c + 1] = next_dir(beta, po[j][c].y, ps[j][c].y);
             }
-            float Av[36];
+            float *out = qblk + size_t(tid + j * kResThreads) * 6;
 #pragma unroll
-            for (int v = 0; v < 9; ++v)
+            for (int aa = 0; aa < 6; ++aa)
             {
-                const float4 f = A[v];
-                Av[4 * v] = f.x;
-                Av[4 * v + 1] = f.y;
-                Av[4 * v + 2] = f.z;
-                Av[4 * v + 3] = f.w;
-            }
-#pragma unroll
-            for (int r6 = 0; r6 < 6; ++r6)
+                float q = 0.f;
 #pragma unroll
                 for (int c = 0; c < 6; ++c)
-                    qa[r6] = fmaf(Av[6 * r6 + c], p[c], qa[r6]);
+                    q = fmaf(A[j][6 * aa + c], p[c], q);
+                out[aa] = q;
+            }
         }
+        __syncthreads();
         float q = 0.f;
-#pragma unroll
-        for (int r6 = 0; r6 < 6; ++r6)
-        {
-            const float v = wave_sum(qa[r6]);
-            q = (a == r6) ? v : q;
-        }
-        tot = group_sum(owner ? double(dn) * double(q) : 0.0, sh);
-        if (threadIdx.x == 0)
-            part_dq[blockIdx.x] = tot;
-        grid.sync();
+        for (uint32_t e = e0; e < e1; ++e)
+            q += qblk[e * 6 + a];
+        const double dq =
+            solve_barrier_sum(group_sum(own ? double(dn) * double(q) : 0.0, sh), slots, b0, G, ++ep, err, sh);
 
         // ---- phase B: alpha, x, r, s = M^-1 r, partial r.s
-        const double dq = partial_total(part_dq, b0, b1, sh);
         const double alpha = (dq != 0.0) ? delta / dq : 0.0;
-        if (owner)
+        x = float(fma(alpha, double(dn), double(x)));
+        r = float(fma(-alpha, double(q), double(r)));
+        rsh[tid] = r;
+        __syncthreads();
+        if (own)
         {
-            x = float(fma(alpha, double(dn), double(x)));
-            r = float(fma(-alpha, double(q), double(r)));
-        }
-        double acc = 0.0;
+            double acc = 0.0;
 #pragma unroll
-        for (int c = 0; c < 6; ++c)
-            acc += M[c] * double(__shfl(r, c, 64));
-        if (owner)
-        {
+            for (int c = 0; c < 6; ++c)
+                acc += M[c] * double(rsh[lrow + c]);
             s = float(acc);
-            d.s[row] = s;
+            d.s[i6] = s;
         }
-        tot = group_sum(owner ? double(r) * double(s) : 0.0, sh);
-        if (threadIdx.x == 0)
-            part_rs[blockIdx.x] = tot;
         dk = dn;
         delta_prev = delta;
-        grid.sync();
+        delta = solve_barrier_sum(group_sum(own ? double(r) * double(s) : 0.0, sh), slots, b0, G, ++ep, err, sh);
     }
-    if (owner)
+    if (own)
     {
-        d.x[row] = x;
-        d.r[row] = r;
+        d.x[i6] = x;
+        d.r[i6] = r;
     }
+    if (blockIdx.x == 0 && tid == 0)
+        __hip_atomic_store(epoch_base, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+hipError_t launch_pcg_resident(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
+                               PcgSlot *slots, uint32_t *epoch_base, uint32_t *err, hipStream_t st)
+{
+    void *args[] = {(void *)&sds, (void *)&wmap, (void *)&n_solve, (void *)&iters,
+                    (void *)&slots, (void *)&epoch_base, (void *)&err};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_pcg_resident), dim3(n_wg), dim3(kResThreads),
+                                      args, 0, st);
+}
+
+int pcg_resident_max_groups()
+{
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_pcg_resident),
+                                                     kResThreads, 0) != hipSuccess)
+        return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    return per_cu * cus;
 }
 
 hipError_t launch_pcg_coop(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
-                           double *part_rs, double *part_dq, hipStream_t st)
+                           PcgSlot *slots, uint32_t *epoch_base, uint32_t *err, uint32_t tile, hipStream_t st)
 {
-    void *args[] = {(void *)&sds, (void *)&wmap, (void *)&n_solve, (void *)&iters, (void *)&part_rs, (void *)&part_dq};
+    void *args[] = {(void *)&sds, (void *)&wmap, (void *)&n_solve, (void *)&iters,
+                    (void *)&slots, (void *)&epoch_base, (void *)&err, (void *)&tile};
+    const size_t lds = size_t(tile) * 6 * sizeof(float);
+    if (lds > 65536)
+    {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_pcg_coop),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        if (e != hipSuccess)
+            return e;
+    }
     return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_pcg_coop), dim3(n_wg), dim3(kCoopThreads), args,
-                                      0, st);
+                                      lds, st);
 }
 
-int pcg_coop_max_groups()
+int pcg_coop_max_groups(size_t lds)
 {
     int dev = 0, per_cu = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess)
         return 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_pcg_coop),
-                                                     kCoopThreads, 0) != hipSuccess)
+                                                     kCoopThreads, lds) != hipSuccess)
         return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 0;

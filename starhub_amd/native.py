@@ -19,7 +19,7 @@ _LIB = None
 SH_OK = 0
 SH_STREAM_FP32, SH_STREAM_FP16 = 0, 1
 SH_SPMV_MATERIALIZED, SH_SPMV_MATRIX_FREE, SH_SPMV_AUTO = 0, 1, 2
-SH_FLAG_REFERENCE_INDEX, SH_FLAG_PAIR_MAJOR, SH_FLAG_NO_PERSISTENT_PCG = 1, 2, 4
+SH_FLAG_REFERENCE_INDEX, SH_FLAG_PAIR_MAJOR, SH_FLAG_NO_PERSISTENT_PCG, SH_FLAG_STREAMING_PCG = 1, 2, 4, 8
 
 _VP = ctypes.c_void_p
 
