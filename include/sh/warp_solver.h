@@ -63,7 +63,13 @@ typedef enum sh_solver_flags
     SH_FLAG_NO_PERSISTENT_PCG = 4u,
     /* Keep the persistent PCG but stream the blocks from memory each iteration instead of
      * holding them in registers (the variant used when a solve's rows do not fit). */
-    SH_FLAG_STREAMING_PCG = 8u
+    SH_FLAG_STREAMING_PCG = 8u,
+    /* Record a HIP event before every GN-phase kernel launch on the solve's stream and
+     * keep the summed time per phase (sh_solver_phase_times). */
+    SH_FLAG_PHASE_TIMING = 16u,
+    /* Materialise row-major from node-row segments (each per-pixel record read once per
+     * node of the pixel) instead of from pixel tiles (each record read once). */
+    SH_FLAG_ROW_SEGMENTS = 32u
 } sh_solver_flags;
 
 /* Dual quaternion as stored by the reference: two float4 (x, y, z, w),
@@ -231,6 +237,17 @@ int sh_batch_set_inputs(sh_batch *batch, uint32_t slot, const sh_camera *camera,
 /* Solves slots [0, num_solves) together: one launch per GN phase covers all. */
 int sh_batch_solve(sh_batch *batch, uint32_t num_solves, sh_stream_t stream);
 int sh_batch_node_se3(const sh_batch *batch, uint32_t slot, const sh_dual_quat **device_view, uint32_t *num_nodes);
+
+/* ------------------------------------------------------- measurement hook */
+/* No counterpart in the reference (which times nothing on the device).  With
+ * SH_FLAG_PHASE_TIMING in the config: the event-to-event time (ms) and the
+ * launch count of every GN phase since the last reset, for phases
+ * 0 .. num_phases-1 (names from sh_phase_name); reset != 0 zeroes them after
+ * the copy.  Phases: dense_terms, reg_terms, rows_seg, rows_combine,
+ * diag_rhs, offdiag, dinv, pcg, update, tiles, tiles_combine. */
+const char *sh_phase_name(uint32_t phase);
+int sh_solver_phase_times(sh_solver *solver, double *ms, uint32_t *calls, uint32_t num_phases, uint32_t reset);
+int sh_batch_phase_times(sh_batch *batch, double *ms, uint32_t *calls, uint32_t num_phases, uint32_t reset);
 
 #ifdef __cplusplus
 }

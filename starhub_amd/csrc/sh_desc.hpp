@@ -59,6 +59,16 @@ struct SolveDesc
     int n_seg;              // dense-term segments in all
     uint32_t part_stride;   // floats per segment partial: 8 (rhs) + 21 per block of the longest row
     float *part;            // n_seg * part_stride partial block rows
+    // pixel tiles (kTilePix compacted pixels each) and their slot partials
+    int n_tiles;
+    uint32_t *tile_cbase;    // n_tiles+1: first contribution of each tile
+    uint32_t *tile_sbase;    // n_tiles+1: first slot of each tile
+    uint16_t *tc_contrib;    // contributions, per slot in pixel order: pixel | ka << 7 | kb << 10
+    uint32_t *tc_slot_first; // S+1: first contribution of each slot
+    uint32_t *list_off;      // N+2U+1: first entry of each block in list_items
+    uint32_t *list_items;    // S: slot ids sorted by block (stable: tile order)
+    uint32_t *blk_mirror;    // N+2U: the upper block holding each block's dense values
+    float *tpart;            // S * 28 slot partials (21 H, 6 g, pad)
     // GN iteration state
     dq8 *dq[2];       // double buffer (SolverIterationData.h)
     void *jr;         // P*24 residual/Jacobian records, FP32 or FP16

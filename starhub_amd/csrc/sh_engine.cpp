@@ -52,6 +52,10 @@ void Engine::release()
             (void)hipFree(sl.part_buf);
         sl.part_buf = nullptr;
         sl.part_bytes = 0;
+        if (sl.tile_buf)
+            (void)hipFree(sl.tile_buf);
+        sl.tile_buf = nullptr;
+        sl.tile_bytes = 0;
     }
     if (h_meta_)
         (void)hipHostFree(h_meta_);
@@ -62,6 +66,10 @@ void Engine::release()
     if (graph_exec_)
         (void)hipGraphExecDestroy(graph_exec_);
     graph_exec_ = nullptr;
+    for (Mark &m : marks_)
+        (void)hipEventDestroy(m.ev);
+    marks_.clear();
+    n_marks_ = 0;
 }
 
 int Engine::alloc(void **p, size_t bytes)
@@ -89,7 +97,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         return SH_E_ARG;
     if (c.pcg_iterations == 0 || c.pcg_iterations > 1000 || c.gn_iterations > 1000)
         return SH_E_ARG;
-    if (c.stream_precision > SH_STREAM_FP16 || c.spmv_mode > SH_SPMV_AUTO || (c.flags & ~15u))
+    if (c.stream_precision > SH_STREAM_FP16 || c.spmv_mode > SH_SPMV_AUTO || (c.flags & ~63u))
         return SH_E_ARG;
     cfg_ = c;
     if (cfg_.flags & SH_FLAG_PAIR_MAJOR)
@@ -137,6 +145,10 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         A(d.dq[1], size_t(ncap_) * 32);
         A(d.dense_cnt, size_t(ncap_) * 4);
         A(d.seg_ptr, size_t(ncap_ + 1) * 4);
+        A(d.tile_cbase, size_t(div_up(pcap_, kTilePix) + 1) * 4);
+        A(d.tile_sbase, size_t(div_up(pcap_, kTilePix) + 1) * 4);
+        A(d.list_off, (size_t(ncap_) + 2 * size_t(ucap_) + 1) * 4);
+        A(d.blk_mirror, (size_t(ncap_) + 2 * size_t(ucap_)) * 4);
         A(d.jr, size_t(pcap_) * kHG * (fp16_ ? 2 : 4));
         A(d.reg, size_t(ecap_) * kReg * 4);
         A(d.ttrans, size_t(ncap_) * 16);
@@ -169,6 +181,15 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
     A(ws_bitmap_, size_t(ncap_) * kBitmapWords * 4);
     A(ws_rowlen_, size_t(ncap_ + 1) * 4);
     A(ws_segcnt_, size_t(ncap_ + 1) * 4);
+    tiles_cap_ = div_up(pcap_, kTilePix);
+    {
+        const size_t tstride = size_t(kTilePix) * (K_ * (K_ + 1) / 2);
+        A(ws_tcontrib_, size_t(tiles_cap_) * tstride * 2);
+        A(ws_tslot_block_, size_t(tiles_cap_) * tstride * 4);
+        A(ws_tslot_first_, size_t(tiles_cap_) * tstride * 2);
+        A(ws_tnslot_, size_t(tiles_cap_ + 1) * 4);
+        A(ws_tncontrib_, size_t(tiles_cap_ + 1) * 4);
+    }
     {
         size_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
         SH_HIP(sort_pairs_u16_u32(nullptr, b0, nullptr, nullptr, nullptr, nullptr, n2t_cap_, 16, nullptr));
@@ -310,6 +331,8 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
         SH_HIP(exclusive_scan_u32(sort_temp_, tb, rowlen, d.row_ptr, uint32_t(d.N) + 1, st));
         SH_HIP(launch_row_cols(d, bitmap, meta, st));
         SH_HIP(hipMemcpyAsync(h_meta_ + 4, meta, 8, hipMemcpyDeviceToHost, st));
+        // the row structure is complete here (blk_col bounded by blk_cap)
+        SH_HIP(launch_tile_build_async(d, st));
         SH_HIP(hipStreamSynchronize(st));
         const uint32_t U = h_meta_[5];
         if (U > ucap_)
@@ -320,6 +343,8 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
         d.n_seg = int(h_meta_[6]);
         int rc = size_partials(sl);
         if (rc != SH_OK)
+            return rc;
+        if ((rc = build_tiles(sl, st)) != SH_OK)
             return rc;
         sl.indexed = true;
         return SH_OK;
@@ -361,7 +386,98 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
         SH_HIP(sort_pairs_u32_u32(sort_temp_, tb, k32a, k32b, p32a, p32b, 2 * U, 24, st));
     SH_HIP(launch_sym_scatter(d, k32b, p32b, st));
     SH_HIP(launch_row_ptr(d, k32b, st));
+    SH_HIP(launch_tile_build_async(d, st));
+    SH_HIP(hipStreamSynchronize(st));
+    if (int rc = build_tiles(sl, st); rc != SH_OK)
+        return rc;
     sl.indexed = true;
+    return SH_OK;
+}
+
+// Tile index, first part (asynchronous): per-tile sorted contributions and
+// slots in the workspace, and the two count scans whose totals the host reads
+// (into h_meta_[10], h_meta_[11]) at the next synchronisation.
+hipError_t Engine::launch_tile_build_async(SolveDesc &d, hipStream_t st)
+{
+    d.n_tiles = int(div_up(uint32_t(d.P), kTilePix));
+    const uint32_t nt = uint32_t(d.n_tiles);
+    uint32_t *ns = static_cast<uint32_t *>(ws_tnslot_), *nc = static_cast<uint32_t *>(ws_tncontrib_);
+    hipError_t e;
+    if ((e = hipMemsetAsync(ns + nt, 0, 4, st)) != hipSuccess || (e = hipMemsetAsync(nc + nt, 0, 4, st)) != hipSuccess)
+        return e;
+    if ((e = launch_tile_build(d, static_cast<uint16_t *>(ws_tcontrib_), static_cast<uint32_t *>(ws_tslot_block_),
+                               static_cast<uint16_t *>(ws_tslot_first_), ns, nc, st)) != hipSuccess)
+        return e;
+    size_t tb = sort_temp_bytes_;
+    if ((e = exclusive_scan_u32(sort_temp_, tb, ns, d.tile_sbase, nt + 1, st)) != hipSuccess)
+        return e;
+    tb = sort_temp_bytes_;
+    if ((e = exclusive_scan_u32(sort_temp_, tb, nc, d.tile_cbase, nt + 1, st)) != hipSuccess)
+        return e;
+    if ((e = hipMemcpyAsync(h_meta_ + 10, d.tile_sbase + nt, 4, hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return e;
+    return hipMemcpyAsync(h_meta_ + 11, d.tile_cbase + nt, 4, hipMemcpyDeviceToHost, st);
+}
+
+// Tile index, second part (after the synchronisation): size the slot arrays,
+// compact, sort the slots by block (stable, so each block's list is in tile
+// order), list offsets and the mirror map.
+int Engine::build_tiles(Slot &sl, hipStream_t st)
+{
+    SolveDesc &d = sl.d;
+    const uint32_t S = h_meta_[10], C = h_meta_[11];
+    if (uint64_t(S) > pair_cap_)
+        return SH_E_CAPACITY;
+    if (int rc = size_tiles(sl, S, C); rc != SH_OK)
+        return rc;
+    uint32_t *k32a = static_cast<uint32_t *>(ws_k32a_), *k32b = static_cast<uint32_t *>(ws_k32b_);
+    uint32_t *p32a = static_cast<uint32_t *>(ws_p32a_);
+    SH_HIP(launch_tile_compact(d, static_cast<const uint16_t *>(ws_tcontrib_),
+                               static_cast<const uint32_t *>(ws_tslot_block_),
+                               static_cast<const uint16_t *>(ws_tslot_first_), k32a, p32a, st));
+    const uint32_t nb = uint32_t(d.N) + 2u * uint32_t(d.U);
+    int end_bit = 1;
+    while (end_bit < 32 && (1u << end_bit) <= nb)
+        ++end_bit;
+    size_t tb = sort_temp_bytes_;
+    if (S > 0)
+        SH_HIP(sort_pairs_u32_u32(sort_temp_, tb, k32a, k32b, p32a, d.list_items, S, end_bit, st));
+    SH_HIP(launch_lower_bound_u32(k32b, S, nb + 1, d.list_off, st));
+    SH_HIP(launch_block_mirror(d, st));
+    return SH_OK;
+}
+
+// One grown device buffer per slot, cut into the tile arrays.
+int Engine::size_tiles(Slot &sl, uint32_t n_slots, uint32_t n_contrib)
+{
+    SolveDesc &d = sl.d;
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t b_contrib = up(size_t(n_contrib) * 2 + 2);
+    const size_t b_first = up((size_t(n_slots) + 1) * 4);
+    const size_t b_items = up(size_t(n_slots) * 4 + 4);
+    const size_t b_part = up(size_t(n_slots) * 28 * 4 + 16);
+    const size_t need = b_contrib + b_first + b_items + b_part;
+    if (need > sl.tile_bytes)
+    {
+        if (sl.tile_buf)
+            SH_HIP(hipFree(sl.tile_buf));
+        sl.tile_buf = nullptr;
+        sl.tile_bytes = 0;
+        const size_t bytes = need + need / 4;
+        const hipError_t e = hipMalloc(&sl.tile_buf, bytes);
+        if (e != hipSuccess)
+        {
+            last_hip_error = e;
+            sl.tile_buf = nullptr;
+            return SH_E_NOMEM;
+        }
+        sl.tile_bytes = bytes;
+    }
+    char *b = static_cast<char *>(sl.tile_buf);
+    d.tc_contrib = reinterpret_cast<uint16_t *>(b);
+    d.tc_slot_first = reinterpret_cast<uint32_t *>(b + b_contrib);
+    d.list_items = reinterpret_cast<uint32_t *>(b + b_contrib + b_first);
+    d.tpart = reinterpret_cast<float *>(b + b_contrib + b_first + b_items);
     return SH_OK;
 }
 
@@ -449,6 +565,9 @@ void Engine::build_maps(uint32_t ns)
             case kMapSeg:
                 wg = uint32_t(d.n_seg);
                 break;
+            case kMapTile:
+                wg = uint32_t(d.n_tiles);
+                break;
             }
             maps_[size_t(m) * (ns + 1) + s] = acc;
             acc += wg;
@@ -509,24 +628,42 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st)
                                         map_total_[kMapCoop] <= uint32_t(coop_max_groups_)));
     uint32_t *flags = static_cast<uint32_t *>(d_flags_);
     const RowsPlan plan = rows_plan(max_n, max_l);
-    if (!(cfg_.flags & SH_FLAG_PAIR_MAJOR) && std::max(plan.lds, plan.lds_combine) > 160u * 1024u)
+    const size_t tiles_lds = tiles_combine_lds(max_n, max_l);
+    if ((cfg_.flags & SH_FLAG_ROW_SEGMENTS) && std::max(plan.lds, plan.lds_combine) > 160u * 1024u)
+        return SH_E_CAPACITY;
+    if (!(cfg_.flags & (SH_FLAG_ROW_SEGMENTS | SH_FLAG_PAIR_MAJOR)) && tiles_lds > 160u * 1024u)
         return SH_E_CAPACITY;
     for (uint32_t it = 0; it < cfg_.gn_iterations; ++it)
     {
         const int parity = int(it & 1u);
+        SH_HIP(mark(kPhDense, st));
         SH_HIP(launch_dense_terms(K_, fp16, sds, map(kMapPixel, ns), n, map_total_[kMapPixel], parity, st));
+        SH_HIP(mark(kPhReg, st));
         SH_HIP(launch_reg_terms(sds, map(kMapReg, ns), n, map_total_[kMapReg], parity, st));
         if (cfg_.flags & SH_FLAG_PAIR_MAJOR)
         {
+            SH_HIP(mark(kPhDiagRhs, st));
             SH_HIP(launch_diag_rhs(K_, fp16, sds, map(kMapNodeWave, ns), n, map_total_[kMapNodeWave], st));
+            SH_HIP(mark(kPhOffdiag, st));
             SH_HIP(launch_offdiag(K_, fp16, sds, map(kMapPairWave, ns), n, map_total_[kMapPairWave], st));
+        }
+        else if (cfg_.flags & SH_FLAG_ROW_SEGMENTS)
+        {
+            SH_HIP(mark(kPhRowsSeg, st));
+            SH_HIP(launch_rows_seg(K_, fp16, sds, map(kMapSeg, ns), n, map_total_[kMapSeg], plan, st));
+            SH_HIP(mark(kPhRowsCombine, st));
+            SH_HIP(launch_rows_combine(sds, map(kMapRow, ns), n, map_total_[kMapRow], plan, st));
         }
         else
         {
-            SH_HIP(launch_rows(K_, fp16, sds, map(kMapSeg, ns), n, map_total_[kMapSeg], map(kMapRow, ns),
-                               map_total_[kMapRow], plan, st));
+            SH_HIP(mark(kPhTiles, st));
+            SH_HIP(launch_tiles(K_, fp16, sds, map(kMapTile, ns), n, map_total_[kMapTile], st));
+            SH_HIP(mark(kPhTilesCombine, st));
+            SH_HIP(launch_tiles_combine(sds, map(kMapRow, ns), n, map_total_[kMapRow], tiles_lds, st));
         }
+        SH_HIP(mark(kPhDinv, st));
         SH_HIP(launch_dinv(sds, map(kMapNode, ns), n, map_total_[kMapNode], st));
+        SH_HIP(mark(kPhPcg, st));
         if (use_coop)
         {
             if (resident)
@@ -546,13 +683,73 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st)
                 SH_HIP(launch_pcg_update(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), pdq, prs, st));
             }
         }
+        SH_HIP(mark(kPhUpdate, st));
         SH_HIP(launch_update(sds, map(kMapNode, ns), n, map_total_[kMapNode], parity, st));
     }
+    SH_HIP(mark(kNumPhases, st));
     return SH_OK;
+}
+
+// Phase timing (SH_FLAG_PHASE_TIMING): an event before each GN-phase launch;
+// the span from one mark to the next is charged to the earlier mark's phase.
+hipError_t Engine::mark(int phase, hipStream_t st)
+{
+    if (!(cfg_.flags & SH_FLAG_PHASE_TIMING))
+        return hipSuccess;
+    if (n_marks_ == marks_.size())
+    {
+        hipEvent_t e;
+        if (hipError_t r = hipEventCreate(&e); r != hipSuccess)
+            return r;
+        marks_.push_back({e, 0});
+    }
+    marks_[n_marks_].phase = phase;
+    return hipEventRecord(marks_[n_marks_++].ev, st);
+}
+
+void Engine::collect_marks()
+{
+    for (size_t m = 0; m + 1 < n_marks_; ++m)
+    {
+        float ms = 0.f;
+        if (marks_[m].phase < kNumPhases && hipEventElapsedTime(&ms, marks_[m].ev, marks_[m + 1].ev) == hipSuccess)
+        {
+            phase_ms_[marks_[m].phase] += double(ms);
+            phase_calls_[marks_[m].phase] += 1;
+        }
+    }
+    n_marks_ = 0;
+}
+
+int Engine::phase_times(double *ms, uint32_t *calls, uint32_t n, bool reset)
+{
+    for (uint32_t p = 0; p < n && p < uint32_t(kNumPhases); ++p)
+    {
+        if (ms)
+            ms[p] = phase_ms_[p];
+        if (calls)
+            calls[p] = phase_calls_[p];
+    }
+    if (reset)
+        for (int p = 0; p < kNumPhases; ++p)
+        {
+            phase_ms_[p] = 0.0;
+            phase_calls_[p] = 0;
+        }
+    return SH_OK;
+}
+
+const char *Engine::phase_name(uint32_t p)
+{
+    static const char *names[kNumPhases] = {"dense_terms", "reg_terms", "rows_seg", "rows_combine",
+                                            "diag_rhs",    "offdiag",   "dinv",     "pcg",
+                                            "update",      "tiles",     "tiles_combine"};
+    return p < uint32_t(kNumPhases) ? names[p] : nullptr;
 }
 
 int Engine::solve(uint32_t ns, hipStream_t st)
 {
+    n_marks_ = 0;
     if (ns == 0 || ns > slots_.size())
         return SH_E_ARG;
     for (uint32_t s = 0; s < ns; ++s)
@@ -569,6 +766,7 @@ int Engine::solve(uint32_t ns, hipStream_t st)
         return rc;
     SH_HIP(hipMemcpyAsync(h_meta_ + 8, static_cast<uint32_t *>(d_flags_) + 1, 4, hipMemcpyDeviceToHost, st));
     SH_HIP(hipStreamSynchronize(st));
+    collect_marks();
     if (h_meta_[8] != 0)
         return SH_E_HIP; // a PCG barrier wait ran out (never expected)
     for (uint32_t s = 0; s < ns; ++s)
@@ -610,6 +808,10 @@ int Engine::debug_view(uint32_t slot, const char *name, const void **ptr, size_t
         {"hg", d.jr, P * kHG},
         {"dense_cnt", d.dense_cnt, N},
         {"seg_ptr", d.seg_ptr, N + 1},
+        {"tile_cbase", d.tile_cbase, size_t(d.n_tiles) + 1},
+        {"tile_sbase", d.tile_sbase, size_t(d.n_tiles) + 1},
+        {"list_off", d.list_off, N + 2 * U + 1},
+        {"blk_mirror", d.blk_mirror, N + 2 * U},
         {"reg", d.reg, E * kReg},
         {"blk", d.blk, (N + 2 * U) * 36},
         {"dinv", d.dinv, N * 36},
@@ -846,5 +1048,21 @@ extern "C"
         *view = reinterpret_cast<const sh_dual_quat *>(batch->eng.result(slot));
         *num_nodes = uint32_t(batch->eng.desc(slot).N);
         return SH_OK;
+    }
+
+    const char *sh_phase_name(uint32_t phase) { return sh::Engine::phase_name(phase); }
+
+    int sh_solver_phase_times(sh_solver *solver, double *ms, uint32_t *calls, uint32_t num_phases, uint32_t reset)
+    {
+        if (!solver)
+            return SH_E_ARG;
+        return solver->eng.phase_times(ms, calls, num_phases, reset != 0);
+    }
+
+    int sh_batch_phase_times(sh_batch *batch, double *ms, uint32_t *calls, uint32_t num_phases, uint32_t reset)
+    {
+        if (!batch)
+            return SH_E_ARG;
+        return batch->eng.phase_times(ms, calls, num_phases, reset != 0);
     }
 }

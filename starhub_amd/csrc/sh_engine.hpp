@@ -32,6 +32,24 @@ public:
     uint32_t num_slots() const { return uint32_t(slots_.size()); }
     int debug_view(uint32_t slot, const char *name, const void **ptr, size_t *count) const;
     const sh_solver_config &config() const { return cfg_; }
+    // GN-phase kernel time (SH_FLAG_PHASE_TIMING), summed since the last reset
+    enum Phase
+    {
+        kPhDense = 0,
+        kPhReg,
+        kPhRowsSeg,
+        kPhRowsCombine,
+        kPhDiagRhs,
+        kPhOffdiag,
+        kPhDinv,
+        kPhPcg,
+        kPhUpdate,
+        kPhTiles,
+        kPhTilesCombine,
+        kNumPhases
+    };
+    int phase_times(double *ms, uint32_t *calls, uint32_t n, bool reset);
+    static const char *phase_name(uint32_t p);
 
 private:
     struct Slot
@@ -43,6 +61,8 @@ private:
         uint32_t max_row_len = 0;
         void *part_buf = nullptr; // segment partials, grown on demand
         size_t part_bytes = 0;
+        void *tile_buf = nullptr; // tile contributions, slots, lists and partials, grown on demand
+        size_t tile_bytes = 0;
     };
     enum
     {
@@ -55,6 +75,7 @@ private:
         kMapRow,
         kMapCoop,
         kMapSeg,
+        kMapTile,
         kNumMaps
     };
 
@@ -62,6 +83,11 @@ private:
     void release();
     int build_index(uint32_t slot, hipStream_t st);
     int size_partials(Slot &sl);
+    int build_tiles(Slot &sl, hipStream_t st);
+    hipError_t launch_tile_build_async(SolveDesc &d, hipStream_t st);
+    int size_tiles(Slot &sl, uint32_t n_slots, uint32_t n_contrib);
+    hipError_t mark(int phase, hipStream_t st);
+    void collect_marks();
     void build_maps(uint32_t ns);
     int upload(uint32_t ns, hipStream_t st);
     int enqueue_gn(uint32_t ns, hipStream_t st);
@@ -79,6 +105,9 @@ private:
     void *ws_k32a_ = nullptr, *ws_k32b_ = nullptr, *ws_p32a_ = nullptr, *ws_p32b_ = nullptr;
     void *ws_head_ = nullptr, *ws_hpos_ = nullptr, *ws_meta_ = nullptr;
     void *ws_bitmap_ = nullptr, *ws_rowlen_ = nullptr, *ws_segcnt_ = nullptr;
+    void *ws_tcontrib_ = nullptr, *ws_tslot_block_ = nullptr, *ws_tslot_first_ = nullptr;
+    void *ws_tnslot_ = nullptr, *ws_tncontrib_ = nullptr;
+    uint32_t tiles_cap_ = 0;
     int coop_max_groups_ = 0;
     size_t coop_lds_ = 0;
     bool coop_resident_ = false; // every solve of the launch fits the register-resident PCG
@@ -95,5 +124,14 @@ private:
     std::vector<uint32_t> maps_;
     uint32_t map_total_[kNumMaps] = {0};
     hipGraphExec_t graph_exec_ = nullptr;
+    struct Mark
+    {
+        hipEvent_t ev;
+        int phase;
+    };
+    std::vector<Mark> marks_;
+    size_t n_marks_ = 0;
+    double phase_ms_[kNumPhases] = {0.0};
+    uint32_t phase_calls_[kNumPhases] = {0};
 };
 } // namespace sh

@@ -59,11 +59,12 @@ struct RowsPlan
     size_t lds_combine = 0;  // the combine kernel's
 };
 RowsPlan rows_plan(uint32_t max_nodes, uint32_t max_row_len);
-// Segment kernel over seg_map (one work-group per segment), then the combine
-// kernel over row_map (one work-group per node row).
-hipError_t launch_rows(int K, bool fp16, const SolveDesc *sds, const uint32_t *seg_map, int n_solve,
-                       uint32_t n_seg_wg, const uint32_t *row_map, uint32_t n_row_wg, const RowsPlan &plan,
-                       hipStream_t st);
+// Segment kernel over seg_map (one work-group per segment) and the combine
+// kernel over row_map (one work-group per node row), launched in that order.
+hipError_t launch_rows_seg(int K, bool fp16, const SolveDesc *sds, const uint32_t *seg_map, int n_solve,
+                           uint32_t n_seg_wg, const RowsPlan &plan, hipStream_t st);
+hipError_t launch_rows_combine(const SolveDesc *sds, const uint32_t *row_map, int n_solve, uint32_t n_row_wg,
+                               const RowsPlan &plan, hipStream_t st);
 // Barrier slot of one persistent-PCG work-group.
 struct PcgSlot
 {
@@ -91,6 +92,22 @@ constexpr uint32_t kBitmapWords = 128;
 constexpr uint32_t kThreads = 256;
 constexpr uint32_t kWavesPerWg = 4;
 constexpr uint32_t kPcgNodesPerWg = 32;  // split PCG kernels
+
+// Pixel tiles (sh_tiles.hip)
+constexpr uint32_t kTilePix = 128;
+hipError_t launch_tile_build(const SolveDesc &d, uint16_t *ws_contrib, uint32_t *ws_slot_block,
+                             uint16_t *ws_slot_first, uint32_t *tile_nslot, uint32_t *tile_ncontrib, hipStream_t st);
+hipError_t launch_tile_compact(const SolveDesc &d, const uint16_t *ws_contrib, const uint32_t *ws_slot_block,
+                               const uint16_t *ws_slot_first, uint32_t *sort_keys, uint32_t *sort_vals,
+                               hipStream_t st);
+hipError_t launch_block_mirror(const SolveDesc &d, hipStream_t st);
+hipError_t launch_lower_bound_u32(const uint32_t *keys, uint32_t n_keys, uint32_t n_out, uint32_t *out,
+                                  hipStream_t st);
+hipError_t launch_tiles(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
+                        hipStream_t st);
+size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len);
+hipError_t launch_tiles_combine(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, size_t lds,
+                                hipStream_t st);
 
 // rocPRIM wrappers (sh_sort.hip); temp == nullptr queries the temp size
 hipError_t sort_pairs_u16_u32(void *temp, size_t &temp_bytes, const uint16_t *kin, uint16_t *kout,

@@ -535,39 +535,41 @@ static hipError_t allow_lds(size_t lds)
                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
 }
 
-hipError_t launch_rows(int K, bool fp16, const SolveDesc *sds, const uint32_t *seg_map, int n_solve,
-                       uint32_t n_seg_wg, const uint32_t *row_map, uint32_t n_row_wg, const RowsPlan &plan,
-                       hipStream_t st)
+hipError_t launch_rows_seg(int K, bool fp16, const SolveDesc *sds, const uint32_t *seg_map, int n_solve,
+                           uint32_t n_seg_wg, const RowsPlan &plan, hipStream_t st)
 {
     const size_t lds = plan.lds;
     const uint32_t chunk = plan.chunk;
-    if (n_seg_wg > 0)
+    if (n_seg_wg == 0)
+        return hipSuccess;
+    hipError_t e = (K == 8) ? (fp16 ? allow_lds<8, __half>(lds) : allow_lds<8, float>(lds))
+                            : (fp16 ? allow_lds<4, __half>(lds) : allow_lds<4, float>(lds));
+    if (e != hipSuccess)
+        return e;
+    if (K == 8)
     {
-        hipError_t e = (K == 8) ? (fp16 ? allow_lds<8, __half>(lds) : allow_lds<8, float>(lds))
-                                : (fp16 ? allow_lds<4, __half>(lds) : allow_lds<4, float>(lds));
-        if (e != hipSuccess)
-            return e;
-        if (K == 8)
-        {
-            if (fp16)
-                hipLaunchKernelGGL((k_rows_seg<8, __half>), dim3(n_seg_wg), dim3(kRowThreads), lds, st, sds, seg_map,
-                                   n_solve, chunk);
-            else
-                hipLaunchKernelGGL((k_rows_seg<8, float>), dim3(n_seg_wg), dim3(kRowThreads), lds, st, sds, seg_map,
-                                   n_solve, chunk);
-        }
+        if (fp16)
+            hipLaunchKernelGGL((k_rows_seg<8, __half>), dim3(n_seg_wg), dim3(kRowThreads), lds, st, sds, seg_map,
+                               n_solve, chunk);
         else
-        {
-            if (fp16)
-                hipLaunchKernelGGL((k_rows_seg<4, __half>), dim3(n_seg_wg), dim3(kRowThreads), lds, st, sds, seg_map,
-                                   n_solve, chunk);
-            else
-                hipLaunchKernelGGL((k_rows_seg<4, float>), dim3(n_seg_wg), dim3(kRowThreads), lds, st, sds, seg_map,
-                                   n_solve, chunk);
-        }
-        if (hipError_t e2 = hipGetLastError(); e2 != hipSuccess)
-            return e2;
+            hipLaunchKernelGGL((k_rows_seg<8, float>), dim3(n_seg_wg), dim3(kRowThreads), lds, st, sds, seg_map,
+                               n_solve, chunk);
     }
+    else
+    {
+        if (fp16)
+            hipLaunchKernelGGL((k_rows_seg<4, __half>), dim3(n_seg_wg), dim3(kRowThreads), lds, st, sds, seg_map,
+                               n_solve, chunk);
+        else
+            hipLaunchKernelGGL((k_rows_seg<4, float>), dim3(n_seg_wg), dim3(kRowThreads), lds, st, sds, seg_map,
+                               n_solve, chunk);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_rows_combine(const SolveDesc *sds, const uint32_t *row_map, int n_solve, uint32_t n_row_wg,
+                               const RowsPlan &plan, hipStream_t st)
+{
     if (n_row_wg == 0)
         return hipSuccess;
     if (plan.lds_combine > 65536)

@@ -1,0 +1,646 @@
+// Pixel-major materialisation of the dense part of J^T J and J^T r:
+// per-tile slot partials and a sorted-key segmented reduction into blocks.
+//
+// The reference gathers each node pair's terms again for every block
+// (JtJMaterializer.cu:66-175, PreconditionerRhsBuilder.cu:110-262), so every
+// per-pixel record is read once per pair it belongs to.  Here the compacted
+// pixels are cut into tiles of kTilePix consecutive pixels (raster order, so a
+// tile covers a compact image strip and few nodes).
+//
+// Once per solve (index build), one work-group per tile lists the tile's
+// contributions -- (node pair (a <= b) of distinct nodes of a pixel, pixel,
+// slot of a, slot of b) -- keys them by (upper block of the pair, pixel) and
+// radix-sorts them in LDS.  Equal blocks form a slot; slots are numbered
+// across tiles in tile order, and a stable sort of the slots by block gives
+// every block the list of its slot partials in tile order.
+//
+// Per GN iteration, one work-group per tile stages its records, weights and
+// contribution list in LDS and sums every slot's contributions in pixel
+// order (w_a w_b H, and w_a^2 g on diagonal slots); the combine kernel adds a
+// block's slot partials in list order, the row's ARAP and node-translation
+// terms and the damping.  Each record is read once per GN iteration, there
+// are no atomics, and every sum has a fixed order.
+#include <rocprim/block/block_radix_sort.hpp>
+
+#include "sh_common.hpp"
+#include "sh_desc.hpp"
+#include "sh_kernels.hpp"
+
+namespace sh
+{
+constexpr int kTileThreads = 256;
+constexpr int kTileRec = 28; // staged record: 21 H + 6 g + pad = 7 float4
+
+template <int K>
+struct TileGeom
+{
+    static constexpr int kPairs = K * (K + 1) / 2;                  // (ka <= kb) slot pairs per pixel
+    static constexpr int kItems = kTilePix * kPairs / kTileThreads; // sort items per thread
+    static_assert(kTilePix * kPairs % kTileThreads == 0, "tile geometry");
+    static_assert(kTileThreads == 2 * kTilePix, "two threads per pixel");
+};
+
+__device__ __forceinline__ int find_solve_t(const uint32_t *__restrict__ begin, int n, uint32_t wg)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1)
+    {
+        const int mid = (lo + hi) >> 1;
+        if (begin[mid] <= wg)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Full-block index of (lo, hi), lo <= hi: the diagonal block of row lo or the
+// position of hi among row lo's ascending neighbours.
+__device__ __forceinline__ uint32_t block_of_pair(const SolveDesc &d, uint32_t lo, uint32_t hi)
+{
+    const uint32_t base = d.row_ptr[lo] + lo;
+    if (lo == hi)
+        return base;
+    uint32_t a = base + 1, b = d.row_ptr[lo + 1] + lo + 1; // [a, b) ascending
+    while (a < b)
+    {
+        const uint32_t m = (a + b) >> 1;
+        if (d.blk_col[m] < hi)
+            a = m + 1;
+        else
+            b = m;
+    }
+    return a;
+}
+
+// (ka, kb) of pair q in 0 .. K(K+1)/2 - 1, ka <= kb, lexicographic.
+template <int K>
+__device__ __forceinline__ void slot_pair(int q, int &ka, int &kb)
+{
+    int a = 0, rem = q;
+#pragma unroll
+    for (int r = 0; r < K; ++r)
+    {
+        if (rem >= K - a)
+        {
+            rem -= K - a;
+            ++a;
+        }
+    }
+    ka = a;
+    kb = a + rem;
+}
+
+// Exclusive scan of one value per thread over the work-group, in thread order;
+// returns the thread's prefix, *total gets the sum.
+__device__ __forceinline__ uint32_t group_exclusive_scan(uint32_t v, uint32_t *sh, uint32_t *total)
+{
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+    {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (int(lane) >= o)
+            x += y;
+    }
+    if (lane == 63)
+        sh[w] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int i = 0; i < kTileThreads / 64; ++i)
+    {
+        before += (uint32_t(i) < w) ? sh[i] : 0u;
+        all += sh[i];
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+
+// ------------------------------------------------------------ index build
+// Per tile: contributions keyed (block << 7 | pixel), sorted in LDS.  Writes,
+// at the tile's fixed stride in the workspace, the packed contributions
+// (pixel | ka << 7 | kb << 10) in sorted order, each slot's block and first
+// contribution, and the tile's slot and contribution counts.
+template <int K>
+__global__ __launch_bounds__(kTileThreads) void k_tile_build(SolveDesc d, uint16_t *__restrict__ ws_contrib,
+                                                             uint32_t *__restrict__ ws_slot_block,
+                                                             uint16_t *__restrict__ ws_slot_first,
+                                                             uint32_t *__restrict__ tile_nslot,
+                                                             uint32_t *__restrict__ tile_ncontrib)
+{
+    using G = TileGeom<K>;
+    using Sort = rocprim::block_radix_sort<uint32_t, kTileThreads, G::kItems, uint8_t>;
+    __shared__ typename Sort::storage_type sort_storage;
+    __shared__ uint32_t last_key[kTileThreads];
+    __shared__ uint32_t red[8];
+    const uint32_t t = blockIdx.x, tid = threadIdx.x;
+    const uint32_t pl = tid >> 1, half = tid & 1;
+    const uint32_t p = t * kTilePix + pl;
+    // a row structure that overflowed its capacity is rejected by the host
+    // after this launch; read none of it
+    if (d.row_ptr[d.N] + uint32_t(d.N) > d.blk_cap)
+    {
+        if (tid == 0)
+        {
+            tile_nslot[t] = 0;
+            tile_ncontrib[t] = 0;
+        }
+        return; // uniform over the group
+    }
+
+    uint32_t keys[G::kItems];
+    uint8_t vals[G::kItems];
+#pragma unroll
+    for (int i = 0; i < G::kItems; ++i)
+    {
+        keys[i] = 0xffffffffu;
+        vals[i] = 0;
+    }
+    if (p < uint32_t(d.P))
+    {
+        uint16_t kn[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            kn[k] = d.pknn[size_t(p) * K + k];
+        // slot k is active when no later slot names the same node (last match,
+        // dense_image_jacobian.cuh:17-23)
+        bool act[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+        {
+            act[k] = true;
+#pragma unroll
+            for (int k2 = k + 1; k2 < K; ++k2)
+                if (kn[k2] == kn[k])
+                    act[k] = false;
+        }
+#pragma unroll
+        for (int i = 0; i < G::kItems; ++i)
+        {
+            int ka, kb;
+            slot_pair<K>(int(half) * G::kItems + i, ka, kb);
+            bool on_a = false, on_b = false;
+            uint32_t na = 0, nb = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+            {
+                if (k == ka)
+                {
+                    on_a = act[k];
+                    na = kn[k];
+                }
+                if (k == kb)
+                {
+                    on_b = act[k];
+                    nb = kn[k];
+                }
+            }
+            if (on_a && on_b)
+            {
+                const uint32_t lo = min(na, nb), hi = max(na, nb);
+                keys[i] = (block_of_pair(d, lo, hi) << 7) | pl;
+                vals[i] = uint8_t(ka | (kb << 3));
+            }
+        }
+    }
+    Sort().sort(keys, vals, sort_storage);
+
+    // slot heads: item gi = tid * kItems + i starts a slot when its block
+    // differs from the previous item's (blocked arrangement after the sort)
+    last_key[tid] = keys[G::kItems - 1];
+    __syncthreads();
+    const uint32_t prev0 = tid > 0 ? last_key[tid - 1] : 0xffffffffu;
+    uint32_t prev = prev0, nhead = 0, nvalid = 0;
+#pragma unroll
+    for (int i = 0; i < G::kItems; ++i)
+    {
+        const bool valid = keys[i] != 0xffffffffu;
+        nhead += (valid && (prev == 0xffffffffu || (keys[i] >> 7) != (prev >> 7))) ? 1u : 0u;
+        nvalid += valid ? 1u : 0u;
+        prev = keys[i];
+    }
+    uint32_t total_heads = 0, total_valid = 0;
+    uint32_t slot = group_exclusive_scan(nhead, red, &total_heads);
+    (void)group_exclusive_scan(nvalid, red, &total_valid);
+    const size_t stride = size_t(kTilePix) * G::kPairs;
+    uint16_t *__restrict__ cout = ws_contrib + size_t(t) * stride;
+    prev = prev0;
+#pragma unroll
+    for (int i = 0; i < G::kItems; ++i)
+    {
+        const uint32_t gi = tid * G::kItems + i;
+        const bool valid = keys[i] != 0xffffffffu;
+        const bool head = valid && (prev == 0xffffffffu || (keys[i] >> 7) != (prev >> 7));
+        if (valid)
+            cout[gi] = uint16_t((keys[i] & 127u) | (uint32_t(vals[i]) << 7));
+        if (head)
+        {
+            ws_slot_block[size_t(t) * stride + slot] = keys[i] >> 7;
+            ws_slot_first[size_t(t) * stride + slot] = uint16_t(gi);
+            ++slot;
+        }
+        prev = keys[i];
+    }
+    if (tid == 0)
+    {
+        tile_nslot[t] = total_heads;
+        tile_ncontrib[t] = total_valid; // valid items lead after the sort
+    }
+}
+
+// Per tile: move the tile's contributions and slots to their compact places
+// (bases from the scans of the counts); slot_first becomes absolute, and
+// slot_block / slot id pairs are laid out for the block sort.
+template <int K>
+__global__ __launch_bounds__(kTileThreads) void k_tile_compact(SolveDesc d, const uint16_t *__restrict__ ws_contrib,
+                                                               const uint32_t *__restrict__ ws_slot_block,
+                                                               const uint16_t *__restrict__ ws_slot_first,
+                                                               uint32_t *__restrict__ sort_keys,
+                                                               uint32_t *__restrict__ sort_vals)
+{
+    using G = TileGeom<K>;
+    const uint32_t t = blockIdx.x, tid = threadIdx.x;
+    const size_t stride = size_t(kTilePix) * G::kPairs;
+    const uint32_t c0 = d.tile_cbase[t], nc = d.tile_cbase[t + 1] - c0;
+    const uint32_t s0 = d.tile_sbase[t], ns = d.tile_sbase[t + 1] - s0;
+    for (uint32_t i = tid; i < nc; i += kTileThreads)
+        d.tc_contrib[c0 + i] = ws_contrib[size_t(t) * stride + i];
+    for (uint32_t s = tid; s < ns; s += kTileThreads)
+    {
+        d.tc_slot_first[s0 + s] = c0 + ws_slot_first[size_t(t) * stride + s];
+        sort_keys[s0 + s] = ws_slot_block[size_t(t) * stride + s];
+        sort_vals[s0 + s] = s0 + s;
+    }
+    if (t + 1 == gridDim.x && tid == 0)
+        d.tc_slot_first[d.tile_sbase[t + 1]] = d.tile_cbase[t + 1];
+}
+
+// For every block of row i: the block holding the same dense values on the
+// upper side ((i, c) itself for c >= i, (c, i) for c < i).
+__global__ __launch_bounds__(kTileThreads) void k_block_mirror(SolveDesc d)
+{
+    const uint32_t i = blockIdx.x;
+    const uint32_t base = d.row_ptr[i] + i, L1 = d.row_ptr[i + 1] - d.row_ptr[i] + 1;
+    for (uint32_t c = threadIdx.x; c < L1; c += kTileThreads)
+    {
+        const uint32_t col = d.blk_col[base + c];
+        d.blk_mirror[base + c] = col >= i ? base + c : block_of_pair(d, col, i);
+    }
+}
+
+// list_off[b] = first position of block b among the sorted slot blocks.
+__global__ void k_lower_bound_u32(const uint32_t *__restrict__ keys, uint32_t n_keys, uint32_t n_out,
+                                  uint32_t *__restrict__ out)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n_out)
+        return;
+    uint32_t lo = 0, hi = n_keys;
+    while (lo < hi)
+    {
+        const uint32_t m = (lo + hi) >> 1;
+        if (keys[m] < b)
+            lo = m + 1;
+        else
+            hi = m;
+    }
+    out[b] = lo;
+}
+
+// ------------------------------------------------------------ GN phase
+// One work-group per tile: stage records (28 floats), folded weights, the
+// contribution list and slot starts; then task (slot s, float4 group g)
+// sums coef * record[4g .. 4g+3] over the slot's contributions in order.
+template <int K, class JT>
+__global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restrict__ sds,
+                                                        const uint32_t *__restrict__ wg_begin, int n_solve)
+{
+    using G = TileGeom<K>;
+    __shared__ float4 Hs[kTilePix * (kTileRec / 4)];
+    __shared__ float wS[kTilePix * K];
+    __shared__ uint16_t ent[kTilePix * G::kPairs];
+    __shared__ uint16_t sst[kTilePix * G::kPairs + 1];
+    const int si = find_solve_t(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const uint32_t t = blockIdx.x - wg_begin[si];
+    if (t >= uint32_t(d.n_tiles))
+        return; // uniform over the group
+    const uint32_t tid = threadIdx.x;
+    const uint32_t p0 = t * kTilePix;
+    const uint32_t np = min(uint32_t(kTilePix), uint32_t(d.P) - p0);
+    if (tid < np)
+    {
+        const uint32_t p = p0 + tid;
+        float hg[kHG];
+        load_hg<JT>(d.jr, p, hg);
+#pragma unroll
+        for (int v = 0; v < kTileRec / 4; ++v)
+            Hs[tid * (kTileRec / 4) + v] = make_float4(hg[4 * v], hg[4 * v + 1], hg[4 * v + 2], hg[4 * v + 3]);
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            wS[tid * K + k] = d.pw[size_t(p) * K + k];
+    }
+    const uint32_t c0 = d.tile_cbase[t], nc = d.tile_cbase[t + 1] - c0;
+    const uint32_t s0 = d.tile_sbase[t], ns = d.tile_sbase[t + 1] - s0;
+    for (uint32_t i = tid; i < nc; i += kTileThreads)
+        ent[i] = d.tc_contrib[c0 + i];
+    for (uint32_t s = tid; s <= ns; s += kTileThreads)
+        sst[s] = uint16_t(d.tc_slot_first[s0 + s] - c0);
+    __syncthreads();
+    float4 *__restrict__ out = reinterpret_cast<float4 *>(d.tpart) + size_t(s0) * (kTileRec / 4);
+    for (uint32_t task = tid; task < ns * (kTileRec / 4); task += kTileThreads)
+    {
+        const uint32_t s = task / (kTileRec / 4), g = task % (kTileRec / 4);
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        const uint32_t e1 = sst[s + 1];
+        for (uint32_t e = sst[s]; e < e1; ++e)
+        {
+            const uint32_t c = ent[e];
+            const uint32_t pl = c & 127u, ka = (c >> 7) & 7u, kb = (c >> 10) & 7u;
+            const float coef = wS[pl * K + ka] * wS[pl * K + kb];
+            const float4 h = Hs[pl * (kTileRec / 4) + g];
+            acc.x = fmaf(coef, h.x, acc.x);
+            acc.y = fmaf(coef, h.y, acc.y);
+            acc.z = fmaf(coef, h.z, acc.z);
+            acc.w = fmaf(coef, h.w, acc.w);
+        }
+        out[task] = acc;
+    }
+}
+
+// ARAP entries of the x-side row [-[p]x, I] (see reg_elem in the row kernels).
+__device__ __forceinline__ float reg_elem_t(f3 p, int c, int q)
+{
+    float v = (q - 3 == c) ? 1.f : 0.f;
+    v = (c == 0 && q == 1) ? p.z : v;
+    v = (c == 0 && q == 2) ? -p.y : v;
+    v = (c == 1 && q == 0) ? -p.z : v;
+    v = (c == 1 && q == 2) ? p.x : v;
+    v = (c == 2 && q == 0) ? p.y : v;
+    v = (c == 2 && q == 1) ? -p.x : v;
+    return v;
+}
+
+__device__ __forceinline__ void sym_pair_t(int e, int &a, int &b)
+{
+    a = (e >= 6) + (e >= 11) + (e >= 15) + (e >= 18) + (e >= 20);
+    b = e - (a * 6 - a * (a - 1) / 2) + a;
+}
+
+constexpr int kCombTasks = 4; // (block, float4 group) tasks per thread and pass
+
+// One work-group per node row: every block's slot partials summed in list
+// (tile) order, the ARAP terms (node_graph_reg_jacobian.cuh:166-358) and the
+// node-translation term (node_motion_jacobian.cuh:7-35) of the row added by
+// wave 0 in list order, the damping on the diagonal; full 6x6 blocks and
+// rhs_i = -(g of the diagonal slots + ARAP/translation g).
+__global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc *__restrict__ sds,
+                                                                const uint32_t *__restrict__ wg_begin, int n_solve)
+{
+    extern __shared__ float smem[];
+    const int si = find_solve_t(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const uint32_t i = blockIdx.x - wg_begin[si];
+    if (i >= uint32_t(d.N))
+        return; // uniform over the group
+    const uint32_t tid = threadIdx.x;
+    const uint32_t base = d.row_ptr[i] + i;
+    const uint32_t L1 = d.row_ptr[i + 1] - d.row_ptr[i] + 1;
+    // LDS: regblk[L1][36] | colmap u16[N] | red[16]
+    float *regblk = smem;
+    uint16_t *colmap = reinterpret_cast<uint16_t *>(regblk + size_t(L1) * 36);
+    float *red = reinterpret_cast<float *>(colmap + ((d.N + 1) & ~1));
+    for (uint32_t t = tid; t < L1; t += kTileThreads)
+        colmap[d.blk_col[base + t]] = uint16_t(t);
+    for (uint32_t t = tid; t < L1 * 36; t += kTileThreads)
+        regblk[t] = 0.f;
+    if (tid < 16)
+        red[tid] = 0.f;
+    __syncthreads();
+
+    const uint32_t dend = d.n2t_off[i] + d.dense_cnt[i];
+    const uint32_t end = d.n2t_off[i + 1];
+    if (tid < 64)
+    {
+        const int lane = int(tid);
+        float gA = 0.f;
+        const uint32_t P = d.P, E = d.E;
+        for (uint32_t it = dend; it < end; ++it)
+        {
+            const uint32_t t = d.n2t_terms[it];
+            if (t < P + E)
+            {
+                const uint32_t e = t - P;
+                const float4 *rec = reinterpret_cast<const float4 *>(d.reg + size_t(e) * kReg);
+                const float4 u0 = rec[0], u1 = rec[1];
+                if (u1.w == 0.f)
+                    continue;
+                const f3 a = mk3(u0.x, u0.y, u0.z), b = mk3(u0.w, u1.x, u1.y);
+                const float cw = u1.z;
+                const uint32_t ex = d.edges[3 * e], ey = d.edges[3 * e + 1];
+                const bool is_x = (ex == i);
+                const uint32_t other = is_x ? ey : ex;
+                const float ws = d.lam2_reg * (cw * cw);
+                // unweighted ARAP rows: x side [-[a]x, I], y side [[b]x, -I]
+                // (node_graph_reg_jacobian.cuh:250-275)
+                const f3 pm = is_x ? a : b; // this node's warped point
+                const f3 po = is_x ? b : a; // the other node's
+                const float sm = is_x ? 1.f : -1.f;
+                if (lane < 36)
+                {
+                    const int aa = lane / 6, bb = lane % 6;
+                    float vd = 0.f, vo = 0.f;
+#pragma unroll
+                    for (int ch = 0; ch < 3; ++ch)
+                    {
+                        const float ga = sm * reg_elem_t(pm, ch, aa);
+                        const float gb = sm * reg_elem_t(pm, ch, bb);
+                        const float ob = -sm * reg_elem_t(po, ch, bb);
+                        vd += ga * gb;
+                        vo += ga * ob;
+                    }
+                    regblk[lane] += ws * vd;
+                    if (other != i)
+                        regblk[colmap[other] * 36 + lane] += ws * vo;
+                }
+                if (lane < 6)
+                {
+                    const float res[3] = {(a.x - b.x), (a.y - b.y), (a.z - b.z)};
+                    float v = 0.f;
+#pragma unroll
+                    for (int ch = 0; ch < 3; ++ch)
+                        v += res[ch] * (sm * reg_elem_t(pm, ch, lane));
+                    gA += ws * v;
+                }
+            }
+            else
+            {
+                const uint32_t m = t - P - E;
+                const float4 pred = d.node_pred ? d.node_pred[m] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const float4 tt = d.ttrans[m];
+                const float c2 = pred.w * pred.w;
+                if (lane >= 3 && lane < 6)
+                {
+                    regblk[lane * 7] += d.lam2_nt * c2;
+                    const float tv = (lane == 3) ? tt.x - pred.x : (lane == 4) ? tt.y - pred.y : tt.z - pred.z;
+                    gA += d.lam2_nt * tv * c2;
+                }
+            }
+        }
+        if (lane < 6)
+            red[8 + lane] = gA;
+    }
+
+    const float4 *__restrict__ part = reinterpret_cast<const float4 *>(d.tpart);
+    const size_t out0 = size_t(base) * 36;
+    const uint32_t n_tasks = L1 * (kTileRec / 4);
+    for (uint32_t tp = 0; tp < n_tasks; tp += kTileThreads * kCombTasks)
+    {
+        float4 acc[kCombTasks];
+#pragma unroll
+        for (int s = 0; s < kCombTasks; ++s)
+        {
+            acc[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+            const uint32_t task = tp + tid + s * kTileThreads;
+            if (task < n_tasks)
+            {
+                const uint32_t c = task / (kTileRec / 4), g = task % (kTileRec / 4);
+                const uint32_t ub = d.blk_mirror[base + c];
+                const uint32_t l1 = d.list_off[ub + 1];
+                for (uint32_t l = d.list_off[ub]; l < l1; ++l)
+                {
+                    const float4 v = part[size_t(d.list_items[l]) * (kTileRec / 4) + g];
+                    acc[s].x += v.x;
+                    acc[s].y += v.y;
+                    acc[s].z += v.z;
+                    acc[s].w += v.w;
+                }
+            }
+        }
+        __syncthreads(); // regblk complete (wave 0) before the first writes
+#pragma unroll
+        for (int s = 0; s < kCombTasks; ++s)
+        {
+            const uint32_t task = tp + tid + s * kTileThreads;
+            if (task >= n_tasks)
+                continue;
+            const uint32_t c = task / (kTileRec / 4), g = task % (kTileRec / 4);
+            const float av[4] = {acc[s].x, acc[s].y, acc[s].z, acc[s].w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                const int e = int(g) * 4 + q;
+                if (e < 21)
+                {
+                    int a, b;
+                    sym_pair_t(e, a, b);
+                    float u = av[q] + regblk[c * 36 + a * 6 + b];
+                    if (c == 0 && a == b)
+                        u += d.damping;
+                    d.blk[out0 + size_t(c) * 36 + a * 6 + b] = u;
+                    if (a != b)
+                        d.blk[out0 + size_t(c) * 36 + b * 6 + a] = av[q] + regblk[c * 36 + b * 6 + a];
+                }
+                else if (c == 0 && e < 27)
+                    red[e - 21] = av[q]; // g of the diagonal slots
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < 6)
+        d.rhs[size_t(i) * 6 + tid] = -(red[tid] + red[8 + tid]);
+}
+
+// ------------------------------------------------------------ launchers
+hipError_t launch_tile_build(const SolveDesc &d, uint16_t *ws_contrib, uint32_t *ws_slot_block,
+                             uint16_t *ws_slot_first, uint32_t *tile_nslot, uint32_t *tile_ncontrib, hipStream_t st)
+{
+    if (d.n_tiles == 0)
+        return hipSuccess;
+    if (d.K == 8)
+        hipLaunchKernelGGL(k_tile_build<8>, dim3(d.n_tiles), dim3(kTileThreads), 0, st, d, ws_contrib, ws_slot_block,
+                           ws_slot_first, tile_nslot, tile_ncontrib);
+    else
+        hipLaunchKernelGGL(k_tile_build<4>, dim3(d.n_tiles), dim3(kTileThreads), 0, st, d, ws_contrib, ws_slot_block,
+                           ws_slot_first, tile_nslot, tile_ncontrib);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_compact(const SolveDesc &d, const uint16_t *ws_contrib, const uint32_t *ws_slot_block,
+                               const uint16_t *ws_slot_first, uint32_t *sort_keys, uint32_t *sort_vals,
+                               hipStream_t st)
+{
+    if (d.n_tiles == 0)
+        return hipSuccess;
+    if (d.K == 8)
+        hipLaunchKernelGGL(k_tile_compact<8>, dim3(d.n_tiles), dim3(kTileThreads), 0, st, d, ws_contrib,
+                           ws_slot_block, ws_slot_first, sort_keys, sort_vals);
+    else
+        hipLaunchKernelGGL(k_tile_compact<4>, dim3(d.n_tiles), dim3(kTileThreads), 0, st, d, ws_contrib,
+                           ws_slot_block, ws_slot_first, sort_keys, sort_vals);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_mirror(const SolveDesc &d, hipStream_t st)
+{
+    if (d.N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_block_mirror, dim3(d.N), dim3(kTileThreads), 0, st, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_lower_bound_u32(const uint32_t *keys, uint32_t n_keys, uint32_t n_out, uint32_t *out,
+                                  hipStream_t st)
+{
+    if (n_out == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_lower_bound_u32, dim3((n_out + 255) / 256), dim3(256), 0, st, keys, n_keys, n_out, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_tiles(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
+                        hipStream_t st)
+{
+    if (n_wg == 0)
+        return hipSuccess;
+    if (K == 8)
+    {
+        if (fp16)
+            hipLaunchKernelGGL((k_tiles<8, __half>), dim3(n_wg), dim3(kTileThreads), 0, st, sds, wmap, n_solve);
+        else
+            hipLaunchKernelGGL((k_tiles<8, float>), dim3(n_wg), dim3(kTileThreads), 0, st, sds, wmap, n_solve);
+    }
+    else
+    {
+        if (fp16)
+            hipLaunchKernelGGL((k_tiles<4, __half>), dim3(n_wg), dim3(kTileThreads), 0, st, sds, wmap, n_solve);
+        else
+            hipLaunchKernelGGL((k_tiles<4, float>), dim3(n_wg), dim3(kTileThreads), 0, st, sds, wmap, n_solve);
+    }
+    return hipGetLastError();
+}
+
+size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len)
+{
+    return sizeof(float) * ((size_t(max_row_len) + 1) * 36 + 16) + sizeof(uint16_t) * ((max_nodes + 1) & ~1u);
+}
+
+hipError_t launch_tiles_combine(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, size_t lds,
+                                hipStream_t st)
+{
+    if (n_wg == 0)
+        return hipSuccess;
+    if (lds > 65536)
+    {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_tiles_combine),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        if (e != hipSuccess)
+            return e;
+    }
+    hipLaunchKernelGGL(k_tiles_combine, dim3(n_wg), dim3(kTileThreads), lds, st, sds, wmap, n_solve);
+    return hipGetLastError();
+}
+} // namespace sh

@@ -20,6 +20,7 @@ SH_OK = 0
 SH_STREAM_FP32, SH_STREAM_FP16 = 0, 1
 SH_SPMV_MATERIALIZED, SH_SPMV_MATRIX_FREE, SH_SPMV_AUTO = 0, 1, 2
 SH_FLAG_REFERENCE_INDEX, SH_FLAG_PAIR_MAJOR, SH_FLAG_NO_PERSISTENT_PCG, SH_FLAG_STREAMING_PCG = 1, 2, 4, 8
+SH_FLAG_PHASE_TIMING, SH_FLAG_ROW_SEGMENTS = 16, 32
 
 _VP = ctypes.c_void_p
 
@@ -245,6 +246,22 @@ class WarpSolver:
               f"debug_view({name})")
         return _device_copy(view.value, cnt.value, dtype)
 
+    def phase_times(self, reset: bool = False) -> dict:
+        """{phase: (total ms, launches)} since the last reset (needs SH_FLAG_PHASE_TIMING)."""
+        return _phase_times(lib().sh_solver_phase_times, self._h, reset)
+
+
+NUM_PHASES = 11
+
+
+def _phase_times(fn, handle, reset: bool) -> dict:
+    ms = (ctypes.c_double * NUM_PHASES)()
+    calls = (ctypes.c_uint32 * NUM_PHASES)()
+    check(fn(handle, ms, calls, NUM_PHASES, int(reset)), "phase_times")
+    L = lib()
+    L.sh_phase_name.restype = ctypes.c_char_p
+    return {L.sh_phase_name(p).decode(): (ms[p], calls[p]) for p in range(NUM_PHASES)}
+
 
 class BatchSolver:
     """Batched uneven solves (sh_batch): one launch per GN phase covers every slot."""
@@ -280,6 +297,9 @@ class BatchSolver:
         n = ctypes.c_uint32()
         check(lib().sh_batch_node_se3(self._h, slot, ctypes.byref(view), ctypes.byref(n)), "sh_batch_node_se3")
         return _device_copy(view.value, n.value * 8, torch.float32).view(n.value, 8)
+
+    def phase_times(self, reset: bool = False) -> dict:
+        return _phase_times(lib().sh_batch_phase_times, self._h, reset)
 
 
 def _device_copy(addr: int, count: int, dtype):

@@ -179,15 +179,16 @@ def test_stage_values(c1_one_iter, torch_dev):
     assert np.abs(x - ref["pcg_x"]).max() <= 1e-3 * np.abs(ref["pcg_x"]).max()
 
 
-@pytest.mark.parametrize("flags", ["pair_major", "split_pcg", "streaming_pcg"])
+@pytest.mark.parametrize("flags", ["pair_major", "row_segments", "split_pcg", "streaming_pcg"])
 def test_alternative_paths(torch_dev, flags):
     """The pair-major materialisation (the reference's traversal), the
-    two-launch-per-iteration PCG and the streaming persistent PCG solve C1 to
-    the same tolerance as the default (register-resident persistent PCG)."""
+    row-segment materialisation, the two-launch-per-iteration PCG and the
+    streaming persistent PCG solve C1 to the same tolerance as the default
+    (pixel tiles + register-resident persistent PCG)."""
     import sho
     from starhub_amd import native, scene, workload
-    f = {"pair_major": native.SH_FLAG_PAIR_MAJOR, "split_pcg": native.SH_FLAG_NO_PERSISTENT_PCG,
-         "streaming_pcg": native.SH_FLAG_STREAMING_PCG}[flags]
+    f = {"pair_major": native.SH_FLAG_PAIR_MAJOR, "row_segments": native.SH_FLAG_ROW_SEGMENTS,
+         "split_pcg": native.SH_FLAG_NO_PERSISTENT_PCG, "streaming_pcg": native.SH_FLAG_STREAMING_PCG}[flags]
     say("alternative path", flags)
     sc = scene.generate(scene.config_c1())
     solver, frame, prod, _ = _solve(sc, flags=f)
