@@ -46,3 +46,37 @@ def test_producers_reject_bad_arguments_without_device():
     null = ctypes.c_void_p()
     assert L.sh_skin_surfels(null, 10, null, 10, 5, 1.0, null, null, null, null) == 1
     assert L.sh_build_node_graph(null, 3, 8, 1.0, null, null, null, null, null) == 1
+
+
+def test_cpp_wrapper_compiles_links_and_reports_errors(tmp_path):
+    """The C++ drop-in (include/sh/warp_solver.hpp) builds against the C-ABI
+    with the host compiler alone, and a rejected configuration surfaces as
+    sh::Error carrying SH_E_ARG -- no GPU needed."""
+    root = native.lib_path().parents[2]
+    src = tmp_path / "wrap.cpp"
+    src.write_text(r'''
+#include <sh/warp_solver.hpp>
+#include <cstdio>
+int main() {
+    sh_solver_config c{};
+    c.max_width = 64; c.max_height = 48; c.max_nodes = 128; c.knn = 5;  // knn 5: invalid
+    c.gn_iterations = 1; c.pcg_iterations = 1;
+    sh_camera cam{64, 48, 50.f, 50.f, 32.f, 24.f};
+    float pose[16] = {1,0,0,0, 0,1,0,0, 0,0,1,0, 0,0,0,1};
+    try {
+        sh::OptimizationProcessorWarpSolver proc(c, cam, pose);
+        std::puts("constructed");
+        return 2;
+    } catch (const sh::Error &e) {
+        std::printf("%d %s\n", e.status(), e.what());
+        return e.status() == SH_E_ARG ? 0 : 3;
+    }
+}
+''')
+    exe = tmp_path / "wrap"
+    lib = native.lib_path()
+    subprocess.run(["g++", "-std=c++17", "-O1", f"-I{root / 'include'}", str(src), str(lib),
+                    f"-Wl,-rpath,{lib.parent}", "-o", str(exe)], check=True, capture_output=True)
+    res = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "invalid argument" in res.stdout
