@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64-oracle check of the warp field")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline sample budget")
+    ap.add_argument("--no-served", action="store_true", help="skip the serving (64 streams) and batch (32 solves) legs")
+    ap.add_argument("--served-requests", type=int, default=10000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain device copies of 256 MiB (the reference reading for HBM counters)")
     return ap.parse_args()
@@ -126,6 +128,104 @@ def cpu_baseline(sc, prod, cfg, seconds: float):
     return {"value": 1000.0 / med, "unit": "solves/s", "cores": threads, "kind": "port",
             "sample": f"{len(times)} full FP32 solves of the same frame (median solve time), OpenMP over "
                       f"{threads} host threads"}
+
+
+def plain_rate(frames, cfgs, native, reps: int = 1):
+    """Plain path on a mix: one solver, the frames solved one after another."""
+    import time as _t
+    W = max(c.width for c in cfgs)
+    H = max(c.height for c in cfgs)
+    N = max(c.num_nodes for c in cfgs)
+    solver = native.WarpSolver(native.make_config(cfgs[0].solver, W, H, N, cfgs[0].K))
+    for f in frames[:2]:
+        solver.process_frame(f)
+    t0 = _t.perf_counter()
+    for _ in range(reps):
+        for f in frames:
+            solver.process_frame(f)
+    dt = _t.perf_counter() - t0
+    solver.close()
+    return reps * len(frames) / dt
+
+
+def batched_leg(native, scene, workload):
+    """32 uneven solves (N log-uniform in [128, 4096], 300 surfels/node) in one
+    batch: one launch per GN phase for all of them."""
+    import time as _t
+    cfgs = scene.config_c4()
+    frames = [workload.device_frame(scene.generate(c))[0] for c in cfgs]
+    W = max(c.width for c in cfgs)
+    H = max(c.height for c in cfgs)
+    N = max(c.num_nodes for c in cfgs)
+    b = native.BatchSolver(native.make_config(cfgs[0].solver, W, H, N, cfgs[0].K), len(cfgs))
+    for i, f in enumerate(frames):
+        b.set_inputs(i, f)
+    b.solve(len(cfgs))
+    reps = 5
+    t0 = _t.perf_counter()
+    for _ in range(reps):
+        b.solve(len(cfgs))
+    dt = _t.perf_counter() - t0
+    b.close()
+    batched = reps * len(cfgs) / dt
+    plain = plain_rate(frames, cfgs, native)
+    return {"workload": "32 solves, N log-uniform [128, 4096], 300 surfels/node, 320x240..1280x720, "
+                        f"{cfgs[0].solver.gn_iterations} GN x {cfgs[0].solver.pcg_iterations} PCG",
+            "solves_per_s": batched, "plain_solves_per_s": plain, "vs_plain": batched / plain}
+
+
+def served_leg(native, scene, workload, n_requests: int):
+    """64 camera streams, Poisson arrivals at 30 Hz each, continuous batching."""
+    import ctypes
+    import time as _t
+    import numpy as np
+    cfgs = scene.config_c5_streams()
+    frames = [workload.device_frame(scene.generate(c))[0] for c in cfgs]
+    W = max(c.width for c in cfgs)
+    H = max(c.height for c in cfgs)
+    N = max(c.num_nodes for c in cfgs)
+    scfg = native.make_config(cfgs[0].solver, W, H, N, cfgs[0].K)
+    plain = plain_rate(frames, cfgs, native)
+    srv = native.Server(scfg, max_active=64)
+    for i, f in enumerate(frames):
+        native.check(srv.add_stream(i, f), "add_stream")
+    L = native.lib()
+    args = []
+    for i, f in enumerate(frames):
+        cam, meas, rend, geo, ng, nf, fl, kp, c2w = f.structs()
+        args.append((meas, rend, fl, nf))
+    rng = np.random.default_rng(5)
+    arrivals = []
+    for i in range(len(frames)):
+        t = np.cumsum(rng.exponential(1.0 / 30.0, size=n_requests // len(frames) + 50))
+        arrivals += [(float(x), i) for x in t]
+    arrivals.sort()
+    arrivals = arrivals[:n_requests]
+    tickets = []
+    tk = ctypes.c_uint64()
+    t0 = _t.perf_counter()
+    for when, i in arrivals:
+        d = when - (_t.perf_counter() - t0)
+        if d > 0:
+            _t.sleep(d)
+        meas, rend, fl, nf = args[i]
+        native.check(L.sh_server_submit(srv._h, i, ctypes.byref(meas), ctypes.byref(rend), ctypes.byref(fl),
+                                        ctypes.byref(nf), None, ctypes.byref(tk)), "submit")
+        tickets.append(tk.value)
+    res = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    srv.close()
+    ok = [r for r in res if r["status"] == 0]
+    lat = np.array([r["done_ms"] - r["submit_ms"] for r in ok])
+    span_s = (max(r["done_ms"] for r in ok) - min(r["submit_ms"] for r in ok)) / 1e3
+    served = len(ok) / span_s
+    return {"workload": f"64 streams, N log-uniform [256, 4096], 300 surfels/node, Poisson 30 Hz/stream, "
+                        f"{len(res)} requests, {cfgs[0].solver.gn_iterations} GN x {cfgs[0].solver.pcg_iterations} PCG",
+            "solves_per_s": served, "offered_per_s": len(res) / arrivals[-1][0],
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "failed": len(res) - len(ok), "slots": st["slots"], "max_batch": st["max_batch"],
+            "mean_batch": st["slot_iterations"] / max(1, st["steps"]),
+            "plain_solves_per_s": plain, "vs_plain": served / plain}
 
 
 def main():
@@ -209,6 +309,10 @@ def main():
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, prod, cfg, args.cpu_seconds)
+    if not args.no_served:
+        del solver
+        out["batched"] = batched_leg(native, scene, workload)
+        out["served"] = served_leg(native, scene, workload, args.served_requests)
     print(json.dumps(out), flush=True)
 
 

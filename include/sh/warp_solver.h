@@ -69,7 +69,10 @@ typedef enum sh_solver_flags
     SH_FLAG_PHASE_TIMING = 16u,
     /* Materialise row-major from node-row segments (each per-pixel record read once per
      * node of the pixel) instead of from pixel tiles (each record read once). */
-    SH_FLAG_ROW_SEGMENTS = 32u
+    SH_FLAG_ROW_SEGMENTS = 32u,
+    /* Also evaluate the cost at the final warp field (one more term pass), so the
+     * cost history has gn_iterations + 1 entries. */
+    SH_FLAG_COST_HISTORY = 64u
 } sh_solver_flags;
 
 /* Dual quaternion as stored by the reference: two float4 (x, y, z, w),
@@ -205,7 +208,11 @@ int sh_solver_sizes(const sh_solver *solver, uint32_t *num_pixels, uint32_t *num
  * returns the device pointer and element count. */
 int sh_solver_debug_view(const sh_solver *solver, const char *name, const void **device_ptr, size_t *count);
 
-/* Per-GN-iteration total weighted cost (FP64), host copy, count = gn_iterations+1. */
+/* FP64 total weighted cost sum lambda^2 r^2 (dense depth + flow, ARAP) at the start
+ * of every GN iteration, plus the final one with SH_FLAG_COST_HISTORY: the GN
+ * acceptance residual (the reference sums it in FP32 for logging only,
+ * DenseImageDebug.cu:5-27).  Host copy; *count = number of entries; out may
+ * be NULL to query the count. */
 int sh_solver_cost_history(const sh_solver *solver, double *out, uint32_t capacity, uint32_t *count);
 
 /* ApplySpMVBase<6>::ApplySpMV (libs/star/pcg_solver/ApplySpMVBase.h:19-37):
@@ -237,6 +244,64 @@ int sh_batch_set_inputs(sh_batch *batch, uint32_t slot, const sh_camera *camera,
 /* Solves slots [0, num_solves) together: one launch per GN phase covers all. */
 int sh_batch_solve(sh_batch *batch, uint32_t num_solves, sh_stream_t stream);
 int sh_batch_node_se3(const sh_batch *batch, uint32_t slot, const sh_dual_quat **device_view, uint32_t *num_nodes);
+
+/* ------------------------------------------------------------- serving */
+/* No single counterpart in the reference, which solves one frame of one camera
+ * per ProcessFrame call (examples/mono_star/app.cu:132-141).  A server takes
+ * tracking-solve requests from many camera streams: each stream's solver
+ * state (surfel KNN ids and weights, node graph, last warp field) stays
+ * resident in server-owned HBM; solves are batched continuously at
+ * Gauss-Newton-iteration granularity (requests join at the next iteration
+ * boundary as slots free up); frames of one stream are solved in order. */
+typedef struct sh_server sh_server;
+
+typedef struct sh_server_config
+{
+    sh_solver_config solver;   /* per-solve capacity (the largest request) and solver constants */
+    uint32_t max_active;       /* solves in flight at most (continuous-batch width) */
+    uint64_t hbm_budget_bytes; /* device bytes the server may hold (solve slots + resident streams); 0 = no limit */
+} sh_server_config;
+
+typedef struct sh_request_result
+{
+    uint64_t ticket;
+    uint32_t stream_id;
+    int status;
+    double submit_ms, admit_ms, done_ms; /* server clock, ms since sh_server_create */
+} sh_request_result;
+
+typedef struct sh_server_stats
+{
+    uint64_t slots;           /* solve slots the budget allowed */
+    uint64_t streams, streams_refused;
+    uint64_t submitted, completed, failed;
+    uint64_t steps;           /* GN-iteration launches */
+    uint64_t max_batch;       /* most solves in one launch */
+    uint64_t slot_iterations; /* sum of batch sizes over steps */
+    uint64_t hbm_reserved;    /* bytes held */
+    double now_ms;
+} sh_server_stats;
+
+/* Creates the server and its two worker threads.  Fewer than max_active slots
+ * are made when the budget cannot hold them; SH_E_CAPACITY when not even one fits. */
+int sh_server_create(const sh_server_config *config, sh_server **out);
+int sh_server_destroy(sh_server *server);
+/* Registers a stream and copies its geometry and node graph into resident HBM;
+ * SH_E_CAPACITY when the budget cannot hold it (admission by memory budget). */
+int sh_server_add_stream(sh_server *server, uint32_t stream_id, const sh_camera *camera, const sh_geometry *geometry,
+                         const sh_node_graph *node_graph, const float cam2world[16]);
+/* Queues one frame of a stream.  The maps are caller-owned device memory and must
+ * stay valid until the request is done; out_device (may be NULL) receives the
+ * solved node dual quaternions. */
+int sh_server_submit(sh_server *server, uint32_t stream_id, const sh_measure_maps *measure,
+                     const sh_render_maps *render, const sh_flow_map *flow, const sh_node_flow *node_flow,
+                     sh_dual_quat *out_device, uint64_t *ticket);
+/* Blocks until the request is done; returns its status and timestamps. */
+int sh_server_wait(sh_server *server, uint64_t ticket, sh_request_result *result);
+/* Device view of a stream's resident warp field (the last solve's result). */
+int sh_server_warp_field(sh_server *server, uint32_t stream_id, const sh_dual_quat **device_view,
+                         uint32_t *num_nodes);
+int sh_server_get_stats(sh_server *server, sh_server_stats *stats);
 
 /* ------------------------------------------------------- measurement hook */
 /* No counterpart in the reference (which times nothing on the device).  With

@@ -26,6 +26,7 @@ struct SolveDesc
     int U;               // unique upper node pairs
     int n_pair_entries;  // valid (pair, term) entries
     int pcg_iters;
+    int gn_iter;         // GN iterations done: node DQs are read from dq[(gn_iter + launch parity) & 1]
     uint32_t blk_cap;    // capacity of blk_col / blocks (N_max + 2 U_max)
 
     // caller-owned inputs
@@ -80,7 +81,8 @@ struct SolveDesc
     // PCG vectors (6N each)
     float *x, *r, *s, *q, *dv[2];
     double *delta;    // pcg_iters + 1 : r.s per iteration
-    double *cost;     // weighted cost per GN iteration
+    double *cost;     // weighted cost (dense + ARAP) at the start of each GN iteration, then the final one
+    double *cost_part; // work-group partials: ceil(P/256) dense, then ceil((E+N)/256) ARAP
     // matrix-free scratch
     float *mf_z;      // P*6 per-pixel H_p s_p
     float *mf_reg;    // E*3 per-edge J x

@@ -60,6 +60,9 @@ void Engine::release()
     if (h_meta_)
         (void)hipHostFree(h_meta_);
     h_meta_ = nullptr;
+    if (h_flag_)
+        (void)hipHostFree(h_flag_);
+    h_flag_ = nullptr;
     if (h_stage_)
         (void)hipHostFree(h_stage_);
     h_stage_ = nullptr;
@@ -86,6 +89,7 @@ int Engine::alloc(void **p, size_t bytes)
         return SH_E_NOMEM;
     }
     allocs_.push_back(*p);
+    bytes_ += bytes;
     return SH_OK;
 }
 
@@ -97,7 +101,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         return SH_E_ARG;
     if (c.pcg_iterations == 0 || c.pcg_iterations > 1000 || c.gn_iterations > 1000)
         return SH_E_ARG;
-    if (c.stream_precision > SH_STREAM_FP16 || c.spmv_mode > SH_SPMV_AUTO || (c.flags & ~63u))
+    if (c.stream_precision > SH_STREAM_FP16 || c.spmv_mode > SH_SPMV_AUTO || (c.flags & ~127u))
         return SH_E_ARG;
     cfg_ = c;
     if (cfg_.flags & SH_FLAG_PAIR_MAJOR)
@@ -134,11 +138,15 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         A(d.pw, size_t(pcap_) * K_ * 4);
         A(d.n2t_off, size_t(ncap_ + 1) * 4);
         A(d.n2t_terms, size_t(n2t_cap_) * 4);
-        A(d.half_keys, size_t(ucap_) * 4);
-        A(d.half_off, size_t(ucap_ + 1) * 4);
-        A(d.pair_entries, size_t(pcap_) * np * 4 + size_t(ecap_) * 4);
-        A(d.upper_pos, size_t(ucap_) * 4);
-        A(d.lower_pos, size_t(ucap_) * 4);
+        if (c.flags & SH_FLAG_REFERENCE_INDEX)
+        {
+            // the reference's node-pair index, only built on request
+            A(d.half_keys, size_t(ucap_) * 4);
+            A(d.half_off, size_t(ucap_ + 1) * 4);
+            A(d.pair_entries, size_t(pcap_) * np * 4 + size_t(ecap_) * 4);
+            A(d.upper_pos, size_t(ucap_) * 4);
+            A(d.lower_pos, size_t(ucap_) * 4);
+        }
         A(d.row_ptr, size_t(ncap_ + 1) * 4);
         A(d.blk_col, (size_t(ncap_) + 2 * size_t(ucap_)) * 4);
         A(d.dq[0], size_t(ncap_) * 32);
@@ -163,6 +171,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         A(d.dv[1], size_t(ncap_) * 6 * 4);
         A(d.delta, size_t(c.pcg_iterations + 1) * 8);
         A(d.cost, size_t(c.gn_iterations + 2) * 8);
+        A(d.cost_part, (size_t(div_up(pcap_, 256)) + div_up(ecap_ + ncap_, 256) + 1) * 8);
     }
     // shared index-build workspace
     const size_t npix = size_t(pcap_);
@@ -211,6 +220,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
     SH_HIP(hipMemset(d_flags_, 0, 64));
 #undef A
     SH_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_meta_), 64, hipHostMallocDefault));
+    SH_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_flag_), 64, hipHostMallocDefault));
     stage_bytes_ = size_t(max_slots) * sizeof(SolveDesc) + size_t(kNumMaps) * (max_slots + 1) * 4;
     SH_HIP(hipHostMalloc(&h_stage_, stage_bytes_, hipHostMallocDefault));
     return SH_OK;
@@ -525,14 +535,14 @@ void Engine::build_maps(uint32_t ns)
     maps_.assign(size_t(kNumMaps) * (ns + 1), 0u);
     coop_resident_ = !(cfg_.flags & SH_FLAG_STREAMING_PCG);
     for (uint32_t s = 0; s < ns; ++s)
-        if (resident_groups(slots_[s].d.N, slots_[s].max_row_len) == 0)
+        if (resident_groups(slots_[batch_[s]].d.N, slots_[batch_[s]].max_row_len) == 0)
             coop_resident_ = false;
     for (int m = 0; m < kNumMaps; ++m)
     {
         uint32_t acc = 0;
         for (uint32_t s = 0; s < ns; ++s)
         {
-            const SolveDesc &d = slots_[s].d;
+            const SolveDesc &d = slots_[batch_[s]].d;
             uint32_t wg = 0;
             switch (m)
             {
@@ -558,7 +568,7 @@ void Engine::build_maps(uint32_t ns)
                 wg = uint32_t(d.N);
                 break;
             case kMapCoop:
-                wg = coop_resident_ ? resident_groups(d.N, slots_[s].max_row_len)
+                wg = coop_resident_ ? resident_groups(d.N, slots_[batch_[s]].max_row_len)
                                     : std::min(kPcgMaxGroups, std::max(1u, div_up(uint32_t(d.N) + 2u * uint32_t(d.U),
                                                                                   kPcgBlocksPerGroup)));
                 break;
@@ -581,7 +591,7 @@ int Engine::upload(uint32_t ns, hipStream_t st)
 {
     char *h = static_cast<char *>(h_stage_);
     for (uint32_t s = 0; s < ns; ++s)
-        std::memcpy(h + size_t(s) * sizeof(SolveDesc), &slots_[s].d, sizeof(SolveDesc));
+        std::memcpy(h + size_t(s) * sizeof(SolveDesc), &slots_[batch_[s]].d, sizeof(SolveDesc));
     const size_t mb = maps_.size() * 4;
     std::memcpy(h + size_t(ns) * sizeof(SolveDesc), maps_.data(), mb);
     SH_HIP(hipMemcpyAsync(d_descs_, h, size_t(ns) * sizeof(SolveDesc), hipMemcpyHostToDevice, st));
@@ -594,25 +604,28 @@ const uint32_t *Engine::map(int m, uint32_t ns) const
     return static_cast<const uint32_t *>(d_wmaps_) + size_t(m) * (ns + 1);
 }
 
-// The Gauss-Newton loop (WarpSolver::SolveStreamed :260-337) for slots [0, ns).
-int Engine::enqueue_gn(uint32_t ns, hipStream_t st)
+// The Gauss-Newton loop (WarpSolver::SolveStreamed :260-337) for the slots of
+// batch_: `iters` iterations, the first starting at each solve's gn_iter, from
+// the identity warp field when `identity` is set.
+int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identity)
 {
     const SolveDesc *sds = static_cast<const SolveDesc *>(d_descs_);
     const int n = int(ns);
     const bool fp16 = fp16_;
-    SH_HIP(launch_identity(sds, map(kMapNode, ns), n, map_total_[kMapNode], st));
+    if (identity)
+        SH_HIP(launch_identity(sds, map(kMapNode, ns), n, map_total_[kMapNode], st));
     double *prs = static_cast<double *>(d_part_rs_), *pdq = static_cast<double *>(d_part_dq_);
     uint32_t max_n = 0, max_l = 0, tile = 1;
     const uint32_t *coop_begin = maps_.data() + size_t(kMapCoop) * (ns + 1);
     for (uint32_t s = 0; s < ns; ++s)
     {
-        const SolveDesc &d = slots_[s].d;
+        const SolveDesc &d = slots_[batch_[s]].d;
         max_n = std::max(max_n, uint32_t(d.N));
-        max_l = std::max(max_l, slots_[s].max_row_len);
+        max_l = std::max(max_l, slots_[batch_[s]].max_row_len);
         // blocks of one persistent-PCG group: its share plus one whole row
         const uint32_t G = coop_begin[s + 1] - coop_begin[s];
         const uint32_t nb = uint32_t(d.N) + 2u * uint32_t(d.U);
-        tile = std::max(tile, div_up(nb, G) + slots_[s].max_row_len + 2u);
+        tile = std::max(tile, div_up(nb, G) + slots_[batch_[s]].max_row_len + 2u);
     }
     const size_t coop_lds = size_t(tile) * 6 * sizeof(float);
     if (coop_lds != coop_lds_)
@@ -633,9 +646,9 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st)
         return SH_E_CAPACITY;
     if (!(cfg_.flags & (SH_FLAG_ROW_SEGMENTS | SH_FLAG_PAIR_MAJOR)) && tiles_lds > 160u * 1024u)
         return SH_E_CAPACITY;
-    for (uint32_t it = 0; it < cfg_.gn_iterations; ++it)
+    for (uint32_t it = 0; it < iters; ++it)
     {
-        const int parity = int(it & 1u);
+        const int parity = int(it); // iteration offset: DQ buffer (gn_iter + it) & 1, cost slot gn_iter + it
         SH_HIP(mark(kPhDense, st));
         SH_HIP(launch_dense_terms(K_, fp16, sds, map(kMapPixel, ns), n, map_total_[kMapPixel], parity, st));
         SH_HIP(mark(kPhReg, st));
@@ -759,18 +772,105 @@ int Engine::solve(uint32_t ns, hipStream_t st)
     for (uint32_t s = 0; s < ns; ++s)
         if ((rc = build_index(s, st)) != SH_OK)
             return rc;
+    batch_.resize(ns);
+    for (uint32_t s = 0; s < ns; ++s)
+    {
+        batch_[s] = s;
+        slots_[s].d.gn_iter = 0;
+    }
     build_maps(ns);
     if ((rc = upload(ns, st)) != SH_OK)
         return rc;
-    if ((rc = enqueue_gn(ns, st)) != SH_OK)
+    if ((rc = enqueue_gn(ns, st, cfg_.gn_iterations, true)) != SH_OK)
         return rc;
-    SH_HIP(hipMemcpyAsync(h_meta_ + 8, static_cast<uint32_t *>(d_flags_) + 1, 4, hipMemcpyDeviceToHost, st));
+    if (cfg_.flags & SH_FLAG_COST_HISTORY)
+    {
+        // cost at the final warp field: the terms once more, then the reduction
+        const SolveDesc *sds = static_cast<const SolveDesc *>(d_descs_);
+        const int it = int(cfg_.gn_iterations);
+        SH_HIP(launch_dense_terms(K_, fp16_, sds, map(kMapPixel, ns), int(ns), map_total_[kMapPixel], it, st));
+        SH_HIP(launch_reg_terms(sds, map(kMapReg, ns), int(ns), map_total_[kMapReg], it, st));
+        SH_HIP(launch_cost_final(sds, int(ns), it, st));
+    }
+    if ((rc = finish(st)) != SH_OK)
+        return rc;
+    for (uint32_t s = 0; s < ns; ++s)
+    {
+        slots_[s].d.gn_iter = int(cfg_.gn_iterations);
+        slots_[s].solved = true;
+    }
+    return SH_OK;
+}
+
+// FP64 cost history of a solved slot (host copy).
+int Engine::cost_history(uint32_t slot, double *out, uint32_t capacity, uint32_t *count) const
+{
+    if (slot >= slots_.size() || !slots_[slot].solved)
+        return SH_E_STATE;
+    const uint32_t n = cfg_.gn_iterations + ((cfg_.flags & SH_FLAG_COST_HISTORY) ? 1u : 0u);
+    if (count)
+        *count = n;
+    if (!out)
+        return SH_OK;
+    if (capacity < n)
+        return SH_E_CAPACITY;
+    SH_HIP(hipMemcpy(out, slots_[slot].d.cost, size_t(n) * sizeof(double), hipMemcpyDeviceToHost));
+    return SH_OK;
+}
+
+// Synchronise the launch stream and check the PCG barrier flag.
+int Engine::finish(hipStream_t st)
+{
+    SH_HIP(hipMemcpyAsync(h_flag_, static_cast<uint32_t *>(d_flags_) + 1, 4, hipMemcpyDeviceToHost, st));
     SH_HIP(hipStreamSynchronize(st));
     collect_marks();
-    if (h_meta_[8] != 0)
+    if (*h_flag_ != 0)
         return SH_E_HIP; // a PCG barrier wait ran out (never expected)
-    for (uint32_t s = 0; s < ns; ++s)
-        slots_[s].solved = true;
+    return SH_OK;
+}
+
+// Index build for one slot and its identity start: the admission half of
+// continuous batching (uses the index workspace only).
+int Engine::admit(uint32_t slot, hipStream_t st)
+{
+    if (slot >= slots_.size() || !slots_[slot].has_inputs)
+        return SH_E_STATE;
+    Slot &sl = slots_[slot];
+    sl.solved = false;
+    if (int rc = build_index(slot, st); rc != SH_OK)
+        return rc;
+    SH_HIP(launch_identity_dq(sl.d.dq[0], uint32_t(sl.d.N), st));
+    sl.d.gn_iter = 0;
+    SH_HIP(hipStreamSynchronize(st));
+    return SH_OK;
+}
+
+// One GN iteration for an arbitrary set of admitted slots, each at its own
+// iteration: the step half of continuous batching (uses the launch state only).
+int Engine::step(const uint32_t *list, uint32_t n, hipStream_t st)
+{
+    n_marks_ = 0;
+    if (n == 0)
+        return SH_OK;
+    batch_.assign(list, list + n);
+    for (uint32_t s = 0; s < n; ++s)
+        if (batch_[s] >= slots_.size() || !slots_[batch_[s]].indexed ||
+            slots_[batch_[s]].d.gn_iter >= int(cfg_.gn_iterations))
+            return SH_E_STATE;
+    build_maps(n);
+    int rc;
+    if ((rc = upload(n, st)) != SH_OK)
+        return rc;
+    if ((rc = enqueue_gn(n, st, 1, false)) != SH_OK)
+        return rc;
+    if ((rc = finish(st)) != SH_OK)
+        return rc;
+    for (uint32_t s = 0; s < n; ++s)
+    {
+        Slot &sl = slots_[batch_[s]];
+        sl.d.gn_iter += 1;
+        sl.solved = sl.d.gn_iter == int(cfg_.gn_iterations);
+    }
     return SH_OK;
 }
 
@@ -944,12 +1044,9 @@ extern "C"
 
     int sh_solver_cost_history(const sh_solver *solver, double *out, uint32_t capacity, uint32_t *count)
     {
-        (void)solver;
-        (void)out;
-        (void)capacity;
-        if (count)
-            *count = 0;
-        return SH_E_STATE;
+        if (!solver)
+            return SH_E_ARG;
+        return solver->eng.cost_history(0, out, capacity, count);
     }
 
     int sh_solver_apply_jtj(sh_solver *solver, const float *x, float *y, uint32_t matrix_free, sh_stream_t stream)

@@ -26,6 +26,21 @@ public:
                    const sh_geometry *geo, const sh_node_graph *ng, const sh_node_flow *nf, const sh_flow_map *flow,
                    const sh_keypoints *kp, const float cam2world[16]);
     int solve(uint32_t num_solves, hipStream_t st);
+    // continuous batching: admit() builds a slot's index and identity start
+    // (index workspace); step() runs one GN iteration over a set of admitted
+    // slots (launch state).  The two may run on different threads and streams.
+    int admit(uint32_t slot, hipStream_t st);
+    int step(const uint32_t *slots, uint32_t n, hipStream_t st);
+    int gn_iter(uint32_t slot) const { return slots_[slot].d.gn_iter; }
+    int cost_history(uint32_t slot, double *out, uint32_t capacity, uint32_t *count) const;
+    // device bytes held: fixed allocations plus the grown per-slot buffers
+    size_t device_bytes() const
+    {
+        size_t b = bytes_;
+        for (const Slot &sl : slots_)
+            b += sl.part_bytes + sl.tile_bytes;
+        return b;
+    }
     const dq8 *result(uint32_t slot) const;
     bool solved(uint32_t slot) const { return slot < slots_.size() && slots_[slot].solved; }
     const SolveDesc &desc(uint32_t slot) const { return slots_[slot].d; }
@@ -90,7 +105,8 @@ private:
     void collect_marks();
     void build_maps(uint32_t ns);
     int upload(uint32_t ns, hipStream_t st);
-    int enqueue_gn(uint32_t ns, hipStream_t st);
+    int enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identity);
+    int finish(hipStream_t st);
     const uint32_t *map(int m, uint32_t ns) const;
 
     sh_solver_config cfg_{};
@@ -100,6 +116,7 @@ private:
     uint64_t ucap_ = 0, pair_cap_ = 0;
     std::vector<Slot> slots_;
     std::vector<void *> allocs_;
+    size_t bytes_ = 0;
     // index workspace
     void *ws_flags_ = nullptr, *ws_pos_ = nullptr, *ws_k16a_ = nullptr, *ws_k16b_ = nullptr, *ws_v32a_ = nullptr;
     void *ws_k32a_ = nullptr, *ws_k32b_ = nullptr, *ws_p32a_ = nullptr, *ws_p32b_ = nullptr;
@@ -116,7 +133,9 @@ private:
     void *d_flags_ = nullptr; // [0] PCG epoch base, [1] PCG barrier error
     void *sort_temp_ = nullptr;
     size_t sort_temp_bytes_ = 0;
-    uint32_t *h_meta_ = nullptr;
+    uint32_t *h_meta_ = nullptr; // index-build readbacks
+    uint32_t *h_flag_ = nullptr; // launch-side readback (PCG barrier flag)
+    std::vector<uint32_t> batch_; // slots of the current launch, in order
     void *h_stage_ = nullptr;
     size_t stage_bytes_ = 0;
     // launch state

@@ -359,18 +359,24 @@ __device__ __forceinline__ dq8 load_dq(const dq8 *__restrict__ a, uint32_t i)
     return r;
 }
 
-// Per-pixel residual / Jacobian (computeDenseImageJacobianKernel,
-// DenseImageJacobianResidual.cu:164-310; forms :12-157).
-template <int K, class JT>
-__global__ __launch_bounds__(256) void k_dense_terms(const SolveDesc *__restrict__ sds,
-                                                     const uint32_t *__restrict__ wg_begin, int n_solve, int parity)
+// Sum over a 256-thread work-group in FP64, fixed order (wave trees, then waves 0..3).
+__device__ __forceinline__ double block_sum_256(double v)
 {
-    const int si = find_solve(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
-    const uint32_t p = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
-    if (p >= uint32_t(d.P))
-        return;
-    const dq8 *__restrict__ dq = d.dq[parity];
+    __shared__ double part[4];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0)
+        part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double r = (part[0] + part[1]) + (part[2] + part[3]);
+    __syncthreads();
+    return r;
+}
+
+// One pixel: residual, Jacobian, the folded H/g record; returns its cost.
+template <int K, class JT>
+__device__ __forceinline__ double dense_pixel(const SolveDesc &d, uint32_t p, int parity)
+{
+    const dq8 *__restrict__ dq = d.dq[(d.gn_iter + parity) & 1];
     uint16_t kn[K];
     float w[K];
     load_knn<K>(d, p, kn, w);
@@ -478,7 +484,28 @@ __global__ __launch_bounds__(256) void k_dense_terms(const SolveDesc *__restrict
     }
     hg[kHG - 1] = 0.f;
     store_hg<JT>(d.jr, p, hg);
+    return double(d.lam2_depth) * double(out[18]) * double(out[18]) +
+           double(d.lam2_flow) * (double(out[19]) * double(out[19]) + double(out[20]) * double(out[20]));
 }
+
+// Per-pixel residual / Jacobian (computeDenseImageJacobianKernel,
+// DenseImageJacobianResidual.cu:164-310; forms :12-157).
+template <int K, class JT>
+__global__ __launch_bounds__(256) void k_dense_terms(const SolveDesc *__restrict__ sds,
+                                                     const uint32_t *__restrict__ wg_begin, int n_solve, int parity)
+{
+    const int si = find_solve(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const uint32_t p = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
+    double pix_cost = 0.0;
+    if (p < uint32_t(d.P))
+        pix_cost = dense_pixel<K, JT>(d, p, parity);
+    // FP64 cost partial of the work-group (fixed order)
+    const double wc = block_sum_256(pix_cost);
+    if (threadIdx.x == 0)
+        d.cost_part[blockIdx.x - wg_begin[si]] = wc;
+}
+
 
 // ARAP forward warp per edge + node translations (forwardWarpSmootherNodeKernel,
 // NodeGraphHandler.cu:7-46; ComputeNodeMotionKernel, NodeMotionHandler.cu:9-21).
@@ -488,7 +515,8 @@ __global__ __launch_bounds__(256) void k_reg_terms(const SolveDesc *__restrict__
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
     const SolveDesc &d = sds[si];
     const uint32_t item = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
-    const dq8 *__restrict__ dq = d.dq[parity];
+    const dq8 *__restrict__ dq = d.dq[(d.gn_iter + parity) & 1];
+    double edge_cost = 0.0;
     if (item < uint32_t(d.E))
     {
         const uint32_t i = d.edges[3 * item], j = d.edges[3 * item + 1], k = d.edges[3 * item + 2];
@@ -504,6 +532,12 @@ __global__ __launch_bounds__(256) void k_reg_terms(const SolveDesc *__restrict__
         float4 *o = reinterpret_cast<float4 *>(d.reg + size_t(item) * kReg);
         o[0] = make_float4(a.x, a.y, a.z, b.x);
         o[1] = make_float4(b.y, b.z, c, valid);
+        if (valid != 0.f)
+        {
+            const double ex = double(a.x) - double(b.x), ey = double(a.y) - double(b.y),
+                         ez = double(a.z) - double(b.z);
+            edge_cost = double(d.lam2_reg) * double(c) * double(c) * (ex * ex + ey * ey + ez * ez);
+        }
     }
     else if (item < uint32_t(d.E + d.N))
     {
@@ -511,6 +545,9 @@ __global__ __launch_bounds__(256) void k_reg_terms(const SolveDesc *__restrict__
         const rt34 T = dq_se3(load_dq(dq, n));
         d.ttrans[n] = make_float4(T.t.x, T.t.y, T.t.z, 0.f);
     }
+    const double wc = block_sum_256(edge_cost);
+    if (threadIdx.x == 0)
+        d.cost_part[(uint32_t(d.P) + 255u) / 256u + (blockIdx.x - wg_begin[si])] = wc;
 }
 
 // ARAP Jacobian rows of one side, scaled by c (node_graph_reg_jacobian.cuh:250-275).
@@ -1013,6 +1050,25 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(const SolveDesc *__r
         part_rs[blockIdx.x] = tot;
 }
 
+// Total weighted cost (dense + ARAP) from the work-group partials of
+// k_dense_terms and k_reg_terms, FP64, fixed order; one wave.
+__device__ void reduce_cost(const SolveDesc &d, int slot)
+{
+    const uint32_t n = (uint32_t(d.P) + 255u) / 256u + (uint32_t(d.E + d.N) + 255u) / 256u;
+    double v = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 64)
+        v += d.cost_part[i];
+    v = wave_sum(v);
+    if (threadIdx.x == 0)
+        d.cost[slot] = v;
+}
+
+// One wave per solve: the cost after the last update (SH_FLAG_COST_HISTORY).
+__global__ void k_cost_final(const SolveDesc *__restrict__ sds, int slot)
+{
+    reduce_cost(sds[blockIdx.x], slot);
+}
+
 // T <- exp(xi) T, stored as a dual quaternion (ApplyWarpFieldUpdateKernel,
 // SolverIterationData.cu:19-74)
 __global__ __launch_bounds__(256) void k_update(const SolveDesc *__restrict__ sds, const uint32_t *__restrict__ wg_begin,
@@ -1020,6 +1076,8 @@ __global__ __launch_bounds__(256) void k_update(const SolveDesc *__restrict__ sd
 {
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
     const SolveDesc &d = sds[si];
+    if (blockIdx.x == wg_begin[si] && threadIdx.x < 64)
+        reduce_cost(d, d.gn_iter + parity); // cost of the state this iteration started from
     const uint32_t n = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
     if (n >= uint32_t(d.N))
         return;
@@ -1055,7 +1113,7 @@ __global__ __launch_bounds__(256) void k_update(const SolveDesc *__restrict__ sd
         S.r[8] = tt * ax.z * ax.z + c;
     }
     S.t = t;
-    const rt34 prev = dq_to_mat34(load_dq(d.dq[parity], n));
+    const rt34 prev = dq_to_mat34(load_dq(d.dq[(d.gn_iter + parity) & 1], n));
     rt34 nw;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -1064,7 +1122,7 @@ __global__ __launch_bounds__(256) void k_update(const SolveDesc *__restrict__ sd
             nw.r[3 * i + j] = S.r[3 * i] * prev.r[j] + S.r[3 * i + 1] * prev.r[3 + j] + S.r[3 * i + 2] * prev.r[6 + j];
     nw.t = rot_apply(S.r, prev.t) + S.t;
     const dq8 out = dq_from_mat34(nw);
-    float4 *o = reinterpret_cast<float4 *>(d.dq[parity ^ 1] + n);
+    float4 *o = reinterpret_cast<float4 *>(d.dq[((d.gn_iter + parity) & 1) ^ 1] + n);
     o[0] = out.q0;
     o[1] = out.q1;
 }
@@ -1079,6 +1137,16 @@ __global__ void k_identity(const SolveDesc *__restrict__ sds, const uint32_t *__
     float4 *o = reinterpret_cast<float4 *>(d.dq[0] + n);
     o[0] = make_float4(0.f, 0.f, 0.f, 1.f);
     o[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Identity warp field for one solve (SolverIterationData.cu:78-88).
+__global__ void k_identity_dq(float4 *__restrict__ dq, uint32_t n_nodes)
+{
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_nodes)
+        return;
+    dq[2 * n] = make_float4(0.f, 0.f, 0.f, 1.f);
+    dq[2 * n + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // ============================================================ input producers
@@ -1407,6 +1475,23 @@ hipError_t launch_row_ptr(const SolveDesc &d, const uint32_t *skeys, hipStream_t
 hipError_t launch_identity(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, hipStream_t st)
 {
     hipLaunchKernelGGL(k_identity, dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve);
+    return hipGetLastError();
+}
+
+hipError_t launch_cost_final(const SolveDesc *sds, int n_solve, int slot, hipStream_t st)
+{
+    if (n_solve == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_cost_final, dim3(n_solve), dim3(64), 0, st, sds, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_identity_dq(dq8 *dq, uint32_t n_nodes, hipStream_t st)
+{
+    if (n_nodes == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_identity_dq, dim3((n_nodes + 255) / 256), dim3(256), 0, st, reinterpret_cast<float4 *>(dq),
+                       n_nodes);
     return hipGetLastError();
 }
 

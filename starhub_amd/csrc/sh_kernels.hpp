@@ -25,6 +25,8 @@ hipError_t launch_sym_scatter(const SolveDesc &d, const uint32_t *skeys, const u
 hipError_t launch_row_ptr(const SolveDesc &d, const uint32_t *skeys, hipStream_t st);
 
 hipError_t launch_identity(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, hipStream_t st);
+hipError_t launch_identity_dq(dq8 *dq, uint32_t n_nodes, hipStream_t st);
+hipError_t launch_cost_final(const SolveDesc *sds, int n_solve, int slot, hipStream_t st);
 hipError_t launch_dense_terms(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve,
                               uint32_t n_wg, int parity, hipStream_t st);
 hipError_t launch_reg_terms(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int parity,

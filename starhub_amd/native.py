@@ -20,7 +20,7 @@ SH_OK = 0
 SH_STREAM_FP32, SH_STREAM_FP16 = 0, 1
 SH_SPMV_MATERIALIZED, SH_SPMV_MATRIX_FREE, SH_SPMV_AUTO = 0, 1, 2
 SH_FLAG_REFERENCE_INDEX, SH_FLAG_PAIR_MAJOR, SH_FLAG_NO_PERSISTENT_PCG, SH_FLAG_STREAMING_PCG = 1, 2, 4, 8
-SH_FLAG_PHASE_TIMING, SH_FLAG_ROW_SEGMENTS = 16, 32
+SH_FLAG_PHASE_TIMING, SH_FLAG_ROW_SEGMENTS, SH_FLAG_COST_HISTORY = 16, 32, 64
 
 _VP = ctypes.c_void_p
 
@@ -246,6 +246,15 @@ class WarpSolver:
               f"debug_view({name})")
         return _device_copy(view.value, cnt.value, dtype)
 
+    def cost_history(self) -> np.ndarray:
+        """FP64 weighted cost at the start of every GN iteration (+ the final one with SH_FLAG_COST_HISTORY)."""
+        n = ctypes.c_uint32()
+        check(lib().sh_solver_cost_history(self._h, None, 0, ctypes.byref(n)), "sh_solver_cost_history")
+        out = np.zeros(n.value, np.float64)
+        check(lib().sh_solver_cost_history(self._h, out.ctypes.data_as(ctypes.c_void_p), n.value, ctypes.byref(n)),
+              "sh_solver_cost_history")
+        return out
+
     def phase_times(self, reset: bool = False) -> dict:
         """{phase: (total ms, launches)} since the last reset (needs SH_FLAG_PHASE_TIMING)."""
         return _phase_times(lib().sh_solver_phase_times, self._h, reset)
@@ -300,6 +309,79 @@ class BatchSolver:
 
     def phase_times(self, reset: bool = False) -> dict:
         return _phase_times(lib().sh_batch_phase_times, self._h, reset)
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("solver", SolverConfig), ("max_active", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_uint64), ("stream_id", ctypes.c_uint32), ("status", ctypes.c_int),
+                ("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double), ("done_ms", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("slots", "streams", "streams_refused", "submitted", "completed",
+                                                "failed", "steps", "max_batch", "slot_iterations", "hbm_reserved")]
+    _fields_ += [("now_ms", ctypes.c_double)]
+
+
+class Server:
+    """Serving loop (sh_server): continuous batching of tracking solves from many streams."""
+
+    def __init__(self, solver_config: SolverConfig, max_active: int, hbm_budget_bytes: int = 0):
+        L = lib()
+        L.sh_server_wait.argtypes = [_VP, ctypes.c_uint64, ctypes.POINTER(RequestResult)]
+        L.sh_server_submit.argtypes = [_VP, ctypes.c_uint32, _VP, _VP, _VP, _VP, _VP, ctypes.POINTER(ctypes.c_uint64)]
+        self._h = ctypes.c_void_p()
+        self.cfg = ServerConfig(solver_config, max_active, hbm_budget_bytes)
+        check(L.sh_server_create(ctypes.byref(self.cfg), ctypes.byref(self._h)), "sh_server_create")
+        self._frames = {}
+
+    def close(self):
+        if self._h:
+            lib().sh_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_stream(self, stream_id: int, frame: "DeviceFrame") -> int:
+        """Registers a stream; returns the status (SH_E_CAPACITY when the HBM budget refuses it)."""
+        cam, meas, rend, geo, ng, nf, fl, kp, c2w = frame.structs()
+        return lib().sh_server_add_stream(self._h, stream_id, ctypes.byref(cam), ctypes.byref(geo),
+                                          ctypes.byref(ng), c2w)
+
+    def submit(self, stream_id: int, frame: "DeviceFrame", out=None) -> int:
+        cam, meas, rend, geo, ng, nf, fl, kp, c2w = frame.structs()
+        self._frames[stream_id] = frame  # keep the maps alive
+        t = ctypes.c_uint64()
+        out_p = ctypes.c_void_p(out.data_ptr()) if out is not None else ctypes.c_void_p()
+        check(lib().sh_server_submit(self._h, stream_id, ctypes.byref(meas), ctypes.byref(rend), ctypes.byref(fl),
+                                     ctypes.byref(nf), out_p, ctypes.byref(t)), "sh_server_submit")
+        return t.value
+
+    def wait(self, ticket: int) -> dict:
+        r = RequestResult()
+        rc = lib().sh_server_wait(self._h, ticket, ctypes.byref(r))
+        return {"ticket": r.ticket, "stream": r.stream_id, "status": rc, "submit_ms": r.submit_ms,
+                "admit_ms": r.admit_ms, "done_ms": r.done_ms}
+
+    def warp_field(self, stream_id: int):
+        import torch
+        view = ctypes.c_void_p()
+        n = ctypes.c_uint32()
+        check(lib().sh_server_warp_field(self._h, stream_id, ctypes.byref(view), ctypes.byref(n)),
+              "sh_server_warp_field")
+        return _device_copy(view.value, n.value * 8, torch.float32).view(n.value, 8)
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(lib().sh_server_get_stats(self._h, ctypes.byref(st)), "sh_server_get_stats")
+        return {n: getattr(st, n) for n, _ in ServerStats._fields_}
 
 
 def _device_copy(addr: int, count: int, dtype):

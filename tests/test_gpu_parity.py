@@ -314,3 +314,83 @@ def test_capacity_errors(torch_dev):
     s = native.WarpSolver(few)
     with pytest.raises(native.SolverError, match="capacity"):
         s.set_inputs(frame)
+
+
+def test_served_same_alone_and_in_mix(torch_dev):
+    """Serving: requests of 6 streams of uneven sizes, interleaved, run with
+    continuous batching (4 slots: solves at different GN iterations share
+    launches).  Every result equals the same frame solved alone, bit for bit;
+    the stream's resident warp field holds its last result; the launches did
+    mix solves."""
+    from starhub_amd import native, scene, workload
+    torch = torch_dev
+    cfgs = [scene.config_batch_member(n, 700 + i) for i, n in enumerate([128, 400, 256, 900, 180, 600])]
+    scs = [scene.generate(c) for c in cfgs]
+    frames = [workload.device_frame(s)[0] for s in scs]
+    W = max(c.width for c in cfgs)
+    H = max(c.height for c in cfgs)
+    N = max(c.num_nodes for c in cfgs)
+    scfg = native.make_config(cfgs[0].solver, W, H, N, 8)
+    alone = []
+    for f in frames:
+        s = native.WarpSolver(scfg)
+        s.process_frame(f)
+        alone.append(s.node_se3().cpu().numpy())
+        s.close()
+    srv = native.Server(scfg, max_active=4)
+    for i, f in enumerate(frames):
+        assert srv.add_stream(i, f) == 0
+    outs, tickets = [], []
+    for rep in range(3):
+        for i, f in enumerate(frames):
+            out = torch.zeros(cfgs[i].num_nodes, 8, dtype=torch.float32, device="cuda")
+            outs.append((i, out))
+            tickets.append(srv.submit(i, f, out))
+    results = [srv.wait(t) for t in tickets]
+    torch.cuda.synchronize()
+    assert all(r["status"] == 0 for r in results)
+    for i, out in outs:
+        assert np.array_equal(out.cpu().numpy(), alone[i]), i
+    for i in range(len(frames)):
+        assert np.array_equal(srv.warp_field(i).cpu().numpy(), alone[i])
+    st = srv.stats()
+    say("server stats", st)
+    assert st["completed"] == len(tickets) and st["failed"] == 0
+    assert st["max_batch"] > 1  # solves overlapped on the device
+    srv.close()
+
+
+def test_server_budget_admission(torch_dev):
+    """A budget that holds the solve slots but not every stream refuses the
+    streams beyond it with SH_E_CAPACITY (admission by HBM budget)."""
+    from starhub_amd import native, scene, workload
+    cfg = scene.config_batch_member(256, 801)
+    sc = scene.generate(cfg)
+    frame, _ = workload.device_frame(sc)
+    scfg = native.make_config(cfg.solver, cfg.width, cfg.height, cfg.num_nodes, 8)
+    probe = native.Server(scfg, max_active=1)
+    base = probe.stats()["hbm_reserved"]
+    assert probe.add_stream(0, frame) == 0
+    per_stream = probe.stats()["hbm_reserved"] - base
+    probe.close()
+    srv = native.Server(scfg, max_active=1, hbm_budget_bytes=base + 2 * per_stream + per_stream // 2)
+    assert [srv.add_stream(i, frame) for i in range(4)] == [0, 0, 2, 2]  # SH_E_CAPACITY = 2
+    assert srv.stats()["streams_refused"] == 2
+    srv.close()
+
+
+def test_cost_history_fp64_vs_oracle(torch_dev):
+    """The FP64 GN cost at the start of every iteration and after the last
+    (SH_FLAG_COST_HISTORY) matches the FP64 oracle's to 1e-5 relative (the
+    residuals themselves are FP32)."""
+    import sho
+    from starhub_amd import native, scene, workload
+    sc = scene.generate(scene.config_c1())
+    solver, frame, prod, _ = _solve(sc, flags=native.SH_FLAG_COST_HISTORY)
+    gpu = solver.cost_history()
+    ref = sho.solve(workload.host_inputs(sc, prod), sc.cfg.solver, 64, sho.max_threads())["cost"]
+    say("cost gpu", gpu, "fp64", ref)
+    assert gpu.shape == ref.shape == (sc.cfg.solver.gn_iterations + 1,)
+    assert np.allclose(gpu, ref, rtol=1e-5, atol=0)
+    plain, _, _, _ = _solve(sc)
+    assert np.array_equal(plain.cost_history(), gpu[:-1])  # the flag only adds the final entry
