@@ -636,7 +636,11 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
     if (res_max_groups_ < 0)
         res_max_groups_ = pcg_resident_max_groups();
     const bool resident = coop_resident_ && map_total_[kMapCoop] <= uint32_t(res_max_groups_);
-    const bool use_coop = !(cfg_.flags & SH_FLAG_NO_PERSISTENT_PCG) &&
+    // the matrix-free operator reads the tile index: not with the other materialisations
+    const bool matrix_free = cfg_.spmv_mode == SH_SPMV_MATRIX_FREE;
+    if (matrix_free && (cfg_.flags & (SH_FLAG_PAIR_MAJOR | SH_FLAG_ROW_SEGMENTS)))
+        return SH_E_ARG;
+    const bool use_coop = !matrix_free && !(cfg_.flags & SH_FLAG_NO_PERSISTENT_PCG) &&
                           (resident || (!coop_resident_ && coop_max_groups_ > 0 &&
                                         map_total_[kMapCoop] <= uint32_t(coop_max_groups_)));
     uint32_t *flags = static_cast<uint32_t *>(d_flags_);
@@ -686,6 +690,21 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
             else
                 SH_HIP(launch_pcg_coop(sds, map(kMapCoop, ns), n, map_total_[kMapCoop], int(cfg_.pcg_iterations),
                                        static_cast<PcgSlot *>(d_slots_), flags + 0, flags + 1, tile, st));
+        }
+        else if (matrix_free)
+        {
+            // ApplyJtJ without the blocks: per iteration d_k, then q = J^T W J d_k from
+            // the per-pixel records, then the dot and the update
+            SH_HIP(launch_pcg_init(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], prs, st));
+            for (uint32_t k = 0; k < cfg_.pcg_iterations; ++k)
+            {
+                SH_HIP(launch_pcg_dir(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), prs, st));
+                SH_HIP(launch_mf_tiles(K_, fp16, sds, map(kMapTile, ns), n, map_total_[kMapTile], int(k), nullptr,
+                                       st));
+                SH_HIP(launch_mf_rows(sds, map(kMapRow, ns), n, map_total_[kMapRow], int(k), nullptr, nullptr, st));
+                SH_HIP(launch_pcg_dot(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), pdq, st));
+                SH_HIP(launch_pcg_update(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), pdq, prs, st));
+            }
         }
         else
         {
@@ -788,8 +807,8 @@ int Engine::solve(uint32_t ns, hipStream_t st)
         // cost at the final warp field: the terms once more, then the reduction
         const SolveDesc *sds = static_cast<const SolveDesc *>(d_descs_);
         const int it = int(cfg_.gn_iterations);
-        SH_HIP(launch_dense_terms(K_, fp16_, sds, map(kMapPixel, ns), int(ns), map_total_[kMapPixel], it, st));
-        SH_HIP(launch_reg_terms(sds, map(kMapReg, ns), int(ns), map_total_[kMapReg], it, st));
+        SH_HIP(launch_dense_terms(K_, fp16_, sds, map(kMapPixel, ns), int(ns), map_total_[kMapPixel], it, st, 0));
+        SH_HIP(launch_reg_terms(sds, map(kMapReg, ns), int(ns), map_total_[kMapReg], it, st, 0));
         SH_HIP(launch_cost_final(sds, int(ns), it, st));
     }
     if ((rc = finish(st)) != SH_OK)
@@ -799,6 +818,31 @@ int Engine::solve(uint32_t ns, hipStream_t st)
         slots_[s].d.gn_iter = int(cfg_.gn_iterations);
         slots_[s].solved = true;
     }
+    return SH_OK;
+}
+
+// y = J^T W J x (+ damping) with the system of the slot's last GN iteration:
+// over the materialised blocks, or matrix-free from the per-pixel records.
+int Engine::apply_jtj(uint32_t slot, const float *x, float *y, bool matrix_free, hipStream_t st)
+{
+    if (slot >= slots_.size() || !slots_[slot].solved || !x || !y)
+        return !x || !y ? SH_E_ARG : SH_E_STATE;
+    if (matrix_free && (cfg_.flags & (SH_FLAG_PAIR_MAJOR | SH_FLAG_ROW_SEGMENTS)))
+        return SH_E_ARG; // the tile index is not built for those paths
+    batch_.assign(1, slot);
+    build_maps(1);
+    int rc;
+    if ((rc = upload(1, st)) != SH_OK)
+        return rc;
+    const SolveDesc *sds = static_cast<const SolveDesc *>(d_descs_);
+    if (matrix_free)
+    {
+        SH_HIP(launch_mf_tiles(K_, fp16_, sds, map(kMapTile, 1), 1, map_total_[kMapTile], 0, x, st));
+        SH_HIP(launch_mf_rows(sds, map(kMapRow, 1), 1, map_total_[kMapRow], 0, x, y, st));
+    }
+    else
+        SH_HIP(launch_spmv_rows(sds, map(kMapRow, 1), 1, map_total_[kMapRow], x, y, st));
+    SH_HIP(hipStreamSynchronize(st));
     return SH_OK;
 }
 
@@ -1051,12 +1095,9 @@ extern "C"
 
     int sh_solver_apply_jtj(sh_solver *solver, const float *x, float *y, uint32_t matrix_free, sh_stream_t stream)
     {
-        (void)solver;
-        (void)x;
-        (void)y;
-        (void)matrix_free;
-        (void)stream;
-        return SH_E_STATE;
+        if (!solver)
+            return SH_E_ARG;
+        return solver->eng.apply_jtj(0, x, y, matrix_free != 0, reinterpret_cast<hipStream_t>(stream));
     }
 
     int sh_skin_surfels(const float *surfel_vertex4, uint32_t num_surfels, const float *node_coords4,

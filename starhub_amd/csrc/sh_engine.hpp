@@ -33,6 +33,7 @@ public:
     int step(const uint32_t *slots, uint32_t n, hipStream_t st);
     int gn_iter(uint32_t slot) const { return slots_[slot].d.gn_iter; }
     int cost_history(uint32_t slot, double *out, uint32_t capacity, uint32_t *count) const;
+    int apply_jtj(uint32_t slot, const float *x, float *y, bool matrix_free, hipStream_t st);
     // device bytes held: fixed allocations plus the grown per-slot buffers
     size_t device_bytes() const
     {

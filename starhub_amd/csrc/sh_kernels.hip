@@ -374,7 +374,7 @@ __device__ __forceinline__ double block_sum_256(double v)
 
 // One pixel: residual, Jacobian, the folded H/g record; returns its cost.
 template <int K, class JT>
-__device__ __forceinline__ double dense_pixel(const SolveDesc &d, uint32_t p, int parity)
+__device__ __forceinline__ double dense_pixel(const SolveDesc &d, uint32_t p, int parity, int store)
 {
     const dq8 *__restrict__ dq = d.dq[(d.gn_iter + parity) & 1];
     uint16_t kn[K];
@@ -483,7 +483,8 @@ __device__ __forceinline__ double dense_pixel(const SolveDesc &d, uint32_t p, in
         hg[kHGg + q] = g;
     }
     hg[kHG - 1] = 0.f;
-    store_hg<JT>(d.jr, p, hg);
+    if (store)
+        store_hg<JT>(d.jr, p, hg);
     return double(d.lam2_depth) * double(out[18]) * double(out[18]) +
            double(d.lam2_flow) * (double(out[19]) * double(out[19]) + double(out[20]) * double(out[20]));
 }
@@ -492,14 +493,15 @@ __device__ __forceinline__ double dense_pixel(const SolveDesc &d, uint32_t p, in
 // DenseImageJacobianResidual.cu:164-310; forms :12-157).
 template <int K, class JT>
 __global__ __launch_bounds__(256) void k_dense_terms(const SolveDesc *__restrict__ sds,
-                                                     const uint32_t *__restrict__ wg_begin, int n_solve, int parity)
+                                                     const uint32_t *__restrict__ wg_begin, int n_solve, int parity,
+                                                     int store)
 {
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
     const SolveDesc &d = sds[si];
     const uint32_t p = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
     double pix_cost = 0.0;
     if (p < uint32_t(d.P))
-        pix_cost = dense_pixel<K, JT>(d, p, parity);
+        pix_cost = dense_pixel<K, JT>(d, p, parity, store);
     // FP64 cost partial of the work-group (fixed order)
     const double wc = block_sum_256(pix_cost);
     if (threadIdx.x == 0)
@@ -510,7 +512,8 @@ __global__ __launch_bounds__(256) void k_dense_terms(const SolveDesc *__restrict
 // ARAP forward warp per edge + node translations (forwardWarpSmootherNodeKernel,
 // NodeGraphHandler.cu:7-46; ComputeNodeMotionKernel, NodeMotionHandler.cu:9-21).
 __global__ __launch_bounds__(256) void k_reg_terms(const SolveDesc *__restrict__ sds,
-                                                   const uint32_t *__restrict__ wg_begin, int n_solve, int parity)
+                                                   const uint32_t *__restrict__ wg_begin, int n_solve, int parity,
+                                                   int store)
 {
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
     const SolveDesc &d = sds[si];
@@ -529,9 +532,12 @@ __global__ __launch_bounds__(256) void k_reg_terms(const SolveDesc *__restrict__
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         const float valid = (dx * dx + dy * dy + dz * dz > 64.f * d.node_radius_square) ? 0.f : 1.f;
         const float c = d.node_connect[size_t(i) * d.K + k];
-        float4 *o = reinterpret_cast<float4 *>(d.reg + size_t(item) * kReg);
-        o[0] = make_float4(a.x, a.y, a.z, b.x);
-        o[1] = make_float4(b.y, b.z, c, valid);
+        if (store)
+        {
+            float4 *o = reinterpret_cast<float4 *>(d.reg + size_t(item) * kReg);
+            o[0] = make_float4(a.x, a.y, a.z, b.x);
+            o[1] = make_float4(b.y, b.z, c, valid);
+        }
         if (valid != 0.f)
         {
             const double ex = double(a.x) - double(b.x), ey = double(a.y) - double(b.y),
@@ -543,7 +549,8 @@ __global__ __launch_bounds__(256) void k_reg_terms(const SolveDesc *__restrict__
     {
         const uint32_t n = item - uint32_t(d.E);
         const rt34 T = dq_se3(load_dq(dq, n));
-        d.ttrans[n] = make_float4(T.t.x, T.t.y, T.t.z, 0.f);
+        if (store)
+            d.ttrans[n] = make_float4(T.t.x, T.t.y, T.t.z, 0.f);
     }
     const double wc = block_sum_256(edge_cost);
     if (threadIdx.x == 0)
@@ -998,6 +1005,48 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_spmv(const SolveDesc *__res
         }
         d.q[row] = acc;
         prod = double(dn) * double(acc);
+    }
+    const double tot = block_sum_192(prod, sh);
+    if (threadIdx.x == 0)
+        part_dq[blockIdx.x] = tot;
+}
+
+// d_k = s + beta d_{k-1} only (the matrix-free loop applies A separately).
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_dir(const SolveDesc *__restrict__ sds,
+                                                         const uint32_t *__restrict__ wg_begin, int n_solve, int k,
+                                                         const double *__restrict__ part_rs)
+{
+    __shared__ double sh[4];
+    const int si = find_solve(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const double delta = reduce_partials(part_rs, wg_begin[si], wg_begin[si + 1], sh);
+    double beta = 0.0;
+    if (k > 0 && d.delta[k - 1] != 0.0)
+        beta = delta / d.delta[k - 1];
+    if (threadIdx.x == 0 && blockIdx.x == wg_begin[si])
+        d.delta[k] = delta;
+    const uint32_t node = (blockIdx.x - wg_begin[si]) * kPcgNodes + threadIdx.x / 6;
+    if (node < uint32_t(d.N))
+    {
+        const size_t row = size_t(node) * 6 + threadIdx.x % 6;
+        d.dv[(k + 1) & 1][row] = d_next(beta, d.dv[k & 1][row], d.s[row]);
+    }
+}
+
+// partial d_k . q
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_dot(const SolveDesc *__restrict__ sds,
+                                                         const uint32_t *__restrict__ wg_begin, int n_solve, int k,
+                                                         double *__restrict__ part_dq)
+{
+    __shared__ double sh[4];
+    const int si = find_solve(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const uint32_t node = (blockIdx.x - wg_begin[si]) * kPcgNodes + threadIdx.x / 6;
+    double prod = 0.0;
+    if (node < uint32_t(d.N))
+    {
+        const size_t row = size_t(node) * 6 + threadIdx.x % 6;
+        prod = double(d.dv[(k + 1) & 1][row]) * double(d.q[row]);
     }
     const double tot = block_sum_192(prod, sh);
     if (threadIdx.x == 0)
@@ -1496,33 +1545,33 @@ hipError_t launch_identity_dq(dq8 *dq, uint32_t n_nodes, hipStream_t st)
 }
 
 hipError_t launch_dense_terms(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve,
-                              uint32_t n_wg, int parity, hipStream_t st)
+                              uint32_t n_wg, int parity, hipStream_t st, int store)
 {
     if (n_wg == 0)
         return hipSuccess;
     if (K == 8)
     {
         if (fp16)
-            hipLaunchKernelGGL((k_dense_terms<8, __half>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity);
+            hipLaunchKernelGGL((k_dense_terms<8, __half>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity, store);
         else
-            hipLaunchKernelGGL((k_dense_terms<8, float>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity);
+            hipLaunchKernelGGL((k_dense_terms<8, float>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity, store);
     }
     else
     {
         if (fp16)
-            hipLaunchKernelGGL((k_dense_terms<4, __half>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity);
+            hipLaunchKernelGGL((k_dense_terms<4, __half>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity, store);
         else
-            hipLaunchKernelGGL((k_dense_terms<4, float>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity);
+            hipLaunchKernelGGL((k_dense_terms<4, float>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity, store);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_reg_terms(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int parity,
-                            hipStream_t st)
+                            hipStream_t st, int store)
 {
     if (n_wg == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_reg_terms, dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity);
+    hipLaunchKernelGGL(k_reg_terms, dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity, store);
     return hipGetLastError();
 }
 
@@ -1596,6 +1645,24 @@ hipError_t launch_pcg_update(const SolveDesc *sds, const uint32_t *wmap, int n_s
                              const double *part_dq, double *part_rs, hipStream_t st)
 {
     hipLaunchKernelGGL(k_pcg_update, dim3(n_wg), dim3(kPcgThreads), 0, st, sds, wmap, n_solve, k, part_dq, part_rs);
+    return hipGetLastError();
+}
+
+hipError_t launch_pcg_dir(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int k,
+                          const double *part_rs, hipStream_t st)
+{
+    if (n_wg == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_pcg_dir, dim3(n_wg), dim3(kPcgThreads), 0, st, sds, wmap, n_solve, k, part_rs);
+    return hipGetLastError();
+}
+
+hipError_t launch_pcg_dot(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int k,
+                          double *part_dq, hipStream_t st)
+{
+    if (n_wg == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_pcg_dot, dim3(n_wg), dim3(kPcgThreads), 0, st, sds, wmap, n_solve, k, part_dq);
     return hipGetLastError();
 }
 

@@ -27,10 +27,11 @@ hipError_t launch_row_ptr(const SolveDesc &d, const uint32_t *skeys, hipStream_t
 hipError_t launch_identity(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, hipStream_t st);
 hipError_t launch_identity_dq(dq8 *dq, uint32_t n_nodes, hipStream_t st);
 hipError_t launch_cost_final(const SolveDesc *sds, int n_solve, int slot, hipStream_t st);
+// store = 0: only the cost partials (the records of the last linearisation stay)
 hipError_t launch_dense_terms(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve,
-                              uint32_t n_wg, int parity, hipStream_t st);
+                              uint32_t n_wg, int parity, hipStream_t st, int store = 1);
 hipError_t launch_reg_terms(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int parity,
-                            hipStream_t st);
+                            hipStream_t st, int store = 1);
 hipError_t launch_diag_rhs(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
                            hipStream_t st);
 hipError_t launch_dinv(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, hipStream_t st);
@@ -42,6 +43,10 @@ hipError_t launch_pcg_spmv(const SolveDesc *sds, const uint32_t *wmap, int n_sol
                            const double *part_rs, double *part_dq, hipStream_t st);
 hipError_t launch_pcg_update(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int k,
                              const double *part_dq, double *part_rs, hipStream_t st);
+hipError_t launch_pcg_dir(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int k,
+                          const double *part_rs, hipStream_t st);
+hipError_t launch_pcg_dot(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int k,
+                          double *part_dq, hipStream_t st);
 hipError_t launch_update(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int parity,
                          hipStream_t st);
 hipError_t launch_skin(uint32_t K, const float4 *surfels, uint32_t S, const float4 *nodes, uint32_t N, float r2,
@@ -108,6 +113,13 @@ hipError_t launch_lower_bound_u32(const uint32_t *keys, uint32_t n_keys, uint32_
 hipError_t launch_tiles(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
                         hipStream_t st);
 size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len);
+// matrix-free J^T W J x (x = d_{k+1} of each solve unless x_user; y = d.q unless y_user)
+hipError_t launch_mf_tiles(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
+                           int k_it, const float *x_user, hipStream_t st);
+hipError_t launch_mf_rows(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int k_it,
+                          const float *x_user, float *y_user, hipStream_t st);
+hipError_t launch_spmv_rows(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, const float *x,
+                            float *y, hipStream_t st);
 hipError_t launch_tiles_combine(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, size_t lds,
                                 hipStream_t st);
 

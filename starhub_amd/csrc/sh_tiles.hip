@@ -554,6 +554,208 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc 
         d.rhs[size_t(i) * 6 + tid] = -(red[tid] + red[8 + tid]);
 }
 
+// ------------------------------------------------------------ matrix-free J^T W J x
+// ApplyJtJ without the blocks (the reference names it in its sanity checker,
+// JtJMaterializerSanityCheck.cu:47-79, but has no implementation): per pixel
+// z = sum_k w_k x_k over its distinct nodes and u = H z (H = sum_c lam_c^2
+// J_c^T J_c, the same record the materialisation reads); each diagonal slot of
+// a tile (node i, its pixels in order) sums w_i u; the row kernel adds node
+// i's diagonal-slot partials in tile order, the ARAP terms J_i^T (J_i x_i +
+// J_j x_j), the node-translation term and the damping.  x is the solve's
+// next PCG direction d_{k+1} (dv[(k+1)&1]) unless x_user is given.
+template <int K, class JT>
+__global__ __launch_bounds__(kTileThreads) void k_mf_tiles(const SolveDesc *__restrict__ sds,
+                                                           const uint32_t *__restrict__ wg_begin, int n_solve, int k_it,
+                                                           const float *__restrict__ x_user)
+{
+    using G = TileGeom<K>;
+    __shared__ float uS[kTilePix * 6];
+    __shared__ float wS[kTilePix * K];
+    __shared__ uint16_t ent[kTilePix * G::kPairs];
+    __shared__ uint16_t sst[kTilePix * G::kPairs + 1];
+    const int si = find_solve_t(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const uint32_t t = blockIdx.x - wg_begin[si];
+    if (t >= uint32_t(d.n_tiles))
+        return; // uniform over the group
+    const float *__restrict__ x = x_user ? x_user : d.dv[(k_it + 1) & 1];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t p0 = t * kTilePix;
+    const uint32_t np = min(uint32_t(kTilePix), uint32_t(d.P) - p0);
+    if (tid < np)
+    {
+        const uint32_t p = p0 + tid;
+        uint16_t kn[K];
+        float w[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+        {
+            kn[k] = d.pknn[size_t(p) * K + k];
+            w[k] = d.pw[size_t(p) * K + k];
+            wS[tid * K + k] = w[k];
+        }
+        float z[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+        {
+            bool act = true; // last matching slot of its node
+#pragma unroll
+            for (int k2 = k + 1; k2 < K; ++k2)
+                act = act && kn[k2] != kn[k];
+            if (act)
+            {
+                const float *xv = x + size_t(kn[k]) * 6;
+#pragma unroll
+                for (int q = 0; q < 6; ++q)
+                    z[q] = fmaf(w[k], xv[q], z[q]);
+            }
+        }
+        float hg[kHG];
+        load_hg<JT>(d.jr, p, hg);
+#pragma unroll
+        for (int a = 0; a < 6; ++a)
+        {
+            float u = 0.f;
+#pragma unroll
+            for (int b = 0; b < 6; ++b)
+            {
+                int e;
+                if (a <= b)
+                    e = a * 6 - a * (a - 1) / 2 + (b - a);
+                else
+                    e = b * 6 - b * (b - 1) / 2 + (a - b);
+                u = fmaf(hg[e], z[b], u);
+            }
+            uS[tid * 6 + a] = u;
+        }
+    }
+    const uint32_t c0 = d.tile_cbase[t], nc = d.tile_cbase[t + 1] - c0;
+    const uint32_t s0 = d.tile_sbase[t], ns = d.tile_sbase[t + 1] - s0;
+    for (uint32_t i = tid; i < nc; i += kTileThreads)
+        ent[i] = d.tc_contrib[c0 + i];
+    for (uint32_t s = tid; s <= ns; s += kTileThreads)
+        sst[s] = uint16_t(d.tc_slot_first[s0 + s] - c0);
+    __syncthreads();
+    for (uint32_t task = tid; task < ns * 6; task += kTileThreads)
+    {
+        const uint32_t s = task / 6, comp = task % 6;
+        const uint32_t e0 = sst[s], e1 = sst[s + 1];
+        const uint32_t first = ent[e0];
+        if (((first >> 7) & 7u) != ((first >> 10) & 7u))
+            continue; // off-diagonal slot: not needed
+        float acc = 0.f;
+        for (uint32_t e = e0; e < e1; ++e)
+        {
+            const uint32_t c = ent[e];
+            const uint32_t pl = c & 127u, ka = (c >> 7) & 7u;
+            acc = fmaf(wS[pl * K + ka], uS[pl * 6 + comp], acc);
+        }
+        d.tpart[size_t(s0 + s) * kTileRec + comp] = acc;
+    }
+}
+
+// One wave per node row: y_i = damping x_i + diagonal-slot partials (tile
+// order) + ARAP + node translation.  y is d.q unless y_user is given.
+__global__ __launch_bounds__(64) void k_mf_rows(const SolveDesc *__restrict__ sds,
+                                                const uint32_t *__restrict__ wg_begin, int n_solve, int k_it,
+                                                const float *__restrict__ x_user, float *__restrict__ y_user)
+{
+    const int si = find_solve_t(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const uint32_t i = blockIdx.x - wg_begin[si];
+    if (i >= uint32_t(d.N))
+        return;
+    const float *__restrict__ x = x_user ? x_user : d.dv[(k_it + 1) & 1];
+    float *__restrict__ y = y_user ? y_user : d.q;
+    const int lane = threadIdx.x;
+    const int a = lane < 6 ? lane : 0;
+    const uint32_t base = d.row_ptr[i] + i;
+    float xi[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+        xi[q] = x[size_t(i) * 6 + q];
+    // dense: diagonal-slot partials of block (i, i)
+    float dense = 0.f;
+    const uint32_t l1 = d.list_off[base + 1];
+    for (uint32_t l = d.list_off[base]; l < l1; ++l)
+        dense += d.tpart[size_t(d.list_items[l]) * kTileRec + a];
+    // ARAP and node translation, in list order
+    float reg = 0.f;
+    const uint32_t dend = d.n2t_off[i] + d.dense_cnt[i], end = d.n2t_off[i + 1];
+    const uint32_t P = d.P, E = d.E;
+    for (uint32_t it = dend; it < end; ++it)
+    {
+        const uint32_t t = d.n2t_terms[it];
+        if (t < P + E)
+        {
+            const uint32_t e = t - P;
+            const float4 *rec = reinterpret_cast<const float4 *>(d.reg + size_t(e) * kReg);
+            const float4 u0 = rec[0], u1 = rec[1];
+            if (u1.w == 0.f)
+                continue;
+            const f3 pa = mk3(u0.x, u0.y, u0.z), pb = mk3(u0.w, u1.x, u1.y);
+            const float cw = u1.z;
+            const uint32_t ex = d.edges[3 * e], ey = d.edges[3 * e + 1];
+            const bool is_x = (ex == i);
+            const uint32_t other = is_x ? ey : ex;
+            const float ws = d.lam2_reg * (cw * cw);
+            const f3 pm = is_x ? pa : pb, po = is_x ? pb : pa;
+            const float sm = is_x ? 1.f : -1.f;
+            float xo[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                xo[q] = x[size_t(other) * 6 + q];
+            float v = 0.f;
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch)
+            {
+                float rc = 0.f; // (J_m x_i + J_o x_o)[ch]
+#pragma unroll
+                for (int b = 0; b < 6; ++b)
+                    rc += (sm * reg_elem_t(pm, ch, b)) * xi[b] + (-sm * reg_elem_t(po, ch, b)) * xo[b];
+                v += (sm * reg_elem_t(pm, ch, a)) * rc;
+            }
+            reg += ws * v;
+        }
+        else
+        {
+            const uint32_t m = t - P - E;
+            const float4 pred = d.node_pred ? d.node_pred[m] : make_float4(0.f, 0.f, 0.f, 0.f);
+            if (a >= 3)
+                reg += d.lam2_nt * (pred.w * pred.w) * xi[a];
+        }
+    }
+    if (lane < 6)
+        y[size_t(i) * 6 + a] = d.damping * xi[a] + dense + reg;
+}
+
+// Materialised y = A x over the blocks of each row (one wave per row), for
+// sh_solver_apply_jtj.
+__global__ __launch_bounds__(64) void k_spmv_rows(const SolveDesc *__restrict__ sds,
+                                                  const uint32_t *__restrict__ wg_begin, int n_solve,
+                                                  const float *__restrict__ x, float *__restrict__ y)
+{
+    const int si = find_solve_t(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const uint32_t i = blockIdx.x - wg_begin[si];
+    if (i >= uint32_t(d.N))
+        return;
+    const int lane = threadIdx.x;
+    const int a = lane < 6 ? lane : 0;
+    float acc = 0.f;
+    const uint32_t b0 = d.row_ptr[i] + i, b1 = d.row_ptr[i + 1] + i + 1;
+    for (uint32_t bb = b0; bb < b1; ++bb)
+    {
+        const uint32_t col = d.blk_col[bb];
+        const float *A = d.blk + size_t(bb) * 36 + 6 * a;
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            acc = fmaf(A[c], x[size_t(col) * 6 + c], acc);
+    }
+    if (lane < 6)
+        y[size_t(i) * 6 + a] = acc;
+}
+
 // ------------------------------------------------------------ launchers
 hipError_t launch_tile_build(const SolveDesc &d, uint16_t *ws_contrib, uint32_t *ws_slot_block,
                              uint16_t *ws_slot_first, uint32_t *tile_nslot, uint32_t *tile_ncontrib, hipStream_t st)
@@ -620,6 +822,50 @@ hipError_t launch_tiles(int K, bool fp16, const SolveDesc *sds, const uint32_t *
         else
             hipLaunchKernelGGL((k_tiles<4, float>), dim3(n_wg), dim3(kTileThreads), 0, st, sds, wmap, n_solve);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_mf_tiles(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
+                           int k_it, const float *x_user, hipStream_t st)
+{
+    if (n_wg == 0)
+        return hipSuccess;
+    if (K == 8)
+    {
+        if (fp16)
+            hipLaunchKernelGGL((k_mf_tiles<8, __half>), dim3(n_wg), dim3(kTileThreads), 0, st, sds, wmap, n_solve,
+                               k_it, x_user);
+        else
+            hipLaunchKernelGGL((k_mf_tiles<8, float>), dim3(n_wg), dim3(kTileThreads), 0, st, sds, wmap, n_solve,
+                               k_it, x_user);
+    }
+    else
+    {
+        if (fp16)
+            hipLaunchKernelGGL((k_mf_tiles<4, __half>), dim3(n_wg), dim3(kTileThreads), 0, st, sds, wmap, n_solve,
+                               k_it, x_user);
+        else
+            hipLaunchKernelGGL((k_mf_tiles<4, float>), dim3(n_wg), dim3(kTileThreads), 0, st, sds, wmap, n_solve,
+                               k_it, x_user);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_mf_rows(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int k_it,
+                          const float *x_user, float *y_user, hipStream_t st)
+{
+    if (n_wg == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_mf_rows, dim3(n_wg), dim3(64), 0, st, sds, wmap, n_solve, k_it, x_user, y_user);
+    return hipGetLastError();
+}
+
+hipError_t launch_spmv_rows(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, const float *x,
+                            float *y, hipStream_t st)
+{
+    if (n_wg == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_spmv_rows, dim3(n_wg), dim3(64), 0, st, sds, wmap, n_solve, x, y);
     return hipGetLastError();
 }
 

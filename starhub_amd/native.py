@@ -246,6 +246,14 @@ class WarpSolver:
               f"debug_view({name})")
         return _device_copy(view.value, cnt.value, dtype)
 
+    def apply_jtj(self, x, matrix_free: bool):
+        """y = J^T W J x (+ damping) with the last GN iteration's system; x: 6N float32 device tensor."""
+        import torch
+        y = torch.empty_like(x)
+        check(lib().sh_solver_apply_jtj(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                        int(matrix_free), ctypes.c_void_p()), "sh_solver_apply_jtj")
+        return y
+
     def cost_history(self) -> np.ndarray:
         """FP64 weighted cost at the start of every GN iteration (+ the final one with SH_FLAG_COST_HISTORY)."""
         n = ctypes.c_uint32()
