@@ -99,6 +99,12 @@ KERNEL_OF_PHASE = {"dense_terms": "k_dense_terms", "reg_terms": "k_reg_terms", "
                    "update": "k_update", "tiles": "k_tiles", "tiles_combine": "k_tiles_combine"}
 
 
+def pcg_kernel(tuning: dict) -> str:
+    if tuning["spmv_mode"] == 1 or tuning["flags"] & 4:
+        return "k_pcg_spmv/k_pcg_update"
+    return "k_pcg_coop" if tuning["flags"] & 8 else "k_pcg_resident"
+
+
 def measured_traffic(kernel: str, workload: str):
     """HBM bytes per launch from a committed rocprofv3 counter summary, if any."""
     f = ROOT / "profiles" / f"{kernel}_{workload}_traffic.json"
@@ -160,6 +166,7 @@ def batched_leg(native, scene, workload):
     b = native.BatchSolver(native.make_config(cfgs[0].solver, W, H, N, cfgs[0].K), len(cfgs))
     for i, f in enumerate(frames):
         b.set_inputs(i, f)
+    tuning = native.autotune(b, batch_solves=len(cfgs), repeats=2)
     b.solve(len(cfgs))
     reps = 5
     t0 = _t.perf_counter()
@@ -171,7 +178,7 @@ def batched_leg(native, scene, workload):
     plain = plain_rate(frames, cfgs, native)
     return {"workload": "32 solves, N log-uniform [128, 4096], 300 surfels/node, 320x240..1280x720, "
                         f"{cfgs[0].solver.gn_iterations} GN x {cfgs[0].solver.pcg_iterations} PCG",
-            "solves_per_s": batched, "plain_solves_per_s": plain, "vs_plain": batched / plain}
+            "solves_per_s": batched, "plain_solves_per_s": plain, "vs_plain": batched / plain, "tuning": tuning}
 
 
 def served_leg(native, scene, workload, n_requests: int):
@@ -247,6 +254,7 @@ def main():
     frame, prod = workload.device_frame(sc)
     solver = workload.solver_for(sc, flags=native.SH_FLAG_PHASE_TIMING)
     solver.set_inputs(frame)
+    tuning = native.autotune(solver, repeats=3)  # fastest variant on this frame, kept for the timed steps
     for _ in range(args.warmup):
         solver.solve()
     torch.cuda.synchronize()
@@ -282,6 +290,7 @@ def main():
                                f"{cfg.solver.pcg_iterations} PCG",
                    "valid_pixels": P, "node_pairs": U},
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        "tuning": tuning,
     }
 
     # roofline of the kernel with the largest device-time share
@@ -290,7 +299,7 @@ def main():
     avg_s = phases[dom][0] / phases[dom][1] / 1e3
     nbytes = phase_bytes(dom, cfg, P, U)
     achieved = nbytes / avg_s / 1e9
-    kernel = KERNEL_OF_PHASE.get(dom, dom)
+    kernel = pcg_kernel(tuning) if dom == "pcg" else KERNEL_OF_PHASE.get(dom, dom)
     traffic, tsrc = measured_traffic(kernel, args.config)
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,

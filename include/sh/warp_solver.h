@@ -303,6 +303,24 @@ int sh_server_warp_field(sh_server *server, uint32_t stream_id, const sh_dual_qu
                          uint32_t *num_nodes);
 int sh_server_get_stats(sh_server *server, sh_server_stats *stats);
 
+/* ------------------------------------------------------------ autotuner */
+/* No counterpart in the reference (its launch shapes are compile-time).  A
+ * tuning is what can change between solves without reallocating. */
+typedef struct sh_tuning
+{
+    uint32_t flags;              /* subset of SH_FLAG_ROW_SEGMENTS | SH_FLAG_STREAMING_PCG | SH_FLAG_NO_PERSISTENT_PCG */
+    uint32_t spmv_mode;          /* sh_spmv_mode */
+    uint32_t pcg_rows_per_group; /* register-resident PCG: node rows per work-group, 1..85 (0 = 85) */
+    double ms;                   /* autotune: median solve time of the chosen tuning */
+} sh_tuning;
+
+/* Times each candidate tuning on the inputs already set (median of `repeats`
+ * solves after one warm-up, HIP events on `stream`), keeps the fastest and
+ * returns it.  Every candidate solves the same problem to the same tolerance. */
+int sh_solver_autotune(sh_solver *solver, sh_stream_t stream, uint32_t repeats, sh_tuning *best);
+int sh_solver_set_tuning(sh_solver *solver, const sh_tuning *tuning);
+int sh_batch_autotune(sh_batch *batch, uint32_t num_solves, sh_stream_t stream, uint32_t repeats, sh_tuning *best);
+
 /* ------------------------------------------------------- measurement hook */
 /* No counterpart in the reference (which times nothing on the device).  With
  * SH_FLAG_PHASE_TIMING in the config: the event-to-event time (ms) and the

@@ -519,11 +519,11 @@ int Engine::size_partials(Slot &sl)
 }
 
 // Groups of the register-resident PCG for one solve: rows split evenly, at
-// most kPcgResRows rows and kPcgResBlocks blocks (rows x longest row) each;
-// 0 when that takes more than kPcgMaxGroups groups.
-static uint32_t resident_groups(int n, uint32_t max_row_len)
+// most `max_rows` (<= kPcgResRows) rows and kPcgResBlocks blocks (rows x
+// longest row) each; 0 when that takes more than kPcgMaxGroups groups.
+static uint32_t resident_groups(int n, uint32_t max_row_len, uint32_t max_rows)
 {
-    const uint32_t rows = std::min(kPcgResRows, kPcgResBlocks / (max_row_len + 1));
+    const uint32_t rows = std::min(std::min(max_rows, kPcgResRows), kPcgResBlocks / (max_row_len + 1));
     if (rows == 0 || n <= 0)
         return n <= 0 && rows > 0 ? 1u : 0u;
     const uint32_t g = div_up(uint32_t(n), rows);
@@ -535,7 +535,7 @@ void Engine::build_maps(uint32_t ns)
     maps_.assign(size_t(kNumMaps) * (ns + 1), 0u);
     coop_resident_ = !(cfg_.flags & SH_FLAG_STREAMING_PCG);
     for (uint32_t s = 0; s < ns; ++s)
-        if (resident_groups(slots_[batch_[s]].d.N, slots_[batch_[s]].max_row_len) == 0)
+        if (resident_groups(slots_[batch_[s]].d.N, slots_[batch_[s]].max_row_len, res_rows_) == 0)
             coop_resident_ = false;
     for (int m = 0; m < kNumMaps; ++m)
     {
@@ -568,7 +568,7 @@ void Engine::build_maps(uint32_t ns)
                 wg = uint32_t(d.N);
                 break;
             case kMapCoop:
-                wg = coop_resident_ ? resident_groups(d.N, slots_[batch_[s]].max_row_len)
+                wg = coop_resident_ ? resident_groups(d.N, slots_[batch_[s]].max_row_len, res_rows_)
                                     : std::min(kPcgMaxGroups, std::max(1u, div_up(uint32_t(d.N) + 2u * uint32_t(d.U),
                                                                                   kPcgBlocksPerGroup)));
                 break;
@@ -846,6 +846,90 @@ int Engine::apply_jtj(uint32_t slot, const float *x, float *y, bool matrix_free,
     return SH_OK;
 }
 
+// Tuning that can change between solves without reallocating: the
+// materialisation path, the PCG variant and the resident PCG's rows per group.
+static constexpr uint32_t kTunableFlags = SH_FLAG_ROW_SEGMENTS | SH_FLAG_STREAMING_PCG | SH_FLAG_NO_PERSISTENT_PCG;
+
+int Engine::set_tuning(const sh_tuning &t)
+{
+    if ((t.flags & ~kTunableFlags) || t.spmv_mode > SH_SPMV_AUTO || t.pcg_rows_per_group > kPcgResRows ||
+        (cfg_.flags & (SH_FLAG_PAIR_MAJOR)) ||
+        (t.spmv_mode == SH_SPMV_MATRIX_FREE && (t.flags & SH_FLAG_ROW_SEGMENTS)))
+        return SH_E_ARG;
+    cfg_.flags = (cfg_.flags & ~kTunableFlags) | t.flags;
+    cfg_.spmv_mode = t.spmv_mode;
+    res_rows_ = t.pcg_rows_per_group ? t.pcg_rows_per_group : kPcgResRows;
+    return SH_OK;
+}
+
+sh_tuning Engine::tuning() const
+{
+    sh_tuning t{};
+    t.flags = cfg_.flags & kTunableFlags;
+    t.spmv_mode = cfg_.spmv_mode;
+    t.pcg_rows_per_group = res_rows_;
+    return t;
+}
+
+// Times every candidate tuning on the slots' current inputs (median of
+// `repeats` solves each, HIP events on `st`) and keeps the fastest.
+int Engine::autotune(uint32_t ns, hipStream_t st, uint32_t repeats, sh_tuning *best_out)
+{
+    if (cfg_.flags & SH_FLAG_PAIR_MAJOR)
+        return SH_E_ARG;
+    std::vector<sh_tuning> cands;
+    for (uint32_t rows : {85u, 64u, 48u, 32u})
+        cands.push_back({0u, SH_SPMV_MATERIALIZED, rows, 0.0});
+    cands.push_back({SH_FLAG_STREAMING_PCG, SH_SPMV_MATERIALIZED, kPcgResRows, 0.0});
+    cands.push_back({SH_FLAG_NO_PERSISTENT_PCG, SH_SPMV_MATERIALIZED, kPcgResRows, 0.0});
+    cands.push_back({SH_FLAG_ROW_SEGMENTS, SH_SPMV_MATERIALIZED, kPcgResRows, 0.0});
+    cands.push_back({0u, SH_SPMV_MATRIX_FREE, kPcgResRows, 0.0});
+    hipEvent_t e0, e1;
+    SH_HIP(hipEventCreate(&e0));
+    SH_HIP(hipEventCreate(&e1));
+    const sh_tuning keep = tuning();
+    sh_tuning best{};
+    best.ms = 1e30;
+    int rc = SH_OK;
+    for (sh_tuning &c : cands)
+    {
+        if ((rc = set_tuning(c)) != SH_OK)
+            break;
+        std::vector<double> t;
+        for (uint32_t r = 0; r < repeats + 1 && rc == SH_OK; ++r)
+        {
+            float ms = 0.f;
+            if (hipEventRecord(e0, st) != hipSuccess)
+                rc = SH_E_HIP;
+            if (rc == SH_OK)
+                rc = solve(ns, st);
+            if (rc == SH_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = SH_E_HIP;
+            if (r > 0) // the first run warms up
+                t.push_back(ms);
+        }
+        if (rc != SH_OK)
+            break;
+        std::sort(t.begin(), t.end());
+        c.ms = t[t.size() / 2];
+        if (c.ms < best.ms)
+            best = c;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != SH_OK)
+    {
+        (void)set_tuning(keep);
+        return rc;
+    }
+    if ((rc = set_tuning(best)) != SH_OK)
+        return rc;
+    if (best_out)
+        *best_out = best;
+    return SH_OK;
+}
+
 // FP64 cost history of a solved slot (host copy).
 int Engine::cost_history(uint32_t slot, double *out, uint32_t capacity, uint32_t *count) const
 {
@@ -1091,6 +1175,28 @@ extern "C"
         if (!solver)
             return SH_E_ARG;
         return solver->eng.cost_history(0, out, capacity, count);
+    }
+
+    int sh_solver_autotune(sh_solver *solver, sh_stream_t stream, uint32_t repeats, sh_tuning *best)
+    {
+        if (!solver || repeats == 0)
+            return SH_E_ARG;
+        return solver->eng.autotune(1, reinterpret_cast<hipStream_t>(stream), repeats, best);
+    }
+
+    int sh_solver_set_tuning(sh_solver *solver, const sh_tuning *tuning)
+    {
+        if (!solver || !tuning)
+            return SH_E_ARG;
+        return solver->eng.set_tuning(*tuning);
+    }
+
+    int sh_batch_autotune(sh_batch *batch, uint32_t num_solves, sh_stream_t stream, uint32_t repeats,
+                          sh_tuning *best)
+    {
+        if (!batch || repeats == 0 || num_solves == 0 || num_solves > batch->eng.num_slots())
+            return SH_E_ARG;
+        return batch->eng.autotune(num_solves, reinterpret_cast<hipStream_t>(stream), repeats, best);
     }
 
     int sh_solver_apply_jtj(sh_solver *solver, const float *x, float *y, uint32_t matrix_free, sh_stream_t stream)

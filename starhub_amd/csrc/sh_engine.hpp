@@ -34,6 +34,9 @@ public:
     int gn_iter(uint32_t slot) const { return slots_[slot].d.gn_iter; }
     int cost_history(uint32_t slot, double *out, uint32_t capacity, uint32_t *count) const;
     int apply_jtj(uint32_t slot, const float *x, float *y, bool matrix_free, hipStream_t st);
+    int set_tuning(const sh_tuning &t);
+    sh_tuning tuning() const;
+    int autotune(uint32_t ns, hipStream_t st, uint32_t repeats, sh_tuning *best);
     // device bytes held: fixed allocations plus the grown per-slot buffers
     size_t device_bytes() const
     {
@@ -130,6 +133,7 @@ private:
     size_t coop_lds_ = 0;
     bool coop_resident_ = false; // every solve of the launch fits the register-resident PCG
     int res_max_groups_ = -1;
+    uint32_t res_rows_ = 85; // resident-PCG rows per group (tunable, <= kPcgResRows)
     void *d_slots_ = nullptr; // persistent-PCG barrier slots
     void *d_flags_ = nullptr; // [0] PCG epoch base, [1] PCG barrier error
     void *sort_temp_ = nullptr;

@@ -319,6 +319,22 @@ class BatchSolver:
         return _phase_times(lib().sh_batch_phase_times, self._h, reset)
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("flags", ctypes.c_uint32), ("spmv_mode", ctypes.c_uint32), ("pcg_rows_per_group", ctypes.c_uint32),
+                ("ms", ctypes.c_double)]
+
+
+def autotune(handle, batch_solves: int | None = None, repeats: int = 3) -> dict:
+    """Runs the autotuner on a WarpSolver (or a BatchSolver with batch_solves) and returns the chosen tuning."""
+    t = Tuning()
+    if batch_solves is None:
+        check(lib().sh_solver_autotune(handle._h, ctypes.c_void_p(), repeats, ctypes.byref(t)), "sh_solver_autotune")
+    else:
+        check(lib().sh_batch_autotune(handle._h, batch_solves, ctypes.c_void_p(), repeats, ctypes.byref(t)),
+              "sh_batch_autotune")
+    return {"flags": t.flags, "spmv_mode": t.spmv_mode, "pcg_rows_per_group": t.pcg_rows_per_group, "ms": t.ms}
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("solver", SolverConfig), ("max_active", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64)]
 

@@ -431,3 +431,24 @@ def test_matrix_free_pcg_solve(torch_dev):
     err = workload.warp_error(gpu, ref["dq"].reshape(-1, 8))
     say("matrix-free PCG warp error vs fp64", err, "straddles", straddles(ref))
     assert err <= tolerance(ref)
+
+
+def test_autotune_picks_a_valid_tuning(torch_dev):
+    """The autotuner times its candidates on the frame, keeps the fastest, and
+    the tuned solver still meets the FP64 tolerance."""
+    import ctypes
+    import sho
+    from starhub_amd import native, scene, workload
+    sc = scene.generate(scene.config_c1())
+    frame, prod = workload.device_frame(sc)
+    solver = workload.solver_for(sc)
+    solver.set_inputs(frame)
+    best = native.autotune(solver, repeats=2)
+    say("autotune", best)
+    assert best["ms"] > 0 and 1 <= best["pcg_rows_per_group"] <= 85
+    solver.solve()
+    gpu = workload.device_dq_to_wxyz(solver.node_se3())
+    ref = sho.solve(workload.host_inputs(sc, prod), sc.cfg.solver, 64, sho.max_threads())
+    assert workload.warp_error(gpu, ref["dq"].reshape(-1, 8)) <= tolerance(ref)
+    bad = native.Tuning(native.SH_FLAG_REFERENCE_INDEX, 0, 85, 0.0)
+    assert native.lib().sh_solver_set_tuning(solver._h, ctypes.byref(bad)) == 1  # SH_E_ARG
