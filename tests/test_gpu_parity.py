@@ -452,3 +452,30 @@ def test_autotune_picks_a_valid_tuning(torch_dev):
     assert workload.warp_error(gpu, ref["dq"].reshape(-1, 8)) <= tolerance(ref)
     bad = native.Tuning(native.SH_FLAG_REFERENCE_INDEX, 0, 85, 0.0)
     assert native.lib().sh_solver_set_tuning(solver._h, ctypes.byref(bad)) == 1  # SH_E_ARG
+
+
+def test_c3_fp16_full_size_per_iteration(torch_dev):
+    """The mixed-precision workload at full size (1.2M surfels, 4096 nodes,
+    1280x720, FP16 record streams, 10 GN x 30 PCG): the warp-field error
+    against the FP64 oracle after every GN iteration (the GPU solved with
+    gn_iterations = k, the oracle's state after k iterations)."""
+    import sho
+    from starhub_amd import native, scene, workload
+    cfg = scene.config_c3()
+    sc = scene.generate(cfg)
+    frame, prod = workload.device_frame(sc)
+    ref = sho.solve(workload.host_inputs(sc, prod), cfg.solver, 64, sho.max_threads())
+    N = cfg.num_nodes
+    states = ref["dq_iter"].reshape(-1, N, 8)
+    errs = []
+    for k in range(1, cfg.solver.gn_iterations + 1):
+        params = replace(cfg.solver, gn_iterations=k)
+        c = native.make_config(params, cfg.width, cfg.height, N, cfg.K)
+        s = native.WarpSolver(c)
+        s.process_frame(frame)
+        gpu = workload.device_dq_to_wxyz(s.node_se3())
+        s.close()
+        errs.append(workload.warp_error(gpu, states[k]))
+        near = {"twist_l1": ref["twist_l1"][: k * N]}
+        assert errs[-1] <= tolerance(near), (k, errs[-1])
+    say("c3 fp16 warp error per GN iteration:", ["%.2e" % e for e in errs], "straddles", straddles(ref))
