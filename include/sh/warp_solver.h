@@ -72,7 +72,10 @@ typedef enum sh_solver_flags
     SH_FLAG_ROW_SEGMENTS = 32u,
     /* Also evaluate the cost at the final warp field (one more term pass), so the
      * cost history has gn_iterations + 1 entries. */
-    SH_FLAG_COST_HISTORY = 64u
+    SH_FLAG_COST_HISTORY = 64u,
+    /* Issue the split / matrix-free PCG launches one by one instead of replaying them
+     * from a captured hipGraph. */
+    SH_FLAG_NO_GRAPH = 128u
 } sh_solver_flags;
 
 /* Dual quaternion as stored by the reference: two float4 (x, y, z, w),

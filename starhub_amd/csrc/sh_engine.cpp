@@ -101,7 +101,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         return SH_E_ARG;
     if (c.pcg_iterations == 0 || c.pcg_iterations > 1000 || c.gn_iterations > 1000)
         return SH_E_ARG;
-    if (c.stream_precision > SH_STREAM_FP16 || c.spmv_mode > SH_SPMV_AUTO || (c.flags & ~127u))
+    if (c.stream_precision > SH_STREAM_FP16 || c.spmv_mode > SH_SPMV_AUTO || (c.flags & ~255u))
         return SH_E_ARG;
     cfg_ = c;
     if (cfg_.flags & SH_FLAG_PAIR_MAJOR)
@@ -614,7 +614,6 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
     const bool fp16 = fp16_;
     if (identity)
         SH_HIP(launch_identity(sds, map(kMapNode, ns), n, map_total_[kMapNode], st));
-    double *prs = static_cast<double *>(d_part_rs_), *pdq = static_cast<double *>(d_part_dq_);
     uint32_t max_n = 0, max_l = 0, tile = 1;
     const uint32_t *coop_begin = maps_.data() + size_t(kMapCoop) * (ns + 1);
     for (uint32_t s = 0; s < ns; ++s)
@@ -691,35 +690,88 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
                 SH_HIP(launch_pcg_coop(sds, map(kMapCoop, ns), n, map_total_[kMapCoop], int(cfg_.pcg_iterations),
                                        static_cast<PcgSlot *>(d_slots_), flags + 0, flags + 1, tile, st));
         }
-        else if (matrix_free)
-        {
-            // ApplyJtJ without the blocks: per iteration d_k, then q = J^T W J d_k from
-            // the per-pixel records, then the dot and the update
-            SH_HIP(launch_pcg_init(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], prs, st));
-            for (uint32_t k = 0; k < cfg_.pcg_iterations; ++k)
-            {
-                SH_HIP(launch_pcg_dir(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), prs, st));
-                SH_HIP(launch_mf_tiles(K_, fp16, sds, map(kMapTile, ns), n, map_total_[kMapTile], int(k), nullptr,
-                                       st));
-                SH_HIP(launch_mf_rows(sds, map(kMapRow, ns), n, map_total_[kMapRow], int(k), nullptr, nullptr, st));
-                SH_HIP(launch_pcg_dot(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), pdq, st));
-                SH_HIP(launch_pcg_update(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), pdq, prs, st));
-            }
-        }
+        else if (cfg_.flags & SH_FLAG_NO_GRAPH)
+            SH_HIP(enqueue_pcg_launches(ns, matrix_free, st));
         else
         {
-            SH_HIP(launch_pcg_init(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], prs, st));
-            for (uint32_t k = 0; k < cfg_.pcg_iterations; ++k)
-            {
-                SH_HIP(launch_pcg_spmv(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), prs, pdq, st));
-                SH_HIP(launch_pcg_update(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), pdq, prs, st));
-            }
+            // the PCG loop as one hipGraph, captured once per launch shape and
+            // replayed for every GN iteration and every solve of that shape
+            SH_HIP(pcg_graph(ns, matrix_free, st));
+            SH_HIP(hipGraphLaunch(graph_exec_, st));
         }
         SH_HIP(mark(kPhUpdate, st));
         SH_HIP(launch_update(sds, map(kMapNode, ns), n, map_total_[kMapNode], parity, st));
     }
     SH_HIP(mark(kNumPhases, st));
     return SH_OK;
+}
+
+// The per-iteration PCG launches (split or matrix-free) of the current batch.
+hipError_t Engine::enqueue_pcg_launches(uint32_t ns, bool matrix_free, hipStream_t st)
+{
+    const SolveDesc *sds = static_cast<const SolveDesc *>(d_descs_);
+    const int n = int(ns);
+    double *prs = static_cast<double *>(d_part_rs_), *pdq = static_cast<double *>(d_part_dq_);
+    hipError_t e;
+#define SH_TRY(x)                                                                                                      \
+    if ((e = (x)) != hipSuccess)                                                                                       \
+        return e;
+    SH_TRY(launch_pcg_init(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], prs, st));
+    for (uint32_t k = 0; k < cfg_.pcg_iterations; ++k)
+    {
+        if (matrix_free)
+        {
+            // ApplyJtJ without the blocks: d_k, then q = J^T W J d_k from the
+            // per-pixel records, then the dot and the update
+            SH_TRY(launch_pcg_dir(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), prs, st));
+            SH_TRY(launch_mf_tiles(K_, fp16_, sds, map(kMapTile, ns), n, map_total_[kMapTile], int(k), nullptr, st));
+            SH_TRY(launch_mf_rows(sds, map(kMapRow, ns), n, map_total_[kMapRow], int(k), nullptr, nullptr, st));
+            SH_TRY(launch_pcg_dot(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), pdq, st));
+        }
+        else
+            SH_TRY(launch_pcg_spmv(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), prs, pdq, st));
+        SH_TRY(launch_pcg_update(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), pdq, prs, st));
+    }
+#undef SH_TRY
+    return hipSuccess;
+}
+
+// Captures the PCG launches into graph_exec_ unless the graph of this launch
+// shape (solve count, work-group totals, mode) is already there.  Kernel
+// arguments are device pointers to the uploaded descriptors and maps, which
+// keep their addresses, so a replay reads the current solves.
+hipError_t Engine::pcg_graph(uint32_t ns, bool matrix_free, hipStream_t st)
+{
+    const uint64_t key[5] = {ns, map_total_[kMapPcg], map_total_[kMapTile], map_total_[kMapRow],
+                             uint64_t(matrix_free) | (uint64_t(K_) << 1) | (uint64_t(fp16_) << 8)};
+    if (graph_exec_ && std::equal(key, key + 5, graph_key_))
+        return hipSuccess;
+    if (graph_exec_)
+    {
+        (void)hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
+    }
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess)
+        return e;
+    const hipError_t le = enqueue_pcg_launches(ns, matrix_free, st);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(st, &g);
+    if (le != hipSuccess || e != hipSuccess)
+    {
+        if (g)
+            (void)hipGraphDestroy(g);
+        return le != hipSuccess ? le : e;
+    }
+    e = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess)
+    {
+        graph_exec_ = nullptr;
+        return e;
+    }
+    std::copy(key, key + 5, graph_key_);
+    return hipSuccess;
 }
 
 // Phase timing (SH_FLAG_PHASE_TIMING): an event before each GN-phase launch;

@@ -148,6 +148,9 @@ private:
     std::vector<uint32_t> maps_;
     uint32_t map_total_[kNumMaps] = {0};
     hipGraphExec_t graph_exec_ = nullptr;
+    uint64_t graph_key_[5] = {0, 0, 0, 0, 0};
+    hipError_t enqueue_pcg_launches(uint32_t ns, bool matrix_free, hipStream_t st);
+    hipError_t pcg_graph(uint32_t ns, bool matrix_free, hipStream_t st);
     struct Mark
     {
         hipEvent_t ev;

@@ -479,3 +479,31 @@ def test_c3_fp16_full_size_per_iteration(torch_dev):
         near = {"twist_l1": ref["twist_l1"][: k * N]}
         assert errs[-1] <= tolerance(near), (k, errs[-1])
     say("c3 fp16 warp error per GN iteration:", ["%.2e" % e for e in errs], "straddles", straddles(ref))
+
+
+@pytest.mark.parametrize("spmv", [0, 1])
+def test_pcg_graph_replays_match_eager(torch_dev, spmv):
+    """The split and matrix-free PCG loops replayed from their captured hipGraph
+    give the bits of the eager launches, on every replay (three solves, with
+    eager work and synchronisation between them, plus a different frame in
+    between so the graph is re-captured and then replayed again)."""
+    from starhub_amd import native, scene, workload
+    torch = torch_dev
+    sc = scene.generate(scene.config_c1())
+    other = scene.generate(replace(scene.config_c1(), seed=77))
+    f_a, _ = workload.device_frame(sc)
+    f_b, _ = workload.device_frame(other)
+    base = native.SH_FLAG_NO_PERSISTENT_PCG
+    eager = workload.solver_for(sc, spmv, base | native.SH_FLAG_NO_GRAPH)
+    eager.process_frame(f_a)
+    want_a = eager.node_se3().cpu().numpy()
+    eager.process_frame(f_b)
+    want_b = eager.node_se3().cpu().numpy()
+    g = workload.solver_for(sc, spmv, base)
+    for rep in range(3):
+        g.process_frame(f_a)
+        assert np.array_equal(g.node_se3().cpu().numpy(), want_a), rep
+        junk = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
+        torch.cuda.synchronize()
+        g.process_frame(f_b)
+        assert np.array_equal(g.node_se3().cpu().numpy(), want_b), rep
