@@ -138,7 +138,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         A(d.pw, size_t(pcap_) * K_ * 4);
         A(d.n2t_off, size_t(ncap_ + 1) * 4);
         A(d.n2t_terms, size_t(n2t_cap_) * 4);
-        if (c.flags & SH_FLAG_REFERENCE_INDEX)
+        if (cfg_.flags & SH_FLAG_REFERENCE_INDEX) // normalised: PAIR_MAJOR implies it
         {
             // the reference's node-pair index, only built on request
             A(d.half_keys, size_t(ucap_) * 4);
@@ -361,6 +361,8 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
     }
 
     // reference node pair -> term index, half index (NodePair2TermsIndex::BuildHalfIndex, :294-298,591-654)
+    if (!d.pair_entries || !d.half_keys || !d.half_off || !d.upper_pos || !d.lower_pos)
+        return SH_E_STATE; // never launch on unallocated pair arrays
     const uint32_t np = uint32_t(K_ * (K_ - 1) / 2);
     const uint32_t n_pkv = P * np + uint32_t(d.E);
     uint32_t *k32a = static_cast<uint32_t *>(ws_k32a_), *k32b = static_cast<uint32_t *>(ws_k32b_);
