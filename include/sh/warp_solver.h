@@ -178,6 +178,8 @@ typedef struct sh_solver sh_solver;
 
 const char *sh_version(void);
 const char *sh_status_string(int status);
+/* Text of the last HIP runtime error behind an SH_E_HIP status (process-wide). */
+const char *sh_last_hip_error(void);
 
 /* --------------------------------------------------------------- one solve */
 /* WarpSolver() + AllocateBuffer (WarpSolver.cu:50-103): all capacity is

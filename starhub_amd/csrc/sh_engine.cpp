@@ -69,6 +69,9 @@ void Engine::release()
     if (graph_exec_)
         (void)hipGraphExecDestroy(graph_exec_);
     graph_exec_ = nullptr;
+    if (cap_stream_)
+        (void)hipStreamDestroy(cap_stream_);
+    cap_stream_ = nullptr;
     for (Mark &m : marks_)
         (void)hipEventDestroy(m.ev);
     marks_.clear();
@@ -753,12 +756,17 @@ hipError_t Engine::pcg_graph(uint32_t ns, bool matrix_free, hipStream_t st)
         (void)hipGraphExecDestroy(graph_exec_);
         graph_exec_ = nullptr;
     }
-    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess)
+    // capture on an engine-owned stream: the caller's may be the legacy NULL
+    // stream, which cannot be captured; the graph is then launched on st
+    (void)st;
+    hipError_t e = hipSuccess;
+    if (!cap_stream_ && (e = hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking)) != hipSuccess)
         return e;
-    const hipError_t le = enqueue_pcg_launches(ns, matrix_free, st);
+    if ((e = hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        return e;
+    const hipError_t le = enqueue_pcg_launches(ns, matrix_free, cap_stream_);
     hipGraph_t g = nullptr;
-    e = hipStreamEndCapture(st, &g);
+    e = hipStreamEndCapture(cap_stream_, &g);
     if (le != hipSuccess || e != hipSuccess)
     {
         if (g)
@@ -1128,6 +1136,8 @@ struct sh_batch
 extern "C"
 {
     const char *sh_version(void) { return "starhub_amd 0.1.0 gfx950"; }
+
+    const char *sh_last_hip_error(void) { return hipGetErrorString(sh::last_hip_error); }
 
     const char *sh_status_string(int s)
     {

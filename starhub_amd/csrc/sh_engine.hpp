@@ -149,6 +149,7 @@ private:
     uint32_t map_total_[kNumMaps] = {0};
     hipGraphExec_t graph_exec_ = nullptr;
     uint64_t graph_key_[5] = {0, 0, 0, 0, 0};
+    hipStream_t cap_stream_ = nullptr; // graph capture stream
     hipError_t enqueue_pcg_launches(uint32_t ns, bool matrix_free, hipStream_t st);
     hipError_t pcg_graph(uint32_t ns, bool matrix_free, hipStream_t st);
     struct Mark

@@ -131,6 +131,10 @@ class SolverError(RuntimeError):
 def check(rc: int, what: str) -> None:
     if rc != SH_OK:
         msg = lib().sh_status_string(rc).decode()
+        if rc == 3:  # SH_E_HIP
+            L = lib()
+            L.sh_last_hip_error.restype = ctypes.c_char_p
+            msg += ": " + L.sh_last_hip_error().decode()
         raise SolverError(f"{what} failed: {msg} ({rc})")
 
 
