@@ -250,6 +250,18 @@ int sh_batch_set_inputs(sh_batch *batch, uint32_t slot, const sh_camera *camera,
 int sh_batch_solve(sh_batch *batch, uint32_t num_solves, sh_stream_t stream);
 int sh_batch_node_se3(const sh_batch *batch, uint32_t slot, const sh_dual_quat **device_view, uint32_t *num_nodes);
 
+/* SurfelNodeDeformer::ForwardWarpSurfelsAndNodes (libs/star/geometry/surfel/
+ * SurfelNodeDeformer.cu:13-80): live surfels and nodes from the reference ones
+ * under a warp field (num_nodes dual quaternions).  Each node DQ is normalised
+ * before the K-slot blend (DualQuaternion.hpp:289-306); v' = R v + t,
+ * n' = R n, w components kept.  Either set may be empty (count 0). */
+int sh_forward_warp(const sh_dual_quat *warp_field, uint32_t num_nodes, uint32_t knn,
+                    const float *reference_vertex4, const float *reference_normal4, const uint16_t *surfel_knn,
+                    const float *surfel_spatial_weight, const float *surfel_connect_weight, uint32_t num_surfels,
+                    const float *reference_node4, const uint16_t *node_knn, const float *node_spatial_weight,
+                    const float *node_connect_weight, float *live_vertex4, float *live_normal4, float *live_node4,
+                    sh_stream_t stream);
+
 /* ------------------------------------------------------------- serving */
 /* No single counterpart in the reference, which solves one frame of one camera
  * per ProcessFrame call (examples/mono_star/app.cu:132-141).  A server takes

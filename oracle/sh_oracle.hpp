@@ -1324,4 +1324,58 @@ public:
     }
 };
 
+// ------------------------------------------------ §8f.1 forward warp (test infrastructure)
+// SurfelNodeDeformer::ForwardWarpSurfelsAndNodes (SurfelNodeDeformer.cu:13-80):
+// blend of the K node DQs, each normalised first, weights spatial x connect
+// (averageDualQuaternion, DualQuaternion.hpp:289-306), then se3_matrix() of the
+// blend; v' = R v + t, n' = R n.  warp: N x 8 (q0 w x y z, q1 w x y z).
+template <class T>
+inline M34<T> blend_normalised(const double *warp, const uint16_t *knn, const float *spatial, const float *connect,
+                               size_t i, int K)
+{
+    DQ<T> avg{{0, 0, 0, 0}, {0, 0, 0, 0}};
+    for (int k = 0; k < K; ++k)
+    {
+        const size_t n = knn[i * K + k];
+        const double *q = warp + 8 * n;
+        DQ<T> d{{T(q[0]), T(q[1]), T(q[2]), T(q[3])}, {T(q[4]), T(q[5]), T(q[6]), T(q[7])}};
+        d = dq_normalized(d);
+        const T w = T(spatial[i * K + k]) * T(connect[i * K + k]);
+        avg.q0 = {avg.q0.w + w * d.q0.w, avg.q0.x + w * d.q0.x, avg.q0.y + w * d.q0.y, avg.q0.z + w * d.q0.z};
+        avg.q1 = {avg.q1.w + w * d.q1.w, avg.q1.x + w * d.q1.x, avg.q1.y + w * d.q1.y, avg.q1.z + w * d.q1.z};
+    }
+    return dq_se3(avg);
+}
+
+template <class T>
+inline void forward_warp(const double *warp, int K, const float *ref_v, const float *ref_n, const uint16_t *s_knn,
+                         const float *s_spatial, const float *s_connect, size_t num_surfels, const float *ref_node,
+                         const uint16_t *n_knn, const float *n_spatial, const float *n_connect, size_t num_nodes,
+                         double *live_v, double *live_n, double *live_node)
+{
+    for (size_t i = 0; i < num_surfels; ++i)
+    {
+        const M34<T> M = blend_normalised<T>(warp, s_knn, s_spatial, s_connect, i, K);
+        const V3<T> v{T(ref_v[4 * i]), T(ref_v[4 * i + 1]), T(ref_v[4 * i + 2])};
+        const V3<T> n{T(ref_n[4 * i]), T(ref_n[4 * i + 1]), T(ref_n[4 * i + 2])};
+        const V3<T> v2 = mul(M.R, v) + M.t, n2 = mul(M.R, n);
+        const double ov[4] = {double(v2.x), double(v2.y), double(v2.z), double(ref_v[4 * i + 3])};
+        const double on[4] = {double(n2.x), double(n2.y), double(n2.z), double(ref_n[4 * i + 3])};
+        for (int c = 0; c < 4; ++c)
+        {
+            live_v[4 * i + c] = ov[c];
+            live_n[4 * i + c] = on[c];
+        }
+    }
+    for (size_t i = 0; i < num_nodes; ++i)
+    {
+        const M34<T> M = blend_normalised<T>(warp, n_knn, n_spatial, n_connect, i, K);
+        const V3<T> v{T(ref_node[4 * i]), T(ref_node[4 * i + 1]), T(ref_node[4 * i + 2])};
+        const V3<T> v2 = mul(M.R, v) + M.t;
+        const double o[4] = {double(v2.x), double(v2.y), double(v2.z), double(ref_node[4 * i + 3])};
+        for (int c = 0; c < 4; ++c)
+            live_node[4 * i + c] = o[c];
+    }
+}
+
 } // namespace sho

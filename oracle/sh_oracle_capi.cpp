@@ -239,6 +239,23 @@ extern "C"
         out34[7] = m.t.y;
         out34[11] = m.t.z;
     }
+    // forward warp in FP64 (precision 64) or FP32 (32); outputs as double
+    int sho_forward_warp(const double *warp_wxyz, int K, const float *ref_v, const float *ref_n,
+                         const uint16_t *s_knn, const float *s_spatial, const float *s_connect, uint64_t num_surfels,
+                         const float *ref_node, const uint16_t *n_knn, const float *n_spatial, const float *n_connect,
+                         uint64_t num_nodes, double *live_v, double *live_n, double *live_node, int precision)
+    {
+        if ((K != 4 && K != 8) || !warp_wxyz)
+            return 1;
+        if (precision == 32)
+            sho::forward_warp<float>(warp_wxyz, K, ref_v, ref_n, s_knn, s_spatial, s_connect, num_surfels, ref_node,
+                                     n_knn, n_spatial, n_connect, num_nodes, live_v, live_n, live_node);
+        else
+            sho::forward_warp<double>(warp_wxyz, K, ref_v, ref_n, s_knn, s_spatial, s_connect, num_surfels, ref_node,
+                                      n_knn, n_spatial, n_connect, num_nodes, live_v, live_n, live_node);
+        return 0;
+    }
+
     void sho_dq_from_mat34(const double *in34, double *dq)
     {
         sho::M34<double> m;

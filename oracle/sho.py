@@ -246,3 +246,30 @@ def eval_terms(inputs: dict, params, dq_wxyz: np.ndarray):
                          va.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
     assert P >= 0
     return r[:3 * P].reshape(P, 3), J[:18 * P].reshape(P, 18), ti.reshape(E, 3), tj.reshape(E, 3), va
+
+
+def forward_warp(warp_wxyz, K, ref_v, ref_n, s_knn, s_spatial, s_connect, ref_node=None, n_knn=None,
+                 n_spatial=None, n_connect=None, precision: int = 64):
+    """SurfelNodeDeformer::ForwardWarpSurfelsAndNodes restated: live surfel vertex/normal (S x 4) and live
+    node coordinates (N x 4) under the warp field warp_wxyz (N x 8, q0 wxyz, q1 wxyz)."""
+    L = lib()
+    f32 = lambda a: np.ascontiguousarray(a, np.float32)
+    u16 = lambda a: np.ascontiguousarray(a, np.uint16)
+    warp = np.ascontiguousarray(warp_wxyz, np.float64)
+    ref_v, ref_n, s_knn, s_spatial, s_connect = f32(ref_v), f32(ref_n), u16(s_knn), f32(s_spatial), f32(s_connect)
+    S = ref_v.shape[0]
+    Nn = 0 if ref_node is None else ref_node.shape[0]
+    if Nn:
+        ref_node, n_knn, n_spatial, n_connect = f32(ref_node), u16(n_knn), f32(n_spatial), f32(n_connect)
+    lv = np.zeros((S, 4)); ln = np.zeros((S, 4)); lnode = np.zeros((max(Nn, 1), 4))
+    P = lambda a, t: a.ctypes.data_as(ctypes.POINTER(t)) if a is not None else None
+    L.sho_forward_warp.restype = ctypes.c_int
+    rc = L.sho_forward_warp(P(warp, ctypes.c_double), K, P(ref_v, ctypes.c_float), P(ref_n, ctypes.c_float),
+                            P(s_knn, ctypes.c_uint16), P(s_spatial, ctypes.c_float), P(s_connect, ctypes.c_float),
+                            ctypes.c_uint64(S), P(ref_node, ctypes.c_float) if Nn else None,
+                            P(n_knn, ctypes.c_uint16) if Nn else None, P(n_spatial, ctypes.c_float) if Nn else None,
+                            P(n_connect, ctypes.c_float) if Nn else None, ctypes.c_uint64(Nn),
+                            P(lv, ctypes.c_double), P(ln, ctypes.c_double), P(lnode, ctypes.c_double), precision)
+    if rc != 0:
+        raise ValueError("sho_forward_warp rejected its arguments")
+    return lv, ln, lnode[:Nn]

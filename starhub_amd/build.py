@@ -37,8 +37,8 @@ HIP_FLAGS = [
     "-Wno-unknown-pragmas",
 ]
 
-HIP_SOURCES = ["sh_kernels.hip", "sh_rows.hip", "sh_tiles.hip", "sh_pcg.hip", "sh_sort.hip", "sh_engine.cpp",
-               "sh_server.cpp"]
+HIP_SOURCES = ["sh_kernels.hip", "sh_rows.hip", "sh_tiles.hip", "sh_pcg.hip", "sh_sort.hip", "sh_warp.hip",
+               "sh_engine.cpp", "sh_server.cpp"]
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:

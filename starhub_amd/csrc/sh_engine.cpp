@@ -1289,6 +1289,32 @@ extern "C"
         return SH_OK;
     }
 
+    int sh_forward_warp(const sh_dual_quat *warp_field, uint32_t num_nodes, uint32_t knn, const float *ref_v4,
+                        const float *ref_n4, const uint16_t *s_knn, const float *s_spatial, const float *s_connect,
+                        uint32_t num_surfels, const float *ref_node4, const uint16_t *n_knn, const float *n_spatial,
+                        const float *n_connect, float *live_v4, float *live_n4, float *live_node4, sh_stream_t stream)
+    {
+        if ((knn != 4 && knn != 8) || !warp_field || num_nodes == 0)
+            return SH_E_ARG;
+        if (num_surfels && (!ref_v4 || !ref_n4 || !s_knn || !s_spatial || !s_connect || !live_v4 || !live_n4))
+            return SH_E_ARG;
+        const bool nodes = ref_node4 != nullptr;
+        if (nodes && (!n_knn || !n_spatial || !n_connect || !live_node4))
+            return SH_E_ARG;
+        const hipError_t e = sh::launch_forward_warp(
+            knn, reinterpret_cast<const float4 *>(warp_field), reinterpret_cast<const float4 *>(ref_v4),
+            reinterpret_cast<const float4 *>(ref_n4), s_knn, s_spatial, s_connect, num_surfels,
+            reinterpret_cast<const float4 *>(ref_node4), n_knn, n_spatial, n_connect, nodes ? num_nodes : 0,
+            reinterpret_cast<float4 *>(live_v4), reinterpret_cast<float4 *>(live_n4),
+            reinterpret_cast<float4 *>(live_node4), reinterpret_cast<hipStream_t>(stream));
+        if (e != hipSuccess)
+        {
+            sh::last_hip_error = e;
+            return SH_E_HIP;
+        }
+        return SH_OK;
+    }
+
     int sh_build_node_graph(const float *node_coords4, uint32_t num_nodes, uint32_t knn, float node_radius_square,
                             uint16_t *node_knn, float *node_spatial_weight, uint16_t *node_graph,
                             float *node_connect_weight, sh_stream_t stream)

@@ -100,6 +100,13 @@ constexpr uint32_t kThreads = 256;
 constexpr uint32_t kWavesPerWg = 4;
 constexpr uint32_t kPcgNodesPerWg = 32;  // split PCG kernels
 
+// Forward warp of surfels and nodes (sh_warp.hip)
+hipError_t launch_forward_warp(uint32_t K, const float4 *warp, const float4 *ref_v, const float4 *ref_n,
+                               const uint16_t *s_knn, const float *s_spatial, const float *s_connect,
+                               uint32_t num_surfels, const float4 *ref_node, const uint16_t *n_knn,
+                               const float *n_spatial, const float *n_connect, uint32_t num_nodes, float4 *live_v,
+                               float4 *live_n, float4 *live_node, hipStream_t st);
+
 // Pixel tiles (sh_tiles.hip)
 constexpr uint32_t kTilePix = 128;
 hipError_t launch_tile_build(const SolveDesc &d, uint16_t *ws_contrib, uint32_t *ws_slot_block,

@@ -507,3 +507,36 @@ def test_pcg_graph_replays_match_eager(torch_dev, spmv):
         torch.cuda.synchronize()
         g.process_frame(f_b)
         assert np.array_equal(g.node_se3().cpu().numpy(), want_b), rep
+
+
+def test_forward_warp_vs_oracle(torch_dev):
+    """Forward warp of C2's 300k surfels and 1024 nodes by the GPU-solved warp
+    field: the GPU kernel against the FP64 oracle's restatement."""
+    import ctypes
+    import sho
+    from starhub_amd import native, scene, workload
+    torch = torch_dev
+    sc = scene.generate(scene.config_c2())
+    solver, frame, prod, _ = _solve(sc)
+    field = solver.node_se3()  # N x 8, q0 xyzw, q1 xyzw (device)
+    S, N, K = sc.surfel_pos.shape[0], sc.nodes.shape[0], sc.cfg.K
+    ref_v = torch.from_numpy(sc.surfel_pos).cuda()
+    ref_n = torch.from_numpy(sc.surfel_normal).cuda()
+    lv = torch.empty(S, 4, device="cuda")
+    ln = torch.empty(S, 4, device="cuda")
+    lnode = torch.empty(N, 4, device="cuda")
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    rc = native.lib().sh_forward_warp(p(field), N, K, p(ref_v), p(ref_n), p(frame.surfel_knn), p(frame.surfel_spatial),
+                                      p(frame.surfel_connect), S, p(frame.node_coords), p(frame.node_knn),
+                                      p(frame.node_spatial), p(frame.node_connect), p(lv), p(ln), p(lnode),
+                                      ctypes.c_void_p())
+    assert rc == 0
+    torch.cuda.synchronize()
+    wxyz = workload.device_dq_to_wxyz(field)
+    ov, on, onode = sho.forward_warp(wxyz, K, sc.surfel_pos, sc.surfel_normal, prod["surfel_knn"].reshape(S, K),
+                                     prod["surfel_spatial"].reshape(S, K), prod["surfel_connect"].reshape(S, K),
+                                     sc.nodes, prod["node_knn"].reshape(N, K), prod["node_spatial"].reshape(N, K),
+                                     prod["node_connect"].reshape(N, K))
+    assert np.abs(lv.cpu().numpy() - ov).max() < 5e-6
+    assert np.abs(ln.cpu().numpy() - on).max() < 5e-6
+    assert np.abs(lnode.cpu().numpy() - onode).max() < 5e-6
