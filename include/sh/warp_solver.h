@@ -304,9 +304,17 @@ typedef struct sh_server_stats
 int sh_server_create(const sh_server_config *config, sh_server **out);
 int sh_server_destroy(sh_server *server);
 /* Registers a stream and copies its geometry and node graph into resident HBM;
- * SH_E_CAPACITY when the budget cannot hold it (admission by memory budget). */
+ * SH_E_CAPACITY when the budget cannot hold it (admission by memory budget).
+ * With reference surfels (float4 vertex and normal, both or neither), the
+ * server keeps them resident too and forward-warps them and the nodes after
+ * every solve of the stream (sh_server_live_surfels). */
 int sh_server_add_stream(sh_server *server, uint32_t stream_id, const sh_camera *camera, const sh_geometry *geometry,
-                         const sh_node_graph *node_graph, const float cam2world[16]);
+                         const sh_node_graph *node_graph, const float cam2world[16], const float *reference_vertex4,
+                         const float *reference_normal4);
+/* Device views of a stream's live surfels (vertex, normal: num_surfels float4)
+ * and live nodes (float4): the reference ones under the last solved warp field. */
+int sh_server_live_surfels(sh_server *server, uint32_t stream_id, const float **live_vertex4,
+                           const float **live_normal4, const float **live_node4, uint32_t *num_surfels);
 /* Queues one frame of a stream.  The maps are caller-owned device memory and must
  * stay valid until the request is done; out_device (may be NULL) receives the
  * solved node dual quaternions. */

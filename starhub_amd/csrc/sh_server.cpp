@@ -33,6 +33,7 @@
 
 #include "../../include/sh/warp_solver.h"
 #include "sh_engine.hpp"
+#include "sh_kernels.hpp"
 
 namespace
 {
@@ -55,6 +56,8 @@ struct StreamState
     float *node_connect = nullptr, *node_spatial = nullptr;
     uint16_t *node_knn = nullptr;
     sh_dual_quat *warp = nullptr; // warp field of the last solve
+    // reference surfels and the live ones (forward-warped after every solve), optional
+    float *ref_v = nullptr, *ref_n = nullptr, *live_v = nullptr, *live_n = nullptr, *live_node = nullptr;
     bool busy = false;            // a request of this stream is admitted
     uint64_t solved = 0;
 };
@@ -210,6 +213,16 @@ void sh_server::step_loop()
                 copy_rc = hip_status(hipMemcpyAsync(warp, res, nb, hipMemcpyDeviceToDevice, s_step));
             if (copy_rc == SH_OK && out)
                 copy_rc = hip_status(hipMemcpyAsync(out, res, nb, hipMemcpyDeviceToDevice, s_step));
+            if (copy_rc == SH_OK && ss.ref_v)
+            {
+                // roll the resident surfels and nodes forward with the new warp field
+                copy_rc = hip_status(sh::launch_forward_warp(
+                    ss.K, reinterpret_cast<const float4 *>(warp), reinterpret_cast<const float4 *>(ss.ref_v),
+                    reinterpret_cast<const float4 *>(ss.ref_n), ss.surfel_knn, ss.surfel_spatial, ss.surfel_connect,
+                    ss.num_surfels, reinterpret_cast<const float4 *>(ss.node_coords), ss.node_knn, ss.node_spatial,
+                    ss.node_connect, ss.num_nodes, reinterpret_cast<float4 *>(ss.live_v),
+                    reinterpret_cast<float4 *>(ss.live_n), reinterpret_cast<float4 *>(ss.live_node), s_step));
+            }
         }
         if (!finished.empty() && copy_rc == SH_OK)
             copy_rc = hip_status(hipStreamSynchronize(s_step));
@@ -325,8 +338,12 @@ extern "C"
     }
 
     int sh_server_add_stream(sh_server *s, uint32_t stream_id, const sh_camera *camera, const sh_geometry *geometry,
-                             const sh_node_graph *node_graph, const float cam2world[16])
+                             const sh_node_graph *node_graph, const float cam2world[16],
+                             const float *reference_vertex4, const float *reference_normal4)
     {
+        if ((reference_vertex4 == nullptr) != (reference_normal4 == nullptr) ||
+            (reference_vertex4 && !node_graph->node_knn))
+            return SH_E_ARG;
         if (!s || !camera || !geometry || !node_graph || !cam2world || !geometry->surfel_knn ||
             !geometry->surfel_knn_spatial_weight || !geometry->surfel_knn_connect_weight ||
             !node_graph->reference_node_coords || !node_graph->node_graph || !node_graph->node_knn_connect_weight)
@@ -348,7 +365,9 @@ extern "C"
         auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
         const size_t b_sk = up(S * K * 2), b_sw = up(S * K * 4), b_nc = up(N * 16), b_e = up(E * 6),
                      b_nw = up(N * K * 4), b_nk = up(N * K * 2), b_w = up(N * sizeof(sh_dual_quat));
-        ss.bytes = b_sk + 2 * b_sw + b_nc + b_e + 2 * b_nw + b_nk + b_w;
+        const bool surf = reference_vertex4 != nullptr;
+        const size_t b_sv = surf ? up(S * 16) : 0, b_node = surf ? up(N * 16) : 0;
+        ss.bytes = b_sk + 2 * b_sw + b_nc + b_e + 2 * b_nw + b_nk + b_w + 4 * b_sv + b_node;
         {
             std::lock_guard<std::mutex> lk(s->mu);
             if (s->streams.count(stream_id))
@@ -376,6 +395,14 @@ extern "C"
         ss.node_spatial = reinterpret_cast<float *>(p += b_nw);
         ss.node_knn = reinterpret_cast<uint16_t *>(p += b_nw);
         ss.warp = reinterpret_cast<sh_dual_quat *>(p += b_nk);
+        if (surf)
+        {
+            ss.ref_v = reinterpret_cast<float *>(p += b_w);
+            ss.ref_n = reinterpret_cast<float *>(p += b_sv);
+            ss.live_v = reinterpret_cast<float *>(p += b_sv);
+            ss.live_n = reinterpret_cast<float *>(p += b_sv);
+            ss.live_node = reinterpret_cast<float *>(p += b_sv);
+        }
         hipStream_t st = s->s_admit;
         int rc = SH_OK;
         auto cp = [&](void *dst, const void *src, size_t bytes) {
@@ -390,6 +417,14 @@ extern "C"
         cp(ss.node_connect, node_graph->node_knn_connect_weight, N * K * 4);
         cp(ss.node_spatial, node_graph->node_knn_spatial_weight, N * K * 4);
         cp(ss.node_knn, node_graph->node_knn, N * K * 2);
+        if (surf)
+        {
+            cp(ss.ref_v, reference_vertex4, S * 16);
+            cp(ss.ref_n, reference_normal4, S * 16);
+            cp(ss.live_v, reference_vertex4, S * 16); // live = reference until the first solve
+            cp(ss.live_n, reference_normal4, S * 16);
+            cp(ss.live_node, node_graph->reference_node_coords, N * 16);
+        }
         if (rc == SH_OK)
             rc = hip_status(hipMemsetAsync(ss.warp, 0, N * sizeof(sh_dual_quat), st));
         if (rc == SH_OK)
@@ -471,6 +506,22 @@ extern "C"
             return SH_E_STATE;
         *device_view = it->second.warp;
         *num_nodes = it->second.num_nodes;
+        return SH_OK;
+    }
+
+    int sh_server_live_surfels(sh_server *s, uint32_t stream_id, const float **live_vertex4,
+                               const float **live_normal4, const float **live_node4, uint32_t *num_surfels)
+    {
+        if (!s || !live_vertex4 || !live_normal4 || !live_node4 || !num_surfels)
+            return SH_E_ARG;
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->streams.find(stream_id);
+        if (it == s->streams.end() || !it->second.ref_v)
+            return SH_E_STATE;
+        *live_vertex4 = it->second.live_v;
+        *live_normal4 = it->second.live_n;
+        *live_node4 = it->second.live_node;
+        *num_surfels = it->second.num_surfels;
         return SH_OK;
     }
 

@@ -377,11 +377,27 @@ class Server:
         except Exception:
             pass
 
-    def add_stream(self, stream_id: int, frame: "DeviceFrame") -> int:
-        """Registers a stream; returns the status (SH_E_CAPACITY when the HBM budget refuses it)."""
+    def add_stream(self, stream_id: int, frame: "DeviceFrame", ref_vertex=None, ref_normal=None) -> int:
+        """Registers a stream; returns the status (SH_E_CAPACITY when the HBM budget refuses it).
+        With reference surfels (S x 4 float32 device tensors) they are kept resident and forward-warped."""
         cam, meas, rend, geo, ng, nf, fl, kp, c2w = frame.structs()
+        pv = ctypes.c_void_p(ref_vertex.data_ptr()) if ref_vertex is not None else ctypes.c_void_p()
+        pn = ctypes.c_void_p(ref_normal.data_ptr()) if ref_normal is not None else ctypes.c_void_p()
         return lib().sh_server_add_stream(self._h, stream_id, ctypes.byref(cam), ctypes.byref(geo),
-                                          ctypes.byref(ng), c2w)
+                                          ctypes.byref(ng), c2w, pv, pn)
+
+    def live_surfels(self, stream_id: int):
+        """(live vertex S x 4, live normal S x 4, live nodes N x 4) copies as device tensors."""
+        import torch
+        v, n, nd = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        cnt = ctypes.c_uint32()
+        check(lib().sh_server_live_surfels(self._h, stream_id, ctypes.byref(v), ctypes.byref(n), ctypes.byref(nd),
+                                           ctypes.byref(cnt)), "sh_server_live_surfels")
+        S = cnt.value
+        N = self.warp_field(stream_id).shape[0]
+        return (_device_copy(v.value, S * 4, torch.float32).view(S, 4),
+                _device_copy(n.value, S * 4, torch.float32).view(S, 4),
+                _device_copy(nd.value, N * 4, torch.float32).view(N, 4))
 
     def submit(self, stream_id: int, frame: "DeviceFrame", out=None) -> int:
         cam, meas, rend, geo, ng, nf, fl, kp, c2w = frame.structs()

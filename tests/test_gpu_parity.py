@@ -338,8 +338,9 @@ def test_served_same_alone_and_in_mix(torch_dev):
         alone.append(s.node_se3().cpu().numpy())
         s.close()
     srv = native.Server(scfg, max_active=4)
+    refs = [(torch.from_numpy(s.surfel_pos).cuda(), torch.from_numpy(s.surfel_normal).cuda()) for s in scs]
     for i, f in enumerate(frames):
-        assert srv.add_stream(i, f) == 0
+        assert srv.add_stream(i, f, *refs[i]) == 0
     outs, tickets = [], []
     for rep in range(3):
         for i, f in enumerate(frames):
@@ -353,6 +354,22 @@ def test_served_same_alone_and_in_mix(torch_dev):
         assert np.array_equal(out.cpu().numpy(), alone[i]), i
     for i in range(len(frames)):
         assert np.array_equal(srv.warp_field(i).cpu().numpy(), alone[i])
+        # the resident surfels were rolled forward with that field
+        import ctypes
+        f = frames[i]
+        S, N = f.num_surfels, f.num_nodes
+        field = torch.from_numpy(alone[i]).cuda()
+        lv, ln, lnode = (torch.empty(S, 4, device="cuda"), torch.empty(S, 4, device="cuda"),
+                         torch.empty(N, 4, device="cuda"))
+        p = lambda t: ctypes.c_void_p(t.data_ptr())
+        assert native.lib().sh_forward_warp(p(field), N, 8, p(refs[i][0]), p(refs[i][1]), p(f.surfel_knn),
+                                            p(f.surfel_spatial), p(f.surfel_connect), S, p(f.node_coords),
+                                            p(f.node_knn), p(f.node_spatial), p(f.node_connect), p(lv), p(ln),
+                                            p(lnode), ctypes.c_void_p()) == 0
+        torch.cuda.synchronize()
+        got = srv.live_surfels(i)
+        for a, b in zip(got, (lv, ln, lnode)):
+            assert torch.equal(a, b), i
     st = srv.stats()
     say("server stats", st)
     assert st["completed"] == len(tickets) and st["failed"] == 0
