@@ -263,8 +263,8 @@ class WarpSolver:
         n = ctypes.c_uint32()
         check(lib().sh_solver_cost_history(self._h, None, 0, ctypes.byref(n)), "sh_solver_cost_history")
         out = np.zeros(n.value, np.float64)
-        check(lib().sh_solver_cost_history(self._h, out.ctypes.data_as(ctypes.c_void_p), n.value, ctypes.byref(n)),
-              "sh_solver_cost_history")
+        check(lib().sh_solver_cost_history(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n.value,
+                                           ctypes.byref(n)), "sh_solver_cost_history")
         return out
 
     def phase_times(self, reset: bool = False) -> dict:
