@@ -398,8 +398,11 @@ def test_server_budget_admission(torch_dev):
 
 def test_cost_history_fp64_vs_oracle(torch_dev):
     """The FP64 GN cost at the start of every iteration and after the last
-    (SH_FLAG_COST_HISTORY) matches the FP64 oracle's to 1e-5 relative (the
-    residuals themselves are FP32)."""
+    (SH_FLAG_COST_HISTORY).  At iteration 0 both sides evaluate the identity
+    warp field, so the cost arithmetic is compared alone: 1e-5 relative (the
+    residuals themselves are FP32).  Later entries are taken at the GPU's own
+    FP32 states, which differ from the FP64 oracle's by the warp-field error
+    (up to ~1e-4 after a rotation-cutoff event): 1e-4 relative."""
     import sho
     from starhub_amd import native, scene, workload
     sc = scene.generate(scene.config_c1())
@@ -408,7 +411,9 @@ def test_cost_history_fp64_vs_oracle(torch_dev):
     ref = sho.solve(workload.host_inputs(sc, prod), sc.cfg.solver, 64, sho.max_threads())["cost"]
     say("cost gpu", gpu, "fp64", ref)
     assert gpu.shape == ref.shape == (sc.cfg.solver.gn_iterations + 1,)
-    assert np.allclose(gpu, ref, rtol=1e-5, atol=0)
+    assert abs(gpu[0] - ref[0]) <= 1e-5 * ref[0]
+    assert np.allclose(gpu, ref, rtol=1e-4, atol=0)
+    assert np.all(np.diff(gpu) < 0)  # every GN step lowers the cost, as in the oracle
     plain, _, _, _ = _solve(sc)
     assert np.array_equal(plain.cost_history(), gpu[:-1])  # the flag only adds the final entry
 
