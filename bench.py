@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline sample budget")
     ap.add_argument("--no-served", action="store_true", help="skip the serving (64 streams) and batch (32 solves) legs")
     ap.add_argument("--served-requests", type=int, default=10000)
+    ap.add_argument("--tuning", default=None, metavar="FLAGS,SPMV,ROWS",
+                    help="use this tuning instead of running the autotuner (profiled runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain device copies of 256 MiB (the reference reading for HBM counters)")
     return ap.parse_args()
@@ -258,7 +260,12 @@ def main():
     frame, prod = workload.device_frame(sc)
     solver = workload.solver_for(sc, flags=native.SH_FLAG_PHASE_TIMING)
     solver.set_inputs(frame)
-    tuning = native.autotune(solver, repeats=3)  # fastest variant on this frame, kept for the timed steps
+    if args.tuning:
+        f, m, r = (int(v) for v in args.tuning.split(","))
+        native.set_tuning(solver, f, m, r)
+        tuning = {"flags": f, "spmv_mode": m, "pcg_rows_per_group": r, "ms": None}
+    else:
+        tuning = native.autotune(solver, repeats=3)  # fastest variant on this frame, kept for the timed steps
     for _ in range(args.warmup):
         solver.solve()
     torch.cuda.synchronize()

@@ -22,6 +22,7 @@ struct SolveDesc
     float node_radius_square;
     int K;
     int N, E;            // nodes, ARAP edge slots
+    uint32_t S;          // surfels: index_map values below S name a surfel
     int P;               // valid pixels after compaction
     int U;               // unique upper node pairs
     int n_pair_entries;  // valid (pair, term) entries

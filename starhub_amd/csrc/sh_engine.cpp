@@ -275,6 +275,7 @@ int Engine::set_inputs(uint32_t slot, const sh_camera *cam, const sh_measure_map
     d.node_radius_square = ng->node_radius_square;
     d.N = int(ng->num_nodes);
     d.E = int(ng->num_edges);
+    d.S = geo->num_surfels;
     d.meas_v = reinterpret_cast<const float4 *>(meas->vertex_confid);
     d.meas_n = reinterpret_cast<const float4 *>(meas->normal_radius);
     d.ref_v = reinterpret_cast<const float4 *>(rend->reference_vertex);
@@ -305,11 +306,16 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
     const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
     size_t tb = sort_temp_bytes_;
     uint32_t *flags = static_cast<uint32_t *>(ws_flags_), *pos = static_cast<uint32_t *>(ws_pos_);
-    SH_HIP(launch_pixel_flags(d.index_map, n, flags, st));
+    uint32_t *err = static_cast<uint32_t *>(d_flags_) + 2;
+    SH_HIP(hipMemsetAsync(err, 0, 4, st));
+    SH_HIP(launch_pixel_flags(d, n, flags, err, st));
     SH_HIP(exclusive_scan_u32(sort_temp_, tb, flags, pos, n, st));
     SH_HIP(hipMemcpyAsync(h_meta_ + 0, pos + (n - 1), 4, hipMemcpyDeviceToHost, st));
     SH_HIP(hipMemcpyAsync(h_meta_ + 1, flags + (n - 1), 4, hipMemcpyDeviceToHost, st));
+    SH_HIP(hipMemcpyAsync(h_meta_ + 12, err, 4, hipMemcpyDeviceToHost, st));
     SH_HIP(hipStreamSynchronize(st));
+    if (h_meta_[12] != 0)
+        return SH_E_ARG; // a surfel or node id out of range
     const uint32_t P = h_meta_[0] + h_meta_[1];
     if (P > pcap_)
         return SH_E_CAPACITY;

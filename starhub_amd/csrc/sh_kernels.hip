@@ -37,11 +37,30 @@ __device__ __forceinline__ uint32_t f2u_rn_sat(float v)
 }
 
 // ============================================================ index build
-__global__ void k_pixel_flags(const uint32_t *__restrict__ index_map, uint32_t n, uint32_t *__restrict__ flags)
+// Valid-pixel flags, and the range checks every later index kernel relies on:
+// a valid pixel names a surfel below S whose K node ids are below N, and every
+// ARAP edge names nodes below N.  A pixel that fails is flagged 0 and sets
+// *err; the host rejects the frame before anything reads the ids.
+__global__ void k_pixel_flags(SolveDesc d, uint32_t n, uint32_t *__restrict__ flags, uint32_t *__restrict__ err)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t N = uint32_t(d.N);
+    bool bad = false;
     if (i < n)
-        flags[i] = (index_map[i] != kInvalidIndex) ? 1u : 0u;
+    {
+        const uint32_t s = d.index_map[i];
+        bool valid = s != kInvalidIndex;
+        if (valid && s >= d.S)
+            valid = false, bad = true;
+        for (int k = 0; valid && k < d.K; ++k)
+            if (d.surfel_knn[size_t(s) * d.K + k] >= N)
+                valid = false, bad = true;
+        flags[i] = valid ? 1u : 0u;
+    }
+    if (i < uint32_t(d.E) && (d.edges[3 * size_t(i)] >= N || d.edges[3 * size_t(i) + 1] >= N))
+        bad = true;
+    if (bad)
+        atomicOr(err, 1u);
 }
 
 // Stream-compact the valid pixels in row-major order; gathers surfel KNN and
@@ -1423,11 +1442,12 @@ __global__ __launch_bounds__(256) void k_node_graph(const float4 *__restrict__ n
 // ============================================================ host launchers
 #define SH_GRID(n, b) dim3(unsigned(((n) + (b)-1) / (b)))
 
-hipError_t launch_pixel_flags(const uint32_t *index_map, uint32_t n, uint32_t *flags, hipStream_t st)
+hipError_t launch_pixel_flags(const SolveDesc &d, uint32_t n, uint32_t *flags, uint32_t *err, hipStream_t st)
 {
-    if (n == 0)
+    const uint32_t m = max(n, uint32_t(d.E));
+    if (m == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_pixel_flags, SH_GRID(n, 256), dim3(256), 0, st, index_map, n, flags);
+    hipLaunchKernelGGL(k_pixel_flags, SH_GRID(m, 256), dim3(256), 0, st, d, n, flags, err);
     return hipGetLastError();
 }
 

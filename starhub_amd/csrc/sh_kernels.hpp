@@ -9,7 +9,7 @@
 
 namespace sh
 {
-hipError_t launch_pixel_flags(const uint32_t *index_map, uint32_t n, uint32_t *flags, hipStream_t st);
+hipError_t launch_pixel_flags(const SolveDesc &d, uint32_t n, uint32_t *flags, uint32_t *err, hipStream_t st);
 hipError_t launch_compact_pixels(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos, hipStream_t st);
 hipError_t launch_node2term_kv(const SolveDesc &d, uint16_t *keys, uint32_t *vals, hipStream_t st);
 hipError_t launch_lower_bound_u16(const uint16_t *keys, uint32_t n_keys, uint32_t n_out, uint32_t *out,

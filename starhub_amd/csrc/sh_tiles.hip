@@ -311,18 +311,28 @@ __global__ void k_lower_bound_u32(const uint32_t *__restrict__ keys, uint32_t n_
 }
 
 // ------------------------------------------------------------ GN phase
-// One work-group per tile: stage records (28 floats), folded weights, the
-// contribution list and slot starts; then task (slot s, float4 group g)
-// sums coef * record[4g .. 4g+3] over the slot's contributions in order.
+// One work-group per tile.  The records (28 floats: 21 H, 6 g, pad) and the
+// folded weights of the tile's pixels are staged in LDS with the tile's
+// contribution list.  Work items are (slot, float4 group) tasks; each sums its
+// slot's contributions in list order (increasing pixel), coef * record.  The
+// slots are visited longest list first (a counting sort of the list lengths in
+// LDS), so the lanes of a wave run lists of about the same length; which lane
+// sums a slot does not change its result.  Contributions are loaded in batches
+// of kTileBatch so the dependent LDS reads of a batch overlap.
+constexpr int kTileBatch = 8;
+
 template <int K, class JT>
 __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restrict__ sds,
                                                         const uint32_t *__restrict__ wg_begin, int n_solve)
 {
     using G = TileGeom<K>;
+    constexpr int kEnt = kTilePix * G::kPairs;
     __shared__ float4 Hs[kTilePix * (kTileRec / 4)];
     __shared__ float wS[kTilePix * K];
-    __shared__ uint16_t ent[kTilePix * G::kPairs];
-    __shared__ uint16_t sst[kTilePix * G::kPairs + 1];
+    __shared__ uint16_t ent[kEnt];
+    __shared__ uint16_t sst[kEnt + 2];
+    __shared__ uint16_t perm[kEnt];
+    __shared__ uint32_t hist[kTilePix + 2];
     const int si = find_solve_t(wg_begin, n_solve, blockIdx.x);
     const SolveDesc &d = sds[si];
     const uint32_t t = blockIdx.x - wg_begin[si];
@@ -331,6 +341,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
     const uint32_t tid = threadIdx.x;
     const uint32_t p0 = t * kTilePix;
     const uint32_t np = min(uint32_t(kTilePix), uint32_t(d.P) - p0);
+    const uint32_t s0 = d.tile_sbase[t], ns = d.tile_sbase[t + 1] - s0;
+    const uint32_t c0 = d.tile_cbase[t], nc = d.tile_cbase[t + 1] - c0;
+    for (uint32_t i = tid; i < kTilePix + 2; i += kTileThreads)
+        hist[i] = 0u;
     if (tid < np)
     {
         const uint32_t p = p0 + tid;
@@ -343,20 +357,71 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
         for (int k = 0; k < K; ++k)
             wS[tid * K + k] = d.pw[size_t(p) * K + k];
     }
-    const uint32_t c0 = d.tile_cbase[t], nc = d.tile_cbase[t + 1] - c0;
-    const uint32_t s0 = d.tile_sbase[t], ns = d.tile_sbase[t + 1] - s0;
     for (uint32_t i = tid; i < nc; i += kTileThreads)
         ent[i] = d.tc_contrib[c0 + i];
     for (uint32_t s = tid; s <= ns; s += kTileThreads)
         sst[s] = uint16_t(d.tc_slot_first[s0 + s] - c0);
     __syncthreads();
+    // counting sort of the slots by list length (1..kTilePix), longest first
+    for (uint32_t s = tid; s < ns; s += kTileThreads)
+        atomicAdd(&hist[kTilePix - (sst[s + 1] - sst[s])], 1u);
+    __syncthreads();
+    if (tid < 64)
+    {
+        // exclusive scan of hist[0..kTilePix] (129 buckets: 3 per lane)
+        const uint32_t b0 = 3 * tid;
+        const uint32_t h0 = b0 < kTilePix + 1 ? hist[b0] : 0u, h1 = b0 + 1 < kTilePix + 1 ? hist[b0 + 1] : 0u,
+                       h2 = b0 + 2 < kTilePix + 1 ? hist[b0 + 2] : 0u;
+        uint32_t v = h0 + h1 + h2;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            const uint32_t u = __shfl_up(v, o, 64);
+            if (int(tid) >= o)
+                v += u;
+        }
+        const uint32_t ex = v - h0 - h1 - h2;
+        if (b0 < kTilePix + 1)
+            hist[b0] = ex;
+        if (b0 + 1 < kTilePix + 1)
+            hist[b0 + 1] = ex + h0;
+        if (b0 + 2 < kTilePix + 1)
+            hist[b0 + 2] = ex + h0 + h1;
+    }
+    __syncthreads();
+    for (uint32_t s = tid; s < ns; s += kTileThreads)
+        perm[atomicAdd(&hist[kTilePix - (sst[s + 1] - sst[s])], 1u)] = uint16_t(s);
+    __syncthreads();
+
     float4 *__restrict__ out = reinterpret_cast<float4 *>(d.tpart) + size_t(s0) * (kTileRec / 4);
     for (uint32_t task = tid; task < ns * (kTileRec / 4); task += kTileThreads)
     {
-        const uint32_t s = task / (kTileRec / 4), g = task % (kTileRec / 4);
+        const uint32_t s = perm[task / (kTileRec / 4)], g = task % (kTileRec / 4);
         float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        uint32_t e = sst[s];
         const uint32_t e1 = sst[s + 1];
-        for (uint32_t e = sst[s]; e < e1; ++e)
+        for (; e + kTileBatch <= e1; e += kTileBatch)
+        {
+            float coef[kTileBatch];
+            float4 h[kTileBatch];
+#pragma unroll
+            for (int j = 0; j < kTileBatch; ++j)
+            {
+                const uint32_t c = ent[e + j];
+                const uint32_t pl = c & 127u, ka = (c >> 7) & 7u, kb = (c >> 10) & 7u;
+                coef[j] = wS[pl * K + ka] * wS[pl * K + kb];
+                h[j] = Hs[pl * (kTileRec / 4) + g];
+            }
+#pragma unroll
+            for (int j = 0; j < kTileBatch; ++j)
+            {
+                acc.x = fmaf(coef[j], h[j].x, acc.x);
+                acc.y = fmaf(coef[j], h[j].y, acc.y);
+                acc.z = fmaf(coef[j], h[j].z, acc.z);
+                acc.w = fmaf(coef[j], h[j].w, acc.w);
+            }
+        }
+        for (; e < e1; ++e)
         {
             const uint32_t c = ent[e];
             const uint32_t pl = c & 127u, ka = (c >> 7) & 7u, kb = (c >> 10) & 7u;
@@ -367,7 +432,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
             acc.z = fmaf(coef, h.z, acc.z);
             acc.w = fmaf(coef, h.w, acc.w);
         }
-        out[task] = acc;
+        out[size_t(s) * (kTileRec / 4) + g] = acc;
     }
 }
 
@@ -390,7 +455,156 @@ __device__ __forceinline__ void sym_pair_t(int e, int &a, int &b)
     b = e - (a * 6 - a * (a - 1) / 2) + a;
 }
 
+// One ARAP or node-translation term of a node row, as the row kernels use it.
+struct RegTerm
+{
+    float pm[3], po[3]; // this node's and the other node's warped point (ARAP)
+    float sm, ws;       // side sign, lambda^2 cw^2 (0: invalid edge); NT: ws = lam2 c^2, sm = 2
+    float tv[3];        // NT: t - pred
+    int col;            // block column of the other node, -1 when it is the row node itself
+};
+
+// Loads the terms [beg, end) of row i's list, 256 at a time, one per thread,
+// and adds them into regblk (diagonal block at column 0, cross blocks at
+// colmap[other]) and g[0..5], each entry summed over the terms in list order.
+// Must be reached by all 256 threads.
+__device__ void row_reg_terms(const SolveDesc &d, uint32_t i, uint32_t beg, uint32_t end,
+                              const uint16_t *__restrict__ colmap, float *__restrict__ regblk, float *__restrict__ g,
+                              RegTerm *__restrict__ rt)
+{
+    const uint32_t tid = threadIdx.x;
+    const uint32_t P = d.P, E = d.E;
+    for (uint32_t c0 = beg; c0 < end; c0 += kTileThreads)
+    {
+        const uint32_t n = min(uint32_t(kTileThreads), end - c0);
+        if (tid < n)
+        {
+            RegTerm r{};
+            const uint32_t t = d.n2t_terms[c0 + tid];
+            if (t < P + E)
+            {
+                const uint32_t e = t - P;
+                const float4 *rec = reinterpret_cast<const float4 *>(d.reg + size_t(e) * kReg);
+                const float4 u0 = rec[0], u1 = rec[1];
+                const uint32_t ex = d.edges[3 * e], ey = d.edges[3 * e + 1];
+                const bool is_x = (ex == i);
+                const uint32_t other = is_x ? ey : ex;
+                const float pa[3] = {u0.x, u0.y, u0.z}, pb[3] = {u0.w, u1.x, u1.y};
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                {
+                    r.pm[q] = is_x ? pa[q] : pb[q];
+                    r.po[q] = is_x ? pb[q] : pa[q];
+                }
+                r.sm = is_x ? 1.f : -1.f;
+                r.ws = (u1.w == 0.f) ? 0.f : d.lam2_reg * (u1.z * u1.z);
+                r.col = (other != i) ? int(colmap[other]) : -1;
+            }
+            else
+            {
+                const uint32_t m = t - P - E;
+                const float4 pred = d.node_pred ? d.node_pred[m] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const float4 tt = d.ttrans[m];
+                r.sm = 2.f;
+                r.ws = d.lam2_nt * (pred.w * pred.w);
+                r.tv[0] = tt.x - pred.x;
+                r.tv[1] = tt.y - pred.y;
+                r.tv[2] = tt.z - pred.z;
+                r.col = -1;
+            }
+            rt[tid] = r;
+        }
+        __syncthreads();
+        // threads 0..35: diagonal entries, 36..71: cross entries, 72..77: g
+        if (tid < 78)
+        {
+            const int ent = int(tid);
+            float diag = 0.f, gq = 0.f;
+            for (uint32_t k = 0; k < n; ++k)
+            {
+                const RegTerm &r = rt[k];
+                if (r.sm == 2.f)
+                {
+                    // node translation (node_motion_jacobian.cuh:7-35): diagonal a = b = 3..5
+                    if (ent < 36 && ent / 6 == ent % 6 && ent / 6 >= 3)
+                        diag += r.ws;
+                    else if (ent >= 72 && ent - 72 >= 3)
+                        gq += r.ws * r.tv[ent - 75];
+                    continue;
+                }
+                if (r.ws == 0.f)
+                    continue;
+                const f3 pm = mk3(r.pm[0], r.pm[1], r.pm[2]), po = mk3(r.po[0], r.po[1], r.po[2]);
+                if (ent < 72)
+                {
+                    // ARAP rows x side [-[a]x, I], y side [[b]x, -I] (node_graph_reg_jacobian.cuh:250-275)
+                    const int ab = ent < 36 ? ent : ent - 36;
+                    const int aa = ab / 6, bb = ab % 6;
+                    float v = 0.f;
+#pragma unroll
+                    for (int ch = 0; ch < 3; ++ch)
+                    {
+                        const float ga = r.sm * reg_elem_t(pm, ch, aa);
+                        const float gb = ent < 36 ? r.sm * reg_elem_t(pm, ch, bb) : -r.sm * reg_elem_t(po, ch, bb);
+                        v += ga * gb;
+                    }
+                    if (ent < 36)
+                        diag += r.ws * v;
+                    else if (r.col >= 0)
+                        regblk[r.col * 36 + ab] += r.ws * v;
+                }
+                else
+                {
+                    const int q = ent - 72;
+                    const float res[3] = {r.pm[0] - r.po[0], r.pm[1] - r.po[1], r.pm[2] - r.po[2]};
+                    float v = 0.f;
+#pragma unroll
+                    for (int ch = 0; ch < 3; ++ch)
+                        v += (r.sm * res[ch]) * (r.sm * reg_elem_t(pm, ch, q));
+                    gq += r.ws * v;
+                }
+            }
+            if (ent < 36)
+                regblk[ent] += diag;
+            else if (ent >= 72)
+                g[ent - 72] += gq;
+        }
+        __syncthreads();
+    }
+}
+
 constexpr int kCombTasks = 4; // (block, float4 group) tasks per thread and pass
+
+// Sum of float4 group g of the slot partials listed in [l0, l1), in list order.
+// The list entries are fetched kBatch at a time so that their partials load
+// together (one round of dependent loads per batch, not per entry).
+constexpr int kBatch = 8;
+__device__ __forceinline__ float4 sum_list4(const SolveDesc &d, const float4 *__restrict__ part, uint32_t l0,
+                                            uint32_t l1, uint32_t g)
+{
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (uint32_t l = l0; l < l1; l += kBatch)
+    {
+        uint32_t it[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j)
+            it[j] = (l + j < l1) ? d.list_items[l + j] : 0u;
+        float4 v[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j)
+            v[j] = (l + j < l1) ? part[size_t(it[j]) * (kTileRec / 4) + g] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j)
+            if (l + j < l1)
+            {
+                acc.x += v[j].x;
+                acc.y += v[j].y;
+                acc.z += v[j].z;
+                acc.w += v[j].w;
+            }
+    }
+    return acc;
+}
 
 // One work-group per node row: every block's slot partials summed in list
 // (tile) order, the ARAP terms (node_graph_reg_jacobian.cuh:166-358) and the
@@ -409,10 +623,11 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc 
     const uint32_t tid = threadIdx.x;
     const uint32_t base = d.row_ptr[i] + i;
     const uint32_t L1 = d.row_ptr[i + 1] - d.row_ptr[i] + 1;
-    // LDS: regblk[L1][36] | colmap u16[N] | red[16]
+    // LDS: regblk[L1][36] | colmap u16[N] | red[16] | rt[kTileThreads] term parameters
     float *regblk = smem;
     uint16_t *colmap = reinterpret_cast<uint16_t *>(regblk + size_t(L1) * 36);
     float *red = reinterpret_cast<float *>(colmap + ((d.N + 1) & ~1));
+    RegTerm *rt = reinterpret_cast<RegTerm *>(red + 16);
     for (uint32_t t = tid; t < L1; t += kTileThreads)
         colmap[d.blk_col[base + t]] = uint16_t(t);
     for (uint32_t t = tid; t < L1 * 36; t += kTileThreads)
@@ -423,76 +638,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc 
 
     const uint32_t dend = d.n2t_off[i] + d.dense_cnt[i];
     const uint32_t end = d.n2t_off[i + 1];
-    if (tid < 64)
-    {
-        const int lane = int(tid);
-        float gA = 0.f;
-        const uint32_t P = d.P, E = d.E;
-        for (uint32_t it = dend; it < end; ++it)
-        {
-            const uint32_t t = d.n2t_terms[it];
-            if (t < P + E)
-            {
-                const uint32_t e = t - P;
-                const float4 *rec = reinterpret_cast<const float4 *>(d.reg + size_t(e) * kReg);
-                const float4 u0 = rec[0], u1 = rec[1];
-                if (u1.w == 0.f)
-                    continue;
-                const f3 a = mk3(u0.x, u0.y, u0.z), b = mk3(u0.w, u1.x, u1.y);
-                const float cw = u1.z;
-                const uint32_t ex = d.edges[3 * e], ey = d.edges[3 * e + 1];
-                const bool is_x = (ex == i);
-                const uint32_t other = is_x ? ey : ex;
-                const float ws = d.lam2_reg * (cw * cw);
-                // unweighted ARAP rows: x side [-[a]x, I], y side [[b]x, -I]
-                // (node_graph_reg_jacobian.cuh:250-275)
-                const f3 pm = is_x ? a : b; // this node's warped point
-                const f3 po = is_x ? b : a; // the other node's
-                const float sm = is_x ? 1.f : -1.f;
-                if (lane < 36)
-                {
-                    const int aa = lane / 6, bb = lane % 6;
-                    float vd = 0.f, vo = 0.f;
-#pragma unroll
-                    for (int ch = 0; ch < 3; ++ch)
-                    {
-                        const float ga = sm * reg_elem_t(pm, ch, aa);
-                        const float gb = sm * reg_elem_t(pm, ch, bb);
-                        const float ob = -sm * reg_elem_t(po, ch, bb);
-                        vd += ga * gb;
-                        vo += ga * ob;
-                    }
-                    regblk[lane] += ws * vd;
-                    if (other != i)
-                        regblk[colmap[other] * 36 + lane] += ws * vo;
-                }
-                if (lane < 6)
-                {
-                    const float res[3] = {(a.x - b.x), (a.y - b.y), (a.z - b.z)};
-                    float v = 0.f;
-#pragma unroll
-                    for (int ch = 0; ch < 3; ++ch)
-                        v += res[ch] * (sm * reg_elem_t(pm, ch, lane));
-                    gA += ws * v;
-                }
-            }
-            else
-            {
-                const uint32_t m = t - P - E;
-                const float4 pred = d.node_pred ? d.node_pred[m] : make_float4(0.f, 0.f, 0.f, 0.f);
-                const float4 tt = d.ttrans[m];
-                const float c2 = pred.w * pred.w;
-                if (lane >= 3 && lane < 6)
-                {
-                    regblk[lane * 7] += d.lam2_nt * c2;
-                    const float tv = (lane == 3) ? tt.x - pred.x : (lane == 4) ? tt.y - pred.y : tt.z - pred.z;
-                    gA += d.lam2_nt * tv * c2;
-                }
-            }
-        }
-        if (lane < 6)
-            red[8 + lane] = gA;
-    }
+    // ARAP and node-translation terms of the row: parameters loaded one term
+    // per thread, then every entry summed over the terms in list order.
+    row_reg_terms(d, i, dend, end, colmap, regblk, red + 8, rt);
+    __syncthreads();
 
     const float4 *__restrict__ part = reinterpret_cast<const float4 *>(d.tpart);
     const size_t out0 = size_t(base) * 36;
@@ -509,15 +658,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc 
             {
                 const uint32_t c = task / (kTileRec / 4), g = task % (kTileRec / 4);
                 const uint32_t ub = d.blk_mirror[base + c];
-                const uint32_t l1 = d.list_off[ub + 1];
-                for (uint32_t l = d.list_off[ub]; l < l1; ++l)
-                {
-                    const float4 v = part[size_t(d.list_items[l]) * (kTileRec / 4) + g];
-                    acc[s].x += v.x;
-                    acc[s].y += v.y;
-                    acc[s].z += v.z;
-                    acc[s].w += v.w;
-                }
+                acc[s] = sum_list4(d, part, d.list_off[ub], d.list_off[ub + 1], g);
             }
         }
         __syncthreads(); // regblk complete (wave 0) before the first writes
@@ -676,54 +817,94 @@ __global__ __launch_bounds__(64) void k_mf_rows(const SolveDesc *__restrict__ sd
         xi[q] = x[size_t(i) * 6 + q];
     // dense: diagonal-slot partials of block (i, i)
     float dense = 0.f;
-    const uint32_t l1 = d.list_off[base + 1];
-    for (uint32_t l = d.list_off[base]; l < l1; ++l)
-        dense += d.tpart[size_t(d.list_items[l]) * kTileRec + a];
-    // ARAP and node translation, in list order
+    {
+        const uint32_t l0 = d.list_off[base], l1 = d.list_off[base + 1];
+        for (uint32_t l = l0; l < l1; l += kBatch)
+        {
+            uint32_t it[kBatch];
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j)
+                it[j] = (l + j < l1) ? d.list_items[l + j] : 0u;
+            float v[kBatch];
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j)
+                v[j] = (l + j < l1) ? d.tpart[size_t(it[j]) * kTileRec + a] : 0.f;
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j)
+                if (l + j < l1)
+                    dense += v[j];
+        }
+    }
+    // ARAP and node translation: each lane takes one term of the list (64 at a
+    // time), then lanes 0..5 add the terms' contributions in list order
+    __shared__ float contrib[64][6];
     float reg = 0.f;
     const uint32_t dend = d.n2t_off[i] + d.dense_cnt[i], end = d.n2t_off[i + 1];
     const uint32_t P = d.P, E = d.E;
-    for (uint32_t it = dend; it < end; ++it)
+    for (uint32_t c0 = dend; c0 < end; c0 += 64)
     {
-        const uint32_t t = d.n2t_terms[it];
-        if (t < P + E)
+        const uint32_t n = min(64u, end - c0);
+        float yt[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (uint32_t(lane) < n)
         {
-            const uint32_t e = t - P;
-            const float4 *rec = reinterpret_cast<const float4 *>(d.reg + size_t(e) * kReg);
-            const float4 u0 = rec[0], u1 = rec[1];
-            if (u1.w == 0.f)
-                continue;
-            const f3 pa = mk3(u0.x, u0.y, u0.z), pb = mk3(u0.w, u1.x, u1.y);
-            const float cw = u1.z;
-            const uint32_t ex = d.edges[3 * e], ey = d.edges[3 * e + 1];
-            const bool is_x = (ex == i);
-            const uint32_t other = is_x ? ey : ex;
-            const float ws = d.lam2_reg * (cw * cw);
-            const f3 pm = is_x ? pa : pb, po = is_x ? pb : pa;
-            const float sm = is_x ? 1.f : -1.f;
-            float xo[6];
-#pragma unroll
-            for (int q = 0; q < 6; ++q)
-                xo[q] = x[size_t(other) * 6 + q];
-            float v = 0.f;
-#pragma unroll
-            for (int ch = 0; ch < 3; ++ch)
+            const uint32_t t = d.n2t_terms[c0 + lane];
+            if (t < P + E)
             {
-                float rc = 0.f; // (J_m x_i + J_o x_o)[ch]
+                const uint32_t e = t - P;
+                const float4 *rec = reinterpret_cast<const float4 *>(d.reg + size_t(e) * kReg);
+                const float4 u0 = rec[0], u1 = rec[1];
+                if (u1.w != 0.f)
+                {
+                    const f3 pa = mk3(u0.x, u0.y, u0.z), pb = mk3(u0.w, u1.x, u1.y);
+                    const float cw = u1.z;
+                    const uint32_t ex = d.edges[3 * e], ey = d.edges[3 * e + 1];
+                    const bool is_x = (ex == i);
+                    const uint32_t other = is_x ? ey : ex;
+                    const float ws = d.lam2_reg * (cw * cw);
+                    const f3 pm = is_x ? pa : pb, po = is_x ? pb : pa;
+                    const float sm = is_x ? 1.f : -1.f;
+                    float xo[6];
 #pragma unroll
-                for (int b = 0; b < 6; ++b)
-                    rc += (sm * reg_elem_t(pm, ch, b)) * xi[b] + (-sm * reg_elem_t(po, ch, b)) * xo[b];
-                v += (sm * reg_elem_t(pm, ch, a)) * rc;
+                    for (int q = 0; q < 6; ++q)
+                        xo[q] = x[size_t(other) * 6 + q];
+                    float rc[3];
+#pragma unroll
+                    for (int ch = 0; ch < 3; ++ch)
+                    {
+                        float v = 0.f; // (J_m x_i + J_o x_o)[ch]
+#pragma unroll
+                        for (int b = 0; b < 6; ++b)
+                            v += (sm * reg_elem_t(pm, ch, b)) * xi[b] + (-sm * reg_elem_t(po, ch, b)) * xo[b];
+                        rc[ch] = v;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 6; ++q)
+                    {
+                        float v = 0.f;
+#pragma unroll
+                        for (int ch = 0; ch < 3; ++ch)
+                            v += (sm * reg_elem_t(pm, ch, q)) * rc[ch];
+                        yt[q] = ws * v;
+                    }
+                }
             }
-            reg += ws * v;
+            else
+            {
+                const uint32_t m = t - P - E;
+                const float4 pred = d.node_pred ? d.node_pred[m] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                for (int q = 3; q < 6; ++q)
+                    yt[q] = d.lam2_nt * (pred.w * pred.w) * xi[q];
+            }
         }
-        else
-        {
-            const uint32_t m = t - P - E;
-            const float4 pred = d.node_pred ? d.node_pred[m] : make_float4(0.f, 0.f, 0.f, 0.f);
-            if (a >= 3)
-                reg += d.lam2_nt * (pred.w * pred.w) * xi[a];
-        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            contrib[lane][q] = yt[q];
+        __syncthreads();
+        if (lane < 6)
+            for (uint32_t k = 0; k < n; ++k)
+                reg += contrib[k][a];
+        __syncthreads();
     }
     if (lane < 6)
         y[size_t(i) * 6 + a] = d.damping * xi[a] + dense + reg;
@@ -871,7 +1052,8 @@ hipError_t launch_spmv_rows(const SolveDesc *sds, const uint32_t *wmap, int n_so
 
 size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len)
 {
-    return sizeof(float) * ((size_t(max_row_len) + 1) * 36 + 16) + sizeof(uint16_t) * ((max_nodes + 1) & ~1u);
+    return sizeof(float) * ((size_t(max_row_len) + 1) * 36 + 16) + sizeof(uint16_t) * ((max_nodes + 1) & ~1u) +
+           sizeof(RegTerm) * kTileThreads;
 }
 
 hipError_t launch_tiles_combine(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, size_t lds,

@@ -339,6 +339,12 @@ def autotune(handle, batch_solves: int | None = None, repeats: int = 3) -> dict:
     return {"flags": t.flags, "spmv_mode": t.spmv_mode, "pcg_rows_per_group": t.pcg_rows_per_group, "ms": t.ms}
 
 
+def set_tuning(solver, flags: int, spmv_mode: int, pcg_rows_per_group: int) -> None:
+    """Applies a tuning to a WarpSolver without timing anything (sh_solver_set_tuning)."""
+    t = Tuning(flags, spmv_mode, pcg_rows_per_group, 0.0)
+    check(lib().sh_solver_set_tuning(solver._h, ctypes.byref(t)), "sh_solver_set_tuning")
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("solver", SolverConfig), ("max_active", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64)]
 

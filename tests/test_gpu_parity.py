@@ -316,6 +316,36 @@ def test_capacity_errors(torch_dev):
         s.set_inputs(frame)
 
 
+def test_out_of_range_ids_rejected(torch_dev):
+    """A visible surfel naming a node id >= num_nodes, an ARAP edge naming one,
+    and an index_map entry >= num_surfels are each rejected with SH_E_ARG by
+    the range check that runs before any index kernel reads the ids; the
+    restored frame then solves as before, bit for bit."""
+    import dataclasses
+    from starhub_amd import native, scene, workload
+    torch = torch_dev
+    sc = scene.generate(scene.config_c1())
+    frame, _ = workload.device_frame(sc)
+    s = workload.solver_for(sc)
+    s.process_frame(frame)
+    good = s.node_se3().cpu().numpy()
+    idx = sc.index_map.reshape(-1)  # uint32, 0xffffffff = empty
+    first = int(np.flatnonzero(idx != 0xFFFFFFFF)[0])
+    surfel = int(idx[first])
+    knn = frame.surfel_knn.clone()
+    knn.view(-1)[surfel * sc.cfg.K + 1] = frame.num_nodes
+    edges = frame.edges.clone()
+    edges.view(-1)[3 * (frame.num_edges - 1) + 1] = frame.num_nodes
+    imap = frame.index_map.clone()
+    imap.view(-1)[first] = frame.num_surfels
+    for bad in (dataclasses.replace(frame, surfel_knn=knn), dataclasses.replace(frame, edges=edges),
+                dataclasses.replace(frame, index_map=imap)):
+        with pytest.raises(native.SolverError, match="invalid argument"):
+            s.process_frame(bad)
+    s.process_frame(frame)
+    assert np.array_equal(s.node_se3().cpu().numpy(), good)
+
+
 def test_served_same_alone_and_in_mix(torch_dev):
     """Serving: requests of 6 streams of uneven sizes, interleaved, run with
     continuous batching (4 slots: solves at different GN iterations share
