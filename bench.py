@@ -104,7 +104,9 @@ KERNEL_OF_PHASE = {"dense_terms": "k_dense_terms", "reg_terms": "k_reg_terms", "
 def pcg_kernel(tuning: dict) -> str:
     if tuning["spmv_mode"] == 1 or tuning["flags"] & 4:
         return "k_pcg_spmv/k_pcg_update"
-    return "k_pcg_coop" if tuning["flags"] & 8 else "k_pcg_resident"
+    if tuning["flags"] & 8:
+        return "k_pcg_coop"
+    return "k_pcg_pipe" if tuning["flags"] & 256 else "k_pcg_resident"
 
 
 def measured_traffic(kernel: str, workload: str):

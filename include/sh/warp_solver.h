@@ -75,7 +75,11 @@ typedef enum sh_solver_flags
     SH_FLAG_COST_HISTORY = 64u,
     /* Issue the split / matrix-free PCG launches one by one instead of replaying them
      * from a captured hipGraph. */
-    SH_FLAG_NO_GRAPH = 128u
+    SH_FLAG_NO_GRAPH = 128u,
+    /* Register-resident PCG in its pipelined form (Ghysels-Vanroose): one barrier per
+     * iteration instead of two, FP64 vectors.  The same iterates as the reference's PCG
+     * in exact arithmetic (BlockPCG.cuh:325-483). */
+    SH_FLAG_PIPELINED_PCG = 256u
 } sh_solver_flags;
 
 /* Dual quaternion as stored by the reference: two float4 (x, y, z, w),
@@ -335,7 +339,8 @@ int sh_server_get_stats(sh_server *server, sh_server_stats *stats);
  * tuning is what can change between solves without reallocating. */
 typedef struct sh_tuning
 {
-    uint32_t flags;              /* subset of SH_FLAG_ROW_SEGMENTS | SH_FLAG_STREAMING_PCG | SH_FLAG_NO_PERSISTENT_PCG */
+    uint32_t flags;              /* subset of SH_FLAG_ROW_SEGMENTS | SH_FLAG_STREAMING_PCG | SH_FLAG_NO_PERSISTENT_PCG |
+                                    SH_FLAG_PIPELINED_PCG */
     uint32_t spmv_mode;          /* sh_spmv_mode */
     uint32_t pcg_rows_per_group; /* register-resident PCG: node rows per work-group, 1..85 (0 = 85) */
     double ms;                   /* autotune: median solve time of the chosen tuning */
@@ -343,7 +348,9 @@ typedef struct sh_tuning
 
 /* Times each candidate tuning on the inputs already set (median of `repeats`
  * solves after one warm-up, HIP events on `stream`), keeps the fastest and
- * returns it.  Every candidate solves the same problem to the same tolerance. */
+ * returns it; a candidate must beat an earlier one by 2% to replace it, so
+ * near-ties keep the earlier (default-first) choice.  Every candidate solves
+ * the same problem to the same tolerance. */
 int sh_solver_autotune(sh_solver *solver, sh_stream_t stream, uint32_t repeats, sh_tuning *best);
 int sh_solver_set_tuning(sh_solver *solver, const sh_tuning *tuning);
 int sh_batch_autotune(sh_batch *batch, uint32_t num_solves, sh_stream_t stream, uint32_t repeats, sh_tuning *best);

@@ -81,6 +81,7 @@ struct SolveDesc
     float *rhs;       // N*6  = -J^T W r
     // PCG vectors (6N each)
     float *x, *r, *s, *q, *dv[2];
+    double *mv[2];    // pipelined PCG: the exchanged vector m = M^-1 w, 6N each, by iteration parity
     double *delta;    // pcg_iters + 1 : r.s per iteration
     double *cost;     // weighted cost (dense + ARAP) at the start of each GN iteration, then the final one
     double *cost_part; // work-group partials: ceil(P/256) dense, then ceil((E+N)/256) ARAP

@@ -104,7 +104,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         return SH_E_ARG;
     if (c.pcg_iterations == 0 || c.pcg_iterations > 1000 || c.gn_iterations > 1000)
         return SH_E_ARG;
-    if (c.stream_precision > SH_STREAM_FP16 || c.spmv_mode > SH_SPMV_AUTO || (c.flags & ~255u))
+    if (c.stream_precision > SH_STREAM_FP16 || c.spmv_mode > SH_SPMV_AUTO || (c.flags & ~511u))
         return SH_E_ARG;
     cfg_ = c;
     if (cfg_.flags & SH_FLAG_PAIR_MAJOR)
@@ -172,6 +172,8 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         A(d.q, size_t(ncap_) * 6 * 4);
         A(d.dv[0], size_t(ncap_) * 6 * 4);
         A(d.dv[1], size_t(ncap_) * 6 * 4);
+        A(d.mv[0], size_t(ncap_) * 6 * 8);
+        A(d.mv[1], size_t(ncap_) * 6 * 8);
         A(d.delta, size_t(c.pcg_iterations + 1) * 8);
         A(d.cost, size_t(c.gn_iterations + 2) * 8);
         A(d.cost_part, (size_t(div_up(pcap_, 256)) + div_up(ecap_ + ncap_, 256) + 1) * 8);
@@ -218,8 +220,10 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
     A(d_part_rs_, pcg_wg * 8);
     A(d_part_dq_, pcg_wg * 8);
     A(d_slots_, size_t(max_slots) * kPcgMaxGroups * sizeof(PcgSlot));
+    A(d_slots2_, size_t(max_slots) * kPcgMaxGroups * sizeof(PcgSlot2));
     A(d_flags_, 64);
     SH_HIP(hipMemset(d_slots_, 0, size_t(max_slots) * kPcgMaxGroups * sizeof(PcgSlot)));
+    SH_HIP(hipMemset(d_slots2_, 0, size_t(max_slots) * kPcgMaxGroups * sizeof(PcgSlot2)));
     SH_HIP(hipMemset(d_flags_, 0, 64));
 #undef A
     SH_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_meta_), 64, hipHostMallocDefault));
@@ -645,7 +649,11 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
     }
     if (res_max_groups_ < 0)
         res_max_groups_ = pcg_resident_max_groups();
-    const bool resident = coop_resident_ && map_total_[kMapCoop] <= uint32_t(res_max_groups_);
+    if (pipe_max_groups_ < 0)
+        pipe_max_groups_ = pcg_pipe_max_groups();
+    const bool pipelined = (cfg_.flags & SH_FLAG_PIPELINED_PCG) != 0;
+    const bool resident =
+        coop_resident_ && map_total_[kMapCoop] <= uint32_t(pipelined ? pipe_max_groups_ : res_max_groups_);
     // the matrix-free operator reads the tile index: not with the other materialisations
     const bool matrix_free = cfg_.spmv_mode == SH_SPMV_MATRIX_FREE;
     if (matrix_free && (cfg_.flags & (SH_FLAG_PAIR_MAJOR | SH_FLAG_ROW_SEGMENTS)))
@@ -693,7 +701,10 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
         SH_HIP(mark(kPhPcg, st));
         if (use_coop)
         {
-            if (resident)
+            if (resident && pipelined)
+                SH_HIP(launch_pcg_pipe(sds, map(kMapCoop, ns), n, map_total_[kMapCoop], int(cfg_.pcg_iterations),
+                                       static_cast<PcgSlot2 *>(d_slots2_), flags + 0, flags + 1, st));
+            else if (resident)
                 SH_HIP(launch_pcg_resident(sds, map(kMapCoop, ns), n, map_total_[kMapCoop],
                                            int(cfg_.pcg_iterations), static_cast<PcgSlot *>(d_slots_), flags + 0,
                                            flags + 1, st));
@@ -798,8 +809,10 @@ hipError_t Engine::mark(int phase, hipStream_t st)
         return hipSuccess;
     if (n_marks_ == marks_.size())
     {
+        // no system-scope fence: the marks order device work only, and a fence
+        // per mark would add an L2 write-back to every phase it times
         hipEvent_t e;
-        if (hipError_t r = hipEventCreate(&e); r != hipSuccess)
+        if (hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableSystemFence); r != hipSuccess)
             return r;
         marks_.push_back({e, 0});
     }
@@ -916,7 +929,8 @@ int Engine::apply_jtj(uint32_t slot, const float *x, float *y, bool matrix_free,
 
 // Tuning that can change between solves without reallocating: the
 // materialisation path, the PCG variant and the resident PCG's rows per group.
-static constexpr uint32_t kTunableFlags = SH_FLAG_ROW_SEGMENTS | SH_FLAG_STREAMING_PCG | SH_FLAG_NO_PERSISTENT_PCG;
+static constexpr uint32_t kTunableFlags =
+    SH_FLAG_ROW_SEGMENTS | SH_FLAG_STREAMING_PCG | SH_FLAG_NO_PERSISTENT_PCG | SH_FLAG_PIPELINED_PCG;
 
 int Engine::set_tuning(const sh_tuning &t)
 {
@@ -948,6 +962,8 @@ int Engine::autotune(uint32_t ns, hipStream_t st, uint32_t repeats, sh_tuning *b
     std::vector<sh_tuning> cands;
     for (uint32_t rows : {85u, 64u, 48u, 32u})
         cands.push_back({0u, SH_SPMV_MATERIALIZED, rows, 0.0});
+    for (uint32_t rows : {85u, 64u, 48u, 32u})
+        cands.push_back({SH_FLAG_PIPELINED_PCG, SH_SPMV_MATERIALIZED, rows, 0.0});
     cands.push_back({SH_FLAG_STREAMING_PCG, SH_SPMV_MATERIALIZED, kPcgResRows, 0.0});
     cands.push_back({SH_FLAG_NO_PERSISTENT_PCG, SH_SPMV_MATERIALIZED, kPcgResRows, 0.0});
     cands.push_back({SH_FLAG_ROW_SEGMENTS, SH_SPMV_MATERIALIZED, kPcgResRows, 0.0});
@@ -981,7 +997,9 @@ int Engine::autotune(uint32_t ns, hipStream_t st, uint32_t repeats, sh_tuning *b
             break;
         std::sort(t.begin(), t.end());
         c.ms = t[t.size() / 2];
-        if (c.ms < best.ms)
+        // a later candidate must be 2% faster to replace an earlier one, so that
+        // near-ties resolve to the same (earlier, default-first) choice run to run
+        if (c.ms < best.ms * (best.ms < 1e30 ? 0.98 : 1.0))
             best = c;
     }
     (void)hipEventDestroy(e0);

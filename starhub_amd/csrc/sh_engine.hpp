@@ -133,8 +133,10 @@ private:
     size_t coop_lds_ = 0;
     bool coop_resident_ = false; // every solve of the launch fits the register-resident PCG
     int res_max_groups_ = -1;
+    int pipe_max_groups_ = -1;
     uint32_t res_rows_ = 85; // resident-PCG rows per group (tunable, <= kPcgResRows)
     void *d_slots_ = nullptr; // persistent-PCG barrier slots
+    void *d_slots2_ = nullptr; // pipelined-PCG barrier slots (two sums each)
     void *d_flags_ = nullptr; // [0] PCG epoch base, [1] PCG barrier error
     void *sort_temp_ = nullptr;
     size_t sort_temp_bytes_ = 0;

@@ -93,6 +93,18 @@ constexpr uint32_t kPcgResBlocks = 1536;
 hipError_t launch_pcg_resident(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
                                PcgSlot *slots, uint32_t *epoch_base, uint32_t *err, hipStream_t st);
 int pcg_resident_max_groups();
+// Barrier slot of the pipelined PCG (two sums per barrier).
+struct PcgSlot2
+{
+    double v[2][2];
+    uint32_t epoch;
+    uint32_t pad[7];
+};
+// Pipelined resident PCG: one barrier per iteration, FP64 vectors; same
+// groups as the resident kernel.
+hipError_t launch_pcg_pipe(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
+                           PcgSlot2 *slots, uint32_t *epoch_base, uint32_t *err, hipStream_t st);
+int pcg_pipe_max_groups();
 constexpr uint32_t kBitmapWords = 128;
 
 // work-group geometry of the GN-phase kernels

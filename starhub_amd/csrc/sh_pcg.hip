@@ -98,7 +98,9 @@ __device__ double solve_barrier_sum(double part, PcgSlot *__restrict__ slots, ui
         {
             PcgSlot &o = slots[b0 + i];
             unsigned spins = 0;
-            while (__hip_atomic_load(&o.epoch, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < ep)
+            // relaxed polling (no cache invalidation per poll), one acquire
+            // fence once the epoch is seen
+            while (__hip_atomic_load(&o.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ep)
             {
                 __builtin_amdgcn_s_sleep(1);
                 if (++spins > kSpinCap || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
@@ -107,6 +109,7 @@ __device__ double solve_barrier_sum(double part, PcgSlot *__restrict__ slots, ui
                     break;
                 }
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             v += __hip_atomic_load(&o.v[ep & 1u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
 #pragma unroll
@@ -465,6 +468,275 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_resident(const SolveDesc *_
     }
     if (blockIdx.x == 0 && tid == 0)
         __hip_atomic_store(epoch_base, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Pipelined variant (Ghysels and Vanroose, "Hiding global synchronization
+// latency in the preconditioned conjugate gradient algorithm", Parallel
+// Computing 40(7), 2014, Alg. 4): the same Krylov iterates as the reference's
+// PCG in exact arithmetic, with ONE solve barrier per iteration instead of
+// two.  Per iteration the two dot products gamma = r.u and delta = w.u
+// (u = M^-1 r, w = A u) are local, m = M^-1 w is local, and the barrier both
+// sums the dots and publishes m; n = A m then reads the neighbours' m:
+//   beta = gamma / gamma_prev, alpha = gamma / (delta - beta gamma / alpha_prev)
+//   z = n + beta z, q = m + beta q, s = w + beta s, p = u + beta p
+//   x += alpha p, r -= alpha s, u -= alpha q, w -= alpha z.
+// The recurrences drift from the directly computed vectors in FP32, so every
+// vector is FP64 here (registers; the exchanged m as FP64 in memory) and the
+// blocks stay FP32 in registers, converted per product.  gamma_k is the
+// reference's delta_k = r_k . M^-1 r_k.  Same partition and register layout
+// as the resident kernel.
+struct PipeGroupSum
+{
+    double a, b;
+};
+
+__device__ __forceinline__ PipeGroupSum group_sum2(double a, double b, double *sh)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0)
+    {
+        sh[2 * (threadIdx.x >> 6)] = a;
+        sh[2 * (threadIdx.x >> 6) + 1] = b;
+    }
+    __syncthreads();
+    PipeGroupSum r{sh[0], sh[1]};
+#pragma unroll
+    for (int w = 1; w < kResWaves; ++w)
+    {
+        r.a += sh[2 * w];
+        r.b += sh[2 * w + 1];
+    }
+    __syncthreads();
+    return r;
+}
+
+// solve_barrier_sum for two values (same protocol, PcgSlot2 slots).
+__device__ PipeGroupSum solve_barrier_sum2(PipeGroupSum part, PcgSlot2 *__restrict__ slots, uint32_t b0, uint32_t G,
+                                           uint32_t ep, uint32_t *__restrict__ err, double *sh)
+{
+    if (threadIdx.x == 0)
+    {
+        PcgSlot2 &me = slots[blockIdx.x];
+        me.v[ep & 1u][0] = part.a;
+        me.v[ep & 1u][1] = part.b;
+        __hip_atomic_store(&me.epoch, ep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (threadIdx.x < 64)
+    {
+        double va = 0.0, vb = 0.0;
+        for (uint32_t i = threadIdx.x; i < G; i += 64)
+        {
+            PcgSlot2 &o = slots[b0 + i];
+            unsigned spins = 0;
+            while (__hip_atomic_load(&o.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ep)
+            {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kSpinCap || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                {
+                    __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            va += __hip_atomic_load(&o.v[ep & 1u][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            vb += __hip_atomic_load(&o.v[ep & 1u][1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+        {
+            va += __shfl_xor(va, o, 64);
+            vb += __shfl_xor(vb, o, 64);
+        }
+        if (threadIdx.x == 0)
+        {
+            sh[2 * kResWaves] = va;
+            sh[2 * kResWaves + 1] = vb;
+        }
+    }
+    __syncthreads();
+    const PipeGroupSum r{sh[2 * kResWaves], sh[2 * kResWaves + 1]};
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__restrict__ sds,
+                                                          const uint32_t *__restrict__ wg_begin, int n_solve, int iters,
+                                                          PcgSlot2 *__restrict__ slots,
+                                                          uint32_t *__restrict__ epoch_base,
+                                                          uint32_t *__restrict__ err)
+{
+    __shared__ double qblk[kResThreads * kResBlocksPerThread * 6];
+    __shared__ double rsh[kResThreads];
+    __shared__ double sh[2 * kResWaves + 2];
+    const int si = find_solve_p(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc &d = sds[si];
+    const uint32_t b0 = wg_begin[si], G = wg_begin[si + 1] - b0, g = blockIdx.x - b0;
+    const uint32_t N = uint32_t(d.N);
+    const uint32_t r0 = uint32_t(uint64_t(g) * N / G), r1 = uint32_t(uint64_t(g + 1) * N / G);
+    const uint32_t B0 = d.row_ptr[r0] + r0, B1 = d.row_ptr[r1] + r1;
+    const uint32_t tid = threadIdx.x;
+    const bool fits = (r1 - r0) * 6 <= uint32_t(kResThreads) && (B1 - B0) <= uint32_t(kResThreads * kResBlocksPerThread);
+    if (!fits && tid == 0)
+        __hip_atomic_store(err, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t ep = __hip_atomic_load(epoch_base, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+    uint32_t col[kResBlocksPerThread];
+    float A[kResBlocksPerThread][36];
+#pragma unroll
+    for (int j = 0; j < kResBlocksPerThread; ++j)
+    {
+        const uint32_t bb = B0 + tid + j * kResThreads;
+        const bool on = fits && bb < B1;
+        col[j] = on ? d.blk_col[bb] : 0xffffffffu;
+        const float4 *Ab = reinterpret_cast<const float4 *>(d.blk + size_t(on ? bb : 0) * 36);
+#pragma unroll
+        for (int v = 0; v < 9; ++v)
+        {
+            const float4 f = on ? Ab[v] : make_float4(0.f, 0.f, 0.f, 0.f);
+            A[j][4 * v] = f.x;
+            A[j][4 * v + 1] = f.y;
+            A[j][4 * v + 2] = f.z;
+            A[j][4 * v + 3] = f.w;
+        }
+    }
+    const bool own = fits && tid < (r1 - r0) * 6;
+    const uint32_t row = r0 + tid / 6;
+    const int a = int(tid % 6);
+    const size_t i6 = size_t(row) * 6 + a;
+    const uint32_t lrow = (tid / 6) * 6;
+    double M[6];
+    uint32_t e0 = 0, e1 = 0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+        M[c] = own ? d.dinv[size_t(row) * 36 + 6 * a + c] : 0.0;
+    if (own)
+    {
+        e0 = d.row_ptr[row] + row - B0;
+        e1 = d.row_ptr[row + 1] + row + 1 - B0;
+    }
+    // (M^-1 v)_a of the owned entry: the row's six entries through LDS
+    auto precond = [&](double v) -> double {
+        rsh[tid] = v;
+        __syncthreads();
+        double acc = 0.0;
+        if (own)
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                acc += M[c] * rsh[lrow + c];
+        __syncthreads();
+        return acc;
+    };
+    // (A v)_a of the owned entry, v published in vb by every group of the solve
+    auto apply = [&](const double *__restrict__ vb) -> double {
+        double2 pv[kResBlocksPerThread][3];
+        // the blocks look modified here, so their FP64 conversions stay inside
+        // the loop instead of being hoisted into twice the registers
+#pragma unroll
+        for (int j = 0; j < kResBlocksPerThread; ++j)
+#pragma unroll
+            for (int e = 0; e < 36; ++e)
+                __asm__ volatile("" : "+v"(A[j][e]));
+#pragma unroll
+        for (int j = 0; j < kResBlocksPerThread; ++j)
+        {
+            const double2 *src = reinterpret_cast<const double2 *>(vb + size_t(col[j] != 0xffffffffu ? col[j] : 0) * 6);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                pv[j][c] = src[c];
+        }
+#pragma unroll
+        for (int j = 0; j < kResBlocksPerThread; ++j)
+        {
+            if (col[j] == 0xffffffffu)
+                continue;
+            const double p[6] = {pv[j][0].x, pv[j][0].y, pv[j][1].x, pv[j][1].y, pv[j][2].x, pv[j][2].y};
+            double *out = qblk + size_t(tid + j * kResThreads) * 6;
+#pragma unroll
+            for (int aa = 0; aa < 6; ++aa)
+            {
+                double q = 0.0;
+#pragma unroll
+                for (int c = 0; c < 6; ++c)
+                    q = fma(double(A[j][6 * aa + c]), p[c], q);
+                out[aa] = q;
+            }
+        }
+        __syncthreads();
+        double q = 0.0;
+        for (uint32_t e = e0; e < e1; ++e)
+            q += qblk[e * 6 + a];
+        __syncthreads();
+        return q;
+    };
+
+    // r = b, u = M^-1 r, w = A u (x = 0: blockPCGZeroInitKernel, BlockPCG.cuh:68-116)
+    double r = own ? double(d.rhs[i6]) : 0.0;
+    double u = precond(r);
+    if (own)
+        d.mv[0][i6] = u;
+    (void)solve_barrier_sum2(PipeGroupSum{0.0, 0.0}, slots, b0, G, ++ep, err, sh);
+    double w = apply(d.mv[0]);
+    double x = 0.0, z = 0.0, q = 0.0, s = 0.0, p = 0.0, gprev = 0.0, aprev = 0.0;
+    for (int k = 0; k < iters; ++k)
+    {
+        const double m = precond(w);
+        double *__restrict__ mb = d.mv[(k + 1) & 1];
+        if (own)
+            mb[i6] = m;
+        const PipeGroupSum gd = solve_barrier_sum2(
+            group_sum2(own ? r * u : 0.0, own ? w * u : 0.0, sh), slots, b0, G, ++ep, err, sh);
+        const double n = apply(mb);
+        const double gam = gd.a, del = gd.b;
+        if (tid == 0 && g == 0)
+            d.delta[k] = gam;
+        const double beta = (k > 0 && gprev != 0.0) ? gam / gprev : 0.0;
+        const double den = del - ((k > 0 && aprev != 0.0) ? beta * gam / aprev : 0.0);
+        const double alpha = (den != 0.0) ? gam / den : 0.0;
+        z = fma(beta, z, n);
+        q = fma(beta, q, m);
+        s = fma(beta, s, w);
+        p = fma(beta, p, u);
+        x = fma(alpha, p, x);
+        r = fma(-alpha, s, r);
+        u = fma(-alpha, q, u);
+        w = fma(-alpha, z, w);
+        gprev = gam;
+        aprev = alpha;
+    }
+    if (own)
+    {
+        d.x[i6] = float(x);
+        d.r[i6] = float(r);
+    }
+    if (blockIdx.x == 0 && tid == 0)
+        __hip_atomic_store(epoch_base, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+hipError_t launch_pcg_pipe(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
+                           PcgSlot2 *slots, uint32_t *epoch_base, uint32_t *err, hipStream_t st)
+{
+    void *args[] = {(void *)&sds, (void *)&wmap, (void *)&n_solve, (void *)&iters,
+                    (void *)&slots, (void *)&epoch_base, (void *)&err};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_pcg_pipe), dim3(n_wg), dim3(kResThreads), args,
+                                      0, st);
+}
+
+int pcg_pipe_max_groups()
+{
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_pcg_pipe), kResThreads,
+                                                     0) != hipSuccess)
+        return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    return per_cu * cus;
 }
 
 hipError_t launch_pcg_resident(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,

@@ -179,16 +179,39 @@ def test_stage_values(c1_one_iter, torch_dev):
     assert np.abs(x - ref["pcg_x"]).max() <= 1e-3 * np.abs(ref["pcg_x"]).max()
 
 
-@pytest.mark.parametrize("flags", ["pair_major", "row_segments", "split_pcg", "streaming_pcg"])
+def test_pipelined_pcg_step(c1_one_iter, torch_dev):
+    """The pipelined resident PCG (one barrier per iteration, FP64 vectors)
+    gives the first GN step of the FP64 oracle's PCG to 1e-4 of its largest
+    entry (the standard FP32-vector PCG is held to 1e-3 above), and its first
+    preconditioned residual norm r0.M^-1 r0 equals the standard kernel's to
+    the FP32 rounding of M^-1 r0 that the standard kernel stores (1e-6)."""
+    import sho
+    from starhub_amd import native, scene, workload
+    torch = torch_dev
+    solver, ref, sc, _ = c1_one_iter
+    params = replace(sc.cfg.solver, gn_iterations=1)
+    pipe, _, _, _ = _solve(scene.generate(scene.config_c1()), params=params, flags=native.SH_FLAG_PIPELINED_PCG)
+    x = pipe.debug("x", torch.float32).cpu().numpy().astype(np.float64)
+    err = np.abs(x - ref["pcg_x"]).max() / np.abs(ref["pcg_x"]).max()
+    say("pipelined pcg step rel err", err)
+    assert err <= 1e-4
+    d_pipe = pipe.debug("delta", torch.float64).cpu().numpy()
+    d_std = solver.debug("delta", torch.float64).cpu().numpy()
+    assert abs(d_pipe[0] - d_std[0]) <= 1e-6 * abs(d_std[0])
+
+
+@pytest.mark.parametrize("flags", ["pair_major", "row_segments", "split_pcg", "streaming_pcg", "pipelined_pcg"])
 def test_alternative_paths(torch_dev, flags):
     """The pair-major materialisation (the reference's traversal), the
-    row-segment materialisation, the two-launch-per-iteration PCG and the
-    streaming persistent PCG solve C1 to the same tolerance as the default
-    (pixel tiles + register-resident persistent PCG)."""
+    row-segment materialisation, the two-launch-per-iteration PCG, the
+    streaming persistent PCG and the pipelined (one barrier per iteration)
+    resident PCG solve C1 to the same tolerance as the default (pixel tiles +
+    register-resident persistent PCG)."""
     import sho
     from starhub_amd import native, scene, workload
     f = {"pair_major": native.SH_FLAG_PAIR_MAJOR, "row_segments": native.SH_FLAG_ROW_SEGMENTS,
-         "split_pcg": native.SH_FLAG_NO_PERSISTENT_PCG, "streaming_pcg": native.SH_FLAG_STREAMING_PCG}[flags]
+         "split_pcg": native.SH_FLAG_NO_PERSISTENT_PCG, "streaming_pcg": native.SH_FLAG_STREAMING_PCG,
+         "pipelined_pcg": native.SH_FLAG_PIPELINED_PCG}[flags]
     say("alternative path", flags)
     sc = scene.generate(scene.config_c1())
     solver, frame, prod, _ = _solve(sc, flags=f)
