@@ -1,6 +1,8 @@
 // Persistent block-Jacobi PCG: the whole fixed-iteration loop of
 // BlockPCG<6>::SolveNoConvergeCheck (libs/star/pcg_solver/BlockPCG.cuh:325-483)
-// in one cooperative launch covering every solve of a batch.
+// in one launch covering every solve of a batch.  The groups must be
+// co-resident: the host keeps each grid within the kernel's occupancy bound
+// (work-groups per CU x CUs), and every barrier wait is capped.
 //
 // Per iteration two barriers among the groups of one solve replace the
 // reference's 7 launches (SpMV, 3 x cublasSdot, two update kernels, swap):
@@ -10,7 +12,6 @@
 // for every dot product, alpha and beta; fixed-order reductions.  The
 // barriers are per solve (a batch's solves never wait for each other) and
 // carry the dot products, so a barrier costs one publish and one poll.
-#include <hip/hip_cooperative_groups.h>
 
 #include "sh_common.hpp"
 #include "sh_desc.hpp"
@@ -18,7 +19,6 @@
 
 namespace sh
 {
-namespace cg = cooperative_groups;
 
 constexpr int kCoopThreads = 512;
 constexpr int kCoopWaves = kCoopThreads / 64;
@@ -485,39 +485,21 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_resident(const SolveDesc *_
 // blocks stay FP32 in registers, converted per product.  gamma_k is the
 // reference's delta_k = r_k . M^-1 r_k.  Same partition and register layout
 // as the resident kernel.
+// distinct block columns of a group kept in LDS (more: read from memory per block)
+constexpr int kPipeCols = 768;
+constexpr int kPipeFetch = (kPipeCols * 6 + kResThreads - 1) / kResThreads;
+
 struct PipeGroupSum
 {
     double a, b;
 };
 
-__device__ __forceinline__ PipeGroupSum group_sum2(double a, double b, double *sh)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-    {
-        a += __shfl_xor(a, o, 64);
-        b += __shfl_xor(b, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0)
-    {
-        sh[2 * (threadIdx.x >> 6)] = a;
-        sh[2 * (threadIdx.x >> 6) + 1] = b;
-    }
-    __syncthreads();
-    PipeGroupSum r{sh[0], sh[1]};
-#pragma unroll
-    for (int w = 1; w < kResWaves; ++w)
-    {
-        r.a += sh[2 * w];
-        r.b += sh[2 * w + 1];
-    }
-    __syncthreads();
-    return r;
-}
-
-// solve_barrier_sum for two values (same protocol, PcgSlot2 slots).
-__device__ PipeGroupSum solve_barrier_sum2(PipeGroupSum part, PcgSlot2 *__restrict__ slots, uint32_t b0, uint32_t G,
-                                           uint32_t ep, uint32_t *__restrict__ err, double *sh)
+// solve_barrier_sum for two values (same protocol, PcgSlot2 slots), without
+// group syncs of its own: the caller syncs before (the published values and
+// every store they cover are complete) and after (the solve's sums are in
+// out[0..1] for every thread).
+__device__ void solve_barrier_sum2(PipeGroupSum part, PcgSlot2 *__restrict__ slots, uint32_t b0, uint32_t G,
+                                   uint32_t ep, uint32_t *__restrict__ err, double *out)
 {
     if (threadIdx.x == 0)
     {
@@ -554,14 +536,10 @@ __device__ PipeGroupSum solve_barrier_sum2(PipeGroupSum part, PcgSlot2 *__restri
         }
         if (threadIdx.x == 0)
         {
-            sh[2 * kResWaves] = va;
-            sh[2 * kResWaves + 1] = vb;
+            out[0] = va;
+            out[1] = vb;
         }
     }
-    __syncthreads();
-    const PipeGroupSum r{sh[2 * kResWaves], sh[2 * kResWaves + 1]};
-    __syncthreads();
-    return r;
 }
 
 __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__restrict__ sds,
@@ -572,7 +550,13 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
 {
     __shared__ double qblk[kResThreads * kResBlocksPerThread * 6];
     __shared__ double rsh[kResThreads];
-    __shared__ double sh[2 * kResWaves + 2];
+    __shared__ double sh[2 * kResWaves];
+    __shared__ double sh2[2];
+    __shared__ double halo[kPipeCols * 6];
+    __shared__ uint16_t clist[kPipeCols];
+    __shared__ uint32_t cbits[kBitmapWords];
+    __shared__ uint32_t cpre[kBitmapWords];
+    __shared__ uint32_t ncols_sh;
     const int si = find_solve_p(wg_begin, n_solve, blockIdx.x);
     const SolveDesc &d = sds[si];
     const uint32_t b0 = wg_begin[si], G = wg_begin[si + 1] - b0, g = blockIdx.x - b0;
@@ -604,6 +588,50 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
             A[j][4 * v + 3] = f.w;
         }
     }
+    // the group's distinct block columns (ascending), and each block's rank
+    // among them: the exchanged vector is fetched once per distinct node
+    for (uint32_t w = tid; w < kBitmapWords; w += kResThreads)
+        cbits[w] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kResBlocksPerThread; ++j)
+        if (col[j] != 0xffffffffu)
+            atomicOr(&cbits[col[j] >> 5], 1u << (col[j] & 31)); // set union: order independent
+    __syncthreads();
+    if (tid < 64)
+    {
+        const uint32_t c0 = __popc(cbits[2 * tid]), c1 = __popc(cbits[2 * tid + 1]);
+        uint32_t v = c0 + c1;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            const uint32_t u = __shfl_up(v, o, 64);
+            if (int(tid) >= o)
+                v += u;
+        }
+        cpre[2 * tid] = v - c0 - c1;
+        cpre[2 * tid + 1] = v - c1;
+        if (tid == 63)
+            ncols_sh = v;
+    }
+    __syncthreads();
+    const uint32_t ncols = ncols_sh;
+    const bool lds_halo = ncols <= uint32_t(kPipeCols); // uniform over the group
+    uint32_t lcol[kResBlocksPerThread];
+#pragma unroll
+    for (int j = 0; j < kResBlocksPerThread; ++j)
+    {
+        lcol[j] = 0;
+        if (col[j] != 0xffffffffu)
+        {
+            const uint32_t w = col[j] >> 5;
+            lcol[j] = cpre[w] + __popc(cbits[w] & ((1u << (col[j] & 31)) - 1u));
+            if (lds_halo)
+                clist[lcol[j]] = uint16_t(col[j]); // every writer of a rank writes the same id
+        }
+    }
+    __syncthreads();
+
     const bool own = fits && tid < (r1 - r0) * 6;
     const uint32_t row = r0 + tid / 6;
     const int a = int(tid % 6);
@@ -619,19 +647,8 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
         e0 = d.row_ptr[row] + row - B0;
         e1 = d.row_ptr[row + 1] + row + 1 - B0;
     }
-    // (M^-1 v)_a of the owned entry: the row's six entries through LDS
-    auto precond = [&](double v) -> double {
-        rsh[tid] = v;
-        __syncthreads();
-        double acc = 0.0;
-        if (own)
-#pragma unroll
-            for (int c = 0; c < 6; ++c)
-                acc += M[c] * rsh[lrow + c];
-        __syncthreads();
-        return acc;
-    };
-    // (A v)_a of the owned entry, v published in vb by every group of the solve
+    // (A v)_a of the owned entry, v published in vb by every group of the
+    // solve (the caller has passed the solve barrier); two group syncs
     auto apply = [&](const double *__restrict__ vb) -> double {
         double2 pv[kResBlocksPerThread][3];
         // the blocks look modified here, so their FP64 conversions stay inside
@@ -641,13 +658,46 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
 #pragma unroll
             for (int e = 0; e < 36; ++e)
                 __asm__ volatile("" : "+v"(A[j][e]));
-#pragma unroll
-        for (int j = 0; j < kResBlocksPerThread; ++j)
+        if (lds_halo)
         {
-            const double2 *src = reinterpret_cast<const double2 *>(vb + size_t(col[j] != 0xffffffffu ? col[j] : 0) * 6);
+            // the distinct nodes' six entries, in 48-byte runs of ascending nodes
+            const uint32_t n6 = ncols * 6;
+            double hv[kPipeFetch];
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-                pv[j][c] = src[c];
+            for (int f = 0; f < kPipeFetch; ++f)
+            {
+                const uint32_t i = tid + uint32_t(f) * kResThreads;
+                hv[f] = i < n6 ? vb[size_t(clist[i / 6]) * 6 + i % 6] : 0.0;
+            }
+#pragma unroll
+            for (int f = 0; f < kPipeFetch; ++f)
+            {
+                const uint32_t i = tid + uint32_t(f) * kResThreads;
+                if (i < n6)
+                    halo[i] = hv[f];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kResBlocksPerThread; ++j)
+            {
+                const double2 *src = reinterpret_cast<const double2 *>(halo + size_t(lcol[j]) * 6);
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    pv[j][c] = src[c];
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int j = 0; j < kResBlocksPerThread; ++j)
+            {
+                const double2 *src =
+                    reinterpret_cast<const double2 *>(vb + size_t(col[j] != 0xffffffffu ? col[j] : 0) * 6);
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    pv[j][c] = src[c];
+            }
+            __syncthreads(); // keeps the two paths' sync counts equal
         }
 #pragma unroll
         for (int j = 0; j < kResBlocksPerThread; ++j)
@@ -659,37 +709,82 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
 #pragma unroll
             for (int aa = 0; aa < 6; ++aa)
             {
-                double q = 0.0;
+                double qq = 0.0;
 #pragma unroll
                 for (int c = 0; c < 6; ++c)
-                    q = fma(double(A[j][6 * aa + c]), p[c], q);
-                out[aa] = q;
+                    qq = fma(double(A[j][6 * aa + c]), p[c], qq);
+                out[aa] = qq;
             }
         }
         __syncthreads();
-        double q = 0.0;
-        for (uint32_t e = e0; e < e1; ++e)
-            q += qblk[e * 6 + a];
+        // the row's block products: four interleaved partial sums, then a fixed tree
+        double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+        uint32_t e = e0;
+        for (; e + 4 <= e1; e += 4)
+        {
+            q0 += qblk[e * 6 + a];
+            q1 += qblk[(e + 1) * 6 + a];
+            q2 += qblk[(e + 2) * 6 + a];
+            q3 += qblk[(e + 3) * 6 + a];
+        }
+        for (; e < e1; ++e)
+            q0 += qblk[e * 6 + a];
+        return (q0 + q1) + (q2 + q3);
+    };
+    // Stage 1 of an iteration: v's row entries and the wave partials of two
+    // dot products go to LDS, one sync, then (M^-1 v)_a and the group sums.
+    auto stage = [&](double v, double da, double db, double &mv, PipeGroupSum &gs) {
+        rsh[tid] = v;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+        {
+            da += __shfl_xor(da, o, 64);
+            db += __shfl_xor(db, o, 64);
+        }
+        if ((tid & 63) == 0)
+        {
+            sh[2 * (tid >> 6)] = da;
+            sh[2 * (tid >> 6) + 1] = db;
+        }
         __syncthreads();
-        return q;
+        double acc = 0.0;
+        if (own)
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                acc += M[c] * rsh[lrow + c];
+        mv = acc;
+        gs = PipeGroupSum{sh[0], sh[1]};
+#pragma unroll
+        for (int w = 1; w < kResWaves; ++w)
+        {
+            gs.a += sh[2 * w];
+            gs.b += sh[2 * w + 1];
+        }
     };
 
     // r = b, u = M^-1 r, w = A u (x = 0: blockPCGZeroInitKernel, BlockPCG.cuh:68-116)
     double r = own ? double(d.rhs[i6]) : 0.0;
-    double u = precond(r);
+    double u = 0.0;
+    PipeGroupSum gs{};
+    stage(r, 0.0, 0.0, u, gs);
     if (own)
         d.mv[0][i6] = u;
-    (void)solve_barrier_sum2(PipeGroupSum{0.0, 0.0}, slots, b0, G, ++ep, err, sh);
+    __syncthreads(); // every m store is complete before the publish
+    (void)solve_barrier_sum2(PipeGroupSum{0.0, 0.0}, slots, b0, G, ++ep, err, sh2);
+    __syncthreads();
     double w = apply(d.mv[0]);
     double x = 0.0, z = 0.0, q = 0.0, s = 0.0, p = 0.0, gprev = 0.0, aprev = 0.0;
     for (int k = 0; k < iters; ++k)
     {
-        const double m = precond(w);
+        double m = 0.0;
+        stage(w, own ? r * u : 0.0, own ? w * u : 0.0, m, gs);
         double *__restrict__ mb = d.mv[(k + 1) & 1];
         if (own)
             mb[i6] = m;
-        const PipeGroupSum gd = solve_barrier_sum2(
-            group_sum2(own ? r * u : 0.0, own ? w * u : 0.0, sh), slots, b0, G, ++ep, err, sh);
+        __syncthreads(); // m stores complete; rsh and sh read by every thread
+        (void)solve_barrier_sum2(gs, slots, b0, G, ++ep, err, sh2);
+        __syncthreads();
+        const PipeGroupSum gd{sh2[0], sh2[1]};
         const double n = apply(mb);
         const double gam = gd.a, del = gd.b;
         if (tid == 0 && g == 0)
@@ -720,10 +815,14 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
 hipError_t launch_pcg_pipe(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
                            PcgSlot2 *slots, uint32_t *epoch_base, uint32_t *err, hipStream_t st)
 {
-    void *args[] = {(void *)&sds, (void *)&wmap, (void *)&n_solve, (void *)&iters,
-                    (void *)&slots, (void *)&epoch_base, (void *)&err};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_pcg_pipe), dim3(n_wg), dim3(kResThreads), args,
-                                      0, st);
+    // A plain launch: the kernel needs its groups co-resident (the host keeps
+    // n_wg within the occupancy bound, pcg_pipe_max_groups) but no grid-wide
+    // sync, and every barrier wait is capped, so a group that is placed late
+    // only delays the others.  The cooperative launch API adds a per-launch
+    // cost of about 20 us on this device.
+    hipLaunchKernelGGL(k_pcg_pipe, dim3(n_wg), dim3(kResThreads), 0, st, sds, wmap, n_solve, iters, slots, epoch_base,
+                       err);
+    return hipGetLastError();
 }
 
 int pcg_pipe_max_groups()
@@ -742,10 +841,10 @@ int pcg_pipe_max_groups()
 hipError_t launch_pcg_resident(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
                                PcgSlot *slots, uint32_t *epoch_base, uint32_t *err, hipStream_t st)
 {
-    void *args[] = {(void *)&sds, (void *)&wmap, (void *)&n_solve, (void *)&iters,
-                    (void *)&slots, (void *)&epoch_base, (void *)&err};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_pcg_resident), dim3(n_wg), dim3(kResThreads),
-                                      args, 0, st);
+    // plain launch, as for k_pcg_pipe (n_wg within pcg_resident_max_groups)
+    hipLaunchKernelGGL(k_pcg_resident, dim3(n_wg), dim3(kResThreads), 0, st, sds, wmap, n_solve, iters, slots,
+                       epoch_base, err);
+    return hipGetLastError();
 }
 
 int pcg_resident_max_groups()
@@ -764,8 +863,6 @@ int pcg_resident_max_groups()
 hipError_t launch_pcg_coop(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
                            PcgSlot *slots, uint32_t *epoch_base, uint32_t *err, uint32_t tile, hipStream_t st)
 {
-    void *args[] = {(void *)&sds, (void *)&wmap, (void *)&n_solve, (void *)&iters,
-                    (void *)&slots, (void *)&epoch_base, (void *)&err, (void *)&tile};
     const size_t lds = size_t(tile) * 6 * sizeof(float);
     if (lds > 65536)
     {
@@ -774,8 +871,10 @@ hipError_t launch_pcg_coop(const SolveDesc *sds, const uint32_t *wmap, int n_sol
         if (e != hipSuccess)
             return e;
     }
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_pcg_coop), dim3(n_wg), dim3(kCoopThreads), args,
-                                      lds, st);
+    // plain launch, as for k_pcg_pipe (n_wg within pcg_coop_max_groups(lds))
+    hipLaunchKernelGGL(k_pcg_coop, dim3(n_wg), dim3(kCoopThreads), lds, st, sds, wmap, n_solve, iters, slots, epoch_base,
+                       err, tile);
+    return hipGetLastError();
 }
 
 int pcg_coop_max_groups(size_t lds)
