@@ -17,10 +17,17 @@ hipError_t sort_pairs_u16_u32(void *temp, size_t &temp_bytes, const uint16_t *ki
     return rocprim::radix_sort_pairs(temp, temp_bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), st);
 }
 
+// Merge-sort limit 0: 32-bit keys always take the onesweep LSD path (stable;
+// for a few hundred thousand 15..24-bit keys it is a fraction of the
+// block-sort + merge-pass path rocPRIM picks below 1M items by default).
+using OnesweepOnly = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                                rocprim::default_config, 0>;
+
 hipError_t sort_pairs_u32_u32(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout,
                               const uint32_t *vin, uint32_t *vout, uint32_t n, int end_bit, hipStream_t st)
 {
-    return rocprim::radix_sort_pairs(temp, temp_bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), st);
+    return rocprim::radix_sort_pairs<OnesweepOnly>(temp, temp_bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit),
+                                                   st);
 }
 
 hipError_t exclusive_scan_u32(void *temp, size_t &temp_bytes, const uint32_t *in, uint32_t *out, uint32_t n,

@@ -206,7 +206,11 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_build(SolveDesc d, uint16
             }
         }
     }
-    Sort().sort(keys, vals, sort_storage);
+    // keys are (block << 7 | pixel) with block < N + 2U: sort only the bits in
+    // use (an invalid key's low bits, all ones, still sort after every valid key)
+    const uint32_t nb_keys = (d.row_ptr[d.N] + uint32_t(d.N)) << 7;
+    const unsigned end_bit = min(32u, 32u - unsigned(__clz(nb_keys)));
+    Sort().sort(keys, vals, sort_storage, 0u, end_bit);
 
     // slot heads: item gi = tid * kItems + i starts a slot when its block
     // differs from the previous item's (blocked arrangement after the sort)
