@@ -256,6 +256,48 @@ int sh_batch_set_inputs(sh_batch *batch, uint32_t slot, const sh_camera *camera,
 int sh_batch_solve(sh_batch *batch, uint32_t num_solves, sh_stream_t stream);
 int sh_batch_node_se3(const sh_batch *batch, uint32_t slot, const sh_dual_quat **device_view, uint32_t *num_nodes);
 
+/* ---------------------------------------------- depth -> measurement maps
+ * The step before the solve: SurfelMapInitializer::InitFromRGBDImage's
+ * geometry half (libs/star/geometry/geometry_map/SurfelMapInitializer.cu:
+ * 213-301), which MeasureProcessorOffline::ProcessFrame runs on every frame
+ * (examples/mono_star/measure/MeasureProcessorOffline.cu:81).  Images are
+ * linear row-major device arrays; reads outside an image are zero (the
+ * reference's border-addressed textures). */
+typedef struct sh_depth_camera
+{
+    uint32_t width, height;    /* raw depth image */
+    float fx, fy, cx, cy;      /* raw intrinsics (config "intrinsic") */
+    float clip_near, clip_far; /* metres */
+    float downsample_scale;    /* solver maps are floor(scale * width) x floor(scale * height) */
+    float surfel_radius_scale; /* system "surfel_radius_scale" */
+} sh_depth_camera;
+
+/* filterUnreliableDepth (libs/star/img_proc/surfel_reliable_filter.cu:37-191):
+ * raw depth [mm] -> filtered depth [m], raw size.  0 where the depth is nearer
+ * than clip_near or the 7x7 window normal is missing (<= 24 valid points) or
+ * faces away (camera direction . normal > -0.1); 10 beyond clip_far. */
+int sh_filter_depth(const uint16_t *depth_mm, const sh_depth_camera *camera, float *filtered_depth_m,
+                    sh_stream_t stream);
+
+/* computeRawVertexFromDepth + filterAndScaleVertex + createNormalRadiusMap
+ * (SurfelMapInitializer.cu:83-137, 273-343; libs/star/img_proc/normal_map.cu:
+ * 32-136): filtered depth -> the solver's Measure4Solver maps at the
+ * downsampled size: vertex_confid (float4 x, y, z [m], 1; zero outside
+ * [clip_near, clip_far]) and normal_radius (float4 normal, surfel radius [mm]).
+ * Nearest downsampling: scaled pixel (x, y) takes raw texel
+ * (int(x / scale), int(y / scale)). */
+int sh_measure_maps_from_depth(const float *filtered_depth_m, const sh_depth_camera *camera, float *vertex_confid,
+                               float *normal_radius, sh_stream_t stream);
+
+/* createNormalRadiusMap (normal_map.cu:32-136) on any float4 vertex map. */
+int sh_normal_radius_map(const float *vertex_confid, uint32_t width, uint32_t height, float surfel_radius_scale,
+                         float *normal_radius, sh_stream_t stream);
+
+/* createVertexConfidMap (libs/star/img_proc/vertex_map.cu:13-57): raw depth
+ * [mm] -> float4 vertex map (x, y, z [m], confidence 1), no filtering. */
+int sh_vertex_confid_map(const uint16_t *depth_mm, uint32_t width, uint32_t height, float fx, float fy, float cx,
+                         float cy, float *vertex_confid, sh_stream_t stream);
+
 /* SurfelNodeDeformer::ForwardWarpSurfelsAndNodes (libs/star/geometry/surfel/
  * SurfelNodeDeformer.cu:13-80): live surfels and nodes from the reference ones
  * under a warp field (num_nodes dual quaternions).  Each node DQ is normalised

@@ -1,6 +1,7 @@
 // C-ABI over the CPU restatement (sh_oracle.hpp) so that the Python tests and
 // the CPU-baseline leg of the benchmark can drive it through ctypes.
 // TEST INFRASTRUCTURE ONLY: never linked into the product library.
+#include "sh_measure.hpp"
 #include "sh_oracle.hpp"
 
 #include <chrono>
@@ -313,5 +314,99 @@ extern "C"
             valid[e] = s.valid[e];
         }
         return s.P;
+    }
+
+    // ---------------------------------------------------- measurement maps
+    struct sho_depth_camera
+    {
+        int width, height;
+        float fx, fy, cx, cy, clip_near, clip_far, scale, radius_scale;
+    };
+    static sho::DepthCamera depth_cam(const sho_depth_camera *c)
+    {
+        return {c->width, c->height, c->fx, c->fy, c->cx, c->cy, c->clip_near, c->clip_far, c->scale, c->radius_scale};
+    }
+
+    // smallest eigenvector of n packed PSD 3x3 matrices (00 01 02 11 12 22)
+    void sho_eigen33(const double *cov6, int n, double *vec3, int precision)
+    {
+        for (int i = 0; i < n; ++i)
+        {
+            if (precision == 32)
+            {
+                float c[6];
+                for (int k = 0; k < 6; ++k)
+                    c[k] = float(cov6[6 * i + k]);
+                const sho::V3<float> v = sho::Eig33<float>::smallest(c);
+                vec3[3 * i] = v.x;
+                vec3[3 * i + 1] = v.y;
+                vec3[3 * i + 2] = v.z;
+            }
+            else
+            {
+                const sho::V3<double> v = sho::Eig33<double>::smallest(cov6 + 6 * i);
+                vec3[3 * i] = v.x;
+                vec3[3 * i + 1] = v.y;
+                vec3[3 * i + 2] = v.z;
+            }
+        }
+    }
+
+    void sho_filter_depth(const uint16_t *raw, const sho_depth_camera *c, float *out, float *camera_dot,
+                          int precision)
+    {
+        if (precision == 32)
+            sho::filter_depth<float>(raw, depth_cam(c), out, camera_dot);
+        else
+            sho::filter_depth<double>(raw, depth_cam(c), out, camera_dot);
+    }
+
+    // filtered depth -> scaled vertex (w*h*4) and normal/radius (w*h*4) maps
+    void sho_measure_maps(const float *filtered, const sho_depth_camera *c, double *vertex4, double *normal4,
+                          int precision)
+    {
+        const sho::DepthCamera cam = depth_cam(c);
+        int w, h;
+        sho::scaled_size(cam, w, h);
+        const size_t n = size_t(w) * h * 4;
+        if (precision == 32)
+        {
+            std::vector<float> v(n), nr(n);
+            sho::scaled_vertex_map<float>(filtered, cam, v.data());
+            sho::normal_radius_map<float>(v.data(), w, h, cam.radius_scale, nr.data());
+            for (size_t i = 0; i < n; ++i)
+            {
+                vertex4[i] = v[i];
+                normal4[i] = nr[i];
+            }
+        }
+        else
+        {
+            sho::scaled_vertex_map<double>(filtered, cam, vertex4);
+            sho::normal_radius_map<double>(vertex4, w, h, cam.radius_scale, normal4);
+        }
+    }
+
+    void sho_normal_radius_map(const float *vertex4, int w, int h, float radius_scale, double *normal4, int precision)
+    {
+        const size_t n = size_t(w) * h * 4;
+        if (precision == 32)
+        {
+            std::vector<float> nr(n);
+            sho::normal_radius_map<float>(vertex4, w, h, radius_scale, nr.data());
+            for (size_t i = 0; i < n; ++i)
+                normal4[i] = nr[i];
+        }
+        else
+        {
+            std::vector<double> v(vertex4, vertex4 + n);
+            sho::normal_radius_map<double>(v.data(), w, h, radius_scale, normal4);
+        }
+    }
+
+    void sho_vertex_confid_map(const uint16_t *raw, int w, int h, float fx, float fy, float cx, float cy,
+                               float *vertex4)
+    {
+        sho::vertex_confid_map<float>(raw, w, h, fx, fy, cx, cy, vertex4);
     }
 }

@@ -273,3 +273,71 @@ def forward_warp(warp_wxyz, K, ref_v, ref_n, s_knn, s_spatial, s_connect, ref_no
     if rc != 0:
         raise ValueError("sho_forward_warp rejected its arguments")
     return lv, ln, lnode[:Nn]
+
+
+# ------------------------------------------------------------ measurement maps
+class DepthCamera(ctypes.Structure):
+    """Raw depth camera: image size, intrinsics, clip range [m], downsample scale, surfel radius scale."""
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int), ("fx", ctypes.c_float), ("fy", ctypes.c_float),
+                ("cx", ctypes.c_float), ("cy", ctypes.c_float), ("clip_near", ctypes.c_float),
+                ("clip_far", ctypes.c_float), ("scale", ctypes.c_float), ("radius_scale", ctypes.c_float)]
+
+
+def scaled_size(cam: DepthCamera) -> tuple[int, int]:
+    """(w, h) of the downsampled maps: truncation of the float product, as filterAndScaleVertex."""
+    f = np.float32
+    return int(f(cam.width) * f(cam.scale)), int(f(cam.height) * f(cam.scale))
+
+
+def eigen33(cov6, precision: int = 64):
+    """eigen33::compute restated: the smallest-eigenvalue eigenvector of packed PSD 3x3 matrices (n x 6)."""
+    cov = np.ascontiguousarray(np.asarray(cov6, np.float64).reshape(-1, 6))
+    out = np.zeros((cov.shape[0], 3))
+    lib().sho_eigen33(cov.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), cov.shape[0],
+                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), precision)
+    return out
+
+
+def filter_depth(raw_mm, cam: DepthCamera, precision: int = 32):
+    """filterUnreliableDepth restated: (filtered depth [m], camera_dot statistic) at the raw size."""
+    raw = np.ascontiguousarray(raw_mm, np.uint16)
+    out = np.zeros(raw.shape, np.float32)
+    dots = np.zeros(raw.shape, np.float32)
+    lib().sho_filter_depth(raw.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), ctypes.byref(cam),
+                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                           dots.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), precision)
+    return out, dots
+
+
+def measure_maps(filtered_m, cam: DepthCamera, precision: int = 32):
+    """Filtered depth -> (vertex_confid, normal_radius), each h x w x 4 at the downsampled size."""
+    f = np.ascontiguousarray(filtered_m, np.float32)
+    w, h = scaled_size(cam)
+    v = np.zeros((h, w, 4))
+    n = np.zeros((h, w, 4))
+    lib().sho_measure_maps(f.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(cam),
+                           v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                           n.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), precision)
+    return v, n
+
+
+def normal_radius_map(vertex4, radius_scale: float, precision: int = 32):
+    """createNormalRadiusMap restated on an h x w x 4 float vertex map."""
+    v = np.ascontiguousarray(vertex4, np.float32)
+    h, w = v.shape[:2]
+    n = np.zeros((h, w, 4))
+    lib().sho_normal_radius_map(v.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), w, h,
+                                ctypes.c_float(radius_scale), n.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                precision)
+    return n
+
+
+def vertex_confid_map(raw_mm, fx, fy, cx, cy):
+    """createVertexConfidMap restated: h x w x 4 float32."""
+    raw = np.ascontiguousarray(raw_mm, np.uint16)
+    h, w = raw.shape
+    v = np.zeros((h, w, 4), np.float32)
+    lib().sho_vertex_confid_map(raw.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), w, h, ctypes.c_float(fx),
+                                ctypes.c_float(fy), ctypes.c_float(cx), ctypes.c_float(cy),
+                                v.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return v

@@ -1313,6 +1313,66 @@ extern "C"
         return SH_OK;
     }
 
+    static int hip_rc(hipError_t e)
+    {
+        if (e != hipSuccess)
+        {
+            sh::last_hip_error = e;
+            return SH_E_HIP;
+        }
+        return SH_OK;
+    }
+
+    static bool depth_camera_ok(const sh_depth_camera *c)
+    {
+        return c && c->width > 0 && c->height > 0 && c->width <= 16384 && c->height <= 16384 && c->fx != 0.f &&
+               c->fy != 0.f && c->downsample_scale > 0.f && c->downsample_scale <= 1.f;
+    }
+
+    int sh_filter_depth(const uint16_t *depth_mm, const sh_depth_camera *c, float *filtered, sh_stream_t stream)
+    {
+        if (!depth_camera_ok(c) || !depth_mm || !filtered)
+            return SH_E_ARG;
+        return hip_rc(sh::launch_filter_depth(depth_mm, int(c->width), int(c->height), c->fx, c->fy, c->cx, c->cy,
+                                              c->clip_near, c->clip_far, filtered,
+                                              reinterpret_cast<hipStream_t>(stream)));
+    }
+
+    int sh_measure_maps_from_depth(const float *filtered, const sh_depth_camera *c, float *vertex_confid,
+                                   float *normal_radius, sh_stream_t stream)
+    {
+        if (!depth_camera_ok(c) || !filtered || !vertex_confid || !normal_radius)
+            return SH_E_ARG;
+        // filterAndScaleVertex's size: the float product, truncated (SurfelMapInitializer.cu:319-320)
+        const int w = int(unsigned(float(c->width) * c->downsample_scale));
+        const int h = int(unsigned(float(c->height) * c->downsample_scale));
+        const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+        int rc = hip_rc(sh::launch_scaled_vertex(filtered, int(c->width), int(c->height), w, h, c->downsample_scale,
+                                                 c->fx, c->fy, c->cx, c->cy, c->clip_near, c->clip_far,
+                                                 vertex_confid, st));
+        if (rc == SH_OK)
+            rc = hip_rc(sh::launch_normal_radius(vertex_confid, w, h, c->surfel_radius_scale, normal_radius, st));
+        return rc;
+    }
+
+    int sh_normal_radius_map(const float *vertex_confid, uint32_t width, uint32_t height, float radius_scale,
+                             float *normal_radius, sh_stream_t stream)
+    {
+        if (!vertex_confid || !normal_radius || width > 16384 || height > 16384)
+            return SH_E_ARG;
+        return hip_rc(sh::launch_normal_radius(vertex_confid, int(width), int(height), radius_scale, normal_radius,
+                                               reinterpret_cast<hipStream_t>(stream)));
+    }
+
+    int sh_vertex_confid_map(const uint16_t *depth_mm, uint32_t width, uint32_t height, float fx, float fy, float cx,
+                             float cy, float *vertex_confid, sh_stream_t stream)
+    {
+        if (!depth_mm || !vertex_confid || width > 16384 || height > 16384 || fx == 0.f || fy == 0.f)
+            return SH_E_ARG;
+        return hip_rc(sh::launch_vertex_confid(depth_mm, int(width), int(height), fx, fy, cx, cy, vertex_confid,
+                                               reinterpret_cast<hipStream_t>(stream)));
+    }
+
     int sh_forward_warp(const sh_dual_quat *warp_field, uint32_t num_nodes, uint32_t knn, const float *ref_v4,
                         const float *ref_n4, const uint16_t *s_knn, const float *s_spatial, const float *s_connect,
                         uint32_t num_surfels, const float *ref_node4, const uint16_t *n_knn, const float *n_spatial,

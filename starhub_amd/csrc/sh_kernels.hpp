@@ -113,6 +113,15 @@ constexpr uint32_t kWavesPerWg = 4;
 constexpr uint32_t kPcgNodesPerWg = 32;  // split PCG kernels
 
 // Forward warp of surfels and nodes (sh_warp.hip)
+// depth -> measurement maps (sh_measure.hip)
+hipError_t launch_filter_depth(const uint16_t *raw, int W, int H, float fx, float fy, float cx, float cy,
+                               float clip_near, float clip_far, float *out, hipStream_t st);
+hipError_t launch_scaled_vertex(const float *filtered, int W, int H, int w, int h, float scale, float fx, float fy,
+                                float cx, float cy, float clip_near, float clip_far, float *vertex4, hipStream_t st);
+hipError_t launch_normal_radius(const float *vertex4, int w, int h, float radius_scale, float *normal4,
+                                hipStream_t st);
+hipError_t launch_vertex_confid(const uint16_t *raw, int W, int H, float fx, float fy, float cx, float cy,
+                                float *vertex4, hipStream_t st);
 hipError_t launch_forward_warp(uint32_t K, const float4 *warp, const float4 *ref_v, const float4 *ref_n,
                                const uint16_t *s_knn, const float *s_spatial, const float *s_connect,
                                uint32_t num_surfels, const float4 *ref_node, const uint16_t *n_knn,

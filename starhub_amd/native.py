@@ -485,3 +485,58 @@ def build_node_graph(nodes, K: int, r2: float, stream=None):
                                     ctypes.c_void_p(w.data_ptr()), ctypes.c_void_p(edges.data_ptr()),
                                     ctypes.c_void_p(c.data_ptr()), _stream_ptr(stream)), "sh_build_node_graph")
     return knn, w, edges, c
+
+
+# ------------------------------------------------------------ depth -> measurement maps
+class DepthCamera(ctypes.Structure):
+    """sh_depth_camera: raw depth image size, raw intrinsics, clip range [m], downsample scale, surfel radius scale."""
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("fx", ctypes.c_float),
+                ("fy", ctypes.c_float), ("cx", ctypes.c_float), ("cy", ctypes.c_float),
+                ("clip_near", ctypes.c_float), ("clip_far", ctypes.c_float), ("downsample_scale", ctypes.c_float),
+                ("surfel_radius_scale", ctypes.c_float)]
+
+    def scaled_size(self) -> tuple[int, int]:
+        f = np.float32
+        return int(f(self.width) * f(self.downsample_scale)), int(f(self.height) * f(self.downsample_scale))
+
+
+def filter_depth(depth_mm, cam: DepthCamera, stream=None):
+    """sh_filter_depth: H x W int16 (uint16 bits) device tensor of depth [mm] -> H x W float32 filtered depth [m]."""
+    import torch
+    out = torch.empty((cam.height, cam.width), dtype=torch.float32, device="cuda")
+    check(lib().sh_filter_depth(ctypes.c_void_p(depth_mm.data_ptr()), ctypes.byref(cam),
+                                ctypes.c_void_p(out.data_ptr()), _stream_ptr(stream)), "sh_filter_depth")
+    return out
+
+
+def measure_maps(filtered_m, cam: DepthCamera, stream=None):
+    """sh_measure_maps_from_depth: filtered depth -> (vertex_confid, normal_radius), each h x w x 4 float32."""
+    import torch
+    w, h = cam.scaled_size()
+    v = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+    n = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+    check(lib().sh_measure_maps_from_depth(ctypes.c_void_p(filtered_m.data_ptr()), ctypes.byref(cam),
+                                           ctypes.c_void_p(v.data_ptr()), ctypes.c_void_p(n.data_ptr()),
+                                           _stream_ptr(stream)), "sh_measure_maps_from_depth")
+    return v, n
+
+
+def normal_radius_map(vertex4, radius_scale: float, stream=None):
+    """sh_normal_radius_map on an h x w x 4 float32 device vertex map."""
+    import torch
+    h, w = vertex4.shape[:2]
+    n = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+    check(lib().sh_normal_radius_map(ctypes.c_void_p(vertex4.data_ptr()), w, h, ctypes.c_float(radius_scale),
+                                     ctypes.c_void_p(n.data_ptr()), _stream_ptr(stream)), "sh_normal_radius_map")
+    return n
+
+
+def vertex_confid_map(depth_mm, fx: float, fy: float, cx: float, cy: float, stream=None):
+    """sh_vertex_confid_map: H x W depth [mm] -> H x W x 4 float32 vertex map."""
+    import torch
+    h, w = depth_mm.shape
+    v = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+    check(lib().sh_vertex_confid_map(ctypes.c_void_p(depth_mm.data_ptr()), w, h, ctypes.c_float(fx),
+                                     ctypes.c_float(fy), ctypes.c_float(cx), ctypes.c_float(cy),
+                                     ctypes.c_void_p(v.data_ptr()), _stream_ptr(stream)), "sh_vertex_confid_map")
+    return v
