@@ -69,8 +69,8 @@ struct SolveDesc
     uint32_t *tc_slot_first; // S+1: first contribution of each slot
     uint32_t *list_off;      // N+2U+1: first entry of each block in list_items
     uint32_t *list_items;    // S: slot ids sorted by block (stable: tile order)
-    uint32_t *blk_mirror;    // N+2U: the upper block holding each block's dense values
-    float *tpart;            // S * 28 slot partials (21 H, 6 g, pad)
+    uint32_t *blk_transpose; // N+2U: the index of each block's transpose (block (j, i) for (i, j))
+    float *tpart;            // S * 32 slot partials (21 H, 6 g, pad; 128 B each)
     // GN iteration state
     dq8 *dq[2];       // double buffer (SolverIterationData.h)
     void *jr;         // P*24 residual/Jacobian records, FP32 or FP16

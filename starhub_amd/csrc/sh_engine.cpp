@@ -159,7 +159,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         A(d.tile_cbase, size_t(div_up(pcap_, kTilePix) + 1) * 4);
         A(d.tile_sbase, size_t(div_up(pcap_, kTilePix) + 1) * 4);
         A(d.list_off, (size_t(ncap_) + 2 * size_t(ucap_) + 1) * 4);
-        A(d.blk_mirror, (size_t(ncap_) + 2 * size_t(ucap_)) * 4);
+        A(d.blk_transpose, (size_t(ncap_) + 2 * size_t(ucap_)) * 4);
         A(d.jr, size_t(pcap_) * kHG * (fp16_ ? 2 : 4));
         A(d.reg, size_t(ecap_) * kReg * 4);
         A(d.ttrans, size_t(ncap_) * 16);
@@ -468,7 +468,7 @@ int Engine::build_tiles(Slot &sl, hipStream_t st)
     if (S > 0)
         SH_HIP(sort_pairs_u32_u32(sort_temp_, tb, k32a, k32b, p32a, d.list_items, S, end_bit, st));
     SH_HIP(launch_lower_bound_u32(k32b, S, nb + 1, d.list_off, st));
-    SH_HIP(launch_block_mirror(d, st));
+    SH_HIP(launch_block_transpose(d, st));
     return SH_OK;
 }
 
@@ -480,7 +480,7 @@ int Engine::size_tiles(Slot &sl, uint32_t n_slots, uint32_t n_contrib)
     const size_t b_contrib = up(size_t(n_contrib) * 2 + 2);
     const size_t b_first = up((size_t(n_slots) + 1) * 4);
     const size_t b_items = up(size_t(n_slots) * 4 + 4);
-    const size_t b_part = up(size_t(n_slots) * 28 * 4 + 16);
+    const size_t b_part = up(size_t(n_slots) * kPartStride * 4 + 16); // 128-byte partials, 256-aligned base
     const size_t need = b_contrib + b_first + b_items + b_part;
     if (need > sl.tile_bytes)
     {
@@ -1125,7 +1125,7 @@ int Engine::debug_view(uint32_t slot, const char *name, const void **ptr, size_t
         {"tile_cbase", d.tile_cbase, size_t(d.n_tiles) + 1},
         {"tile_sbase", d.tile_sbase, size_t(d.n_tiles) + 1},
         {"list_off", d.list_off, N + 2 * U + 1},
-        {"blk_mirror", d.blk_mirror, N + 2 * U},
+        {"blk_transpose", d.blk_transpose, N + 2 * U},
         {"reg", d.reg, E * kReg},
         {"blk", d.blk, (N + 2 * U) * 36},
         {"dinv", d.dinv, N * 36},

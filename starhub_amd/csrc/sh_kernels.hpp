@@ -106,6 +106,9 @@ hipError_t launch_pcg_pipe(const SolveDesc *sds, const uint32_t *wmap, int n_sol
                            PcgSlot2 *slots, uint32_t *epoch_base, uint32_t *err, hipStream_t st);
 int pcg_pipe_max_groups();
 constexpr uint32_t kBitmapWords = 128;
+// Floats per tile-slot partial in HBM: the 28 of the record (21 H, 6 g, pad)
+// padded to 32, so a partial is one 128-byte line of an aligned buffer.
+constexpr int kPartStride = 32;
 
 // work-group geometry of the GN-phase kernels
 constexpr uint32_t kThreads = 256;
@@ -135,7 +138,7 @@ hipError_t launch_tile_build(const SolveDesc &d, uint16_t *ws_contrib, uint32_t 
 hipError_t launch_tile_compact(const SolveDesc &d, const uint16_t *ws_contrib, const uint32_t *ws_slot_block,
                                const uint16_t *ws_slot_first, uint32_t *sort_keys, uint32_t *sort_vals,
                                hipStream_t st);
-hipError_t launch_block_mirror(const SolveDesc &d, hipStream_t st);
+hipError_t launch_block_transpose(const SolveDesc &d, hipStream_t st);
 hipError_t launch_lower_bound_u32(const uint32_t *keys, uint32_t n_keys, uint32_t n_out, uint32_t *out,
                                   hipStream_t st);
 hipError_t launch_tiles(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,

@@ -284,14 +284,14 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_compact(SolveDesc d, cons
 
 // For every block of row i: the block holding the same dense values on the
 // upper side ((i, c) itself for c >= i, (c, i) for c < i).
-__global__ __launch_bounds__(kTileThreads) void k_block_mirror(SolveDesc d)
+__global__ __launch_bounds__(kTileThreads) void k_block_transpose(SolveDesc d)
 {
     const uint32_t i = blockIdx.x;
     const uint32_t base = d.row_ptr[i] + i, L1 = d.row_ptr[i + 1] - d.row_ptr[i] + 1;
     for (uint32_t c = threadIdx.x; c < L1; c += kTileThreads)
     {
         const uint32_t col = d.blk_col[base + c];
-        d.blk_mirror[base + c] = col >= i ? base + c : block_of_pair(d, col, i);
+        d.blk_transpose[base + c] = col == i ? base + c : block_of_pair(d, col, i); // block (col, i) of row col
     }
 }
 
@@ -397,7 +397,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
         perm[atomicAdd(&hist[kTilePix - (sst[s + 1] - sst[s])], 1u)] = uint16_t(s);
     __syncthreads();
 
-    float4 *__restrict__ out = reinterpret_cast<float4 *>(d.tpart) + size_t(s0) * (kTileRec / 4);
+    float4 *__restrict__ out = reinterpret_cast<float4 *>(d.tpart) + size_t(s0) * (kPartStride / 4);
     for (uint32_t task = tid; task < ns * (kTileRec / 4); task += kTileThreads)
     {
         const uint32_t s = perm[task / (kTileRec / 4)], g = task % (kTileRec / 4);
@@ -436,7 +436,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
             acc.z = fmaf(coef, h.z, acc.z);
             acc.w = fmaf(coef, h.w, acc.w);
         }
-        out[size_t(s) * (kTileRec / 4) + g] = acc;
+        out[size_t(s) * (kPartStride / 4) + g] = acc;
     }
 }
 
@@ -596,7 +596,7 @@ __device__ __forceinline__ float4 sum_list4(const SolveDesc &d, const float4 *__
         float4 v[kBatch];
 #pragma unroll
         for (int j = 0; j < kBatch; ++j)
-            v[j] = (l + j < l1) ? part[size_t(it[j]) * (kTileRec / 4) + g] : make_float4(0.f, 0.f, 0.f, 0.f);
+            v[j] = (l + j < l1) ? part[size_t(it[j]) * (kPartStride / 4) + g] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int j = 0; j < kBatch; ++j)
             if (l + j < l1)
@@ -649,7 +649,20 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc 
 
     const float4 *__restrict__ part = reinterpret_cast<const float4 *>(d.tpart);
     const size_t out0 = size_t(base) * 36;
-    const uint32_t n_tasks = L1 * (kTileRec / 4);
+    // the diagonal block and the upper blocks (columns above i) of the row:
+    // each upper block (i, j) is also written transposed into (j, i), so every
+    // block's slot partials are read once; lower blocks are left to their rows
+    uint32_t c_up = 1, hi = L1; // first block with a column above i
+    while (c_up < hi)
+    {
+        const uint32_t m = (c_up + hi) >> 1;
+        if (d.blk_col[base + m] < i)
+            c_up = m + 1;
+        else
+            hi = m;
+    }
+    const uint32_t n_up = L1 - c_up; // upper blocks: c_up .. L1-1
+    const uint32_t n_tasks = (1 + n_up) * (kTileRec / 4);
     for (uint32_t tp = 0; tp < n_tasks; tp += kTileThreads * kCombTasks)
     {
         float4 acc[kCombTasks];
@@ -660,9 +673,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc 
             const uint32_t task = tp + tid + s * kTileThreads;
             if (task < n_tasks)
             {
-                const uint32_t c = task / (kTileRec / 4), g = task % (kTileRec / 4);
-                const uint32_t ub = d.blk_mirror[base + c];
-                acc[s] = sum_list4(d, part, d.list_off[ub], d.list_off[ub + 1], g);
+                const uint32_t t7 = task / (kTileRec / 4), g = task % (kTileRec / 4);
+                const uint32_t c = t7 == 0 ? 0u : c_up + t7 - 1;
+                acc[s] = sum_list4(d, part, d.list_off[base + c], d.list_off[base + c + 1], g);
             }
         }
         __syncthreads(); // regblk complete (wave 0) before the first writes
@@ -672,7 +685,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc 
             const uint32_t task = tp + tid + s * kTileThreads;
             if (task >= n_tasks)
                 continue;
-            const uint32_t c = task / (kTileRec / 4), g = task % (kTileRec / 4);
+            const uint32_t t7 = task / (kTileRec / 4), g = task % (kTileRec / 4);
+            const uint32_t c = t7 == 0 ? 0u : c_up + t7 - 1;
+            const size_t tr = size_t(d.blk_transpose[base + c]) * 36; // (j, i) for c > 0
             const float av[4] = {acc[s].x, acc[s].y, acc[s].z, acc[s].w};
 #pragma unroll
             for (int q = 0; q < 4; ++q)
@@ -686,8 +701,16 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc 
                     if (c == 0 && a == b)
                         u += d.damping;
                     d.blk[out0 + size_t(c) * 36 + a * 6 + b] = u;
+                    const float v = av[q] + regblk[c * 36 + b * 6 + a];
                     if (a != b)
-                        d.blk[out0 + size_t(c) * 36 + b * 6 + a] = av[q] + regblk[c * 36 + b * 6 + a];
+                        d.blk[out0 + size_t(c) * 36 + b * 6 + a] = v;
+                    if (c > 0)
+                    {
+                        // the transposed block: (j, i)[b][a] = (i, j)[a][b]
+                        d.blk[tr + b * 6 + a] = u;
+                        if (a != b)
+                            d.blk[tr + a * 6 + b] = v;
+                    }
                 }
                 else if (c == 0 && e < 27)
                     red[e - 21] = av[q]; // g of the diagonal slots
@@ -795,7 +818,7 @@ __global__ __launch_bounds__(kTileThreads) void k_mf_tiles(const SolveDesc *__re
             const uint32_t pl = c & 127u, ka = (c >> 7) & 7u;
             acc = fmaf(wS[pl * K + ka], uS[pl * 6 + comp], acc);
         }
-        d.tpart[size_t(s0 + s) * kTileRec + comp] = acc;
+        d.tpart[size_t(s0 + s) * kPartStride + comp] = acc;
     }
 }
 
@@ -832,7 +855,7 @@ __global__ __launch_bounds__(64) void k_mf_rows(const SolveDesc *__restrict__ sd
             float v[kBatch];
 #pragma unroll
             for (int j = 0; j < kBatch; ++j)
-                v[j] = (l + j < l1) ? d.tpart[size_t(it[j]) * kTileRec + a] : 0.f;
+                v[j] = (l + j < l1) ? d.tpart[size_t(it[j]) * kPartStride + a] : 0.f;
 #pragma unroll
             for (int j = 0; j < kBatch; ++j)
                 if (l + j < l1)
@@ -971,11 +994,11 @@ hipError_t launch_tile_compact(const SolveDesc &d, const uint16_t *ws_contrib, c
     return hipGetLastError();
 }
 
-hipError_t launch_block_mirror(const SolveDesc &d, hipStream_t st)
+hipError_t launch_block_transpose(const SolveDesc &d, hipStream_t st)
 {
     if (d.N == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_block_mirror, dim3(d.N), dim3(kTileThreads), 0, st, d);
+    hipLaunchKernelGGL(k_block_transpose, dim3(d.N), dim3(kTileThreads), 0, st, d);
     return hipGetLastError();
 }
 
