@@ -215,8 +215,12 @@ int sh_solver_node_se3(const sh_solver *solver, const sh_dual_quat **device_view
  * (upper triangle), node->term entries. */
 int sh_solver_sizes(const sh_solver *solver, uint32_t *num_pixels, uint32_t *num_pairs, uint32_t *num_node_terms);
 
-/* Device views of internal arrays for parity checks (names in DESIGN.md):
- * returns the device pointer and element count. */
+/* Device views of internal arrays for parity checks: returns the device
+ * pointer and element count.  Names: pix, pknn, pw, n2t_off, n2t_terms,
+ * half_keys, half_off, pair_entries, upper_pos, lower_pos (with
+ * SH_FLAG_REFERENCE_INDEX), row_ptr, blk_col, hg, dense_cnt, seg_ptr,
+ * tile_cbase, tile_sbase, list_off, blk_transpose, reg, blk, dinv, rhs, x,
+ * dq0, dq1, delta. */
 int sh_solver_debug_view(const sh_solver *solver, const char *name, const void **device_ptr, size_t *count);
 
 /* FP64 total weighted cost sum lambda^2 r^2 (dense depth + flow, ARAP) at the start
