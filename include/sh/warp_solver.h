@@ -302,6 +302,27 @@ int sh_normal_radius_map(const float *vertex_confid, uint32_t width, uint32_t he
 int sh_vertex_confid_map(const uint16_t *depth_mm, uint32_t width, uint32_t height, float fx, float fy, float cx,
                          float cy, float *vertex_confid, sh_stream_t stream);
 
+/* ------------------------------------------------------- solver maps
+ * Renderer::DrawSolverMapsWithRecentObservation / ...ConfidentObservation
+ * (libs/star/geometry/render/RendererDraw.cu:104-174, shaders/solver_map.vert
+ * :21-53, solver_map.frag:1-17): the model surfels rendered as one-pixel
+ * points into the solver's reference maps, without OpenGL.  Per surfel:
+ * p = world2camera (row-major, host) * (x, y, z, 1); culled when p.z >
+ * max_depth or p.z <= 0, or when its confidence (vertex w) is below 1 and
+ * |color_time.z - current_time| >= 3 (always, with_recent_observation = 0);
+ * color_time4 NULL skips that test.  Pixel (floor(fx p.x/p.z + cx + 0.5),
+ * floor(fy p.y/p.z + cy + 0.5)) of camera (the downsampled size and
+ * intrinsics); the nearest point wins (depth (p.z/max_depth)/2 + 1/2 < 1),
+ * equal depths go to the lower surfel index.  Outputs (width*height, row-major):
+ * reference vertex (x, y, z, 1) and normal (x, y, z, 0) of the winner, its
+ * index; zero / 0xffffffff where no point lands.  zbuffer: width*height*8
+ * bytes of caller workspace. */
+int sh_render_solver_maps(const float *reference_vertex4, const float *reference_normal4, const float *color_time4,
+                          uint32_t num_surfels, const float world2camera[16], const sh_camera *camera,
+                          float max_depth, float current_time, int with_recent_observation, void *zbuffer,
+                          float *reference_vertex_map4, float *reference_normal_map4, uint32_t *index_map,
+                          sh_stream_t stream);
+
 /* SurfelNodeDeformer::ForwardWarpSurfelsAndNodes (libs/star/geometry/surfel/
  * SurfelNodeDeformer.cu:13-80): live surfels and nodes from the reference ones
  * under a warp field (num_nodes dual quaternions).  Each node DQ is normalised

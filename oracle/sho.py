@@ -341,3 +341,56 @@ def vertex_confid_map(raw_mm, fx, fy, cx, cy):
                                 ctypes.c_float(fy), ctypes.c_float(cx), ctypes.c_float(cy),
                                 v.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
     return v
+
+
+# ------------------------------------------------------------ solver-map rasteriser
+def render_solver_maps(ref_vertex4, ref_normal4, color_time4, world2camera, width, height, fx, fy, cx, cy,
+                       max_depth, current_time, with_recent_observation=True):
+    """Renderer::drawSolverMaps restated (libs/star/geometry/render/RendererDraw.cu:104-155 with
+    shaders/solver_map.vert:21-53 and solver_map.frag:1-17): every model surfel is a one-pixel point.
+
+    Per surfel, in float32 and in the order written:
+      camera point p = R v + t of the row-major world2camera (rows summed left to right);
+      culled if p.z > max_depth, p.z <= 0, or (confidence < 1 and |time - current_time| >= 3;
+      the time bound is -1 without recent observations), with color_time4 None skipping this test;
+      pixel (floor(fx p.x / p.z + cx + 0.5), floor(fy p.y / p.z + cy + 0.5)), dropped outside the image;
+      depth (p.z / max_depth) * 0.5 + 0.5 (the [0, 1] depth range of a 32-bit float depth buffer
+      cleared to 1, GL_LESS): the nearest point wins, equal depths go to the lower vertex id (the
+      in-order depth test).
+    Returns (reference_vertex h x w x 4 with w = 1, reference_normal h x w x 4 with w = 0,
+    index_map h x w uint32, 0xffffffff where empty)."""
+    f = np.float32
+    V = np.ascontiguousarray(ref_vertex4, f).reshape(-1, 4)
+    Nn = np.ascontiguousarray(ref_normal4, f).reshape(-1, 4)
+    M = np.asarray(world2camera, f).reshape(4, 4)
+    x, y, z = V[:, 0], V[:, 1], V[:, 2]
+    px = ((M[0, 0] * x + M[0, 1] * y) + M[0, 2] * z) + M[0, 3]
+    py = ((M[1, 0] * x + M[1, 1] * y) + M[1, 2] * z) + M[1, 3]
+    pz = ((M[2, 0] * x + M[2, 1] * y) + M[2, 2] * z) + M[2, 3]
+    keep = ~((pz > f(max_depth)) | (pz <= f(0)))
+    if color_time4 is not None:
+        T = np.ascontiguousarray(color_time4, f).reshape(-1, 4)
+        tthr = f(3) if with_recent_observation else f(-1)
+        keep &= ~((V[:, 3] < f(1)) & (np.abs(T[:, 2] - f(current_time)) >= tthr))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        wx = ((f(fx) * px) / pz + f(cx)) + f(0.5)
+        wy = ((f(fy) * py) / pz + f(cy)) + f(0.5)
+        u = np.floor(np.where(keep, wx, f(-1)))
+        v = np.floor(np.where(keep, wy, f(-1)))
+    keep &= (u >= 0) & (u < width) & (v >= 0) & (v < height)
+    ids = np.nonzero(keep)[0]
+    depth = (pz[ids] / f(max_depth)) * f(0.5) + f(0.5)
+    ids, depth = ids[depth < f(1)], depth[depth < f(1)]  # depth 1 does not pass GL_LESS against the clear value
+    key = (depth.astype(f).view(np.uint32).astype(np.uint64) << np.uint64(32)) | ids.astype(np.uint64)
+    zbuf = np.full(width * height, np.iinfo(np.uint64).max, np.uint64)
+    np.minimum.at(zbuf, v[ids].astype(np.int64) * width + u[ids].astype(np.int64), key)
+    hit = zbuf != np.iinfo(np.uint64).max
+    win = (zbuf[hit] & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    vert = np.zeros((width * height, 4), f)
+    nrm = np.zeros((width * height, 4), f)
+    index = np.full(width * height, 0xFFFFFFFF, np.uint32)
+    vert[hit, :3] = V[win, :3]
+    vert[hit, 3] = 1
+    nrm[hit, :3] = Nn[win, :3]
+    index[hit] = win.astype(np.uint32)
+    return vert.reshape(height, width, 4), nrm.reshape(height, width, 4), index.reshape(height, width)

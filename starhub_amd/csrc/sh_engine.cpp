@@ -1373,6 +1373,22 @@ extern "C"
                                                reinterpret_cast<hipStream_t>(stream)));
     }
 
+    int sh_render_solver_maps(const float *vertex4, const float *normal4, const float *color_time4,
+                              uint32_t num_surfels, const float world2camera[16], const sh_camera *cam,
+                              float max_depth, float current_time, int with_recent_observation, void *zbuffer,
+                              float *out_vertex4, float *out_normal4, uint32_t *out_index, sh_stream_t stream)
+    {
+        if (!world2camera || !cam || !zbuffer || !out_vertex4 || !out_normal4 || !out_index ||
+            (num_surfels && (!vertex4 || !normal4)) || cam->width == 0 || cam->height == 0 || cam->width > 16384 ||
+            cam->height > 16384 || !(max_depth > 0.f))
+            return SH_E_ARG;
+        return hip_rc(sh::launch_render_solver_maps(
+            vertex4, normal4, color_time4, num_surfels, world2camera, int(cam->width), int(cam->height), cam->fx,
+            cam->fy, cam->cx, cam->cy, max_depth, current_time, with_recent_observation != 0,
+            static_cast<unsigned long long *>(zbuffer), out_vertex4, out_normal4, out_index,
+            reinterpret_cast<hipStream_t>(stream)));
+    }
+
     int sh_forward_warp(const sh_dual_quat *warp_field, uint32_t num_nodes, uint32_t knn, const float *ref_v4,
                         const float *ref_n4, const uint16_t *s_knn, const float *s_spatial, const float *s_connect,
                         uint32_t num_surfels, const float *ref_node4, const uint16_t *n_knn, const float *n_spatial,

@@ -116,6 +116,12 @@ constexpr uint32_t kWavesPerWg = 4;
 constexpr uint32_t kPcgNodesPerWg = 32;  // split PCG kernels
 
 // Forward warp of surfels and nodes (sh_warp.hip)
+// solver maps by point splatting (sh_render.hip)
+hipError_t launch_render_solver_maps(const float *vertex4, const float *normal4, const float *color_time4,
+                                     uint32_t num_surfels, const float world2camera[16], int width, int height,
+                                     float fx, float fy, float cx, float cy, float max_depth, float current_time,
+                                     bool with_recent_observation, unsigned long long *zbuf, float *out_vertex4,
+                                     float *out_normal4, uint32_t *out_index, hipStream_t st);
 // depth -> measurement maps (sh_measure.hip)
 hipError_t launch_filter_depth(const uint16_t *raw, int W, int H, float fx, float fy, float cx, float cy,
                                float clip_near, float clip_far, float *out, hipStream_t st);

@@ -540,3 +540,25 @@ def vertex_confid_map(depth_mm, fx: float, fy: float, cx: float, cy: float, stre
                                      ctypes.c_float(fy), ctypes.c_float(cx), ctypes.c_float(cy),
                                      ctypes.c_void_p(v.data_ptr()), _stream_ptr(stream)), "sh_vertex_confid_map")
     return v
+
+
+# ------------------------------------------------------------ solver maps (point splatting)
+def render_solver_maps(ref_vertex4, ref_normal4, color_time4, world2camera, width: int, height: int, fx: float,
+                       fy: float, cx: float, cy: float, max_depth: float, current_time: float,
+                       with_recent_observation: bool = True, stream=None):
+    """sh_render_solver_maps: (reference_vertex h x w x 4, reference_normal h x w x 4, index_map h x w int32
+    holding uint32 bits) as device tensors."""
+    import torch
+    n = ref_vertex4.shape[0]
+    zbuf = torch.empty(width * height, dtype=torch.int64, device="cuda")
+    v = torch.empty((height, width, 4), dtype=torch.float32, device="cuda")
+    nr = torch.empty((height, width, 4), dtype=torch.float32, device="cuda")
+    idx = torch.empty((height, width), dtype=torch.int32, device="cuda")
+    m = (ctypes.c_float * 16)(*np.asarray(world2camera, np.float32).reshape(-1).tolist())
+    cam = Camera(width, height, fx, fy, cx, cy)
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p()
+    check(lib().sh_render_solver_maps(ptr(ref_vertex4), ptr(ref_normal4), ptr(color_time4), ctypes.c_uint32(n), m,
+                                      ctypes.byref(cam), ctypes.c_float(max_depth), ctypes.c_float(current_time),
+                                      int(with_recent_observation), ptr(zbuf), ptr(v), ptr(nr), ptr(idx),
+                                      _stream_ptr(stream)), "sh_render_solver_maps")
+    return v, nr, idx
