@@ -302,6 +302,35 @@ int sh_normal_radius_map(const float *vertex_confid, uint32_t width, uint32_t he
 int sh_vertex_confid_map(const uint16_t *depth_mm, uint32_t width, uint32_t height, float fx, float fy, float cx,
                          float cy, float *vertex_confid, sh_stream_t stream);
 
+/* ------------------------------------------------- optical flow and node motion
+ * VolumeDeformFileFetch::FetchOFImage (libs/star/io/VolumeDeformFileFetch.cpp:
+ * 39-52) + createScaledOpticalFlowMap (libs/star/img_proc/optical_flow_map.cu:
+ * 5-57): a decoded .of.png (16-bit, 3 channels interleaved as read, the first
+ * two (flow + 1000) * 10 of dx, dy in raw pixels; tools/transfer_hdf5.py:42-52)
+ * -> the solver's flow map (float2) at floor(scale*width) x floor(scale*height):
+ * raw pixel (round-half-even(x / scale), ...), flow = (u16 / 10 - 1000) * scale. */
+int sh_flow_map_from_of16(const uint16_t *of16, uint32_t raw_width, uint32_t raw_height, float downsample_scale,
+                          float *flow_map2, sh_stream_t stream);
+
+/* ResetNodeMotion + AtomicSumIdealNodeMotionFromOpticalFlowKernel +
+ * AverageNodeMotion (libs/star/geometry/node_graph/node_graph_opticalflow.cu:
+ * 14-92, 180-245): the node-motion prior (sh_node_flow.node_motion_pred,
+ * float4: world motion, weight) from the flow between the previous and the
+ * current frame.  Per previous-frame pixel with a surfel s (index_map_prev) and
+ * a current depth at the flowed pixel (rgbd w = 1 / depth, point-sampled):
+ * the camera-space motion of the back-projected points (dropped when |dz| >
+ * 0.05 m), times the spatial weight of s's first KNN slot, rotated by
+ * cam2world, summed onto that node with the weight; each node is then divided
+ * by its weight sum.  The sums are exact integers of 2^-31 units (the
+ * reference's float atomics are order-dependent), so the result is
+ * deterministic.  Surfel / node ids out of range contribute nothing.
+ * workspace: num_nodes * 32 bytes. */
+int sh_node_motion_from_flow(const float *rgbd_prev4, const float *rgbd4, const float *flow_map2,
+                             const uint32_t *index_map_prev, uint32_t width, uint32_t height,
+                             const uint16_t *surfel_knn, const float *surfel_spatial_weight, uint32_t num_surfels,
+                             uint32_t knn, const float cam2world[16], float fx, float fy, float cx, float cy,
+                             uint32_t num_nodes, void *workspace, float *node_motion_pred4, sh_stream_t stream);
+
 /* ------------------------------------------------------- solver maps
  * Renderer::DrawSolverMapsWithRecentObservation / ...ConfidentObservation
  * (libs/star/geometry/render/RendererDraw.cu:104-174, shaders/solver_map.vert

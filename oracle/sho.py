@@ -394,3 +394,77 @@ def render_solver_maps(ref_vertex4, ref_normal4, color_time4, world2camera, widt
     nrm[hit, :3] = Nn[win, :3]
     index[hit] = win.astype(np.uint32)
     return vert.reshape(height, width, 4), nrm.reshape(height, width, 4), index.reshape(height, width)
+
+
+# ------------------------------------------------------------ optical flow and node motion
+def flow_map_from_of16(of16, scale: float):
+    """FetchOFImage's decode (u16 / 10 - 1000, the double expression rounded once to float) of the first two
+    channels, sampled at (round-half-even(x / scale), ...) and times scale (createScaledOpticalFlowMap)."""
+    f = np.float32
+    img = np.asarray(of16, np.uint16)
+    H, W = img.shape[:2]
+    w, h = int(np.floor(f(W) * f(scale))), int(np.floor(f(H) * f(scale)))
+    win = f(1) / f(scale)
+    rx = np.rint(np.arange(w, dtype=f) * win).astype(np.int64)  # numpy rint: round half to even
+    ry = np.rint(np.arange(h, dtype=f) * win).astype(np.int64)
+    ok = (rx[None, :] < W) & (ry[:, None] < H)
+    sub = img[np.minimum(ry, H - 1)][:, np.minimum(rx, W - 1), :2].astype(np.float64)
+    flow = (sub * 0.1 - 1000.0).astype(f)
+    flow[~ok] = 0
+    return (flow * f(scale)).astype(f)
+
+
+FIX_SCALE = 2.0 ** 31
+
+
+def node_motion_from_flow(rgbd_prev4, rgbd4, flow2, index_prev, surfel_knn, surfel_spatial, cam2world, fx, fy, cx,
+                          cy, num_nodes):
+    """AtomicSumIdealNodeMotionFromOpticalFlowKernel + AverageNodeMotionKernel restated with the exact
+    fixed-point sums (integers of 2^-31) the HIP kernels use; float32 arithmetic in the order written."""
+    f = np.float32
+    P = np.asarray(rgbd_prev4, f)
+    C = np.asarray(rgbd4, f)
+    F = np.asarray(flow2, f)
+    I = np.asarray(index_prev, np.uint32)
+    knn = np.asarray(surfel_knn, np.uint16)
+    sw = np.asarray(surfel_spatial, f)
+    R = np.asarray(cam2world, f).reshape(4, 4)[:3, :3]
+    H, W = I.shape
+    S = knn.shape[0]
+    fx, fy, cx, cy = f(fx), f(fy), f(cx), f(cy)
+    ys, xs = np.mgrid[0:H, 0:W]
+    xf = xs.astype(f) + F[..., 0]
+    yf = ys.astype(f) + F[..., 1]
+    tx, ty = np.floor(xf).astype(np.int64), np.floor(yf).astype(np.int64)
+    inside = (tx >= 0) & (ty >= 0) & (tx < W) & (ty < H)
+    cw = np.zeros((H, W), f)
+    cw[inside] = C[ty[inside], tx[inside], 3]
+    keep = (I != 0xFFFFFFFF) & (cw != 0) & (I < S)
+    s = I[keep].astype(np.int64)
+    node = knn[s, 0].astype(np.int64)
+    ok = node < num_nodes
+    s, node = s[ok], node[ok]
+    sel = np.nonzero(keep)
+    sel = (sel[0][ok], sel[1][ok])
+    wgt = sw[s, 0]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        depth = f(1) / cw[sel]
+        depth_p = f(1) / P[sel][:, 3]
+        vx = (xf[sel] - cx) / fx * depth
+        vy = (yf[sel] - cy) / fy * depth
+        ux = (xs[sel].astype(f) - cx) / fx * depth_p
+        uy = (ys[sel].astype(f) - cy) / fy * depth_p
+        mx, my, mz = vx - ux, vy - uy, depth - depth_p
+        good = ~(np.abs(mz) > f(0.05))
+    mx, my, mz, wgt, node = mx[good] * wgt[good], my[good] * wgt[good], mz[good] * wgt[good], wgt[good], node[good]
+    wx = (R[0, 0] * mx + R[0, 1] * my) + R[0, 2] * mz
+    wy = (R[1, 0] * mx + R[1, 1] * my) + R[1, 2] * mz
+    wz = (R[2, 0] * mx + R[2, 1] * my) + R[2, 2] * mz
+    sums = np.zeros((num_nodes, 4), np.int64)
+    for c, v in enumerate((wx, wy, wz, wgt)):
+        np.add.at(sums[:, c], node, np.rint(v.astype(np.float64) * FIX_SCALE).astype(np.int64))
+    m = (sums.astype(np.float64) / FIX_SCALE).astype(f)
+    nz = m[:, 3] != 0
+    k = f(1) / m[nz, 3]
+    m[nz] = m[nz] * k[:, None]
+    return m

@@ -37,7 +37,7 @@ HIP_FLAGS = [
     "-Wno-unknown-pragmas",
 ]
 
-HIP_SOURCES = ["sh_kernels.hip", "sh_rows.hip", "sh_tiles.hip", "sh_pcg.hip", "sh_sort.hip", "sh_warp.hip", "sh_measure.hip", "sh_render.hip",
+HIP_SOURCES = ["sh_kernels.hip", "sh_rows.hip", "sh_tiles.hip", "sh_pcg.hip", "sh_sort.hip", "sh_warp.hip", "sh_measure.hip", "sh_render.hip", "sh_flow.hip",
                "sh_engine.cpp", "sh_server.cpp"]
 
 

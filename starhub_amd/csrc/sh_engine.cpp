@@ -1373,6 +1373,33 @@ extern "C"
                                                reinterpret_cast<hipStream_t>(stream)));
     }
 
+    int sh_flow_map_from_of16(const uint16_t *of16, uint32_t raw_width, uint32_t raw_height, float scale,
+                              float *flow_map2, sh_stream_t stream)
+    {
+        if (!of16 || !flow_map2 || raw_width == 0 || raw_height == 0 || raw_width > 16384 || raw_height > 16384 ||
+            !(scale > 0.f) || scale > 1.f)
+            return SH_E_ARG;
+        return hip_rc(sh::launch_flow_from_of16(of16, int(raw_width), int(raw_height), scale, flow_map2,
+                                                reinterpret_cast<hipStream_t>(stream)));
+    }
+
+    int sh_node_motion_from_flow(const float *rgbd_prev4, const float *rgbd4, const float *flow_map2,
+                                 const uint32_t *index_map_prev, uint32_t width, uint32_t height,
+                                 const uint16_t *surfel_knn, const float *surfel_spatial, uint32_t num_surfels,
+                                 uint32_t knn, const float cam2world[16], float fx, float fy, float cx, float cy,
+                                 uint32_t num_nodes, void *workspace, float *out4, sh_stream_t stream)
+    {
+        if ((knn != 4 && knn != 8) || !cam2world || !workspace || !out4 || width > 16384 || height > 16384 ||
+            fx == 0.f || fy == 0.f ||
+            (width && height && (!rgbd_prev4 || !rgbd4 || !flow_map2 || !index_map_prev || !surfel_knn ||
+                                 !surfel_spatial)))
+            return SH_E_ARG;
+        return hip_rc(sh::launch_node_motion(rgbd_prev4, rgbd4, flow_map2, index_map_prev, int(width), int(height),
+                                             surfel_knn, surfel_spatial, num_surfels, int(knn), cam2world, fx, fy, cx,
+                                             cy, num_nodes, static_cast<unsigned long long *>(workspace), out4,
+                                             reinterpret_cast<hipStream_t>(stream)));
+    }
+
     int sh_render_solver_maps(const float *vertex4, const float *normal4, const float *color_time4,
                               uint32_t num_surfels, const float world2camera[16], const sh_camera *cam,
                               float max_depth, float current_time, int with_recent_observation, void *zbuffer,

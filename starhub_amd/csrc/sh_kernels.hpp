@@ -116,6 +116,13 @@ constexpr uint32_t kWavesPerWg = 4;
 constexpr uint32_t kPcgNodesPerWg = 32;  // split PCG kernels
 
 // Forward warp of surfels and nodes (sh_warp.hip)
+// optical-flow input and node-motion prior (sh_flow.hip)
+hipError_t launch_flow_from_of16(const uint16_t *of16, int W, int H, float scale, float *flow2, hipStream_t st);
+hipError_t launch_node_motion(const float *rgbd_prev4, const float *rgbd4, const float *flow2,
+                              const uint32_t *index_prev, int width, int height, const uint16_t *surfel_knn,
+                              const float *surfel_spatial, uint32_t num_surfels, int K, const float cam2world[16],
+                              float fx, float fy, float cx, float cy, uint32_t num_nodes, unsigned long long *sums,
+                              float *out4, hipStream_t st);
 // solver maps by point splatting (sh_render.hip)
 hipError_t launch_render_solver_maps(const float *vertex4, const float *normal4, const float *color_time4,
                                      uint32_t num_surfels, const float world2camera[16], int width, int height,

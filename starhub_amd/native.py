@@ -562,3 +562,33 @@ def render_solver_maps(ref_vertex4, ref_normal4, color_time4, world2camera, widt
                                       int(with_recent_observation), ptr(zbuf), ptr(v), ptr(nr), ptr(idx),
                                       _stream_ptr(stream)), "sh_render_solver_maps")
     return v, nr, idx
+
+
+# ------------------------------------------------------------ optical flow and node motion
+def flow_map_from_of16(of16, scale: float, stream=None):
+    """sh_flow_map_from_of16: H x W x 3 int16 (uint16 bits) device tensor -> h x w x 2 float32 flow map."""
+    import torch
+    H, W = of16.shape[:2]
+    f = np.float32
+    w, h = int(np.floor(f(W) * f(scale))), int(np.floor(f(H) * f(scale)))
+    out = torch.empty((h, w, 2), dtype=torch.float32, device="cuda")
+    check(lib().sh_flow_map_from_of16(ctypes.c_void_p(of16.data_ptr()), W, H, ctypes.c_float(scale),
+                                      ctypes.c_void_p(out.data_ptr()), _stream_ptr(stream)), "sh_flow_map_from_of16")
+    return out
+
+
+def node_motion_from_flow(rgbd_prev4, rgbd4, flow2, index_prev, surfel_knn, surfel_spatial, cam2world, fx, fy, cx,
+                          cy, num_nodes: int, stream=None):
+    """sh_node_motion_from_flow: device tensors in, num_nodes x 4 float32 node-motion prior out."""
+    import torch
+    H, W = index_prev.shape
+    S, K = surfel_knn.shape
+    ws = torch.empty(max(1, num_nodes) * 4, dtype=torch.int64, device="cuda")
+    out = torch.empty((max(1, num_nodes), 4), dtype=torch.float32, device="cuda")
+    m = (ctypes.c_float * 16)(*np.asarray(cam2world, np.float32).reshape(-1).tolist())
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    check(lib().sh_node_motion_from_flow(p(rgbd_prev4), p(rgbd4), p(flow2), p(index_prev), W, H, p(surfel_knn),
+                                         p(surfel_spatial), S, K, m, ctypes.c_float(fx), ctypes.c_float(fy),
+                                         ctypes.c_float(cx), ctypes.c_float(cy), num_nodes, p(ws), p(out),
+                                         _stream_ptr(stream)), "sh_node_motion_from_flow")
+    return out[:num_nodes]
