@@ -57,6 +57,19 @@ void Engine::release()
         sl.tile_buf = nullptr;
         sl.tile_bytes = 0;
     }
+    for (size_t l = 0; l < lanes_.size(); ++l)
+    {
+        IndexLane &ln = lanes_[l];
+        if (ln.stream)
+            (void)hipStreamSynchronize(ln.stream);
+        if (ln.ev)
+            (void)hipEventDestroy(ln.ev);
+        if (ln.stream)
+            (void)hipStreamDestroy(ln.stream);
+        if (l > 0 && ln.h_meta)
+            (void)hipHostFree(ln.h_meta); // lane 0's is h_meta_
+    }
+    lanes_.clear();
     if (h_meta_)
         (void)hipHostFree(h_meta_);
     h_meta_ = nullptr;
@@ -228,6 +241,44 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
 #undef A
     SH_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_meta_), 64, hipHostMallocDefault));
     SH_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_flag_), 64, hipHostMallocDefault));
+    // index lanes: lane 0 is the workspace above; a batch engine gets up to
+    // kIndexLanes - 1 more (tile path only) so index builds overlap
+    {
+        constexpr uint32_t kIndexLanes = 4;
+        const uint32_t n_lanes =
+            (cfg_.flags & SH_FLAG_REFERENCE_INDEX) ? 1u : std::min(max_slots, kIndexLanes);
+        lanes_.resize(n_lanes);
+        IndexLane &l0 = lanes_[0];
+        l0.flags = static_cast<uint32_t *>(ws_flags_);
+        l0.pos = static_cast<uint32_t *>(ws_pos_);
+        l0.k16a = static_cast<uint16_t *>(ws_k16a_);
+        l0.k16b = static_cast<uint16_t *>(ws_k16b_);
+        l0.v32a = static_cast<uint32_t *>(ws_v32a_);
+        l0.k32a = static_cast<uint32_t *>(ws_k32a_);
+        l0.k32b = static_cast<uint32_t *>(ws_k32b_);
+        l0.p32a = static_cast<uint32_t *>(ws_p32a_);
+        l0.meta = static_cast<uint32_t *>(ws_meta_);
+        l0.bitmap = static_cast<uint32_t *>(ws_bitmap_);
+        l0.rowlen = static_cast<uint32_t *>(ws_rowlen_);
+        l0.segcnt = static_cast<uint32_t *>(ws_segcnt_);
+        l0.tcontrib = static_cast<uint16_t *>(ws_tcontrib_);
+        l0.tslot_block = static_cast<uint32_t *>(ws_tslot_block_);
+        l0.tslot_first = static_cast<uint16_t *>(ws_tslot_first_);
+        l0.tnslot = static_cast<uint32_t *>(ws_tnslot_);
+        l0.tncontrib = static_cast<uint32_t *>(ws_tncontrib_);
+        l0.sort_temp = sort_temp_;
+        l0.err = static_cast<uint32_t *>(d_flags_) + 2;
+        l0.h_meta = h_meta_;
+        for (uint32_t l = 1; l < n_lanes; ++l)
+            if (int rc = alloc_lane(lanes_[l], npix); rc != SH_OK)
+                return rc;
+        if (n_lanes > 1)
+            for (IndexLane &ln : lanes_)
+            {
+                SH_HIP(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking));
+                SH_HIP(hipEventCreateWithFlags(&ln.ev, hipEventDisableTiming));
+            }
+    }
     stage_bytes_ = size_t(max_slots) * sizeof(SolveDesc) + size_t(kNumMaps) * (max_slots + 1) * 4;
     SH_HIP(hipHostMalloc(&h_stage_, stage_bytes_, hipHostMallocDefault));
     return SH_OK;
@@ -302,9 +353,219 @@ int Engine::set_inputs(uint32_t slot, const sh_camera *cam, const sh_measure_map
     return SH_OK;
 }
 
+// Workspace of an extra index lane (the tile-path arrays only).
+int Engine::alloc_lane(IndexLane &ln, size_t npix)
+{
+    void *p = nullptr;
+    int rc;
+#define L(field, type, bytes)                                                                                         \
+    if ((rc = alloc(&p, (bytes))) != SH_OK)                                                                           \
+        return rc;                                                                                                    \
+    ln.field = static_cast<type *>(p);
+    L(flags, uint32_t, npix * 4);
+    L(pos, uint32_t, npix * 4);
+    L(k16a, uint16_t, size_t(n2t_cap_) * 2);
+    L(k16b, uint16_t, size_t(n2t_cap_) * 2);
+    L(v32a, uint32_t, size_t(n2t_cap_) * 4);
+    L(k32a, uint32_t, size_t(pair_cap_) * 4);
+    L(k32b, uint32_t, size_t(pair_cap_) * 4);
+    L(p32a, uint32_t, size_t(pair_cap_) * 4);
+    L(meta, uint32_t, 64);
+    L(err, uint32_t, 64);
+    L(bitmap, uint32_t, size_t(ncap_) * kBitmapWords * 4);
+    L(rowlen, uint32_t, size_t(ncap_ + 1) * 4);
+    L(segcnt, uint32_t, size_t(ncap_ + 1) * 4);
+    const size_t tstride = size_t(kTilePix) * (K_ * (K_ + 1) / 2);
+    L(tcontrib, uint16_t, size_t(tiles_cap_) * tstride * 2);
+    L(tslot_block, uint32_t, size_t(tiles_cap_) * tstride * 4);
+    L(tslot_first, uint16_t, size_t(tiles_cap_) * tstride * 2);
+    L(tnslot, uint32_t, size_t(tiles_cap_ + 1) * 4);
+    L(tncontrib, uint32_t, size_t(tiles_cap_ + 1) * 4);
+#undef L
+    if ((rc = alloc(&ln.sort_temp, sort_temp_bytes_)) != SH_OK)
+        return rc;
+    SH_HIP(hipHostMalloc(reinterpret_cast<void **>(&ln.h_meta), 64, hipHostMallocDefault));
+    return SH_OK;
+}
+
+// Index build, stage 1 (asynchronous): valid-pixel flags with the id range
+// check and their scan; the pixel count and the check flag go to the lane's
+// host buffer.
+int Engine::index_pixels(uint32_t slot, IndexLane &ln, hipStream_t st)
+{
+    SolveDesc &d = slots_[slot].d;
+    const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
+    size_t tb = sort_temp_bytes_;
+    SH_HIP(hipMemsetAsync(ln.err, 0, 4, st));
+    SH_HIP(launch_pixel_flags(d, n, ln.flags, ln.err, st));
+    SH_HIP(exclusive_scan_u32(ln.sort_temp, tb, ln.flags, ln.pos, n, st));
+    SH_HIP(hipMemcpyAsync(ln.h_meta + 0, ln.pos + (n - 1), 4, hipMemcpyDeviceToHost, st));
+    SH_HIP(hipMemcpyAsync(ln.h_meta + 1, ln.flags + (n - 1), 4, hipMemcpyDeviceToHost, st));
+    SH_HIP(hipMemcpyAsync(ln.h_meta + 12, ln.err, 4, hipMemcpyDeviceToHost, st));
+    return SH_OK;
+}
+
+// Stage 2 (after stage 1's readback): compaction, Node2Term, the block-row
+// structure and the per-tile contribution sort; the row meta and the tile
+// totals go to the host buffer.
+int Engine::index_rows(uint32_t slot, IndexLane &ln, hipStream_t st)
+{
+    Slot &sl = slots_[slot];
+    SolveDesc &d = sl.d;
+    if (ln.h_meta[12] != 0)
+        return SH_E_ARG; // a surfel or node id out of range
+    const uint32_t P = ln.h_meta[0] + ln.h_meta[1];
+    if (P > pcap_)
+        return SH_E_CAPACITY;
+    d.P = int(P);
+    SH_HIP(launch_compact_pixels(d, ln.flags, ln.pos, st));
+    // node -> term (Node2TermsIndex::BuildIndex, Node2TermsIndex.cu:191-199,312-360)
+    const uint32_t n_kv = P * uint32_t(K_) + 2 * uint32_t(d.E) + uint32_t(d.N);
+    SH_HIP(launch_node2term_kv(d, ln.k16a, ln.v32a, st));
+    size_t tb = sort_temp_bytes_;
+    SH_HIP(sort_pairs_u16_u32(ln.sort_temp, tb, ln.k16a, ln.k16b, ln.v32a, d.n2t_terms, n_kv, 16, st));
+    SH_HIP(launch_lower_bound_u16(ln.k16b, n_kv, uint32_t(d.N) + 1, d.n2t_off, st));
+    SH_HIP(hipMemsetAsync(ln.meta, 0, 64, st));
+    SH_HIP(hipMemsetAsync(ln.rowlen + d.N, 0, 4, st));
+    SH_HIP(hipMemsetAsync(ln.segcnt + d.N, 0, 4, st));
+    SH_HIP(launch_row_bitmap(d, ln.bitmap, ln.rowlen, ln.segcnt, ln.meta, st));
+    tb = sort_temp_bytes_;
+    SH_HIP(exclusive_scan_u32(ln.sort_temp, tb, ln.segcnt, d.seg_ptr, uint32_t(d.N) + 1, st));
+    SH_HIP(hipMemcpyAsync(ln.h_meta + 6, d.seg_ptr + d.N, 4, hipMemcpyDeviceToHost, st));
+    tb = sort_temp_bytes_;
+    SH_HIP(exclusive_scan_u32(ln.sort_temp, tb, ln.rowlen, d.row_ptr, uint32_t(d.N) + 1, st));
+    SH_HIP(launch_row_cols(d, ln.bitmap, ln.meta, st));
+    SH_HIP(hipMemcpyAsync(ln.h_meta + 4, ln.meta, 8, hipMemcpyDeviceToHost, st));
+    // the row structure is complete here (blk_col bounded by blk_cap)
+    SH_HIP(launch_tile_build_async(d, ln, st));
+    return SH_OK;
+}
+
+// Stage 3 (after stage 2's readback): sizes, slot partial buffers, the slot
+// lists by block (asynchronous).
+int Engine::index_tiles(uint32_t slot, IndexLane &ln, hipStream_t st)
+{
+    Slot &sl = slots_[slot];
+    SolveDesc &d = sl.d;
+    const uint32_t U = ln.h_meta[5];
+    if (U > ucap_)
+        return SH_E_CAPACITY;
+    d.U = int(U);
+    d.n_pair_entries = 0;
+    sl.max_row_len = ln.h_meta[4];
+    d.n_seg = int(ln.h_meta[6]);
+    int rc = size_partials(sl);
+    if (rc != SH_OK)
+        return rc;
+    if ((rc = build_tiles(sl, ln, st)) != SH_OK)
+        return rc;
+    sl.indexed = true;
+    return SH_OK;
+}
+
+// The indexes of several slots on the lanes: each lane runs one slot's
+// stages on its own stream, and the host advances whichever lane's readback
+// has arrived.  Every lane starts after the work already on st and st waits
+// for every lane at the end.
+int Engine::build_indexes(const uint32_t *list, uint32_t n, hipStream_t st)
+{
+    if (lanes_.size() <= 1 || n <= 1 || (cfg_.flags & SH_FLAG_REFERENCE_INDEX))
+    {
+        for (uint32_t i = 0; i < n; ++i)
+            if (int rc = build_index(list[i], st); rc != SH_OK)
+                return rc;
+        return SH_OK;
+    }
+    const uint32_t L = uint32_t(std::min<size_t>(lanes_.size(), n));
+    SH_HIP(hipEventRecord(lanes_[0].ev, st));
+    for (uint32_t l = 0; l < L; ++l)
+    {
+        if (l > 0)
+            SH_HIP(hipStreamWaitEvent(lanes_[l].stream, lanes_[0].ev, 0));
+        lanes_[l].slot = -1;
+    }
+    SH_HIP(hipStreamWaitEvent(lanes_[0].stream, lanes_[0].ev, 0));
+    int rc = SH_OK;
+    uint32_t next = 0, done = 0;
+    while (rc == SH_OK && done < n)
+    {
+        bool progress = false;
+        int waiting = -1;
+        for (uint32_t l = 0; l < L && rc == SH_OK; ++l)
+        {
+            IndexLane &ln = lanes_[l];
+            if (ln.slot < 0)
+            {
+                if (next == n)
+                    continue;
+                ln.slot = int(list[next++]);
+                ln.stage = 1;
+                if ((rc = index_pixels(uint32_t(ln.slot), ln, ln.stream)) == SH_OK &&
+                    hipEventRecord(ln.ev, ln.stream) != hipSuccess)
+                    rc = SH_E_HIP;
+                progress = true;
+                continue;
+            }
+            const hipError_t q = hipEventQuery(ln.ev);
+            if (q == hipErrorNotReady)
+            {
+                if (waiting < 0)
+                    waiting = int(l);
+                continue;
+            }
+            if (q != hipSuccess)
+            {
+                last_hip_error = q;
+                rc = SH_E_HIP;
+                break;
+            }
+            if (ln.stage == 1)
+            {
+                ln.stage = 2;
+                if ((rc = index_rows(uint32_t(ln.slot), ln, ln.stream)) == SH_OK &&
+                    hipEventRecord(ln.ev, ln.stream) != hipSuccess)
+                    rc = SH_E_HIP;
+            }
+            else
+            {
+                rc = index_tiles(uint32_t(ln.slot), ln, ln.stream);
+                ln.slot = -1;
+                ++done;
+            }
+            progress = true;
+        }
+        if (rc == SH_OK && !progress && waiting >= 0 && hipEventSynchronize(lanes_[waiting].ev) != hipSuccess)
+            rc = SH_E_HIP;
+    }
+    // st continues after every lane's work (also after a failure: nothing may
+    // still be writing a slot's arrays when the caller reuses them)
+    for (uint32_t l = 0; l < L; ++l)
+    {
+        lanes_[l].slot = -1;
+        if (hipEventRecord(lanes_[l].ev, lanes_[l].stream) != hipSuccess ||
+            hipStreamWaitEvent(st, lanes_[l].ev, 0) != hipSuccess)
+            rc = rc == SH_OK ? SH_E_HIP : rc;
+        if (rc != SH_OK)
+            (void)hipStreamSynchronize(lanes_[l].stream);
+    }
+    return rc;
+}
+
 // Once-per-solve index build (WarpSolver::buildSolverIndexStreamed, WarpSolver.cu:194-223).
 int Engine::build_index(uint32_t slot, hipStream_t st)
 {
+    if (!(cfg_.flags & SH_FLAG_REFERENCE_INDEX))
+    {
+        IndexLane &ln = lanes_[0];
+        int rc;
+        if ((rc = index_pixels(slot, ln, st)) != SH_OK)
+            return rc;
+        SH_HIP(hipStreamSynchronize(st));
+        if ((rc = index_rows(slot, ln, st)) != SH_OK)
+            return rc;
+        SH_HIP(hipStreamSynchronize(st));
+        return index_tiles(slot, ln, st);
+    }
     Slot &sl = slots_[slot];
     SolveDesc &d = sl.d;
     const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
@@ -348,31 +609,6 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
     tb = sort_temp_bytes_;
     SH_HIP(exclusive_scan_u32(sort_temp_, tb, segcnt, d.seg_ptr, uint32_t(d.N) + 1, st));
     SH_HIP(hipMemcpyAsync(h_meta_ + 6, d.seg_ptr + d.N, 4, hipMemcpyDeviceToHost, st));
-    if (!(cfg_.flags & SH_FLAG_REFERENCE_INDEX))
-    {
-        tb = sort_temp_bytes_;
-        SH_HIP(exclusive_scan_u32(sort_temp_, tb, rowlen, d.row_ptr, uint32_t(d.N) + 1, st));
-        SH_HIP(launch_row_cols(d, bitmap, meta, st));
-        SH_HIP(hipMemcpyAsync(h_meta_ + 4, meta, 8, hipMemcpyDeviceToHost, st));
-        // the row structure is complete here (blk_col bounded by blk_cap)
-        SH_HIP(launch_tile_build_async(d, st));
-        SH_HIP(hipStreamSynchronize(st));
-        const uint32_t U = h_meta_[5];
-        if (U > ucap_)
-            return SH_E_CAPACITY;
-        d.U = int(U);
-        d.n_pair_entries = 0;
-        sl.max_row_len = h_meta_[4];
-        d.n_seg = int(h_meta_[6]);
-        int rc = size_partials(sl);
-        if (rc != SH_OK)
-            return rc;
-        if ((rc = build_tiles(sl, st)) != SH_OK)
-            return rc;
-        sl.indexed = true;
-        return SH_OK;
-    }
-
     // reference node pair -> term index, half index (NodePair2TermsIndex::BuildHalfIndex, :294-298,591-654)
     if (!d.pair_entries || !d.half_keys || !d.half_off || !d.upper_pos || !d.lower_pos)
         return SH_E_STATE; // never launch on unallocated pair arrays
@@ -411,9 +647,9 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
         SH_HIP(sort_pairs_u32_u32(sort_temp_, tb, k32a, k32b, p32a, p32b, 2 * U, 24, st));
     SH_HIP(launch_sym_scatter(d, k32b, p32b, st));
     SH_HIP(launch_row_ptr(d, k32b, st));
-    SH_HIP(launch_tile_build_async(d, st));
+    SH_HIP(launch_tile_build_async(d, lanes_[0], st));
     SH_HIP(hipStreamSynchronize(st));
-    if (int rc = build_tiles(sl, st); rc != SH_OK)
+    if (int rc = build_tiles(sl, lanes_[0], st); rc != SH_OK)
         return rc;
     sl.indexed = true;
     return SH_OK;
@@ -422,51 +658,47 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
 // Tile index, first part (asynchronous): per-tile sorted contributions and
 // slots in the workspace, and the two count scans whose totals the host reads
 // (into h_meta_[10], h_meta_[11]) at the next synchronisation.
-hipError_t Engine::launch_tile_build_async(SolveDesc &d, hipStream_t st)
+hipError_t Engine::launch_tile_build_async(SolveDesc &d, IndexLane &ln, hipStream_t st)
 {
     d.n_tiles = int(div_up(uint32_t(d.P), kTilePix));
     const uint32_t nt = uint32_t(d.n_tiles);
-    uint32_t *ns = static_cast<uint32_t *>(ws_tnslot_), *nc = static_cast<uint32_t *>(ws_tncontrib_);
+    uint32_t *ns = ln.tnslot, *nc = ln.tncontrib;
     hipError_t e;
     if ((e = hipMemsetAsync(ns + nt, 0, 4, st)) != hipSuccess || (e = hipMemsetAsync(nc + nt, 0, 4, st)) != hipSuccess)
         return e;
-    if ((e = launch_tile_build(d, static_cast<uint16_t *>(ws_tcontrib_), static_cast<uint32_t *>(ws_tslot_block_),
-                               static_cast<uint16_t *>(ws_tslot_first_), ns, nc, st)) != hipSuccess)
+    if ((e = launch_tile_build(d, ln.tcontrib, ln.tslot_block, ln.tslot_first, ns, nc, st)) != hipSuccess)
         return e;
     size_t tb = sort_temp_bytes_;
-    if ((e = exclusive_scan_u32(sort_temp_, tb, ns, d.tile_sbase, nt + 1, st)) != hipSuccess)
+    if ((e = exclusive_scan_u32(ln.sort_temp, tb, ns, d.tile_sbase, nt + 1, st)) != hipSuccess)
         return e;
     tb = sort_temp_bytes_;
-    if ((e = exclusive_scan_u32(sort_temp_, tb, nc, d.tile_cbase, nt + 1, st)) != hipSuccess)
+    if ((e = exclusive_scan_u32(ln.sort_temp, tb, nc, d.tile_cbase, nt + 1, st)) != hipSuccess)
         return e;
-    if ((e = hipMemcpyAsync(h_meta_ + 10, d.tile_sbase + nt, 4, hipMemcpyDeviceToHost, st)) != hipSuccess)
+    if ((e = hipMemcpyAsync(ln.h_meta + 10, d.tile_sbase + nt, 4, hipMemcpyDeviceToHost, st)) != hipSuccess)
         return e;
-    return hipMemcpyAsync(h_meta_ + 11, d.tile_cbase + nt, 4, hipMemcpyDeviceToHost, st);
+    return hipMemcpyAsync(ln.h_meta + 11, d.tile_cbase + nt, 4, hipMemcpyDeviceToHost, st);
 }
 
 // Tile index, second part (after the synchronisation): size the slot arrays,
 // compact, sort the slots by block (stable, so each block's list is in tile
 // order), list offsets and the mirror map.
-int Engine::build_tiles(Slot &sl, hipStream_t st)
+int Engine::build_tiles(Slot &sl, IndexLane &ln, hipStream_t st)
 {
     SolveDesc &d = sl.d;
-    const uint32_t S = h_meta_[10], C = h_meta_[11];
+    const uint32_t S = ln.h_meta[10], C = ln.h_meta[11];
     if (uint64_t(S) > pair_cap_)
         return SH_E_CAPACITY;
     if (int rc = size_tiles(sl, S, C); rc != SH_OK)
         return rc;
-    uint32_t *k32a = static_cast<uint32_t *>(ws_k32a_), *k32b = static_cast<uint32_t *>(ws_k32b_);
-    uint32_t *p32a = static_cast<uint32_t *>(ws_p32a_);
-    SH_HIP(launch_tile_compact(d, static_cast<const uint16_t *>(ws_tcontrib_),
-                               static_cast<const uint32_t *>(ws_tslot_block_),
-                               static_cast<const uint16_t *>(ws_tslot_first_), k32a, p32a, st));
+    uint32_t *k32a = ln.k32a, *k32b = ln.k32b, *p32a = ln.p32a;
+    SH_HIP(launch_tile_compact(d, ln.tcontrib, ln.tslot_block, ln.tslot_first, k32a, p32a, st));
     const uint32_t nb = uint32_t(d.N) + 2u * uint32_t(d.U);
     int end_bit = 1;
     while (end_bit < 32 && (1u << end_bit) <= nb)
         ++end_bit;
     size_t tb = sort_temp_bytes_;
     if (S > 0)
-        SH_HIP(sort_pairs_u32_u32(sort_temp_, tb, k32a, k32b, p32a, d.list_items, S, end_bit, st));
+        SH_HIP(sort_pairs_u32_u32(ln.sort_temp, tb, k32a, k32b, p32a, d.list_items, S, end_bit, st));
     SH_HIP(launch_lower_bound_u32(k32b, S, nb + 1, d.list_off, st));
     SH_HIP(launch_block_transpose(d, st));
     return SH_OK;
@@ -869,9 +1101,13 @@ int Engine::solve(uint32_t ns, hipStream_t st)
         if (!slots_[s].has_inputs)
             return SH_E_STATE;
     int rc;
-    for (uint32_t s = 0; s < ns; ++s)
-        if ((rc = build_index(s, st)) != SH_OK)
+    {
+        std::vector<uint32_t> all(ns);
+        for (uint32_t s = 0; s < ns; ++s)
+            all[s] = s;
+        if ((rc = build_indexes(all.data(), ns, st)) != SH_OK)
             return rc;
+    }
     batch_.resize(ns);
     for (uint32_t s = 0; s < ns; ++s)
     {

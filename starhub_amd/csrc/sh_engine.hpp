@@ -98,12 +98,38 @@ private:
         kNumMaps
     };
 
+    // Index-build workspace of one lane: the arrays a slot's index build
+    // uses between its host readbacks.  Lane 0 is the ws_* set below; a batch
+    // builds the indexes of its slots on several lanes at once, each with its
+    // own stream, so one slot's host readback overlaps the others' kernels.
+    struct IndexLane
+    {
+        uint32_t *flags = nullptr, *pos = nullptr, *v32a = nullptr, *meta = nullptr;
+        uint16_t *k16a = nullptr, *k16b = nullptr;
+        uint32_t *k32a = nullptr, *k32b = nullptr, *p32a = nullptr;
+        uint32_t *bitmap = nullptr, *rowlen = nullptr, *segcnt = nullptr;
+        uint16_t *tcontrib = nullptr, *tslot_first = nullptr;
+        uint32_t *tslot_block = nullptr, *tnslot = nullptr, *tncontrib = nullptr;
+        void *sort_temp = nullptr;
+        uint32_t *err = nullptr;    // device: the range-check flag
+        uint32_t *h_meta = nullptr; // pinned host readbacks
+        hipStream_t stream = nullptr;
+        hipEvent_t ev = nullptr;
+        int slot = -1; // scheduler state: the slot in progress and its next stage
+        int stage = 0;
+    };
+
     int alloc(void **p, size_t bytes);
     void release();
+    int alloc_lane(IndexLane &ln, size_t npix);
     int build_index(uint32_t slot, hipStream_t st);
+    int build_indexes(const uint32_t *slots, uint32_t n, hipStream_t st);
+    int index_pixels(uint32_t slot, IndexLane &ln, hipStream_t st);
+    int index_rows(uint32_t slot, IndexLane &ln, hipStream_t st);
+    int index_tiles(uint32_t slot, IndexLane &ln, hipStream_t st);
     int size_partials(Slot &sl);
-    int build_tiles(Slot &sl, hipStream_t st);
-    hipError_t launch_tile_build_async(SolveDesc &d, hipStream_t st);
+    int build_tiles(Slot &sl, IndexLane &ln, hipStream_t st);
+    hipError_t launch_tile_build_async(SolveDesc &d, IndexLane &ln, hipStream_t st);
     int size_tiles(Slot &sl, uint32_t n_slots, uint32_t n_contrib);
     hipError_t mark(int phase, hipStream_t st);
     void collect_marks();
@@ -129,6 +155,7 @@ private:
     void *ws_tcontrib_ = nullptr, *ws_tslot_block_ = nullptr, *ws_tslot_first_ = nullptr;
     void *ws_tnslot_ = nullptr, *ws_tncontrib_ = nullptr;
     uint32_t tiles_cap_ = 0;
+    std::vector<IndexLane> lanes_;
     int coop_max_groups_ = 0;
     size_t coop_lds_ = 0;
     bool coop_resident_ = false; // every solve of the launch fits the register-resident PCG
