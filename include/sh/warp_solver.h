@@ -402,7 +402,11 @@ typedef struct sh_server_stats
 } sh_server_stats;
 
 /* Creates the server and its two worker threads.  Fewer than max_active slots
- * are made when the budget cannot hold them; SH_E_CAPACITY when not even one fits. */
+ * are made when the budget cannot hold them; SH_E_CAPACITY when not even one fits.
+ * Index builds run on other streams while GN steps run, so the server solves
+ * with the split PCG (SH_FLAG_NO_PERSISTENT_PCG: no kernel waits on another
+ * group); a served result equals a solve alone with that flag, bit for bit.
+ * SH_E_ARG for SH_FLAG_PAIR_MAJOR configurations (their PCG cannot change). */
 int sh_server_create(const sh_server_config *config, sh_server **out);
 int sh_server_destroy(sh_server *server);
 /* Registers a stream and copies its geometry and node graph into resident HBM;

@@ -466,14 +466,24 @@ int Engine::index_tiles(uint32_t slot, IndexLane &ln, hipStream_t st)
 // The indexes of several slots on the lanes: each lane runs one slot's
 // stages on its own stream, and the host advances whichever lane's readback
 // has arrived.  Every lane starts after the work already on st and st waits
-// for every lane at the end.
-int Engine::build_indexes(const uint32_t *list, uint32_t n, hipStream_t st)
+// for every lane at the end.  With rcs, a slot whose inputs are rejected
+// (SH_E_ARG / SH_E_CAPACITY / SH_E_NOMEM) records its status there and the
+// others go on; without, the first failure ends the build.
+int Engine::build_indexes(const uint32_t *list, uint32_t n, hipStream_t st, int *rcs)
 {
+    auto per_slot = [&](int rc) { return rcs && rc != SH_OK && rc != SH_E_HIP; };
+    if (rcs)
+        for (uint32_t i = 0; i < n; ++i)
+            rcs[i] = SH_OK;
     if (lanes_.size() <= 1 || n <= 1 || (cfg_.flags & SH_FLAG_REFERENCE_INDEX))
     {
         for (uint32_t i = 0; i < n; ++i)
             if (int rc = build_index(list[i], st); rc != SH_OK)
-                return rc;
+            {
+                if (!per_slot(rc))
+                    return rc;
+                rcs[i] = rc;
+            }
         return SH_OK;
     }
     const uint32_t L = uint32_t(std::min<size_t>(lanes_.size(), n));
@@ -498,6 +508,7 @@ int Engine::build_indexes(const uint32_t *list, uint32_t n, hipStream_t st)
             {
                 if (next == n)
                     continue;
+                ln.item = int(next);
                 ln.slot = int(list[next++]);
                 ln.stage = 1;
                 if ((rc = index_pixels(uint32_t(ln.slot), ln, ln.stream)) == SH_OK &&
@@ -531,6 +542,15 @@ int Engine::build_indexes(const uint32_t *list, uint32_t n, hipStream_t st)
                 rc = index_tiles(uint32_t(ln.slot), ln, ln.stream);
                 ln.slot = -1;
                 ++done;
+            }
+            if (per_slot(rc))
+            {
+                // this slot's inputs were rejected: record it, free the lane
+                rcs[ln.item] = rc;
+                if (ln.slot >= 0)
+                    ++done;
+                ln.slot = -1;
+                rc = SH_OK;
             }
             progress = true;
         }
@@ -1275,7 +1295,10 @@ int Engine::finish(hipStream_t st)
     SH_HIP(hipStreamSynchronize(st));
     collect_marks();
     if (*h_flag_ != 0)
-        return SH_E_HIP; // a PCG barrier wait ran out (never expected)
+    {
+        last_hip_error = hipErrorLaunchTimeOut; // a PCG barrier wait ran out (never expected)
+        return SH_E_HIP;
+    }
     return SH_OK;
 }
 
@@ -1291,6 +1314,27 @@ int Engine::admit(uint32_t slot, hipStream_t st)
         return rc;
     SH_HIP(launch_identity_dq(sl.d.dq[0], uint32_t(sl.d.N), st));
     sl.d.gn_iter = 0;
+    SH_HIP(hipStreamSynchronize(st));
+    return SH_OK;
+}
+
+int Engine::admit_many(const uint32_t *list, uint32_t n, hipStream_t st, int *rcs)
+{
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        if (list[i] >= slots_.size() || !slots_[list[i]].has_inputs)
+            return SH_E_STATE;
+        slots_[list[i]].solved = false;
+    }
+    if (int rc = build_indexes(list, n, st, rcs); rc != SH_OK)
+        return rc;
+    for (uint32_t i = 0; i < n; ++i)
+        if (rcs[i] == SH_OK)
+        {
+            Slot &sl = slots_[list[i]];
+            SH_HIP(launch_identity_dq(sl.d.dq[0], uint32_t(sl.d.N), st));
+            sl.d.gn_iter = 0;
+        }
     SH_HIP(hipStreamSynchronize(st));
     return SH_OK;
 }

@@ -30,6 +30,11 @@ public:
     // (index workspace); step() runs one GN iteration over a set of admitted
     // slots (launch state).  The two may run on different threads and streams.
     int admit(uint32_t slot, hipStream_t st);
+    // admit several slots at once (their index builds share the lanes); a
+    // slot whose inputs are rejected gets its status in rcs and the others
+    // proceed; the return value is SH_OK or a HIP error that failed them all
+    int admit_many(const uint32_t *slots, uint32_t n, hipStream_t st, int *rcs);
+    uint32_t num_lanes() const { return uint32_t(lanes_.size()); }
     int step(const uint32_t *slots, uint32_t n, hipStream_t st);
     int gn_iter(uint32_t slot) const { return slots_[slot].d.gn_iter; }
     int cost_history(uint32_t slot, double *out, uint32_t capacity, uint32_t *count) const;
@@ -115,7 +120,8 @@ private:
         uint32_t *h_meta = nullptr; // pinned host readbacks
         hipStream_t stream = nullptr;
         hipEvent_t ev = nullptr;
-        int slot = -1; // scheduler state: the slot in progress and its next stage
+        int slot = -1; // scheduler state: the slot in progress, its position in the list and next stage
+        int item = -1;
         int stage = 0;
     };
 
@@ -123,7 +129,7 @@ private:
     void release();
     int alloc_lane(IndexLane &ln, size_t npix);
     int build_index(uint32_t slot, hipStream_t st);
-    int build_indexes(const uint32_t *slots, uint32_t n, hipStream_t st);
+    int build_indexes(const uint32_t *slots, uint32_t n, hipStream_t st, int *rcs = nullptr);
     int index_pixels(uint32_t slot, IndexLane &ln, hipStream_t st);
     int index_rows(uint32_t slot, IndexLane &ln, hipStream_t st);
     int index_tiles(uint32_t slot, IndexLane &ln, hipStream_t st);

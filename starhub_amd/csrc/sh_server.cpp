@@ -117,64 +117,111 @@ static int hip_status(hipError_t e)
 void sh_server::admit_loop()
 {
     std::unique_lock<std::mutex> lk(mu);
+    const uint32_t lanes = std::max(1u, engine->num_lanes());
+    std::vector<uint64_t> picked;
+    std::vector<uint32_t> pslots;
+    std::vector<int> rcs;
     for (;;)
     {
-        // oldest waiting request whose stream has nothing in flight
-        auto pick = waiting.end();
-        if (!free_slots.empty())
-            for (auto it = waiting.begin(); it != waiting.end(); ++it)
-                if (!streams[reqs[*it].stream].busy)
-                {
-                    pick = it;
-                    break;
-                }
+        // the oldest waiting requests whose streams have nothing in flight,
+        // one per index lane and free slot (frames of one stream stay in order)
+        picked.clear();
+        for (auto it = waiting.begin(); it != waiting.end() && picked.size() < lanes &&
+                                        picked.size() < free_slots.size();)
+        {
+            StreamState &ss = streams[reqs[*it].stream];
+            if (!ss.busy)
+            {
+                ss.busy = true;
+                picked.push_back(*it);
+                it = waiting.erase(it);
+            }
+            else
+                ++it;
+        }
         if (stop)
             return;
-        if (pick == waiting.end())
+        if (picked.empty())
         {
             cv_admit.wait(lk);
             continue;
         }
-        const uint64_t ticket = *pick;
-        waiting.erase(pick);
-        Request &rq = reqs[ticket];
-        StreamState &ss = streams[rq.stream];
-        ss.busy = true;
-        const uint32_t slot = free_slots.back();
-        free_slots.pop_back();
-        rq.slot = slot;
-        rq.t_admit = now_ms();
-        // inputs: resident stream state + this frame's maps
-        sh_geometry geo{ss.surfel_knn, ss.surfel_spatial, ss.surfel_connect, ss.num_surfels};
-        sh_node_graph ng{ss.node_coords, ss.edges,      ss.node_connect,       ss.node_knn,
-                         ss.node_spatial, ss.num_nodes, ss.num_edges,          ss.node_radius_square};
-        sh_keypoints kp{0};
-        const sh_camera cam = ss.cam;
-        float pose[16];
-        std::memcpy(pose, ss.cam2world, sizeof(pose));
-        const sh_measure_maps meas = rq.meas;
-        const sh_render_maps rend = rq.rend;
-        const sh_flow_map flow = rq.flow;
-        const sh_node_flow nflow = rq.nflow;
-        lk.unlock();
-        int rc = engine->set_inputs(slot, &cam, &meas, &rend, &geo, &ng, &nflow, &flow, &kp, pose);
-        if (rc == SH_OK)
-            rc = engine->admit(slot, s_admit);
-        lk.lock();
-        if (rc != SH_OK)
+        // inputs: resident stream state + each frame's maps
+        struct In
         {
-            rq.status = rc;
-            rq.done = true;
-            rq.t_done = now_ms();
-            ss.busy = false;
-            free_slots.push_back(slot);
-            stats.completed += 1;
-            stats.failed += 1;
-            cv_done.notify_all();
-            continue;
+            sh_camera cam;
+            float pose[16];
+            sh_geometry geo;
+            sh_node_graph ng;
+            sh_measure_maps meas;
+            sh_render_maps rend;
+            sh_flow_map flow;
+            sh_node_flow nflow;
+        };
+        std::vector<In> in(picked.size());
+        pslots.resize(picked.size());
+        for (size_t k = 0; k < picked.size(); ++k)
+        {
+            Request &rq = reqs[picked[k]];
+            StreamState &ss = streams[rq.stream];
+            const uint32_t slot = free_slots.back();
+            free_slots.pop_back();
+            rq.slot = slot;
+            rq.t_admit = now_ms();
+            pslots[k] = slot;
+            In &x = in[k];
+            x.cam = ss.cam;
+            std::memcpy(x.pose, ss.cam2world, sizeof(x.pose));
+            x.geo = sh_geometry{ss.surfel_knn, ss.surfel_spatial, ss.surfel_connect, ss.num_surfels};
+            x.ng = sh_node_graph{ss.node_coords,  ss.edges,     ss.node_connect, ss.node_knn,
+                                 ss.node_spatial, ss.num_nodes, ss.num_edges,    ss.node_radius_square};
+            x.meas = rq.meas;
+            x.rend = rq.rend;
+            x.flow = rq.flow;
+            x.nflow = rq.nflow;
         }
-        slot_req[slot] = ticket;
-        active.push_back(slot);
+        lk.unlock();
+        sh_keypoints kp{0};
+        rcs.assign(picked.size(), SH_OK);
+        std::vector<uint32_t> ok_slots;
+        std::vector<size_t> ok_pos;
+        for (size_t k = 0; k < picked.size(); ++k)
+        {
+            In &x = in[k];
+            rcs[k] = engine->set_inputs(pslots[k], &x.cam, &x.meas, &x.rend, &x.geo, &x.ng, &x.nflow, &x.flow, &kp,
+                                        x.pose);
+            if (rcs[k] == SH_OK)
+            {
+                ok_slots.push_back(pslots[k]);
+                ok_pos.push_back(k);
+            }
+        }
+        std::vector<int> rc_admit(ok_slots.size(), SH_OK);
+        int fatal_rc = SH_OK;
+        if (!ok_slots.empty())
+            fatal_rc = engine->admit_many(ok_slots.data(), uint32_t(ok_slots.size()), s_admit, rc_admit.data());
+        for (size_t j = 0; j < ok_pos.size(); ++j)
+            rcs[ok_pos[j]] = fatal_rc != SH_OK ? fatal_rc : rc_admit[j];
+        lk.lock();
+        for (size_t k = 0; k < picked.size(); ++k)
+        {
+            Request &rq = reqs[picked[k]];
+            StreamState &ss = streams[rq.stream];
+            if (rcs[k] != SH_OK)
+            {
+                rq.status = rcs[k];
+                rq.done = true;
+                rq.t_done = now_ms();
+                ss.busy = false;
+                free_slots.push_back(pslots[k]);
+                stats.completed += 1;
+                stats.failed += 1;
+                cv_done.notify_all();
+                continue;
+            }
+            slot_req[pslots[k]] = picked[k];
+            active.push_back(pslots[k]);
+        }
         cv_step.notify_one();
     }
 }
@@ -288,6 +335,22 @@ extern "C"
         {
             delete s;
             return rc;
+        }
+        // Index builds (the admission thread, on several streams) run while
+        // GN steps run.  The persistent PCG kernels need all their groups
+        // resident at once, which work on other streams can delay without
+        // bound, so the server's PCG is the split one (kernels with no
+        // cross-group waits, replayed from a hipGraph).
+        {
+            sh_tuning t = s->engine->tuning();
+            t.flags = (t.flags & ~uint32_t(SH_FLAG_STREAMING_PCG | SH_FLAG_PIPELINED_PCG)) | SH_FLAG_NO_PERSISTENT_PCG;
+            if (t.spmv_mode == SH_SPMV_AUTO)
+                t.spmv_mode = SH_SPMV_MATERIALIZED;
+            if ((rc = s->engine->set_tuning(t)) != SH_OK)
+            {
+                delete s;
+                return rc;
+            }
         }
         s->reserved = s->engine->device_bytes();
         s->stats.slots = n;

@@ -383,10 +383,12 @@ def test_served_same_alone_and_in_mix(torch_dev):
     W = max(c.width for c in cfgs)
     H = max(c.height for c in cfgs)
     N = max(c.num_nodes for c in cfgs)
+    # the server solves with the split PCG: the same PCG alone gives the same bits
     scfg = native.make_config(cfgs[0].solver, W, H, N, 8)
+    solo_cfg = native.make_config(cfgs[0].solver, W, H, N, 8, flags=native.SH_FLAG_NO_PERSISTENT_PCG)
     alone = []
     for f in frames:
-        s = native.WarpSolver(scfg)
+        s = native.WarpSolver(solo_cfg)
         s.process_frame(f)
         alone.append(s.node_se3().cpu().numpy())
         s.close()
