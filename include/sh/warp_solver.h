@@ -454,6 +454,23 @@ typedef struct sh_tuning
 int sh_solver_autotune(sh_solver *solver, sh_stream_t stream, uint32_t repeats, sh_tuning *best);
 int sh_solver_set_tuning(sh_solver *solver, const sh_tuning *tuning);
 int sh_batch_autotune(sh_batch *batch, uint32_t num_solves, sh_stream_t stream, uint32_t repeats, sh_tuning *best);
+/* The tuning in use (ms = 0). */
+int sh_solver_get_tuning(const sh_solver *solver, sh_tuning *tuning);
+
+/* Persistent-PCG fallback (no counterpart in the reference, whose PCG is one
+ * launch per step).  The persistent PCG kernels need every work-group of a
+ * solve resident at once and wait for each other at their barriers; kernels
+ * that the caller runs on other streams at the same time can hold CUs long
+ * enough for a wait to run out.  Each wait polls at most `polls` times
+ * (0: the default, 2^22, about 0.1 s), after which the launch drains with
+ * meaningless sums.  A solve (or batch step) whose wait ran out is re-run
+ * with the split PCG, which has no inter-group waits, and the solver keeps
+ * the split PCG afterwards (its tuning then holds SH_FLAG_NO_PERSISTENT_PCG;
+ * a later set_tuning or autotune may choose a persistent PCG again; an
+ * autotune candidate that ran out is not chosen).  sh_solver_pcg_fallbacks
+ * counts the re-runs. */
+int sh_solver_set_wait_limit(sh_solver *solver, uint32_t polls);
+int sh_solver_pcg_fallbacks(const sh_solver *solver, uint32_t *count);
 
 /* ------------------------------------------------------- measurement hook */
 /* No counterpart in the reference (which times nothing on the device).  With

@@ -955,14 +955,14 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
         {
             if (resident && pipelined)
                 SH_HIP(launch_pcg_pipe(sds, map(kMapCoop, ns), n, map_total_[kMapCoop], int(cfg_.pcg_iterations),
-                                       static_cast<PcgSlot2 *>(d_slots2_), flags + 0, flags + 1, st));
+                                       static_cast<PcgSlot2 *>(d_slots2_), flags + 0, flags + 1, wait_limit_, st));
             else if (resident)
                 SH_HIP(launch_pcg_resident(sds, map(kMapCoop, ns), n, map_total_[kMapCoop],
                                            int(cfg_.pcg_iterations), static_cast<PcgSlot *>(d_slots_), flags + 0,
-                                           flags + 1, st));
+                                           flags + 1, wait_limit_, st));
             else
                 SH_HIP(launch_pcg_coop(sds, map(kMapCoop, ns), n, map_total_[kMapCoop], int(cfg_.pcg_iterations),
-                                       static_cast<PcgSlot *>(d_slots_), flags + 0, flags + 1, tile, st));
+                                       static_cast<PcgSlot *>(d_slots_), flags + 0, flags + 1, wait_limit_, tile, st));
         }
         else if (cfg_.flags & SH_FLAG_NO_GRAPH)
             SH_HIP(enqueue_pcg_launches(ns, matrix_free, st));
@@ -1137,18 +1137,31 @@ int Engine::solve(uint32_t ns, hipStream_t st)
     build_maps(ns);
     if ((rc = upload(ns, st)) != SH_OK)
         return rc;
-    if ((rc = enqueue_gn(ns, st, cfg_.gn_iterations, true)) != SH_OK)
-        return rc;
-    if (cfg_.flags & SH_FLAG_COST_HISTORY)
+    auto run = [&]() -> int {
+        int r;
+        if ((r = enqueue_gn(ns, st, cfg_.gn_iterations, true)) != SH_OK)
+            return r;
+        if (cfg_.flags & SH_FLAG_COST_HISTORY)
+        {
+            // cost at the final warp field: the terms once more, then the reduction
+            const SolveDesc *sds = static_cast<const SolveDesc *>(d_descs_);
+            const int it = int(cfg_.gn_iterations);
+            SH_HIP(launch_dense_terms(K_, fp16_, sds, map(kMapPixel, ns), int(ns), map_total_[kMapPixel], it, st, 0));
+            SH_HIP(launch_reg_terms(sds, map(kMapReg, ns), int(ns), map_total_[kMapReg], it, st, 0));
+            SH_HIP(launch_cost_final(sds, int(ns), it, st));
+        }
+        return finish(st);
+    };
+    rc = run();
+    if (rc == SH_E_HIP && barrier_timeout_)
     {
-        // cost at the final warp field: the terms once more, then the reduction
-        const SolveDesc *sds = static_cast<const SolveDesc *>(d_descs_);
-        const int it = int(cfg_.gn_iterations);
-        SH_HIP(launch_dense_terms(K_, fp16_, sds, map(kMapPixel, ns), int(ns), map_total_[kMapPixel], it, st, 0));
-        SH_HIP(launch_reg_terms(sds, map(kMapReg, ns), int(ns), map_total_[kMapReg], it, st, 0));
-        SH_HIP(launch_cost_final(sds, int(ns), it, st));
+        // the whole solve again, from the identity, with the split PCG
+        if ((rc = fall_back_to_split_pcg(st)) != SH_OK)
+            return rc;
+        n_marks_ = 0;
+        rc = run();
     }
-    if ((rc = finish(st)) != SH_OK)
+    if (rc != SH_OK)
         return rc;
     for (uint32_t s = 0; s < ns; ++s)
     {
@@ -1235,6 +1248,7 @@ int Engine::autotune(uint32_t ns, hipStream_t st, uint32_t repeats, sh_tuning *b
     {
         if ((rc = set_tuning(c)) != SH_OK)
             break;
+        const uint32_t fallbacks = pcg_fallbacks_;
         std::vector<double> t;
         for (uint32_t r = 0; r < repeats + 1 && rc == SH_OK; ++r)
         {
@@ -1251,6 +1265,8 @@ int Engine::autotune(uint32_t ns, hipStream_t st, uint32_t repeats, sh_tuning *b
         }
         if (rc != SH_OK)
             break;
+        if (pcg_fallbacks_ != fallbacks)
+            continue; // its barrier waits ran out: timed as the split PCG, and not chosen
         std::sort(t.begin(), t.end());
         c.ms = t[t.size() / 2];
         // a later candidate must be 2% faster to replace an earlier one, so that
@@ -1291,14 +1307,37 @@ int Engine::cost_history(uint32_t slot, double *out, uint32_t capacity, uint32_t
 // Synchronise the launch stream and check the PCG barrier flag.
 int Engine::finish(hipStream_t st)
 {
+    barrier_timeout_ = false;
     SH_HIP(hipMemcpyAsync(h_flag_, static_cast<uint32_t *>(d_flags_) + 1, 4, hipMemcpyDeviceToHost, st));
     SH_HIP(hipStreamSynchronize(st));
     collect_marks();
     if (*h_flag_ != 0)
     {
-        last_hip_error = hipErrorLaunchTimeOut; // a PCG barrier wait ran out (never expected)
+        barrier_timeout_ = true;
+        last_hip_error = hipErrorLaunchTimeOut; // a persistent-PCG barrier wait ran out
         return SH_E_HIP;
     }
+    return SH_OK;
+}
+
+// A persistent PCG needs every group of a solve resident at once.  Kernels
+// that the caller runs on other streams can hold CUs long enough for a
+// barrier wait to run out; the launch then drained with meaningless sums and
+// may have left the slots' epochs out of step with the epoch base.  Clear the
+// barrier state and switch this engine to the split PCG, which has no
+// inter-group waits; the caller re-runs the GN iterations from the warp field
+// they started from (the failed updates only wrote the other DQ buffer or,
+// for a whole solve, start again from the identity).
+int Engine::fall_back_to_split_pcg(hipStream_t st)
+{
+    const size_t groups = slots_.size() * size_t(kPcgMaxGroups);
+    SH_HIP(hipMemsetAsync(d_slots_, 0, groups * sizeof(PcgSlot), st));
+    SH_HIP(hipMemsetAsync(d_slots2_, 0, groups * sizeof(PcgSlot2), st));
+    SH_HIP(hipMemsetAsync(d_flags_, 0, 2 * sizeof(uint32_t), st)); // epoch base and barrier error
+    SH_HIP(hipStreamSynchronize(st));
+    cfg_.flags = (cfg_.flags & ~uint32_t(SH_FLAG_STREAMING_PCG | SH_FLAG_PIPELINED_PCG)) | SH_FLAG_NO_PERSISTENT_PCG;
+    barrier_timeout_ = false;
+    ++pcg_fallbacks_;
     return SH_OK;
 }
 
@@ -1357,7 +1396,19 @@ int Engine::step(const uint32_t *list, uint32_t n, hipStream_t st)
         return rc;
     if ((rc = enqueue_gn(n, st, 1, false)) != SH_OK)
         return rc;
-    if ((rc = finish(st)) != SH_OK)
+    rc = finish(st);
+    if (rc == SH_E_HIP && barrier_timeout_)
+    {
+        // the iteration again from the same DQ buffer (the failed update only
+        // wrote the other one), with the split PCG
+        if ((rc = fall_back_to_split_pcg(st)) != SH_OK)
+            return rc;
+        n_marks_ = 0;
+        if ((rc = enqueue_gn(n, st, 1, false)) != SH_OK)
+            return rc;
+        rc = finish(st);
+    }
+    if (rc != SH_OK)
         return rc;
     for (uint32_t s = 0; s < n; ++s)
     {
@@ -1557,6 +1608,30 @@ extern "C"
         if (!solver || !tuning)
             return SH_E_ARG;
         return solver->eng.set_tuning(*tuning);
+    }
+
+    int sh_solver_get_tuning(const sh_solver *solver, sh_tuning *tuning)
+    {
+        if (!solver || !tuning)
+            return SH_E_ARG;
+        *tuning = solver->eng.tuning();
+        return SH_OK;
+    }
+
+    int sh_solver_set_wait_limit(sh_solver *solver, uint32_t polls)
+    {
+        if (!solver)
+            return SH_E_ARG;
+        solver->eng.set_wait_limit(polls);
+        return SH_OK;
+    }
+
+    int sh_solver_pcg_fallbacks(const sh_solver *solver, uint32_t *count)
+    {
+        if (!solver || !count)
+            return SH_E_ARG;
+        *count = solver->eng.pcg_fallbacks();
+        return SH_OK;
     }
 
     int sh_batch_autotune(sh_batch *batch, uint32_t num_solves, sh_stream_t stream, uint32_t repeats,

@@ -42,6 +42,10 @@ public:
     int set_tuning(const sh_tuning &t);
     sh_tuning tuning() const;
     int autotune(uint32_t ns, hipStream_t st, uint32_t repeats, sh_tuning *best);
+    // polls per barrier wait of the persistent PCG kernels (0: the default)
+    void set_wait_limit(uint32_t polls) { wait_limit_ = polls; }
+    // solves (or steps) re-run with the split PCG after a barrier wait ran out
+    uint32_t pcg_fallbacks() const { return pcg_fallbacks_; }
     // device bytes held: fixed allocations plus the grown per-slot buffers
     size_t device_bytes() const
     {
@@ -143,6 +147,7 @@ private:
     int upload(uint32_t ns, hipStream_t st);
     int enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identity);
     int finish(hipStream_t st);
+    int fall_back_to_split_pcg(hipStream_t st);
     const uint32_t *map(int m, uint32_t ns) const;
 
     sh_solver_config cfg_{};
@@ -171,6 +176,9 @@ private:
     void *d_slots_ = nullptr; // persistent-PCG barrier slots
     void *d_slots2_ = nullptr; // pipelined-PCG barrier slots (two sums each)
     void *d_flags_ = nullptr; // [0] PCG epoch base, [1] PCG barrier error
+    uint32_t wait_limit_ = 0;
+    uint32_t pcg_fallbacks_ = 0;
+    bool barrier_timeout_ = false; // the last finish() saw the PCG barrier flag
     void *sort_temp_ = nullptr;
     size_t sort_temp_bytes_ = 0;
     uint32_t *h_meta_ = nullptr; // index-build readbacks

@@ -72,7 +72,10 @@ hipError_t launch_rows_seg(int K, bool fp16, const SolveDesc *sds, const uint32_
                            uint32_t n_seg_wg, const RowsPlan &plan, hipStream_t st);
 hipError_t launch_rows_combine(const SolveDesc *sds, const uint32_t *row_map, int n_solve, uint32_t n_row_wg,
                                const RowsPlan &plan, hipStream_t st);
-// Barrier slot of one persistent-PCG work-group.
+// Barrier slot of one persistent-PCG work-group.  Every barrier wait of the
+// persistent kernels polls at most `wait_limit` times (0: 2^22 polls, about
+// 0.1 s) and then sets *err, after which every wait of the launch returns at
+// once: the launch always drains, with meaningless sums.
 struct PcgSlot
 {
     double v[2];
@@ -84,14 +87,16 @@ struct PcgSlot
 constexpr uint32_t kPcgBlocksPerGroup = 1024;
 constexpr uint32_t kPcgMaxGroups = 64;
 hipError_t launch_pcg_coop(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
-                           PcgSlot *slots, uint32_t *epoch_base, uint32_t *err, uint32_t tile, hipStream_t st);
+                           PcgSlot *slots, uint32_t *epoch_base, uint32_t *err, uint32_t wait_limit, uint32_t tile,
+                           hipStream_t st);
 int pcg_coop_max_groups(size_t lds);
 // Register-resident persistent PCG: groups of at most kPcgResRows rows and
 // kPcgResBlocks blocks, split evenly by rows.
 constexpr uint32_t kPcgResRows = 85;
 constexpr uint32_t kPcgResBlocks = 1536;
 hipError_t launch_pcg_resident(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
-                               PcgSlot *slots, uint32_t *epoch_base, uint32_t *err, hipStream_t st);
+                               PcgSlot *slots, uint32_t *epoch_base, uint32_t *err, uint32_t wait_limit,
+                               hipStream_t st);
 int pcg_resident_max_groups();
 // Barrier slot of the pipelined PCG (two sums per barrier).
 struct PcgSlot2
@@ -103,7 +108,8 @@ struct PcgSlot2
 // Pipelined resident PCG: one barrier per iteration, FP64 vectors; same
 // groups as the resident kernel.
 hipError_t launch_pcg_pipe(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
-                           PcgSlot2 *slots, uint32_t *epoch_base, uint32_t *err, hipStream_t st);
+                           PcgSlot2 *slots, uint32_t *epoch_base, uint32_t *err, uint32_t wait_limit,
+                           hipStream_t st);
 int pcg_pipe_max_groups();
 constexpr uint32_t kBitmapWords = 128;
 // Floats per tile-slot partial in HBM: the 28 of the record (21 H, 6 g, pad)

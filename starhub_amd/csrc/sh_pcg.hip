@@ -22,7 +22,7 @@ namespace sh
 
 constexpr int kCoopThreads = 512;
 constexpr int kCoopWaves = kCoopThreads / 64;
-constexpr unsigned kSpinCap = 1u << 22;
+constexpr unsigned kSpinCap = 1u << 22; // default polls per barrier wait
 
 __device__ __forceinline__ int find_solve_p(const uint32_t *__restrict__ begin, int n, uint32_t wg)
 {
@@ -83,7 +83,7 @@ __device__ __forceinline__ double group_sum(double v, double *sh)
 // every group has read it.  A wait that exceeds the spin cap sets *err and
 // ends every later wait at once, so each wave always exits.
 __device__ double solve_barrier_sum(double part, PcgSlot *__restrict__ slots, uint32_t b0, uint32_t G, uint32_t ep,
-                                    uint32_t *__restrict__ err, double *sh)
+                                    uint32_t *__restrict__ err, uint32_t cap, double *sh)
 {
     if (threadIdx.x == 0)
     {
@@ -103,7 +103,7 @@ __device__ double solve_barrier_sum(double part, PcgSlot *__restrict__ slots, ui
             while (__hip_atomic_load(&o.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ep)
             {
                 __builtin_amdgcn_s_sleep(1);
-                if (++spins > kSpinCap || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                if (++spins >= cap || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                 {
                     __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(kCoopThreads) void k_pcg_coop(const SolveDesc *__re
                                                            const uint32_t *__restrict__ wg_begin, int n_solve,
                                                            int iters, PcgSlot *__restrict__ slots,
                                                            uint32_t *__restrict__ epoch_base,
-                                                           uint32_t *__restrict__ err, uint32_t tile)
+                                                           uint32_t *__restrict__ err, uint32_t cap, uint32_t tile)
 {
     extern __shared__ float qblk[]; // tile x 6
     __shared__ double sh[kCoopWaves + 1];
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(kCoopThreads) void k_pcg_coop(const SolveDesc *__re
             part += double(rv[a]) * double(sv);
         }
     }
-    double delta = solve_barrier_sum(group_sum(part, sh), slots, b0, G, ++ep, err, sh);
+    double delta = solve_barrier_sum(group_sum(part, sh), slots, b0, G, ++ep, err, cap, sh);
 
     double delta_prev = 0.0;
     for (int k = 0; k < iters; ++k)
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(kCoopThreads) void k_pcg_coop(const SolveDesc *__re
                 part += double(dn) * double(q);
             }
         }
-        const double dq = solve_barrier_sum(group_sum(part, sh), slots, b0, G, ++ep, err, sh);
+        const double dq = solve_barrier_sum(group_sum(part, sh), slots, b0, G, ++ep, err, cap, sh);
 
         // ---- phase B: alpha, x, r, s = M^-1 r, partial r.s
         const double alpha = (dq != 0.0) ? delta / dq : 0.0;
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(kCoopThreads) void k_pcg_coop(const SolveDesc *__re
             }
         }
         delta_prev = delta;
-        delta = solve_barrier_sum(group_sum(part, sh), slots, b0, G, ++ep, err, sh);
+        delta = solve_barrier_sum(group_sum(part, sh), slots, b0, G, ++ep, err, cap, sh);
     }
     // next launch starts above every epoch used here; every group has read
     // epoch_base before its first barrier, and this group has passed them all
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_resident(const SolveDesc *_
                                                               const uint32_t *__restrict__ wg_begin, int n_solve,
                                                               int iters, PcgSlot *__restrict__ slots,
                                                               uint32_t *__restrict__ epoch_base,
-                                                              uint32_t *__restrict__ err)
+                                                              uint32_t *__restrict__ err, uint32_t cap)
 {
     __shared__ float qblk[kResThreads * kResBlocksPerThread * 6];
     __shared__ float rsh[kResThreads];
@@ -384,7 +384,7 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_resident(const SolveDesc *_
         d.s[i6] = s;
         d.dv[0][i6] = 0.f;
     }
-    double delta = solve_barrier_sum(group_sum(own ? double(r) * double(s) : 0.0, sh), slots, b0, G, ++ep, err, sh);
+    double delta = solve_barrier_sum(group_sum(own ? double(r) * double(s) : 0.0, sh), slots, b0, G, ++ep, err, cap, sh);
 
     double delta_prev = 0.0;
     for (int k = 0; k < iters; ++k)
@@ -440,7 +440,7 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_resident(const SolveDesc *_
         for (uint32_t e = e0; e < e1; ++e)
             q += qblk[e * 6 + a];
         const double dq =
-            solve_barrier_sum(group_sum(own ? double(dn) * double(q) : 0.0, sh), slots, b0, G, ++ep, err, sh);
+            solve_barrier_sum(group_sum(own ? double(dn) * double(q) : 0.0, sh), slots, b0, G, ++ep, err, cap, sh);
 
         // ---- phase B: alpha, x, r, s = M^-1 r, partial r.s
         const double alpha = (dq != 0.0) ? delta / dq : 0.0;
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_resident(const SolveDesc *_
         }
         dk = dn;
         delta_prev = delta;
-        delta = solve_barrier_sum(group_sum(own ? double(r) * double(s) : 0.0, sh), slots, b0, G, ++ep, err, sh);
+        delta = solve_barrier_sum(group_sum(own ? double(r) * double(s) : 0.0, sh), slots, b0, G, ++ep, err, cap, sh);
     }
     if (own)
     {
@@ -499,7 +499,7 @@ struct PipeGroupSum
 // every store they cover are complete) and after (the solve's sums are in
 // out[0..1] for every thread).
 __device__ void solve_barrier_sum2(PipeGroupSum part, PcgSlot2 *__restrict__ slots, uint32_t b0, uint32_t G,
-                                   uint32_t ep, uint32_t *__restrict__ err, double *out)
+                                   uint32_t ep, uint32_t *__restrict__ err, uint32_t cap, double *out)
 {
     if (threadIdx.x == 0)
     {
@@ -518,7 +518,7 @@ __device__ void solve_barrier_sum2(PipeGroupSum part, PcgSlot2 *__restrict__ slo
             while (__hip_atomic_load(&o.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ep)
             {
                 __builtin_amdgcn_s_sleep(1);
-                if (++spins > kSpinCap || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                if (++spins >= cap || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                 {
                     __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
@@ -546,7 +546,7 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
                                                           const uint32_t *__restrict__ wg_begin, int n_solve, int iters,
                                                           PcgSlot2 *__restrict__ slots,
                                                           uint32_t *__restrict__ epoch_base,
-                                                          uint32_t *__restrict__ err)
+                                                          uint32_t *__restrict__ err, uint32_t cap)
 {
     __shared__ double qblk[kResThreads * kResBlocksPerThread * 6];
     __shared__ double rsh[kResThreads];
@@ -770,7 +770,7 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
     if (own)
         d.mv[0][i6] = u;
     __syncthreads(); // every m store is complete before the publish
-    (void)solve_barrier_sum2(PipeGroupSum{0.0, 0.0}, slots, b0, G, ++ep, err, sh2);
+    (void)solve_barrier_sum2(PipeGroupSum{0.0, 0.0}, slots, b0, G, ++ep, err, cap, sh2);
     __syncthreads();
     double w = apply(d.mv[0]);
     double x = 0.0, z = 0.0, q = 0.0, s = 0.0, p = 0.0, gprev = 0.0, aprev = 0.0;
@@ -782,7 +782,7 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
         if (own)
             mb[i6] = m;
         __syncthreads(); // m stores complete; rsh and sh read by every thread
-        (void)solve_barrier_sum2(gs, slots, b0, G, ++ep, err, sh2);
+        (void)solve_barrier_sum2(gs, slots, b0, G, ++ep, err, cap, sh2);
         __syncthreads();
         const PipeGroupSum gd{sh2[0], sh2[1]};
         const double n = apply(mb);
@@ -813,7 +813,8 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
 }
 
 hipError_t launch_pcg_pipe(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
-                           PcgSlot2 *slots, uint32_t *epoch_base, uint32_t *err, hipStream_t st)
+                           PcgSlot2 *slots, uint32_t *epoch_base, uint32_t *err, uint32_t wait_limit,
+                           hipStream_t st)
 {
     // A plain launch: the kernel needs its groups co-resident (the host keeps
     // n_wg within the occupancy bound, pcg_pipe_max_groups) but no grid-wide
@@ -821,7 +822,7 @@ hipError_t launch_pcg_pipe(const SolveDesc *sds, const uint32_t *wmap, int n_sol
     // only delays the others.  The cooperative launch API adds a per-launch
     // cost of about 20 us on this device.
     hipLaunchKernelGGL(k_pcg_pipe, dim3(n_wg), dim3(kResThreads), 0, st, sds, wmap, n_solve, iters, slots, epoch_base,
-                       err);
+                       err, wait_limit ? wait_limit : kSpinCap);
     return hipGetLastError();
 }
 
@@ -839,11 +840,12 @@ int pcg_pipe_max_groups()
 }
 
 hipError_t launch_pcg_resident(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
-                               PcgSlot *slots, uint32_t *epoch_base, uint32_t *err, hipStream_t st)
+                               PcgSlot *slots, uint32_t *epoch_base, uint32_t *err, uint32_t wait_limit,
+                               hipStream_t st)
 {
     // plain launch, as for k_pcg_pipe (n_wg within pcg_resident_max_groups)
     hipLaunchKernelGGL(k_pcg_resident, dim3(n_wg), dim3(kResThreads), 0, st, sds, wmap, n_solve, iters, slots,
-                       epoch_base, err);
+                       epoch_base, err, wait_limit ? wait_limit : kSpinCap);
     return hipGetLastError();
 }
 
@@ -861,7 +863,8 @@ int pcg_resident_max_groups()
 }
 
 hipError_t launch_pcg_coop(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
-                           PcgSlot *slots, uint32_t *epoch_base, uint32_t *err, uint32_t tile, hipStream_t st)
+                           PcgSlot *slots, uint32_t *epoch_base, uint32_t *err, uint32_t wait_limit, uint32_t tile,
+                           hipStream_t st)
 {
     const size_t lds = size_t(tile) * 6 * sizeof(float);
     if (lds > 65536)
@@ -873,7 +876,7 @@ hipError_t launch_pcg_coop(const SolveDesc *sds, const uint32_t *wmap, int n_sol
     }
     // plain launch, as for k_pcg_pipe (n_wg within pcg_coop_max_groups(lds))
     hipLaunchKernelGGL(k_pcg_coop, dim3(n_wg), dim3(kCoopThreads), lds, st, sds, wmap, n_solve, iters, slots, epoch_base,
-                       err, tile);
+                       err, wait_limit ? wait_limit : kSpinCap, tile);
     return hipGetLastError();
 }
 

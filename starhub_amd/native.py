@@ -346,6 +346,26 @@ def set_tuning(solver, flags: int, spmv_mode: int, pcg_rows_per_group: int) -> N
     check(lib().sh_solver_set_tuning(solver._h, ctypes.byref(t)), "sh_solver_set_tuning")
 
 
+def get_tuning(solver) -> dict:
+    """The tuning a WarpSolver uses now (sh_solver_get_tuning)."""
+    t = Tuning()
+    check(lib().sh_solver_get_tuning(solver._h, ctypes.byref(t)), "sh_solver_get_tuning")
+    return {"flags": t.flags, "spmv_mode": t.spmv_mode, "pcg_rows_per_group": t.pcg_rows_per_group}
+
+
+def set_wait_limit(solver, polls: int) -> None:
+    """Polls per persistent-PCG barrier wait before the solve falls back to the
+    split PCG (0 restores the default, 2^22)."""
+    check(lib().sh_solver_set_wait_limit(solver._h, ctypes.c_uint32(polls)), "sh_solver_set_wait_limit")
+
+
+def pcg_fallbacks(solver) -> int:
+    """Solves re-run with the split PCG after a persistent-PCG wait ran out."""
+    n = ctypes.c_uint32()
+    check(lib().sh_solver_pcg_fallbacks(solver._h, ctypes.byref(n)), "sh_solver_pcg_fallbacks")
+    return n.value
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("solver", SolverConfig), ("max_active", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64)]
 

@@ -200,6 +200,44 @@ def test_pipelined_pcg_step(c1_one_iter, torch_dev):
     assert abs(d_pipe[0] - d_std[0]) <= 1e-6 * abs(d_std[0])
 
 
+@pytest.mark.parametrize("variant", ["resident", "pipelined"])
+def test_persistent_pcg_wait_fallback(torch_dev, variant):
+    """A persistent-PCG barrier wait that runs out (forced from the host: a
+    wait limit of one poll, 32 groups per solve) drains the launch, and the
+    solve re-runs itself with the split PCG: the warp field is bit-identical
+    to a solver on the split PCG from the start, the tuning then holds
+    SH_FLAG_NO_PERSISTENT_PCG, and later solves stay on it.  Choosing the
+    persistent PCG again with the default limit gives its own result back."""
+    from starhub_amd import native, scene
+    torch = torch_dev
+    sc = scene.generate(scene.config_c1())
+    flags = native.SH_FLAG_PIPELINED_PCG if variant == "pipelined" else 0
+    split, _, _, _ = _solve(sc, flags=native.SH_FLAG_NO_PERSISTENT_PCG)
+    want = split.node_se3().cpu().numpy()
+    solver, _, _, _ = _solve(sc, flags=flags)
+    native.set_tuning(solver, flags, 0, 8)
+    solver.solve()
+    persistent = solver.node_se3().cpu().numpy()
+    assert native.pcg_fallbacks(solver) == 0
+    native.set_wait_limit(solver, 1)
+    say("persistent pcg", variant, "with a one-poll wait limit")
+    solver.solve()
+    n = native.pcg_fallbacks(solver)
+    t = native.get_tuning(solver)
+    say("fallbacks", n, "tuning", t)
+    assert n == 1
+    assert t["flags"] & native.SH_FLAG_NO_PERSISTENT_PCG and not t["flags"] & native.SH_FLAG_PIPELINED_PCG
+    assert np.array_equal(solver.node_se3().cpu().numpy(), want)
+    solver.solve()
+    assert native.pcg_fallbacks(solver) == 1
+    assert np.array_equal(solver.node_se3().cpu().numpy(), want)
+    native.set_wait_limit(solver, 0)
+    native.set_tuning(solver, flags, 0, 8)
+    solver.solve()
+    assert native.pcg_fallbacks(solver) == 1
+    assert np.array_equal(solver.node_se3().cpu().numpy(), persistent)
+
+
 @pytest.mark.parametrize("flags", ["pair_major", "row_segments", "split_pcg", "streaming_pcg", "pipelined_pcg"])
 def test_alternative_paths(torch_dev, flags):
     """The pair-major materialisation (the reference's traversal), the
