@@ -90,4 +90,50 @@ struct SolveDesc
     float *mf_reg;    // E*3 per-edge J x
 };
 
+// A generic pointer that is known to name device (global) memory.  The pointers
+// a kernel reads out of a descriptor array are generic, so their accesses
+// compile to flat_* instructions, which count against the LDS counter as well:
+// every LDS wait then also waits for the outstanding flat loads and stores,
+// and a flat load cannot pass an earlier LDS store it might alias.  Passing
+// the address through an integer into the global address space and back lets
+// the compiler's address-space inference emit global_* accesses instead.
+template <class T>
+__device__ __forceinline__ T *as_global(T *p)
+{
+    typedef __attribute__((address_space(1))) T global_t;
+    return (T *)(global_t *)(uintptr_t)p;
+}
+
+// Entry i & 1 of a two-entry pointer array of a descriptor (both entries are
+// uniform), marked global.  A descriptor copy indexed at run time would be
+// kept in scratch memory, and the compiler turns a select between two loads
+// of the array into such an indexed load; the empty asm makes the two loaded
+// values opaque, so the select stays a select of registers.
+template <class T, class I>
+__device__ __forceinline__ T *pick2(T *const (&a)[2], I i)
+{
+    uintptr_t u0 = reinterpret_cast<uintptr_t>(a[0]), u1 = reinterpret_cast<uintptr_t>(a[1]);
+    asm("" : "+s"(u0), "+s"(u1));
+    return as_global(reinterpret_cast<T *>((i & 1) ? u1 : u0));
+}
+
+// A kernel's copy of its solve's descriptor with every pointer marked global
+// (only the fields a kernel uses are loaded).
+__device__ __forceinline__ SolveDesc global_desc(const SolveDesc &src)
+{
+    SolveDesc d = src;
+#define SH_G(f) d.f = as_global(d.f)
+    SH_G(meas_v); SH_G(meas_n); SH_G(ref_v); SH_G(ref_n); SH_G(index_map); SH_G(flow);
+    SH_G(surfel_knn); SH_G(surfel_spatial); SH_G(surfel_connect); SH_G(node_coords); SH_G(edges);
+    SH_G(node_connect); SH_G(node_pred); SH_G(pix); SH_G(pknn); SH_G(pw); SH_G(n2t_off); SH_G(n2t_terms);
+    SH_G(half_keys); SH_G(half_off); SH_G(pair_entries); SH_G(upper_pos); SH_G(lower_pos); SH_G(row_ptr);
+    SH_G(blk_col); SH_G(dense_cnt); SH_G(seg_ptr); SH_G(part); SH_G(tile_cbase); SH_G(tile_sbase);
+    SH_G(tc_contrib); SH_G(tc_slot_first); SH_G(list_off); SH_G(list_items); SH_G(blk_transpose); SH_G(tpart);
+    SH_G(dq[0]); SH_G(dq[1]); SH_G(jr); SH_G(reg); SH_G(ttrans); SH_G(blk); SH_G(dinv); SH_G(rhs);
+    SH_G(x); SH_G(r); SH_G(s); SH_G(q); SH_G(dv[0]); SH_G(dv[1]); SH_G(mv[0]); SH_G(mv[1]);
+    SH_G(delta); SH_G(cost); SH_G(cost_part); SH_G(mf_z); SH_G(mf_reg);
+#undef SH_G
+    return d;
+}
+
 } // namespace sh

@@ -214,7 +214,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         A(ws_tcontrib_, size_t(tiles_cap_) * tstride * 2);
         A(ws_tslot_block_, size_t(tiles_cap_) * tstride * 4);
         A(ws_tslot_first_, size_t(tiles_cap_) * tstride * 2);
-        A(ws_tnslot_, size_t(tiles_cap_ + 1) * 4);
+        A(ws_tnslot_, size_t(tiles_cap_ + 2) * 4); // counts, scan total, largest count
         A(ws_tncontrib_, size_t(tiles_cap_ + 1) * 4);
     }
     {
@@ -379,7 +379,7 @@ int Engine::alloc_lane(IndexLane &ln, size_t npix)
     L(tcontrib, uint16_t, size_t(tiles_cap_) * tstride * 2);
     L(tslot_block, uint32_t, size_t(tiles_cap_) * tstride * 4);
     L(tslot_first, uint16_t, size_t(tiles_cap_) * tstride * 2);
-    L(tnslot, uint32_t, size_t(tiles_cap_ + 1) * 4);
+    L(tnslot, uint32_t, size_t(tiles_cap_ + 2) * 4);
     L(tncontrib, uint32_t, size_t(tiles_cap_ + 1) * 4);
 #undef L
     if ((rc = alloc(&ln.sort_temp, sort_temp_bytes_)) != SH_OK)
@@ -684,7 +684,7 @@ hipError_t Engine::launch_tile_build_async(SolveDesc &d, IndexLane &ln, hipStrea
     const uint32_t nt = uint32_t(d.n_tiles);
     uint32_t *ns = ln.tnslot, *nc = ln.tncontrib;
     hipError_t e;
-    if ((e = hipMemsetAsync(ns + nt, 0, 4, st)) != hipSuccess || (e = hipMemsetAsync(nc + nt, 0, 4, st)) != hipSuccess)
+    if ((e = hipMemsetAsync(ns + nt, 0, 8, st)) != hipSuccess || (e = hipMemsetAsync(nc + nt, 0, 4, st)) != hipSuccess)
         return e;
     if ((e = launch_tile_build(d, ln.tcontrib, ln.tslot_block, ln.tslot_first, ns, nc, st)) != hipSuccess)
         return e;
@@ -695,6 +695,8 @@ hipError_t Engine::launch_tile_build_async(SolveDesc &d, IndexLane &ln, hipStrea
     if ((e = exclusive_scan_u32(ln.sort_temp, tb, nc, d.tile_cbase, nt + 1, st)) != hipSuccess)
         return e;
     if ((e = hipMemcpyAsync(ln.h_meta + 10, d.tile_sbase + nt, 4, hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return e;
+    if ((e = hipMemcpyAsync(ln.h_meta + 13, ns + nt + 1, 4, hipMemcpyDeviceToHost, st)) != hipSuccess)
         return e;
     return hipMemcpyAsync(ln.h_meta + 11, d.tile_cbase + nt, 4, hipMemcpyDeviceToHost, st);
 }
@@ -708,6 +710,7 @@ int Engine::build_tiles(Slot &sl, IndexLane &ln, hipStream_t st)
     const uint32_t S = ln.h_meta[10], C = ln.h_meta[11];
     if (uint64_t(S) > pair_cap_)
         return SH_E_CAPACITY;
+    sl.tile_slot_max = d.n_tiles > 0 ? ln.h_meta[13] : 0u;
     if (int rc = size_tiles(sl, S, C); rc != SH_OK)
         return rc;
     uint32_t *k32a = ln.k32a, *k32b = ln.k32b, *p32a = ln.p32a;
@@ -881,13 +884,14 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
     const bool fp16 = fp16_;
     if (identity)
         SH_HIP(launch_identity(sds, map(kMapNode, ns), n, map_total_[kMapNode], st));
-    uint32_t max_n = 0, max_l = 0, tile = 1;
+    uint32_t max_n = 0, max_l = 0, tile = 1, slot_cap = 1;
     const uint32_t *coop_begin = maps_.data() + size_t(kMapCoop) * (ns + 1);
     for (uint32_t s = 0; s < ns; ++s)
     {
         const SolveDesc &d = slots_[batch_[s]].d;
         max_n = std::max(max_n, uint32_t(d.N));
         max_l = std::max(max_l, slots_[batch_[s]].max_row_len);
+        slot_cap = std::max(slot_cap, slots_[batch_[s]].tile_slot_max);
         // blocks of one persistent-PCG group: its share plus one whole row
         const uint32_t G = coop_begin[s + 1] - coop_begin[s];
         const uint32_t nb = uint32_t(d.N) + 2u * uint32_t(d.U);
@@ -944,7 +948,7 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
         else
         {
             SH_HIP(mark(kPhTiles, st));
-            SH_HIP(launch_tiles(K_, fp16, sds, map(kMapTile, ns), n, map_total_[kMapTile], st));
+            SH_HIP(launch_tiles(K_, fp16, sds, map(kMapTile, ns), n, map_total_[kMapTile], slot_cap, st));
             SH_HIP(mark(kPhTilesCombine, st));
             SH_HIP(launch_tiles_combine(sds, map(kMapRow, ns), n, map_total_[kMapRow], tiles_lds, st));
         }

@@ -87,6 +87,7 @@ private:
         bool indexed = false;
         bool solved = false;
         uint32_t max_row_len = 0;
+        uint32_t tile_slot_max = 0; // largest slot count of one tile
         void *part_buf = nullptr; // segment partials, grown on demand
         size_t part_bytes = 0;
         void *tile_buf = nullptr; // tile contributions, slots, lists and partials, grown on demand

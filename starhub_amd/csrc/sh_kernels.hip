@@ -395,7 +395,7 @@ __device__ __forceinline__ double block_sum_256(double v)
 template <int K, class JT>
 __device__ __forceinline__ double dense_pixel(const SolveDesc &d, uint32_t p, int parity, int store)
 {
-    const dq8 *__restrict__ dq = d.dq[(d.gn_iter + parity) & 1];
+    const dq8 *__restrict__ dq = pick2(d.dq, (d.gn_iter + parity) & 1);
     uint16_t kn[K];
     float w[K];
     load_knn<K>(d, p, kn, w);
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(256) void k_dense_terms(const SolveDesc *__restrict
                                                      int store)
 {
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const uint32_t p = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
     double pix_cost = 0.0;
     if (p < uint32_t(d.P))
@@ -535,9 +535,9 @@ __global__ __launch_bounds__(256) void k_reg_terms(const SolveDesc *__restrict__
                                                    int store)
 {
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const uint32_t item = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
-    const dq8 *__restrict__ dq = d.dq[(d.gn_iter + parity) & 1];
+    const dq8 *__restrict__ dq = pick2(d.dq, (d.gn_iter + parity) & 1);
     double edge_cost = 0.0;
     if (item < uint32_t(d.E))
     {
@@ -677,7 +677,7 @@ __global__ __launch_bounds__(256) void k_diag_rhs(const SolveDesc *__restrict__ 
 {
     __shared__ float stage[4][48];
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t n = (blockIdx.x - wg_begin[si]) * 4 + wave;
     if (n >= uint32_t(d.N))
@@ -797,7 +797,7 @@ __global__ __launch_bounds__(256) void k_dinv(const SolveDesc *__restrict__ sds,
                                               int n_solve)
 {
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const uint32_t n = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
     if (n >= uint32_t(d.N))
         return;
@@ -825,7 +825,7 @@ __global__ __launch_bounds__(256) void k_offdiag(const SolveDesc *__restrict__ s
 {
     __shared__ float stage[4][36];
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t u = (blockIdx.x - wg_begin[si]) * 4 + wave;
     if (u >= uint32_t(d.U))
@@ -949,7 +949,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(const SolveDesc *__res
     __shared__ double sh[4];
     __shared__ float rb[kPcgThreads];
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const uint32_t node = (blockIdx.x - wg_begin[si]) * kPcgNodes + threadIdx.x / 6;
     const int a = threadIdx.x % 6;
     const bool live = node < uint32_t(d.N);
@@ -993,15 +993,15 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_spmv(const SolveDesc *__res
 {
     __shared__ double sh[4];
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const double delta = reduce_partials(part_rs, wg_begin[si], wg_begin[si + 1], sh);
     double beta = 0.0;
     if (k > 0 && d.delta[k - 1] != 0.0)
         beta = delta / d.delta[k - 1];
     if (threadIdx.x == 0 && blockIdx.x == wg_begin[si])
         d.delta[k] = delta;
-    const float *__restrict__ dold = d.dv[k & 1];
-    float *__restrict__ dnew = d.dv[(k + 1) & 1];
+    const float *__restrict__ dold = pick2(d.dv, k & 1);
+    float *__restrict__ dnew = pick2(d.dv, (k + 1) & 1);
     const uint32_t node = (blockIdx.x - wg_begin[si]) * kPcgNodes + threadIdx.x / 6;
     const int a = threadIdx.x % 6;
     double prod = 0.0;
@@ -1037,7 +1037,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_dir(const SolveDesc *__rest
 {
     __shared__ double sh[4];
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const double delta = reduce_partials(part_rs, wg_begin[si], wg_begin[si + 1], sh);
     double beta = 0.0;
     if (k > 0 && d.delta[k - 1] != 0.0)
@@ -1048,7 +1048,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_dir(const SolveDesc *__rest
     if (node < uint32_t(d.N))
     {
         const size_t row = size_t(node) * 6 + threadIdx.x % 6;
-        d.dv[(k + 1) & 1][row] = d_next(beta, d.dv[k & 1][row], d.s[row]);
+        pick2(d.dv, (k + 1) & 1)[row] = d_next(beta, pick2(d.dv, k & 1)[row], d.s[row]);
     }
 }
 
@@ -1059,13 +1059,13 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_dot(const SolveDesc *__rest
 {
     __shared__ double sh[4];
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const uint32_t node = (blockIdx.x - wg_begin[si]) * kPcgNodes + threadIdx.x / 6;
     double prod = 0.0;
     if (node < uint32_t(d.N))
     {
         const size_t row = size_t(node) * 6 + threadIdx.x % 6;
-        prod = double(d.dv[(k + 1) & 1][row]) * double(d.q[row]);
+        prod = double(pick2(d.dv, (k + 1) & 1)[row]) * double(d.q[row]);
     }
     const double tot = block_sum_192(prod, sh);
     if (threadIdx.x == 0)
@@ -1082,10 +1082,10 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(const SolveDesc *__r
     __shared__ double sh[4];
     __shared__ float rb[kPcgThreads];
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const double dq = reduce_partials(part_dq, wg_begin[si], wg_begin[si + 1], sh);
     const double alpha = (dq != 0.0) ? d.delta[k] / dq : 0.0;
-    const float *__restrict__ dnew = d.dv[(k + 1) & 1];
+    const float *__restrict__ dnew = pick2(d.dv, (k + 1) & 1);
     const uint32_t node = (blockIdx.x - wg_begin[si]) * kPcgNodes + threadIdx.x / 6;
     const int a = threadIdx.x % 6;
     const bool live = node < uint32_t(d.N);
@@ -1134,7 +1134,7 @@ __device__ void reduce_cost(const SolveDesc &d, int slot)
 // One wave per solve: the cost after the last update (SH_FLAG_COST_HISTORY).
 __global__ void k_cost_final(const SolveDesc *__restrict__ sds, int slot)
 {
-    reduce_cost(sds[blockIdx.x], slot);
+    reduce_cost(global_desc(sds[blockIdx.x]), slot);
 }
 
 // T <- exp(xi) T, stored as a dual quaternion (ApplyWarpFieldUpdateKernel,
@@ -1143,7 +1143,7 @@ __global__ __launch_bounds__(256) void k_update(const SolveDesc *__restrict__ sd
                                                 int n_solve, int parity)
 {
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     if (blockIdx.x == wg_begin[si] && threadIdx.x < 64)
         reduce_cost(d, d.gn_iter + parity); // cost of the state this iteration started from
     const uint32_t n = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
@@ -1181,7 +1181,7 @@ __global__ __launch_bounds__(256) void k_update(const SolveDesc *__restrict__ sd
         S.r[8] = tt * ax.z * ax.z + c;
     }
     S.t = t;
-    const rt34 prev = dq_to_mat34(load_dq(d.dq[(d.gn_iter + parity) & 1], n));
+    const rt34 prev = dq_to_mat34(load_dq(pick2(d.dq, (d.gn_iter + parity) & 1), n));
     rt34 nw;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -1190,7 +1190,7 @@ __global__ __launch_bounds__(256) void k_update(const SolveDesc *__restrict__ sd
             nw.r[3 * i + j] = S.r[3 * i] * prev.r[j] + S.r[3 * i + 1] * prev.r[3 + j] + S.r[3 * i + 2] * prev.r[6 + j];
     nw.t = rot_apply(S.r, prev.t) + S.t;
     const dq8 out = dq_from_mat34(nw);
-    float4 *o = reinterpret_cast<float4 *>(d.dq[((d.gn_iter + parity) & 1) ^ 1] + n);
+    float4 *o = reinterpret_cast<float4 *>(pick2(d.dq, ((d.gn_iter + parity) & 1) ^ 1) + n);
     o[0] = out.q0;
     o[1] = out.q1;
 }
@@ -1198,7 +1198,7 @@ __global__ __launch_bounds__(256) void k_update(const SolveDesc *__restrict__ sd
 __global__ void k_identity(const SolveDesc *__restrict__ sds, const uint32_t *__restrict__ wg_begin, int n_solve)
 {
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const uint32_t n = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
     if (n >= uint32_t(d.N))
         return;

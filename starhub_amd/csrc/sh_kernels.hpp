@@ -161,7 +161,7 @@ hipError_t launch_block_transpose(const SolveDesc &d, hipStream_t st);
 hipError_t launch_lower_bound_u32(const uint32_t *keys, uint32_t n_keys, uint32_t n_out, uint32_t *out,
                                   hipStream_t st);
 hipError_t launch_tiles(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
-                        hipStream_t st);
+                        uint32_t slot_cap, hipStream_t st);
 size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len);
 // matrix-free J^T W J x (x = d_{k+1} of each solve unless x_user; y = d.q unless y_user)
 hipError_t launch_mf_tiles(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,

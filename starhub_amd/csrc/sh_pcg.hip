@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kCoopThreads) void k_pcg_coop(const SolveDesc *__re
     extern __shared__ float qblk[]; // tile x 6
     __shared__ double sh[kCoopWaves + 1];
     const int si = find_solve_p(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const uint32_t b0 = wg_begin[si], G = wg_begin[si + 1] - b0, g = blockIdx.x - b0;
     const uint32_t N = uint32_t(d.N);
     const uint32_t NB = d.row_ptr[N] + N;
@@ -187,8 +187,8 @@ __global__ __launch_bounds__(kCoopThreads) void k_pcg_coop(const SolveDesc *__re
         const double beta = (k > 0 && delta_prev != 0.0) ? delta / delta_prev : 0.0;
         if (tid == 0 && g == 0)
             d.delta[k] = delta;
-        const float *__restrict__ dold = d.dv[k & 1];
-        float *__restrict__ dnew = d.dv[(k + 1) & 1];
+        const float *__restrict__ dold = pick2(d.dv, k & 1);
+        float *__restrict__ dnew = pick2(d.dv, (k + 1) & 1);
         if (fits)
         {
             for (uint32_t bb0 = B0 + tid; bb0 < B1; bb0 += 2 * kCoopThreads)
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_resident(const SolveDesc *_
     __shared__ float rsh[kResThreads];
     __shared__ double sh[kCoopWaves + 1];
     const int si = find_solve_p(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const uint32_t b0 = wg_begin[si], G = wg_begin[si + 1] - b0, g = blockIdx.x - b0;
     const uint32_t N = uint32_t(d.N);
     const uint32_t r0 = uint32_t(uint64_t(g) * N / G), r1 = uint32_t(uint64_t(g + 1) * N / G);
@@ -393,8 +393,8 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_resident(const SolveDesc *_
         const double beta = (k > 0 && delta_prev != 0.0) ? delta / delta_prev : 0.0;
         if (tid == 0 && g == 0)
             d.delta[k] = delta;
-        const float *__restrict__ dold = d.dv[k & 1];
-        float *__restrict__ dnew = d.dv[(k + 1) & 1];
+        const float *__restrict__ dold = pick2(d.dv, k & 1);
+        float *__restrict__ dnew = pick2(d.dv, (k + 1) & 1);
         float2 po[kResBlocksPerThread][3], ps[kResBlocksPerThread][3];
 #pragma unroll
         for (int j = 0; j < kResBlocksPerThread; ++j)
@@ -558,7 +558,7 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
     __shared__ uint32_t cpre[kBitmapWords];
     __shared__ uint32_t ncols_sh;
     const int si = find_solve_p(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const uint32_t b0 = wg_begin[si], G = wg_begin[si + 1] - b0, g = blockIdx.x - b0;
     const uint32_t N = uint32_t(d.N);
     const uint32_t r0 = uint32_t(uint64_t(g) * N / G), r1 = uint32_t(uint64_t(g + 1) * N / G);
@@ -778,7 +778,7 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
     {
         double m = 0.0;
         stage(w, own ? r * u : 0.0, own ? w * u : 0.0, m, gs);
-        double *__restrict__ mb = d.mv[(k + 1) & 1];
+        double *__restrict__ mb = pick2(d.mv, (k + 1) & 1);
         if (own)
             mb[i6] = m;
         __syncthreads(); // m stores complete; rsh and sh read by every thread

@@ -214,7 +214,7 @@ __global__ __launch_bounds__(kRowThreads) void k_rows_seg(const SolveDesc *__res
 {
     extern __shared__ float smem[];
     const int si = find_solve_r(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const uint32_t seg = blockIdx.x - wg_begin[si];
     if (seg >= uint32_t(d.n_seg))
         return; // uniform over the group
@@ -372,7 +372,7 @@ __global__ __launch_bounds__(kRowThreads) void k_rows_combine(const SolveDesc *_
 {
     extern __shared__ float smem[];
     const int si = find_solve_r(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const uint32_t i = blockIdx.x - wg_begin[si];
     if (i >= uint32_t(d.N))
         return; // uniform over the group

@@ -252,6 +252,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_build(SolveDesc d, uint16
     {
         tile_nslot[t] = total_heads;
         tile_ncontrib[t] = total_valid; // valid items lead after the sort
+        atomicMax(tile_nslot + gridDim.x + 1, total_heads); // largest slot count of the solve's tiles
     }
 }
 
@@ -322,23 +323,28 @@ __global__ void k_lower_bound_u32(const uint32_t *__restrict__ keys, uint32_t n_
 // slots are visited longest list first (a counting sort of the list lengths in
 // LDS), so the lanes of a wave run lists of about the same length; which lane
 // sums a slot does not change its result.  Contributions are loaded in batches
-// of kTileBatch so the dependent LDS reads of a batch overlap.
+// of kTileBatch, the last one predicated, so the dependent LDS reads of a
+// batch overlap.  The slot lists take LDS for the launch's largest tile only,
+// which lets five tiles share a CU.
 constexpr int kTileBatch = 8;
 
 template <int K, class JT>
 __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restrict__ sds,
-                                                        const uint32_t *__restrict__ wg_begin, int n_solve)
+                                                        const uint32_t *__restrict__ wg_begin, int n_solve,
+                                                        uint32_t slot_cap)
 {
     using G = TileGeom<K>;
     constexpr int kEnt = kTilePix * G::kPairs;
     __shared__ float4 Hs[kTilePix * (kTileRec / 4)];
     __shared__ float wS[kTilePix * K];
     __shared__ uint16_t ent[kEnt];
-    __shared__ uint16_t sst[kEnt + 2];
-    __shared__ uint16_t perm[kEnt];
     __shared__ uint32_t hist[kTilePix + 2];
+    // slot list starts sst[slot_cap + 2], then the visiting order perm[slot_cap]
+    extern __shared__ uint16_t slot_lds[];
+    uint16_t *__restrict__ sst = slot_lds;
+    uint16_t *__restrict__ perm = slot_lds + slot_cap + 2;
     const int si = find_solve_t(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const uint32_t t = blockIdx.x - wg_begin[si];
     if (t >= uint32_t(d.n_tiles))
         return; // uniform over the group
@@ -347,6 +353,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
     const uint32_t np = min(uint32_t(kTilePix), uint32_t(d.P) - p0);
     const uint32_t s0 = d.tile_sbase[t], ns = d.tile_sbase[t + 1] - s0;
     const uint32_t c0 = d.tile_cbase[t], nc = d.tile_cbase[t + 1] - c0;
+    if (ns > slot_cap)
+        return; // never taken: slot_cap is the largest slot count of the launch's tiles
     for (uint32_t i = tid; i < kTilePix + 2; i += kTileThreads)
         hist[i] = 0u;
     if (tid < np)
@@ -402,16 +410,16 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
     {
         const uint32_t s = perm[task / (kTileRec / 4)], g = task % (kTileRec / 4);
         float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-        uint32_t e = sst[s];
         const uint32_t e1 = sst[s + 1];
-        for (; e + kTileBatch <= e1; e += kTileBatch)
+        for (uint32_t e = sst[s]; e < e1; e += kTileBatch)
         {
+            // entries past the list end read pixel 0 and leave the sum unchanged
             float coef[kTileBatch];
             float4 h[kTileBatch];
 #pragma unroll
             for (int j = 0; j < kTileBatch; ++j)
             {
-                const uint32_t c = ent[e + j];
+                const uint32_t c = e + j < e1 ? uint32_t(ent[e + j]) : 0u;
                 const uint32_t pl = c & 127u, ka = (c >> 7) & 7u, kb = (c >> 10) & 7u;
                 coef[j] = wS[pl * K + ka] * wS[pl * K + kb];
                 h[j] = Hs[pl * (kTileRec / 4) + g];
@@ -419,22 +427,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
 #pragma unroll
             for (int j = 0; j < kTileBatch; ++j)
             {
-                acc.x = fmaf(coef[j], h[j].x, acc.x);
-                acc.y = fmaf(coef[j], h[j].y, acc.y);
-                acc.z = fmaf(coef[j], h[j].z, acc.z);
-                acc.w = fmaf(coef[j], h[j].w, acc.w);
+                const bool on = e + j < e1;
+                acc.x = on ? fmaf(coef[j], h[j].x, acc.x) : acc.x;
+                acc.y = on ? fmaf(coef[j], h[j].y, acc.y) : acc.y;
+                acc.z = on ? fmaf(coef[j], h[j].z, acc.z) : acc.z;
+                acc.w = on ? fmaf(coef[j], h[j].w, acc.w) : acc.w;
             }
-        }
-        for (; e < e1; ++e)
-        {
-            const uint32_t c = ent[e];
-            const uint32_t pl = c & 127u, ka = (c >> 7) & 7u, kb = (c >> 10) & 7u;
-            const float coef = wS[pl * K + ka] * wS[pl * K + kb];
-            const float4 h = Hs[pl * (kTileRec / 4) + g];
-            acc.x = fmaf(coef, h.x, acc.x);
-            acc.y = fmaf(coef, h.y, acc.y);
-            acc.z = fmaf(coef, h.z, acc.z);
-            acc.w = fmaf(coef, h.w, acc.w);
         }
         out[size_t(s) * (kPartStride / 4) + g] = acc;
     }
@@ -620,7 +618,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc 
 {
     extern __shared__ float smem[];
     const int si = find_solve_t(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const uint32_t i = blockIdx.x - wg_begin[si];
     if (i >= uint32_t(d.N))
         return; // uniform over the group
@@ -742,11 +740,11 @@ __global__ __launch_bounds__(kTileThreads) void k_mf_tiles(const SolveDesc *__re
     __shared__ uint16_t ent[kTilePix * G::kPairs];
     __shared__ uint16_t sst[kTilePix * G::kPairs + 1];
     const int si = find_solve_t(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const uint32_t t = blockIdx.x - wg_begin[si];
     if (t >= uint32_t(d.n_tiles))
         return; // uniform over the group
-    const float *__restrict__ x = x_user ? x_user : d.dv[(k_it + 1) & 1];
+    const float *__restrict__ x = x_user ? x_user : pick2(d.dv, (k_it + 1) & 1);
     const uint32_t tid = threadIdx.x;
     const uint32_t p0 = t * kTilePix;
     const uint32_t np = min(uint32_t(kTilePix), uint32_t(d.P) - p0);
@@ -829,11 +827,11 @@ __global__ __launch_bounds__(64) void k_mf_rows(const SolveDesc *__restrict__ sd
                                                 const float *__restrict__ x_user, float *__restrict__ y_user)
 {
     const int si = find_solve_t(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const uint32_t i = blockIdx.x - wg_begin[si];
     if (i >= uint32_t(d.N))
         return;
-    const float *__restrict__ x = x_user ? x_user : d.dv[(k_it + 1) & 1];
+    const float *__restrict__ x = x_user ? x_user : pick2(d.dv, (k_it + 1) & 1);
     float *__restrict__ y = y_user ? y_user : d.q;
     const int lane = threadIdx.x;
     const int a = lane < 6 ? lane : 0;
@@ -944,7 +942,7 @@ __global__ __launch_bounds__(64) void k_spmv_rows(const SolveDesc *__restrict__ 
                                                   const float *__restrict__ x, float *__restrict__ y)
 {
     const int si = find_solve_t(wg_begin, n_solve, blockIdx.x);
-    const SolveDesc &d = sds[si];
+    const SolveDesc d = global_desc(sds[si]);
     const uint32_t i = blockIdx.x - wg_begin[si];
     if (i >= uint32_t(d.N))
         return;
@@ -1012,23 +1010,29 @@ hipError_t launch_lower_bound_u32(const uint32_t *keys, uint32_t n_keys, uint32_
 }
 
 hipError_t launch_tiles(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
-                        hipStream_t st)
+                        uint32_t slot_cap, hipStream_t st)
 {
     if (n_wg == 0)
         return hipSuccess;
+    // the slot lists in LDS are sized for the launch's largest tile
+    const size_t lds = ((size_t(slot_cap) * 2 + 2) * sizeof(uint16_t) + 3) & ~size_t(3);
     if (K == 8)
     {
         if (fp16)
-            hipLaunchKernelGGL((k_tiles<8, __half>), dim3(n_wg), dim3(kTileThreads), 0, st, sds, wmap, n_solve);
+            hipLaunchKernelGGL((k_tiles<8, __half>), dim3(n_wg), dim3(kTileThreads), lds, st, sds, wmap, n_solve,
+                               slot_cap);
         else
-            hipLaunchKernelGGL((k_tiles<8, float>), dim3(n_wg), dim3(kTileThreads), 0, st, sds, wmap, n_solve);
+            hipLaunchKernelGGL((k_tiles<8, float>), dim3(n_wg), dim3(kTileThreads), lds, st, sds, wmap, n_solve,
+                               slot_cap);
     }
     else
     {
         if (fp16)
-            hipLaunchKernelGGL((k_tiles<4, __half>), dim3(n_wg), dim3(kTileThreads), 0, st, sds, wmap, n_solve);
+            hipLaunchKernelGGL((k_tiles<4, __half>), dim3(n_wg), dim3(kTileThreads), lds, st, sds, wmap, n_solve,
+                               slot_cap);
         else
-            hipLaunchKernelGGL((k_tiles<4, float>), dim3(n_wg), dim3(kTileThreads), 0, st, sds, wmap, n_solve);
+            hipLaunchKernelGGL((k_tiles<4, float>), dim3(n_wg), dim3(kTileThreads), lds, st, sds, wmap, n_solve,
+                               slot_cap);
     }
     return hipGetLastError();
 }
