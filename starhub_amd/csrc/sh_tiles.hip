@@ -575,48 +575,57 @@ __device__ void row_reg_terms(const SolveDesc &d, uint32_t i, uint32_t beg, uint
     }
 }
 
-constexpr int kCombTasks = 4; // (block, float4 group) tasks per thread and pass
-
-// Sum of float4 group g of the slot partials listed in [l0, l1), in list order.
-// The list entries are fetched kBatch at a time so that their partials load
-// together (one round of dependent loads per batch, not per entry).
+// List entries fetched per round of dependent loads (matrix-free row kernel).
 constexpr int kBatch = 8;
-__device__ __forceinline__ float4 sum_list4(const SolveDesc &d, const float4 *__restrict__ part, uint32_t l0,
-                                            uint32_t l1, uint32_t g)
+
+// The combine cuts every block's slot list into chunks of kChunk entries; a
+// chunk's sum (list order) is one task, and a block's sum adds its chunk sums
+// in chunk order, so the result depends on the list alone.  Chunk sums are
+// kept for kChunkCap chunks at a time (a window), in LDS that the ARAP term
+// parameters use before them.
+constexpr uint32_t kChunk = 16;
+constexpr uint32_t kChunkCap = 128;
+constexpr uint32_t kCombUnionF4 = kChunkCap * (kTileRec / 4); // float4s of the shared region
+static_assert(sizeof(RegTerm) * kTileThreads <= kCombUnionF4 * sizeof(float4), "combine LDS union");
+
+// Sum of float4 group g over list entries [l0, l1), at most kChunk of them, in
+// list order: the entries, then their partials, each loaded in one round.
+__device__ __forceinline__ float4 sum_chunk4(const SolveDesc &d, const float4 *__restrict__ part, uint32_t l0,
+                                             uint32_t l1, uint32_t g)
 {
+    uint32_t it[kChunk];
+#pragma unroll
+    for (uint32_t j = 0; j < kChunk; ++j)
+        it[j] = (l0 + j < l1) ? d.list_items[l0 + j] : 0u;
+    float4 v[kChunk];
+#pragma unroll
+    for (uint32_t j = 0; j < kChunk; ++j)
+        v[j] = (l0 + j < l1) ? part[size_t(it[j]) * (kPartStride / 4) + g] : make_float4(0.f, 0.f, 0.f, 0.f);
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (uint32_t l = l0; l < l1; l += kBatch)
-    {
-        uint32_t it[kBatch];
 #pragma unroll
-        for (int j = 0; j < kBatch; ++j)
-            it[j] = (l + j < l1) ? d.list_items[l + j] : 0u;
-        float4 v[kBatch];
-#pragma unroll
-        for (int j = 0; j < kBatch; ++j)
-            v[j] = (l + j < l1) ? part[size_t(it[j]) * (kPartStride / 4) + g] : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-        for (int j = 0; j < kBatch; ++j)
-            if (l + j < l1)
-            {
-                acc.x += v[j].x;
-                acc.y += v[j].y;
-                acc.z += v[j].z;
-                acc.w += v[j].w;
-            }
-    }
+    for (uint32_t j = 0; j < kChunk; ++j)
+        if (l0 + j < l1)
+        {
+            acc.x += v[j].x;
+            acc.y += v[j].y;
+            acc.z += v[j].z;
+            acc.w += v[j].w;
+        }
     return acc;
 }
 
-// One work-group per node row: every block's slot partials summed in list
-// (tile) order, the ARAP terms (node_graph_reg_jacobian.cuh:166-358) and the
-// node-translation term (node_motion_jacobian.cuh:7-35) of the row added by
-// wave 0 in list order, the damping on the diagonal; full 6x6 blocks and
-// rhs_i = -(g of the diagonal slots + ARAP/translation g).
+// One work-group per node row: every block's slot partials summed (chunks in
+// list order, then the chunk sums in order), the ARAP terms
+// (node_graph_reg_jacobian.cuh:166-358) and the node-translation term
+// (node_motion_jacobian.cuh:7-35) of the row added in list order, the damping
+// on the diagonal; full 6x6 blocks and rhs_i = -(g of the diagonal slots +
+// ARAP/translation g).  The diagonal block and the upper blocks (columns above
+// i) are summed; each upper block (i, j) is also written transposed into
+// (j, i), so every block's slot partials are read once.
 __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc *__restrict__ sds,
                                                                 const uint32_t *__restrict__ wg_begin, int n_solve)
 {
-    extern __shared__ float smem[];
+    extern __shared__ float4 smem4[];
     const int si = find_solve_t(wg_begin, n_solve, blockIdx.x);
     const SolveDesc d = global_desc(sds[si]);
     const uint32_t i = blockIdx.x - wg_begin[si];
@@ -625,94 +634,152 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc 
     const uint32_t tid = threadIdx.x;
     const uint32_t base = d.row_ptr[i] + i;
     const uint32_t L1 = d.row_ptr[i + 1] - d.row_ptr[i] + 1;
-    // LDS: regblk[L1][36] | colmap u16[N] | red[16] | rt[kTileThreads] term parameters
-    float *regblk = smem;
-    uint16_t *colmap = reinterpret_cast<uint16_t *>(regblk + size_t(L1) * 36);
-    float *red = reinterpret_cast<float *>(colmap + ((d.N + 1) & ~1));
-    RegTerm *rt = reinterpret_cast<RegTerm *>(red + 16);
-    for (uint32_t t = tid; t < L1; t += kTileThreads)
-        colmap[d.blk_col[base + t]] = uint16_t(t);
+    constexpr uint32_t kG = kTileRec / 4;
+    // LDS: union{chunk sums csum[kChunkCap][7] | term parameters rt[256]} |
+    // run[L1][7] | regblk[L1][36] | red[16] | ls[L1] | le[L1] | cc[L1+1] | colmap u16[N]
+    float4 *csum = smem4;
+    RegTerm *rt = reinterpret_cast<RegTerm *>(smem4);
+    float4 *run = smem4 + kCombUnionF4;
+    float *regblk = reinterpret_cast<float *>(run + size_t(L1) * kG);
+    float *red = regblk + size_t(L1) * 36;
+    uint32_t *ls = reinterpret_cast<uint32_t *>(red + 16);
+    uint32_t *le = ls + L1;
+    uint32_t *cc = le + L1;
+    uint16_t *colmap = reinterpret_cast<uint16_t *>(cc + L1 + 1);
+
+    // columns (and how many lie below i: the upper blocks start after them)
+    uint32_t n_below = 0;
+    for (uint32_t t0 = 0; t0 < L1; t0 += kTileThreads)
+    {
+        const uint32_t t = t0 + tid;
+        bool below = false;
+        if (t < L1)
+        {
+            const uint32_t col = d.blk_col[base + t];
+            colmap[col] = uint16_t(t);
+            below = t > 0 && col < i;
+        }
+        n_below += uint32_t(__syncthreads_count(below));
+    }
+    const uint32_t c_up = 1 + n_below; // blocks c_up .. L1-1 have columns above i
+    const uint32_t nq = 1 + (L1 - c_up); // summed blocks: q = 0 the diagonal, q >= 1 block c_up + q - 1
     for (uint32_t t = tid; t < L1 * 36; t += kTileThreads)
         regblk[t] = 0.f;
+    for (uint32_t t = tid; t < nq * kG; t += kTileThreads)
+        run[t] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (tid < 16)
         red[tid] = 0.f;
+    for (uint32_t q = tid; q < nq; q += kTileThreads)
+    {
+        const uint32_t c = q == 0 ? 0u : c_up + q - 1;
+        ls[q] = d.list_off[base + c];
+        le[q] = d.list_off[base + c + 1];
+    }
     __syncthreads();
+    // chunk offsets cc[q] (exclusive prefix of the chunk counts), by wave 0
+    if (tid < 64)
+    {
+        uint32_t carry = 0;
+        for (uint32_t q0 = 0; q0 < nq; q0 += 64)
+        {
+            const uint32_t q = q0 + tid;
+            const uint32_t n = q < nq ? (le[q] - ls[q] + kChunk - 1) / kChunk : 0u;
+            uint32_t v = n;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1)
+            {
+                const uint32_t u = __shfl_up(v, o, 64);
+                if (int(tid) >= o)
+                    v += u;
+            }
+            if (q < nq)
+                cc[q] = carry + v - n;
+            carry += __shfl(v, 63, 64);
+        }
+        if (tid == 0)
+            cc[nq] = carry;
+    }
 
     const uint32_t dend = d.n2t_off[i] + d.dense_cnt[i];
     const uint32_t end = d.n2t_off[i + 1];
     // ARAP and node-translation terms of the row: parameters loaded one term
-    // per thread, then every entry summed over the terms in list order.
+    // per thread, then every entry summed over the terms in list order (ends
+    // with a group barrier, after which the shared region holds chunk sums)
     row_reg_terms(d, i, dend, end, colmap, regblk, red + 8, rt);
     __syncthreads();
 
     const float4 *__restrict__ part = reinterpret_cast<const float4 *>(d.tpart);
-    const size_t out0 = size_t(base) * 36;
-    // the diagonal block and the upper blocks (columns above i) of the row:
-    // each upper block (i, j) is also written transposed into (j, i), so every
-    // block's slot partials are read once; lower blocks are left to their rows
-    uint32_t c_up = 1, hi = L1; // first block with a column above i
-    while (c_up < hi)
+    const uint32_t total = cc[nq];
+    for (uint32_t w0 = 0; w0 < total; w0 += kChunkCap)
     {
-        const uint32_t m = (c_up + hi) >> 1;
-        if (d.blk_col[base + m] < i)
-            c_up = m + 1;
-        else
-            hi = m;
-    }
-    const uint32_t n_up = L1 - c_up; // upper blocks: c_up .. L1-1
-    const uint32_t n_tasks = (1 + n_up) * (kTileRec / 4);
-    for (uint32_t tp = 0; tp < n_tasks; tp += kTileThreads * kCombTasks)
-    {
-        float4 acc[kCombTasks];
-#pragma unroll
-        for (int s = 0; s < kCombTasks; ++s)
+        const uint32_t w1 = min(total, w0 + kChunkCap);
+        for (uint32_t task = tid; task < (w1 - w0) * kG; task += kTileThreads)
         {
-            acc[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-            const uint32_t task = tp + tid + s * kTileThreads;
-            if (task < n_tasks)
+            const uint32_t k = w0 + task / kG, g = task % kG;
+            uint32_t lo = 0, hi = nq; // the block of chunk k: last q with cc[q] <= k
+            while (hi - lo > 1)
             {
-                const uint32_t t7 = task / (kTileRec / 4), g = task % (kTileRec / 4);
-                const uint32_t c = t7 == 0 ? 0u : c_up + t7 - 1;
-                acc[s] = sum_list4(d, part, d.list_off[base + c], d.list_off[base + c + 1], g);
+                const uint32_t m = (lo + hi) >> 1;
+                if (cc[m] <= k)
+                    lo = m;
+                else
+                    hi = m;
             }
+            const uint32_t l0 = ls[lo] + (k - cc[lo]) * kChunk;
+            csum[task] = sum_chunk4(d, part, l0, min(l0 + kChunk, le[lo]), g);
         }
-        __syncthreads(); // regblk complete (wave 0) before the first writes
-#pragma unroll
-        for (int s = 0; s < kCombTasks; ++s)
+        __syncthreads();
+        for (uint32_t task = tid; task < nq * kG; task += kTileThreads)
         {
-            const uint32_t task = tp + tid + s * kTileThreads;
-            if (task >= n_tasks)
-                continue;
-            const uint32_t t7 = task / (kTileRec / 4), g = task % (kTileRec / 4);
-            const uint32_t c = t7 == 0 ? 0u : c_up + t7 - 1;
-            const size_t tr = size_t(d.blk_transpose[base + c]) * 36; // (j, i) for c > 0
-            const float av[4] = {acc[s].x, acc[s].y, acc[s].z, acc[s].w};
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
+            const uint32_t q = task / kG, g = task % kG;
+            const uint32_t k0 = max(cc[q], w0), k1 = min(cc[q + 1], w1);
+            float4 a = run[task];
+            for (uint32_t k = k0; k < k1; ++k)
             {
-                const int e = int(g) * 4 + q;
-                if (e < 21)
-                {
-                    int a, b;
-                    sym_pair_t(e, a, b);
-                    float u = av[q] + regblk[c * 36 + a * 6 + b];
-                    if (c == 0 && a == b)
-                        u += d.damping;
-                    d.blk[out0 + size_t(c) * 36 + a * 6 + b] = u;
-                    const float v = av[q] + regblk[c * 36 + b * 6 + a];
-                    if (a != b)
-                        d.blk[out0 + size_t(c) * 36 + b * 6 + a] = v;
-                    if (c > 0)
-                    {
-                        // the transposed block: (j, i)[b][a] = (i, j)[a][b]
-                        d.blk[tr + b * 6 + a] = u;
-                        if (a != b)
-                            d.blk[tr + a * 6 + b] = v;
-                    }
-                }
-                else if (c == 0 && e < 27)
-                    red[e - 21] = av[q]; // g of the diagonal slots
+                const float4 v = csum[(k - w0) * kG + g];
+                a.x += v.x;
+                a.y += v.y;
+                a.z += v.z;
+                a.w += v.w;
             }
+            run[task] = a;
+        }
+        __syncthreads();
+    }
+
+    const size_t out0 = size_t(base) * 36;
+    for (uint32_t task = tid; task < nq * kG; task += kTileThreads)
+    {
+        const uint32_t q = task / kG, g = task % kG;
+        const uint32_t c = q == 0 ? 0u : c_up + q - 1;
+        const size_t tr = size_t(d.blk_transpose[base + c]) * 36; // (j, i) for c > 0
+        const float4 acc = run[task];
+        const float av[4] = {acc.x, acc.y, acc.z, acc.w};
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+            const int e = int(g) * 4 + r;
+            if (e < 21)
+            {
+                int a, b;
+                sym_pair_t(e, a, b);
+                float u = av[r] + regblk[c * 36 + a * 6 + b];
+                if (c == 0 && a == b)
+                    u += d.damping;
+                d.blk[out0 + size_t(c) * 36 + a * 6 + b] = u;
+                const float v = av[r] + regblk[c * 36 + b * 6 + a];
+                if (a != b)
+                    d.blk[out0 + size_t(c) * 36 + b * 6 + a] = v;
+                if (c > 0)
+                {
+                    // the transposed block: (j, i)[b][a] = (i, j)[a][b]
+                    d.blk[tr + b * 6 + a] = u;
+                    if (a != b)
+                        d.blk[tr + a * 6 + b] = v;
+                }
+            }
+            else if (c == 0 && e < 27)
+                red[e - 21] = av[r]; // g of the diagonal slots
         }
     }
     __syncthreads();
@@ -1083,8 +1150,9 @@ hipError_t launch_spmv_rows(const SolveDesc *sds, const uint32_t *wmap, int n_so
 
 size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len)
 {
-    return sizeof(float) * ((size_t(max_row_len) + 1) * 36 + 16) + sizeof(uint16_t) * ((max_nodes + 1) & ~1u) +
-           sizeof(RegTerm) * kTileThreads;
+    const size_t L1 = size_t(max_row_len) + 1;
+    return sizeof(float4) * (kCombUnionF4 + L1 * (kTileRec / 4)) + sizeof(float) * (L1 * 36 + 16) +
+           sizeof(uint32_t) * (3 * L1 + 1) + sizeof(uint16_t) * ((max_nodes + 1) & ~1u);
 }
 
 hipError_t launch_tiles_combine(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, size_t lds,
