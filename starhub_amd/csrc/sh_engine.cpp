@@ -953,7 +953,8 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
             SH_HIP(launch_tiles_combine(sds, map(kMapRow, ns), n, map_total_[kMapRow], tiles_lds, st));
         }
         SH_HIP(mark(kPhDinv, st));
-        SH_HIP(launch_dinv(sds, map(kMapNode, ns), n, map_total_[kMapNode], st));
+        SH_HIP(launch_dinv(sds, map(kMapNode, ns), n, map_total_[kMapNode], static_cast<PcgSlot2 *>(d_slots2_),
+                           use_coop && resident && pipelined ? map_total_[kMapCoop] : 0u, st));
         SH_HIP(mark(kPhPcg, st));
         if (use_coop)
         {

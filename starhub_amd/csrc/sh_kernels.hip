@@ -794,8 +794,16 @@ __global__ __launch_bounds__(256) void k_diag_rhs(const SolveDesc *__restrict__ 
 // FP64 inverse of each damped diagonal block, one thread per node
 // (blockDiagonalInverseKernel, BlockDiagonalPreconditionerInverse.cuh:15-73).
 __global__ __launch_bounds__(256) void k_dinv(const SolveDesc *__restrict__ sds, const uint32_t *__restrict__ wg_begin,
-                                              int n_solve)
+                                              int n_solve, PcgSlot2 *__restrict__ slots2, uint32_t n_slots2)
 {
+    // the pipelined PCG that follows publishes into zeroed slots and vectors
+    // (tag 0, which its launch's first publications do not use); the grid has
+    // a thread for every one of its groups
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slots2 && gid < n_slots2)
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+            slots2[gid].v[a >> 1][a & 1] = 0.0;
     const int si = find_solve(wg_begin, n_solve, blockIdx.x);
     const SolveDesc d = global_desc(sds[si]);
     const uint32_t n = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
@@ -812,6 +820,12 @@ __global__ __launch_bounds__(256) void k_dinv(const SolveDesc *__restrict__ sds,
 #pragma unroll
     for (int a = 0; a < 36; ++a)
         o[a] = inv[a];
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+    {
+        d.mv[0][size_t(n) * 6 + a] = 0.0;
+        d.mv[1][size_t(n) * 6 + a] = 0.0;
+    }
 }
 
 // Off-diagonal blocks, one wave per unique pair (i<j): lanes stride the pair's
@@ -1617,11 +1631,14 @@ hipError_t launch_diag_rhs(int K, bool fp16, const SolveDesc *sds, const uint32_
     return hipGetLastError();
 }
 
-hipError_t launch_dinv(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, hipStream_t st)
+hipError_t launch_dinv(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, PcgSlot2 *slots2,
+                       uint32_t n_slots2, hipStream_t st)
 {
     if (n_wg == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_dinv, dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve);
+    if (slots2 && size_t(n_slots2) > size_t(n_wg) * 256)
+        return hipErrorInvalidValue; // never: a solve has no more PCG groups than nodes
+    hipLaunchKernelGGL(k_dinv, dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, slots2, n_slots2);
     return hipGetLastError();
 }
 

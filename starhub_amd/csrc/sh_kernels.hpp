@@ -34,7 +34,11 @@ hipError_t launch_reg_terms(const SolveDesc *sds, const uint32_t *wmap, int n_so
                             hipStream_t st, int store = 1);
 hipError_t launch_diag_rhs(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
                            hipStream_t st);
-hipError_t launch_dinv(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, hipStream_t st);
+struct PcgSlot2;
+// Inverse diagonal blocks; also zeroes slots2[0 .. n_slots2) and every solve's
+// published vectors for a pipelined PCG launch that follows (slots2 may be null).
+hipError_t launch_dinv(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, PcgSlot2 *slots2,
+                       uint32_t n_slots2, hipStream_t st);
 hipError_t launch_offdiag(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
                           hipStream_t st);
 hipError_t launch_pcg_init(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, double *part_rs,

@@ -494,39 +494,90 @@ struct PipeGroupSum
     double a, b;
 };
 
-// solve_barrier_sum for two values (same protocol, PcgSlot2 slots), without
-// group syncs of its own: the caller syncs before (the published values and
-// every store they cover are complete) and after (the solve's sums are in
-// out[0..1] for every thread).
-__device__ void solve_barrier_sum2(PipeGroupSum part, PcgSlot2 *__restrict__ slots, uint32_t b0, uint32_t G,
-                                   uint32_t ep, uint32_t *__restrict__ err, uint32_t cap, double *out)
+// Tagged publication.  A value published at sequence number ep goes to
+// buffer ep & 1 with ((ep + 1) >> 1) & 1 in the lowest bit of its mantissa
+// (one ulp, the same value for every reader).  The buffer's previous value,
+// from ep - 2, carries the other tag, and older values can no longer be seen:
+// every thread waits for its stores to complete (s_waitcnt vmcnt(0)) before
+// its group publishes the next sequence number, which a reader of ep has
+// waited for.  The kernel before each launch zeroes the launch's slots and
+// vectors (tag 0), and a launch starts its sequence numbers after a multiple
+// of 4, so its first two carry tag 1 and both slot entries are written by
+// then.  No group runs two sequence numbers ahead of another, because each
+// needs the others' values of the previous one.  A reader polls each value
+// until it carries the expected tag, so every value says by itself that it is
+// fresh: no epoch word, no cache flush or invalidation, and the solve's sums
+// and the halo arrive in one round trip.  Every value is one 8-byte atomic
+// access at agent scope.
+__device__ __forceinline__ uint64_t pub_tag(uint32_t ep)
 {
+    return uint64_t(((ep + 1u) >> 1) & 1u);
+}
+
+__device__ __forceinline__ double with_tag(double v, uint64_t t)
+{
+    return __longlong_as_double(
+        static_cast<long long>((static_cast<uint64_t>(__double_as_longlong(v)) & ~1ull) | t));
+}
+
+__device__ __forceinline__ bool has_tag(double v, uint64_t t)
+{
+    return (static_cast<uint64_t>(__double_as_longlong(v)) & 1ull) == t;
+}
+
+__device__ __forceinline__ void store_pub(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double load_pub(const double *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Re-reads *p until it carries tag t (v is the first read).  A wait that
+// exceeds the spin cap sets *err and ends every later wait at once, so each
+// wave always exits.
+__device__ __forceinline__ double await_pub(const double *p, double v, uint64_t t, uint32_t *__restrict__ err,
+                                            uint32_t cap)
+{
+    unsigned spins = 0;
+    while (!has_tag(v, t))
+    {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins >= cap || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        {
+            __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        v = load_pub(p);
+    }
+    return v;
+}
+
+// The solve's two sums of sequence number ep: the group's tagged partials go
+// to its slot, and wave 0 reads every group's in slot order and sums them in a
+// fixed tree, so every group gets the same totals, bit for bit; thread 0
+// leaves them in out[0..1] (read after the caller's next group sync).
+__device__ void publish_sum2(PipeGroupSum part, PcgSlot2 *__restrict__ slots, uint32_t b0, uint32_t G, uint32_t ep,
+                             uint32_t *__restrict__ err, uint32_t cap, double *out)
+{
+    const uint64_t t = pub_tag(ep);
     if (threadIdx.x == 0)
     {
         PcgSlot2 &me = slots[blockIdx.x];
-        me.v[ep & 1u][0] = part.a;
-        me.v[ep & 1u][1] = part.b;
-        __hip_atomic_store(&me.epoch, ep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        store_pub(&me.v[ep & 1u][0], with_tag(part.a, t));
+        store_pub(&me.v[ep & 1u][1], with_tag(part.b, t));
     }
     if (threadIdx.x < 64)
     {
         double va = 0.0, vb = 0.0;
         for (uint32_t i = threadIdx.x; i < G; i += 64)
         {
-            PcgSlot2 &o = slots[b0 + i];
-            unsigned spins = 0;
-            while (__hip_atomic_load(&o.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ep)
-            {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins >= cap || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                {
-                    __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            va += __hip_atomic_load(&o.v[ep & 1u][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            vb += __hip_atomic_load(&o.v[ep & 1u][1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const double *pa = &slots[b0 + i].v[ep & 1u][0], *pb = &slots[b0 + i].v[ep & 1u][1];
+            const double a = load_pub(pa), b = load_pub(pb);
+            va += await_pub(pa, a, t, err, cap);
+            vb += await_pub(pb, b, t, err, cap);
         }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1)
@@ -567,7 +618,8 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
     const bool fits = (r1 - r0) * 6 <= uint32_t(kResThreads) && (B1 - B0) <= uint32_t(kResThreads * kResBlocksPerThread);
     if (!fits && tid == 0)
         __hip_atomic_store(err, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t ep = __hip_atomic_load(epoch_base, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // publications of this launch start after a multiple of 4 (tag 1 twice)
+    uint32_t ep = (__hip_atomic_load(epoch_base, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 3u) & ~3u;
 
     uint32_t col[kResBlocksPerThread];
     float A[kResBlocksPerThread][36];
@@ -647,9 +699,9 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
         e0 = d.row_ptr[row] + row - B0;
         e1 = d.row_ptr[row + 1] + row + 1 - B0;
     }
-    // (A v)_a of the owned entry, v published in vb by every group of the
-    // solve (the caller has passed the solve barrier); two group syncs
-    auto apply = [&](const double *__restrict__ vb) -> double {
+    // (A v)_a of the owned entry, v published in vb with tag t by every group
+    // of the solve (each value is awaited as it is read); two group syncs
+    auto apply = [&](const double *__restrict__ vb, uint64_t t) -> double {
         double2 pv[kResBlocksPerThread][3];
         // the blocks look modified here, so their FP64 conversions stay inside
         // the loop instead of being hoisted into twice the registers
@@ -667,14 +719,14 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
             for (int f = 0; f < kPipeFetch; ++f)
             {
                 const uint32_t i = tid + uint32_t(f) * kResThreads;
-                hv[f] = i < n6 ? vb[size_t(clist[i / 6]) * 6 + i % 6] : 0.0;
+                hv[f] = i < n6 ? load_pub(vb + size_t(clist[i / 6]) * 6 + i % 6) : 0.0;
             }
 #pragma unroll
             for (int f = 0; f < kPipeFetch; ++f)
             {
                 const uint32_t i = tid + uint32_t(f) * kResThreads;
                 if (i < n6)
-                    halo[i] = hv[f];
+                    halo[i] = await_pub(vb + size_t(clist[i / 6]) * 6 + i % 6, hv[f], t, err, cap);
             }
             __syncthreads();
 #pragma unroll
@@ -691,11 +743,20 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
 #pragma unroll
             for (int j = 0; j < kResBlocksPerThread; ++j)
             {
-                const double2 *src =
-                    reinterpret_cast<const double2 *>(vb + size_t(col[j] != 0xffffffffu ? col[j] : 0) * 6);
+                double v6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                if (col[j] != 0xffffffffu)
+                {
+                    const double *src = vb + size_t(col[j]) * 6;
+#pragma unroll
+                    for (int c = 0; c < 6; ++c)
+                        v6[c] = load_pub(src + c);
+#pragma unroll
+                    for (int c = 0; c < 6; ++c)
+                        v6[c] = await_pub(src + c, v6[c], t, err, cap);
+                }
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    pv[j][c] = src[c];
+                    pv[j][c] = make_double2(v6[2 * c], v6[2 * c + 1]);
             }
             __syncthreads(); // keeps the two paths' sync counts equal
         }
@@ -734,6 +795,7 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
     // Stage 1 of an iteration: v's row entries and the wave partials of two
     // dot products go to LDS, one sync, then (M^-1 v)_a and the group sums.
     auto stage = [&](double v, double da, double db, double &mv, PipeGroupSum &gs) {
+        __builtin_amdgcn_s_waitcnt(0x0F70); // vmcnt(0): this thread's publications are complete
         rsh[tid] = v;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1)
@@ -767,25 +829,31 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
     double u = 0.0;
     PipeGroupSum gs{};
     stage(r, 0.0, 0.0, u, gs);
+    // every group uses the published (tagged) u and m, its own entries too
+    ++ep;
+    u = with_tag(u, pub_tag(ep));
     if (own)
-        d.mv[0][i6] = u;
-    __syncthreads(); // every m store is complete before the publish
-    (void)solve_barrier_sum2(PipeGroupSum{0.0, 0.0}, slots, b0, G, ++ep, err, cap, sh2);
-    __syncthreads();
-    double w = apply(d.mv[0]);
+        store_pub(pick2(d.mv, ep & 1u) + i6, u);
+    if (tid == 0) // the slot entry this number selects is written too (no sums yet)
+    {
+        store_pub(&slots[blockIdx.x].v[ep & 1u][0], with_tag(0.0, pub_tag(ep)));
+        store_pub(&slots[blockIdx.x].v[ep & 1u][1], with_tag(0.0, pub_tag(ep)));
+    }
+    double w = apply(pick2(d.mv, ep & 1u), pub_tag(ep));
     double x = 0.0, z = 0.0, q = 0.0, s = 0.0, p = 0.0, gprev = 0.0, aprev = 0.0;
     for (int k = 0; k < iters; ++k)
     {
         double m = 0.0;
         stage(w, own ? r * u : 0.0, own ? w * u : 0.0, m, gs);
-        double *__restrict__ mb = pick2(d.mv, (k + 1) & 1);
+        ++ep;
+        const uint64_t t = pub_tag(ep);
+        double *__restrict__ mb = pick2(d.mv, ep & 1u);
+        m = with_tag(m, t);
         if (own)
-            mb[i6] = m;
-        __syncthreads(); // m stores complete; rsh and sh read by every thread
-        (void)solve_barrier_sum2(gs, slots, b0, G, ++ep, err, cap, sh2);
-        __syncthreads();
+            store_pub(mb + i6, m);
+        publish_sum2(gs, slots, b0, G, ep, err, cap, sh2);
+        const double n = apply(mb, t); // its group syncs order sh2 for every thread
         const PipeGroupSum gd{sh2[0], sh2[1]};
-        const double n = apply(mb);
         const double gam = gd.a, del = gd.b;
         if (tid == 0 && g == 0)
             d.delta[k] = gam;
