@@ -924,6 +924,8 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
         return SH_E_CAPACITY;
     if (!(cfg_.flags & (SH_FLAG_ROW_SEGMENTS | SH_FLAG_PAIR_MAJOR)) && tiles_lds > 160u * 1024u)
         return SH_E_CAPACITY;
+    const bool tiles_path = !(cfg_.flags & (SH_FLAG_ROW_SEGMENTS | SH_FLAG_PAIR_MAJOR));
+    const uint32_t pipe_slots = use_coop && resident && pipelined ? map_total_[kMapCoop] : 0u;
     for (uint32_t it = 0; it < iters; ++it)
     {
         const int parity = int(it); // iteration offset: DQ buffer (gn_iter + it) & 1, cost slot gn_iter + it
@@ -950,11 +952,15 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
             SH_HIP(mark(kPhTiles, st));
             SH_HIP(launch_tiles(K_, fp16, sds, map(kMapTile, ns), n, map_total_[kMapTile], slot_cap, st));
             SH_HIP(mark(kPhTilesCombine, st));
-            SH_HIP(launch_tiles_combine(sds, map(kMapRow, ns), n, map_total_[kMapRow], tiles_lds, st));
+            SH_HIP(launch_tiles_combine(sds, map(kMapRow, ns), n, map_total_[kMapRow], tiles_lds,
+                                        static_cast<PcgSlot2 *>(d_slots2_), pipe_slots, st));
         }
-        SH_HIP(mark(kPhDinv, st));
-        SH_HIP(launch_dinv(sds, map(kMapNode, ns), n, map_total_[kMapNode], static_cast<PcgSlot2 *>(d_slots2_),
-                           use_coop && resident && pipelined ? map_total_[kMapCoop] : 0u, st));
+        if (!tiles_path) // the tile combine inverts the diagonal blocks itself
+        {
+            SH_HIP(mark(kPhDinv, st));
+            SH_HIP(launch_dinv(sds, map(kMapNode, ns), n, map_total_[kMapNode], static_cast<PcgSlot2 *>(d_slots2_),
+                               pipe_slots, st));
+        }
         SH_HIP(mark(kPhPcg, st));
         if (use_coop)
         {

@@ -605,65 +605,6 @@ __device__ __forceinline__ void load_reg(const SolveDesc &d, uint32_t e, f3 &a, 
     valid = u1.w;
 }
 
-// Gauss-Jordan 6x6 inverse with partial pivoting in FP64
-// (DenseGaussian<6>::Inverse, libs/star/math/DenseGaussian.hpp:34-113).
-__device__ void invert6_fp64(const float *A, double *out)
-{
-    double m[6][12];
-#pragma unroll
-    for (int i = 0; i < 6; ++i)
-#pragma unroll
-        for (int j = 0; j < 6; ++j)
-        {
-            m[i][j] = double(A[6 * i + j]);
-            m[i][6 + j] = (i == j) ? 1.0 : 0.0;
-        }
-#pragma unroll
-    for (int i = 0; i < 6; ++i)
-    {
-        int piv = i;
-        double pv = fabs(m[i][i]);
-#pragma unroll
-        for (int j = i + 1; j < 6; ++j)
-            if (pv < fabs(m[j][i]))
-            {
-                pv = fabs(m[j][i]);
-                piv = j;
-            }
-        if (piv != i)
-        {
-#pragma unroll
-            for (int j = i + 1; j < 6; ++j)
-                if (j == piv)
-#pragma unroll
-                    for (int k = 0; k < 12; ++k)
-                    {
-                        const double t = m[i][k];
-                        m[i][k] = m[j][k];
-                        m[j][k] = t;
-                    }
-        }
-        const double inv = 1.0 / m[i][i];
-#pragma unroll
-        for (int k = 0; k < 12; ++k)
-            m[i][k] *= inv;
-#pragma unroll
-        for (int r = 0; r < 6; ++r)
-            if (r != i)
-            {
-                const double f = m[r][i];
-#pragma unroll
-                for (int k = 0; k < 12; ++k)
-                    m[r][k] -= f * m[i][k];
-            }
-    }
-#pragma unroll
-    for (int i = 0; i < 6; ++i)
-#pragma unroll
-        for (int j = 0; j < 6; ++j)
-            out[6 * i + j] = m[i][6 + j];
-}
-
 constexpr int kSymIdx[6][6] = {{0, 1, 2, 3, 4, 5},     {1, 6, 7, 8, 9, 10},    {2, 7, 11, 12, 13, 14},
                                {3, 8, 12, 15, 16, 17}, {4, 9, 13, 16, 18, 19}, {5, 10, 14, 17, 19, 20}};
 

@@ -174,8 +174,10 @@ hipError_t launch_mf_rows(const SolveDesc *sds, const uint32_t *wmap, int n_solv
                           const float *x_user, float *y_user, hipStream_t st);
 hipError_t launch_spmv_rows(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, const float *x,
                             float *y, hipStream_t st);
+// Also the FP64 inverse diagonal blocks (as k_dinv) and the zeroed state of a
+// pipelined PCG launch that follows (slots2[0 .. n_slots2), may be null).
 hipError_t launch_tiles_combine(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, size_t lds,
-                                hipStream_t st);
+                                PcgSlot2 *slots2, uint32_t n_slots2, hipStream_t st);
 
 // rocPRIM wrappers (sh_sort.hip); temp == nullptr queries the temp size
 hipError_t sort_pairs_u16_u32(void *temp, size_t &temp_bytes, const uint16_t *kin, uint16_t *kout,

@@ -614,6 +614,81 @@ __device__ __forceinline__ float4 sum_chunk4(const SolveDesc &d, const float4 *_
     return acc;
 }
 
+// Orders the LDS accesses of one wave's lanes.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// invert6_fp64 (Gauss-Jordan with partial pivoting, FP64) run by one wave on
+// an LDS copy of the augmented 6 x 12 matrix: lane l owns entries l and l + 64.
+// The pivots and the elimination order are those of invert6_fp64.
+__device__ void invert6_fp64_wave(const float *__restrict__ A, double (*gm)[12], double *__restrict__ out)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (uint32_t h = 0; h < 2; ++h)
+    {
+        const uint32_t e = lane + 64u * h, r = e / 12u, k = e % 12u;
+        if (e < 72u)
+            gm[r][k] = k < 6u ? double(A[6u * r + k]) : (k - 6u == r ? 1.0 : 0.0);
+    }
+    wave_lds_sync();
+    for (int i = 0; i < 6; ++i)
+    {
+        int piv = i;
+        double pv = fabs(gm[i][i]);
+        for (int j = i + 1; j < 6; ++j)
+            if (pv < fabs(gm[j][i]))
+            {
+                pv = fabs(gm[j][i]);
+                piv = j;
+            }
+        if (piv != i) // uniform over the wave
+        {
+            double a = 0.0, b = 0.0;
+            if (lane < 12u)
+            {
+                a = gm[i][lane];
+                b = gm[piv][lane];
+            }
+            wave_lds_sync();
+            if (lane < 12u)
+            {
+                gm[i][lane] = b;
+                gm[piv][lane] = a;
+            }
+            wave_lds_sync();
+        }
+        const double inv = 1.0 / gm[i][i];
+        wave_lds_sync();
+        if (lane < 12u)
+            gm[i][lane] *= inv;
+        wave_lds_sync();
+        double f[2], p[2];
+#pragma unroll
+        for (uint32_t h = 0; h < 2; ++h)
+        {
+            const uint32_t e = lane + 64u * h, r = e / 12u, k = e % 12u;
+            f[h] = (e < 72u && int(r) != i) ? gm[r][i] : 0.0;
+            p[h] = e < 72u ? gm[i][k] : 0.0;
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (uint32_t h = 0; h < 2; ++h)
+        {
+            const uint32_t e = lane + 64u * h, r = e / 12u, k = e % 12u;
+            if (e < 72u && int(r) != i)
+                gm[r][k] -= f[h] * p[h];
+        }
+        wave_lds_sync();
+    }
+    if (lane < 36u)
+        out[lane] = gm[lane / 6u][6u + lane % 6u];
+}
+
 // One work-group per node row: every block's slot partials summed (chunks in
 // list order, then the chunk sums in order), the ARAP terms
 // (node_graph_reg_jacobian.cuh:166-358) and the node-translation term
@@ -623,9 +698,16 @@ __device__ __forceinline__ float4 sum_chunk4(const SolveDesc &d, const float4 *_
 // i) are summed; each upper block (i, j) is also written transposed into
 // (j, i), so every block's slot partials are read once.
 __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc *__restrict__ sds,
-                                                                const uint32_t *__restrict__ wg_begin, int n_solve)
+                                                                const uint32_t *__restrict__ wg_begin, int n_solve,
+                                                                PcgSlot2 *__restrict__ slots2, uint32_t n_slots2)
 {
     extern __shared__ float4 smem4[];
+    __shared__ float dg[36];       // the diagonal block, for its inverse
+    __shared__ double gm[6][12];   // its augmented matrix during the inversion
+    // a pipelined PCG that follows publishes into zeroed slots (the row map
+    // has a work-group for every one of its groups)
+    if (slots2 && blockIdx.x < n_slots2 && threadIdx.x < 4)
+        slots2[blockIdx.x].v[threadIdx.x >> 1][threadIdx.x & 1] = 0.0;
     const int si = find_solve_t(wg_begin, n_solve, blockIdx.x);
     const SolveDesc d = global_desc(sds[si]);
     const uint32_t i = blockIdx.x - wg_begin[si];
@@ -770,6 +852,11 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc 
                 const float v = av[r] + regblk[c * 36 + b * 6 + a];
                 if (a != b)
                     d.blk[out0 + size_t(c) * 36 + b * 6 + a] = v;
+                if (c == 0)
+                {
+                    dg[a * 6 + b] = u;
+                    dg[b * 6 + a] = a != b ? v : u;
+                }
                 if (c > 0)
                 {
                     // the transposed block: (j, i)[b][a] = (i, j)[a][b]
@@ -785,6 +872,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc 
     __syncthreads();
     if (tid < 6)
         d.rhs[size_t(i) * 6 + tid] = -(red[tid] + red[8 + tid]);
+    // the FP64 inverse of the diagonal block (the same float values the block
+    // store holds) and the row's zeroed pipelined-PCG vectors
+    if (tid >= 64 && tid < 128)
+        invert6_fp64_wave(dg, gm, d.dinv + size_t(i) * 36);
+    if (tid >= 128 && tid < 140)
+        d.mv[(tid - 128) / 6][size_t(i) * 6 + (tid - 128) % 6] = 0.0;
 }
 
 // ------------------------------------------------------------ matrix-free J^T W J x
@@ -1156,7 +1249,7 @@ size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len)
 }
 
 hipError_t launch_tiles_combine(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, size_t lds,
-                                hipStream_t st)
+                                PcgSlot2 *slots2, uint32_t n_slots2, hipStream_t st)
 {
     if (n_wg == 0)
         return hipSuccess;
@@ -1167,7 +1260,9 @@ hipError_t launch_tiles_combine(const SolveDesc *sds, const uint32_t *wmap, int 
         if (e != hipSuccess)
             return e;
     }
-    hipLaunchKernelGGL(k_tiles_combine, dim3(n_wg), dim3(kTileThreads), lds, st, sds, wmap, n_solve);
+    if (slots2 && n_slots2 > n_wg)
+        return hipErrorInvalidValue; // never: a solve has no more PCG groups than rows
+    hipLaunchKernelGGL(k_tiles_combine, dim3(n_wg), dim3(kTileThreads), lds, st, sds, wmap, n_solve, slots2, n_slots2);
     return hipGetLastError();
 }
 } // namespace sh
