@@ -966,7 +966,8 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
         {
             if (resident && pipelined)
                 SH_HIP(launch_pcg_pipe(sds, map(kMapCoop, ns), n, map_total_[kMapCoop], int(cfg_.pcg_iterations),
-                                       static_cast<PcgSlot2 *>(d_slots2_), flags + 0, flags + 1, wait_limit_, st));
+                                       static_cast<PcgSlot2 *>(d_slots2_), flags + 0, flags + 1, wait_limit_,
+                                       parity, st));
             else if (resident)
                 SH_HIP(launch_pcg_resident(sds, map(kMapCoop, ns), n, map_total_[kMapCoop],
                                            int(cfg_.pcg_iterations), static_cast<PcgSlot *>(d_slots_), flags + 0,
@@ -984,8 +985,11 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
             SH_HIP(pcg_graph(ns, matrix_free, st));
             SH_HIP(hipGraphLaunch(graph_exec_, st));
         }
-        SH_HIP(mark(kPhUpdate, st));
-        SH_HIP(launch_update(sds, map(kMapNode, ns), n, map_total_[kMapNode], parity, st));
+        if (!pipe_slots) // the pipelined PCG applies the update itself
+        {
+            SH_HIP(mark(kPhUpdate, st));
+            SH_HIP(launch_update(sds, map(kMapNode, ns), n, map_total_[kMapNode], parity, st));
+        }
     }
     SH_HIP(mark(kNumPhases, st));
     return SH_OK;

@@ -9,6 +9,7 @@
 #include "sh_common.hpp"
 #include "sh_desc.hpp"
 #include "sh_kernels.hpp"
+#include "sh_update.hpp"
 
 namespace sh
 {
@@ -367,15 +368,6 @@ __device__ __forceinline__ void load_knn(const SolveDesc &d, uint32_t p, uint16_
             w[k] = d.pw[size_t(p) * K + k];
         }
     }
-}
-
-__device__ __forceinline__ dq8 load_dq(const dq8 *__restrict__ a, uint32_t i)
-{
-    const float4 *p = reinterpret_cast<const float4 *>(a + i);
-    dq8 r;
-    r.q0 = p[0];
-    r.q1 = p[1];
-    return r;
 }
 
 // Sum over a 256-thread work-group in FP64, fixed order (wave trees, then waves 0..3).
@@ -1073,18 +1065,6 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(const SolveDesc *__r
         part_rs[blockIdx.x] = tot;
 }
 
-// Total weighted cost (dense + ARAP) from the work-group partials of
-// k_dense_terms and k_reg_terms, FP64, fixed order; one wave.
-__device__ void reduce_cost(const SolveDesc &d, int slot)
-{
-    const uint32_t n = (uint32_t(d.P) + 255u) / 256u + (uint32_t(d.E + d.N) + 255u) / 256u;
-    double v = 0.0;
-    for (uint32_t i = threadIdx.x; i < n; i += 64)
-        v += d.cost_part[i];
-    v = wave_sum(v);
-    if (threadIdx.x == 0)
-        d.cost[slot] = v;
-}
 
 // One wave per solve: the cost after the last update (SH_FLAG_COST_HISTORY).
 __global__ void k_cost_final(const SolveDesc *__restrict__ sds, int slot)
@@ -1104,50 +1084,7 @@ __global__ __launch_bounds__(256) void k_update(const SolveDesc *__restrict__ sd
     const uint32_t n = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
     if (n >= uint32_t(d.N))
         return;
-    const float *xv = d.x + size_t(n) * 6;
-    const f3 w = mk3(xv[0], xv[1], xv[2]);
-    const f3 t = mk3(xv[3], xv[4], xv[5]);
-    rt34 S;
-    if (fabsf(w.x) + fabsf(w.y) + fabsf(w.z) < 1e-4f)
-    {
-        S.r[0] = 1.f;
-        S.r[1] = 0.f;
-        S.r[2] = 0.f;
-        S.r[3] = 0.f;
-        S.r[4] = 1.f;
-        S.r[5] = 0.f;
-        S.r[6] = 0.f;
-        S.r[7] = 0.f;
-        S.r[8] = 1.f;
-    }
-    else
-    {
-        const float angle = sqrtf(dot3(w, w));
-        const f3 ax = (1.0f / angle) * w;
-        const float c = cosf(angle), s = sinf(angle), tt = 1.0f - c;
-        S.r[0] = tt * ax.x * ax.x + c;
-        S.r[1] = tt * ax.x * ax.y - ax.z * s;
-        S.r[2] = tt * ax.x * ax.z + ax.y * s;
-        S.r[3] = tt * ax.x * ax.y + ax.z * s;
-        S.r[4] = tt * ax.y * ax.y + c;
-        S.r[5] = tt * ax.y * ax.z - ax.x * s;
-        S.r[6] = tt * ax.x * ax.z - ax.y * s;
-        S.r[7] = tt * ax.y * ax.z + ax.x * s;
-        S.r[8] = tt * ax.z * ax.z + c;
-    }
-    S.t = t;
-    const rt34 prev = dq_to_mat34(load_dq(pick2(d.dq, (d.gn_iter + parity) & 1), n));
-    rt34 nw;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            nw.r[3 * i + j] = S.r[3 * i] * prev.r[j] + S.r[3 * i + 1] * prev.r[3 + j] + S.r[3 * i + 2] * prev.r[6 + j];
-    nw.t = rot_apply(S.r, prev.t) + S.t;
-    const dq8 out = dq_from_mat34(nw);
-    float4 *o = reinterpret_cast<float4 *>(pick2(d.dq, ((d.gn_iter + parity) & 1) ^ 1) + n);
-    o[0] = out.q0;
-    o[1] = out.q1;
+    warp_update_node(d, n, parity, d.x + size_t(n) * 6);
 }
 
 __global__ void k_identity(const SolveDesc *__restrict__ sds, const uint32_t *__restrict__ wg_begin, int n_solve)

@@ -111,9 +111,11 @@ struct PcgSlot2
 };
 // Pipelined resident PCG: one barrier per iteration, FP64 vectors; same
 // groups as the resident kernel.
+// update_parity >= 0: the kernel also applies the warp update (k_update's
+// work for that GN-iteration offset) to each group's own nodes.
 hipError_t launch_pcg_pipe(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
                            PcgSlot2 *slots, uint32_t *epoch_base, uint32_t *err, uint32_t wait_limit,
-                           hipStream_t st);
+                           int update_parity, hipStream_t st);
 int pcg_pipe_max_groups();
 constexpr uint32_t kBitmapWords = 128;
 // Floats per tile-slot partial in HBM: the 28 of the record (21 H, 6 g, pad)

@@ -16,6 +16,7 @@
 #include "sh_common.hpp"
 #include "sh_desc.hpp"
 #include "sh_kernels.hpp"
+#include "sh_update.hpp"
 
 namespace sh
 {
@@ -597,7 +598,8 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
                                                           const uint32_t *__restrict__ wg_begin, int n_solve, int iters,
                                                           PcgSlot2 *__restrict__ slots,
                                                           uint32_t *__restrict__ epoch_base,
-                                                          uint32_t *__restrict__ err, uint32_t cap)
+                                                          uint32_t *__restrict__ err, uint32_t cap,
+                                                          int update_parity)
 {
     __shared__ double qblk[kResThreads * kResBlocksPerThread * 6];
     __shared__ double rsh[kResThreads];
@@ -876,13 +878,30 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
         d.x[i6] = float(x);
         d.r[i6] = float(r);
     }
+    if (update_parity >= 0)
+    {
+        // the warp update of the group's own nodes, from the FP32 x as stored
+        // (k_update's work, with its cost reduction on each solve's group 0)
+        rsh[tid] = own ? double(float(x)) : 0.0;
+        __syncthreads();
+        if (g == 0 && tid < 64)
+            reduce_cost(d, d.gn_iter + update_parity);
+        if (tid < r1 - r0)
+        {
+            float xv[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                xv[c] = float(rsh[tid * 6 + c]);
+            warp_update_node(d, r0 + tid, update_parity, xv);
+        }
+    }
     if (blockIdx.x == 0 && tid == 0)
         __hip_atomic_store(epoch_base, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 hipError_t launch_pcg_pipe(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
                            PcgSlot2 *slots, uint32_t *epoch_base, uint32_t *err, uint32_t wait_limit,
-                           hipStream_t st)
+                           int update_parity, hipStream_t st)
 {
     // A plain launch: the kernel needs its groups co-resident (the host keeps
     // n_wg within the occupancy bound, pcg_pipe_max_groups) but no grid-wide
@@ -890,7 +909,7 @@ hipError_t launch_pcg_pipe(const SolveDesc *sds, const uint32_t *wmap, int n_sol
     // only delays the others.  The cooperative launch API adds a per-launch
     // cost of about 20 us on this device.
     hipLaunchKernelGGL(k_pcg_pipe, dim3(n_wg), dim3(kResThreads), 0, st, sds, wmap, n_solve, iters, slots, epoch_base,
-                       err, wait_limit ? wait_limit : kSpinCap);
+                       err, wait_limit ? wait_limit : kSpinCap, update_parity);
     return hipGetLastError();
 }
 
