@@ -384,6 +384,7 @@ int Engine::alloc_lane(IndexLane &ln, size_t npix)
 #undef L
     if ((rc = alloc(&ln.sort_temp, sort_temp_bytes_)) != SH_OK)
         return rc;
+    SH_HIP(hipMemset(ln.err, 0, 64)); // then kept zero by the compaction kernel of each build
     SH_HIP(hipHostMalloc(reinterpret_cast<void **>(&ln.h_meta), 64, hipHostMallocDefault));
     return SH_OK;
 }
@@ -396,12 +397,12 @@ int Engine::index_pixels(uint32_t slot, IndexLane &ln, hipStream_t st)
     SolveDesc &d = slots_[slot].d;
     const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
     size_t tb = sort_temp_bytes_;
-    SH_HIP(hipMemsetAsync(ln.err, 0, 4, st));
+    // ln.err is zero here: zeroed at allocation, by every build's compaction
+    // and on the rejection path
     SH_HIP(launch_pixel_flags(d, n, ln.flags, ln.err, st));
     SH_HIP(exclusive_scan_u32(ln.sort_temp, tb, ln.flags, ln.pos, n, st));
-    SH_HIP(hipMemcpyAsync(ln.h_meta + 0, ln.pos + (n - 1), 4, hipMemcpyDeviceToHost, st));
-    SH_HIP(hipMemcpyAsync(ln.h_meta + 1, ln.flags + (n - 1), 4, hipMemcpyDeviceToHost, st));
-    SH_HIP(hipMemcpyAsync(ln.h_meta + 12, ln.err, 4, hipMemcpyDeviceToHost, st));
+    const Readback rb{{ln.pos + (n - 1), ln.flags + (n - 1), ln.err, nullptr}, {0, 1, 12, 0}, 3};
+    SH_HIP(launch_readback(rb, ln.h_meta, st));
     return SH_OK;
 }
 
@@ -413,29 +414,31 @@ int Engine::index_rows(uint32_t slot, IndexLane &ln, hipStream_t st)
     Slot &sl = slots_[slot];
     SolveDesc &d = sl.d;
     if (ln.h_meta[12] != 0)
-        return SH_E_ARG; // a surfel or node id out of range
+    {
+        SH_HIP(hipMemsetAsync(ln.err, 0, 4, st)); // for the next build on this lane
+        return SH_E_ARG;                          // a surfel or node id out of range
+    }
     const uint32_t P = ln.h_meta[0] + ln.h_meta[1];
     if (P > pcap_)
         return SH_E_CAPACITY;
     d.P = int(P);
-    SH_HIP(launch_compact_pixels(d, ln.flags, ln.pos, st));
+    const uint32_t nt = div_up(P, kTilePix);
+    const IndexZero z{ln.err, ln.meta, ln.rowlen + d.N, ln.segcnt + d.N, ln.tnslot + nt, ln.tncontrib + nt};
+    SH_HIP(launch_compact_pixels(d, ln.flags, ln.pos, z, st));
     // node -> term (Node2TermsIndex::BuildIndex, Node2TermsIndex.cu:191-199,312-360)
     const uint32_t n_kv = P * uint32_t(K_) + 2 * uint32_t(d.E) + uint32_t(d.N);
     SH_HIP(launch_node2term_kv(d, ln.k16a, ln.v32a, st));
     size_t tb = sort_temp_bytes_;
     SH_HIP(sort_pairs_u16_u32(ln.sort_temp, tb, ln.k16a, ln.k16b, ln.v32a, d.n2t_terms, n_kv, 16, st));
     SH_HIP(launch_lower_bound_u16(ln.k16b, n_kv, uint32_t(d.N) + 1, d.n2t_off, st));
-    SH_HIP(hipMemsetAsync(ln.meta, 0, 64, st));
-    SH_HIP(hipMemsetAsync(ln.rowlen + d.N, 0, 4, st));
-    SH_HIP(hipMemsetAsync(ln.segcnt + d.N, 0, 4, st));
     SH_HIP(launch_row_bitmap(d, ln.bitmap, ln.rowlen, ln.segcnt, ln.meta, st));
     tb = sort_temp_bytes_;
     SH_HIP(exclusive_scan_u32(ln.sort_temp, tb, ln.segcnt, d.seg_ptr, uint32_t(d.N) + 1, st));
-    SH_HIP(hipMemcpyAsync(ln.h_meta + 6, d.seg_ptr + d.N, 4, hipMemcpyDeviceToHost, st));
     tb = sort_temp_bytes_;
     SH_HIP(exclusive_scan_u32(ln.sort_temp, tb, ln.rowlen, d.row_ptr, uint32_t(d.N) + 1, st));
     SH_HIP(launch_row_cols(d, ln.bitmap, ln.meta, st));
-    SH_HIP(hipMemcpyAsync(ln.h_meta + 4, ln.meta, 8, hipMemcpyDeviceToHost, st));
+    const Readback rb{{d.seg_ptr + d.N, ln.meta, ln.meta + 1, nullptr}, {6, 4, 5, 0}, 3};
+    SH_HIP(launch_readback(rb, ln.h_meta, st));
     // the row structure is complete here (blk_col bounded by blk_cap)
     SH_HIP(launch_tile_build_async(d, ln, st));
     return SH_OK;
@@ -605,7 +608,10 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
     if (P > pcap_)
         return SH_E_CAPACITY;
     d.P = int(P);
-    SH_HIP(launch_compact_pixels(d, flags, pos, st));
+    const uint32_t nt = div_up(P, kTilePix);
+    const IndexZero z{err, static_cast<uint32_t *>(ws_meta_), static_cast<uint32_t *>(ws_rowlen_) + d.N,
+                      static_cast<uint32_t *>(ws_segcnt_) + d.N, lanes_[0].tnslot + nt, lanes_[0].tncontrib + nt};
+    SH_HIP(launch_compact_pixels(d, flags, pos, z, st));
 
     // node -> term (Node2TermsIndex::BuildIndex, Node2TermsIndex.cu:191-199,312-360)
     const uint32_t n_kv = P * uint32_t(K_) + 2 * uint32_t(d.E) + uint32_t(d.N);
@@ -684,8 +690,7 @@ hipError_t Engine::launch_tile_build_async(SolveDesc &d, IndexLane &ln, hipStrea
     const uint32_t nt = uint32_t(d.n_tiles);
     uint32_t *ns = ln.tnslot, *nc = ln.tncontrib;
     hipError_t e;
-    if ((e = hipMemsetAsync(ns + nt, 0, 8, st)) != hipSuccess || (e = hipMemsetAsync(nc + nt, 0, 4, st)) != hipSuccess)
-        return e;
+    // ns[nt], ns[nt + 1] and nc[nt] were zeroed by the compaction kernel
     if ((e = launch_tile_build(d, ln.tcontrib, ln.tslot_block, ln.tslot_first, ns, nc, st)) != hipSuccess)
         return e;
     size_t tb = sort_temp_bytes_;
@@ -694,11 +699,8 @@ hipError_t Engine::launch_tile_build_async(SolveDesc &d, IndexLane &ln, hipStrea
     tb = sort_temp_bytes_;
     if ((e = exclusive_scan_u32(ln.sort_temp, tb, nc, d.tile_cbase, nt + 1, st)) != hipSuccess)
         return e;
-    if ((e = hipMemcpyAsync(ln.h_meta + 10, d.tile_sbase + nt, 4, hipMemcpyDeviceToHost, st)) != hipSuccess)
-        return e;
-    if ((e = hipMemcpyAsync(ln.h_meta + 13, ns + nt + 1, 4, hipMemcpyDeviceToHost, st)) != hipSuccess)
-        return e;
-    return hipMemcpyAsync(ln.h_meta + 11, d.tile_cbase + nt, 4, hipMemcpyDeviceToHost, st);
+    const Readback rb{{d.tile_sbase + nt, d.tile_cbase + nt, ns + nt + 1, nullptr}, {10, 11, 13, 0}, 3};
+    return launch_readback(rb, ln.h_meta, st);
 }
 
 // Tile index, second part (after the synchronisation): size the slot arrays,

@@ -69,8 +69,25 @@ __global__ void k_pixel_flags(SolveDesc d, uint32_t n, uint32_t *__restrict__ fl
 // WarpSolver.cu:7-48, + compactPontentialImageTermPixelsKernel,
 // ImageTermKNNFetcher.cu:56-88).
 template <int K>
-__global__ void k_compact_pixels(SolveDesc d, const uint32_t *__restrict__ flags, const uint32_t *__restrict__ pos)
+__global__ void k_compact_pixels(SolveDesc d, const uint32_t *__restrict__ flags, const uint32_t *__restrict__ pos,
+                                 IndexZero z)
 {
+    if (blockIdx.x == 0)
+    {
+        const uint32_t t = threadIdx.x;
+        if (t < 16)
+            z.meta[t] = 0u;
+        else if (t == 16)
+            *z.err = 0u;
+        else if (t == 17)
+            *z.rowlen_end = 0u;
+        else if (t == 18)
+            *z.segcnt_end = 0u;
+        else if (t == 19 || t == 20)
+            z.tile_nslot_end[t - 19] = 0u;
+        else if (t == 21)
+            *z.tile_ncontrib_end = 0u;
+    }
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
     if (i >= n || flags[i] == 0u)
@@ -1343,13 +1360,27 @@ hipError_t launch_pixel_flags(const SolveDesc &d, uint32_t n, uint32_t *flags, u
     return hipGetLastError();
 }
 
-hipError_t launch_compact_pixels(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos, hipStream_t st)
+hipError_t launch_compact_pixels(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos, const IndexZero &z,
+                                 hipStream_t st)
 {
     const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
+    const dim3 grid(max(1u, (n + 255u) / 256u)); // block 0 always runs: it clears z
     if (d.K == 8)
-        hipLaunchKernelGGL(k_compact_pixels<8>, SH_GRID(n, 256), dim3(256), 0, st, d, flags, pos);
+        hipLaunchKernelGGL(k_compact_pixels<8>, grid, dim3(256), 0, st, d, flags, pos, z);
     else
-        hipLaunchKernelGGL(k_compact_pixels<4>, SH_GRID(n, 256), dim3(256), 0, st, d, flags, pos);
+        hipLaunchKernelGGL(k_compact_pixels<4>, grid, dim3(256), 0, st, d, flags, pos, z);
+    return hipGetLastError();
+}
+
+__global__ void k_readback(Readback r, uint32_t *host)
+{
+    if (threadIdx.x < r.n)
+        host[r.dst[threadIdx.x]] = *r.src[threadIdx.x];
+}
+
+hipError_t launch_readback(const Readback &r, uint32_t *host, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, st, r, host);
     return hipGetLastError();
 }
 

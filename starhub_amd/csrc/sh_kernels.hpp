@@ -10,7 +10,28 @@
 namespace sh
 {
 hipError_t launch_pixel_flags(const SolveDesc &d, uint32_t n, uint32_t *flags, uint32_t *err, hipStream_t st);
-hipError_t launch_compact_pixels(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos, hipStream_t st);
+// Words an index build accumulates into, zeroed by the compaction kernel for
+// the stages after it (and err for the next build on the same workspace).
+struct IndexZero
+{
+    uint32_t *err;               // 1 word
+    uint32_t *meta;              // 16 words
+    uint32_t *rowlen_end;        // 1 word
+    uint32_t *segcnt_end;        // 1 word
+    uint32_t *tile_nslot_end;    // 2 words: the scan total and the largest count
+    uint32_t *tile_ncontrib_end; // 1 word
+};
+hipError_t launch_compact_pixels(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos, const IndexZero &z,
+                                 hipStream_t st);
+// Up to four device words copied into a pinned host buffer by one small
+// kernel (host[dst[i]] = *src[i]), in place of one copy per word.
+struct Readback
+{
+    const uint32_t *src[4];
+    uint32_t dst[4];
+    uint32_t n;
+};
+hipError_t launch_readback(const Readback &r, uint32_t *host, hipStream_t st);
 hipError_t launch_node2term_kv(const SolveDesc &d, uint16_t *keys, uint32_t *vals, hipStream_t st);
 hipError_t launch_lower_bound_u16(const uint16_t *keys, uint32_t n_keys, uint32_t n_out, uint32_t *out,
                                   hipStream_t st);
