@@ -71,9 +71,10 @@ def phase_bytes(name: str, cfg, P: int, U: int) -> float:
     Per-unit figures (K slots, record = the per-pixel H/g record: 28 values,
     4 B each in FP32 streams, 2 B in FP16 streams):
       dense_terms  P * (8 pixel pair + 2K ids + 4K folded weights + 64 texels + record) + N * 32 (node DQs)
-      reg_terms    E * 40 + N * 48
+                   + E * 40 + N * 48 (the ARAP terms, same launch: k_terms)
+      reg_terms    E * 40 + N * 48 (its own launch, other paths)
       tiles        P * (record + 2K + 4K)  (each record and its slots read once)
-      tiles_combine E * 40 + (N + 2U) * 144 (blocks out) + N * 24 (rhs out)
+      tiles_combine E * 40 + (N + 2U) * 144 (blocks out) + N * 24 (rhs out) + N * 288 (FP64 inverse out)
       rows_seg, rows_combine: as tiles, tiles_combine (the row-segment variant)
       dinv         N * (144 in + 288 FP64 out)
       pcg          iterations * ((N + 2U) * 148 + N * (10 * 24 + 144))
@@ -84,19 +85,19 @@ def phase_bytes(name: str, cfg, P: int, U: int) -> float:
     rec = 28 * (2 if cfg.solver.fp16 else 4)
     it = cfg.solver.pcg_iterations
     return {
-        "dense_terms": P * (8 + 2 * K + 4 * K + 64 + rec) + N * 32,
+        "dense_terms": P * (8 + 2 * K + 4 * K + 64 + rec) + N * 32 + E * 40 + N * 48,
         "reg_terms": E * 40 + N * 48,
         "rows_seg": P * (rec + 6 * K),
         "rows_combine": E * 40 + (N + 2 * U) * 144 + N * 24,
         "tiles": P * (rec + 6 * K),
-        "tiles_combine": E * 40 + (N + 2 * U) * 144 + N * 24,
+        "tiles_combine": E * 40 + (N + 2 * U) * 144 + N * 24 + N * 288,
         "dinv": N * (144 + 288),
         "pcg": it * ((N + 2 * U) * 148 + N * (10 * 24 + 144)),
         "update": N * 88,
     }.get(name, 0.0)
 
 
-KERNEL_OF_PHASE = {"dense_terms": "k_dense_terms", "reg_terms": "k_reg_terms", "rows_seg": "k_rows_seg",
+KERNEL_OF_PHASE = {"dense_terms": "k_terms", "reg_terms": "k_reg_terms", "rows_seg": "k_rows_seg",
                    "rows_combine": "k_rows_combine", "dinv": "k_dinv", "pcg": "k_pcg_resident",
                    "update": "k_update", "tiles": "k_tiles", "tiles_combine": "k_tiles_combine"}
 

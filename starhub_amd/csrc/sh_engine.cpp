@@ -227,8 +227,9 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         A(sort_temp_, sort_temp_bytes_);
     }
     // batch-wide launch state
-    A(d_descs_, size_t(max_slots) * sizeof(SolveDesc));
-    A(d_wmaps_, size_t(kNumMaps) * (max_slots + 1) * 4);
+    // the launch's descriptors followed by its work maps: one upload per launch
+    A(d_descs_, size_t(max_slots) * sizeof(SolveDesc) + size_t(kNumMaps) * (max_slots + 1) * 4);
+    d_wmaps_ = static_cast<char *>(d_descs_) + size_t(max_slots) * sizeof(SolveDesc);
     const size_t pcg_wg = size_t(max_slots) * div_up(ncap_, kPcgNodesPerWg);
     A(d_part_rs_, pcg_wg * 8);
     A(d_part_dq_, pcg_wg * 8);
@@ -866,8 +867,9 @@ int Engine::upload(uint32_t ns, hipStream_t st)
         std::memcpy(h + size_t(s) * sizeof(SolveDesc), &slots_[batch_[s]].d, sizeof(SolveDesc));
     const size_t mb = maps_.size() * 4;
     std::memcpy(h + size_t(ns) * sizeof(SolveDesc), maps_.data(), mb);
-    SH_HIP(hipMemcpyAsync(d_descs_, h, size_t(ns) * sizeof(SolveDesc), hipMemcpyHostToDevice, st));
-    SH_HIP(hipMemcpyAsync(d_wmaps_, h + size_t(ns) * sizeof(SolveDesc), mb, hipMemcpyHostToDevice, st));
+    // the maps follow the ns descriptors directly (u32 entries, 8-byte aligned)
+    d_wmaps_ = static_cast<char *>(d_descs_) + size_t(ns) * sizeof(SolveDesc);
+    SH_HIP(hipMemcpyAsync(d_descs_, h, size_t(ns) * sizeof(SolveDesc) + mb, hipMemcpyHostToDevice, st));
     return SH_OK;
 }
 
@@ -932,9 +934,8 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
     {
         const int parity = int(it); // iteration offset: DQ buffer (gn_iter + it) & 1, cost slot gn_iter + it
         SH_HIP(mark(kPhDense, st));
-        SH_HIP(launch_dense_terms(K_, fp16, sds, map(kMapPixel, ns), n, map_total_[kMapPixel], parity, st));
-        SH_HIP(mark(kPhReg, st));
-        SH_HIP(launch_reg_terms(sds, map(kMapReg, ns), n, map_total_[kMapReg], parity, st));
+        SH_HIP(launch_terms(K_, fp16, sds, map(kMapPixel, ns), map_total_[kMapPixel], map(kMapReg, ns),
+                            map_total_[kMapReg], n, parity, st));
         if (cfg_.flags & SH_FLAG_PAIR_MAJOR)
         {
             SH_HIP(mark(kPhDiagRhs, st));
@@ -1163,8 +1164,8 @@ int Engine::solve(uint32_t ns, hipStream_t st)
             // cost at the final warp field: the terms once more, then the reduction
             const SolveDesc *sds = static_cast<const SolveDesc *>(d_descs_);
             const int it = int(cfg_.gn_iterations);
-            SH_HIP(launch_dense_terms(K_, fp16_, sds, map(kMapPixel, ns), int(ns), map_total_[kMapPixel], it, st, 0));
-            SH_HIP(launch_reg_terms(sds, map(kMapReg, ns), int(ns), map_total_[kMapReg], it, st, 0));
+            SH_HIP(launch_terms(K_, fp16_, sds, map(kMapPixel, ns), map_total_[kMapPixel], map(kMapReg, ns),
+                                map_total_[kMapReg], int(ns), it, st, 0));
             SH_HIP(launch_cost_final(sds, int(ns), it, st));
         }
         return finish(st);

@@ -519,33 +519,41 @@ __device__ __forceinline__ double dense_pixel(const SolveDesc &d, uint32_t p, in
 
 // Per-pixel residual / Jacobian (computeDenseImageJacobianKernel,
 // DenseImageJacobianResidual.cu:164-310; forms :12-157).
+// Work-group wg of the pixel map.
 template <int K, class JT>
-__global__ __launch_bounds__(256) void k_dense_terms(const SolveDesc *__restrict__ sds,
-                                                     const uint32_t *__restrict__ wg_begin, int n_solve, int parity,
-                                                     int store)
+__device__ __forceinline__ void dense_terms_wg(const SolveDesc *__restrict__ sds, const uint32_t *__restrict__ wg_begin,
+                                               int n_solve, uint32_t wg, int parity, int store)
 {
-    const int si = find_solve(wg_begin, n_solve, blockIdx.x);
+    const int si = find_solve(wg_begin, n_solve, wg);
     const SolveDesc d = global_desc(sds[si]);
-    const uint32_t p = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
+    const uint32_t p = (wg - wg_begin[si]) * blockDim.x + threadIdx.x;
     double pix_cost = 0.0;
     if (p < uint32_t(d.P))
         pix_cost = dense_pixel<K, JT>(d, p, parity, store);
     // FP64 cost partial of the work-group (fixed order)
     const double wc = block_sum_256(pix_cost);
     if (threadIdx.x == 0)
-        d.cost_part[blockIdx.x - wg_begin[si]] = wc;
+        d.cost_part[wg - wg_begin[si]] = wc;
+}
+
+template <int K, class JT>
+__global__ __launch_bounds__(256) void k_dense_terms(const SolveDesc *__restrict__ sds,
+                                                     const uint32_t *__restrict__ wg_begin, int n_solve, int parity,
+                                                     int store)
+{
+    dense_terms_wg<K, JT>(sds, wg_begin, n_solve, blockIdx.x, parity, store);
 }
 
 
 // ARAP forward warp per edge + node translations (forwardWarpSmootherNodeKernel,
 // NodeGraphHandler.cu:7-46; ComputeNodeMotionKernel, NodeMotionHandler.cu:9-21).
-__global__ __launch_bounds__(256) void k_reg_terms(const SolveDesc *__restrict__ sds,
-                                                   const uint32_t *__restrict__ wg_begin, int n_solve, int parity,
-                                                   int store)
+// Work-group wg of the edge map.
+__device__ __forceinline__ void reg_terms_wg(const SolveDesc *__restrict__ sds, const uint32_t *__restrict__ wg_begin,
+                                             int n_solve, uint32_t wg, int parity, int store)
 {
-    const int si = find_solve(wg_begin, n_solve, blockIdx.x);
+    const int si = find_solve(wg_begin, n_solve, wg);
     const SolveDesc d = global_desc(sds[si]);
-    const uint32_t item = (blockIdx.x - wg_begin[si]) * blockDim.x + threadIdx.x;
+    const uint32_t item = (wg - wg_begin[si]) * blockDim.x + threadIdx.x;
     const dq8 *__restrict__ dq = pick2(d.dq, (d.gn_iter + parity) & 1);
     double edge_cost = 0.0;
     if (item < uint32_t(d.E))
@@ -582,7 +590,27 @@ __global__ __launch_bounds__(256) void k_reg_terms(const SolveDesc *__restrict__
     }
     const double wc = block_sum_256(edge_cost);
     if (threadIdx.x == 0)
-        d.cost_part[(uint32_t(d.P) + 255u) / 256u + (blockIdx.x - wg_begin[si])] = wc;
+        d.cost_part[(uint32_t(d.P) + 255u) / 256u + (wg - wg_begin[si])] = wc;
+}
+
+__global__ __launch_bounds__(256) void k_reg_terms(const SolveDesc *__restrict__ sds,
+                                                   const uint32_t *__restrict__ wg_begin, int n_solve, int parity,
+                                                   int store)
+{
+    reg_terms_wg(sds, wg_begin, n_solve, blockIdx.x, parity, store);
+}
+
+// Both term kernels as one launch: the pixel map's work-groups, then the edge
+// map's (independent work, so the two overlap on the device).
+template <int K, class JT>
+__global__ __launch_bounds__(256) void k_terms(const SolveDesc *__restrict__ sds, const uint32_t *__restrict__ pix_map,
+                                               uint32_t n_pix_wg, const uint32_t *__restrict__ reg_map, int n_solve,
+                                               int parity, int store)
+{
+    if (blockIdx.x < n_pix_wg) // uniform over the group
+        dense_terms_wg<K, JT>(sds, pix_map, n_solve, blockIdx.x, parity, store);
+    else
+        reg_terms_wg(sds, reg_map, n_solve, blockIdx.x - n_pix_wg, parity, store);
 }
 
 // ARAP Jacobian rows of one side, scaled by c (node_graph_reg_jacobian.cuh:250-275).
@@ -1505,6 +1533,33 @@ hipError_t launch_dense_terms(int K, bool fp16, const SolveDesc *sds, const uint
             hipLaunchKernelGGL((k_dense_terms<4, __half>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity, store);
         else
             hipLaunchKernelGGL((k_dense_terms<4, float>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity, store);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_terms(int K, bool fp16, const SolveDesc *sds, const uint32_t *pix_map, uint32_t n_pix_wg,
+                        const uint32_t *reg_map, uint32_t n_reg_wg, int n_solve, int parity, hipStream_t st, int store)
+{
+    const dim3 grid(n_pix_wg + n_reg_wg);
+    if (grid.x == 0)
+        return hipSuccess;
+    if (K == 8)
+    {
+        if (fp16)
+            hipLaunchKernelGGL((k_terms<8, __half>), grid, dim3(256), 0, st, sds, pix_map, n_pix_wg, reg_map, n_solve,
+                               parity, store);
+        else
+            hipLaunchKernelGGL((k_terms<8, float>), grid, dim3(256), 0, st, sds, pix_map, n_pix_wg, reg_map, n_solve,
+                               parity, store);
+    }
+    else
+    {
+        if (fp16)
+            hipLaunchKernelGGL((k_terms<4, __half>), grid, dim3(256), 0, st, sds, pix_map, n_pix_wg, reg_map, n_solve,
+                               parity, store);
+        else
+            hipLaunchKernelGGL((k_terms<4, float>), grid, dim3(256), 0, st, sds, pix_map, n_pix_wg, reg_map, n_solve,
+                               parity, store);
     }
     return hipGetLastError();
 }

@@ -53,6 +53,10 @@ hipError_t launch_dense_terms(int K, bool fp16, const SolveDesc *sds, const uint
                               uint32_t n_wg, int parity, hipStream_t st, int store = 1);
 hipError_t launch_reg_terms(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int parity,
                             hipStream_t st, int store = 1);
+// launch_dense_terms and launch_reg_terms as one launch
+hipError_t launch_terms(int K, bool fp16, const SolveDesc *sds, const uint32_t *pix_map, uint32_t n_pix_wg,
+                        const uint32_t *reg_map, uint32_t n_reg_wg, int n_solve, int parity, hipStream_t st,
+                        int store = 1);
 hipError_t launch_diag_rhs(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
                            hipStream_t st);
 struct PcgSlot2;
