@@ -454,6 +454,7 @@ typedef struct sh_tuning
 int sh_solver_autotune(sh_solver *solver, sh_stream_t stream, uint32_t repeats, sh_tuning *best);
 int sh_solver_set_tuning(sh_solver *solver, const sh_tuning *tuning);
 int sh_batch_autotune(sh_batch *batch, uint32_t num_solves, sh_stream_t stream, uint32_t repeats, sh_tuning *best);
+int sh_batch_set_tuning(sh_batch *batch, const sh_tuning *tuning);
 /* The tuning in use (ms = 0). */
 int sh_solver_get_tuning(const sh_solver *solver, sh_tuning *tuning);
 

@@ -1249,7 +1249,7 @@ int Engine::autotune(uint32_t ns, hipStream_t st, uint32_t repeats, sh_tuning *b
     std::vector<sh_tuning> cands;
     for (uint32_t rows : {85u, 64u, 48u, 32u})
         cands.push_back({0u, SH_SPMV_MATERIALIZED, rows, 0.0});
-    for (uint32_t rows : {85u, 64u, 48u, 32u})
+    for (uint32_t rows : {85u, 64u, 48u, 32u, 24u, 16u})
         cands.push_back({SH_FLAG_PIPELINED_PCG, SH_SPMV_MATERIALIZED, rows, 0.0});
     cands.push_back({SH_FLAG_STREAMING_PCG, SH_SPMV_MATERIALIZED, kPcgResRows, 0.0});
     cands.push_back({SH_FLAG_NO_PERSISTENT_PCG, SH_SPMV_MATERIALIZED, kPcgResRows, 0.0});
@@ -1650,6 +1650,13 @@ extern "C"
             return SH_E_ARG;
         *count = solver->eng.pcg_fallbacks();
         return SH_OK;
+    }
+
+    int sh_batch_set_tuning(sh_batch *batch, const sh_tuning *tuning)
+    {
+        if (!batch || !tuning)
+            return SH_E_ARG;
+        return batch->eng.set_tuning(*tuning);
     }
 
     int sh_batch_autotune(sh_batch *batch, uint32_t num_solves, sh_stream_t stream, uint32_t repeats,

@@ -346,6 +346,12 @@ def set_tuning(solver, flags: int, spmv_mode: int, pcg_rows_per_group: int) -> N
     check(lib().sh_solver_set_tuning(solver._h, ctypes.byref(t)), "sh_solver_set_tuning")
 
 
+def set_batch_tuning(batch, flags: int, spmv_mode: int, pcg_rows_per_group: int) -> None:
+    """Applies a tuning to a BatchSolver without timing anything (sh_batch_set_tuning)."""
+    t = Tuning(flags, spmv_mode, pcg_rows_per_group, 0.0)
+    check(lib().sh_batch_set_tuning(batch._h, ctypes.byref(t)), "sh_batch_set_tuning")
+
+
 def get_tuning(solver) -> dict:
     """The tuning a WarpSolver uses now (sh_solver_get_tuning)."""
     t = Tuning()

@@ -352,6 +352,38 @@ def test_batch_equals_solo(torch_dev):
         assert err <= tolerance(ref)
 
 
+def test_pipelined_batch_subsets_equal_solo(torch_dev):
+    """The pipelined PCG's tagged publications: a batch solved with 4, then
+    2, then 4 members (slots idle for a launch keep stale publications) gives
+    every member the bits of its solo solve on the same kernel, and no solve
+    falls back to the split PCG."""
+    from starhub_amd import native, scene, workload
+    cfgs = [scene.config_batch_member(n, 910 + i) for i, n in enumerate([200, 500, 160, 320])]
+    scs = [scene.generate(c) for c in cfgs]
+    frames = [workload.device_frame(s)[0] for s in scs]
+    W = max(c.width for c in cfgs)
+    H = max(c.height for c in cfgs)
+    N = max(c.num_nodes for c in cfgs)
+    bcfg = native.make_config(cfgs[0].solver, W, H, N, 8)
+    flags, rows = native.SH_FLAG_PIPELINED_PCG, 24
+    solo = []
+    for f in frames:
+        s = native.WarpSolver(bcfg)
+        native.set_tuning(s, flags, 0, rows)
+        s.process_frame(f)
+        assert native.pcg_fallbacks(s) == 0
+        solo.append(s.node_se3().cpu().numpy())
+    batch = native.BatchSolver(bcfg, len(cfgs))
+    native.set_batch_tuning(batch, flags, 0, rows)
+    for i, f in enumerate(frames):
+        batch.set_inputs(i, f)
+    for n in (4, 2, 4, 2):
+        batch.solve(n)
+        say("pipelined batch of", n)
+        for i in range(n):
+            assert np.array_equal(batch.node_se3(i).cpu().numpy(), solo[i]), (n, i)
+
+
 def test_empty_frame_gives_identity(torch_dev):
     from starhub_amd import scene, workload
     sc = scene.generate(scene.config_c1())
