@@ -69,6 +69,7 @@ struct SolveDesc
     uint32_t *tc_slot_first; // S+1: first contribution of each slot
     uint32_t *list_off;      // N+2U+1: first entry of each block in list_items
     uint32_t *list_items;    // S: slot ids sorted by block (stable: tile order)
+    uint32_t *slot_pos;      // S: each slot's position in list_items (where k_tiles puts its partial)
     uint32_t *blk_transpose; // N+2U: the index of each block's transpose (block (j, i) for (i, j))
     float *tpart;            // S * 32 slot partials (21 H, 6 g, pad; 128 B each)
     // GN iteration state
@@ -128,7 +129,7 @@ __device__ __forceinline__ SolveDesc global_desc(const SolveDesc &src)
     SH_G(node_connect); SH_G(node_pred); SH_G(pix); SH_G(pknn); SH_G(pw); SH_G(n2t_off); SH_G(n2t_terms);
     SH_G(half_keys); SH_G(half_off); SH_G(pair_entries); SH_G(upper_pos); SH_G(lower_pos); SH_G(row_ptr);
     SH_G(blk_col); SH_G(dense_cnt); SH_G(seg_ptr); SH_G(part); SH_G(tile_cbase); SH_G(tile_sbase);
-    SH_G(tc_contrib); SH_G(tc_slot_first); SH_G(list_off); SH_G(list_items); SH_G(blk_transpose); SH_G(tpart);
+    SH_G(tc_contrib); SH_G(tc_slot_first); SH_G(list_off); SH_G(list_items); SH_G(slot_pos); SH_G(blk_transpose); SH_G(tpart);
     SH_G(dq[0]); SH_G(dq[1]); SH_G(jr); SH_G(reg); SH_G(ttrans); SH_G(blk); SH_G(dinv); SH_G(rhs);
     SH_G(x); SH_G(r); SH_G(s); SH_G(q); SH_G(dv[0]); SH_G(dv[1]); SH_G(mv[0]); SH_G(mv[1]);
     SH_G(delta); SH_G(cost); SH_G(cost_part); SH_G(mf_z); SH_G(mf_reg);

@@ -725,6 +725,7 @@ int Engine::build_tiles(Slot &sl, IndexLane &ln, hipStream_t st)
     size_t tb = sort_temp_bytes_;
     if (S > 0)
         SH_HIP(sort_pairs_u32_u32(ln.sort_temp, tb, k32a, k32b, p32a, d.list_items, S, end_bit, st));
+    SH_HIP(launch_slot_pos(d.list_items, S, d.slot_pos, st));
     SH_HIP(launch_lower_bound_u32(k32b, S, nb + 1, d.list_off, st));
     SH_HIP(launch_block_transpose(d, st));
     return SH_OK;
@@ -739,7 +740,7 @@ int Engine::size_tiles(Slot &sl, uint32_t n_slots, uint32_t n_contrib)
     const size_t b_first = up((size_t(n_slots) + 1) * 4);
     const size_t b_items = up(size_t(n_slots) * 4 + 4);
     const size_t b_part = up(size_t(n_slots) * kPartStride * 4 + 16); // 128-byte partials, 256-aligned base
-    const size_t need = b_contrib + b_first + b_items + b_part;
+    const size_t need = b_contrib + b_first + 2 * b_items + b_part;
     if (need > sl.tile_bytes)
     {
         if (sl.tile_buf)
@@ -761,6 +762,7 @@ int Engine::size_tiles(Slot &sl, uint32_t n_slots, uint32_t n_contrib)
     d.tc_slot_first = reinterpret_cast<uint32_t *>(b + b_contrib);
     d.list_items = reinterpret_cast<uint32_t *>(b + b_contrib + b_first);
     d.tpart = reinterpret_cast<float *>(b + b_contrib + b_first + b_items);
+    d.slot_pos = reinterpret_cast<uint32_t *>(b + b_contrib + b_first + b_items + b_part);
     return SH_OK;
 }
 

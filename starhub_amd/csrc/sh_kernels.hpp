@@ -194,6 +194,8 @@ hipError_t launch_lower_bound_u32(const uint32_t *keys, uint32_t n_keys, uint32_
 hipError_t launch_tiles(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
                         uint32_t slot_cap, hipStream_t st);
 size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len);
+// slot_pos[list_items[l]] = l for l < n: the list position of every slot
+hipError_t launch_slot_pos(const uint32_t *list_items, uint32_t n, uint32_t *slot_pos, hipStream_t st);
 // matrix-free J^T W J x (x = d_{k+1} of each solve unless x_user; y = d.q unless y_user)
 hipError_t launch_mf_tiles(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
                            int k_it, const float *x_user, hipStream_t st);

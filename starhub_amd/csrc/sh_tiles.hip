@@ -405,10 +405,13 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
         perm[atomicAdd(&hist[kTilePix - (sst[s + 1] - sst[s])], 1u)] = uint16_t(s);
     __syncthreads();
 
-    float4 *__restrict__ out = reinterpret_cast<float4 *>(d.tpart) + size_t(s0) * (kPartStride / 4);
+    // each slot's partial goes to its list position, so a block's partials
+    // are contiguous for the combine
+    float4 *__restrict__ out = reinterpret_cast<float4 *>(d.tpart);
     for (uint32_t task = tid; task < ns * (kTileRec / 4); task += kTileThreads)
     {
         const uint32_t s = perm[task / (kTileRec / 4)], g = task % (kTileRec / 4);
+        const uint32_t dst = d.slot_pos[s0 + s];
         float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
         const uint32_t e1 = sst[s + 1];
         for (uint32_t e = sst[s]; e < e1; e += kTileBatch)
@@ -434,7 +437,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
                 acc.w = on ? fmaf(coef[j], h[j].w, acc.w) : acc.w;
             }
         }
-        out[size_t(s) * (kPartStride / 4) + g] = acc;
+        out[size_t(dst) * (kPartStride / 4) + g] = acc;
     }
 }
 
@@ -588,19 +591,15 @@ constexpr uint32_t kChunkCap = 128;
 constexpr uint32_t kCombUnionF4 = kChunkCap * (kTileRec / 4); // float4s of the shared region
 static_assert(sizeof(RegTerm) * kTileThreads <= kCombUnionF4 * sizeof(float4), "combine LDS union");
 
-// Sum of float4 group g over list entries [l0, l1), at most kChunk of them, in
-// list order: the entries, then their partials, each loaded in one round.
-__device__ __forceinline__ float4 sum_chunk4(const SolveDesc &d, const float4 *__restrict__ part, uint32_t l0,
-                                             uint32_t l1, uint32_t g)
+// Sum of float4 group g over list positions [l0, l1), at most kChunk of them,
+// in list order; k_tiles stored each slot's partial at its list position, so
+// they load in one round.
+__device__ __forceinline__ float4 sum_chunk4(const float4 *__restrict__ part, uint32_t l0, uint32_t l1, uint32_t g)
 {
-    uint32_t it[kChunk];
-#pragma unroll
-    for (uint32_t j = 0; j < kChunk; ++j)
-        it[j] = (l0 + j < l1) ? d.list_items[l0 + j] : 0u;
     float4 v[kChunk];
 #pragma unroll
     for (uint32_t j = 0; j < kChunk; ++j)
-        v[j] = (l0 + j < l1) ? part[size_t(it[j]) * (kPartStride / 4) + g] : make_float4(0.f, 0.f, 0.f, 0.f);
+        v[j] = (l0 + j < l1) ? part[size_t(l0 + j) * (kPartStride / 4) + g] : make_float4(0.f, 0.f, 0.f, 0.f);
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (uint32_t j = 0; j < kChunk; ++j)
@@ -808,7 +807,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc 
                     hi = m;
             }
             const uint32_t l0 = ls[lo] + (k - cc[lo]) * kChunk;
-            csum[task] = sum_chunk4(d, part, l0, min(l0 + kChunk, le[lo]), g);
+            csum[task] = sum_chunk4(part, l0, min(l0 + kChunk, le[lo]), g);
         }
         __syncthreads();
         for (uint32_t task = tid; task < nq * kG; task += kTileThreads)
@@ -1246,6 +1245,21 @@ size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len)
     const size_t L1 = size_t(max_row_len) + 1;
     return sizeof(float4) * (kCombUnionF4 + L1 * (kTileRec / 4)) + sizeof(float) * (L1 * 36 + 16) +
            sizeof(uint32_t) * (3 * L1 + 1) + sizeof(uint16_t) * ((max_nodes + 1) & ~1u);
+}
+
+__global__ void k_slot_pos(const uint32_t *__restrict__ list_items, uint32_t n, uint32_t *__restrict__ slot_pos)
+{
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n)
+        slot_pos[list_items[l]] = l;
+}
+
+hipError_t launch_slot_pos(const uint32_t *list_items, uint32_t n, uint32_t *slot_pos, hipStream_t st)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_slot_pos, dim3((n + 255) / 256), dim3(256), 0, st, list_items, n, slot_pos);
+    return hipGetLastError();
 }
 
 hipError_t launch_tiles_combine(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, size_t lds,
