@@ -326,7 +326,7 @@ __global__ void k_lower_bound_u32(const uint32_t *__restrict__ keys, uint32_t n_
 // of kTileBatch, the last one predicated, so the dependent LDS reads of a
 // batch overlap.  The slot lists take LDS for the launch's largest tile only,
 // which lets five tiles share a CU.
-constexpr int kTileBatch = 8;
+constexpr int kTileBatch = 2; // measured on C2: 2 -> 37.0 us, 4 -> 38.1, 8 -> 42.7, 16 -> 48.8 per launch
 
 template <int K, class JT>
 __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restrict__ sds,
