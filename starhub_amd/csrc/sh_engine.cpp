@@ -914,8 +914,16 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
     if (pipe_max_groups_ < 0)
         pipe_max_groups_ = pcg_pipe_max_groups();
     const bool pipelined = (cfg_.flags & SH_FLAG_PIPELINED_PCG) != 0;
+    // more groups than can be co-resident: the pipelined PCG runs in rounds
+    // of consecutive solves whose groups fit together
+    bool pipe_rounds = coop_resident_ && pipelined && pipe_max_groups_ > 0 &&
+                       map_total_[kMapCoop] > uint32_t(pipe_max_groups_);
+    for (uint32_t s = 0; pipe_rounds && s < ns; ++s)
+        if (coop_begin[s + 1] - coop_begin[s] > uint32_t(pipe_max_groups_))
+            pipe_rounds = false;
     const bool resident =
-        coop_resident_ && map_total_[kMapCoop] <= uint32_t(pipelined ? pipe_max_groups_ : res_max_groups_);
+        coop_resident_ &&
+        (map_total_[kMapCoop] <= uint32_t(pipelined ? pipe_max_groups_ : res_max_groups_) || pipe_rounds);
     // the matrix-free operator reads the tile index: not with the other materialisations
     const bool matrix_free = cfg_.spmv_mode == SH_SPMV_MATRIX_FREE;
     if (matrix_free && (cfg_.flags & (SH_FLAG_PAIR_MAJOR | SH_FLAG_ROW_SEGMENTS)))
@@ -970,9 +978,17 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
         if (use_coop)
         {
             if (resident && pipelined)
-                SH_HIP(launch_pcg_pipe(sds, map(kMapCoop, ns), n, map_total_[kMapCoop], int(cfg_.pcg_iterations),
-                                       static_cast<PcgSlot2 *>(d_slots2_), flags + 0, flags + 1, wait_limit_,
-                                       parity, st));
+                for (uint32_t s0 = 0; s0 < ns;)
+                {
+                    uint32_t s1 = s0 + 1;
+                    while (s1 < ns && coop_begin[s1 + 1] - coop_begin[s0] <= uint32_t(pipe_max_groups_))
+                        ++s1;
+                    SH_HIP(launch_pcg_pipe(sds + s0, map(kMapCoop, ns) + s0, int(s1 - s0),
+                                           coop_begin[s1] - coop_begin[s0], int(cfg_.pcg_iterations),
+                                           static_cast<PcgSlot2 *>(d_slots2_), flags + 0, flags + 1, wait_limit_,
+                                           parity, st));
+                    s0 = s1;
+                }
             else if (resident)
                 SH_HIP(launch_pcg_resident(sds, map(kMapCoop, ns), n, map_total_[kMapCoop],
                                            int(cfg_.pcg_iterations), static_cast<PcgSlot *>(d_slots_), flags + 0,

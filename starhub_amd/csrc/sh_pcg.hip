@@ -560,13 +560,13 @@ __device__ __forceinline__ double await_pub(const double *p, double v, uint64_t 
 // to its slot, and wave 0 reads every group's in slot order and sums them in a
 // fixed tree, so every group gets the same totals, bit for bit; thread 0
 // leaves them in out[0..1] (read after the caller's next group sync).
-__device__ void publish_sum2(PipeGroupSum part, PcgSlot2 *__restrict__ slots, uint32_t b0, uint32_t G, uint32_t ep,
-                             uint32_t *__restrict__ err, uint32_t cap, double *out)
+__device__ void publish_sum2(PipeGroupSum part, PcgSlot2 *__restrict__ slots, uint32_t self, uint32_t b0, uint32_t G,
+                             uint32_t ep, uint32_t *__restrict__ err, uint32_t cap, double *out)
 {
     const uint64_t t = pub_tag(ep);
     if (threadIdx.x == 0)
     {
-        PcgSlot2 &me = slots[blockIdx.x];
+        PcgSlot2 &me = slots[self];
         store_pub(&me.v[ep & 1u][0], with_tag(part.a, t));
         store_pub(&me.v[ep & 1u][1], with_tag(part.b, t));
     }
@@ -610,9 +610,12 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
     __shared__ uint32_t cbits[kBitmapWords];
     __shared__ uint32_t cpre[kBitmapWords];
     __shared__ uint32_t ncols_sh;
-    const int si = find_solve_p(wg_begin, n_solve, blockIdx.x);
+    // a launch may cover a subset of the solves (wg_begin points into the
+    // whole batch's map): its groups are numbered from wg_begin[0]
+    const uint32_t bid = blockIdx.x + wg_begin[0];
+    const int si = find_solve_p(wg_begin, n_solve, bid);
     const SolveDesc d = global_desc(sds[si]);
-    const uint32_t b0 = wg_begin[si], G = wg_begin[si + 1] - b0, g = blockIdx.x - b0;
+    const uint32_t b0 = wg_begin[si], G = wg_begin[si + 1] - b0, g = bid - b0;
     const uint32_t N = uint32_t(d.N);
     const uint32_t r0 = uint32_t(uint64_t(g) * N / G), r1 = uint32_t(uint64_t(g + 1) * N / G);
     const uint32_t B0 = d.row_ptr[r0] + r0, B1 = d.row_ptr[r1] + r1;
@@ -838,8 +841,8 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
         store_pub(pick2(d.mv, ep & 1u) + i6, u);
     if (tid == 0) // the slot entry this number selects is written too (no sums yet)
     {
-        store_pub(&slots[blockIdx.x].v[ep & 1u][0], with_tag(0.0, pub_tag(ep)));
-        store_pub(&slots[blockIdx.x].v[ep & 1u][1], with_tag(0.0, pub_tag(ep)));
+        store_pub(&slots[bid].v[ep & 1u][0], with_tag(0.0, pub_tag(ep)));
+        store_pub(&slots[bid].v[ep & 1u][1], with_tag(0.0, pub_tag(ep)));
     }
     double w = apply(pick2(d.mv, ep & 1u), pub_tag(ep));
     double x = 0.0, z = 0.0, q = 0.0, s = 0.0, p = 0.0, gprev = 0.0, aprev = 0.0;
@@ -853,7 +856,7 @@ __global__ __launch_bounds__(kResThreads) void k_pcg_pipe(const SolveDesc *__res
         m = with_tag(m, t);
         if (own)
             store_pub(mb + i6, m);
-        publish_sum2(gs, slots, b0, G, ep, err, cap, sh2);
+        publish_sum2(gs, slots, bid, b0, G, ep, err, cap, sh2);
         const double n = apply(mb, t); // its group syncs order sh2 for every thread
         const PipeGroupSum gd{sh2[0], sh2[1]};
         const double gam = gd.a, del = gd.b;

@@ -384,6 +384,33 @@ def test_pipelined_batch_subsets_equal_solo(torch_dev):
             assert np.array_equal(batch.node_se3(i).cpu().numpy(), solo[i]), (n, i)
 
 
+def test_pipelined_batch_in_rounds_equals_solo(torch_dev):
+    """A batch with more pipelined-PCG groups than can be co-resident (ten
+    solves of 1,000 nodes at 16 rows per group, 630 groups) runs the PCG in
+    rounds of consecutive solves; every member still gets its solo bits and no
+    solve falls back."""
+    from starhub_amd import native, scene, workload
+    cfgs = [scene.config_batch_member(1000, 930 + i) for i in range(10)]
+    scs = [scene.generate(c) for c in cfgs]
+    frames = [workload.device_frame(s)[0] for s in scs]
+    W = max(c.width for c in cfgs)
+    H = max(c.height for c in cfgs)
+    bcfg = native.make_config(cfgs[0].solver, W, H, 1000, 8)
+    flags, rows = native.SH_FLAG_PIPELINED_PCG, 16
+    batch = native.BatchSolver(bcfg, len(cfgs))
+    native.set_batch_tuning(batch, flags, 0, rows)
+    for i, f in enumerate(frames):
+        batch.set_inputs(i, f)
+    batch.solve(len(cfgs))
+    say("pipelined batch of", len(cfgs), "solved")
+    for i, f in enumerate(frames):
+        s = native.WarpSolver(bcfg)
+        native.set_tuning(s, flags, 0, rows)
+        s.process_frame(f)
+        assert native.pcg_fallbacks(s) == 0
+        assert np.array_equal(batch.node_se3(i).cpu().numpy(), s.node_se3().cpu().numpy()), i
+
+
 def test_empty_frame_gives_identity(torch_dev):
     from starhub_amd import scene, workload
     sc = scene.generate(scene.config_c1())
