@@ -114,7 +114,7 @@ struct PcgSlot
 // Blocks of the normal matrix per persistent-PCG work-group (sets the
 // number of groups per solve, at most kPcgMaxGroups).
 constexpr uint32_t kPcgBlocksPerGroup = 1024;
-constexpr uint32_t kPcgMaxGroups = 64;
+constexpr uint32_t kPcgMaxGroups = 256;
 hipError_t launch_pcg_coop(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int iters,
                            PcgSlot *slots, uint32_t *epoch_base, uint32_t *err, uint32_t wait_limit, uint32_t tile,
                            hipStream_t st);

@@ -572,13 +572,26 @@ __device__ void publish_sum2(PipeGroupSum part, PcgSlot2 *__restrict__ slots, ui
     }
     if (threadIdx.x < 64)
     {
-        double va = 0.0, vb = 0.0;
-        for (uint32_t i = threadIdx.x; i < G; i += 64)
+        // up to kPcgMaxGroups / 64 slots per lane, all read before any is awaited
+        constexpr uint32_t kPer = kPcgMaxGroups / 64;
+        double a[kPer], b[kPer];
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j)
         {
-            const double *pa = &slots[b0 + i].v[ep & 1u][0], *pb = &slots[b0 + i].v[ep & 1u][1];
-            const double a = load_pub(pa), b = load_pub(pb);
-            va += await_pub(pa, a, t, err, cap);
-            vb += await_pub(pb, b, t, err, cap);
+            const uint32_t i = threadIdx.x + 64u * j;
+            a[j] = i < G ? load_pub(&slots[b0 + i].v[ep & 1u][0]) : 0.0;
+            b[j] = i < G ? load_pub(&slots[b0 + i].v[ep & 1u][1]) : 0.0;
+        }
+        double va = 0.0, vb = 0.0;
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j)
+        {
+            const uint32_t i = threadIdx.x + 64u * j;
+            if (i < G)
+            {
+                va += await_pub(&slots[b0 + i].v[ep & 1u][0], a[j], t, err, cap);
+                vb += await_pub(&slots[b0 + i].v[ep & 1u][1], b[j], t, err, cap);
+            }
         }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1)
