@@ -572,25 +572,18 @@ __device__ void publish_sum2(PipeGroupSum part, PcgSlot2 *__restrict__ slots, ui
     }
     if (threadIdx.x < 64)
     {
-        // up to kPcgMaxGroups / 64 slots per lane, all read before any is awaited
-        constexpr uint32_t kPer = kPcgMaxGroups / 64;
-        double a[kPer], b[kPer];
-#pragma unroll
-        for (uint32_t j = 0; j < kPer; ++j)
-        {
-            const uint32_t i = threadIdx.x + 64u * j;
-            a[j] = i < G ? load_pub(&slots[b0 + i].v[ep & 1u][0]) : 0.0;
-            b[j] = i < G ? load_pub(&slots[b0 + i].v[ep & 1u][1]) : 0.0;
-        }
+        // one slot per lane, in rounds of 64 when the solve has more groups
+        // (registers for one round only: the loop below is uniform)
         double va = 0.0, vb = 0.0;
-#pragma unroll
-        for (uint32_t j = 0; j < kPer; ++j)
+        for (uint32_t c = 0; c < G; c += 64u)
         {
-            const uint32_t i = threadIdx.x + 64u * j;
+            const uint32_t i = c + threadIdx.x;
             if (i < G)
             {
-                va += await_pub(&slots[b0 + i].v[ep & 1u][0], a[j], t, err, cap);
-                vb += await_pub(&slots[b0 + i].v[ep & 1u][1], b[j], t, err, cap);
+                const double *pa = &slots[b0 + i].v[ep & 1u][0];
+                const double a = load_pub(pa), b = load_pub(pa + 1);
+                va += await_pub(pa, a, t, err, cap);
+                vb += await_pub(pa + 1, b, t, err, cap);
             }
         }
 #pragma unroll

@@ -411,6 +411,31 @@ def test_pipelined_batch_in_rounds_equals_solo(torch_dev):
         assert np.array_equal(batch.node_se3(i).cpu().numpy(), s.node_se3().cpu().numpy()), i
 
 
+def test_pipelined_pcg_many_groups_vs_fp64(torch_dev):
+    """A 4,096-node solve at 24 rows per group: 171 pipelined-PCG groups, so
+    each lane of the sum gather reads three rounds of slots (the last one
+    partial).  The warp field is within the FP64 oracle's tolerance, and no
+    wait fell back."""
+    import sho
+    from starhub_amd import native, scene, workload
+    cfg = scene.config_batch_member(4096, 977)
+    sc = scene.generate(cfg)
+    frame, prod = workload.device_frame(sc)
+    s = native.WarpSolver(native.make_config(cfg.solver, cfg.width, cfg.height, cfg.num_nodes, cfg.K,
+                                             flags=native.SH_FLAG_PHASE_TIMING))
+    native.set_tuning(s, native.SH_FLAG_PIPELINED_PCG, 0, 24)
+    s.process_frame(frame)
+    say("4096-node pipelined solve done")
+    assert native.pcg_fallbacks(s) == 0
+    ph = s.phase_times()
+    assert ph["pcg"][1] == cfg.solver.gn_iterations and ph["update"][1] == 0  # the fused pipelined kernel ran
+    gpu = workload.device_dq_to_wxyz(s.node_se3())
+    ref = sho.solve(workload.host_inputs(sc, prod), cfg.solver, 64, sho.max_threads())
+    err = workload.warp_error(gpu, ref["dq"].reshape(-1, 8))
+    say("4096-node pipelined warp error vs fp64", err, "straddles", straddles(ref))
+    assert err <= tolerance(ref)
+
+
 def test_empty_frame_gives_identity(torch_dev):
     from starhub_amd import scene, workload
     sc = scene.generate(scene.config_c1())
