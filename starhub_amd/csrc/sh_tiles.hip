@@ -829,8 +829,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc 
     }
 
     const size_t out0 = size_t(base) * 36;
-    for (uint32_t task = tid; task < nq * kG; task += kTileThreads)
-    {
+    // stores block q's float4 group g of sums, plus ARAP and damping, and its
+    // transpose; the diagonal block also goes to dg and its g part to red
+    auto store_task = [&](uint32_t task) {
         const uint32_t q = task / kG, g = task % kG;
         const uint32_t c = q == 0 ? 0u : c_up + q - 1;
         const size_t tr = size_t(d.blk_transpose[base + c]) * 36; // (j, i) for c > 0
@@ -867,16 +868,27 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc 
             else if (c == 0 && e < 27)
                 red[e - 21] = av[r]; // g of the diagonal slots
         }
-    }
-    __syncthreads();
-    if (tid < 6)
-        d.rhs[size_t(i) * 6 + tid] = -(red[tid] + red[8 + tid]);
-    // the FP64 inverse of the diagonal block (the same float values the block
-    // store holds) and the row's zeroed pipelined-PCG vectors
-    if (tid >= 64 && tid < 128)
+    };
+    if (tid < 64)
+    {
+        // wave 0: the diagonal block, rhs, and the FP64 inverse of the block
+        // (the same float values the block store holds), while the other
+        // waves store the off-diagonal blocks
+        if (tid < kG)
+            store_task(tid);
+        wave_lds_sync();
+        if (tid < 6)
+            d.rhs[size_t(i) * 6 + tid] = -(red[tid] + red[8 + tid]);
         invert6_fp64_wave(dg, gm, d.dinv + size_t(i) * 36);
-    if (tid >= 128 && tid < 140)
-        d.mv[(tid - 128) / 6][size_t(i) * 6 + (tid - 128) % 6] = 0.0;
+    }
+    else
+    {
+        for (uint32_t task = kG + tid - 64; task < nq * kG; task += kTileThreads - 64)
+            store_task(task);
+        // the row's zeroed pipelined-PCG vectors
+        if (tid >= 128 && tid < 140)
+            d.mv[(tid - 128) / 6][size_t(i) * 6 + (tid - 128) % 6] = 0.0;
+    }
 }
 
 // ------------------------------------------------------------ matrix-free J^T W J x
