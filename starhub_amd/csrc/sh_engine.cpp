@@ -245,7 +245,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
     // index lanes: lane 0 is the workspace above; a batch engine gets up to
     // kIndexLanes - 1 more (tile path only) so index builds overlap
     {
-        constexpr uint32_t kIndexLanes = 4;
+        constexpr uint32_t kIndexLanes = 8;
         const uint32_t n_lanes =
             (cfg_.flags & SH_FLAG_REFERENCE_INDEX) ? 1u : std::min(max_slots, kIndexLanes);
         lanes_.resize(n_lanes);
