@@ -91,6 +91,9 @@ void Engine::release()
     if (sync_ev_)
         (void)hipEventDestroy(sync_ev_);
     sync_ev_ = nullptr;
+    if (index_ev_)
+        (void)hipEventDestroy(index_ev_);
+    index_ev_ = nullptr;
     n_marks_ = 0;
 }
 
@@ -284,6 +287,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
             }
     }
     SH_HIP(hipEventCreateWithFlags(&sync_ev_, hipEventDisableTiming));
+    SH_HIP(hipEventCreateWithFlags(&index_ev_, hipEventDisableTiming));
     stage_bytes_ = size_t(max_slots) * sizeof(SolveDesc) + size_t(kNumMaps) * (max_slots + 1) * 4;
     SH_HIP(hipHostMalloc(&h_stage_, stage_bytes_, hipHostMallocDefault));
     return SH_OK;
@@ -586,9 +590,11 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
     {
         IndexLane &ln = lanes_[0];
         int rc;
-        if ((rc = index_pixels(slot, ln, st)) != SH_OK || (rc = wait_stream(st)) != SH_OK)
+        // the index half's own event: a server admits on one thread while it
+        // steps (and finishes) on another
+        if ((rc = index_pixels(slot, ln, st)) != SH_OK || (rc = wait_stream(st, index_ev_)) != SH_OK)
             return rc;
-        if ((rc = index_rows(slot, ln, st)) != SH_OK || (rc = wait_stream(st)) != SH_OK)
+        if ((rc = index_rows(slot, ln, st)) != SH_OK || (rc = wait_stream(st, index_ev_)) != SH_OK)
             return rc;
         return index_tiles(slot, ln, st);
     }
@@ -1342,14 +1348,15 @@ int Engine::cost_history(uint32_t slot, double *out, uint32_t capacity, uint32_t
     return SH_OK;
 }
 
-// Waits for the work on st by polling an event: the host sees the readbacks
-// of an index build and the end of a solve within microseconds, where a
-// blocking synchronisation measured 22-38 us late on this device.
-int Engine::wait_stream(hipStream_t st)
+// Waits for the work on st by polling ev (one event per thread that waits):
+// the host sees the readbacks of an index build and the end of a solve within
+// microseconds, where a blocking synchronisation measured 22-38 us late on
+// this device.
+int Engine::wait_stream(hipStream_t st, hipEvent_t ev)
 {
-    SH_HIP(hipEventRecord(sync_ev_, st));
+    SH_HIP(hipEventRecord(ev, st));
     hipError_t q;
-    while ((q = hipEventQuery(sync_ev_)) == hipErrorNotReady)
+    while ((q = hipEventQuery(ev)) == hipErrorNotReady)
         __builtin_ia32_pause();
     if (q != hipSuccess)
     {
@@ -1364,7 +1371,7 @@ int Engine::finish(hipStream_t st)
 {
     barrier_timeout_ = false;
     SH_HIP(hipMemcpyAsync(h_flag_, static_cast<uint32_t *>(d_flags_) + 1, 4, hipMemcpyDeviceToHost, st));
-    if (int rc = wait_stream(st); rc != SH_OK)
+    if (int rc = wait_stream(st, sync_ev_); rc != SH_OK)
         return rc;
     collect_marks();
     if (*h_flag_ != 0)

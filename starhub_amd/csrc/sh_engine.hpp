@@ -147,7 +147,7 @@ private:
     void build_maps(uint32_t ns);
     int upload(uint32_t ns, hipStream_t st);
     int enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identity);
-    int wait_stream(hipStream_t st);
+    int wait_stream(hipStream_t st, hipEvent_t ev);
     int finish(hipStream_t st);
     int fall_back_to_split_pcg(hipStream_t st);
     const uint32_t *map(int m, uint32_t ns) const;
@@ -185,7 +185,8 @@ private:
     size_t sort_temp_bytes_ = 0;
     uint32_t *h_meta_ = nullptr; // index-build readbacks
     uint32_t *h_flag_ = nullptr; // launch-side readback (PCG barrier flag)
-    hipEvent_t sync_ev_ = nullptr; // polled by wait_stream
+    hipEvent_t sync_ev_ = nullptr;  // polled by finish (the launch half)
+    hipEvent_t index_ev_ = nullptr; // polled by build_index (the index half)
     std::vector<uint32_t> batch_; // slots of the current launch, in order
     void *h_stage_ = nullptr;
     size_t stage_bytes_ = 0;
