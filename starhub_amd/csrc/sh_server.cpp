@@ -84,6 +84,7 @@ struct sh_server
     sh_server_config cfg{};
     clk::time_point t0;
     hipStream_t s_admit = nullptr, s_step = nullptr;
+    hipEvent_t ev_step = nullptr; // polled by the step thread after a step's copies
     std::mutex mu;
     std::condition_variable cv_admit, cv_step, cv_done;
     std::unordered_map<uint32_t, StreamState> streams;
@@ -272,7 +273,16 @@ void sh_server::step_loop()
             }
         }
         if (!finished.empty() && copy_rc == SH_OK)
-            copy_rc = hip_status(hipStreamSynchronize(s_step));
+        {
+            // a polled event: a blocking synchronisation wakes tens of
+            // microseconds after the copies end
+            copy_rc = hip_status(hipEventRecord(ev_step, s_step));
+            hipError_t q = hipErrorNotReady;
+            while (copy_rc == SH_OK && (q = hipEventQuery(ev_step)) == hipErrorNotReady)
+                __builtin_ia32_pause();
+            if (copy_rc == SH_OK)
+                copy_rc = hip_status(q);
+        }
         lk.lock();
         stats.steps += 1;
         stats.max_batch = std::max<uint64_t>(stats.max_batch, batch.size());
@@ -359,7 +369,8 @@ extern "C"
         for (uint32_t i = 0; i < n; ++i)
             s->free_slots.push_back(n - 1 - i);
         if (hipStreamCreateWithFlags(&s->s_admit, hipStreamNonBlocking) != hipSuccess ||
-            hipStreamCreateWithFlags(&s->s_step, hipStreamNonBlocking) != hipSuccess)
+            hipStreamCreateWithFlags(&s->s_step, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&s->ev_step, hipEventDisableTiming) != hipSuccess)
         {
             delete s;
             return SH_E_HIP;
@@ -396,6 +407,8 @@ extern "C"
             (void)hipStreamDestroy(s->s_admit);
         if (s->s_step)
             (void)hipStreamDestroy(s->s_step);
+        if (s->ev_step)
+            (void)hipEventDestroy(s->ev_step);
         delete s;
         return SH_OK;
     }
