@@ -201,7 +201,10 @@ int sh_solver_set_inputs(sh_solver *solver, const sh_camera *camera, const sh_me
 /* WarpSolver::SolveStreamed (WarpSolver.cu:260-279): index build, then
  * gn_iterations x (terms, JtJ/JtR, PCG, warp update) starting from identity.
  * Work is enqueued on `stream`; the call returns after the stream has been
- * synchronised (OptimizationProcessorWarpSolver.cu:51).  SH_E_ARG when a
+ * synchronised (OptimizationProcessorWarpSolver.cu:51).  The calling thread
+ * waits by polling an event (it stays busy on one core for the index build's
+ * two readbacks and the end of the solve) rather than sleeping in a blocking
+ * synchronisation, which wakes tens of microseconds late.  SH_E_ARG when a
  * valid index_map entry is not below num_surfels, or a surfel KNN or edge
  * node id is not below num_nodes (the reference reads such ids unchecked). */
 int sh_solver_solve(sh_solver *solver, sh_stream_t stream);
