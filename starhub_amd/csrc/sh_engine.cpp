@@ -1275,7 +1275,9 @@ int Engine::autotune(uint32_t ns, hipStream_t st, uint32_t repeats, sh_tuning *b
     std::vector<sh_tuning> cands;
     for (uint32_t rows : {85u, 64u, 48u, 32u})
         cands.push_back({0u, SH_SPMV_MATERIALIZED, rows, 0.0});
-    for (uint32_t rows : {85u, 64u, 48u, 32u, 24u, 16u})
+    // smallest first: a later candidate must win by 2% (below), and 16 rows
+    // beat 24 by less than that on a 1,024-node frame
+    for (uint32_t rows : {16u, 24u, 32u, 48u, 64u, 85u})
         cands.push_back({SH_FLAG_PIPELINED_PCG, SH_SPMV_MATERIALIZED, rows, 0.0});
     cands.push_back({SH_FLAG_STREAMING_PCG, SH_SPMV_MATERIALIZED, kPcgResRows, 0.0});
     cands.push_back({SH_FLAG_NO_PERSISTENT_PCG, SH_SPMV_MATERIALIZED, kPcgResRows, 0.0});
