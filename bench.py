@@ -97,7 +97,7 @@ def phase_bytes(name: str, cfg, P: int, U: int) -> float:
     }.get(name, 0.0)
 
 
-KERNEL_OF_PHASE = {"dense_terms": "k_terms", "reg_terms": "k_reg_terms", "rows_seg": "k_rows_seg",
+KERNEL_OF_PHASE = {"dense_terms": "k_terms", "reg_terms": "k_terms", "rows_seg": "k_rows_seg",
                    "rows_combine": "k_rows_combine", "dinv": "k_dinv", "pcg": "k_pcg_resident",
                    "update": "k_update", "tiles": "k_tiles", "tiles_combine": "k_tiles_combine"}
 

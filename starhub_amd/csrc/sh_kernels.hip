@@ -536,15 +536,6 @@ __device__ __forceinline__ void dense_terms_wg(const SolveDesc *__restrict__ sds
         d.cost_part[wg - wg_begin[si]] = wc;
 }
 
-template <int K, class JT>
-__global__ __launch_bounds__(256) void k_dense_terms(const SolveDesc *__restrict__ sds,
-                                                     const uint32_t *__restrict__ wg_begin, int n_solve, int parity,
-                                                     int store)
-{
-    dense_terms_wg<K, JT>(sds, wg_begin, n_solve, blockIdx.x, parity, store);
-}
-
-
 // ARAP forward warp per edge + node translations (forwardWarpSmootherNodeKernel,
 // NodeGraphHandler.cu:7-46; ComputeNodeMotionKernel, NodeMotionHandler.cu:9-21).
 // Work-group wg of the edge map.
@@ -591,13 +582,6 @@ __device__ __forceinline__ void reg_terms_wg(const SolveDesc *__restrict__ sds, 
     const double wc = block_sum_256(edge_cost);
     if (threadIdx.x == 0)
         d.cost_part[(uint32_t(d.P) + 255u) / 256u + (wg - wg_begin[si])] = wc;
-}
-
-__global__ __launch_bounds__(256) void k_reg_terms(const SolveDesc *__restrict__ sds,
-                                                   const uint32_t *__restrict__ wg_begin, int n_solve, int parity,
-                                                   int store)
-{
-    reg_terms_wg(sds, wg_begin, n_solve, blockIdx.x, parity, store);
 }
 
 // Both term kernels as one launch: the pixel map's work-groups, then the edge
@@ -1515,28 +1499,6 @@ hipError_t launch_identity_dq(dq8 *dq, uint32_t n_nodes, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_dense_terms(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve,
-                              uint32_t n_wg, int parity, hipStream_t st, int store)
-{
-    if (n_wg == 0)
-        return hipSuccess;
-    if (K == 8)
-    {
-        if (fp16)
-            hipLaunchKernelGGL((k_dense_terms<8, __half>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity, store);
-        else
-            hipLaunchKernelGGL((k_dense_terms<8, float>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity, store);
-    }
-    else
-    {
-        if (fp16)
-            hipLaunchKernelGGL((k_dense_terms<4, __half>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity, store);
-        else
-            hipLaunchKernelGGL((k_dense_terms<4, float>), dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity, store);
-    }
-    return hipGetLastError();
-}
-
 hipError_t launch_terms(int K, bool fp16, const SolveDesc *sds, const uint32_t *pix_map, uint32_t n_pix_wg,
                         const uint32_t *reg_map, uint32_t n_reg_wg, int n_solve, int parity, hipStream_t st, int store)
 {
@@ -1561,15 +1523,6 @@ hipError_t launch_terms(int K, bool fp16, const SolveDesc *sds, const uint32_t *
             hipLaunchKernelGGL((k_terms<4, float>), grid, dim3(256), 0, st, sds, pix_map, n_pix_wg, reg_map, n_solve,
                                parity, store);
     }
-    return hipGetLastError();
-}
-
-hipError_t launch_reg_terms(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int parity,
-                            hipStream_t st, int store)
-{
-    if (n_wg == 0)
-        return hipSuccess;
-    hipLaunchKernelGGL(k_reg_terms, dim3(n_wg), dim3(256), 0, st, sds, wmap, n_solve, parity, store);
     return hipGetLastError();
 }
 

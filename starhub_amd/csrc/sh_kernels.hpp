@@ -48,12 +48,9 @@ hipError_t launch_row_ptr(const SolveDesc &d, const uint32_t *skeys, hipStream_t
 hipError_t launch_identity(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, hipStream_t st);
 hipError_t launch_identity_dq(dq8 *dq, uint32_t n_nodes, hipStream_t st);
 hipError_t launch_cost_final(const SolveDesc *sds, int n_solve, int slot, hipStream_t st);
-// store = 0: only the cost partials (the records of the last linearisation stay)
-hipError_t launch_dense_terms(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve,
-                              uint32_t n_wg, int parity, hipStream_t st, int store = 1);
-hipError_t launch_reg_terms(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int parity,
-                            hipStream_t st, int store = 1);
-// launch_dense_terms and launch_reg_terms as one launch
+// The dense (per compacted pixel) and ARAP (per edge, then per node) terms as
+// one launch; store = 0: only the cost partials (the records of the last
+// linearisation stay)
 hipError_t launch_terms(int K, bool fp16, const SolveDesc *sds, const uint32_t *pix_map, uint32_t n_pix_wg,
                         const uint32_t *reg_map, uint32_t n_reg_wg, int n_solve, int parity, hipStream_t st,
                         int store = 1);

@@ -16,7 +16,7 @@ __device__ __forceinline__ dq8 load_dq(const dq8 *__restrict__ a, uint32_t i)
 }
 
 // Total weighted cost (dense + ARAP) from the work-group partials of
-// k_dense_terms and k_reg_terms, FP64, fixed order; one wave.
+// the dense and ARAP parts of k_terms, FP64, fixed order; one wave.
 __device__ inline void reduce_cost(const SolveDesc &d, int slot)
 {
     const uint32_t n = (uint32_t(d.P) + 255u) / 256u + (uint32_t(d.E + d.N) + 255u) / 256u;
