@@ -1,33 +1,47 @@
 """Benchmark: warp-field Gauss-Newton tracking solves per second on one MI355X.
 
-Workload (default): the full-frame plain solve -- 300k surfels, 1024 nodes,
-640x480 depth + optical flow, ARAP weight 2.3, 10 GN x 20 PCG, K = 8.  One
-"step" is one complete solve (index build + GN loop) of that frame with every
-input already resident in HBM.  Prints one JSON line with:
+Headline (`value`): the serving workload -- 64 camera streams (N log-uniform
+in [256, 4096], 300 surfels per node, 320x240 .. 1280x720, 6 GN x 10 PCG),
+each stream's surfels and warp field resident in HBM, admission by the free
+HBM budget, continuous batching -- at capacity.  One "step" is one frame
+from each of the 64 streams (64 requests, Poisson-ordered arrival times);
+the K timed steps' requests are all queued when the timed region starts, so
+the server runs as fast as it can take them.  Every input is resident in HBM
+before the timed region.  Beside it, in the same run:
 
-* value / ms_per_step / latency_ms: solves per second over the timed steps and
-  the p50/p99 of the per-solve wall time (each solve ends in a stream sync);
-* roofline: the GN-phase kernel with the largest share of device time, its
-  average launch time from HIP events recorded on the solve's stream around
-  every launch in the timed region, and its algorithmic bytes per launch
-  (each necessary input read once, each output written once, at the storage
-  precision used) -> achieved GB/s against the 8 TB/s HBM peak.  `traffic`
-  is the measured HBM bytes per launch from the rocprofv3 FETCH_SIZE /
-  WRITE_SIZE summary committed under profiles/ for this kernel and workload,
-  when one is there (null otherwise);
-* cpu_baseline: the FP32 CPU restatement of the same solve (OpenMP over the
-  host cores), timed on a bounded sample of the same frame;
-* warp_error_vs_fp64: max over nodes of ||T - T_fp64||_F / ||T_fp64||_F of the
-  last timed solve against the FP64 CPU oracle.
+* latency_ms / served_30hz: the serving workload as configured -- Poisson
+  arrivals at 30 Hz per stream (1,920 requests/s offered), 10,000 requests --
+  its p50/p99 request latency and served rate;
+* plain: the plain single-solve path on the full-frame workload (300k
+  surfels, 1024 nodes, 640x480, 10 GN x 20 PCG), autotuned, with the
+  roofline of its dominant kernel and the CPU baseline on the same frame;
+* vs_plain: the plain path (autotuned per frame) on the same request mix;
+* c3_fp16: the mixed-precision workload (1.2M surfels, 4096 nodes,
+  1280x720, FP16 record streams, 10 GN x 30 PCG), autotuned, with its
+  warp-field error after every GN iteration;
+* batched: 32 uneven solves in one batch (one launch per GN phase);
+* warp errors: against the FP64 CPU oracle replaying the GPU's
+  rotation-cutoff decisions (|w|_1 < 1e-4 drops a node's rotation).
 
-  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+roofline (top level): the GN-phase kernel with the largest share of the
+served run's device time (HIP events recorded on the step stream around every
+launch), achieved = the algorithmic bytes of its launches (each necessary
+input read once, each output written once, at the storage precision used,
+per solve and GN iteration) / their summed duration, against the 8 TB/s HBM
+peak; `traffic` is the measured HBM bytes per launch from the rocprofv3
+FETCH_SIZE / WRITE_SIZE summary committed under profiles/ for that kernel and
+workload, when one is there.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--quick]
 """
 from __future__ import annotations
 
 import argparse
 import json
+import os
 import sys
 import time
+from dataclasses import replace
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
@@ -40,19 +54,25 @@ METRIC = "GN tracking solves/s + p50/p99 latency at ≤1e-4 err vs FP64; served 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
-    ap.add_argument("--no-check", action="store_true", help="skip the FP64-oracle check of the warp field")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline sample budget")
-    ap.add_argument("--no-served", action="store_true", help="skip the serving (64 streams) and batch (32 solves) legs")
-    ap.add_argument("--served-requests", type=int, default=10000)
+    ap.add_argument("--steps", type=int, default=50, help="timed steps: rounds of one frame per served stream")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed warm-up rounds (and plain warm-up solves)")
+    ap.add_argument("--plain-steps", type=int, default=50, help="timed solves of the plain full-frame leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline legs")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64-oracle checks of the warp field")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline sample budget per leg")
+    ap.add_argument("--served-requests", type=int, default=10000, help="requests of the 30 Hz serving run")
+    ap.add_argument("--quick", action="store_true", help="plain full-frame leg only (profiling)")
+    ap.add_argument("--only", default=None, choices=["plain", "c3", "batched", "served"],
+                    help="run one leg only and print its object (profiling)")
     ap.add_argument("--tuning", default=None, metavar="FLAGS,SPMV,ROWS",
-                    help="use this tuning instead of running the autotuner (profiled runs)")
+                    help="plain leg: use this tuning instead of running the autotuner (profiled runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain device copies of 256 MiB (the reference reading for HBM counters)")
     return ap.parse_args()
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
 def copy_calibration():
@@ -65,11 +85,12 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_per_launch": x.numel() * 4, "launches": 5}), flush=True)
 
 
+# ------------------------------------------------------------------ algorithmic bytes
 def phase_bytes(name: str, cfg, P: int, U: int) -> float:
-    """Algorithmic bytes of one launch of a GN-phase kernel.
+    """Algorithmic bytes of one launch of a GN-phase kernel for one solve.
 
-    Per-unit figures (K slots, record = the per-pixel H/g record: 28 values,
-    4 B each in FP32 streams, 2 B in FP16 streams):
+    Per-unit figures (K slots, E = N*K ARAP edge slots, record = the per-pixel
+    H/g record: 28 values, 4 B each with FP32 streams, 2 B with FP16):
       dense_terms  P * (8 pixel pair + 2K ids + 4K folded weights + 64 texels + record) + N * 32 (node DQs)
                    + E * 40 + N * 48 (the ARAP terms, same launch: k_terms)
       reg_terms    E * 40 + N * 48 (its own launch, other paths)
@@ -97,6 +118,19 @@ def phase_bytes(name: str, cfg, P: int, U: int) -> float:
     }.get(name, 0.0)
 
 
+def solve_bytes(cfg, P: int, U: int) -> float:
+    """Algorithmic bytes of one whole solve: the index build once, then per GN
+    iteration the terms, the materialisation, the inverse, the PCG and the
+    update.  Index build: W*H*8 (index map and flags) + P*(8 + 6K) (compacted
+    streams) + (PK + 2E + N)*6*2 (Node2Term keys and values, sorted once)
+    + (28P + E)*8*2 (the node-pair keys the reference sorts)."""
+    K, N = cfg.K, cfg.num_nodes
+    E = N * K
+    index = cfg.width * cfg.height * 8 + P * (8 + 6 * K) + (P * K + 2 * E + N) * 12 + (28 * P + E) * 16
+    per_gn = sum(phase_bytes(p, cfg, P, U) for p in ("dense_terms", "tiles", "tiles_combine", "pcg", "update"))
+    return index + cfg.solver.gn_iterations * per_gn
+
+
 KERNEL_OF_PHASE = {"dense_terms": "k_terms", "reg_terms": "k_terms", "rows_seg": "k_rows_seg",
                    "rows_combine": "k_rows_combine", "dinv": "k_dinv", "pcg": "k_pcg_resident",
                    "update": "k_update", "tiles": "k_tiles", "tiles_combine": "k_tiles_combine"}
@@ -122,49 +156,189 @@ def measured_traffic(kernel: str, workload: str):
         return None, None
 
 
-def cpu_baseline(sc, prod, cfg, seconds: float):
+def roofline(phases: dict, bytes_of_phase, tuning: dict, workload: str) -> dict:
+    """The phase with the largest device time: its algorithmic bytes over its
+    launches' summed duration."""
+    total = {k: v[0] for k, v in phases.items() if v[1] > 0}
+    dom = max(total, key=total.get)
+    ms, launches = phases[dom]
+    nbytes = bytes_of_phase(dom)
+    achieved = nbytes / (ms / 1e3) / 1e9
+    kernel = pcg_kernel(tuning) if dom == "pcg" else KERNEL_OF_PHASE.get(dom, dom)
+    traffic, tsrc = measured_traffic(kernel, workload)
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
+           "avg_launch_us": ms * 1e3 / launches, "bytes_per_launch": nbytes / launches, "launches": launches,
+           "phase_share": {k: round(v / sum(total.values()), 4) for k, v in total.items()}}
+    if tsrc:
+        out["traffic_source"] = tsrc
+    return out
+
+
+# ------------------------------------------------------------------ CPU baseline
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(items, seconds: float, what: str) -> dict:
+    """The FP32 CPU restatement (OpenMP over the host threads) solving `items`
+    ((scene, producers) pairs) one after another, as many rounds as fit in
+    about `seconds`; solves/s over the whole sample."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import sho  # the CPU restatement: the baseline, not the thing measured
     from starhub_amd import workload
     threads = sho.max_threads()
-    host = workload.host_inputs(sc, prod)
-    sho.solve(host, cfg.solver, 32, threads)  # warm-up
-    times = []
-    t_start = time.perf_counter()
-    while time.perf_counter() - t_start < seconds and len(times) < 50:
-        r = sho.solve(host, cfg.solver, 32, threads)
-        times.append(r["solve_ms"])
-    times.sort()
-    med = times[len(times) // 2]
-    return {"value": 1000.0 / med, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"{len(times)} full FP32 solves of the same frame (median solve time), OpenMP over "
-                      f"{threads} host threads"}
+    inputs = [(workload.host_inputs(sc, prod), sc.cfg.solver) for sc, prod in items]
+    sho.solve(inputs[0][0], inputs[0][1], 32, threads, variant="native")  # warm-up
+    n, spent = 0, 0.0
+    t0 = time.perf_counter()
+    while True:
+        for inp, params in inputs:
+            r = sho.solve(inp, params, 32, threads, variant="native")
+            spent += r["solve_ms"] / 1e3
+            n += 1
+            if time.perf_counter() - t0 > seconds and n >= 3:
+                break
+        if time.perf_counter() - t0 > seconds and n >= 3:
+            break
+    return {"value": n / spent, "unit": "solves/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(), "build": sho.variant_name("native"),
+            "sample": f"{n} FP32 solves of {what} (the solve time summed), OpenMP over {threads} host threads"}
 
 
-def plain_rate(frames, cfgs, native, reps: int = 1):
-    """Plain path on a mix: one solver, the frames solved one after another."""
-    import time as _t
+# ------------------------------------------------------------------ legs
+def oracle_check(solver, sc, prod, node_motion_pred=None):
+    """Warp-field error of the solver's last solve against the FP64 oracle
+    replaying its rotation-cutoff decisions; also how many of those decisions
+    differ from the ones the oracle would take itself."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import sho  # the checker
+    from starhub_amd import workload
+    gpu = workload.device_dq_to_wxyz(solver.node_se3())
+    cut = solver.debug("rot_cut", torch.uint8).cpu().numpy()
+    ref = sho.solve(workload.host_inputs(sc, prod, node_motion_pred), sc.cfg.solver, 64, sho.max_threads(),
+                    rot_cut=cut)
+    flips = int(np.sum((ref["twist_l1"] < float(np.float32(1e-4))) != cut.astype(bool)))
+    return workload.warp_error(gpu, ref["dq"].reshape(-1, 8)), flips, ref
+
+
+def plain_leg(args, name: str = "c2") -> dict:
+    """One workload solved plain (one solve per call), autotuned."""
+    import numpy as np
+    import torch
+    from starhub_amd import native, scene, workload
+    cfg = {"c1": scene.config_c1, "c2": scene.config_c2, "c3": scene.config_c3}[name]()
+    sc = scene.generate(cfg)
+    frame, prod = workload.device_frame(sc)
+    solver = workload.solver_for(sc, flags=native.SH_FLAG_PHASE_TIMING)
+    solver.set_inputs(frame)
+    if args.tuning and name == "c2":
+        f, m, r = (int(v) for v in args.tuning.split(","))
+        native.set_tuning(solver, f, m, r)
+        tuning = {"flags": f, "spmv_mode": m, "pcg_rows_per_group": r, "ms": None}
+    else:
+        tuning = native.autotune(solver, repeats=9)
+    steps = args.plain_steps if name == "c2" else max(10, args.plain_steps // 3)
+    for _ in range(args.warmup):
+        solver.solve()
+    torch.cuda.synchronize()
+    solver.phase_times(reset=True)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        solver.solve()  # ends in a stream synchronisation
+        lat.append(time.perf_counter() - s0)
+    dt = time.perf_counter() - t0
+    phases = solver.phase_times(reset=True)
+    P, U, _ = solver.sizes()
+    lat_ms = np.array(lat) * 1e3
+    out = {"workload": f"{name}: plain single solve, {cfg.num_surfels} surfels, {cfg.num_nodes} nodes, "
+                       f"{cfg.width}x{cfg.height}, K={cfg.K}, {cfg.solver.gn_iterations} GN x "
+                       f"{cfg.solver.pcg_iterations} PCG" + (", FP16 record streams" if cfg.solver.fp16 else ""),
+           "solves_per_s": steps / dt, "steps": steps, "ms_per_solve": 1e3 * dt / steps,
+           "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+           "valid_pixels": P, "node_pairs": U, "tuning": tuning,
+           "achieved_GBps": solve_bytes(cfg, P, U) * steps / dt / 1e9}
+    out["roofline"] = roofline(phases, lambda ph: phase_bytes(ph, cfg, P, U) * phases[ph][1], tuning, name)
+    if not args.no_check:
+        err, flips, ref = oracle_check(solver, sc, prod)
+        out["warp_error_vs_fp64"] = err
+        out["cutoff_flips"] = flips
+        if name == "c3":
+            # the warp-field error after every GN iteration: the same solve with
+            # gn_iterations = k against the replayed oracle's state after k
+            N = cfg.num_nodes
+            states = ref["dq_iter"].reshape(-1, N, 8)
+            errs = []
+            for k in range(1, cfg.solver.gn_iterations + 1):
+                c = native.make_config(replace(cfg.solver, gn_iterations=k), cfg.width, cfg.height, N, cfg.K)
+                s = native.WarpSolver(c)
+                native.set_tuning(s, tuning["flags"], tuning["spmv_mode"], tuning["pcg_rows_per_group"])
+                s.process_frame(frame)
+                errs.append(workload.warp_error(workload.device_dq_to_wxyz(s.node_se3()), states[k]))
+                s.close()
+            out["warp_error_per_gn_iteration"] = errs
+    solver.close()
+    out["_scene"] = (sc, prod)
+    return out
+
+
+def make_frames(cfgs):
+    from starhub_amd import scene, workload
+    scs = [scene.generate(c) for c in cfgs]
+    dev = [workload.device_frame(s) for s in scs]
+    return scs, [f for f, _ in dev], [p for _, p in dev]
+
+
+def plain_rate_tuned(frames, cfgs, native, repeats: int = 2):
+    """The plain path on a mix, autotuned per frame: one solver, each frame
+    tuned once (the autotuner's candidates, median of three solves each), then
+    the frames solved one after another with their own tunings.  Returns
+    (solves/s, per-frame sizes (P, U), the tunings' PCG kernels)."""
+    import torch
     W = max(c.width for c in cfgs)
     H = max(c.height for c in cfgs)
     N = max(c.num_nodes for c in cfgs)
     solver = native.WarpSolver(native.make_config(cfgs[0].solver, W, H, N, cfgs[0].K))
-    for f in frames[:2]:
+    tunings, sizes = [], []
+    for f in frames:
+        solver.set_inputs(f)
+        t = native.autotune(solver, repeats=3)
+        tunings.append(t)
+        sizes.append(solver.sizes()[:2])
+    for f, t in list(zip(frames, tunings))[:2]:
+        native.set_tuning(solver, t["flags"], t["spmv_mode"], t["pcg_rows_per_group"])
         solver.process_frame(f)
-    t0 = _t.perf_counter()
-    for _ in range(reps):
-        for f in frames:
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        for f, t in zip(frames, tunings):
+            native.set_tuning(solver, t["flags"], t["spmv_mode"], t["pcg_rows_per_group"])
             solver.process_frame(f)
-    dt = _t.perf_counter() - t0
+    dt = time.perf_counter() - t0
     solver.close()
-    return reps * len(frames) / dt
+    kinds = {}
+    for t in tunings:
+        k = pcg_kernel(t) + ("/row segments" if t["flags"] & 32 else "")
+        kinds[k] = kinds.get(k, 0) + 1
+    return repeats * len(frames) / dt, sizes, kinds
 
 
-def batched_leg(native, scene, workload):
+def batched_leg(args) -> dict:
     """32 uneven solves (N log-uniform in [128, 4096], 300 surfels/node) in one
     batch: one launch per GN phase for all of them."""
-    import time as _t
+    from starhub_amd import native, scene
     cfgs = scene.config_c4()
-    frames = [workload.device_frame(scene.generate(c))[0] for c in cfgs]
+    scs, frames, prods = make_frames(cfgs)
     W = max(c.width for c in cfgs)
     H = max(c.height for c in cfgs)
     N = max(c.num_nodes for c in cfgs)
@@ -174,74 +348,153 @@ def batched_leg(native, scene, workload):
     tuning = native.autotune(b, batch_solves=len(cfgs), repeats=2)
     b.solve(len(cfgs))
     reps = 5
-    t0 = _t.perf_counter()
+    t0 = time.perf_counter()
     for _ in range(reps):
         b.solve(len(cfgs))
-    dt = _t.perf_counter() - t0
+    dt = time.perf_counter() - t0
     b.close()
     batched = reps * len(cfgs) / dt
-    plain = plain_rate(frames, cfgs, native)
-    return {"workload": "32 solves, N log-uniform [128, 4096], 300 surfels/node, 320x240..1280x720, "
-                        f"{cfgs[0].solver.gn_iterations} GN x {cfgs[0].solver.pcg_iterations} PCG",
-            "solves_per_s": batched, "plain_solves_per_s": plain, "vs_plain": batched / plain, "tuning": tuning}
+    plain, sizes, kinds = plain_rate_tuned(frames, cfgs, native)
+    nbytes = sum(solve_bytes(c, P, U) for c, (P, U) in zip(cfgs, sizes))
+    out = {"workload": "32 solves, N log-uniform [128, 4096], 300 surfels/node, 320x240..1280x720, "
+                       f"{cfgs[0].solver.gn_iterations} GN x {cfgs[0].solver.pcg_iterations} PCG",
+           "solves_per_s": batched, "ms_per_batch": 1e3 * dt / reps, "plain_solves_per_s": plain,
+           "plain_tunings": kinds, "vs_plain": batched / plain, "tuning": tuning,
+           "algorithmic_bytes_per_batch": nbytes, "achieved_GBps": nbytes * reps / dt / 1e9}
+    out["achieved_frac"] = out["achieved_GBps"] / HBM_PEAK_GBS
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(list(zip(scs, prods)), args.cpu_seconds, "the 32-solve mix")
+        out["vs_cpu"] = batched / out["cpu_baseline"]["value"]
+    return out
 
 
-def served_leg(native, scene, workload, n_requests: int):
-    """64 camera streams, Poisson arrivals at 30 Hz each, continuous batching."""
+def served_legs(args) -> dict:
+    """The serving workload: at capacity (the timed steps' requests queued at
+    once) and as configured (Poisson arrivals at 30 Hz per stream)."""
     import ctypes
-    import time as _t
     import numpy as np
     import torch
+    from starhub_amd import native, scene
     cfgs = scene.config_c5_streams()
-    scs = [scene.generate(c) for c in cfgs]
-    frames = [workload.device_frame(s)[0] for s in scs]
+    scs, frames, prods = make_frames(cfgs)
     refs = [(torch.from_numpy(s.surfel_pos).cuda(), torch.from_numpy(s.surfel_normal).cuda()) for s in scs]
     W = max(c.width for c in cfgs)
     H = max(c.height for c in cfgs)
     N = max(c.num_nodes for c in cfgs)
-    scfg = native.make_config(cfgs[0].solver, W, H, N, cfgs[0].K)
-    plain = plain_rate(frames, cfgs, native)
-    srv = native.Server(scfg, max_active=64)
+    plain, sizes, kinds = plain_rate_tuned(frames, cfgs, native)
+    bytes_of = [solve_bytes(c, P, U) for c, (P, U) in zip(cfgs, sizes)]
+    scfg = native.make_config(cfgs[0].solver, W, H, N, cfgs[0].K, flags=native.SH_FLAG_PHASE_TIMING)
+    srv = native.Server(scfg, max_active=64, hbm_budget_bytes=native.HBM_BUDGET_FREE)
     for i, f in enumerate(frames):
         native.check(srv.add_stream(i, f, *refs[i]), "add_stream")  # surfels resident, warped after each solve
     L = native.lib()
-    args = []
-    for i, f in enumerate(frames):
+    args_of = []
+    for f in frames:
         cam, meas, rend, geo, ng, nf, fl, kp, c2w = f.structs()
-        args.append((meas, rend, fl, nf))
-    rng = np.random.default_rng(5)
-    arrivals = []
-    for i in range(len(frames)):
-        t = np.cumsum(rng.exponential(1.0 / 30.0, size=n_requests // len(frames) + 50))
-        arrivals += [(float(x), i) for x in t]
-    arrivals.sort()
-    arrivals = arrivals[:n_requests]
-    tickets = []
+        args_of.append((meas, rend, fl, nf))
     tk = ctypes.c_uint64()
-    t0 = _t.perf_counter()
-    for when, i in arrivals:
-        d = when - (_t.perf_counter() - t0)
-        if d > 0:
-            _t.sleep(d)
-        meas, rend, fl, nf = args[i]
+
+    def submit(i):
+        meas, rend, fl, nf = args_of[i]
         native.check(L.sh_server_submit(srv._h, i, ctypes.byref(meas), ctypes.byref(rend), ctypes.byref(fl),
                                         ctypes.byref(nf), None, ctypes.byref(tk)), "submit")
-        tickets.append(tk.value)
+        return tk.value
+
+    rng = np.random.default_rng(5)
+
+    def arrivals(n_req, rate_hz):
+        """Poisson arrivals at rate_hz per stream, the first n_req in time order."""
+        out = []
+        for i in range(len(frames)):
+            t = np.cumsum(rng.exponential(1.0 / rate_hz, size=n_req // len(frames) + 50))
+            out += [(float(x), i) for x in t]
+        out.sort()
+        return out[:n_req]
+
+    # warm-up rounds (slot buffers grown, graphs captured), untimed
+    for t in [submit(i) for _, i in arrivals(args.warmup * len(frames), 30.0)]:
+        srv.wait(t)
+    srv.phase_times(reset=True)
+    # capacity: the K steps' requests (one frame per stream per step) all queued at once
+    order = arrivals(args.steps * len(frames), 30.0)
+    st0 = srv.stats()
+    t0 = time.perf_counter()
+    tickets = [submit(i) for _, i in order]
     res = [srv.wait(t) for t in tickets]
-    st = srv.stats()
+    dt = time.perf_counter() - t0
+    st1 = srv.stats()
+    phases = srv.phase_times(reset=True)
+    assert all(r["status"] == 0 for r in res), "a served request failed"
+    lat = np.array([r["done_ms"] - r["submit_ms"] for r in res])
+    capacity = len(res) / dt
+    steps = st1["steps"] - st0["steps"]
+    per_stream = np.bincount([i for _, i in order], minlength=len(frames))
+    total_bytes = sum(per_stream[i] * bytes_of[i] for i in range(len(frames)))
+
+    # the same streams at the configured offered load
+    order30 = arrivals(args.served_requests, 30.0)
+    tickets = []
+    t0 = time.perf_counter()
+    for when, i in order30:
+        d = when - (time.perf_counter() - t0)
+        if d > 0:
+            time.sleep(d)
+        tickets.append(submit(i))
+    res30 = [srv.wait(t) for t in tickets]
+    st2 = srv.stats()
     srv.close()
-    ok = [r for r in res if r["status"] == 0]
-    lat = np.array([r["done_ms"] - r["submit_ms"] for r in ok])
-    span_s = (max(r["done_ms"] for r in ok) - min(r["submit_ms"] for r in ok)) / 1e3
-    served = len(ok) / span_s
-    return {"workload": f"64 streams, N log-uniform [256, 4096], 300 surfels/node, Poisson 30 Hz/stream, "
-                        f"resident surfels forward-warped after each solve, "
-                        f"{len(res)} requests, {cfgs[0].solver.gn_iterations} GN x {cfgs[0].solver.pcg_iterations} PCG",
-            "solves_per_s": served, "offered_per_s": len(res) / arrivals[-1][0],
-            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "failed": len(res) - len(ok), "slots": st["slots"], "max_batch": st["max_batch"],
-            "mean_batch": st["slot_iterations"] / max(1, st["steps"]),
-            "plain_solves_per_s": plain, "vs_plain": served / plain}
+    ok = [r for r in res30 if r["status"] == 0]
+    lat30 = np.array([r["done_ms"] - r["submit_ms"] for r in ok])
+    span = (max(r["done_ms"] for r in ok) - min(r["submit_ms"] for r in ok)) / 1e3
+
+    # the roofline of the served run's dominant kernel: every solve of the
+    # timed requests ran each GN phase once per GN iteration
+    gn = cfgs[0].solver.gn_iterations
+
+    def served_phase_bytes(ph):
+        return sum(per_stream[i] * gn * phase_bytes(ph, cfgs[i], *sizes[i]) for i in range(len(frames)))
+
+    server_tuning = {"flags": native.SH_FLAG_NO_PERSISTENT_PCG, "spmv_mode": 0, "pcg_rows_per_group": 0}
+    roof = roofline(phases, served_phase_bytes, server_tuning, "c5")
+    roof["note"] = ("achieved = the algorithmic bytes of every timed solve's launches of this phase / the summed "
+                    "duration of the phase's launches (each launch covers every admitted solve)")
+    return {
+        "capacity": {"workload": f"served at capacity: 64 streams, N log-uniform [256, 4096], 300 surfels/node, "
+                                 f"320x240..1280x720, {cfgs[0].solver.gn_iterations} GN x "
+                                 f"{cfg_pcg(cfgs)} PCG, {len(res)} requests ({args.steps} per stream, Poisson "
+                                 f"30 Hz order) queued at once; continuous batching; resident surfels "
+                                 f"forward-warped after each solve; admission by the free-HBM budget",
+                     "solves_per_s": capacity, "requests": len(res), "wall_s": dt,
+                     "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                     "slots": st1["slots"], "hbm_budget": st1["hbm_budget"], "hbm_reserved": st1["hbm_reserved"],
+                     "mean_batch": (st1["slot_iterations"] - st0["slot_iterations"]) / max(1, steps),
+                     "max_batch": st1["max_batch"], "algorithmic_bytes": total_bytes,
+                     "achieved_GBps": total_bytes / dt / 1e9},
+        "served_30hz": {"workload": f"served as configured: the same 64 streams, Poisson arrivals at 30 Hz per "
+                                    f"stream, {len(res30)} requests",
+                        "offered_per_s": len(res30) / order30[-1][0], "solves_per_s": len(ok) / span,
+                        "p50_ms": float(np.percentile(lat30, 50)), "p99_ms": float(np.percentile(lat30, 99)),
+                        "failed": len(res30) - len(ok), "max_batch": st2["max_batch"]},
+        "plain_solves_per_s": plain, "plain_tunings": kinds,
+        "roofline": roof,
+        "_items": list(zip(scs, prods)),
+    }
+
+
+def cfg_pcg(cfgs):
+    return cfgs[0].solver.pcg_iterations
+
+
+def _plain_json(o):
+    """numpy scalars and arrays in the JSON line"""
+    import numpy as np
+    if isinstance(o, np.integer):
+        return int(o)
+    if isinstance(o, np.floating):
+        return float(o)
+    if isinstance(o, np.ndarray):
+        return o.tolist()
+    raise TypeError(f"{type(o).__name__} is not JSON serializable")
 
 
 def main():
@@ -254,89 +507,74 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
-    import numpy as np
-    import torch
-    from starhub_amd import native, scene, workload
+    import torch  # noqa: F401
+    from starhub_amd import native
 
-    cfg = {"c1": scene.config_c1, "c2": scene.config_c2, "c3": scene.config_c3}[args.config]()
-    sc = scene.generate(cfg)
-    frame, prod = workload.device_frame(sc)
-    solver = workload.solver_for(sc, flags=native.SH_FLAG_PHASE_TIMING)
-    solver.set_inputs(frame)
-    if args.tuning:
-        f, m, r = (int(v) for v in args.tuning.split(","))
-        native.set_tuning(solver, f, m, r)
-        tuning = {"flags": f, "spmv_mode": m, "pcg_rows_per_group": r, "ms": None}
-    else:
-        tuning = native.autotune(solver, repeats=9)  # fastest variant on this frame (median of 9), kept for the timed steps
-    for _ in range(args.warmup):
-        solver.solve()
-    torch.cuda.synchronize()
-    solver.phase_times(reset=True)
-    lat = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s0 = time.perf_counter()
-        solver.solve()  # ends in a stream synchronisation
-        lat.append(time.perf_counter() - s0)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    phases = solver.phase_times(reset=True)
-    P, U, _ = solver.sizes()
+    try:
+        if args.quick or args.only == "plain":
+            p = plain_leg(args, "c2")
+            p.pop("_scene")
+            print(json.dumps(p, default=_plain_json), flush=True)
+            return
+        if args.only == "c3":
+            p = plain_leg(args, "c3")
+            p.pop("_scene")
+            print(json.dumps(p, default=_plain_json), flush=True)
+            return
+        if args.only == "batched":
+            print(json.dumps(batched_leg(args), default=_plain_json), flush=True)
+            return
+        if args.only == "served":
+            s = served_legs(args)
+            s.pop("_items")
+            print(json.dumps(s, default=_plain_json), flush=True)
+            return
 
-    lat_ms = np.sort(np.array(lat) * 1e3)
-    out = {
-        "metric": METRIC,
-        "value": args.steps / dt,
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1000.0 * dt / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp16 streams, fp32 accumulate, fp64 PCG scalars + 6x6 inverse" if cfg.solver.fp16
-        else "fp32 (fp64 PCG scalars + 6x6 inverse)",
-        "data": "synthetic",
-        "config": {"workload": f"{args.config}: plain single solve, {cfg.num_surfels} surfels, {cfg.num_nodes} "
-                               f"nodes, {cfg.width}x{cfg.height}, K={cfg.K}, {cfg.solver.gn_iterations} GN x "
-                               f"{cfg.solver.pcg_iterations} PCG",
-                   "valid_pixels": P, "node_pairs": U},
-        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
-        "tuning": tuning,
-    }
-
-    # roofline of the kernel with the largest device-time share
-    total = {k: v[0] for k, v in phases.items() if v[1] > 0}
-    dom = max(total, key=total.get)
-    avg_s = phases[dom][0] / phases[dom][1] / 1e3
-    nbytes = phase_bytes(dom, cfg, P, U)
-    achieved = nbytes / avg_s / 1e9
-    kernel = pcg_kernel(tuning) if dom == "pcg" else KERNEL_OF_PHASE.get(dom, dom)
-    traffic, tsrc = measured_traffic(kernel, args.config)
-    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
-                       "avg_launch_us": avg_s * 1e6, "bytes_per_launch": nbytes,
-                       "launches": phases[dom][1]}
-    if tsrc:
-        out["roofline"]["traffic_source"] = tsrc
-    out["phase_share"] = {k: round(v / sum(total.values()), 4) for k, v in total.items()}
-
-    if not args.no_check:
-        sys.path.insert(0, str(ROOT / "oracle"))
-        import sho  # the checker
-        gpu = workload.device_dq_to_wxyz(solver.node_se3())
-        ref = sho.solve(workload.host_inputs(sc, prod), cfg.solver, 64, sho.max_threads())
-        out["warp_error_vs_fp64"] = workload.warp_error(gpu, ref["dq"].reshape(-1, 8))
-
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sc, prod, cfg, args.cpu_seconds)
-    if not args.no_served:
-        del solver
-        out["batched"] = batched_leg(native, scene, workload)
-        out["served"] = served_leg(native, scene, workload, args.served_requests)
-    print(json.dumps(out), flush=True)
+        say("plain full-frame leg")
+        plain = plain_leg(args, "c2")
+        c2_scene = plain.pop("_scene")
+        if not args.no_cpu:
+            plain["cpu_baseline"] = cpu_baseline([c2_scene], args.cpu_seconds, "the full-frame workload")
+        say("mixed-precision leg")
+        c3 = plain_leg(args, "c3")
+        c3.pop("_scene")
+        say("batched leg")
+        batched = batched_leg(args)
+        say("serving legs")
+        served = served_legs(args)
+        items = served.pop("_items")
+        cap = served["capacity"]
+        out = {
+            "metric": METRIC,
+            "value": cap["solves_per_s"],
+            "unit": "solves/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": 1e3 * cap["wall_s"] / args.steps,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "fp32 streams (fp64 PCG scalars, 6x6 inverse, cost)",
+            "data": "synthetic",
+            "config": {"workload": cap["workload"], "streams": 64, "requests_per_step": 64},
+            "latency_ms": {"p50": served["served_30hz"]["p50_ms"], "p99": served["served_30hz"]["p99_ms"],
+                           "at": "the configured load: Poisson 30 Hz per stream"},
+            "vs_plain": cap["solves_per_s"] / served["plain_solves_per_s"],
+            "roofline": served["roofline"],
+        }
+        if not args.no_cpu:
+            out["cpu_baseline"] = cpu_baseline(items, args.cpu_seconds, "the 64 served streams' frames")
+            out["vs_cpu"] = cap["solves_per_s"] / out["cpu_baseline"]["value"]
+        out["served"] = cap
+        out["served_30hz"] = served["served_30hz"]
+        out["plain_same_mix"] = {"solves_per_s": served["plain_solves_per_s"], "tunings": served["plain_tunings"]}
+        out["plain"] = plain
+        out["c3_fp16"] = c3
+        out["batched"] = batched
+        print(json.dumps(out, default=_plain_json), flush=True)
+    finally:
+        native.close_all()  # HIP objects go before the runtime's own teardown
 
 
 if __name__ == "__main__":

@@ -211,7 +211,10 @@ int sh_solver_solve(sh_solver *solver, sh_stream_t stream);
 
 /* WarpSolver::SolvedNodeSE3 / OptimizationProcessorWarpSolver::SolvedSE3
  * (WarpSolver.h:45, OptimizationProcessorWarpSolver.h:28-30): device view of
- * num_nodes dual quaternions. */
+ * num_nodes dual quaternions.  Before the first solve the view holds
+ * max_nodes identity transforms, as the reference's SolverIterationData
+ * starts (SolverIterationData.cu:78-88, 127-132); afterwards it is the last
+ * solve's field (num_nodes of that solve) until the next solve overwrites it. */
 int sh_solver_node_se3(const sh_solver *solver, const sh_dual_quat **device_view, uint32_t *num_nodes);
 
 /* Sizes of the last index build: number of valid pixels P, unique node pairs U
@@ -223,7 +226,9 @@ int sh_solver_sizes(const sh_solver *solver, uint32_t *num_pixels, uint32_t *num
  * half_keys, half_off, pair_entries, upper_pos, lower_pos (with
  * SH_FLAG_REFERENCE_INDEX), row_ptr, blk_col, hg, dense_cnt, seg_ptr,
  * tile_cbase, tile_sbase, list_off, blk_transpose, reg, blk, dinv, rhs, x,
- * dq0, dq1, delta. */
+ * dq0, dq1, delta, rot_cut (uint8, gn_iterations x num_nodes: 1 where the
+ * warp update dropped the node's rotation increment because |w|_1 < 1e-4,
+ * SolverIterationData.cu:42). */
 int sh_solver_debug_view(const sh_solver *solver, const char *name, const void **device_ptr, size_t *count);
 
 /* FP64 total weighted cost sum lambda^2 r^2 (dense depth + flow, ARAP) at the start
@@ -236,6 +241,75 @@ int sh_solver_cost_history(const sh_solver *solver, double *out, uint32_t capaci
 /* ApplySpMVBase<6>::ApplySpMV (libs/star/pcg_solver/ApplySpMVBase.h:19-37):
  * y = A x with the system of the last GN iteration (materialised or matrix-free). */
 int sh_solver_apply_jtj(sh_solver *solver, const float *x, float *y, uint32_t matrix_free, sh_stream_t stream);
+
+/* ---------------------------------------------- linear-solver plug-in
+ * The reference's BlockPCG<6> over an abstract ApplySpMVBase<6>
+ * (libs/star/pcg_solver/BlockPCG.h:14-115, ApplySpMVBase.h:9-47), for a
+ * system the caller brings.  Vectors are FP32 device arrays of matrix_size
+ * entries (6 per node); dot products, alpha and beta are FP64. */
+
+/* ApplySpMVBase<6> (ApplySpMVBase.h:9-37) as a table of host callbacks that
+ * enqueue device work on `stream` and return an sh_status:
+ *   apply_spmv     y = A x             (ApplySpMV, :22-24)
+ *   init_residual  r = b - A x_init    (InitResidual, :32-36); NULL: the
+ *                  solver computes it through apply_spmv
+ * matrix_size is MatrixSize() (:19). */
+typedef struct sh_spmv_ops
+{
+    void *user;
+    uint32_t matrix_size;
+    int (*apply_spmv)(void *user, const float *x, float *y, sh_stream_t stream);
+    int (*init_residual)(void *user, const float *x_init, const float *b, float *residual, sh_stream_t stream);
+} sh_spmv_ops;
+
+/* ApplySpMVBinBlockCSR<6> (ApplySpMVBinBlockCSR.h:8-52) in this library's own
+ * block layout: a caller-owned 6x6 block-CSR with num_block_rows rows,
+ * row_ptr[num_block_rows + 1] block offsets, block_col[num_blocks] block
+ * columns and blocks[num_blocks * 36] FP32 row-major 6x6 blocks; (i, j) holds
+ * A[6i..6i+5][6j..6j+5].  The arrays are borrowed and must outlive the handle.
+ * sh_spmv_check verifies the structure on the device (SH_E_ARG when a row
+ * range or a column is out of bounds); apply and init_residual trust it. */
+typedef struct sh_spmv sh_spmv;
+int sh_spmv_create_block_csr(uint32_t num_block_rows, uint32_t num_blocks, const uint32_t *row_ptr,
+                             const uint32_t *block_col, const float *blocks, sh_spmv **out);
+int sh_spmv_destroy(sh_spmv *spmv);
+int sh_spmv_check(const sh_spmv *spmv, sh_stream_t stream);
+int sh_spmv_matrix_size(const sh_spmv *spmv, uint32_t *matrix_size);
+int sh_spmv_apply(sh_spmv *spmv, const float *x, float *y, sh_stream_t stream);
+int sh_spmv_init_residual(sh_spmv *spmv, const float *x_init, const float *b, float *residual, sh_stream_t stream);
+/* The callback table of a block-CSR operator (for sh_block_pcg_set_input). */
+int sh_spmv_ops_of(sh_spmv *spmv, sh_spmv_ops *ops);
+/* The callback table of a solver's own system of its last GN iteration
+ * (materialised blocks, or matrix-free when matrix_free != 0; each call
+ * synchronises `stream`).  Valid while the solver lives. */
+int sh_solver_spmv_ops(sh_solver *solver, uint32_t matrix_free, sh_spmv_ops *ops);
+
+/* BlockPCG<6> (BlockPCG.h:14-115).  sh_block_pcg_create allocates its
+ * vectors for max_matrix_size (a multiple of 6), BlockPCG(max_matrix_size)
+ * (:50).  set_input is SetSolverInput(inv_diag_blks, spmv_handler, b, x_init,
+ * actual_size) (:55-61): inv_diag_blocks holds matrix_size / 6 FP32 6x6
+ * inverse diagonal blocks, row-major; x is the caller's solution vector,
+ * read as x_init when a solve does not zero-initialise; actual_size 0 takes
+ * the operator's matrix_size; SH_E_CAPACITY beyond max_matrix_size.  All
+ * pointers are borrowed until the next set_input. */
+typedef struct sh_block_pcg sh_block_pcg;
+int sh_block_pcg_create(uint32_t max_matrix_size, sh_block_pcg **out);
+int sh_block_pcg_destroy(sh_block_pcg *pcg);
+int sh_block_pcg_set_input(sh_block_pcg *pcg, const float *inv_diag_blocks, const sh_spmv_ops *spmv, const float *b,
+                           float *x, uint32_t actual_size);
+/* Solve / SolveNoConvergeCheck (BlockPCG.h:64-65, BlockPCG.cuh:325-349):
+ * exactly max_iterations block-Jacobi PCG iterations from x = 0 (zero_init)
+ * or from x (r = b - A x through InitResidual); the result is left in x.
+ * Returns after synchronising `stream`. */
+int sh_block_pcg_solve(sh_block_pcg *pcg, uint32_t max_iterations, int zero_init, sh_stream_t stream);
+/* SolveConvergeChecked (BlockPCG.h:68-71, BlockPCG.cuh:351-393): stops after
+ * the first iteration with |r.M^-1 r| < epsilon^2 |r0.M^-1 r0| (one host
+ * readback per iteration, as in the reference); *iterations = the count run. */
+int sh_block_pcg_solve_converge_checked(sh_block_pcg *pcg, uint32_t max_iterations, int zero_init, float epsilon,
+                                        sh_stream_t stream, uint32_t *iterations);
+/* r.M^-1 r (FP64) before the first iteration and after each iteration of the
+ * last solve: *count = iterations + 1; out may be NULL to query the count. */
+int sh_block_pcg_delta_history(const sh_block_pcg *pcg, double *out, uint32_t capacity, uint32_t *count);
 
 /* ------------------------------------------------------- input producers */
 /* SkinnerKernel (libs/star/geometry/node_graph/Skinner.cu:10-94): K nearest
@@ -377,11 +451,17 @@ int sh_forward_warp(const sh_dual_quat *warp_field, uint32_t num_nodes, uint32_t
  * boundary as slots free up); frames of one stream are solved in order. */
 typedef struct sh_server sh_server;
 
+/* hbm_budget_bytes value: the device's free HBM when the server is created
+ * (hipMemGetInfo), less 1/16 of it as headroom. */
+#define SH_HBM_BUDGET_FREE (~(uint64_t)0)
+
 typedef struct sh_server_config
 {
     sh_solver_config solver;   /* per-solve capacity (the largest request) and solver constants */
     uint32_t max_active;       /* solves in flight at most (continuous-batch width) */
-    uint64_t hbm_budget_bytes; /* device bytes the server may hold (solve slots + resident streams); 0 = no limit */
+    uint64_t hbm_budget_bytes; /* device bytes the server may hold: solve slots, their tile and partial buffers
+                                  (which grow to the largest frame a slot has held) and resident streams;
+                                  0 = no limit; SH_HBM_BUDGET_FREE = the free HBM at creation */
 } sh_server_config;
 
 typedef struct sh_request_result
@@ -401,6 +481,8 @@ typedef struct sh_server_stats
     uint64_t max_batch;       /* most solves in one launch */
     uint64_t slot_iterations; /* sum of batch sizes over steps */
     uint64_t hbm_reserved;    /* bytes held */
+    uint64_t hbm_budget;      /* the budget in force (0 = no limit) */
+    uint64_t growth_refused;  /* index builds refused because a slot's buffers would pass the budget */
     double now_ms;
 } sh_server_stats;
 
@@ -414,6 +496,8 @@ int sh_server_create(const sh_server_config *config, sh_server **out);
 int sh_server_destroy(sh_server *server);
 /* Registers a stream and copies its geometry and node graph into resident HBM;
  * SH_E_CAPACITY when the budget cannot hold it (admission by memory budget).
+ * A request whose index build would grow its slot's buffers past the budget
+ * completes with SH_E_CAPACITY (sh_server_stats.growth_refused).
  * With reference surfels (float4 vertex and normal, both or neither), the
  * server keeps them resident too and forward-warps them and the nodes after
  * every solve of the stream (sh_server_live_surfels). */
@@ -486,6 +570,10 @@ int sh_solver_pcg_fallbacks(const sh_solver *solver, uint32_t *count);
 const char *sh_phase_name(uint32_t phase);
 int sh_solver_phase_times(sh_solver *solver, double *ms, uint32_t *calls, uint32_t num_phases, uint32_t reset);
 int sh_batch_phase_times(sh_batch *batch, double *ms, uint32_t *calls, uint32_t num_phases, uint32_t reset);
+/* The same for a server created with SH_FLAG_PHASE_TIMING in its solver
+ * config: the GN-phase kernel time of its steps (a launch covers every solve
+ * admitted at that step). */
+int sh_server_phase_times(sh_server *server, double *ms, uint32_t *calls, uint32_t num_phases, uint32_t reset);
 
 #ifdef __cplusplus
 }

@@ -16,6 +16,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 
 #include "warp_solver.h"
 
@@ -95,6 +96,163 @@ private:
     sh_camera camera_;
 };
 
+// star::ApplySpMVBase<6> (libs/star/pcg_solver/ApplySpMVBase.h:9-47): the
+// operator a BlockPCG applies.  Derive and override ApplySpMV (and InitResidual
+// with HasInitResidual() returning true, or leave both and the solver forms
+// b - A x through ApplySpMV).  Vectors are device arrays of MatrixSize() floats;
+// work is enqueued on `stream`.  An exception thrown by an override is caught
+// at the C boundary and the solve returns SH_E_ARG (sh::Error in BlockPCG).
+class ApplySpMVBase
+{
+public:
+    using Ptr = std::shared_ptr<ApplySpMVBase>;
+    ApplySpMVBase() = default;
+    virtual ~ApplySpMVBase() = default;
+    ApplySpMVBase(const ApplySpMVBase &) = delete;
+    ApplySpMVBase &operator=(const ApplySpMVBase &) = delete;
+
+    virtual size_t MatrixSize() const = 0;
+    virtual void ApplySpMV(const float *x, float *spmv_x, sh_stream_t stream) = 0;
+    virtual bool HasInitResidual() const { return false; }
+    // residual <- b - A x_init
+    virtual void InitResidual(const float *x_init, const float *b, float *residual, sh_stream_t stream)
+    {
+        (void)x_init, (void)b, (void)residual, (void)stream;
+        throw Error(SH_E_STATE, "InitResidual is not implemented");
+    }
+
+    // The C callback table that forwards to the virtuals above.
+    sh_spmv_ops Ops()
+    {
+        sh_spmv_ops o{};
+        o.user = this;
+        o.matrix_size = uint32_t(MatrixSize());
+        o.apply_spmv = &ApplySpMVBase::apply_cb;
+        o.init_residual = HasInitResidual() ? &ApplySpMVBase::init_cb : nullptr;
+        return o;
+    }
+
+private:
+    static int apply_cb(void *user, const float *x, float *y, sh_stream_t stream)
+    {
+        try
+        {
+            static_cast<ApplySpMVBase *>(user)->ApplySpMV(x, y, stream);
+            return SH_OK;
+        }
+        catch (const Error &e)
+        {
+            return e.status();
+        }
+        catch (...)
+        {
+            return SH_E_ARG;
+        }
+    }
+    static int init_cb(void *user, const float *x, const float *b, float *r, sh_stream_t stream)
+    {
+        try
+        {
+            static_cast<ApplySpMVBase *>(user)->InitResidual(x, b, r, stream);
+            return SH_OK;
+        }
+        catch (const Error &e)
+        {
+            return e.status();
+        }
+        catch (...)
+        {
+            return SH_E_ARG;
+        }
+    }
+};
+
+// star::ApplySpMVBinBlockCSR<6> (libs/star/pcg_solver/ApplySpMVBinBlockCSR.h:8-52)
+// over a caller-owned 6x6 block-CSR in this library's layout (sh_spmv_create_block_csr).
+class ApplySpMVBlockCSR : public ApplySpMVBase
+{
+public:
+    ApplySpMVBlockCSR(uint32_t num_block_rows, uint32_t num_blocks, const uint32_t *row_ptr, const uint32_t *block_col,
+                      const float *blocks)
+    {
+        check(sh_spmv_create_block_csr(num_block_rows, num_blocks, row_ptr, block_col, blocks, &h_),
+              "sh_spmv_create_block_csr");
+    }
+    ~ApplySpMVBlockCSR() override { sh_spmv_destroy(h_); }
+    size_t MatrixSize() const override
+    {
+        uint32_t n = 0;
+        check(sh_spmv_matrix_size(h_, &n), "sh_spmv_matrix_size");
+        return n;
+    }
+    void ApplySpMV(const float *x, float *spmv_x, sh_stream_t stream) override
+    {
+        check(sh_spmv_apply(h_, x, spmv_x, stream), "sh_spmv_apply");
+    }
+    bool HasInitResidual() const override { return true; }
+    void InitResidual(const float *x_init, const float *b, float *residual, sh_stream_t stream) override
+    {
+        check(sh_spmv_init_residual(h_, x_init, b, residual, stream), "sh_spmv_init_residual");
+    }
+    sh_spmv *handle() const { return h_; }
+
+private:
+    sh_spmv *h_ = nullptr;
+};
+
+// star::BlockPCG<6> (libs/star/pcg_solver/BlockPCG.h:14-115): the same methods
+// and argument meaning.  Solve returns the solution view (the caller's x).
+class BlockPCG
+{
+public:
+    using Ptr = std::shared_ptr<BlockPCG>;
+    explicit BlockPCG(size_t max_matrix_size, sh_stream_t stream = nullptr) : stream_(stream)
+    {
+        check(sh_block_pcg_create(uint32_t(max_matrix_size), &h_), "sh_block_pcg_create");
+    }
+    ~BlockPCG() { sh_block_pcg_destroy(h_); }
+    BlockPCG(const BlockPCG &) = delete;
+    BlockPCG &operator=(const BlockPCG &) = delete;
+
+    // inv_diag_blks: MatrixSize()/6 row-major FP32 6x6 blocks; x_init: the
+    // solution vector (read as the start unless zero_init)
+    bool SetSolverInput(const float *inv_diag_blks, ApplySpMVBase::Ptr spmv_handler, const float *b, float *x_init,
+                        size_t actual_size = 0)
+    {
+        spmv_ = std::move(spmv_handler);
+        ops_ = spmv_->Ops();
+        x_ = x_init;
+        check(sh_block_pcg_set_input(h_, inv_diag_blks, &ops_, b, x_init, uint32_t(actual_size)),
+              "sh_block_pcg_set_input");
+        return true;
+    }
+    void UpdateCudaStream(sh_stream_t stream) { stream_ = stream; }
+    const float *Solve(size_t max_iterations = 10, bool zero_init = true)
+    {
+        return SolveNoConvergeCheck(max_iterations, zero_init);
+    }
+    const float *SolveNoConvergeCheck(size_t max_iterations, bool zero_init = true)
+    {
+        check(sh_block_pcg_solve(h_, uint32_t(max_iterations), zero_init ? 1 : 0, stream_), "sh_block_pcg_solve");
+        return x_;
+    }
+    const float *SolveConvergeChecked(size_t max_iterations, bool zero_init = true, float epsilon = 2e-3f)
+    {
+        uint32_t done = 0;
+        check(sh_block_pcg_solve_converge_checked(h_, uint32_t(max_iterations), zero_init ? 1 : 0, epsilon, stream_,
+                                                  &done),
+              "sh_block_pcg_solve_converge_checked");
+        return x_;
+    }
+
+private:
+    sh_block_pcg *h_ = nullptr;
+    sh_stream_t stream_ = nullptr;
+    ApplySpMVBase::Ptr spmv_;
+    sh_spmv_ops ops_{};
+    float *x_ = nullptr;
+};
+
 // star::OptimizationProcessorWarpSolver: keeps the camera-to-world extrinsic
 // and forwards one frame to the solver.
 class OptimizationProcessorWarpSolver
@@ -114,7 +272,11 @@ public:
                       OpticalFlow4Solver &opticalflow4solver, KeyPoint4Solver &keypoint4solver,
                       const unsigned frame_idx, sh_stream_t stream)
     {
-        (void)frame_idx; // the reference only logs it
+        // the reference starts operating at frame 1: frame 0 returns without
+        // solving (OptimizationProcessorWarpSolver.cu:35-36), and SolvedSE3()
+        // keeps the solver's field (identity before any solve)
+        if (frame_idx == 0)
+            return;
         solver_->SetSolverInputs(measure4solver, render4solver, geometry4solver, node_graph4solver, nodeflow4solver,
                                  opticalflow4solver, keypoint4solver, cam2world_);
         solver_->SolveStreamed(stream);

@@ -222,13 +222,21 @@ inline DQ<T> dq_from_mat34(const M34<T> &M)
     return {qmul(Q<T>{1, 0, 0, 0}, qr), qadd(qmul(tv, qr), qmul(Q<T>{1, 0, 0, 0}, zero))};
 }
 
-// Rodrigues exponential of a twist; identity rotation when |w|_1 < 1e-4
-// examples/mono_star/opt/SolverIterationData.cu:41-68
+// The reference's rotation cutoff: a twist's rotation is dropped when
+// |w|_1 < 1e-4 (examples/mono_star/opt/SolverIterationData.cu:42)
 template <class T>
-inline M34<T> twist_exp(V3<T> w, V3<T> t)
+inline bool rotation_cut(V3<T> w)
+{
+    return std::fabs(w.x) + std::fabs(w.y) + std::fabs(w.z) < T(1e-4f);
+}
+
+// Rodrigues exponential of a twist; identity rotation when `cut` (by default
+// the cutoff above)  examples/mono_star/opt/SolverIterationData.cu:41-68
+template <class T>
+inline M34<T> twist_exp(V3<T> w, V3<T> t, bool cut)
 {
     M34<T> S;
-    if (std::fabs(w.x) + std::fabs(w.y) + std::fabs(w.z) < T(1e-4f))
+    if (cut)
     {
         S.R = M33<T>::identity();
     }
@@ -249,6 +257,12 @@ inline M34<T> twist_exp(V3<T> w, V3<T> t)
     }
     S.t = t;
     return S;
+}
+
+template <class T>
+inline M34<T> twist_exp(V3<T> w, V3<T> t)
+{
+    return twist_exp(w, t, rotation_cut(w));
 }
 
 // ------------------------------------------------------------- FP16 rounding
@@ -485,6 +499,13 @@ struct Params
     float lambda_reg = 1.0f;
     float lambda_node_translation = 0.0f;
     float damping = 1e-3f;
+    // Decision replay (test infrastructure): when set, gn_iters * N flags that
+    // decide, per GN iteration and node, whether the update drops the rotation
+    // (1) or keeps it (0), in place of the oracle's own |w|_1 < 1e-4 test.  A
+    // node whose FP64 |w|_1 lies at the cutoff can fall on either side of it in
+    // FP32 arithmetic; replaying the FP32 solve's decisions makes the two
+    // trajectories comparable at the bare tolerance.
+    const uint8_t *rot_cut = nullptr;
 };
 
 // Everything the oracle can expose for the checks.
@@ -516,6 +537,7 @@ struct Result
     std::vector<double> cost;          // per GN iteration (incl. a final one): total weighted cost
     std::vector<double> dq_iter;       // (gn_iters+1) * N * 8, (q0 wxyz, q1 wxyz)
     std::vector<double> twist_l1;      // gn_iters * N: |w|_1 of each node's rotation increment
+    std::vector<uint8_t> rot_cut;      // gn_iters * N: 1 where the update dropped the rotation
     std::vector<double> dq;            // N*8 final
 };
 
@@ -1209,13 +1231,15 @@ public:
     }
 
     // ---- a19: T_new = exp(xi) T_old, stored as a dual quaternion  SolverIterationData.cu:19-74
-    void update()
+    void update(int it)
     {
         for (int i = 0; i < N; ++i)
         {
             const V3<T> w{x[6 * i], x[6 * i + 1], x[6 * i + 2]};
             const V3<T> t{x[6 * i + 3], x[6 * i + 4], x[6 * i + 5]};
-            const M34<T> S = twist_exp(w, t);
+            const bool cut = prm.rot_cut ? prm.rot_cut[size_t(it) * size_t(N) + size_t(i)] != 0 : rotation_cut(w);
+            res.rot_cut.push_back(cut ? 1 : 0);
+            const M34<T> S = twist_exp(w, t, cut);
             const M34<T> prev = dq_to_mat34(dq[i]);
             dq[i] = dq_from_mat34(mul(S, prev));
         }
@@ -1273,6 +1297,7 @@ public:
         dq.assign(N, DQ<T>{{1, 0, 0, 0}, {0, 0, 0, 0}}); // identity start  SolverIterationData.cu:78-88
         res.dq_iter.clear();
         res.twist_l1.clear();
+        res.rot_cut.clear();
         res.cost.clear();
         dump_dq(res.dq_iter);
         Dinv.assign(size_t(N) * 36, T(0));
@@ -1309,7 +1334,7 @@ public:
             for (int n = 0; n < N; ++n)
                 res.twist_l1.push_back(std::fabs(double(x[6 * n])) + std::fabs(double(x[6 * n + 1])) +
                                        std::fabs(double(x[6 * n + 2])));
-            update();
+            update(it);
             dump_dq(res.dq_iter);
         }
         // cost at the final warp field
@@ -1323,6 +1348,93 @@ public:
         dump_dq(res.dq);
     }
 };
+
+// ------------------------------------------------ linear-solver plug-in (test infrastructure)
+// BlockPCG<6>::Solve over a 6x6 block-CSR operator, in T:
+//   ZeroIntialize: r = b, x = 0, d = M^-1 r, delta = r.d  (BlockPCG.cuh:421-440)
+//   Initialize:    r = b - A x (ApplySpMVBinBlockCSR::InitResidual,
+//                  ApplySpMVBinBlockCSR.cuh:137-162), d = M^-1 r, delta = r.d  (BlockPCG.cuh:405-419)
+//   iteration:     q = A d; alpha = delta / d.q; x += alpha d; t = r - alpha q;
+//                  s = M^-1 t; delta' = t.s; beta = delta' / delta; d = s + beta d  (BlockPCG.cuh:337-349, 442-483)
+// blocks: nnz row-major 6x6; inv_diag: nblk row-major 6x6.  delta_hist gets
+// delta before the first iteration and after each one.  A zero system gives
+// alpha = beta = 0 instead of 0/0, as the solver's own PCG above.
+template <class T>
+void block_pcg(int nblk, const uint32_t *row_ptr, const uint32_t *col, const double *blocks, const double *inv_diag,
+               const double *b, double *x, int iters, bool zero_init, std::vector<double> *delta_hist)
+{
+    const size_t n = size_t(nblk) * 6;
+    std::vector<T> vx(n), vr(n), vd(n), vq(n), vt(n), vs(n);
+    auto spmv = [&](const std::vector<T> &v, std::vector<T> &y) {
+        for (int i = 0; i < nblk; ++i)
+            for (int a = 0; a < 6; ++a)
+            {
+                T acc = 0;
+                for (uint32_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k)
+                    for (int c = 0; c < 6; ++c)
+                        acc += T(blocks[36 * size_t(k) + 6 * a + c]) * v[6 * size_t(col[k]) + c];
+                y[6 * size_t(i) + a] = acc;
+            }
+    };
+    auto precond = [&](const std::vector<T> &v, std::vector<T> &y) {
+        for (int i = 0; i < nblk; ++i)
+            for (int a = 0; a < 6; ++a)
+            {
+                T acc = 0;
+                for (int c = 0; c < 6; ++c)
+                    acc += T(inv_diag[36 * size_t(i) + 6 * a + c]) * v[6 * size_t(i) + c];
+                y[6 * size_t(i) + a] = acc;
+            }
+    };
+    auto vdot = [&](const std::vector<T> &u, const std::vector<T> &v) {
+        T acc = 0;
+        for (size_t i = 0; i < n; ++i)
+            acc += u[i] * v[i];
+        return acc;
+    };
+    if (zero_init)
+    {
+        for (size_t i = 0; i < n; ++i)
+        {
+            vx[i] = T(0);
+            vr[i] = T(b[i]);
+        }
+    }
+    else
+    {
+        for (size_t i = 0; i < n; ++i)
+            vx[i] = T(x[i]);
+        spmv(vx, vq);
+        for (size_t i = 0; i < n; ++i)
+            vr[i] = T(b[i]) - vq[i];
+    }
+    precond(vr, vd);
+    T delta = vdot(vr, vd);
+    if (delta_hist)
+        delta_hist->assign(1, double(delta));
+    for (int it = 0; it < iters; ++it)
+    {
+        spmv(vd, vq);
+        const T dq = vdot(vd, vq);
+        const T alpha = dq != T(0) ? delta / dq : T(0);
+        for (size_t i = 0; i < n; ++i)
+        {
+            vx[i] += alpha * vd[i];
+            vt[i] = vr[i] - alpha * vq[i];
+        }
+        precond(vt, vs);
+        const T dn = vdot(vt, vs);
+        const T beta = delta != T(0) ? dn / delta : T(0);
+        delta = dn;
+        for (size_t i = 0; i < n; ++i)
+            vd[i] = vs[i] + beta * vd[i];
+        vr.swap(vt);
+        if (delta_hist)
+            delta_hist->push_back(double(delta));
+    }
+    for (size_t i = 0; i < n; ++i)
+        x[i] = double(vx[i]);
+}
 
 // ------------------------------------------------ §8f.1 forward warp (test infrastructure)
 // SurfelNodeDeformer::ForwardWarpSurfelsAndNodes (SurfelNodeDeformer.cu:13-80):

@@ -4,6 +4,7 @@
 #include "sh_measure.hpp"
 #include "sh_oracle.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <map>
@@ -38,6 +39,7 @@ extern "C"
     {
         int gn_iters, pcg_iters;
         float lambda_depth, lambda_flow, lambda_reg, lambda_node_translation, damping;
+        const uint8_t *rot_cut; // NULL, or gn_iters * num_nodes replayed rotation-cutoff decisions
     };
 }
 
@@ -91,6 +93,7 @@ sho::Params to_params(const sho_params *c)
     p.lambda_reg = c->lambda_reg;
     p.lambda_node_translation = c->lambda_node_translation;
     p.damping = c->damping;
+    p.rot_cut = c->rot_cut;
     return p;
 }
 
@@ -198,6 +201,7 @@ extern "C"
         SHO_GET(cost, 8)
         SHO_GET(dq_iter, 8)
         SHO_GET(twist_l1, 8)
+        SHO_GET(rot_cut, 1)
         SHO_GET(dq, 8)
 #undef SHO_GET
         if (n == "num_pixels")
@@ -281,6 +285,21 @@ extern "C"
         out34[11] = m.t.z;
     }
     void sho_invert6(const double *A, double *Ainv) { sho::Solver<double>::invert6(A, Ainv); }
+
+    // BlockPCG<6>::Solve over a block-CSR (sho::block_pcg); precision 64 or 32.
+    // x: in (x_init unless zero_init) and out; delta_hist: iters + 1 entries.
+    void sho_block_pcg(int nblk, const uint32_t *row_ptr, const uint32_t *col, const double *blocks,
+                       const double *inv_diag, const double *b, double *x, int iters, int zero_init, int precision,
+                       double *delta_hist)
+    {
+        std::vector<double> h;
+        if (precision == 64)
+            sho::block_pcg<double>(nblk, row_ptr, col, blocks, inv_diag, b, x, iters, zero_init != 0, &h);
+        else
+            sho::block_pcg<float>(nblk, row_ptr, col, blocks, inv_diag, b, x, iters, zero_init != 0, &h);
+        if (delta_hist)
+            std::copy(h.begin(), h.end(), delta_hist);
+    }
 
     // Evaluate the per-pixel residual/Jacobian and the ARAP records at a given
     // warp field (FP64): dq is N x 8 (q0 wxyz, q1 wxyz).  Outputs: r (P*3),

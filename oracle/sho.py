@@ -40,36 +40,67 @@ class ShoParams(ctypes.Structure):
         ("gn_iters", ctypes.c_int), ("pcg_iters", ctypes.c_int),
         ("lambda_depth", ctypes.c_float), ("lambda_flow", ctypes.c_float), ("lambda_reg", ctypes.c_float),
         ("lambda_node_translation", ctypes.c_float), ("damping", ctypes.c_float),
+        ("rot_cut", ctypes.POINTER(ctypes.c_uint8)),
     ]
 
 
-def lib():
+_VARIANTS = {}
+
+
+def _host_has_avx512() -> bool:
+    try:
+        flags = next(line for line in open("/proc/cpuinfo") if line.startswith("flags")).split()
+    except (OSError, StopIteration):
+        return False
+    return all(f in flags for f in ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"))
+
+
+def variant_name(variant: str = "default") -> str:
+    """Which build a variant loads: "default" is the portable x86-64-v3 build
+    (the checks); "native" is the x86-64-v4 build on hosts with AVX-512, else
+    the portable one (the CPU baseline)."""
+    if variant == "native" and _host_has_avx512() and (_HERE / "_build" / "libsh_oracle_v4.so").exists():
+        return "x86-64-v4 (AVX-512)"
+    return "x86-64-v3"
+
+
+def lib(variant: str = "default"):
     global _LIB
+    if variant != "default":
+        name = variant_name(variant)
+        if name == "x86-64-v3":
+            return lib()
+        if name not in _VARIANTS:
+            _VARIANTS[name] = _load(_HERE / "_build" / "libsh_oracle_v4.so")
+        return _VARIANTS[name]
     if _LIB is None:
-        path = _HERE / "_build" / "libsh_oracle.so"
-        if not path.exists():
-            raise RuntimeError(f"{path} is missing: run `make -C oracle` or `python -m starhub_amd.build`")
-        L = ctypes.CDLL(str(path))
-        L.sho_skin.argtypes = [_FP, ctypes.c_int, _FP, ctypes.c_int, ctypes.c_int, ctypes.c_float,
-                               _U16P, _FP, _FP, ctypes.c_int]
-        L.sho_node_graph.argtypes = [_FP, ctypes.c_int, ctypes.c_int, ctypes.c_float, _U16P, _FP, _U16P, _FP,
-                                     ctypes.c_int]
-        L.sho_solve.argtypes = [ctypes.POINTER(ShoInputs), ctypes.POINTER(ShoParams), ctypes.c_int, ctypes.c_int]
-        L.sho_solve.restype = ctypes.c_void_p
-        L.sho_result_get.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p),
-                                     ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int)]
-        L.sho_result_free.argtypes = [ctypes.c_void_p]
-        L.sho_round_to_half.argtypes = [ctypes.c_float]
-        L.sho_round_to_half.restype = ctypes.c_float
-        L.sho_float2uint_rn.argtypes = [ctypes.c_float]
-        L.sho_float2uint_rn.restype = ctypes.c_uint32
-        L.sho_dq_se3.argtypes = [_DP, _DP]
-        L.sho_dq_from_mat34.argtypes = [_DP, _DP]
-        L.sho_twist_exp.argtypes = [_DP, _DP]
-        L.sho_invert6.argtypes = [_DP, _DP]
-        L.sho_max_threads.restype = ctypes.c_int
-        _LIB = L
+        _LIB = _load(_HERE / "_build" / "libsh_oracle.so")
     return _LIB
+
+
+def _load(path):
+    if not path.exists():
+        raise RuntimeError(f"{path} is missing: run `make -C oracle` or `python -m starhub_amd.build`")
+    L = ctypes.CDLL(str(path))
+    L.sho_skin.argtypes = [_FP, ctypes.c_int, _FP, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                           _U16P, _FP, _FP, ctypes.c_int]
+    L.sho_node_graph.argtypes = [_FP, ctypes.c_int, ctypes.c_int, ctypes.c_float, _U16P, _FP, _U16P, _FP,
+                                 ctypes.c_int]
+    L.sho_solve.argtypes = [ctypes.POINTER(ShoInputs), ctypes.POINTER(ShoParams), ctypes.c_int, ctypes.c_int]
+    L.sho_solve.restype = ctypes.c_void_p
+    L.sho_result_get.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p),
+                                 ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int)]
+    L.sho_result_free.argtypes = [ctypes.c_void_p]
+    L.sho_round_to_half.argtypes = [ctypes.c_float]
+    L.sho_round_to_half.restype = ctypes.c_float
+    L.sho_float2uint_rn.argtypes = [ctypes.c_float]
+    L.sho_float2uint_rn.restype = ctypes.c_uint32
+    L.sho_dq_se3.argtypes = [_DP, _DP]
+    L.sho_dq_from_mat34.argtypes = [_DP, _DP]
+    L.sho_twist_exp.argtypes = [_DP, _DP]
+    L.sho_invert6.argtypes = [_DP, _DP]
+    L.sho_max_threads.restype = ctypes.c_int
+    return L
 
 
 def max_threads() -> int:
@@ -108,7 +139,7 @@ _DTYPES = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.float64}
 
 RESULT_FIELDS = ["pixel_pair", "pixel_knn", "n2t_offset", "n2t_terms", "half_keys", "half_offset", "pair_terms",
                  "sym_keys", "sym_range", "row_offset", "residual", "jacobian", "reg_Ti_xj", "reg_Tj_xj",
-                 "reg_valid", "diag", "rhs", "offdiag", "diag_inv", "pcg_x", "cost", "dq_iter", "twist_l1", "dq"]
+                 "reg_valid", "diag", "rhs", "offdiag", "diag_inv", "pcg_x", "cost", "dq_iter", "twist_l1", "rot_cut", "dq"]
 
 
 def _inputs_struct(inputs: dict):
@@ -157,19 +188,36 @@ def _inputs_struct(inputs: dict):
     return c, keep
 
 
-def solve(inputs: dict, params, precision: int = 64, threads: int = 1) -> dict:
-    """Run the restated GN solve.  `inputs` holds numpy arrays (see starhub_amd.workload.host_inputs)."""
-    c, keep = _inputs_struct(inputs)
+def _params(params, rot_cut=None):
     p = ShoParams(params.gn_iterations, params.pcg_iterations, params.lambda_depth, params.lambda_flow,
                   params.lambda_reg, params.lambda_node_translation, params.damping)
-    h = lib().sho_solve(ctypes.byref(c), ctypes.byref(p), precision, threads)
+    if rot_cut is not None:
+        p.rot_cut = rot_cut.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    return p
+
+
+def solve(inputs: dict, params, precision: int = 64, threads: int = 1, rot_cut=None, variant: str = "default") -> dict:
+    """Run the restated GN solve.  `inputs` holds numpy arrays (see starhub_amd.workload.host_inputs).
+
+    rot_cut: optional gn_iterations x num_nodes array of 0/1 decisions replayed in
+    place of the oracle's own rotation cutoff (|w|_1 < 1e-4, SolverIterationData.cu:42)."""
+    c, keep = _inputs_struct(inputs)
+    if rot_cut is not None:
+        rot_cut = np.ascontiguousarray(rot_cut, np.uint8).reshape(-1)
+        N = int(inputs["node_coords"].shape[0])
+        if rot_cut.size != params.gn_iterations * N:
+            raise ValueError(f"rot_cut holds {rot_cut.size} decisions, expected {params.gn_iterations} x {N}")
+        keep.append(rot_cut)
+    p = _params(params, rot_cut)
+    L = lib(variant)
+    h = L.sho_solve(ctypes.byref(c), ctypes.byref(p), precision, threads)
     try:
         out = {}
         for name in RESULT_FIELDS + ["num_pixels", "solve_ms"]:
             ptr = ctypes.c_void_p()
             cnt = ctypes.c_size_t()
             dt = ctypes.c_int()
-            rc = lib().sho_result_get(h, name.encode(), ctypes.byref(ptr), ctypes.byref(cnt), ctypes.byref(dt))
+            rc = L.sho_result_get(h, name.encode(), ctypes.byref(ptr), ctypes.byref(cnt), ctypes.byref(dt))
             if rc != 0:
                 continue
             typ = _DTYPES[dt.value]
@@ -182,7 +230,7 @@ def solve(inputs: dict, params, precision: int = 64, threads: int = 1) -> dict:
         out["solve_ms"] = float(out["solve_ms"][0])
         return out
     finally:
-        lib().sho_result_free(h)
+        L.sho_result_free(h)
 
 
 def dq_se3(dq8) -> np.ndarray:
@@ -213,6 +261,31 @@ def invert6(A) -> np.ndarray:
     return o.reshape(6, 6)
 
 
+def block_pcg(row_ptr, col, blocks, inv_diag, b, x0=None, iters: int = 10, zero_init: bool = True,
+              precision: int = 64):
+    """BlockPCG<6>::Solve over a 6x6 block-CSR (row-major blocks), restated (sho::block_pcg).
+    Returns (x, delta history of iters + 1 values)."""
+    L = lib()
+    if not hasattr(L, "_bpcg_set"):
+        L.sho_block_pcg.argtypes = [ctypes.c_int, _U32P, _U32P, _DP, _DP, _DP, _DP, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, _DP]
+        L._bpcg_set = True
+    row_ptr = np.ascontiguousarray(row_ptr, np.uint32)
+    col = np.ascontiguousarray(col, np.uint32)
+    blocks = np.ascontiguousarray(blocks, np.float64).reshape(-1)
+    inv_diag = np.ascontiguousarray(inv_diag, np.float64).reshape(-1)
+    b = np.ascontiguousarray(b, np.float64).reshape(-1)
+    nblk = len(row_ptr) - 1
+    assert b.size == 6 * nblk and inv_diag.size == 36 * nblk and blocks.size == 36 * int(row_ptr[-1])
+    assert col.size == int(row_ptr[-1]) and (col.size == 0 or int(col.max()) < nblk)
+    x = np.zeros(6 * nblk) if x0 is None else np.array(x0, np.float64).reshape(-1)
+    hist = np.zeros(iters + 1)
+    L.sho_block_pcg(nblk, row_ptr.ctypes.data_as(_U32P), col.ctypes.data_as(_U32P), blocks.ctypes.data_as(_DP),
+                    inv_diag.ctypes.data_as(_DP), b.ctypes.data_as(_DP), x.ctypes.data_as(_DP), iters,
+                    int(zero_init), precision, hist.ctypes.data_as(_DP))
+    return x, hist
+
+
 def round_to_half(f: float) -> float:
     return float(lib().sho_round_to_half(f))
 
@@ -230,8 +303,7 @@ def eval_terms(inputs: dict, params, dq_wxyz: np.ndarray):
         L.sho_eval_terms.restype = ctypes.c_int
         L._eval_set = True
     c, keep = _inputs_struct(inputs)
-    p = ShoParams(params.gn_iterations, params.pcg_iterations, params.lambda_depth, params.lambda_flow,
-                  params.lambda_reg, params.lambda_node_translation, params.damping)
+    p = _params(params)
     H, W = inputs["index_map"].shape
     cap = H * W
     E = inputs["edges"].shape[0]

@@ -37,7 +37,7 @@ HIP_FLAGS = [
     "-Wno-unknown-pragmas",
 ]
 
-HIP_SOURCES = ["sh_kernels.hip", "sh_rows.hip", "sh_tiles.hip", "sh_pcg.hip", "sh_sort.hip", "sh_warp.hip", "sh_measure.hip", "sh_render.hip", "sh_flow.hip",
+HIP_SOURCES = ["sh_kernels.hip", "sh_rows.hip", "sh_tiles.hip", "sh_pcg.hip", "sh_sort.hip", "sh_warp.hip", "sh_measure.hip", "sh_render.hip", "sh_flow.hip", "sh_bpcg.hip",
                "sh_engine.cpp", "sh_server.cpp"]
 
 
@@ -90,10 +90,25 @@ def build_oracle() -> Path | None:
     return ROOT / "oracle" / "_build" / "libsh_oracle.so"
 
 
+def build_dropin_check() -> Path:
+    """tests/cpp/_build/dropin_check: a C++ caller of the drop-in header, run by
+    the GPU tests (test infrastructure; host compiler only, links the library)."""
+    src = ROOT / "tests" / "cpp" / "dropin_check.cpp"
+    out_dir = ROOT / "tests" / "cpp" / "_build"
+    out_dir.mkdir(exist_ok=True)
+    exe = out_dir / "dropin_check"
+    cxx = shutil.which("g++") or "g++"
+    _run([cxx, "-std=c++17", "-O1", "-Wall", "-D__HIP_PLATFORM_AMD__", f"-I{ROOT / 'include'}", "-I/opt/rocm/include",
+          str(src), str(OUT / "libstarhub_amd.so"), "-L/opt/rocm/lib", "-lamdhip64",
+          "-Wl,-rpath,$ORIGIN/../../../starhub_amd/_build:/opt/rocm/lib", "-o", str(exe)])
+    return exe
+
+
 def build_all() -> None:
     build_scene()
     build_oracle()
     build_hip()
+    build_dropin_check()
 
 
 if __name__ == "__main__":

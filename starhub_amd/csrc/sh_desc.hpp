@@ -41,10 +41,18 @@ struct SolveDesc
     const float *node_connect;
     const float4 *node_pred;
 
-    // compacted pixel streams
+    // compacted pixel streams in raster order (the reference's order: Node2Term
+    // term ids and every index kernel use it)
     ushort4 *pix;     // (x, y, tx, ty)
     uint16_t *pknn;   // P*K
     float *pw;        // P*K, spatial * connect
+    // the same streams in the order the GN phase reads them (the per-pixel
+    // records follow it): tile order -- 32x32-pixel blocks in raster order,
+    // Morton order inside a block -- on the tile path, so that 128 consecutive
+    // pixels cover a compact patch; the raster arrays above on the others
+    ushort4 *gpix;
+    uint16_t *gpknn;
+    float *gpw;
     // node -> term CSR
     uint32_t *n2t_off;   // N+1
     uint32_t *n2t_terms; // sorted term ids
@@ -86,6 +94,7 @@ struct SolveDesc
     double *delta;    // pcg_iters + 1 : r.s per iteration
     double *cost;     // weighted cost (dense + ARAP) at the start of each GN iteration, then the final one
     double *cost_part; // work-group partials: ceil(P/256) dense, then ceil((E+N)/256) ARAP
+    uint8_t *rot_cut;  // gn_iterations * N: 1 where the update dropped a node's rotation (|w|_1 < 1e-4)
     // matrix-free scratch
     float *mf_z;      // P*6 per-pixel H_p s_p
     float *mf_reg;    // E*3 per-edge J x
@@ -126,13 +135,14 @@ __device__ __forceinline__ SolveDesc global_desc(const SolveDesc &src)
 #define SH_G(f) d.f = as_global(d.f)
     SH_G(meas_v); SH_G(meas_n); SH_G(ref_v); SH_G(ref_n); SH_G(index_map); SH_G(flow);
     SH_G(surfel_knn); SH_G(surfel_spatial); SH_G(surfel_connect); SH_G(node_coords); SH_G(edges);
-    SH_G(node_connect); SH_G(node_pred); SH_G(pix); SH_G(pknn); SH_G(pw); SH_G(n2t_off); SH_G(n2t_terms);
+    SH_G(node_connect); SH_G(node_pred); SH_G(pix); SH_G(pknn); SH_G(pw); SH_G(gpix); SH_G(gpknn); SH_G(gpw);
+    SH_G(n2t_off); SH_G(n2t_terms);
     SH_G(half_keys); SH_G(half_off); SH_G(pair_entries); SH_G(upper_pos); SH_G(lower_pos); SH_G(row_ptr);
     SH_G(blk_col); SH_G(dense_cnt); SH_G(seg_ptr); SH_G(part); SH_G(tile_cbase); SH_G(tile_sbase);
     SH_G(tc_contrib); SH_G(tc_slot_first); SH_G(list_off); SH_G(list_items); SH_G(slot_pos); SH_G(blk_transpose); SH_G(tpart);
     SH_G(dq[0]); SH_G(dq[1]); SH_G(jr); SH_G(reg); SH_G(ttrans); SH_G(blk); SH_G(dinv); SH_G(rhs);
     SH_G(x); SH_G(r); SH_G(s); SH_G(q); SH_G(dv[0]); SH_G(dv[1]); SH_G(mv[0]); SH_G(mv[1]);
-    SH_G(delta); SH_G(cost); SH_G(cost_part); SH_G(mf_z); SH_G(mf_reg);
+    SH_G(delta); SH_G(cost); SH_G(cost_part); SH_G(rot_cut); SH_G(mf_z); SH_G(mf_reg);
 #undef SH_G
     return d;
 }

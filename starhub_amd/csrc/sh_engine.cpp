@@ -134,6 +134,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
     pcap_ = c.max_width * c.max_height;
     if (uint64_t(pcap_) >= (1ull << 25))
         return SH_E_ARG; // dense pair entries carry the pixel index in 25 bits
+    tcap_ = tile_order_size(c.max_width, c.max_height);
     ncap_ = c.max_nodes;
     ecap_ = ncap_ * uint32_t(K_);
     ucap_ = std::min<uint64_t>(uint64_t(ncap_) * (ncap_ - 1) / 2, uint64_t(ncap_) * 96u);
@@ -158,6 +159,12 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         A(d.pix, size_t(pcap_) * 8);
         A(d.pknn, size_t(pcap_) * K_ * 2);
         A(d.pw, size_t(pcap_) * K_ * 4);
+        A(sl.tpix, size_t(pcap_) * 8);
+        A(sl.tpknn, size_t(pcap_) * K_ * 2);
+        A(sl.tpw, size_t(pcap_) * K_ * 4);
+        d.gpix = d.pix;
+        d.gpknn = d.pknn;
+        d.gpw = d.pw;
         A(d.n2t_off, size_t(ncap_ + 1) * 4);
         A(d.n2t_terms, size_t(n2t_cap_) * 4);
         if (cfg_.flags & SH_FLAG_REFERENCE_INDEX) // normalised: PAIR_MAJOR implies it
@@ -196,11 +203,14 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         A(d.delta, size_t(c.pcg_iterations + 1) * 8);
         A(d.cost, size_t(c.gn_iterations + 2) * 8);
         A(d.cost_part, (size_t(div_up(pcap_, 256)) + div_up(ecap_ + ncap_, 256) + 1) * 8);
+        A(d.rot_cut, size_t(std::max(1u, c.gn_iterations)) * ncap_);
     }
     // shared index-build workspace
     const size_t npix = size_t(pcap_);
     A(ws_flags_, npix * 4);
     A(ws_pos_, npix * 4);
+    A(ws_tflags_, size_t(tcap_) * 4);
+    A(ws_tpos_, size_t(tcap_) * 4);
     A(ws_k16a_, size_t(n2t_cap_) * 2);
     A(ws_k16b_, size_t(n2t_cap_) * 2);
     A(ws_v32a_, size_t(n2t_cap_) * 4);
@@ -228,7 +238,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         SH_HIP(sort_pairs_u16_u32(nullptr, b0, nullptr, nullptr, nullptr, nullptr, n2t_cap_, 16, nullptr));
         SH_HIP(sort_pairs_u32_u32(nullptr, b1, nullptr, nullptr, nullptr, nullptr, uint32_t(pair_cap_), 24, nullptr));
         SH_HIP(exclusive_scan_u32(nullptr, b2, nullptr, nullptr, uint32_t(pair_cap_), nullptr));
-        SH_HIP(exclusive_scan_u32(nullptr, b3, nullptr, nullptr, uint32_t(npix), nullptr));
+        SH_HIP(exclusive_scan_u32(nullptr, b3, nullptr, nullptr, std::max(uint32_t(npix), tcap_), nullptr));
         sort_temp_bytes_ = std::max(std::max(b0, b1), std::max(b2, b3));
         A(sort_temp_, sort_temp_bytes_);
     }
@@ -258,6 +268,8 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         IndexLane &l0 = lanes_[0];
         l0.flags = static_cast<uint32_t *>(ws_flags_);
         l0.pos = static_cast<uint32_t *>(ws_pos_);
+        l0.tflags = static_cast<uint32_t *>(ws_tflags_);
+        l0.tpos = static_cast<uint32_t *>(ws_tpos_);
         l0.k16a = static_cast<uint16_t *>(ws_k16a_);
         l0.k16b = static_cast<uint16_t *>(ws_k16b_);
         l0.v32a = static_cast<uint32_t *>(ws_v32a_);
@@ -290,6 +302,18 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
     SH_HIP(hipEventCreateWithFlags(&index_ev_, hipEventDisableTiming));
     stage_bytes_ = size_t(max_slots) * sizeof(SolveDesc) + size_t(kNumMaps) * (max_slots + 1) * 4;
     SH_HIP(hipHostMalloc(&h_stage_, stage_bytes_, hipHostMallocDefault));
+    // the result view holds max_nodes identity transforms until the first solve
+    // (the reference's constructor, SolverIterationData.cu:78-88 -> :234-247)
+    {
+        std::vector<float> ident(size_t(ncap_) * 8, 0.f);
+        for (uint32_t n = 0; n < ncap_; ++n)
+            ident[size_t(n) * 8 + 3] = 1.f; // q0 = (0, 0, 0, 1), q1 = 0
+        for (Slot &sl : slots_)
+        {
+            SH_HIP(hipMemcpy(sl.d.dq[cfg_.gn_iterations & 1u], ident.data(), ident.size() * 4, hipMemcpyHostToDevice));
+            sl.result_n = ncap_;
+        }
+    }
     return SH_OK;
 }
 
@@ -362,6 +386,13 @@ int Engine::set_inputs(uint32_t slot, const sh_camera *cam, const sh_measure_map
     return SH_OK;
 }
 
+// The tile path reads the pixel streams in tile order; the row-segment and
+// pair-major paths read the records by Node2Term term id, the raster index.
+bool Engine::tile_order() const
+{
+    return !(cfg_.flags & (SH_FLAG_ROW_SEGMENTS | SH_FLAG_PAIR_MAJOR));
+}
+
 // Workspace of an extra index lane (the tile-path arrays only).
 int Engine::alloc_lane(IndexLane &ln, size_t npix)
 {
@@ -373,6 +404,8 @@ int Engine::alloc_lane(IndexLane &ln, size_t npix)
     ln.field = static_cast<type *>(p);
     L(flags, uint32_t, npix * 4);
     L(pos, uint32_t, npix * 4);
+    L(tflags, uint32_t, size_t(tcap_) * 4);
+    L(tpos, uint32_t, size_t(tcap_) * 4);
     L(k16a, uint16_t, size_t(n2t_cap_) * 2);
     L(k16b, uint16_t, size_t(n2t_cap_) * 2);
     L(v32a, uint32_t, size_t(n2t_cap_) * 4);
@@ -406,10 +439,22 @@ int Engine::index_pixels(uint32_t slot, IndexLane &ln, hipStream_t st)
     SolveDesc &d = slots_[slot].d;
     const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
     size_t tb = sort_temp_bytes_;
+    // the GN phase's pixel order: tile order on the tile path, raster on the others
+    Slot &sl = slots_[slot];
+    const bool tiled = tile_order();
+    d.gpix = tiled ? sl.tpix : d.pix;
+    d.gpknn = tiled ? sl.tpknn : d.pknn;
+    d.gpw = tiled ? sl.tpw : d.pw;
     // ln.err is zero here: zeroed at allocation, by every build's compaction
     // and on the rejection path
-    SH_HIP(launch_pixel_flags(d, n, ln.flags, ln.err, st));
+    SH_HIP(launch_pixel_flags(d, n, ln.flags, tiled ? ln.tflags : nullptr, ln.err, st));
     SH_HIP(exclusive_scan_u32(ln.sort_temp, tb, ln.flags, ln.pos, n, st));
+    if (tiled)
+    {
+        tb = sort_temp_bytes_;
+        SH_HIP(exclusive_scan_u32(ln.sort_temp, tb, ln.tflags, ln.tpos, tile_order_size(uint32_t(d.W), uint32_t(d.H)),
+                                  st));
+    }
     const Readback rb{{ln.pos + (n - 1), ln.flags + (n - 1), ln.err, nullptr}, {0, 1, 12, 0}, 3};
     SH_HIP(launch_readback(rb, ln.h_meta, st));
     return SH_OK;
@@ -433,7 +478,7 @@ int Engine::index_rows(uint32_t slot, IndexLane &ln, hipStream_t st)
     d.P = int(P);
     const uint32_t nt = div_up(P, kTilePix);
     const IndexZero z{ln.err, ln.meta, ln.rowlen + d.N, ln.segcnt + d.N, ln.tnslot + nt, ln.tncontrib + nt};
-    SH_HIP(launch_compact_pixels(d, ln.flags, ln.pos, z, st));
+    SH_HIP(launch_compact_pixels(d, ln.flags, ln.pos, d.gpix != d.pix ? ln.tpos : nullptr, z, st));
     // node -> term (Node2TermsIndex::BuildIndex, Node2TermsIndex.cu:191-199,312-360)
     const uint32_t n_kv = P * uint32_t(K_) + 2 * uint32_t(d.E) + uint32_t(d.N);
     SH_HIP(launch_node2term_kv(d, ln.k16a, ln.v32a, st));
@@ -605,8 +650,18 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
     uint32_t *flags = static_cast<uint32_t *>(ws_flags_), *pos = static_cast<uint32_t *>(ws_pos_);
     uint32_t *err = static_cast<uint32_t *>(d_flags_) + 2;
     SH_HIP(hipMemsetAsync(err, 0, 4, st));
-    SH_HIP(launch_pixel_flags(d, n, flags, err, st));
+    const bool tiled = tile_order(); // raster on the pair-major path, which reads records by term id
+    d.gpix = tiled ? sl.tpix : d.pix;
+    d.gpknn = tiled ? sl.tpknn : d.pknn;
+    d.gpw = tiled ? sl.tpw : d.pw;
+    uint32_t *tflags = static_cast<uint32_t *>(ws_tflags_), *tpos = static_cast<uint32_t *>(ws_tpos_);
+    SH_HIP(launch_pixel_flags(d, n, flags, tiled ? tflags : nullptr, err, st));
     SH_HIP(exclusive_scan_u32(sort_temp_, tb, flags, pos, n, st));
+    if (tiled)
+    {
+        tb = sort_temp_bytes_;
+        SH_HIP(exclusive_scan_u32(sort_temp_, tb, tflags, tpos, tile_order_size(uint32_t(d.W), uint32_t(d.H)), st));
+    }
     SH_HIP(hipMemcpyAsync(h_meta_ + 0, pos + (n - 1), 4, hipMemcpyDeviceToHost, st));
     SH_HIP(hipMemcpyAsync(h_meta_ + 1, flags + (n - 1), 4, hipMemcpyDeviceToHost, st));
     SH_HIP(hipMemcpyAsync(h_meta_ + 12, err, 4, hipMemcpyDeviceToHost, st));
@@ -620,7 +675,7 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
     const uint32_t nt = div_up(P, kTilePix);
     const IndexZero z{err, static_cast<uint32_t *>(ws_meta_), static_cast<uint32_t *>(ws_rowlen_) + d.N,
                       static_cast<uint32_t *>(ws_segcnt_) + d.N, lanes_[0].tnslot + nt, lanes_[0].tncontrib + nt};
-    SH_HIP(launch_compact_pixels(d, flags, pos, z, st));
+    SH_HIP(launch_compact_pixels(d, flags, pos, tiled ? tpos : nullptr, z, st));
 
     // node -> term (Node2TermsIndex::BuildIndex, Node2TermsIndex.cu:191-199,312-360)
     const uint32_t n_kv = P * uint32_t(K_) + 2 * uint32_t(d.E) + uint32_t(d.N);
@@ -739,6 +794,31 @@ int Engine::build_tiles(Slot &sl, IndexLane &ln, hipStream_t st)
     return SH_OK;
 }
 
+// Replaces a slot's grown buffer with one of `bytes`, with the growth hook's
+// consent; on failure the slot holds no buffer.
+int Engine::grow_buffer(void **buf, size_t *held, size_t bytes)
+{
+    if (grow_fn_ && !grow_fn_(grow_ctx_, int64_t(bytes) - int64_t(*held)))
+        return SH_E_CAPACITY;
+    hipError_t e = hipSuccess;
+    if (*buf)
+        e = hipFree(*buf);
+    *buf = nullptr;
+    if (e == hipSuccess)
+        e = hipMalloc(buf, bytes);
+    if (e != hipSuccess)
+    {
+        last_hip_error = e;
+        *buf = nullptr;
+        *held = 0;
+        if (grow_fn_)
+            grow_fn_(grow_ctx_, -int64_t(bytes));
+        return SH_E_NOMEM;
+    }
+    *held = bytes;
+    return SH_OK;
+}
+
 // One grown device buffer per slot, cut into the tile arrays.
 int Engine::size_tiles(Slot &sl, uint32_t n_slots, uint32_t n_contrib)
 {
@@ -750,21 +830,8 @@ int Engine::size_tiles(Slot &sl, uint32_t n_slots, uint32_t n_contrib)
     const size_t b_part = up(size_t(n_slots) * kPartStride * 4 + 16); // 128-byte partials, 256-aligned base
     const size_t need = b_contrib + b_first + 2 * b_items + b_part;
     if (need > sl.tile_bytes)
-    {
-        if (sl.tile_buf)
-            SH_HIP(hipFree(sl.tile_buf));
-        sl.tile_buf = nullptr;
-        sl.tile_bytes = 0;
-        const size_t bytes = need + need / 4;
-        const hipError_t e = hipMalloc(&sl.tile_buf, bytes);
-        if (e != hipSuccess)
-        {
-            last_hip_error = e;
-            sl.tile_buf = nullptr;
-            return SH_E_NOMEM;
-        }
-        sl.tile_bytes = bytes;
-    }
+        if (int rc = grow_buffer(&sl.tile_buf, &sl.tile_bytes, need + need / 4); rc != SH_OK)
+            return rc;
     char *b = static_cast<char *>(sl.tile_buf);
     d.tc_contrib = reinterpret_cast<uint16_t *>(b);
     d.tc_slot_first = reinterpret_cast<uint32_t *>(b + b_contrib);
@@ -782,21 +849,8 @@ int Engine::size_partials(Slot &sl)
     d.part_stride = (sl.max_row_len + 1) * 21 + 8;
     const size_t need = std::max<size_t>(size_t(d.n_seg) * d.part_stride * sizeof(float), 256);
     if (need > sl.part_bytes)
-    {
-        if (sl.part_buf)
-            SH_HIP(hipFree(sl.part_buf));
-        sl.part_buf = nullptr;
-        sl.part_bytes = 0;
-        const size_t bytes = need + need / 4;
-        const hipError_t e = hipMalloc(&sl.part_buf, bytes);
-        if (e != hipSuccess)
-        {
-            last_hip_error = e;
-            sl.part_buf = nullptr;
-            return SH_E_NOMEM;
-        }
-        sl.part_bytes = bytes;
-    }
+        if (int rc = grow_buffer(&sl.part_buf, &sl.part_bytes, need + need / 4); rc != SH_OK)
+            return rc;
     d.part = static_cast<float *>(sl.part_buf);
     return SH_OK;
 }
@@ -1118,6 +1172,7 @@ hipError_t Engine::mark(int phase, hipStream_t st)
 
 void Engine::collect_marks()
 {
+    std::lock_guard<std::mutex> lock(phase_mu_);
     for (size_t m = 0; m + 1 < n_marks_; ++m)
     {
         float ms = 0.f;
@@ -1132,6 +1187,7 @@ void Engine::collect_marks()
 
 int Engine::phase_times(double *ms, uint32_t *calls, uint32_t n, bool reset)
 {
+    std::lock_guard<std::mutex> lock(phase_mu_);
     for (uint32_t p = 0; p < n && p < uint32_t(kNumPhases); ++p)
     {
         if (ms)
@@ -1211,6 +1267,7 @@ int Engine::solve(uint32_t ns, hipStream_t st)
     {
         slots_[s].d.gn_iter = int(cfg_.gn_iterations);
         slots_[s].solved = true;
+        slots_[s].result_n = uint32_t(slots_[s].d.N);
     }
     return SH_OK;
 }
@@ -1480,6 +1537,8 @@ int Engine::step(const uint32_t *list, uint32_t n, hipStream_t st)
         Slot &sl = slots_[batch_[s]];
         sl.d.gn_iter += 1;
         sl.solved = sl.d.gn_iter == int(cfg_.gn_iterations);
+        if (sl.solved)
+            sl.result_n = uint32_t(sl.d.N);
     }
     return SH_OK;
 }
@@ -1504,6 +1563,7 @@ int Engine::debug_view(uint32_t slot, const char *name, const void **ptr, size_t
     };
     const V views[] = {
         {"pix", d.pix, P * 4},
+        {"gpix", d.gpix, P * 4},
         {"pknn", d.pknn, P * K},
         {"pw", d.pw, P * K},
         {"n2t_off", d.n2t_off, N + 1},
@@ -1530,6 +1590,7 @@ int Engine::debug_view(uint32_t slot, const char *name, const void **ptr, size_t
         {"dq0", d.dq[0], N * 8},
         {"dq1", d.dq[1], N * 8},
         {"delta", d.delta, size_t(cfg_.pcg_iterations) + 1},
+        {"rot_cut", d.rot_cut, size_t(cfg_.gn_iterations) * N},
     };
     for (const V &v : views)
         if (n == v.name)
@@ -1547,6 +1608,11 @@ int Engine::debug_view(uint32_t slot, const char *name, const void **ptr, size_t
 struct sh_solver
 {
     sh::Engine eng;
+    struct JtjOp
+    {
+        sh_solver *solver;
+        bool matrix_free;
+    } jtj_op[2] = {{this, false}, {this, true}};
 };
 struct sh_batch
 {
@@ -1626,10 +1692,8 @@ extern "C"
     {
         if (!solver || !view || !num_nodes)
             return SH_E_ARG;
-        if (!solver->eng.solved(0))
-            return SH_E_STATE;
         *view = reinterpret_cast<const sh_dual_quat *>(solver->eng.result(0));
-        *num_nodes = uint32_t(solver->eng.desc(0).N);
+        *num_nodes = solver->eng.result_nodes(0);
         return SH_OK;
     }
 
@@ -1719,6 +1783,25 @@ extern "C"
         if (!solver)
             return SH_E_ARG;
         return solver->eng.apply_jtj(0, x, y, matrix_free != 0, reinterpret_cast<hipStream_t>(stream));
+    }
+
+    static int solver_jtj_cb(void *user, const float *x, float *y, sh_stream_t stream)
+    {
+        const sh_solver::JtjOp *op = static_cast<const sh_solver::JtjOp *>(user);
+        return op->solver->eng.apply_jtj(0, x, y, op->matrix_free, reinterpret_cast<hipStream_t>(stream));
+    }
+
+    int sh_solver_spmv_ops(sh_solver *solver, uint32_t matrix_free, sh_spmv_ops *ops)
+    {
+        if (!solver || !ops)
+            return SH_E_ARG;
+        if (!solver->eng.solved(0))
+            return SH_E_STATE;
+        ops->user = &solver->jtj_op[matrix_free ? 1 : 0];
+        ops->matrix_size = 6u * uint32_t(solver->eng.desc(0).N);
+        ops->apply_spmv = solver_jtj_cb;
+        ops->init_residual = nullptr;
+        return SH_OK;
     }
 
     int sh_skin_surfels(const float *surfel_vertex4, uint32_t num_surfels, const float *node_coords4,

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <vector>
 
 #include "../../include/sh/warp_solver.h"
@@ -42,6 +43,16 @@ public:
     int set_tuning(const sh_tuning &t);
     sh_tuning tuning() const;
     int autotune(uint32_t ns, hipStream_t st, uint32_t repeats, sh_tuning *best);
+    // Called before a slot's grown buffer (tile lists and partials, segment
+    // partials) is allocated, with the bytes the engine's holding grows by,
+    // and with the negative size of what it releases; returning false refuses
+    // the growth, and the index build that needed it fails with SH_E_CAPACITY.
+    using GrowthHook = bool (*)(void *ctx, int64_t delta_bytes);
+    void set_growth_hook(GrowthHook fn, void *ctx)
+    {
+        grow_fn_ = fn;
+        grow_ctx_ = ctx;
+    }
     // polls per barrier wait of the persistent PCG kernels (0: the default)
     void set_wait_limit(uint32_t polls) { wait_limit_ = polls; }
     // solves (or steps) re-run with the split PCG after a barrier wait ran out
@@ -56,6 +67,9 @@ public:
     }
     const dq8 *result(uint32_t slot) const;
     bool solved(uint32_t slot) const { return slot < slots_.size() && slots_[slot].solved; }
+    // nodes in the slot's result view: max_nodes identity transforms until the
+    // first solve (SolverIterationData.cu:78-88), then the last solve's count
+    uint32_t result_nodes(uint32_t slot) const { return slot < slots_.size() ? slots_[slot].result_n : 0u; }
     const SolveDesc &desc(uint32_t slot) const { return slots_[slot].d; }
     uint32_t num_slots() const { return uint32_t(slots_.size()); }
     int debug_view(uint32_t slot, const char *name, const void **ptr, size_t *count) const;
@@ -86,6 +100,10 @@ private:
         bool has_inputs = false;
         bool indexed = false;
         bool solved = false;
+        uint32_t result_n = 0;    // nodes in the result view (see result_nodes)
+        ushort4 *tpix = nullptr;  // tile-order pixel streams (the GN phase's on the tile path)
+        uint16_t *tpknn = nullptr;
+        float *tpw = nullptr;
         uint32_t max_row_len = 0;
         uint32_t tile_slot_max = 0; // largest slot count of one tile
         void *part_buf = nullptr; // segment partials, grown on demand
@@ -115,6 +133,7 @@ private:
     struct IndexLane
     {
         uint32_t *flags = nullptr, *pos = nullptr, *v32a = nullptr, *meta = nullptr;
+        uint32_t *tflags = nullptr, *tpos = nullptr; // tile-order flags and their scan
         uint16_t *k16a = nullptr, *k16b = nullptr;
         uint32_t *k32a = nullptr, *k32b = nullptr, *p32a = nullptr;
         uint32_t *bitmap = nullptr, *rowlen = nullptr, *segcnt = nullptr;
@@ -156,6 +175,7 @@ private:
     int K_ = 8;
     bool fp16_ = false;
     uint32_t pcap_ = 0, ncap_ = 0, ecap_ = 0, n2t_cap_ = 0;
+    uint32_t tcap_ = 0; // tile-order index space of the largest image
     uint64_t ucap_ = 0, pair_cap_ = 0;
     std::vector<Slot> slots_;
     std::vector<void *> allocs_;
@@ -167,6 +187,8 @@ private:
     void *ws_bitmap_ = nullptr, *ws_rowlen_ = nullptr, *ws_segcnt_ = nullptr;
     void *ws_tcontrib_ = nullptr, *ws_tslot_block_ = nullptr, *ws_tslot_first_ = nullptr;
     void *ws_tnslot_ = nullptr, *ws_tncontrib_ = nullptr;
+    void *ws_tflags_ = nullptr, *ws_tpos_ = nullptr;
+    bool tile_order() const; // the GN phase reads the tile-order pixel streams
     uint32_t tiles_cap_ = 0;
     std::vector<IndexLane> lanes_;
     int coop_max_groups_ = 0;
@@ -179,6 +201,9 @@ private:
     void *d_slots2_ = nullptr; // pipelined-PCG barrier slots (two sums each)
     void *d_flags_ = nullptr; // [0] PCG epoch base, [1] PCG barrier error
     uint32_t wait_limit_ = 0;
+    GrowthHook grow_fn_ = nullptr;
+    void *grow_ctx_ = nullptr;
+    int grow_buffer(void **buf, size_t *held, size_t bytes);
     uint32_t pcg_fallbacks_ = 0;
     bool barrier_timeout_ = false; // the last finish() saw the PCG barrier flag
     void *sort_temp_ = nullptr;
@@ -208,5 +233,6 @@ private:
     size_t n_marks_ = 0;
     double phase_ms_[kNumPhases] = {0.0};
     uint32_t phase_calls_[kNumPhases] = {0};
+    std::mutex phase_mu_; // the sums are read from other threads while a server steps
 };
 } // namespace sh

@@ -38,16 +38,62 @@ __device__ __forceinline__ uint32_t f2u_rn_sat(float v)
 }
 
 // ============================================================ index build
+// Tile order of a pixel: 32x32-pixel blocks in raster order (nbx blocks per
+// block row), Morton order inside a block.  Indexes in [0, nbx * nby * 1024).
+__device__ __forceinline__ uint32_t spread5(uint32_t v)
+{
+    v &= 31u;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t squeeze5(uint32_t v)
+{
+    v &= 0x55555555u;
+    v = (v | (v >> 1)) & 0x33333333u;
+    v = (v | (v >> 2)) & 0x0f0f0f0fu;
+    v = (v | (v >> 4)) & 0x00ff00ffu;
+    v = (v | (v >> 8)) & 0x0000ffffu;
+    return v & 31u;
+}
+
+__device__ __forceinline__ uint32_t tile_order(uint32_t x, uint32_t y, uint32_t nbx)
+{
+    return (((y >> 5) * nbx + (x >> 5)) << 10) | spread5(x) | (spread5(y) << 1);
+}
+
 // Valid-pixel flags, and the range checks every later index kernel relies on:
 // a valid pixel names a surfel below S whose K node ids are below N, and every
 // ARAP edge names nodes below N.  A pixel that fails is flagged 0 and sets
-// *err; the host rejects the frame before anything reads the ids.
-__global__ void k_pixel_flags(SolveDesc d, uint32_t n, uint32_t *__restrict__ flags, uint32_t *__restrict__ err)
+// *err; the host rejects the frame before anything reads the ids.  With
+// tflags, the threads walk the tile-order index space (nh entries, nbx
+// blocks per block row) and also write each flag at its tile-order index
+// (0 for the padding outside the image), for the tile-order scan.
+__global__ void k_pixel_flags(SolveDesc d, uint32_t n, uint32_t nh, uint32_t nbx, uint32_t *__restrict__ flags,
+                              uint32_t *__restrict__ tflags, uint32_t *__restrict__ err)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = uint32_t(d.N);
     bool bad = false;
-    if (i < n)
+    uint32_t i = h; // raster index of the pixel this thread decides
+    bool inside = h < n;
+    if (tflags)
+    {
+        inside = false;
+        if (h < nh)
+        {
+            const uint32_t b = h >> 10, m = h & 1023u;
+            const uint32_t x = (b % nbx) * 32u + squeeze5(m), y = (b / nbx) * 32u + squeeze5(m >> 1);
+            inside = x < uint32_t(d.W) && y < uint32_t(d.H);
+            i = y * uint32_t(d.W) + x;
+            if (!inside)
+                tflags[h] = 0u;
+        }
+    }
+    if (inside)
     {
         const uint32_t s = d.index_map[i];
         bool valid = s != kInvalidIndex;
@@ -57,8 +103,10 @@ __global__ void k_pixel_flags(SolveDesc d, uint32_t n, uint32_t *__restrict__ fl
             if (d.surfel_knn[size_t(s) * d.K + k] >= N)
                 valid = false, bad = true;
         flags[i] = valid ? 1u : 0u;
+        if (tflags)
+            tflags[h] = valid ? 1u : 0u;
     }
-    if (i < uint32_t(d.E) && (d.edges[3 * size_t(i)] >= N || d.edges[3 * size_t(i) + 1] >= N))
+    if (h < uint32_t(d.E) && (d.edges[3 * size_t(h)] >= N || d.edges[3 * size_t(h) + 1] >= N))
         bad = true;
     if (bad)
         atomicOr(err, 1u);
@@ -70,7 +118,7 @@ __global__ void k_pixel_flags(SolveDesc d, uint32_t n, uint32_t *__restrict__ fl
 // ImageTermKNNFetcher.cu:56-88).
 template <int K>
 __global__ void k_compact_pixels(SolveDesc d, const uint32_t *__restrict__ flags, const uint32_t *__restrict__ pos,
-                                 IndexZero z)
+                                 const uint32_t *__restrict__ tpos, uint32_t nbx, IndexZero z)
 {
     if (blockIdx.x == 0)
     {
@@ -104,12 +152,33 @@ __global__ void k_compact_pixels(SolveDesc d, const uint32_t *__restrict__ flags
     }
     const uint32_t tx = f2u_rn_sat(fx + float(x));
     const uint32_t ty = f2u_rn_sat(fy + float(y));
-    d.pix[o] = make_ushort4(uint16_t(x), uint16_t(y), uint16_t(tx), uint16_t(ty));
+    const ushort4 px = make_ushort4(uint16_t(x), uint16_t(y), uint16_t(tx), uint16_t(ty));
+    uint16_t kn[K];
+    float w[K];
 #pragma unroll
     for (int k = 0; k < K; ++k)
     {
-        d.pknn[size_t(o) * K + k] = d.surfel_knn[size_t(s) * K + k];
-        d.pw[size_t(o) * K + k] = d.surfel_spatial[size_t(s) * K + k] * d.surfel_connect[size_t(s) * K + k];
+        kn[k] = d.surfel_knn[size_t(s) * K + k];
+        w[k] = d.surfel_spatial[size_t(s) * K + k] * d.surfel_connect[size_t(s) * K + k];
+    }
+    d.pix[o] = px;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+    {
+        d.pknn[size_t(o) * K + k] = kn[k];
+        d.pw[size_t(o) * K + k] = w[k];
+    }
+    if (tpos)
+    {
+        // the GN phase's copy, at the pixel's tile-order position
+        const uint32_t ot = tpos[tile_order(x, y, nbx)];
+        d.gpix[ot] = px;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+        {
+            d.gpknn[size_t(ot) * K + k] = kn[k];
+            d.gpw[size_t(ot) * K + k] = w[k];
+        }
     }
 }
 
@@ -357,7 +426,7 @@ __device__ __forceinline__ void load_knn(const SolveDesc &d, uint32_t p, uint16_
 {
     if constexpr (K == 8)
     {
-        const uint4 kk = *reinterpret_cast<const uint4 *>(d.pknn + size_t(p) * 8);
+        const uint4 kk = *reinterpret_cast<const uint4 *>(d.gpknn + size_t(p) * 8);
         const uint32_t v[4] = {kk.x, kk.y, kk.z, kk.w};
 #pragma unroll
         for (int a = 0; a < 4; ++a)
@@ -365,8 +434,8 @@ __device__ __forceinline__ void load_knn(const SolveDesc &d, uint32_t p, uint16_
             kn[2 * a] = uint16_t(v[a] & 0xffffu);
             kn[2 * a + 1] = uint16_t(v[a] >> 16);
         }
-        const float4 w0 = *reinterpret_cast<const float4 *>(d.pw + size_t(p) * 8);
-        const float4 w1 = *reinterpret_cast<const float4 *>(d.pw + size_t(p) * 8 + 4);
+        const float4 w0 = *reinterpret_cast<const float4 *>(d.gpw + size_t(p) * 8);
+        const float4 w1 = *reinterpret_cast<const float4 *>(d.gpw + size_t(p) * 8 + 4);
         w[0] = w0.x;
         w[1] = w0.y;
         w[2] = w0.z;
@@ -381,8 +450,8 @@ __device__ __forceinline__ void load_knn(const SolveDesc &d, uint32_t p, uint16_
 #pragma unroll
         for (int k = 0; k < K; ++k)
         {
-            kn[k] = d.pknn[size_t(p) * K + k];
-            w[k] = d.pw[size_t(p) * K + k];
+            kn[k] = d.gpknn[size_t(p) * K + k];
+            w[k] = d.gpw[size_t(p) * K + k];
         }
     }
 }
@@ -425,7 +494,7 @@ __device__ __forceinline__ double dense_pixel(const SolveDesc &d, uint32_t p, in
         avg.q1.w += w[k] * n.q1.w;
     }
     const rt34 T = dq_se3(avg);
-    const ushort4 px = d.pix[p];
+    const ushort4 px = d.gpix[p];
     const uint32_t src = uint32_t(px.y) * uint32_t(d.W) + px.x;
     const float4 cv = d.ref_v[src];
     const float4 cn = d.ref_n[src];
@@ -818,7 +887,7 @@ __global__ __launch_bounds__(256) void k_offdiag(const SolveDesc *__restrict__ s
             continue;
         const uint32_t p = pk & 0x1ffffffu;
         const uint32_t sa = (pk >> 25) & 7u, sb = (pk >> 28) & 7u;
-        const float ww = d.pw[size_t(p) * K + sa] * d.pw[size_t(p) * K + sb];
+        const float ww = d.gpw[size_t(p) * K + sa] * d.gpw[size_t(p) * K + sb];
         float hg[kHG];
         load_hg<JT>(d.jr, p, hg);
 #pragma unroll
@@ -1363,24 +1432,35 @@ __global__ __launch_bounds__(256) void k_node_graph(const float4 *__restrict__ n
 // ============================================================ host launchers
 #define SH_GRID(n, b) dim3(unsigned(((n) + (b)-1) / (b)))
 
-hipError_t launch_pixel_flags(const SolveDesc &d, uint32_t n, uint32_t *flags, uint32_t *err, hipStream_t st)
+uint32_t tile_order_blocks_x(uint32_t width) { return (width + 31u) / 32u; }
+
+uint32_t tile_order_size(uint32_t width, uint32_t height)
 {
-    const uint32_t m = max(n, uint32_t(d.E));
+    return tile_order_blocks_x(width) * ((height + 31u) / 32u) * 1024u;
+}
+
+hipError_t launch_pixel_flags(const SolveDesc &d, uint32_t n, uint32_t *flags, uint32_t *tflags, uint32_t *err,
+                              hipStream_t st)
+{
+    const uint32_t nh = tflags ? tile_order_size(uint32_t(d.W), uint32_t(d.H)) : 0u;
+    const uint32_t m = max(max(n, nh), uint32_t(d.E));
     if (m == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_pixel_flags, SH_GRID(m, 256), dim3(256), 0, st, d, n, flags, err);
+    hipLaunchKernelGGL(k_pixel_flags, SH_GRID(m, 256), dim3(256), 0, st, d, n, nh,
+                       tile_order_blocks_x(uint32_t(d.W)), flags, tflags, err);
     return hipGetLastError();
 }
 
-hipError_t launch_compact_pixels(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos, const IndexZero &z,
-                                 hipStream_t st)
+hipError_t launch_compact_pixels(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos, const uint32_t *tpos,
+                                 const IndexZero &z, hipStream_t st)
 {
     const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
     const dim3 grid(max(1u, (n + 255u) / 256u)); // block 0 always runs: it clears z
+    const uint32_t nbx = tile_order_blocks_x(uint32_t(d.W));
     if (d.K == 8)
-        hipLaunchKernelGGL(k_compact_pixels<8>, grid, dim3(256), 0, st, d, flags, pos, z);
+        hipLaunchKernelGGL(k_compact_pixels<8>, grid, dim3(256), 0, st, d, flags, pos, tpos, nbx, z);
     else
-        hipLaunchKernelGGL(k_compact_pixels<4>, grid, dim3(256), 0, st, d, flags, pos, z);
+        hipLaunchKernelGGL(k_compact_pixels<4>, grid, dim3(256), 0, st, d, flags, pos, tpos, nbx, z);
     return hipGetLastError();
 }
 

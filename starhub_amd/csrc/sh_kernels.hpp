@@ -9,7 +9,13 @@
 
 namespace sh
 {
-hipError_t launch_pixel_flags(const SolveDesc &d, uint32_t n, uint32_t *flags, uint32_t *err, hipStream_t st);
+// Tile order (the GN phase's pixel order on the tile path): 32x32-pixel
+// blocks in raster order, Morton order inside a block; index space size.
+uint32_t tile_order_blocks_x(uint32_t width);
+uint32_t tile_order_size(uint32_t width, uint32_t height);
+// tflags NULL: raster flags only
+hipError_t launch_pixel_flags(const SolveDesc &d, uint32_t n, uint32_t *flags, uint32_t *tflags, uint32_t *err,
+                              hipStream_t st);
 // Words an index build accumulates into, zeroed by the compaction kernel for
 // the stages after it (and err for the next build on the same workspace).
 struct IndexZero
@@ -21,8 +27,10 @@ struct IndexZero
     uint32_t *tile_nslot_end;    // 2 words: the scan total and the largest count
     uint32_t *tile_ncontrib_end; // 1 word
 };
-hipError_t launch_compact_pixels(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos, const IndexZero &z,
-                                 hipStream_t st);
+// tpos NULL: raster arrays only (the GN phase reads them); else also the
+// tile-order copies (gpix, gpknn, gpw) at their tile-order scan positions
+hipError_t launch_compact_pixels(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos, const uint32_t *tpos,
+                                 const IndexZero &z, hipStream_t st);
 // Up to four device words copied into a pinned host buffer by one small
 // kernel (host[dst[i]] = *src[i]), in place of one copy per word.
 struct Readback

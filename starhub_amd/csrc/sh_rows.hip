@@ -258,8 +258,8 @@ __global__ __launch_bounds__(kRowThreads) void k_rows_seg(const SolveDesc *__res
 #pragma unroll
                 for (int k = 0; k < K; ++k)
                 {
-                    kn[k] = d.pknn[size_t(t) * K + k];
-                    w[k] = d.pw[size_t(t) * K + k];
+                    kn[k] = d.gpknn[size_t(t) * K + k];
+                    w[k] = d.gpw[size_t(t) * K + k];
                 }
                 float hg[kHG];
                 load_hg<JT>(d.jr, t, hg);

@@ -105,6 +105,23 @@ struct sh_server
     void step_loop();
 };
 
+// The engine's growth hook: a slot's tile and partial buffers grow during index
+// builds (to the largest frame it has held), under the same budget as the
+// slots and the resident streams.
+static bool server_grow(void *ctx, int64_t delta)
+{
+    sh_server *s = static_cast<sh_server *>(ctx);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (delta > 0 && s->cfg.hbm_budget_bytes && s->reserved + uint64_t(delta) > s->cfg.hbm_budget_bytes)
+    {
+        s->stats.growth_refused += 1;
+        return false;
+    }
+    s->reserved = uint64_t(int64_t(s->reserved) + delta);
+    s->stats.hbm_reserved = s->reserved;
+    return true;
+}
+
 static int hip_status(hipError_t e)
 {
     if (e != hipSuccess)
@@ -325,6 +342,17 @@ extern "C"
         *out = nullptr;
         sh_server *s = new sh_server();
         s->cfg = *config;
+        if (config->hbm_budget_bytes == SH_HBM_BUDGET_FREE)
+        {
+            // admission by free HBM: what the device has free now, less 1/16 headroom
+            size_t free_b = 0, total_b = 0;
+            if (hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+            {
+                delete s;
+                return SH_E_HIP;
+            }
+            s->cfg.hbm_budget_bytes = uint64_t(free_b) - uint64_t(free_b) / 16;
+        }
         // solve slots: as many as the budget holds, at most max_active
         uint32_t n = config->max_active;
         int rc = SH_E_CAPACITY;
@@ -333,7 +361,7 @@ extern "C"
             s->engine.reset(new sh::Engine());
             rc = s->engine->init(config->solver, n);
             if (rc == SH_OK &&
-                (config->hbm_budget_bytes == 0 || s->engine->device_bytes() <= config->hbm_budget_bytes))
+                (s->cfg.hbm_budget_bytes == 0 || s->engine->device_bytes() <= s->cfg.hbm_budget_bytes))
                 break;
             s->engine.reset();
             if (rc != SH_OK && rc != SH_E_NOMEM)
@@ -363,7 +391,9 @@ extern "C"
             }
         }
         s->reserved = s->engine->device_bytes();
+        s->engine->set_growth_hook(&server_grow, s);
         s->stats.slots = n;
+        s->stats.hbm_budget = s->cfg.hbm_budget_bytes;
         s->stats.hbm_reserved = s->reserved;
         s->slot_req.assign(n, 0);
         for (uint32_t i = 0; i < n; ++i)
@@ -599,6 +629,13 @@ extern "C"
         *live_node4 = it->second.live_node;
         *num_surfels = it->second.num_surfels;
         return SH_OK;
+    }
+
+    int sh_server_phase_times(sh_server *s, double *ms, uint32_t *calls, uint32_t num_phases, uint32_t reset)
+    {
+        if (!s)
+            return SH_E_ARG;
+        return s->engine->phase_times(ms, calls, num_phases, reset != 0);
     }
 
     int sh_server_get_stats(sh_server *s, sh_server_stats *out)

@@ -4,8 +4,9 @@
 // The reference gathers each node pair's terms again for every block
 // (JtJMaterializer.cu:66-175, PreconditionerRhsBuilder.cu:110-262), so every
 // per-pixel record is read once per pair it belongs to.  Here the compacted
-// pixels are cut into tiles of kTilePix consecutive pixels (raster order, so a
-// tile covers a compact image strip and few nodes).
+// pixels are cut into tiles of kTilePix consecutive pixels of the GN phase's
+// pixel order (tile order: 32x32-pixel blocks, Morton order inside a block),
+// so a tile covers a compact image patch and few nodes.
 //
 // Once per solve (index build), one work-group per tile lists the tile's
 // contributions -- (node pair (a <= b) of distinct nodes of a pixel, pixel,
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_build(SolveDesc d, uint16
         uint16_t kn[K];
 #pragma unroll
         for (int k = 0; k < K; ++k)
-            kn[k] = d.pknn[size_t(p) * K + k];
+            kn[k] = d.gpknn[size_t(p) * K + k];
         // slot k is active when no later slot names the same node (last match,
         // dense_image_jacobian.cuh:17-23)
         bool act[K];
@@ -367,7 +368,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
             Hs[tid * (kTileRec / 4) + v] = make_float4(hg[4 * v], hg[4 * v + 1], hg[4 * v + 2], hg[4 * v + 3]);
 #pragma unroll
         for (int k = 0; k < K; ++k)
-            wS[tid * K + k] = d.pw[size_t(p) * K + k];
+            wS[tid * K + k] = d.gpw[size_t(p) * K + k];
     }
     for (uint32_t i = tid; i < nc; i += kTileThreads)
         ent[i] = d.tc_contrib[c0 + i];
@@ -927,8 +928,8 @@ __global__ __launch_bounds__(kTileThreads) void k_mf_tiles(const SolveDesc *__re
 #pragma unroll
         for (int k = 0; k < K; ++k)
         {
-            kn[k] = d.pknn[size_t(p) * K + k];
-            w[k] = d.pw[size_t(p) * K + k];
+            kn[k] = d.gpknn[size_t(p) * K + k];
+            w[k] = d.gpw[size_t(p) * K + k];
             wS[tid * K + k] = w[k];
         }
         float z[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};

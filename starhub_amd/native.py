@@ -8,13 +8,63 @@ library, and loading fails loudly when that library is missing.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
+import sys
+import weakref
 from dataclasses import dataclass
 from pathlib import Path
 
 import numpy as np
 
 _LIB = None
+
+# Every open handle, closed by an atexit hook: interpreter exit then destroys
+# the library's HIP objects (frees, graph executables, events, streams, the
+# server's worker threads) while the HIP runtime -- and any profiler attached
+# to it -- is still up, rather than leaving them to the runtime's own teardown
+# inside exit().  The hook is registered when the library is loaded, after
+# torch, so it runs before any exit hook torch registered (last in, first out).  A __del__ that runs during interpreter finalisation does
+# nothing: the atexit hook has already closed the handle.
+_OPEN = weakref.WeakSet()
+
+
+def close_all() -> None:
+    """Closes every handle still open (solvers, batches, servers, block PCGs)."""
+    for h in list(_OPEN):
+        try:
+            h.close()
+        except Exception:
+            pass
+
+
+class _Handle:
+    """Owner of one native handle: close() destroys it once."""
+
+    _destroy = None  # name of the sh_*_destroy entry point
+
+    def _register(self):
+        _OPEN.add(self)
+
+    def close(self):
+        if self._h:
+            getattr(lib(), self._destroy)(self._h)
+            self._h = ctypes.c_void_p()
+        _OPEN.discard(self)
+
+    def __del__(self):
+        if sys.is_finalizing():
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 SH_OK = 0
 SH_STREAM_FP32, SH_STREAM_FP16 = 0, 1
@@ -85,6 +135,7 @@ def lib():
             raise RuntimeError(f"native solver library {path} is missing: run `python -m starhub_amd.build`")
         import torch  # noqa: F401  -- load the process's HIP runtime before ours resolves against it
         L = ctypes.CDLL(str(path))
+        atexit.register(close_all)
         P = ctypes.POINTER
         L.sh_version.restype = ctypes.c_char_p
         L.sh_status_string.restype = ctypes.c_char_p
@@ -175,6 +226,7 @@ class DeviceFrame:
     num_surfels: int
     num_nodes: int
     num_edges: int
+    node_motion_pred: object | None = None  # N x 4 float32 (NodeFlow4Solver), optional
 
     def structs(self):
         ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p()
@@ -184,7 +236,7 @@ class DeviceFrame:
         geo = Geometry(ptr(self.surfel_knn), ptr(self.surfel_spatial), ptr(self.surfel_connect), self.num_surfels)
         ng = NodeGraph(ptr(self.node_coords), ptr(self.edges), ptr(self.node_connect), ptr(self.node_knn),
                        ptr(self.node_spatial), self.num_nodes, self.num_edges, self.node_radius_square)
-        nf = NodeFlow(ctypes.c_void_p(), self.num_nodes)
+        nf = NodeFlow(ptr(self.node_motion_pred), self.num_nodes)
         fl = FlowMap(ptr(self.flow))
         kp = KeyPoints(0)
         c2w = (ctypes.c_float * 16)(*[float(v) for v in np.asarray(self.cam2world, np.float32).reshape(16)])
@@ -197,25 +249,17 @@ def _stream_ptr(stream) -> ctypes.c_void_p:
     return ctypes.c_void_p(int(stream.cuda_stream))
 
 
-class WarpSolver:
+class WarpSolver(_Handle):
     """One solve handle (sh_solver)."""
+
+    _destroy = "sh_solver_destroy"
 
     def __init__(self, config: SolverConfig):
         self._h = ctypes.c_void_p()
         check(lib().sh_solver_create(ctypes.byref(config), ctypes.byref(self._h)), "sh_solver_create")
+        self._register()
         self.config = config
         self._frame = None
-
-    def close(self):
-        if self._h:
-            lib().sh_solver_destroy(self._h)
-            self._h = ctypes.c_void_p()
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
 
     def set_inputs(self, frame: DeviceFrame) -> None:
         s = frame.structs()
@@ -285,25 +329,17 @@ def _phase_times(fn, handle, reset: bool) -> dict:
     return {L.sh_phase_name(p).decode(): (ms[p], calls[p]) for p in range(NUM_PHASES)}
 
 
-class BatchSolver:
+class BatchSolver(_Handle):
     """Batched uneven solves (sh_batch): one launch per GN phase covers every slot."""
+
+    _destroy = "sh_batch_destroy"
 
     def __init__(self, config: SolverConfig, max_solves: int):
         self._h = ctypes.c_void_p()
         check(lib().sh_batch_create(ctypes.byref(config), max_solves, ctypes.byref(self._h)), "sh_batch_create")
+        self._register()
         self.config = config
         self.max_solves = max_solves
-
-    def close(self):
-        if self._h:
-            lib().sh_batch_destroy(self._h)
-            self._h = ctypes.c_void_p()
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
 
     def set_inputs(self, slot: int, frame: DeviceFrame) -> None:
         s = frame.structs()
@@ -372,6 +408,9 @@ def pcg_fallbacks(solver) -> int:
     return n.value
 
 
+HBM_BUDGET_FREE = (1 << 64) - 1  # SH_HBM_BUDGET_FREE: the free HBM at creation
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("solver", SolverConfig), ("max_active", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64)]
 
@@ -383,12 +422,15 @@ class RequestResult(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("slots", "streams", "streams_refused", "submitted", "completed",
-                                                "failed", "steps", "max_batch", "slot_iterations", "hbm_reserved")]
+                                                "failed", "steps", "max_batch", "slot_iterations", "hbm_reserved",
+                                                "hbm_budget", "growth_refused")]
     _fields_ += [("now_ms", ctypes.c_double)]
 
 
-class Server:
+class Server(_Handle):
     """Serving loop (sh_server): continuous batching of tracking solves from many streams."""
+
+    _destroy = "sh_server_destroy"
 
     def __init__(self, solver_config: SolverConfig, max_active: int, hbm_budget_bytes: int = 0):
         L = lib()
@@ -397,18 +439,8 @@ class Server:
         self._h = ctypes.c_void_p()
         self.cfg = ServerConfig(solver_config, max_active, hbm_budget_bytes)
         check(L.sh_server_create(ctypes.byref(self.cfg), ctypes.byref(self._h)), "sh_server_create")
+        self._register()
         self._frames = {}
-
-    def close(self):
-        if self._h:
-            lib().sh_server_destroy(self._h)
-            self._h = ctypes.c_void_p()
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
 
     def add_stream(self, stream_id: int, frame: "DeviceFrame", ref_vertex=None, ref_normal=None) -> int:
         """Registers a stream; returns the status (SH_E_CAPACITY when the HBM budget refuses it).
@@ -454,6 +486,10 @@ class Server:
         check(lib().sh_server_warp_field(self._h, stream_id, ctypes.byref(view), ctypes.byref(n)),
               "sh_server_warp_field")
         return _device_copy(view.value, n.value * 8, torch.float32).view(n.value, 8)
+
+    def phase_times(self, reset: bool = False) -> dict:
+        """{phase: (total ms, launches)} of the server's GN steps (needs SH_FLAG_PHASE_TIMING)."""
+        return _phase_times(lib().sh_server_phase_times, self._h, reset)
 
     def stats(self) -> dict:
         st = ServerStats()
@@ -618,3 +654,82 @@ def node_motion_from_flow(rgbd_prev4, rgbd4, flow2, index_prev, surfel_knn, surf
                                          ctypes.c_float(cx), ctypes.c_float(cy), num_nodes, p(ws), p(out),
                                          _stream_ptr(stream)), "sh_node_motion_from_flow")
     return out[:num_nodes]
+
+
+# ------------------------------------------------------------ linear-solver plug-in
+SPMV_APPLY_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+SPMV_INIT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_void_p)
+
+
+class SpmvOps(ctypes.Structure):
+    """sh_spmv_ops: the ApplySpMVBase<6> callback table."""
+    _fields_ = [("user", ctypes.c_void_p), ("matrix_size", ctypes.c_uint32), ("apply_spmv", SPMV_APPLY_FN),
+                ("init_residual", SPMV_INIT_FN)]
+
+
+class BlockCsr(_Handle):
+    """sh_spmv over a caller-owned 6x6 block-CSR (torch device tensors, kept alive here)."""
+
+    _destroy = "sh_spmv_destroy"
+
+    def __init__(self, row_ptr, block_col, blocks):
+        self._h = ctypes.c_void_p()
+        self._keep = (row_ptr, block_col, blocks)
+        p = lambda t: ctypes.c_void_p(t.data_ptr())
+        check(lib().sh_spmv_create_block_csr(row_ptr.numel() - 1, block_col.numel(), p(row_ptr), p(block_col),
+                                             p(blocks), ctypes.byref(self._h)), "sh_spmv_create_block_csr")
+        self._register()
+
+    def check(self) -> int:
+        return lib().sh_spmv_check(self._h, ctypes.c_void_p())
+
+    def ops(self) -> SpmvOps:
+        o = SpmvOps()
+        check(lib().sh_spmv_ops_of(self._h, ctypes.byref(o)), "sh_spmv_ops_of")
+        return o
+
+    def apply(self, x, y, stream=None) -> int:
+        return lib().sh_spmv_apply(self._h, ctypes.c_void_p(x), ctypes.c_void_p(y), _stream_ptr(stream))
+
+
+def solver_spmv_ops(solver, matrix_free: bool = False) -> SpmvOps:
+    """sh_solver_spmv_ops: the solver's own last-iteration system as an operator."""
+    o = SpmvOps()
+    check(lib().sh_solver_spmv_ops(solver._h, int(matrix_free), ctypes.byref(o)), "sh_solver_spmv_ops")
+    return o
+
+
+class BlockPCG(_Handle):
+    """sh_block_pcg: BlockPCG<6> (SetSolverInput / Solve / SolveConvergeChecked)."""
+
+    _destroy = "sh_block_pcg_destroy"
+
+    def __init__(self, max_matrix_size: int):
+        self._h = ctypes.c_void_p()
+        check(lib().sh_block_pcg_create(max_matrix_size, ctypes.byref(self._h)), "sh_block_pcg_create")
+        self._register()
+        self._keep = None
+
+    def set_input(self, inv_diag, ops: SpmvOps, b, x, actual_size: int = 0) -> int:
+        self._keep = (inv_diag, ops, b, x)
+        p = lambda t: ctypes.c_void_p(t.data_ptr())
+        return lib().sh_block_pcg_set_input(self._h, p(inv_diag), ctypes.byref(ops), p(b), p(x), actual_size)
+
+    def solve(self, iterations: int = 10, zero_init: bool = True, stream=None) -> None:
+        check(lib().sh_block_pcg_solve(self._h, iterations, int(zero_init), _stream_ptr(stream)), "sh_block_pcg_solve")
+
+    def solve_converge_checked(self, iterations: int, zero_init: bool, epsilon: float, stream=None) -> int:
+        n = ctypes.c_uint32()
+        check(lib().sh_block_pcg_solve_converge_checked(self._h, iterations, int(zero_init), ctypes.c_float(epsilon),
+                                                        _stream_ptr(stream), ctypes.byref(n)),
+              "sh_block_pcg_solve_converge_checked")
+        return n.value
+
+    def delta_history(self) -> np.ndarray:
+        n = ctypes.c_uint32()
+        check(lib().sh_block_pcg_delta_history(self._h, None, 0, ctypes.byref(n)), "sh_block_pcg_delta_history")
+        out = np.zeros(n.value, np.float64)
+        check(lib().sh_block_pcg_delta_history(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n.value,
+                                               ctypes.byref(n)), "sh_block_pcg_delta_history")
+        return out

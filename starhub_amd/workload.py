@@ -64,15 +64,33 @@ def device_frame(scene: Scene, producers: dict | None = None) -> tuple[native.De
     return frame, prod
 
 
-def host_inputs(scene: Scene, prod: dict) -> dict:
+def host_inputs(scene: Scene, prod: dict, node_motion_pred: np.ndarray | None = None) -> dict:
     """Solver inputs as numpy arrays (the CPU restatement's layout)."""
     cfg = scene.cfg
-    return dict(fx=cfg.fx, fy=cfg.fy, cx=cfg.cx, cy=cfg.cy, cam2world=scene.cam2world,
+    return dict(node_motion_pred=node_motion_pred,fx=cfg.fx, fy=cfg.fy, cx=cfg.cx, cy=cfg.cy, cam2world=scene.cam2world,
                 measure_vertex=scene.measure_vertex, measure_normal=scene.measure_normal,
                 ref_vertex=scene.ref_vertex, ref_normal=scene.ref_normal, index_map=scene.index_map,
                 flow=scene.flow, K=cfg.K, surfel_knn=prod["surfel_knn"], surfel_spatial=prod["surfel_spatial"],
                 surfel_connect=prod["surfel_connect"], node_coords=scene.nodes, edges=prod["edges"],
                 node_connect=prod["node_connect"], node_radius_square=scene.node_radius_square)
+
+
+def previous_frame_maps(scene: Scene) -> tuple[np.ndarray, np.ndarray]:
+    """The rgbd maps (float4, w = 1 / depth, 0 where there is none) of the
+    synthetic pair: the previous frame from the reference vertex map (world,
+    brought into the camera), the current one from the measured vertex map
+    (camera frame).  The node-motion prior's inputs besides the flow."""
+    c2w = np.asarray(scene.cam2world, np.float64)
+    R, t = c2w[:3, :3], c2w[:3, 3]
+    ref = scene.ref_vertex[..., :3].astype(np.float64)
+    z_prev = ((ref - t) @ R)[..., 2]
+    valid = scene.index_map != 0xFFFFFFFF
+    prev = np.zeros(scene.ref_vertex.shape, np.float32)
+    prev[..., 3] = np.where(valid & (z_prev > 0), 1.0 / np.where(z_prev > 0, z_prev, 1.0), 0.0)
+    z = scene.measure_vertex[..., 2].astype(np.float64)
+    cur = np.zeros(scene.measure_vertex.shape, np.float32)
+    cur[..., 3] = np.where(z > 0, 1.0 / np.where(z > 0, z, 1.0), 0.0)
+    return prev, cur
 
 
 def solver_for(scene: Scene, spmv_mode: int = native.SH_SPMV_MATERIALIZED, flags: int = 0) -> native.WarpSolver:

@@ -1,0 +1,207 @@
+"""The batched (32 uneven solves) and served (64 streams) workloads at their
+configured sizes, and the node-translation term fed by the node-motion prior,
+through the C-ABI.
+
+* Batch: the 32 solves of the batched workload (N log-uniform in [128, 4096],
+  300 surfels per node, 320x240 .. 1280x720, 6 GN x 10 PCG) in one batch;
+  every member equals its solve alone bit for bit and is within 1e-4 of the
+  FP64 oracle replaying its rotation-cutoff decisions.
+* Serving: the 64 streams of the serving workload (256-4096 nodes, 80k-1.2M
+  surfels) under an HBM budget that holds fewer slots than asked for and
+  only part of the streams; a few hundred requests in Poisson arrival order;
+  every result equals the same frame solved alone (split PCG) bit for bit,
+  and the bytes held never pass the budget, slot buffer growth included.
+* Node translation: a solve with lambda_node_translation > 0 whose prior
+  comes from sh_node_motion_from_flow (NodeMotionHandler.cu:9-21,
+  node_motion_jacobian.cuh:7-35, node_graph_opticalflow.cu:180-245) is
+  within 1e-4 of the replayed FP64 oracle given the same prior, and differs
+  from the solve without the term.
+"""
+from dataclasses import replace
+
+import numpy as np
+import pytest
+from warp_check import gpu_decisions, replay_check, solver_check
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available()
+    torch.cuda.set_device(0)
+    return torch
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("pcg", ["split", "pipelined"])
+def test_batched_workload_full_size(torch_dev, pcg):
+    """Both PCG forms the batch autotuner chooses between: the split PCG
+    (graph-replayed launches) and the pipelined resident PCG in rounds of
+    co-resident solves; solo solves use the same form."""
+    import sho
+    from starhub_amd import native, scene, workload
+    flags, rows = {"split": (native.SH_FLAG_NO_PERSISTENT_PCG, 0), "pipelined": (native.SH_FLAG_PIPELINED_PCG, 16)}[pcg]
+    cfgs = scene.config_c4()
+    say("batched workload: generating", len(cfgs), "scenes")
+    scs = [scene.generate(c) for c in cfgs]
+    frames = [workload.device_frame(s) for s in scs]
+    W = max(c.width for c in cfgs)
+    H = max(c.height for c in cfgs)
+    N = max(c.num_nodes for c in cfgs)
+    bcfg = native.make_config(cfgs[0].solver, W, H, N, 8)
+    batch = native.BatchSolver(bcfg, len(cfgs))
+    native.set_batch_tuning(batch, flags, 0, rows)
+    for i, (f, _) in enumerate(frames):
+        batch.set_inputs(i, f)
+    batch.solve(len(cfgs))
+    say("batch of", len(cfgs), "solved with the", pcg, "PCG")
+    solo = native.WarpSolver(bcfg)
+    native.set_tuning(solo, flags, 0, rows)
+    worst = 0.0
+    for i, (s, (f, prod)) in enumerate(zip(scs, frames)):
+        solo.process_frame(f)
+        a = batch.node_se3(i).cpu().numpy()
+        assert np.array_equal(a, solo.node_se3().cpu().numpy()), i
+        assert native.pcg_fallbacks(solo) == 0
+        if pcg == "split" and i % 4:
+            continue  # the oracle checks every member once (the pipelined run) and every fourth here
+        err, _, flips = solver_check(solo, workload.host_inputs(s, prod), s.cfg.solver, sho.max_threads())
+        worst = max(worst, err)
+        say("member", i, "nodes", s.cfg.num_nodes, "warp error vs replayed fp64", f"{err:.2e}", "flips", flips)
+    say("batched workload: worst warp error", f"{worst:.2e}")
+    solo.close()
+    batch.close()
+
+
+@pytest.mark.timeout(900)
+def test_served_workload_full_size_budgeted(torch_dev):
+    from starhub_amd import native, scene, workload
+    torch = torch_dev
+    cfgs = scene.config_c5_streams()
+    say("serving workload: generating", len(cfgs), "scenes")
+    scs = [scene.generate(c) for c in cfgs]
+    frames = [workload.device_frame(s)[0] for s in scs]
+    refs = [(torch.from_numpy(s.surfel_pos).cuda(), torch.from_numpy(s.surfel_normal).cuda()) for s in scs]
+    W = max(c.width for c in cfgs)
+    H = max(c.height for c in cfgs)
+    N = max(c.num_nodes for c in cfgs)
+    scfg = native.make_config(cfgs[0].solver, W, H, N, 8)
+    # probe: what each stream holds, and how far one slot's buffers grow over all 64 frames
+    p1 = native.Server(scfg, max_active=1)
+    base1 = p1.stats()["hbm_reserved"]
+    stream_bytes = []
+    for i, f in enumerate(frames):
+        r0 = p1.stats()["hbm_reserved"]
+        assert p1.add_stream(i, f, *refs[i]) == 0
+        stream_bytes.append(p1.stats()["hbm_reserved"] - r0)
+    r0 = p1.stats()["hbm_reserved"]
+    for i, f in enumerate(frames):
+        assert p1.wait(p1.submit(i, f))["status"] == 0
+    growth = p1.stats()["hbm_reserved"] - r0
+    p1.close()
+    slots, streams_fit = 32, 48
+    p32 = native.Server(scfg, max_active=slots)
+    engine32 = p32.stats()["hbm_reserved"]  # 32 solve slots and their index lanes
+    p32.close()
+    say("bytes of 32 slots", engine32, "largest slot growth", growth, "all streams", sum(stream_bytes))
+    assert growth > 0 and engine32 > base1
+    # a budget below 64 slots: the server makes fewer, within the budget
+    small = native.Server(scfg, max_active=64, hbm_budget_bytes=engine32)
+    st = small.stats()
+    say("64 asked for, budget of 32:", st["slots"], "slots,", st["hbm_reserved"], "bytes")
+    assert 0 < st["slots"] <= slots and st["hbm_reserved"] <= engine32
+    small.close()
+    # the served run: 32 slots, their growth, and 48 of the 64 streams fit the budget
+    budget = engine32 + slots * growth + sum(stream_bytes[:streams_fit])
+    srv = native.Server(scfg, max_active=slots, hbm_budget_bytes=budget)
+    st = srv.stats()
+    assert st["hbm_budget"] == budget and st["slots"] == slots
+    assert all(srv.add_stream(i, frames[i], *refs[i]) == 0 for i in range(streams_fit))
+    # Poisson arrivals at 30 Hz per stream, submitted in arrival order
+    rng = np.random.default_rng(5)
+    arrivals = []
+    for i in range(streams_fit):
+        t = np.cumsum(rng.exponential(1.0 / 30.0, size=8))
+        arrivals += [(float(x), i) for x in t]
+    arrivals.sort()
+    arrivals = arrivals[:300]
+    outs = [torch.zeros(cfgs[i].num_nodes, 8, dtype=torch.float32, device="cuda") for _, i in arrivals]
+    tickets = [srv.submit(i, frames[i], out) for (_, i), out in zip(arrivals, outs)]
+    results = [srv.wait(t) for t in tickets]
+    torch.cuda.synchronize()
+    assert all(r["status"] == 0 for r in results), [r for r in results if r["status"] != 0][:3]
+    st = srv.stats()
+    say("served", len(results), "requests; stats", st)
+    assert st["max_batch"] > 1 and st["growth_refused"] == 0
+    assert st["hbm_reserved"] <= budget
+    # the other 16 streams: admitted exactly while what is left of the budget holds them
+    room = budget - st["hbm_reserved"]
+    for i in range(streams_fit, len(frames)):
+        want = 0 if stream_bytes[i] <= room else 2  # SH_E_CAPACITY
+        assert srv.add_stream(i, frames[i], *refs[i]) == want, i
+        room -= stream_bytes[i] if want == 0 else 0
+    # more copies of the largest stream until the budget refuses one
+    big = int(np.argmax(stream_bytes))
+    extra = 1000
+    while srv.add_stream(extra, frames[big], *refs[big]) == 0:
+        extra += 1
+        assert extra < 1200
+    st = srv.stats()
+    assert st["streams_refused"] >= 1 and st["hbm_reserved"] <= budget
+    say("streams refused", st["streams_refused"], "held", st["hbm_reserved"], "budget", budget)
+    # every result equals its frame solved alone with the split PCG
+    solo = native.WarpSolver(native.make_config(cfgs[0].solver, W, H, N, 8,
+                                                flags=native.SH_FLAG_NO_PERSISTENT_PCG))
+    alone = {}
+    for i in sorted(set(i for _, i in arrivals)):
+        solo.process_frame(frames[i])
+        alone[i] = solo.node_se3().cpu().numpy()
+        assert np.array_equal(srv.warp_field(i).cpu().numpy(), alone[i]), i
+    for (_, i), out in zip(arrivals, outs):
+        assert np.array_equal(out.cpu().numpy(), alone[i]), i
+    solo.close()
+    srv.close()
+
+
+def test_node_translation_term_with_flow_prior(torch_dev):
+    import sho
+    from starhub_amd import native, scene, workload
+    torch = torch_dev
+    base = scene.config_c1()
+    cfg = replace(base, with_flow=True, solver=replace(base.solver, lambda_node_translation=0.3))
+    sc = scene.generate(cfg)
+    frame, prod = workload.device_frame(sc)
+    prev, cur = workload.previous_frame_maps(sc)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    K = cfg.K
+    pred = native.node_motion_from_flow(t(prev), t(cur), frame.flow, frame.index_map, frame.surfel_knn.view(-1, K),
+                                        frame.surfel_spatial.view(-1, K), sc.cam2world, cfg.fx, cfg.fy, cfg.cx,
+                                        cfg.cy, cfg.num_nodes)
+    pred_h = pred.cpu().numpy()
+    ref_pred = sho.node_motion_from_flow(prev, cur, sc.flow, sc.index_map, prod["surfel_knn"].reshape(-1, K),
+                                         prod["surfel_spatial"].reshape(-1, K), sc.cam2world, cfg.fx, cfg.fy,
+                                         cfg.cx, cfg.cy, cfg.num_nodes)
+    assert np.array_equal(pred_h, ref_pred)
+    moving = int((pred_h[:, 3] != 0).sum())
+    say("nodes with a motion prior", moving, "of", cfg.num_nodes, "mean |motion|",
+        float(np.abs(pred_h[pred_h[:, 3] != 0, :3]).mean()))
+    assert moving > cfg.num_nodes // 4
+    frame.node_motion_pred = pred
+    s = workload.solver_for(sc)
+    s.process_frame(frame)
+    err, _, flips = solver_check(s, workload.host_inputs(sc, prod, pred_h), cfg.solver, sho.max_threads())
+    say("node-translation solve warp error vs replayed fp64", err, "flips", flips)
+    with_term = s.node_se3().cpu().numpy()
+    off = replace(cfg.solver, lambda_node_translation=0.0)
+    s0 = native.WarpSolver(native.make_config(off, cfg.width, cfg.height, cfg.num_nodes, K))
+    s0.process_frame(frame)
+    diff = workload.warp_error(workload.device_dq_to_wxyz(torch.from_numpy(with_term)),
+                               workload.device_dq_to_wxyz(s0.node_se3()))
+    say("effect of the term on the warp field", diff)
+    assert diff > 1e-4

@@ -291,11 +291,14 @@ def test_fp32_restatement_close_to_fp64(tiny_scene):
 
 def test_rotation_cutoff_straddle_explains_fp32_jump():
     """The FP32 restatement tracks FP64 to ~1e-6 until the iteration in which a
-    node's FP64 rotation increment |w|_1 sits within 2 % of the reference's
+    node's FP64 rotation increment |w|_1 sits near the reference's
     1e-4 cutoff (SolverIterationData.cu:42); there FP32 drops (or keeps) the
-    rotation that FP64 keeps (or drops) and that node jumps by ~1e-4.  This
-    is the event the GPU tolerance allows for (warp_tol.py)."""
-    from warp_tol import CUTOFF, WINDOW
+    rotation that FP64 keeps (or drops) and that node jumps by ~1e-4.  When
+    the FP64 solve replays the FP32 solve's decisions instead, the jump is
+    gone: the two agree to ~1e-6 at every iteration.  This is the check the
+    GPU tests make (warp_check.py): flat 1e-4 against the replayed oracle, and
+    replayed decisions that differ from the oracle's own only near the cutoff."""
+    from warp_check import CUTOFF, WINDOW, check_decisions
     cfg = scene.config_batch_member(700, 902)
     sc = scene.generate(cfg)
     N = cfg.num_nodes
@@ -308,9 +311,11 @@ def test_rotation_cutoff_straddle_explains_fp32_jump():
     d64 = r64["dq_iter"].reshape(-1, N, 8)
     d32 = r32["dq_iter"].reshape(-1, N, 8)
     t64 = r64["twist_l1"].reshape(-1, N)
-    t32 = r32["twist_l1"].reshape(-1, N)
+    # the oracle reports the decisions it took: its own cutoff test here
+    assert np.array_equal(r64["rot_cut"].reshape(-1, N).astype(bool), t64 < CUTOFF)
+    cut32 = r32["rot_cut"].reshape(-1, N).astype(bool)
     near = np.abs(t64 - CUTOFF) <= WINDOW * CUTOFF
-    flip = near & ((t32 < CUTOFF) != (t64 < CUTOFF))  # FP32 on the other side
+    flip = near & (cut32 != (t64 < CUTOFF))  # FP32 on the other side
     assert flip.any()
     first = int(np.argmax(flip.any(axis=1)))
     for it in range(first + 1):  # states before the flipped update
@@ -320,3 +325,44 @@ def test_rotation_cutoff_straddle_explains_fp32_jump():
     T32 = workload.dq_to_mat34(d32[first + 1])
     e = np.linalg.norm((T32 - T64).reshape(N, -1), axis=1) / np.linalg.norm(T64.reshape(N, -1), axis=1)
     assert int(np.argmax(e)) == n and 2e-5 < e[n] < 2e-4
+    # replay: FP64 takes the FP32 decisions and the jump disappears
+    rr = sho.solve(inp, cfg.solver, 64, sho.max_threads(), rot_cut=r32["rot_cut"])
+    assert np.array_equal(rr["rot_cut"], r32["rot_cut"])
+    assert check_decisions(rr, r32["rot_cut"]) >= 1
+    dr = rr["dq_iter"].reshape(-1, N, 8)
+    for it in range(len(dr)):
+        assert workload.warp_error(d32[it], dr[it]) < 5e-6, it
+    # and a replayed decision planted far from the cutoff is caught as a bug
+    bad = r32["rot_cut"].copy().reshape(-1, N)
+    far = np.argwhere(np.abs(t64 - CUTOFF) > 0.5 * CUTOFF)[0]
+    bad[far[0], far[1]] ^= 1
+    rb = sho.solve(inp, cfg.solver, 64, sho.max_threads(), rot_cut=bad)
+    with pytest.raises(AssertionError, match="differ from the FP64 oracle"):
+        check_decisions(rb, bad)
+
+
+def test_block_pcg_restatement(sys1):
+    """The plug-in restatement (BlockPCG<6>::Solve over a block-CSR operator):
+    zero-initialised it is the solver's own PCG (the first GN step's x); from
+    the exact solution it stays there; from any x0 it is x0 plus the
+    zero-initialised solve of A e = b - A x0 (the same Krylov iterates), and
+    it is exact after 6N iterations (CG terminates in n steps)."""
+    sc, prod, inp, params, res = sys1
+    N = sc.cfg.num_nodes
+    from block_system import block_csr_of, dense_of
+    rp, col, blk, dinv, b = block_csr_of(res, N)
+    A = dense_of(rp, col, blk)
+    assert np.allclose(A, A.T, rtol=0, atol=1e-12 * np.abs(A).max())
+    x, hist = sho.block_pcg(rp, col, blk, dinv, b, iters=params.pcg_iterations)
+    assert np.allclose(x, res["pcg_x"], rtol=1e-9, atol=1e-12 * np.abs(x).max())
+    assert hist.shape == (params.pcg_iterations + 1,) and hist[-1] < hist[0]
+    x_star = np.linalg.solve(A, b)
+    xs, hs = sho.block_pcg(rp, col, blk, dinv, b, x0=x_star, iters=5, zero_init=False)
+    assert np.allclose(xs, x_star, rtol=0, atol=1e-10 * np.abs(x_star).max()) and hs[0] < 1e-18 * hist[0]
+    rng = np.random.default_rng(4)
+    x0 = rng.standard_normal(6 * N) * np.abs(x_star).max()
+    xn, _ = sho.block_pcg(rp, col, blk, dinv, b, x0=x0, iters=7, zero_init=False)
+    e, _ = sho.block_pcg(rp, col, blk, dinv, b - A @ x0, iters=7)
+    assert np.allclose(xn, x0 + e, rtol=0, atol=1e-9 * np.abs(xn).max())
+    xe, _ = sho.block_pcg(rp, col, blk, dinv, b, x0=x0, iters=6 * N, zero_init=False)
+    assert np.allclose(xe, x_star, rtol=1e-6, atol=1e-8 * np.abs(x_star).max())
