@@ -855,6 +855,9 @@ int Engine::size_partials(Slot &sl)
     return SH_OK;
 }
 
+// Rows of a launch from which the tile combine runs one wave per row.
+constexpr uint32_t kCombineWaveRows = 8192;
+
 // Groups of the register-resident PCG for one solve: rows split evenly, at
 // most `max_rows` (<= kPcgResRows) rows and kPcgResBlocks blocks (rows x
 // longest row) each; 0 when that takes more than kPcgMaxGroups groups.
@@ -995,7 +998,12 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
                                         map_total_[kMapCoop] <= uint32_t(coop_max_groups_)));
     uint32_t *flags = static_cast<uint32_t *>(d_flags_);
     const RowsPlan plan = rows_plan(max_n, max_l);
-    const size_t tiles_lds = tiles_combine_lds(max_n, max_l);
+    // the tile combine: one wave per row when the launch has enough rows to
+    // fill the CUs several times over (batches, server steps); one work-group
+    // per row otherwise, where a single solve's rows finish sooner.  Both give
+    // the same bits.
+    const bool comb_wave = map_total_[kMapRow] >= kCombineWaveRows;
+    const size_t tiles_lds = tiles_combine_lds(max_n, max_l, comb_wave);
     if ((cfg_.flags & SH_FLAG_ROW_SEGMENTS) && std::max(plan.lds, plan.lds_combine) > 160u * 1024u)
         return SH_E_CAPACITY;
     if (!(cfg_.flags & (SH_FLAG_ROW_SEGMENTS | SH_FLAG_PAIR_MAJOR)) && tiles_lds > 160u * 1024u)
@@ -1027,7 +1035,7 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
             SH_HIP(mark(kPhTiles, st));
             SH_HIP(launch_tiles(K_, fp16, sds, map(kMapTile, ns), n, map_total_[kMapTile], slot_cap, st));
             SH_HIP(mark(kPhTilesCombine, st));
-            SH_HIP(launch_tiles_combine(sds, map(kMapRow, ns), n, map_total_[kMapRow], tiles_lds,
+            SH_HIP(launch_tiles_combine(sds, map(kMapRow, ns), n, map_total_[kMapRow], tiles_lds, comb_wave,
                                         static_cast<PcgSlot2 *>(d_slots2_), pipe_slots, st));
         }
         if (!tiles_path) // the tile combine inverts the diagonal blocks itself

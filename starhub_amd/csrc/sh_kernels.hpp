@@ -198,7 +198,8 @@ hipError_t launch_lower_bound_u32(const uint32_t *keys, uint32_t n_keys, uint32_
                                   hipStream_t st);
 hipError_t launch_tiles(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
                         uint32_t slot_cap, hipStream_t st);
-size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len);
+// LDS of the tile combine: one work-group per row (wave = false) or one wave per row
+size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len, bool wave);
 // slot_pos[list_items[l]] = l for l < n: the list position of every slot
 hipError_t launch_slot_pos(const uint32_t *list_items, uint32_t n, uint32_t *slot_pos, hipStream_t st);
 // matrix-free J^T W J x (x = d_{k+1} of each solve unless x_user; y = d.q unless y_user)
@@ -211,7 +212,7 @@ hipError_t launch_spmv_rows(const SolveDesc *sds, const uint32_t *wmap, int n_so
 // Also the FP64 inverse diagonal blocks (as k_dinv) and the zeroed state of a
 // pipelined PCG launch that follows (slots2[0 .. n_slots2), may be null).
 hipError_t launch_tiles_combine(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, size_t lds,
-                                PcgSlot2 *slots2, uint32_t n_slots2, hipStream_t st);
+                                bool wave, PcgSlot2 *slots2, uint32_t n_slots2, hipStream_t st);
 
 // rocPRIM wrappers (sh_sort.hip); temp == nullptr queries the temp size
 hipError_t sort_pairs_u16_u32(void *temp, size_t &temp_bytes, const uint16_t *kin, uint16_t *kout,

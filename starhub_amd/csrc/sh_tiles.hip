@@ -11,9 +11,10 @@
 // Once per solve (index build), one work-group per tile lists the tile's
 // contributions -- (node pair (a <= b) of distinct nodes of a pixel, pixel,
 // slot of a, slot of b) -- keys them by (upper block of the pair, pixel) and
-// radix-sorts them in LDS.  Equal blocks form a slot; slots are numbered
-// across tiles in tile order, and a stable sort of the slots by block gives
-// every block the list of its slot partials in tile order.
+// radix-sorts them in LDS.  Equal blocks within a group of kSlotPix pixels
+// form a slot; slots are numbered across tiles in tile order, and a stable
+// sort of the slots by block gives every block the list of its slot partials
+// in tile order.
 //
 // Per GN iteration, one work-group per tile stages its records, weights and
 // contribution list in LDS and sums every slot's contributions in pixel
@@ -31,6 +32,15 @@ namespace sh
 {
 constexpr int kTileThreads = 256;
 constexpr int kTileRec = 28; // staged record: 21 H + 6 g + pad = 7 float4
+// A slot is a block's contributions from one group of kSlotPix consecutive
+// pixels of a tile (keys are block << 7 | pixel), so a slot's list is at most
+// kSlotPix long and k_tiles' longest serial sum is bounded.
+#ifndef SH_SLOT_SHIFT
+#define SH_SLOT_SHIFT 6
+#endif
+constexpr int kSlotShift = SH_SLOT_SHIFT;
+constexpr int kSlotPix = 1 << kSlotShift;
+static_assert((1 << kSlotShift) == kSlotPix && kSlotPix <= kTilePix, "slot pixel groups");
 
 template <int K>
 struct TileGeom
@@ -213,8 +223,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_build(SolveDesc d, uint16
     const unsigned end_bit = min(32u, 32u - unsigned(__clz(nb_keys)));
     Sort().sort(keys, vals, sort_storage, 0u, end_bit);
 
-    // slot heads: item gi = tid * kItems + i starts a slot when its block
-    // differs from the previous item's (blocked arrangement after the sort)
+    // slot heads: item gi = tid * kItems + i starts a slot when its block or
+    // its pixel group (kSlotPix consecutive pixels of the tile) differs from
+    // the previous item's (blocked arrangement after the sort), so no slot
+    // holds more than kSlotPix contributions
     last_key[tid] = keys[G::kItems - 1];
     __syncthreads();
     const uint32_t prev0 = tid > 0 ? last_key[tid - 1] : 0xffffffffu;
@@ -223,7 +235,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_build(SolveDesc d, uint16
     for (int i = 0; i < G::kItems; ++i)
     {
         const bool valid = keys[i] != 0xffffffffu;
-        nhead += (valid && (prev == 0xffffffffu || (keys[i] >> 7) != (prev >> 7))) ? 1u : 0u;
+        nhead += (valid && (prev == 0xffffffffu || (keys[i] >> kSlotShift) != (prev >> kSlotShift))) ? 1u : 0u;
         nvalid += valid ? 1u : 0u;
         prev = keys[i];
     }
@@ -238,7 +250,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_build(SolveDesc d, uint16
     {
         const uint32_t gi = tid * G::kItems + i;
         const bool valid = keys[i] != 0xffffffffu;
-        const bool head = valid && (prev == 0xffffffffu || (keys[i] >> 7) != (prev >> 7));
+        const bool head = valid && (prev == 0xffffffffu || (keys[i] >> kSlotShift) != (prev >> kSlotShift));
         if (valid)
             cout[gi] = uint16_t((keys[i] & 127u) | (uint32_t(vals[i]) << 7));
         if (head)
@@ -319,15 +331,15 @@ __global__ void k_lower_bound_u32(const uint32_t *__restrict__ keys, uint32_t n_
 // ------------------------------------------------------------ GN phase
 // One work-group per tile.  The records (28 floats: 21 H, 6 g, pad) and the
 // folded weights of the tile's pixels are staged in LDS with the tile's
-// contribution list.  Work items are (slot, float4 group) tasks; each sums its
-// slot's contributions in list order (increasing pixel), coef * record.  The
-// slots are visited longest list first (a counting sort of the list lengths in
-// LDS), so the lanes of a wave run lists of about the same length; which lane
-// sums a slot does not change its result.  Contributions are loaded in batches
-// of kTileBatch, the last one predicated, so the dependent LDS reads of a
-// batch overlap.  The slot lists take LDS for the launch's largest tile only,
-// which lets five tiles share a CU.
-constexpr int kTileBatch = 2; // measured on C2: 2 -> 37.0 us, 4 -> 38.1, 8 -> 42.7, 16 -> 48.8 per launch
+// contribution list.  A thread sums a whole slot, all seven float4 groups, over
+// the slot's contributions in list order (increasing pixel), coef * record:
+// each contribution's entry and two weights are read once for its 28 products
+// (the earlier form, one task per (slot, group), read them seven times and
+// spent about five VALU instructions per product on decoding, addressing and
+// predication).  The slots are visited longest list first (a counting sort of
+// the list lengths in LDS), so the lanes of a wave run lists of about the
+// same length; which lane sums a slot does not change its result.  The slot
+// lists take LDS for the launch's largest tile only.
 
 template <int K, class JT>
 __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restrict__ sds,
@@ -406,39 +418,40 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
         perm[atomicAdd(&hist[kTilePix - (sst[s + 1] - sst[s])], 1u)] = uint16_t(s);
     __syncthreads();
 
-    // each slot's partial goes to its list position, so a block's partials
-    // are contiguous for the combine
+    // Each thread sums whole slots, all 27 values (7 float4) at once, over the
+    // slot's contributions in list (pixel) order: the contribution's entry and
+    // weights are read once for all seven groups.  A slot's partial goes to its
+    // list position, so a block's partials are contiguous for the combine.
     float4 *__restrict__ out = reinterpret_cast<float4 *>(d.tpart);
-    for (uint32_t task = tid; task < ns * (kTileRec / 4); task += kTileThreads)
+    for (uint32_t k = tid; k < ns; k += kTileThreads)
     {
-        const uint32_t s = perm[task / (kTileRec / 4)], g = task % (kTileRec / 4);
+        const uint32_t s = perm[k];
         const uint32_t dst = d.slot_pos[s0 + s];
-        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 acc[kTileRec / 4];
+#pragma unroll
+        for (int g = 0; g < kTileRec / 4; ++g)
+            acc[g] = make_float4(0.f, 0.f, 0.f, 0.f);
         const uint32_t e1 = sst[s + 1];
-        for (uint32_t e = sst[s]; e < e1; e += kTileBatch)
+        uint32_t c = ent[sst[s]];
+        for (uint32_t e = sst[s]; e < e1; ++e)
         {
-            // entries past the list end read pixel 0 and leave the sum unchanged
-            float coef[kTileBatch];
-            float4 h[kTileBatch];
+            const uint32_t pl = c & 127u, ka = (c >> 7) & 7u, kb = (c >> 10) & 7u;
+            c = ent[min(e + 1, e1 - 1)]; // the next entry, ahead of this one's sums
+            const float coef = wS[pl * K + ka] * wS[pl * K + kb];
+            const float4 *__restrict__ h = Hs + pl * (kTileRec / 4);
 #pragma unroll
-            for (int j = 0; j < kTileBatch; ++j)
+            for (int g = 0; g < kTileRec / 4; ++g)
             {
-                const uint32_t c = e + j < e1 ? uint32_t(ent[e + j]) : 0u;
-                const uint32_t pl = c & 127u, ka = (c >> 7) & 7u, kb = (c >> 10) & 7u;
-                coef[j] = wS[pl * K + ka] * wS[pl * K + kb];
-                h[j] = Hs[pl * (kTileRec / 4) + g];
-            }
-#pragma unroll
-            for (int j = 0; j < kTileBatch; ++j)
-            {
-                const bool on = e + j < e1;
-                acc.x = on ? fmaf(coef[j], h[j].x, acc.x) : acc.x;
-                acc.y = on ? fmaf(coef[j], h[j].y, acc.y) : acc.y;
-                acc.z = on ? fmaf(coef[j], h[j].z, acc.z) : acc.z;
-                acc.w = on ? fmaf(coef[j], h[j].w, acc.w) : acc.w;
+                const float4 v = h[g];
+                acc[g].x = fmaf(coef, v.x, acc[g].x);
+                acc[g].y = fmaf(coef, v.y, acc[g].y);
+                acc[g].z = fmaf(coef, v.z, acc[g].z);
+                acc[g].w = fmaf(coef, v.w, acc[g].w);
             }
         }
-        out[size_t(dst) * (kPartStride / 4) + g] = acc;
+#pragma unroll
+        for (int g = 0; g < kTileRec / 4; ++g)
+            out[size_t(dst) * (kPartStride / 4) + g] = acc[g];
     }
 }
 
@@ -470,6 +483,97 @@ struct RegTerm
     int col;            // block column of the other node, -1 when it is the row node itself
 };
 
+// The parameters of ARAP edge or node-translation term t of row i's list
+// (node_graph_reg_jacobian.cuh:166-358, node_motion_jacobian.cuh:7-35);
+// col_of(j) gives node j's block position in row i, for an edge's cross block.
+template <class ColOf>
+__device__ __forceinline__ RegTerm stage_reg_term(const SolveDesc &d, uint32_t i, uint32_t t, ColOf col_of)
+{
+    RegTerm r{};
+    const uint32_t P = d.P, E = d.E;
+    if (t < P + E)
+    {
+        const uint32_t e = t - P;
+        const float4 *rec = reinterpret_cast<const float4 *>(d.reg + size_t(e) * kReg);
+        const float4 u0 = rec[0], u1 = rec[1];
+        const uint32_t ex = d.edges[3 * e], ey = d.edges[3 * e + 1];
+        const bool is_x = (ex == i);
+        const uint32_t other = is_x ? ey : ex;
+        const float pa[3] = {u0.x, u0.y, u0.z}, pb[3] = {u0.w, u1.x, u1.y};
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+        {
+            r.pm[q] = is_x ? pa[q] : pb[q];
+            r.po[q] = is_x ? pb[q] : pa[q];
+        }
+        r.sm = is_x ? 1.f : -1.f;
+        r.ws = (u1.w == 0.f) ? 0.f : d.lam2_reg * (u1.z * u1.z);
+        r.col = other != i ? col_of(other) : -1; // every ARAP edge of the row is a block of it
+    }
+    else
+    {
+        const uint32_t m = t - P - E;
+        const float4 pred = d.node_pred ? d.node_pred[m] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 tt = d.ttrans[m];
+        r.sm = 2.f;
+        r.ws = d.lam2_nt * (pred.w * pred.w);
+        r.tv[0] = tt.x - pred.x;
+        r.tv[1] = tt.y - pred.y;
+        r.tv[2] = tt.z - pred.z;
+        r.col = -1;
+    }
+    return r;
+}
+
+// Term r's share of row entry ent (0..35 the diagonal block, 36..71 the
+// cross block, 72..77 g): the diagonal and g shares are added to diag and gq,
+// the cross share straight into the edge's block of regblk (entry ent - 36 of
+// every block belongs to one lane, so the adds of a wave do not collide).
+__device__ __forceinline__ void reg_term_entry(const RegTerm &r, int ent, float &diag, float &gq,
+                                               float *__restrict__ regblk)
+{
+    if (r.sm == 2.f)
+    {
+        // node translation (node_motion_jacobian.cuh:7-35): diagonal a = b = 3..5
+        if (ent < 36 && ent / 6 == ent % 6 && ent / 6 >= 3)
+            diag += r.ws;
+        else if (ent >= 72 && ent - 72 >= 3)
+            gq += r.ws * r.tv[ent - 75];
+        return;
+    }
+    if (r.ws == 0.f)
+        return;
+    const f3 pm = mk3(r.pm[0], r.pm[1], r.pm[2]), po = mk3(r.po[0], r.po[1], r.po[2]);
+    if (ent < 72)
+    {
+        // ARAP rows x side [-[a]x, I], y side [[b]x, -I] (node_graph_reg_jacobian.cuh:250-275)
+        const int ab = ent < 36 ? ent : ent - 36;
+        const int aa = ab / 6, bb = ab % 6;
+        float v = 0.f;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch)
+        {
+            const float ga = r.sm * reg_elem_t(pm, ch, aa);
+            const float gb = ent < 36 ? r.sm * reg_elem_t(pm, ch, bb) : -r.sm * reg_elem_t(po, ch, bb);
+            v += ga * gb;
+        }
+        if (ent < 36)
+            diag += r.ws * v;
+        else if (r.col >= 0)
+            regblk[r.col * 36 + ab] += r.ws * v;
+    }
+    else
+    {
+        const int q = ent - 72;
+        const float res[3] = {r.pm[0] - r.po[0], r.pm[1] - r.po[1], r.pm[2] - r.po[2]};
+        float v = 0.f;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch)
+            v += (r.sm * res[ch]) * (r.sm * reg_elem_t(pm, ch, q));
+        gq += r.ws * v;
+    }
+}
+
 // Loads the terms [beg, end) of row i's list, 256 at a time, one per thread,
 // and adds them into regblk (diagonal block at column 0, cross blocks at
 // colmap[other]) and g[0..5], each entry summed over the terms in list order.
@@ -479,47 +583,11 @@ __device__ void row_reg_terms(const SolveDesc &d, uint32_t i, uint32_t beg, uint
                               RegTerm *__restrict__ rt)
 {
     const uint32_t tid = threadIdx.x;
-    const uint32_t P = d.P, E = d.E;
     for (uint32_t c0 = beg; c0 < end; c0 += kTileThreads)
     {
         const uint32_t n = min(uint32_t(kTileThreads), end - c0);
         if (tid < n)
-        {
-            RegTerm r{};
-            const uint32_t t = d.n2t_terms[c0 + tid];
-            if (t < P + E)
-            {
-                const uint32_t e = t - P;
-                const float4 *rec = reinterpret_cast<const float4 *>(d.reg + size_t(e) * kReg);
-                const float4 u0 = rec[0], u1 = rec[1];
-                const uint32_t ex = d.edges[3 * e], ey = d.edges[3 * e + 1];
-                const bool is_x = (ex == i);
-                const uint32_t other = is_x ? ey : ex;
-                const float pa[3] = {u0.x, u0.y, u0.z}, pb[3] = {u0.w, u1.x, u1.y};
-#pragma unroll
-                for (int q = 0; q < 3; ++q)
-                {
-                    r.pm[q] = is_x ? pa[q] : pb[q];
-                    r.po[q] = is_x ? pb[q] : pa[q];
-                }
-                r.sm = is_x ? 1.f : -1.f;
-                r.ws = (u1.w == 0.f) ? 0.f : d.lam2_reg * (u1.z * u1.z);
-                r.col = (other != i) ? int(colmap[other]) : -1;
-            }
-            else
-            {
-                const uint32_t m = t - P - E;
-                const float4 pred = d.node_pred ? d.node_pred[m] : make_float4(0.f, 0.f, 0.f, 0.f);
-                const float4 tt = d.ttrans[m];
-                r.sm = 2.f;
-                r.ws = d.lam2_nt * (pred.w * pred.w);
-                r.tv[0] = tt.x - pred.x;
-                r.tv[1] = tt.y - pred.y;
-                r.tv[2] = tt.z - pred.z;
-                r.col = -1;
-            }
-            rt[tid] = r;
-        }
+            rt[tid] = stage_reg_term(d, i, d.n2t_terms[c0 + tid], [&](uint32_t j) { return int(colmap[j]); });
         __syncthreads();
         // threads 0..35: diagonal entries, 36..71: cross entries, 72..77: g
         if (tid < 78)
@@ -527,49 +595,7 @@ __device__ void row_reg_terms(const SolveDesc &d, uint32_t i, uint32_t beg, uint
             const int ent = int(tid);
             float diag = 0.f, gq = 0.f;
             for (uint32_t k = 0; k < n; ++k)
-            {
-                const RegTerm &r = rt[k];
-                if (r.sm == 2.f)
-                {
-                    // node translation (node_motion_jacobian.cuh:7-35): diagonal a = b = 3..5
-                    if (ent < 36 && ent / 6 == ent % 6 && ent / 6 >= 3)
-                        diag += r.ws;
-                    else if (ent >= 72 && ent - 72 >= 3)
-                        gq += r.ws * r.tv[ent - 75];
-                    continue;
-                }
-                if (r.ws == 0.f)
-                    continue;
-                const f3 pm = mk3(r.pm[0], r.pm[1], r.pm[2]), po = mk3(r.po[0], r.po[1], r.po[2]);
-                if (ent < 72)
-                {
-                    // ARAP rows x side [-[a]x, I], y side [[b]x, -I] (node_graph_reg_jacobian.cuh:250-275)
-                    const int ab = ent < 36 ? ent : ent - 36;
-                    const int aa = ab / 6, bb = ab % 6;
-                    float v = 0.f;
-#pragma unroll
-                    for (int ch = 0; ch < 3; ++ch)
-                    {
-                        const float ga = r.sm * reg_elem_t(pm, ch, aa);
-                        const float gb = ent < 36 ? r.sm * reg_elem_t(pm, ch, bb) : -r.sm * reg_elem_t(po, ch, bb);
-                        v += ga * gb;
-                    }
-                    if (ent < 36)
-                        diag += r.ws * v;
-                    else if (r.col >= 0)
-                        regblk[r.col * 36 + ab] += r.ws * v;
-                }
-                else
-                {
-                    const int q = ent - 72;
-                    const float res[3] = {r.pm[0] - r.po[0], r.pm[1] - r.po[1], r.pm[2] - r.po[2]};
-                    float v = 0.f;
-#pragma unroll
-                    for (int ch = 0; ch < 3; ++ch)
-                        v += (r.sm * res[ch]) * (r.sm * reg_elem_t(pm, ch, q));
-                    gq += r.ws * v;
-                }
-            }
+                reg_term_entry(rt[k], ent, diag, gq, regblk);
             if (ent < 36)
                 regblk[ent] += diag;
             else if (ent >= 72)
@@ -578,6 +604,7 @@ __device__ void row_reg_terms(const SolveDesc &d, uint32_t i, uint32_t beg, uint
         __syncthreads();
     }
 }
+
 
 // List entries fetched per round of dependent loads (matrix-free row kernel).
 constexpr int kBatch = 8;
@@ -591,6 +618,7 @@ constexpr uint32_t kChunk = 16;
 constexpr uint32_t kChunkCap = 128;
 constexpr uint32_t kCombUnionF4 = kChunkCap * (kTileRec / 4); // float4s of the shared region
 static_assert(sizeof(RegTerm) * kTileThreads <= kCombUnionF4 * sizeof(float4), "combine LDS union");
+
 
 // Sum of float4 group g over list positions [l0, l1), at most kChunk of them,
 // in list order; k_tiles stored each slot's partial at its list position, so
@@ -697,7 +725,7 @@ __device__ void invert6_fp64_wave(const float *__restrict__ A, double (*gm)[12],
 // ARAP/translation g).  The diagonal block and the upper blocks (columns above
 // i) are summed; each upper block (i, j) is also written transposed into
 // (j, i), so every block's slot partials are read once.
-__global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc *__restrict__ sds,
+__global__ __launch_bounds__(kTileThreads) void k_tiles_combine_wg(const SolveDesc *__restrict__ sds,
                                                                 const uint32_t *__restrict__ wg_begin, int n_solve,
                                                                 PcgSlot2 *__restrict__ slots2, uint32_t n_slots2)
 {
@@ -890,6 +918,203 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine(const SolveDesc 
         if (tid >= 128 && tid < 140)
             d.mv[(tid - 128) / 6][size_t(i) * 6 + (tid - 128) % 6] = 0.0;
     }
+}
+
+
+// One wave per node row: the same sums as k_tiles_combine_wg, in the same
+// order and therefore to the same bits -- every block's slot partials summed
+// (chunks in list order, then the chunk sums in order), the ARAP terms
+// (node_graph_reg_jacobian.cuh:166-358) and the node-translation term
+// (node_motion_jacobian.cuh:7-35) of the row added in list order, rounds of
+// 256 terms, the damping on the diagonal; full 6x6 blocks, rhs and the FP64
+// inverse of the diagonal block.  A row is a few hundred values of work, so
+// one wave holds it whole: its steps are ordered by wave-level LDS syncs, with
+// no work-group barrier, and many rows share a CU.  For launches with many
+// rows (batches, the server's steps); a single solve's few rows finish sooner
+// with the work-group form.
+__global__ __launch_bounds__(64) void k_tiles_combine_wave(const SolveDesc *__restrict__ sds,
+                                                           const uint32_t *__restrict__ wg_begin, int n_solve,
+                                                           PcgSlot2 *__restrict__ slots2, uint32_t n_slots2)
+{
+    extern __shared__ float4 smem4[];
+    __shared__ float dg[36];     // the diagonal block, for its inverse
+    __shared__ double gm[6][12]; // its augmented matrix during the inversion
+    __shared__ float red[16];    // g of the diagonal slots (0..5), of the terms (8..13)
+    __shared__ RegTerm rt[64];   // one sub-round of ARAP / node-translation terms
+    // a pipelined PCG that follows publishes into zeroed slots (the row map
+    // has a work-group for every one of its groups)
+    if (slots2 && blockIdx.x < n_slots2 && threadIdx.x < 4)
+        slots2[blockIdx.x].v[threadIdx.x >> 1][threadIdx.x & 1] = 0.0;
+    const int si = find_solve_t(wg_begin, n_solve, blockIdx.x);
+    const SolveDesc d = global_desc(sds[si]);
+    const uint32_t i = blockIdx.x - wg_begin[si];
+    if (i >= uint32_t(d.N))
+        return; // uniform over the wave
+    const uint32_t lane = threadIdx.x;
+    const uint32_t base = d.row_ptr[i] + i;
+    const uint32_t L1 = d.row_ptr[i + 1] - d.row_ptr[i] + 1;
+    constexpr uint32_t kG = kTileRec / 4;
+    // LDS: run[L1][7] | regblk[L1][36] | cols[L1] | ls[L1] | le[L1]
+    float4 *run = smem4;
+    float *regblk = reinterpret_cast<float *>(run + size_t(L1) * kG);
+    uint32_t *cols = reinterpret_cast<uint32_t *>(regblk + size_t(L1) * 36);
+    uint32_t *ls = cols + L1;
+    uint32_t *le = ls + L1;
+
+    // columns: the diagonal first, then the ascending neighbours; how many lie
+    // below i (the upper blocks start after them)
+    uint32_t n_below = 0;
+    for (uint32_t c0 = 0; c0 < L1; c0 += 64)
+    {
+        const uint32_t c = c0 + lane;
+        bool below = false;
+        if (c < L1)
+        {
+            const uint32_t col = d.blk_col[base + c];
+            cols[c] = col;
+            below = c > 0 && col < i;
+        }
+        n_below += uint32_t(__popcll(__ballot(below)));
+    }
+    const uint32_t c_up = 1 + n_below; // blocks c_up .. L1-1 have columns above i
+    const uint32_t nq = 1 + (L1 - c_up); // summed blocks: q = 0 the diagonal, q >= 1 block c_up + q - 1
+    for (uint32_t t = lane; t < L1 * 36; t += 64)
+        regblk[t] = 0.f;
+    if (lane < 16)
+        red[lane] = 0.f;
+    for (uint32_t q = lane; q < nq; q += 64)
+    {
+        const uint32_t c = q == 0 ? 0u : c_up + q - 1;
+        ls[q] = d.list_off[base + c];
+        le[q] = d.list_off[base + c + 1];
+    }
+    wave_lds_sync();
+
+    // every summed block's float4 group g: its slot partials in chunks of
+    // kChunk (list order inside a chunk), the chunk sums added in order -- a
+    // task per (block, group), its chunks in sequence (a batch launch has rows
+    // enough to hide the latency of the sequence)
+    const float4 *__restrict__ part = reinterpret_cast<const float4 *>(d.tpart);
+    for (uint32_t task = lane; task < nq * kG; task += 64)
+    {
+        const uint32_t q = task / kG, g = task % kG;
+        const uint32_t l1 = le[q];
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (uint32_t l0 = ls[q]; l0 < l1; l0 += kChunk)
+        {
+            const float4 v = sum_chunk4(part, l0, min(l0 + kChunk, l1), g);
+            a.x += v.x;
+            a.y += v.y;
+            a.z += v.z;
+            a.w += v.w;
+        }
+        run[task] = a;
+    }
+
+    // ARAP and node-translation terms of the row (after its dense terms in
+    // the node's term list), in rounds of 256 as in the work-group form: every
+    // entry summed over a round's terms in list order, the round's sum then
+    // added to the row.  The parameters are staged 64 terms (one per lane) at
+    // a time; a lane's two entry sums carry over the sub-rounds of a round.
+    const uint32_t P = d.P, E = d.E;
+    const uint32_t dend = d.n2t_off[i] + d.dense_cnt[i];
+    const uint32_t tend = d.n2t_off[i + 1];
+    for (uint32_t r0 = dend; r0 < tend; r0 += kTileThreads)
+    {
+        const uint32_t r1 = min(r0 + uint32_t(kTileThreads), tend);
+        float diag[2] = {0.f, 0.f}, gq[2] = {0.f, 0.f};
+        for (uint32_t s0 = r0; s0 < r1; s0 += 64)
+        {
+            const uint32_t n = min(64u, r1 - s0);
+            if (lane < n)
+                rt[lane] = stage_reg_term(d, i, d.n2t_terms[s0 + lane], [&](uint32_t j) {
+                    // the ascending neighbours cols[1..L1)
+                    uint32_t lo = 1, hi = L1;
+                    while (lo < hi)
+                    {
+                        const uint32_t m = (lo + hi) >> 1;
+                        if (cols[m] < j)
+                            lo = m + 1;
+                        else
+                            hi = m;
+                    }
+                    return int(lo);
+                });
+            wave_lds_sync();
+            // entries 0..35: diagonal, 36..71: cross, 72..77: g (two per lane)
+#pragma unroll
+            for (uint32_t h = 0; h < 2; ++h)
+            {
+                const int ent = int(lane + 64u * h);
+                if (ent >= 78)
+                    continue;
+                for (uint32_t k = 0; k < n; ++k)
+                    reg_term_entry(rt[k], ent, diag[h], gq[h], regblk);
+            }
+            wave_lds_sync();
+        }
+#pragma unroll
+        for (uint32_t h = 0; h < 2; ++h)
+        {
+            const int ent = int(lane + 64u * h);
+            if (ent < 36)
+                regblk[ent] += diag[h];
+            else if (ent >= 72 && ent < 78)
+                red[8 + ent - 72] += gq[h];
+        }
+        wave_lds_sync();
+    }
+
+    // block q's float4 group g of sums, plus ARAP and damping, and its
+    // transpose; the diagonal block also goes to dg and its g part to red
+    const size_t out0 = size_t(base) * 36;
+    for (uint32_t task = lane; task < nq * kG; task += 64)
+    {
+        const uint32_t q = task / kG, g = task % kG;
+        const uint32_t c = q == 0 ? 0u : c_up + q - 1;
+        const size_t tr = size_t(d.blk_transpose[base + c]) * 36; // (j, i) for c > 0
+        const float4 acc = run[task];
+        const float av[4] = {acc.x, acc.y, acc.z, acc.w};
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+            const int e = int(g) * 4 + r;
+            if (e < 21)
+            {
+                int a, b;
+                sym_pair_t(e, a, b);
+                float u = av[r] + regblk[c * 36 + a * 6 + b];
+                if (c == 0 && a == b)
+                    u += d.damping;
+                d.blk[out0 + size_t(c) * 36 + a * 6 + b] = u;
+                const float v = av[r] + regblk[c * 36 + b * 6 + a];
+                if (a != b)
+                    d.blk[out0 + size_t(c) * 36 + b * 6 + a] = v;
+                if (c == 0)
+                {
+                    dg[a * 6 + b] = u;
+                    dg[b * 6 + a] = a != b ? v : u;
+                }
+                if (c > 0)
+                {
+                    // the transposed block: (j, i)[b][a] = (i, j)[a][b]
+                    d.blk[tr + b * 6 + a] = u;
+                    if (a != b)
+                        d.blk[tr + a * 6 + b] = v;
+                }
+            }
+            else if (c == 0 && e < 27)
+                red[e - 21] = av[r]; // g of the diagonal slots
+        }
+    }
+    // the row's zeroed pipelined-PCG vectors
+    if (lane >= 32 && lane < 44)
+        d.mv[(lane - 32) / 6][size_t(i) * 6 + (lane - 32) % 6] = 0.0;
+    wave_lds_sync();
+    if (lane < 6)
+        d.rhs[size_t(i) * 6 + lane] = -(red[lane] + red[8 + lane]);
+    // the FP64 inverse of the diagonal block (the same float values the block store holds)
+    invert6_fp64_wave(dg, gm, d.dinv + size_t(i) * 36);
 }
 
 // ------------------------------------------------------------ matrix-free J^T W J x
@@ -1253,9 +1478,11 @@ hipError_t launch_spmv_rows(const SolveDesc *sds, const uint32_t *wmap, int n_so
     return hipGetLastError();
 }
 
-size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len)
+size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len, bool wave)
 {
     const size_t L1 = size_t(max_row_len) + 1;
+    if (wave)
+        return sizeof(float4) * L1 * (kTileRec / 4) + sizeof(float) * L1 * 36 + sizeof(uint32_t) * 3 * L1;
     return sizeof(float4) * (kCombUnionF4 + L1 * (kTileRec / 4)) + sizeof(float) * (L1 * 36 + 16) +
            sizeof(uint32_t) * (3 * L1 + 1) + sizeof(uint16_t) * ((max_nodes + 1) & ~1u);
 }
@@ -1276,20 +1503,25 @@ hipError_t launch_slot_pos(const uint32_t *list_items, uint32_t n, uint32_t *slo
 }
 
 hipError_t launch_tiles_combine(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, size_t lds,
-                                PcgSlot2 *slots2, uint32_t n_slots2, hipStream_t st)
+                                bool wave, PcgSlot2 *slots2, uint32_t n_slots2, hipStream_t st)
 {
     if (n_wg == 0)
         return hipSuccess;
+    const void *fn = wave ? reinterpret_cast<const void *>(k_tiles_combine_wave)
+                          : reinterpret_cast<const void *>(k_tiles_combine_wg);
     if (lds > 65536)
     {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_tiles_combine),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
         if (e != hipSuccess)
             return e;
     }
     if (slots2 && n_slots2 > n_wg)
         return hipErrorInvalidValue; // never: a solve has no more PCG groups than rows
-    hipLaunchKernelGGL(k_tiles_combine, dim3(n_wg), dim3(kTileThreads), lds, st, sds, wmap, n_solve, slots2, n_slots2);
+    if (wave)
+        hipLaunchKernelGGL(k_tiles_combine_wave, dim3(n_wg), dim3(64), lds, st, sds, wmap, n_solve, slots2, n_slots2);
+    else
+        hipLaunchKernelGGL(k_tiles_combine_wg, dim3(n_wg), dim3(kTileThreads), lds, st, sds, wmap, n_solve, slots2,
+                           n_slots2);
     return hipGetLastError();
 }
 } // namespace sh

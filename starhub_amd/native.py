@@ -52,8 +52,8 @@ class _Handle:
             self._h = ctypes.c_void_p()
         _OPEN.discard(self)
 
-    def __del__(self):
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
             return
         try:
             self.close()
