@@ -408,6 +408,23 @@ int sh_node_motion_from_flow(const float *rgbd_prev4, const float *rgbd4, const 
                              uint32_t knn, const float cam2world[16], float fx, float fy, float cx, float cy,
                              uint32_t num_nodes, void *workspace, float *node_motion_pred4, sh_stream_t stream);
 
+/* No counterpart in the reference: sh_node_motion_from_flow without the
+ * workspace argument, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It allocates nothing: the node sums live in a library-owned
+ * device array of SH_NO_ALLOC_MAX_NODES nodes x 32 bytes (128 KiB), one per
+ * GPU, held until the process ends and cleared by a kernel at the start of
+ * every call.  num_nodes above SH_NO_ALLOC_MAX_NODES: SH_E_CAPACITY.  Every
+ * call on one GPU shares that GPU's array, so calls that overlap on one GPU
+ * (other streams or threads) race; serialise them.  Same results as
+ * sh_node_motion_from_flow, bit for bit. */
+#define SH_NO_ALLOC_MAX_NODES 4096u
+int sh_node_motion_from_flow_noalloc(const float *rgbd_prev4, const float *rgbd4, const float *flow_map2,
+                                     const uint32_t *index_map_prev, uint32_t width, uint32_t height,
+                                     const uint16_t *surfel_knn, const float *surfel_spatial, uint32_t num_surfels,
+                                     uint32_t knn, const float cam2world[16], float fx, float fy, float cx, float cy,
+                                     uint32_t num_nodes, float *node_motion_pred4, sh_stream_t stream);
+
 /* ------------------------------------------------------- solver maps
  * Renderer::DrawSolverMapsWithRecentObservation / ...ConfidentObservation
  * (libs/star/geometry/render/RendererDraw.cu:104-174, shaders/solver_map.vert

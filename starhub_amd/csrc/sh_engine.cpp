@@ -1918,6 +1918,28 @@ extern "C"
                                              reinterpret_cast<hipStream_t>(stream)));
     }
 
+    int sh_node_motion_from_flow_noalloc(const float *rgbd_prev4, const float *rgbd4, const float *flow_map2,
+                                         const uint32_t *index_map_prev, uint32_t width, uint32_t height,
+                                         const uint16_t *surfel_knn, const float *surfel_spatial,
+                                         uint32_t num_surfels, uint32_t knn, const float cam2world[16], float fx,
+                                         float fy, float cx, float cy, uint32_t num_nodes, float *out4,
+                                         sh_stream_t stream)
+    {
+        if ((knn != 4 && knn != 8) || !cam2world || !out4 || width > 16384 || height > 16384 || fx == 0.f ||
+            fy == 0.f ||
+            (width && height && (!rgbd_prev4 || !rgbd4 || !flow_map2 || !index_map_prev || !surfel_knn ||
+                                 !surfel_spatial)))
+            return SH_E_ARG;
+        if (num_nodes > sh::kNoAllocMaxNodes)
+            return SH_E_CAPACITY;
+        unsigned long long *sums = nullptr;
+        if (const hipError_t e = sh::noalloc_sums(&sums); e != hipSuccess)
+            return hip_rc(e);
+        return hip_rc(sh::launch_node_motion(rgbd_prev4, rgbd4, flow_map2, index_map_prev, int(width), int(height),
+                                             surfel_knn, surfel_spatial, num_surfels, int(knn), cam2world, fx, fy, cx,
+                                             cy, num_nodes, sums, out4, reinterpret_cast<hipStream_t>(stream)));
+    }
+
     int sh_render_solver_maps(const float *vertex4, const float *normal4, const float *color_time4,
                               uint32_t num_surfels, const float world2camera[16], const sh_camera *cam,
                               float max_depth, float current_time, int with_recent_observation, void *zbuffer,

@@ -17,6 +17,8 @@
 // atomics land in.  Here every contribution is quantised to an integer
 // number of 2^-31 units and added with 64-bit integer atomics: the sums are
 // exact, so the result is the same whatever the order.
+#include <mutex>
+
 #include "sh_common.hpp"
 #include "sh_kernels.hpp"
 
@@ -24,6 +26,12 @@
 
 namespace sh
 {
+// Node sums of the allocation-free entry point (sh_node_motion_from_flow_noalloc):
+// kNoAllocMaxNodes nodes x 4 64-bit words = 128 KiB, one copy per GPU, held
+// until the process ends.  Host code never uses this name as an address: each
+// device's address comes from hipGetSymbolAddress (noalloc_sums below).
+static __device__ __align__(256) unsigned long long scratch[kNoAllocMaxNodes * 4];
+
 namespace
 {
 constexpr float kFixScale = 2147483648.f; // 2^31
@@ -137,6 +145,34 @@ hipError_t launch_flow_from_of16(const uint16_t *of16, int W, int H, float scale
     hipLaunchKernelGGL(k_flow_from_of16, dim3((w + 15) / 16, (h + 15) / 16), dim3(16, 16), 0, st, of16, W, H, w, h,
                        1.f / scale, scale, reinterpret_cast<float2 *>(flow2));
     return hipGetLastError();
+}
+
+hipError_t noalloc_sums(unsigned long long **out)
+{
+    // the device address of `scratch` on the current device, looked up once per
+    // device (each GPU holds its own copy of the array at its own address)
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static unsigned long long *addr[kMaxDevices] = {};
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return e;
+    if (dev < 0 || dev >= kMaxDevices)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev])
+    {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        if (!p)
+            return hipErrorInvalidSymbol;
+        addr[dev] = static_cast<unsigned long long *>(p);
+    }
+    *out = addr[dev];
+    return hipSuccess;
 }
 
 hipError_t launch_node_motion(const float *rgbd_prev4, const float *rgbd4, const float *flow2,

@@ -159,6 +159,11 @@ constexpr uint32_t kPcgNodesPerWg = 32;  // split PCG kernels
 
 // Forward warp of surfels and nodes (sh_warp.hip)
 // optical-flow input and node-motion prior (sh_flow.hip)
+// Nodes the allocation-free node-motion entry point can take (its sums live in
+// a fixed device array of this many nodes x 32 bytes).
+constexpr uint32_t kNoAllocMaxNodes = 4096;
+// Device address of that array on the current device (looked up once per device).
+hipError_t noalloc_sums(unsigned long long **out);
 hipError_t launch_flow_from_of16(const uint16_t *of16, int W, int H, float scale, float *flow2, hipStream_t st);
 hipError_t launch_node_motion(const float *rgbd_prev4, const float *rgbd4, const float *flow2,
                               const uint32_t *index_prev, int width, int height, const uint16_t *surfel_knn,

@@ -656,6 +656,24 @@ def node_motion_from_flow(rgbd_prev4, rgbd4, flow2, index_prev, surfel_knn, surf
     return out[:num_nodes]
 
 
+def node_motion_from_flow_noalloc(rgbd_prev4, rgbd4, flow2, index_prev, surfel_knn, surfel_spatial, cam2world, fx,
+                                  fy, cx, cy, num_nodes: int, stream=None):
+    """sh_node_motion_from_flow_noalloc: as node_motion_from_flow, with the node sums in the library's
+    fixed device array instead of a caller workspace (num_nodes <= 4096)."""
+    import torch
+    H, W = index_prev.shape
+    S, K = surfel_knn.shape
+    out = torch.empty((max(1, num_nodes), 4), dtype=torch.float32, device="cuda")
+    m = (ctypes.c_float * 16)(*np.asarray(cam2world, np.float32).reshape(-1).tolist())
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    check(lib().sh_node_motion_from_flow_noalloc(p(rgbd_prev4), p(rgbd4), p(flow2), p(index_prev), W, H,
+                                                 p(surfel_knn), p(surfel_spatial), S, K, m, ctypes.c_float(fx),
+                                                 ctypes.c_float(fy), ctypes.c_float(cx), ctypes.c_float(cy),
+                                                 num_nodes, p(out), _stream_ptr(stream)),
+          "sh_node_motion_from_flow_noalloc")
+    return out[:num_nodes]
+
+
 # ------------------------------------------------------------ linear-solver plug-in
 SPMV_APPLY_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
 SPMV_INIT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
