@@ -22,6 +22,8 @@
 // block's slot partials in list order, the row's ARAP and node-translation
 // terms and the damping.  Each record is read once per GN iteration, there
 // are no atomics, and every sum has a fixed order.
+#include <type_traits>
+
 #include <rocprim/block/block_radix_sort.hpp>
 
 #include "sh_common.hpp"
@@ -348,10 +350,11 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
 {
     using G = TileGeom<K>;
     constexpr int kEnt = kTilePix * G::kPairs;
+    constexpr uint32_t kNB = 2 * (kTilePix + 1); // visiting-order buckets
     __shared__ float4 Hs[kTilePix * (kTileRec / 4)];
     __shared__ float wS[kTilePix * K];
     __shared__ uint16_t ent[kEnt];
-    __shared__ uint32_t hist[kTilePix + 2];
+    __shared__ uint32_t hist[kNB];
     // slot list starts sst[slot_cap + 2], then the visiting order perm[slot_cap]
     extern __shared__ uint16_t slot_lds[];
     uint16_t *__restrict__ sst = slot_lds;
@@ -364,40 +367,84 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
     const uint32_t tid = threadIdx.x;
     const uint32_t p0 = t * kTilePix;
     const uint32_t np = min(uint32_t(kTilePix), uint32_t(d.P) - p0);
-    const uint32_t s0 = d.tile_sbase[t], ns = d.tile_sbase[t + 1] - s0;
-    const uint32_t c0 = d.tile_cbase[t], nc = d.tile_cbase[t + 1] - c0;
-    if (ns > slot_cap)
-        return; // never taken: slot_cap is the largest slot count of the launch's tiles
-    for (uint32_t i = tid; i < kTilePix + 2; i += kTileThreads)
-        hist[i] = 0u;
+    // the tile's records and weights first: their loads need only the
+    // descriptor, so they are in flight with the tile's list bounds
+    float hg[kHG];
+    float wk[K];
     if (tid < np)
     {
         const uint32_t p = p0 + tid;
-        float hg[kHG];
         load_hg<JT>(d.jr, p, hg);
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            wk[k] = d.gpw[size_t(p) * K + k];
+    }
+    const uint32_t s0 = d.tile_sbase[t], ns = d.tile_sbase[t + 1] - s0;
+    const uint32_t c0 = d.tile_cbase[t], nc = d.tile_cbase[t + 1] - c0;
+    for (uint32_t i = tid; i < kNB; i += kTileThreads)
+        hist[i] = 0u;
+    if (tid < np)
+    {
 #pragma unroll
         for (int v = 0; v < kTileRec / 4; ++v)
             Hs[tid * (kTileRec / 4) + v] = make_float4(hg[4 * v], hg[4 * v + 1], hg[4 * v + 2], hg[4 * v + 3]);
 #pragma unroll
         for (int k = 0; k < K; ++k)
-            wS[tid * K + k] = d.gpw[size_t(p) * K + k];
+            wS[tid * K + k] = wk[k];
     }
-    for (uint32_t i = tid; i < nc; i += kTileThreads)
-        ent[i] = d.tc_contrib[c0 + i];
-    for (uint32_t s = tid; s <= ns; s += kTileThreads)
-        sst[s] = uint16_t(d.tc_slot_first[s0 + s] - c0);
+    if (ns > slot_cap)
+        return; // never taken: slot_cap is the largest slot count of the launch's tiles
+    // the contribution list (at most kEnt entries) and the slot starts: every
+    // load issued before the first LDS store, one round trip for all of them
+    {
+        constexpr int kPer = kEnt / kTileThreads;
+        uint16_t cv[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j)
+        {
+            const uint32_t i = tid + uint32_t(j) * kTileThreads;
+            cv[j] = i < nc ? d.tc_contrib[c0 + i] : uint16_t(0);
+        }
+        uint32_t sv = 0;
+        if (tid <= ns)
+            sv = d.tc_slot_first[s0 + tid];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j)
+        {
+            const uint32_t i = tid + uint32_t(j) * kTileThreads;
+            if (i < nc)
+                ent[i] = cv[j];
+        }
+        if (tid <= ns)
+            sst[tid] = uint16_t(sv - c0);
+        for (uint32_t s = tid + kTileThreads; s <= ns; s += kTileThreads)
+            sst[s] = uint16_t(d.tc_slot_first[s0 + s] - c0);
+    }
     __syncthreads();
-    // counting sort of the slots by list length (1..kTilePix), longest first
+    // visiting order: the diagonal slots (ka == kb, the only ones whose g
+    // part is used) first, then the off-diagonal ones, each longest list
+    // first -- a counting sort of (kind, length)
+    auto bucket = [&](uint32_t s) {
+        const uint32_t c = ent[sst[s]];
+        const uint32_t off = ((c >> 7) & 7u) != ((c >> 10) & 7u) ? uint32_t(kTilePix) + 1u : 0u;
+        return off + uint32_t(kTilePix) - uint32_t(sst[s + 1] - sst[s]);
+    };
     for (uint32_t s = tid; s < ns; s += kTileThreads)
-        atomicAdd(&hist[kTilePix - (sst[s + 1] - sst[s])], 1u);
+        atomicAdd(&hist[bucket(s)], 1u);
     __syncthreads();
     if (tid < 64)
     {
-        // exclusive scan of hist[0..kTilePix] (129 buckets: 3 per lane)
-        const uint32_t b0 = 3 * tid;
-        const uint32_t h0 = b0 < kTilePix + 1 ? hist[b0] : 0u, h1 = b0 + 1 < kTilePix + 1 ? hist[b0 + 1] : 0u,
-                       h2 = b0 + 2 < kTilePix + 1 ? hist[b0 + 2] : 0u;
-        uint32_t v = h0 + h1 + h2;
+        // exclusive scan of hist[0 .. kNB): kPer consecutive buckets per lane
+        constexpr uint32_t kPer = (kNB + 63) / 64;
+        const uint32_t b0 = kPer * tid;
+        uint32_t h[kPer], v = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j)
+        {
+            h[j] = b0 + j < kNB ? hist[b0 + j] : 0u;
+            v += h[j];
+        }
+        const uint32_t own = v;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1)
         {
@@ -405,53 +452,95 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
             if (int(tid) >= o)
                 v += u;
         }
-        const uint32_t ex = v - h0 - h1 - h2;
-        if (b0 < kTilePix + 1)
-            hist[b0] = ex;
-        if (b0 + 1 < kTilePix + 1)
-            hist[b0 + 1] = ex + h0;
-        if (b0 + 2 < kTilePix + 1)
-            hist[b0 + 2] = ex + h0 + h1;
+        uint32_t ex = v - own;
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j)
+            if (b0 + j < kNB)
+            {
+                hist[b0 + j] = ex;
+                ex += h[j];
+            }
     }
     __syncthreads();
     for (uint32_t s = tid; s < ns; s += kTileThreads)
-        perm[atomicAdd(&hist[kTilePix - (sst[s + 1] - sst[s])], 1u)] = uint16_t(s);
+        perm[atomicAdd(&hist[bucket(s)], 1u)] = uint16_t(s);
     __syncthreads();
 
-    // Each thread sums whole slots, all 27 values (7 float4) at once, over the
-    // slot's contributions in list (pixel) order: the contribution's entry and
-    // weights are read once for all seven groups.  A slot's partial goes to its
-    // list position, so a block's partials are contiguous for the combine.
+    // Each thread sums whole slots, all of a slot's float4 groups at once, over
+    // the slot's contributions in list (pixel) order: the contribution's entry
+    // and weights are read once for all groups.  An off-diagonal slot needs
+    // H only (groups 0..5: H[20] shares group 5 with g[0..2]); group 6 (g[3..5],
+    // pad) is summed and stored for diagonal slots only, and a wave with no
+    // diagonal slot skips it altogether.  A slot's partial goes to its list
+    // position, so a block's partials are contiguous for the combine.
     float4 *__restrict__ out = reinterpret_cast<float4 *>(d.tpart);
-    for (uint32_t k = tid; k < ns; k += kTileThreads)
-    {
+    auto sum_slots = [&](uint32_t k, auto n_groups) {
+        constexpr int NG = decltype(n_groups)::value;
         const uint32_t s = perm[k];
         const uint32_t dst = d.slot_pos[s0 + s];
-        float4 acc[kTileRec / 4];
+        float4 acc[NG];
 #pragma unroll
-        for (int g = 0; g < kTileRec / 4; ++g)
+        for (int g = 0; g < NG; ++g)
             acc[g] = make_float4(0.f, 0.f, 0.f, 0.f);
-        const uint32_t e1 = sst[s + 1];
-        uint32_t c = ent[sst[s]];
-        for (uint32_t e = sst[s]; e < e1; ++e)
-        {
-            const uint32_t pl = c & 127u, ka = (c >> 7) & 7u, kb = (c >> 10) & 7u;
-            c = ent[min(e + 1, e1 - 1)]; // the next entry, ahead of this one's sums
-            const float coef = wS[pl * K + ka] * wS[pl * K + kb];
+        const uint32_t e0 = sst[s], e1 = sst[s + 1];
+        // two entries a step: both records are read before the first one's
+        // products, so the second read's latency hides under them (the sums
+        // stay in list order)
+        const uint32_t cfirst = ent[e0];
+        const bool diag = ((cfirst >> 7) & 7u) == ((cfirst >> 10) & 7u);
+        auto fetch = [&](uint32_t cc, float &coef, float4 (&v)[NG]) {
+            const uint32_t pl = cc & 127u, ka = (cc >> 7) & 7u, kb = (cc >> 10) & 7u;
+            coef = wS[pl * K + ka] * wS[pl * K + kb];
             const float4 *__restrict__ h = Hs + pl * (kTileRec / 4);
 #pragma unroll
-            for (int g = 0; g < kTileRec / 4; ++g)
+            for (int g = 0; g < NG; ++g)
+                v[g] = h[g];
+        };
+        auto madd = [&](float coef, const float4 (&v)[NG]) {
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
             {
-                const float4 v = h[g];
-                acc[g].x = fmaf(coef, v.x, acc[g].x);
-                acc[g].y = fmaf(coef, v.y, acc[g].y);
-                acc[g].z = fmaf(coef, v.z, acc[g].z);
-                acc[g].w = fmaf(coef, v.w, acc[g].w);
+                acc[g].x = fmaf(coef, v[g].x, acc[g].x);
+                acc[g].y = fmaf(coef, v[g].y, acc[g].y);
+                acc[g].z = fmaf(coef, v[g].z, acc[g].z);
+                acc[g].w = fmaf(coef, v[g].w, acc[g].w);
             }
+        };
+        // the next step's two list entries are read a step ahead
+        uint32_t e = e0;
+        uint32_t xa = cfirst, xb = ent[min(e0 + 1, e1 - 1)];
+        for (; e + 1 < e1; e += 2)
+        {
+            const uint32_t na = ent[min(e + 2, e1 - 1)], nb = ent[min(e + 3, e1 - 1)];
+            float ca, cb;
+            float4 va[NG], vb[NG];
+            fetch(xa, ca, va);
+            fetch(xb, cb, vb);
+            madd(ca, va);
+            madd(cb, vb);
+            xa = na;
+            xb = nb;
+        }
+        if (e < e1)
+        {
+            float ca;
+            float4 va[NG];
+            fetch(xa, ca, va);
+            madd(ca, va);
         }
 #pragma unroll
-        for (int g = 0; g < kTileRec / 4; ++g)
-            out[size_t(dst) * (kPartStride / 4) + g] = acc[g];
+        for (int g = 0; g < NG; ++g)
+            if (g < kTileRec / 4 - 1 || diag)
+                out[size_t(dst) * (kPartStride / 4) + g] = acc[g];
+    };
+    for (uint32_t k = tid; k < ns; k += kTileThreads)
+    {
+        const uint32_t c = ent[sst[perm[k]]];
+        const bool diag = ((c >> 7) & 7u) == ((c >> 10) & 7u);
+        if (__any(diag))
+            sum_slots(k, std::integral_constant<int, kTileRec / 4>{});
+        else
+            sum_slots(k, std::integral_constant<int, kTileRec / 4 - 1>{});
     }
 }
 
@@ -836,7 +925,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine_wg(const SolveDe
                     hi = m;
             }
             const uint32_t l0 = ls[lo] + (k - cc[lo]) * kChunk;
-            csum[task] = sum_chunk4(part, l0, min(l0 + kChunk, le[lo]), g);
+            // group 6 (g[3..5], pad) is written for diagonal slots only
+            csum[task] = (lo > 0 && g == kG - 1) ? make_float4(0.f, 0.f, 0.f, 0.f)
+                                                 : sum_chunk4(part, l0, min(l0 + kChunk, le[lo]), g);
         }
         __syncthreads();
         for (uint32_t task = tid; task < nq * kG; task += kTileThreads)
@@ -998,7 +1089,8 @@ __global__ __launch_bounds__(64) void k_tiles_combine_wave(const SolveDesc *__re
     for (uint32_t task = lane; task < nq * kG; task += 64)
     {
         const uint32_t q = task / kG, g = task % kG;
-        const uint32_t l1 = le[q];
+        // group 6 (g[3..5], pad) is written for diagonal slots only
+        const uint32_t l1 = (q > 0 && g == kG - 1) ? ls[q] : le[q];
         float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
         for (uint32_t l0 = ls[q]; l0 < l1; l0 += kChunk)
         {
