@@ -563,16 +563,29 @@ __device__ __forceinline__ void sym_pair_t(int e, int &a, int &b)
     b = e - (a * 6 - a * (a - 1) / 2) + a;
 }
 
-// One ARAP or node-translation term of a node row, as the row kernels use it.
+// One ARAP or node-translation term of a node row, as the combine kernels
+// use it: the features every entry of its share of the row is made of.  With
+// p this node's warped point, o the other node's and r = p - o, the ARAP
+// Jacobian rows of this node are J = [-[p]x, I] (times the side's sign, which
+// cancels in every product below; node_graph_reg_jacobian.cuh:250-275), so
+//   diagonal block  J^T J:       [(p.p) I - p p^T,  [p]x ; [p]x^T, I]
+//   cross block     -J^T J_o:    [-(p.o) I + o p^T, [p]x^T ; [o]x, -I]
+//   g               J^T r:       [p x r ; r]
+// and each entry is s0 + s1 f[i1] + s2 f[i2] for per-entry constants.  A node
+// translation term (node_motion_jacobian.cuh:7-35) is the same with p = o = 0
+// and r = t - pred: diagonal I on the translation part, g = [0 ; t - pred].
+constexpr int kRegF = 32;
+// feature offsets: p.p, p.o, p_a p_b (9), o_a p_b (9), p (3), o (3), p x r (3), r (3)
+constexpr int kFpp = 0, kFpo = 1, kFpp9 = 2, kFop9 = 11, kFp = 20, kFo = 23, kFpxr = 26, kFr = 29;
 struct RegTerm
 {
-    float pm[3], po[3]; // this node's and the other node's warped point (ARAP)
-    float sm, ws;       // side sign, lambda^2 cw^2 (0: invalid edge); NT: ws = lam2 c^2, sm = 2
-    float tv[3];        // NT: t - pred
-    int col;            // block column of the other node, -1 when it is the row node itself
+    float f[kRegF];
+    float ws; // lambda^2 cw^2 (0: invalid edge); translation: lambda^2 c^2
+    int col;  // block column of the other node, -1 for none (translation term)
 };
+constexpr uint32_t kTermSub = 32; // terms staged in LDS at a time
 
-// The parameters of ARAP edge or node-translation term t of row i's list
+// The features of ARAP edge or node-translation term t of row i's list
 // (node_graph_reg_jacobian.cuh:166-358, node_motion_jacobian.cuh:7-35);
 // col_of(j) gives node j's block position in row i, for an edge's cross block.
 template <class ColOf>
@@ -580,6 +593,7 @@ __device__ __forceinline__ RegTerm stage_reg_term(const SolveDesc &d, uint32_t i
 {
     RegTerm r{};
     const uint32_t P = d.P, E = d.E;
+    float pm[3] = {0.f, 0.f, 0.f}, po[3] = {0.f, 0.f, 0.f}, res[3];
     if (t < P + E)
     {
         const uint32_t e = t - P;
@@ -592,10 +606,10 @@ __device__ __forceinline__ RegTerm stage_reg_term(const SolveDesc &d, uint32_t i
 #pragma unroll
         for (int q = 0; q < 3; ++q)
         {
-            r.pm[q] = is_x ? pa[q] : pb[q];
-            r.po[q] = is_x ? pb[q] : pa[q];
+            pm[q] = is_x ? pa[q] : pb[q];
+            po[q] = is_x ? pb[q] : pa[q];
+            res[q] = pm[q] - po[q];
         }
-        r.sm = is_x ? 1.f : -1.f;
         r.ws = (u1.w == 0.f) ? 0.f : d.lam2_reg * (u1.z * u1.z);
         r.col = other != i ? col_of(other) : -1; // every ARAP edge of the row is a block of it
     }
@@ -604,92 +618,141 @@ __device__ __forceinline__ RegTerm stage_reg_term(const SolveDesc &d, uint32_t i
         const uint32_t m = t - P - E;
         const float4 pred = d.node_pred ? d.node_pred[m] : make_float4(0.f, 0.f, 0.f, 0.f);
         const float4 tt = d.ttrans[m];
-        r.sm = 2.f;
         r.ws = d.lam2_nt * (pred.w * pred.w);
-        r.tv[0] = tt.x - pred.x;
-        r.tv[1] = tt.y - pred.y;
-        r.tv[2] = tt.z - pred.z;
+        res[0] = tt.x - pred.x;
+        res[1] = tt.y - pred.y;
+        res[2] = tt.z - pred.z;
         r.col = -1;
     }
+    r.f[kFpp] = pm[0] * pm[0] + pm[1] * pm[1] + pm[2] * pm[2];
+    r.f[kFpo] = pm[0] * po[0] + pm[1] * po[1] + pm[2] * po[2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+        {
+            r.f[kFpp9 + 3 * a + b] = pm[a] * pm[b];
+            r.f[kFop9 + 3 * a + b] = po[a] * pm[b];
+        }
+        r.f[kFp + a] = pm[a];
+        r.f[kFo + a] = po[a];
+        r.f[kFr + a] = res[a];
+    }
+    r.f[kFpxr + 0] = pm[1] * res[2] - pm[2] * res[1];
+    r.f[kFpxr + 1] = pm[2] * res[0] - pm[0] * res[2];
+    r.f[kFpxr + 2] = pm[0] * res[1] - pm[1] * res[0];
     return r;
 }
 
-// Term r's share of row entry ent (0..35 the diagonal block, 36..71 the
-// cross block, 72..77 g): the diagonal and g shares are added to diag and gq,
-// the cross share straight into the edge's block of regblk (entry ent - 36 of
-// every block belongs to one lane, so the adds of a wave do not collide).
-__device__ __forceinline__ void reg_term_entry(const RegTerm &r, int ent, float &diag, float &gq,
-                                               float *__restrict__ regblk)
+// An entry of a term's share as s0 + s1 f[i1] + s2 f[i2].
+struct EntryForm
 {
-    if (r.sm == 2.f)
-    {
-        // node translation (node_motion_jacobian.cuh:7-35): diagonal a = b = 3..5
-        if (ent < 36 && ent / 6 == ent % 6 && ent / 6 >= 3)
-            diag += r.ws;
-        else if (ent >= 72 && ent - 72 >= 3)
-            gq += r.ws * r.tv[ent - 75];
-        return;
-    }
-    if (r.ws == 0.f)
-        return;
-    const f3 pm = mk3(r.pm[0], r.pm[1], r.pm[2]), po = mk3(r.po[0], r.po[1], r.po[2]);
+    int i1, i2;
+    float s0, s1, s2;
+};
+
+// [v]x[rr][cc] (v at feature offset off) as a signed feature: (index, sign)
+__device__ __forceinline__ void skew_feature(int off, int rr, int cc, int &idx, float &sgn)
+{
+    // [v]x = [[0, -v2, v1], [v2, 0, -v0], [-v1, v0, 0]]
+    const int k = 3 - rr - cc; // the component off the diagonal
+    idx = off + (rr == cc ? 0 : k);
+    sgn = rr == cc ? 0.f : (((rr + 1) % 3 == cc) ? -1.f : 1.f);
+}
+
+// Row entry ent (0..35 the diagonal block, 36..71 the cross block, 72..77 g)
+// of a term's share, in the form above.
+__device__ __forceinline__ EntryForm entry_form(int ent)
+{
+    EntryForm f{0, 0, 0.f, 0.f, 0.f};
     if (ent < 72)
     {
-        // ARAP rows x side [-[a]x, I], y side [[b]x, -I] (node_graph_reg_jacobian.cuh:250-275)
-        const int ab = ent < 36 ? ent : ent - 36;
-        const int aa = ab / 6, bb = ab % 6;
-        float v = 0.f;
-#pragma unroll
-        for (int ch = 0; ch < 3; ++ch)
+        const bool cross = ent >= 36;
+        const int ab = cross ? ent - 36 : ent;
+        const int a = ab / 6, b = ab % 6;
+        if (a < 3 && b < 3)
         {
-            const float ga = r.sm * reg_elem_t(pm, ch, aa);
-            const float gb = ent < 36 ? r.sm * reg_elem_t(pm, ch, bb) : -r.sm * reg_elem_t(po, ch, bb);
-            v += ga * gb;
+            // (p.p) d_ab - p_a p_b, or -(p.o) d_ab + o_a p_b
+            f.i1 = cross ? kFpo : kFpp;
+            f.s1 = a == b ? (cross ? -1.f : 1.f) : 0.f;
+            f.i2 = (cross ? kFop9 : kFpp9) + 3 * a + b;
+            f.s2 = cross ? 1.f : -1.f;
         }
-        if (ent < 36)
-            diag += r.ws * v;
-        else if (r.col >= 0)
-            regblk[r.col * 36 + ab] += r.ws * v;
+        else if (a < 3)
+            skew_feature(kFp, cross ? b - 3 : a, cross ? a : b - 3, f.i1, f.s1); // [p]x^T or [p]x
+        else if (b < 3)
+        {
+            if (cross)
+                skew_feature(kFo, a - 3, b, f.i1, f.s1); // [o]x
+            else
+                skew_feature(kFp, b, a - 3, f.i1, f.s1); // [p]x^T
+        }
+        else
+            f.s0 = a == b ? (cross ? -1.f : 1.f) : 0.f;
     }
     else
     {
         const int q = ent - 72;
-        const float res[3] = {r.pm[0] - r.po[0], r.pm[1] - r.po[1], r.pm[2] - r.po[2]};
-        float v = 0.f;
-#pragma unroll
-        for (int ch = 0; ch < 3; ++ch)
-            v += (r.sm * res[ch]) * (r.sm * reg_elem_t(pm, ch, q));
-        gq += r.ws * v;
+        f.i1 = q < 3 ? kFpxr + q : kFr + q - 3;
+        f.s1 = 1.f;
     }
+    return f;
 }
 
-// Loads the terms [beg, end) of row i's list, 256 at a time, one per thread,
-// and adds them into regblk (diagonal block at column 0, cross blocks at
-// colmap[other]) and g[0..5], each entry summed over the terms in list order.
-// Must be reached by all 256 threads.
+// Term r's share of row entry ent (form ef): the diagonal and g shares are
+// added to diag and gq, the cross share straight into the edge's block of
+// regblk (entry ent - 36 of every block belongs to one lane or thread, so the
+// adds do not collide).
+__device__ __forceinline__ void reg_term_entry(const RegTerm &r, int ent, const EntryForm &ef, float &diag,
+                                               float &gq, float *__restrict__ regblk)
+{
+    if (r.ws == 0.f)
+        return;
+    const float v = fmaf(ef.s2, r.f[ef.i2], fmaf(ef.s1, r.f[ef.i1], ef.s0));
+    if (ent < 36)
+        diag += r.ws * v;
+    else if (ent < 72)
+    {
+        if (r.col >= 0)
+            regblk[r.col * 36 + ent - 36] += r.ws * v;
+    }
+    else
+        gq += r.ws * v;
+}
+
+// Adds the terms [beg, end) of row i's list into regblk (diagonal block at
+// column 0, cross blocks at colmap[other]) and g[0..5]: in rounds of 256
+// terms, every entry summed over the round's terms in list order and the
+// round's sum then added; the terms are staged kTermSub at a time.  Must be
+// reached by all 256 threads.
 __device__ void row_reg_terms(const SolveDesc &d, uint32_t i, uint32_t beg, uint32_t end,
                               const uint16_t *__restrict__ colmap, float *__restrict__ regblk, float *__restrict__ g,
                               RegTerm *__restrict__ rt)
 {
     const uint32_t tid = threadIdx.x;
-    for (uint32_t c0 = beg; c0 < end; c0 += kTileThreads)
+    const int ent = int(tid);
+    const EntryForm ef = entry_form(ent < 78 ? ent : 0);
+    for (uint32_t r0 = beg; r0 < end; r0 += kTileThreads)
     {
-        const uint32_t n = min(uint32_t(kTileThreads), end - c0);
-        if (tid < n)
-            rt[tid] = stage_reg_term(d, i, d.n2t_terms[c0 + tid], [&](uint32_t j) { return int(colmap[j]); });
-        __syncthreads();
+        const uint32_t r1 = min(r0 + uint32_t(kTileThreads), end);
         // threads 0..35: diagonal entries, 36..71: cross entries, 72..77: g
-        if (tid < 78)
+        float diag = 0.f, gq = 0.f;
+        for (uint32_t s0 = r0; s0 < r1; s0 += kTermSub)
         {
-            const int ent = int(tid);
-            float diag = 0.f, gq = 0.f;
-            for (uint32_t k = 0; k < n; ++k)
-                reg_term_entry(rt[k], ent, diag, gq, regblk);
-            if (ent < 36)
-                regblk[ent] += diag;
-            else if (ent >= 72)
-                g[ent - 72] += gq;
+            const uint32_t n = min(kTermSub, r1 - s0);
+            if (tid < n)
+                rt[tid] = stage_reg_term(d, i, d.n2t_terms[s0 + tid], [&](uint32_t j) { return int(colmap[j]); });
+            __syncthreads();
+            if (tid < 78)
+                for (uint32_t k = 0; k < n; ++k)
+                    reg_term_entry(rt[k], ent, ef, diag, gq, regblk);
+            __syncthreads();
         }
+        if (ent < 36)
+            regblk[ent] += diag;
+        else if (ent >= 72 && ent < 78)
+            g[ent - 72] += gq;
         __syncthreads();
     }
 }
@@ -706,7 +769,7 @@ constexpr int kBatch = 8;
 constexpr uint32_t kChunk = 16;
 constexpr uint32_t kChunkCap = 128;
 constexpr uint32_t kCombUnionF4 = kChunkCap * (kTileRec / 4); // float4s of the shared region
-static_assert(sizeof(RegTerm) * kTileThreads <= kCombUnionF4 * sizeof(float4), "combine LDS union");
+static_assert(sizeof(RegTerm) * kTermSub <= kCombUnionF4 * sizeof(float4), "combine LDS union");
 
 
 // Sum of float4 group g over list positions [l0, l1), at most kChunk of them,
@@ -1031,7 +1094,7 @@ __global__ __launch_bounds__(64) void k_tiles_combine_wave(const SolveDesc *__re
     __shared__ float dg[36];     // the diagonal block, for its inverse
     __shared__ double gm[6][12]; // its augmented matrix during the inversion
     __shared__ float red[16];    // g of the diagonal slots (0..5), of the terms (8..13)
-    __shared__ RegTerm rt[64];   // one sub-round of ARAP / node-translation terms
+    __shared__ RegTerm rt[kTermSub]; // one sub-round of ARAP / node-translation terms
     // a pipelined PCG that follows publishes into zeroed slots (the row map
     // has a work-group for every one of its groups)
     if (slots2 && blockIdx.x < n_slots2 && threadIdx.x < 4)
@@ -1106,18 +1169,18 @@ __global__ __launch_bounds__(64) void k_tiles_combine_wave(const SolveDesc *__re
     // ARAP and node-translation terms of the row (after its dense terms in
     // the node's term list), in rounds of 256 as in the work-group form: every
     // entry summed over a round's terms in list order, the round's sum then
-    // added to the row.  The parameters are staged 64 terms (one per lane) at
-    // a time; a lane's two entry sums carry over the sub-rounds of a round.
-    const uint32_t P = d.P, E = d.E;
+    // added to the row.  The terms are staged kTermSub at a time (one per
+    // lane); a lane's two entry sums carry over the sub-rounds of a round.
     const uint32_t dend = d.n2t_off[i] + d.dense_cnt[i];
     const uint32_t tend = d.n2t_off[i + 1];
+    const EntryForm ef[2] = {entry_form(int(lane)), entry_form(int(lane + 64u) < 78 ? int(lane + 64u) : 0)};
     for (uint32_t r0 = dend; r0 < tend; r0 += kTileThreads)
     {
         const uint32_t r1 = min(r0 + uint32_t(kTileThreads), tend);
         float diag[2] = {0.f, 0.f}, gq[2] = {0.f, 0.f};
-        for (uint32_t s0 = r0; s0 < r1; s0 += 64)
+        for (uint32_t s0 = r0; s0 < r1; s0 += kTermSub)
         {
-            const uint32_t n = min(64u, r1 - s0);
+            const uint32_t n = min(kTermSub, r1 - s0);
             if (lane < n)
                 rt[lane] = stage_reg_term(d, i, d.n2t_terms[s0 + lane], [&](uint32_t j) {
                     // the ascending neighbours cols[1..L1)
@@ -1141,7 +1204,7 @@ __global__ __launch_bounds__(64) void k_tiles_combine_wave(const SolveDesc *__re
                 if (ent >= 78)
                     continue;
                 for (uint32_t k = 0; k < n; ++k)
-                    reg_term_entry(rt[k], ent, diag[h], gq[h], regblk);
+                    reg_term_entry(rt[k], ent, ef[h], diag[h], gq[h], regblk);
             }
             wave_lds_sync();
         }
