@@ -216,7 +216,9 @@ def cpu_baseline(items, seconds: float, what: str) -> dict:
 def oracle_check(solver, sc, prod, node_motion_pred=None):
     """Warp-field error of the solver's last solve against the FP64 oracle
     replaying its rotation-cutoff decisions; also how many of those decisions
-    differ from the ones the oracle would take itself."""
+    differ from the ones the oracle would take itself, and how far the GPU's
+    |w|_1 lies from the oracle's (the largest deviation over the solve, as a
+    fraction of its GN iteration's largest |w|_1)."""
     import numpy as np
     import torch
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -224,10 +226,13 @@ def oracle_check(solver, sc, prod, node_motion_pred=None):
     from starhub_amd import workload
     gpu = workload.device_dq_to_wxyz(solver.node_se3())
     cut = solver.debug("rot_cut", torch.uint8).cpu().numpy()
+    own = solver.debug("twist_l1", torch.float32).cpu().numpy()
     ref = sho.solve(workload.host_inputs(sc, prod, node_motion_pred), sc.cfg.solver, 64, sho.max_threads(),
                     rot_cut=cut)
-    flips = int(np.sum((ref["twist_l1"] < float(np.float32(1e-4))) != cut.astype(bool)))
-    return workload.warp_error(gpu, ref["dq"].reshape(-1, 8)), flips, ref
+    t = ref["twist_l1"].reshape(-1, len(gpu))
+    flips = int(np.sum((t < float(np.float32(1e-4))) != cut.reshape(t.shape).astype(bool)))
+    dev = float((np.abs(own.reshape(t.shape) - t) / t.max(axis=1, keepdims=True)).max())
+    return workload.warp_error(gpu, ref["dq"].reshape(-1, 8)), flips, ref, dev
 
 
 def plain_leg(args, name: str = "c2") -> dict:
@@ -270,9 +275,10 @@ def plain_leg(args, name: str = "c2") -> dict:
            "achieved_GBps": solve_bytes(cfg, P, U) * steps / dt / 1e9}
     out["roofline"] = roofline(phases, lambda ph: phase_bytes(ph, cfg, P, U) * phases[ph][1], tuning, name)
     if not args.no_check:
-        err, flips, ref = oracle_check(solver, sc, prod)
+        err, flips, ref, dev = oracle_check(solver, sc, prod)
         out["warp_error_vs_fp64"] = err
         out["cutoff_flips"] = flips
+        out["twist_l1_deviation"] = dev
         if name == "c3":
             # the warp-field error after every GN iteration: the same solve with
             # gn_iterations = k against the replayed oracle's state after k

@@ -228,7 +228,8 @@ int sh_solver_sizes(const sh_solver *solver, uint32_t *num_pixels, uint32_t *num
  * tile_cbase, tile_sbase, list_off, blk_transpose, reg, blk, dinv, rhs, x,
  * dq0, dq1, delta, rot_cut (uint8, gn_iterations x num_nodes: 1 where the
  * warp update dropped the node's rotation increment because |w|_1 < 1e-4,
- * SolverIterationData.cu:42). */
+ * SolverIterationData.cu:42), twist_l1 (float, gn_iterations x num_nodes:
+ * the FP32 |w|_1 each of those decisions was taken on). */
 int sh_solver_debug_view(const sh_solver *solver, const char *name, const void **device_ptr, size_t *count);
 
 /* FP64 total weighted cost sum lambda^2 r^2 (dense depth + flow, ARAP) at the start

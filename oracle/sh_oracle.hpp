@@ -1331,9 +1331,9 @@ public:
                 to_double(Dinv, res.diag_inv);
                 to_double(x, res.pcg_x);
             }
+            // in T, the sum rotation_cut tests
             for (int n = 0; n < N; ++n)
-                res.twist_l1.push_back(std::fabs(double(x[6 * n])) + std::fabs(double(x[6 * n + 1])) +
-                                       std::fabs(double(x[6 * n + 2])));
+                res.twist_l1.push_back(double(std::fabs(x[6 * n]) + std::fabs(x[6 * n + 1]) + std::fabs(x[6 * n + 2])));
             update(it);
             dump_dq(res.dq_iter);
         }

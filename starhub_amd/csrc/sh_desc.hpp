@@ -95,6 +95,7 @@ struct SolveDesc
     double *cost;     // weighted cost (dense + ARAP) at the start of each GN iteration, then the final one
     double *cost_part; // work-group partials: ceil(P/256) dense, then ceil((E+N)/256) ARAP
     uint8_t *rot_cut;  // gn_iterations * N: 1 where the update dropped a node's rotation (|w|_1 < 1e-4)
+    float *twist_l1;   // gn_iterations * N: the FP32 |w|_1 that decision was taken on
     // matrix-free scratch
     float *mf_z;      // P*6 per-pixel H_p s_p
     float *mf_reg;    // E*3 per-edge J x
@@ -142,7 +143,7 @@ __device__ __forceinline__ SolveDesc global_desc(const SolveDesc &src)
     SH_G(tc_contrib); SH_G(tc_slot_first); SH_G(list_off); SH_G(list_items); SH_G(slot_pos); SH_G(blk_transpose); SH_G(tpart);
     SH_G(dq[0]); SH_G(dq[1]); SH_G(jr); SH_G(reg); SH_G(ttrans); SH_G(blk); SH_G(dinv); SH_G(rhs);
     SH_G(x); SH_G(r); SH_G(s); SH_G(q); SH_G(dv[0]); SH_G(dv[1]); SH_G(mv[0]); SH_G(mv[1]);
-    SH_G(delta); SH_G(cost); SH_G(cost_part); SH_G(rot_cut); SH_G(mf_z); SH_G(mf_reg);
+    SH_G(delta); SH_G(cost); SH_G(cost_part); SH_G(rot_cut); SH_G(twist_l1); SH_G(mf_z); SH_G(mf_reg);
 #undef SH_G
     return d;
 }

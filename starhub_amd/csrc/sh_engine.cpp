@@ -204,6 +204,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         A(d.cost, size_t(c.gn_iterations + 2) * 8);
         A(d.cost_part, (size_t(div_up(pcap_, 256)) + div_up(ecap_ + ncap_, 256) + 1) * 8);
         A(d.rot_cut, size_t(std::max(1u, c.gn_iterations)) * ncap_);
+        A(d.twist_l1, size_t(std::max(1u, c.gn_iterations)) * ncap_ * 4);
     }
     // shared index-build workspace
     const size_t npix = size_t(pcap_);
@@ -1599,6 +1600,7 @@ int Engine::debug_view(uint32_t slot, const char *name, const void **ptr, size_t
         {"dq1", d.dq[1], N * 8},
         {"delta", d.delta, size_t(cfg_.pcg_iterations) + 1},
         {"rot_cut", d.rot_cut, size_t(cfg_.gn_iterations) * N},
+        {"twist_l1", d.twist_l1, size_t(cfg_.gn_iterations) * N},
     };
     for (const V &v : views)
         if (n == v.name)

@@ -31,14 +31,18 @@ __device__ inline void reduce_cost(const SolveDesc &d, int slot)
 // T <- exp(xi) T for node n, xi = xv[0..5] (FP32, as the PCG stores x), stored
 // as a dual quaternion (ApplyWarpFieldUpdateKernel, SolverIterationData.cu:19-74).
 // Records whether the rotation was dropped (|w|_1 < 1e-4, :42) for this GN
-// iteration, so a check can replay the same decisions.
+// iteration, and the |w|_1 it was decided on, so a check can replay the same
+// decisions and compare the increments.
 __device__ inline void warp_update_node(const SolveDesc &d, uint32_t n, int parity, const float *xv)
 {
     const f3 w = mk3(xv[0], xv[1], xv[2]);
     const f3 t = mk3(xv[3], xv[4], xv[5]);
     rt34 S;
-    const bool cut = fabsf(w.x) + fabsf(w.y) + fabsf(w.z) < 1e-4f;
-    d.rot_cut[size_t(d.gn_iter + parity) * uint32_t(d.N) + n] = cut ? 1u : 0u;
+    const float l1 = fabsf(w.x) + fabsf(w.y) + fabsf(w.z);
+    const bool cut = l1 < 1e-4f;
+    const size_t rec = size_t(d.gn_iter + parity) * uint32_t(d.N) + n;
+    d.rot_cut[rec] = cut ? 1u : 0u;
+    d.twist_l1[rec] = l1;
     if (cut)
     {
         S.r[0] = 1.f;

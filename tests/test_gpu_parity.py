@@ -11,7 +11,7 @@ from pathlib import Path
 
 import numpy as np
 import pytest
-from warp_check import WARP_TOL, CUTOFF, check_decisions, gpu_decisions, replay_check, solver_check
+from warp_check import WARP_TOL, CUTOFF, check_decisions, gpu_decisions, gpu_twist, replay_check, solver_check
 
 pytestmark = pytest.mark.gpu
 
@@ -307,12 +307,13 @@ def test_golden_fixture(torch_dev, name):
     assert np.array_equal(sym, g["sym_keys"])
     gpu = workload.device_dq_to_wxyz(solver.node_se3())
     cut = gpu_decisions(solver)
-    flips = check_decisions(g, cut)
+    own = gpu_twist(solver)
+    flips, _ = check_decisions(g, cut, own, cfg.num_nodes)
     if flips == 0:  # the GPU took the fixture's own decisions: its field is the reference
         assert workload.warp_error(gpu, g["dq"].reshape(-1, 8)) <= WARP_TOL
     else:
         import sho
-        replay_check(gpu, cut, workload.host_inputs(sc, prod), cfg.solver, sho.max_threads())
+        replay_check(gpu, cut, own, workload.host_inputs(sc, prod), cfg.solver, sho.max_threads())
 
 
 @pytest.mark.parametrize("which", ["c1", "c2"])
@@ -372,7 +373,8 @@ def test_batch_equals_solo(torch_dev):
         assert np.array_equal(a, b), i
         # the member's bits are its solo solve's, so the solo decisions are the member's
         err, _, flips = replay_check(workload.device_dq_to_wxyz(batch.node_se3(i)), gpu_decisions(solo),
-                                     workload.host_inputs(s, prod), s.cfg.solver, sho.max_threads())
+                                     gpu_twist(solo), workload.host_inputs(s, prod), s.cfg.solver,
+                                     sho.max_threads())
         say("batch member", i, "warp error vs replayed fp64", err, "flips", flips)
 
 
