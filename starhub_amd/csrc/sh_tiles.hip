@@ -132,6 +132,31 @@ __device__ __forceinline__ uint32_t group_exclusive_scan(uint32_t v, uint32_t *s
     return before + x - v;
 }
 
+// Minimum of lo and maximum of hi over the work-group (every thread gets both).
+__device__ __forceinline__ void group_min_max(uint32_t &lo, uint32_t &hi, uint32_t *sh)
+{
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        lo = min(lo, uint32_t(__shfl_xor(lo, o, 64)));
+        hi = max(hi, uint32_t(__shfl_xor(hi, o, 64)));
+    }
+    if (lane == 0)
+    {
+        sh[w] = lo;
+        sh[4 + w] = hi;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kTileThreads / 64; ++i)
+    {
+        lo = min(lo, sh[i]);
+        hi = max(hi, sh[4 + i]);
+    }
+    __syncthreads();
+}
+
 // ------------------------------------------------------------ index build
 // Per tile: contributions keyed (block << 7 | pixel), sorted in LDS.  Writes,
 // at the tile's fixed stride in the workspace, the packed contributions
@@ -219,11 +244,29 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_build(SolveDesc d, uint16
             }
         }
     }
-    // keys are (block << 7 | pixel) with block < N + 2U: sort only the bits in
-    // use (an invalid key's low bits, all ones, still sort after every valid key)
-    const uint32_t nb_keys = (d.row_ptr[d.N] + uint32_t(d.N)) << 7;
-    const unsigned end_bit = min(32u, 32u - unsigned(__clz(nb_keys)));
-    Sort().sort(keys, vals, sort_storage, 0u, end_bit);
+    // keys are (block << 7 | pixel), made relative to the tile's smallest
+    // block, and sorted on their block bits only: the items are generated in
+    // pixel order and the radix sort is stable, so each block's items stay in
+    // pixel order, as with the whole key, in fewer passes.  An invalid key
+    // (all ones) sorts after every valid one: the relative blocks stay below
+    // the largest value the sorted bits hold.
+    uint32_t bmin = 0xffffffffu, bmax = 0u;
+#pragma unroll
+    for (int i = 0; i < G::kItems; ++i)
+        if (keys[i] != 0xffffffffu)
+        {
+            bmin = min(bmin, keys[i] >> 7);
+            bmax = max(bmax, keys[i] >> 7);
+        }
+    group_min_max(bmin, bmax, red);
+    if (bmin == 0xffffffffu)
+        bmin = bmax = 0u; // a tile with no valid item
+#pragma unroll
+    for (int i = 0; i < G::kItems; ++i)
+        if (keys[i] != 0xffffffffu)
+            keys[i] -= bmin << 7;
+    const unsigned block_bits = 32u - unsigned(__clz(bmax - bmin + 1u));
+    Sort().sort(keys, vals, sort_storage, 7u, min(32u, 7u + block_bits));
 
     // slot heads: item gi = tid * kItems + i starts a slot when its block or
     // its pixel group (kSlotPix consecutive pixels of the tile) differs from
@@ -257,7 +300,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_build(SolveDesc d, uint16
             cout[gi] = uint16_t((keys[i] & 127u) | (uint32_t(vals[i]) << 7));
         if (head)
         {
-            ws_slot_block[size_t(t) * stride + slot] = keys[i] >> 7;
+            ws_slot_block[size_t(t) * stride + slot] = (keys[i] >> 7) + bmin;
             ws_slot_first[size_t(t) * stride + slot] = uint16_t(gi);
             ++slot;
         }
@@ -585,39 +628,63 @@ struct RegTerm
 };
 constexpr uint32_t kTermSub = 32; // terms staged in LDS at a time
 
-// The features of ARAP edge or node-translation term t of row i's list
-// (node_graph_reg_jacobian.cuh:166-358, node_motion_jacobian.cuh:7-35);
-// col_of(j) gives node j's block position in row i, for an edge's cross block.
-template <class ColOf>
-__device__ __forceinline__ RegTerm stage_reg_term(const SolveDesc &d, uint32_t i, uint32_t t, ColOf col_of)
+// What a term's features are made of, as loaded from its records.
+struct RawTerm
 {
-    RegTerm r{};
+    float4 u0, u1;   // ARAP: the edge record (Ti_xj, Tj_xj, weight, valid); translation: t and pred
+    uint32_t other;  // ARAP: the edge's other node (the row node itself for none); translation: ~0u
+    bool is_x;       // ARAP: the row node is the edge's x side
+};
+
+// The loads of ARAP edge or node-translation term t of row i's list
+// (node_graph_reg_jacobian.cuh:166-358, node_motion_jacobian.cuh:7-35).
+__device__ __forceinline__ RawTerm load_reg_term(const SolveDesc &d, uint32_t i, uint32_t t)
+{
+    RawTerm w{};
     const uint32_t P = d.P, E = d.E;
-    float pm[3] = {0.f, 0.f, 0.f}, po[3] = {0.f, 0.f, 0.f}, res[3];
     if (t < P + E)
     {
         const uint32_t e = t - P;
         const float4 *rec = reinterpret_cast<const float4 *>(d.reg + size_t(e) * kReg);
-        const float4 u0 = rec[0], u1 = rec[1];
+        w.u0 = rec[0];
+        w.u1 = rec[1];
         const uint32_t ex = d.edges[3 * e], ey = d.edges[3 * e + 1];
-        const bool is_x = (ex == i);
-        const uint32_t other = is_x ? ey : ex;
-        const float pa[3] = {u0.x, u0.y, u0.z}, pb[3] = {u0.w, u1.x, u1.y};
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-        {
-            pm[q] = is_x ? pa[q] : pb[q];
-            po[q] = is_x ? pb[q] : pa[q];
-            res[q] = pm[q] - po[q];
-        }
-        r.ws = (u1.w == 0.f) ? 0.f : d.lam2_reg * (u1.z * u1.z);
-        r.col = other != i ? col_of(other) : -1; // every ARAP edge of the row is a block of it
+        w.is_x = (ex == i);
+        w.other = w.is_x ? ey : ex;
     }
     else
     {
         const uint32_t m = t - P - E;
-        const float4 pred = d.node_pred ? d.node_pred[m] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const float4 tt = d.ttrans[m];
+        w.u0 = d.ttrans[m];
+        w.u1 = d.node_pred ? d.node_pred[m] : make_float4(0.f, 0.f, 0.f, 0.f);
+        w.other = ~0u;
+    }
+    return w;
+}
+
+// The features of a loaded term of row i; col_of(j) gives node j's block
+// position in row i, for an edge's cross block.
+template <class ColOf>
+__device__ __forceinline__ RegTerm make_reg_term(const SolveDesc &d, uint32_t i, const RawTerm &w, ColOf col_of)
+{
+    RegTerm r{};
+    float pm[3] = {0.f, 0.f, 0.f}, po[3] = {0.f, 0.f, 0.f}, res[3];
+    if (w.other != ~0u)
+    {
+        const float pa[3] = {w.u0.x, w.u0.y, w.u0.z}, pb[3] = {w.u0.w, w.u1.x, w.u1.y};
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+        {
+            pm[q] = w.is_x ? pa[q] : pb[q];
+            po[q] = w.is_x ? pb[q] : pa[q];
+            res[q] = pm[q] - po[q];
+        }
+        r.ws = (w.u1.w == 0.f) ? 0.f : d.lam2_reg * (w.u1.z * w.u1.z);
+        r.col = w.other != i ? col_of(w.other) : -1; // every ARAP edge of the row is a block of it
+    }
+    else
+    {
+        const float4 tt = w.u0, pred = w.u1;
         r.ws = d.lam2_nt * (pred.w * pred.w);
         res[0] = tt.x - pred.x;
         res[1] = tt.y - pred.y;
@@ -742,7 +809,8 @@ __device__ void row_reg_terms(const SolveDesc &d, uint32_t i, uint32_t beg, uint
         {
             const uint32_t n = min(kTermSub, r1 - s0);
             if (tid < n)
-                rt[tid] = stage_reg_term(d, i, d.n2t_terms[s0 + tid], [&](uint32_t j) { return int(colmap[j]); });
+                rt[tid] = make_reg_term(d, i, load_reg_term(d, i, d.n2t_terms[s0 + tid]),
+                                        [&](uint32_t j) { return int(colmap[j]); });
             __syncthreads();
             if (tid < 78)
                 for (uint32_t k = 0; k < n; ++k)
@@ -1108,12 +1176,13 @@ __global__ __launch_bounds__(64) void k_tiles_combine_wave(const SolveDesc *__re
     const uint32_t base = d.row_ptr[i] + i;
     const uint32_t L1 = d.row_ptr[i + 1] - d.row_ptr[i] + 1;
     constexpr uint32_t kG = kTileRec / 4;
-    // LDS: run[L1][7] | regblk[L1][36] | cols[L1] | ls[L1] | le[L1]
+    // LDS: run[L1][7] | regblk[L1][36] | cols[L1] | ls[L1] | le[L1] | trb[L1]
     float4 *run = smem4;
     float *regblk = reinterpret_cast<float *>(run + size_t(L1) * kG);
     uint32_t *cols = reinterpret_cast<uint32_t *>(regblk + size_t(L1) * 36);
     uint32_t *ls = cols + L1;
     uint32_t *le = ls + L1;
+    uint32_t *trb = le + L1;
 
     // columns: the diagonal first, then the ascending neighbours; how many lie
     // below i (the upper blocks start after them)
@@ -1141,6 +1210,7 @@ __global__ __launch_bounds__(64) void k_tiles_combine_wave(const SolveDesc *__re
         const uint32_t c = q == 0 ? 0u : c_up + q - 1;
         ls[q] = d.list_off[base + c];
         le[q] = d.list_off[base + c + 1];
+        trb[q] = d.blk_transpose[base + c];
     }
     wave_lds_sync();
 
@@ -1182,7 +1252,7 @@ __global__ __launch_bounds__(64) void k_tiles_combine_wave(const SolveDesc *__re
         {
             const uint32_t n = min(kTermSub, r1 - s0);
             if (lane < n)
-                rt[lane] = stage_reg_term(d, i, d.n2t_terms[s0 + lane], [&](uint32_t j) {
+                rt[lane] = make_reg_term(d, i, load_reg_term(d, i, d.n2t_terms[s0 + lane]), [&](uint32_t j) {
                     // the ascending neighbours cols[1..L1)
                     uint32_t lo = 1, hi = L1;
                     while (lo < hi)
@@ -1227,7 +1297,7 @@ __global__ __launch_bounds__(64) void k_tiles_combine_wave(const SolveDesc *__re
     {
         const uint32_t q = task / kG, g = task % kG;
         const uint32_t c = q == 0 ? 0u : c_up + q - 1;
-        const size_t tr = size_t(d.blk_transpose[base + c]) * 36; // (j, i) for c > 0
+        const size_t tr = size_t(trb[q]) * 36; // (j, i) for c > 0
         const float4 acc = run[task];
         const float av[4] = {acc.x, acc.y, acc.z, acc.w};
 #pragma unroll
@@ -1637,7 +1707,7 @@ size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len, bool wave)
 {
     const size_t L1 = size_t(max_row_len) + 1;
     if (wave)
-        return sizeof(float4) * L1 * (kTileRec / 4) + sizeof(float) * L1 * 36 + sizeof(uint32_t) * 3 * L1;
+        return sizeof(float4) * L1 * (kTileRec / 4) + sizeof(float) * L1 * 36 + sizeof(uint32_t) * 4 * L1;
     return sizeof(float4) * (kCombUnionF4 + L1 * (kTileRec / 4)) + sizeof(float) * (L1 * 36 + 16) +
            sizeof(uint32_t) * (3 * L1 + 1) + sizeof(uint16_t) * ((max_nodes + 1) & ~1u);
 }
