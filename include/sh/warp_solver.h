@@ -79,7 +79,13 @@ typedef enum sh_solver_flags
     /* Register-resident PCG in its pipelined form (Ghysels-Vanroose): one barrier per
      * iteration instead of two, FP64 vectors.  The same iterates as the reference's PCG
      * in exact arithmetic (BlockPCG.cuh:325-483). */
-    SH_FLAG_PIPELINED_PCG = 256u
+    SH_FLAG_PIPELINED_PCG = 256u,
+    /* Pixels per tile of the tile materialisation: 64 or 256 instead of 128 (at most
+     * one of the two).  Smaller tiles give more work-groups and shorter slot lists,
+     * larger ones fewer slot partials per block.  Tunable; a change takes effect at
+     * the next index build. */
+    SH_FLAG_TILE_PIX_64 = 512u,
+    SH_FLAG_TILE_PIX_256 = 1024u
 } sh_solver_flags;
 
 /* Dual quaternion as stored by the reference: two float4 (x, y, z, w),
@@ -545,7 +551,7 @@ int sh_server_get_stats(sh_server *server, sh_server_stats *stats);
 typedef struct sh_tuning
 {
     uint32_t flags;              /* subset of SH_FLAG_ROW_SEGMENTS | SH_FLAG_STREAMING_PCG | SH_FLAG_NO_PERSISTENT_PCG |
-                                    SH_FLAG_PIPELINED_PCG */
+                                    SH_FLAG_PIPELINED_PCG | SH_FLAG_TILE_PIX_64 | SH_FLAG_TILE_PIX_256 */
     uint32_t spmv_mode;          /* sh_spmv_mode */
     uint32_t pcg_rows_per_group; /* register-resident PCG: node rows per work-group, 1..85 (0 = 85) */
     double ms;                   /* autotune: median solve time of the chosen tuning */
@@ -554,8 +560,10 @@ typedef struct sh_tuning
 /* Times each candidate tuning on the inputs already set (median of `repeats`
  * solves after one warm-up, HIP events on `stream`), keeps the fastest and
  * returns it; a candidate must beat an earlier one by 2% to replace it, so
- * near-ties keep the earlier (default-first) choice.  Every candidate solves
- * the same problem to the same tolerance. */
+ * near-ties keep the earlier (default-first) choice.  The candidates are the
+ * materialisation paths and PCG variants with their rows per group, then the
+ * tile sizes 64 and 256 for a winner on the tile path.  Every candidate
+ * solves the same problem to the same tolerance. */
 int sh_solver_autotune(sh_solver *solver, sh_stream_t stream, uint32_t repeats, sh_tuning *best);
 int sh_solver_set_tuning(sh_solver *solver, const sh_tuning *tuning);
 int sh_batch_autotune(sh_batch *batch, uint32_t num_solves, sh_stream_t stream, uint32_t repeats, sh_tuning *best);

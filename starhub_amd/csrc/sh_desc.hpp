@@ -48,7 +48,7 @@ struct SolveDesc
     float *pw;        // P*K, spatial * connect
     // the same streams in the order the GN phase reads them (the per-pixel
     // records follow it): tile order -- 32x32-pixel blocks in raster order,
-    // Morton order inside a block -- on the tile path, so that 128 consecutive
+    // Morton order inside a block -- on the tile path, so that 64 to 256 consecutive
     // pixels cover a compact patch; the raster arrays above on the others
     ushort4 *gpix;
     uint16_t *gpknn;
@@ -69,8 +69,9 @@ struct SolveDesc
     int n_seg;              // dense-term segments in all
     uint32_t part_stride;   // floats per segment partial: 8 (rhs) + 21 per block of the longest row
     float *part;            // n_seg * part_stride partial block rows
-    // pixel tiles (kTilePix compacted pixels each) and their slot partials
+    // pixel tiles (tile_pix compacted pixels each) and their slot partials
     int n_tiles;
+    int tile_pix;            // pixels per tile (64, 128 or 256) of the index build
     uint32_t *tile_cbase;    // n_tiles+1: first contribution of each tile
     uint32_t *tile_sbase;    // n_tiles+1: first slot of each tile
     uint16_t *tc_contrib;    // contributions, per slot in pixel order: pixel | ka << 7 | kb << 10

@@ -123,7 +123,8 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         return SH_E_ARG;
     if (c.pcg_iterations == 0 || c.pcg_iterations > 1000 || c.gn_iterations > 1000)
         return SH_E_ARG;
-    if (c.stream_precision > SH_STREAM_FP16 || c.spmv_mode > SH_SPMV_AUTO || (c.flags & ~511u))
+    if (c.stream_precision > SH_STREAM_FP16 || c.spmv_mode > SH_SPMV_AUTO || (c.flags & ~2047u) ||
+        ((c.flags & SH_FLAG_TILE_PIX_64) && (c.flags & SH_FLAG_TILE_PIX_256)))
         return SH_E_ARG;
     cfg_ = c;
     if (cfg_.flags & SH_FLAG_PAIR_MAJOR)
@@ -182,8 +183,8 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         A(d.dq[1], size_t(ncap_) * 32);
         A(d.dense_cnt, size_t(ncap_) * 4);
         A(d.seg_ptr, size_t(ncap_ + 1) * 4);
-        A(d.tile_cbase, size_t(div_up(pcap_, kTilePix) + 1) * 4);
-        A(d.tile_sbase, size_t(div_up(pcap_, kTilePix) + 1) * 4);
+        A(d.tile_cbase, size_t(div_up(pcap_, kTilePixMin) + 1) * 4);
+        A(d.tile_sbase, size_t(div_up(pcap_, kTilePixMin) + 1) * 4);
         A(d.list_off, (size_t(ncap_) + 2 * size_t(ucap_) + 1) * 4);
         A(d.blk_transpose, (size_t(ncap_) + 2 * size_t(ucap_)) * 4);
         A(d.jr, size_t(pcap_) * kHG * (fp16_ ? 2 : 4));
@@ -225,12 +226,13 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
     A(ws_bitmap_, size_t(ncap_) * kBitmapWords * 4);
     A(ws_rowlen_, size_t(ncap_ + 1) * 4);
     A(ws_segcnt_, size_t(ncap_ + 1) * 4);
-    tiles_cap_ = div_up(pcap_, kTilePix);
+    tiles_cap_ = div_up(pcap_, kTilePixMin); // the most tiles: the smallest tile size
     {
-        const size_t tstride = size_t(kTilePix) * (K_ * (K_ + 1) / 2);
-        A(ws_tcontrib_, size_t(tiles_cap_) * tstride * 2);
-        A(ws_tslot_block_, size_t(tiles_cap_) * tstride * 4);
-        A(ws_tslot_first_, size_t(tiles_cap_) * tstride * 2);
+        // contributions at each tile's fixed stride: the tiles' pixels rounded up to a whole largest tile
+        const size_t tentries = (size_t(pcap_) + kTilePixMax) * size_t(K_ * (K_ + 1) / 2);
+        A(ws_tcontrib_, tentries * 2);
+        A(ws_tslot_block_, tentries * 4);
+        A(ws_tslot_first_, tentries * 2);
         A(ws_tnslot_, size_t(tiles_cap_ + 2) * 4); // counts, scan total, largest count
         A(ws_tncontrib_, size_t(tiles_cap_ + 1) * 4);
     }
@@ -418,10 +420,10 @@ int Engine::alloc_lane(IndexLane &ln, size_t npix)
     L(bitmap, uint32_t, size_t(ncap_) * kBitmapWords * 4);
     L(rowlen, uint32_t, size_t(ncap_ + 1) * 4);
     L(segcnt, uint32_t, size_t(ncap_ + 1) * 4);
-    const size_t tstride = size_t(kTilePix) * (K_ * (K_ + 1) / 2);
-    L(tcontrib, uint16_t, size_t(tiles_cap_) * tstride * 2);
-    L(tslot_block, uint32_t, size_t(tiles_cap_) * tstride * 4);
-    L(tslot_first, uint16_t, size_t(tiles_cap_) * tstride * 2);
+    const size_t tentries = (size_t(pcap_) + kTilePixMax) * size_t(K_ * (K_ + 1) / 2);
+    L(tcontrib, uint16_t, tentries * 2);
+    L(tslot_block, uint32_t, tentries * 4);
+    L(tslot_first, uint16_t, tentries * 2);
     L(tnslot, uint32_t, size_t(tiles_cap_ + 2) * 4);
     L(tncontrib, uint32_t, size_t(tiles_cap_ + 1) * 4);
 #undef L
@@ -446,6 +448,7 @@ int Engine::index_pixels(uint32_t slot, IndexLane &ln, hipStream_t st)
     d.gpix = tiled ? sl.tpix : d.pix;
     d.gpknn = tiled ? sl.tpknn : d.pknn;
     d.gpw = tiled ? sl.tpw : d.pw;
+    d.tile_pix = int(tile_pix());
     // ln.err is zero here: zeroed at allocation, by every build's compaction
     // and on the rejection path
     SH_HIP(launch_pixel_flags(d, n, ln.flags, tiled ? ln.tflags : nullptr, ln.err, st));
@@ -477,7 +480,7 @@ int Engine::index_rows(uint32_t slot, IndexLane &ln, hipStream_t st)
     if (P > pcap_)
         return SH_E_CAPACITY;
     d.P = int(P);
-    const uint32_t nt = div_up(P, kTilePix);
+    const uint32_t nt = div_up(P, uint32_t(d.tile_pix));
     const IndexZero z{ln.err, ln.meta, ln.rowlen + d.N, ln.segcnt + d.N, ln.tnslot + nt, ln.tncontrib + nt};
     SH_HIP(launch_compact_pixels(d, ln.flags, ln.pos, d.gpix != d.pix ? ln.tpos : nullptr, z, st));
     // node -> term (Node2TermsIndex::BuildIndex, Node2TermsIndex.cu:191-199,312-360)
@@ -655,6 +658,7 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
     d.gpix = tiled ? sl.tpix : d.pix;
     d.gpknn = tiled ? sl.tpknn : d.pknn;
     d.gpw = tiled ? sl.tpw : d.pw;
+    d.tile_pix = int(tile_pix());
     uint32_t *tflags = static_cast<uint32_t *>(ws_tflags_), *tpos = static_cast<uint32_t *>(ws_tpos_);
     SH_HIP(launch_pixel_flags(d, n, flags, tiled ? tflags : nullptr, err, st));
     SH_HIP(exclusive_scan_u32(sort_temp_, tb, flags, pos, n, st));
@@ -673,7 +677,7 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
     if (P > pcap_)
         return SH_E_CAPACITY;
     d.P = int(P);
-    const uint32_t nt = div_up(P, kTilePix);
+    const uint32_t nt = div_up(P, uint32_t(d.tile_pix));
     const IndexZero z{err, static_cast<uint32_t *>(ws_meta_), static_cast<uint32_t *>(ws_rowlen_) + d.N,
                       static_cast<uint32_t *>(ws_segcnt_) + d.N, lanes_[0].tnslot + nt, lanes_[0].tncontrib + nt};
     SH_HIP(launch_compact_pixels(d, flags, pos, tiled ? tpos : nullptr, z, st));
@@ -751,7 +755,7 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
 // (into h_meta_[10], h_meta_[11]) at the next synchronisation.
 hipError_t Engine::launch_tile_build_async(SolveDesc &d, IndexLane &ln, hipStream_t st)
 {
-    d.n_tiles = int(div_up(uint32_t(d.P), kTilePix));
+    d.n_tiles = int(div_up(uint32_t(d.P), uint32_t(d.tile_pix)));
     const uint32_t nt = uint32_t(d.n_tiles);
     uint32_t *ns = ln.tnslot, *nc = ln.tncontrib;
     hipError_t e;
@@ -999,6 +1003,11 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
                                         map_total_[kMapCoop] <= uint32_t(coop_max_groups_)));
     uint32_t *flags = static_cast<uint32_t *>(d_flags_);
     const RowsPlan plan = rows_plan(max_n, max_l);
+    // every solve of the launch was indexed with the tile size in use (a tuning
+    // change takes effect at the next index build)
+    for (uint32_t s = 0; s < ns; ++s)
+        if (slots_[batch_[s]].d.tile_pix != int(tile_pix()))
+            return SH_E_STATE;
     // the tile combine: one wave per row when the launch has enough rows to
     // fill the CUs several times over (batches, server steps); one work-group
     // per row otherwise, where a single solve's rows finish sooner.  Both give
@@ -1034,7 +1043,7 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
         else
         {
             SH_HIP(mark(kPhTiles, st));
-            SH_HIP(launch_tiles(K_, fp16, sds, map(kMapTile, ns), n, map_total_[kMapTile], slot_cap, st));
+            SH_HIP(launch_tiles(K_, fp16, tile_pix(), sds, map(kMapTile, ns), n, map_total_[kMapTile], slot_cap, st));
             SH_HIP(mark(kPhTilesCombine, st));
             SH_HIP(launch_tiles_combine(sds, map(kMapRow, ns), n, map_total_[kMapRow], tiles_lds, comb_wave,
                                         static_cast<PcgSlot2 *>(d_slots2_), pipe_slots, st));
@@ -1105,7 +1114,8 @@ hipError_t Engine::enqueue_pcg_launches(uint32_t ns, bool matrix_free, hipStream
             // ApplyJtJ without the blocks: d_k, then q = J^T W J d_k from the
             // per-pixel records, then the dot and the update
             SH_TRY(launch_pcg_dir(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), prs, st));
-            SH_TRY(launch_mf_tiles(K_, fp16_, sds, map(kMapTile, ns), n, map_total_[kMapTile], int(k), nullptr, st));
+            SH_TRY(launch_mf_tiles(K_, fp16_, tile_pix(), sds, map(kMapTile, ns), n, map_total_[kMapTile], int(k),
+                                   nullptr, st));
             SH_TRY(launch_mf_rows(sds, map(kMapRow, ns), n, map_total_[kMapRow], int(k), nullptr, nullptr, st));
             SH_TRY(launch_pcg_dot(sds, map(kMapPcg, ns), n, map_total_[kMapPcg], int(k), pdq, st));
         }
@@ -1297,7 +1307,8 @@ int Engine::apply_jtj(uint32_t slot, const float *x, float *y, bool matrix_free,
     const SolveDesc *sds = static_cast<const SolveDesc *>(d_descs_);
     if (matrix_free)
     {
-        SH_HIP(launch_mf_tiles(K_, fp16_, sds, map(kMapTile, 1), 1, map_total_[kMapTile], 0, x, st));
+        SH_HIP(launch_mf_tiles(K_, fp16_, uint32_t(slots_[slot].d.tile_pix), sds, map(kMapTile, 1), 1,
+                               map_total_[kMapTile], 0, x, st));
         SH_HIP(launch_mf_rows(sds, map(kMapRow, 1), 1, map_total_[kMapRow], 0, x, y, st));
     }
     else
@@ -1308,12 +1319,13 @@ int Engine::apply_jtj(uint32_t slot, const float *x, float *y, bool matrix_free,
 
 // Tuning that can change between solves without reallocating: the
 // materialisation path, the PCG variant and the resident PCG's rows per group.
-static constexpr uint32_t kTunableFlags =
-    SH_FLAG_ROW_SEGMENTS | SH_FLAG_STREAMING_PCG | SH_FLAG_NO_PERSISTENT_PCG | SH_FLAG_PIPELINED_PCG;
+static constexpr uint32_t kTunableFlags = SH_FLAG_ROW_SEGMENTS | SH_FLAG_STREAMING_PCG | SH_FLAG_NO_PERSISTENT_PCG |
+                                          SH_FLAG_PIPELINED_PCG | SH_FLAG_TILE_PIX_64 | SH_FLAG_TILE_PIX_256;
 
 int Engine::set_tuning(const sh_tuning &t)
 {
     if ((t.flags & ~kTunableFlags) || t.spmv_mode > SH_SPMV_AUTO || t.pcg_rows_per_group > kPcgResRows ||
+        ((t.flags & SH_FLAG_TILE_PIX_64) && (t.flags & SH_FLAG_TILE_PIX_256)) ||
         (cfg_.flags & (SH_FLAG_PAIR_MAJOR)) ||
         (t.spmv_mode == SH_SPMV_MATRIX_FREE && (t.flags & SH_FLAG_ROW_SEGMENTS)))
         return SH_E_ARG;
@@ -1334,6 +1346,34 @@ sh_tuning Engine::tuning() const
 
 // Times every candidate tuning on the slots' current inputs (median of
 // `repeats` solves each, HIP events on `st`) and keeps the fastest.
+// Median solve time of tuning c (after one warm-up solve) into *ms; 0 when a
+// persistent PCG's barrier wait ran out during it (it then ran as the split PCG).
+int Engine::time_candidate(const sh_tuning &c, uint32_t ns, hipStream_t st, uint32_t repeats, hipEvent_t e0,
+                           hipEvent_t e1, double *ms_out)
+{
+    int rc;
+    if ((rc = set_tuning(c)) != SH_OK)
+        return rc;
+    const uint32_t fallbacks = pcg_fallbacks_;
+    std::vector<double> t;
+    for (uint32_t r = 0; r < repeats + 1; ++r)
+    {
+        float ms = 0.f;
+        if (hipEventRecord(e0, st) != hipSuccess)
+            return SH_E_HIP;
+        if ((rc = solve(ns, st)) != SH_OK)
+            return rc;
+        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            return SH_E_HIP;
+        if (r > 0) // the first run warms up
+            t.push_back(ms);
+    }
+    std::sort(t.begin(), t.end());
+    *ms_out = pcg_fallbacks_ != fallbacks ? 0.0 : t[t.size() / 2];
+    return SH_OK;
+}
+
 int Engine::autotune(uint32_t ns, hipStream_t st, uint32_t repeats, sh_tuning *best_out)
 {
     if (cfg_.flags & SH_FLAG_PAIR_MAJOR)
@@ -1358,34 +1398,31 @@ int Engine::autotune(uint32_t ns, hipStream_t st, uint32_t repeats, sh_tuning *b
     int rc = SH_OK;
     for (sh_tuning &c : cands)
     {
-        if ((rc = set_tuning(c)) != SH_OK)
+        double ms = 0.0;
+        if ((rc = time_candidate(c, ns, st, repeats, e0, e1, &ms)) != SH_OK)
             break;
-        const uint32_t fallbacks = pcg_fallbacks_;
-        std::vector<double> t;
-        for (uint32_t r = 0; r < repeats + 1 && rc == SH_OK; ++r)
-        {
-            float ms = 0.f;
-            if (hipEventRecord(e0, st) != hipSuccess)
-                rc = SH_E_HIP;
-            if (rc == SH_OK)
-                rc = solve(ns, st);
-            if (rc == SH_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-                rc = SH_E_HIP;
-            if (r > 0) // the first run warms up
-                t.push_back(ms);
-        }
-        if (rc != SH_OK)
-            break;
-        if (pcg_fallbacks_ != fallbacks)
+        if (ms <= 0.0)
             continue; // its barrier waits ran out: timed as the split PCG, and not chosen
-        std::sort(t.begin(), t.end());
-        c.ms = t[t.size() / 2];
+        c.ms = ms;
         // a later candidate must be 2% faster to replace an earlier one, so that
         // near-ties resolve to the same (earlier, default-first) choice run to run
         if (c.ms < best.ms * (best.ms < 1e30 ? 0.98 : 1.0))
             best = c;
     }
+    // then the tile size, for a winner on the tile path: 64 and 256 pixels
+    // against the default 128 (the same 2% margin)
+    if (rc == SH_OK && best.ms < 1e30 && !(best.flags & SH_FLAG_ROW_SEGMENTS))
+        for (uint32_t tf : {uint32_t(SH_FLAG_TILE_PIX_64), uint32_t(SH_FLAG_TILE_PIX_256)})
+        {
+            sh_tuning c = best;
+            c.flags |= tf;
+            double ms = 0.0;
+            if ((rc = time_candidate(c, ns, st, repeats, e0, e1, &ms)) != SH_OK)
+                break;
+            c.ms = ms;
+            if (ms > 0.0 && ms < best.ms * 0.98)
+                best = c;
+        }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc != SH_OK)

@@ -43,6 +43,8 @@ public:
     int set_tuning(const sh_tuning &t);
     sh_tuning tuning() const;
     int autotune(uint32_t ns, hipStream_t st, uint32_t repeats, sh_tuning *best);
+    int time_candidate(const sh_tuning &c, uint32_t ns, hipStream_t st, uint32_t repeats, hipEvent_t e0,
+                       hipEvent_t e1, double *ms_out);
     // Called before a slot's grown buffer (tile lists and partials, segment
     // partials) is allocated, with the bytes the engine's holding grows by,
     // and with the negative size of what it releases; returning false refuses
@@ -197,6 +199,11 @@ private:
     int res_max_groups_ = -1;
     int pipe_max_groups_ = -1;
     uint32_t res_rows_ = 85; // resident-PCG rows per group (tunable, <= kPcgResRows)
+    // pixels per tile of the tile path (tunable: SH_FLAG_TILE_PIX_64 / _256, else 128)
+    uint32_t tile_pix() const
+    {
+        return (cfg_.flags & SH_FLAG_TILE_PIX_64) ? 64u : (cfg_.flags & SH_FLAG_TILE_PIX_256) ? 256u : 128u;
+    }
     void *d_slots_ = nullptr; // persistent-PCG barrier slots
     void *d_slots2_ = nullptr; // pipelined-PCG barrier slots (two sums each)
     void *d_flags_ = nullptr; // [0] PCG epoch base, [1] PCG barrier error

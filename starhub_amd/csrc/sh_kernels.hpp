@@ -191,8 +191,8 @@ hipError_t launch_forward_warp(uint32_t K, const float4 *warp, const float4 *ref
                                const float *n_spatial, const float *n_connect, uint32_t num_nodes, float4 *live_v,
                                float4 *live_n, float4 *live_node, hipStream_t st);
 
-// Pixel tiles (sh_tiles.hip)
-constexpr uint32_t kTilePix = 128;
+// Pixel tiles (sh_tiles.hip): 64, 128 (default) or 256 pixels, per the tuning
+constexpr uint32_t kTilePixMin = 64, kTilePixDefault = 128, kTilePixMax = 256;
 hipError_t launch_tile_build(const SolveDesc &d, uint16_t *ws_contrib, uint32_t *ws_slot_block,
                              uint16_t *ws_slot_first, uint32_t *tile_nslot, uint32_t *tile_ncontrib, hipStream_t st);
 hipError_t launch_tile_compact(const SolveDesc &d, const uint16_t *ws_contrib, const uint32_t *ws_slot_block,
@@ -201,15 +201,15 @@ hipError_t launch_tile_compact(const SolveDesc &d, const uint16_t *ws_contrib, c
 hipError_t launch_block_transpose(const SolveDesc &d, hipStream_t st);
 hipError_t launch_lower_bound_u32(const uint32_t *keys, uint32_t n_keys, uint32_t n_out, uint32_t *out,
                                   hipStream_t st);
-hipError_t launch_tiles(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
-                        uint32_t slot_cap, hipStream_t st);
+hipError_t launch_tiles(int K, bool fp16, uint32_t tile_pix, const SolveDesc *sds, const uint32_t *wmap,
+                        int n_solve, uint32_t n_wg, uint32_t slot_cap, hipStream_t st);
 // LDS of the tile combine: one work-group per row (wave = false) or one wave per row
 size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len, bool wave);
 // slot_pos[list_items[l]] = l for l < n: the list position of every slot
 hipError_t launch_slot_pos(const uint32_t *list_items, uint32_t n, uint32_t *slot_pos, hipStream_t st);
 // matrix-free J^T W J x (x = d_{k+1} of each solve unless x_user; y = d.q unless y_user)
-hipError_t launch_mf_tiles(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
-                           int k_it, const float *x_user, hipStream_t st);
+hipError_t launch_mf_tiles(int K, bool fp16, uint32_t tile_pix, const SolveDesc *sds, const uint32_t *wmap,
+                           int n_solve, uint32_t n_wg, int k_it, const float *x_user, hipStream_t st);
 hipError_t launch_mf_rows(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int k_it,
                           const float *x_user, float *y_user, hipStream_t st);
 hipError_t launch_spmv_rows(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, const float *x,

@@ -4,7 +4,7 @@
 // The reference gathers each node pair's terms again for every block
 // (JtJMaterializer.cu:66-175, PreconditionerRhsBuilder.cu:110-262), so every
 // per-pixel record is read once per pair it belongs to.  Here the compacted
-// pixels are cut into tiles of kTilePix consecutive pixels of the GN phase's
+// pixels are cut into tiles of TP (64, 128 or 256) consecutive pixels of the GN phase's
 // pixel order (tile order: 32x32-pixel blocks, Morton order inside a block),
 // so a tile covers a compact image patch and few nodes.
 //
@@ -34,23 +34,34 @@ namespace sh
 {
 constexpr int kTileThreads = 256;
 constexpr int kTileRec = 28; // staged record: 21 H + 6 g + pad = 7 float4
+// Tiles hold TP = 64, 128 or 256 pixels (the tuning picks one); a tile's
+// contributions are packed as pixel | ka << 8 | kb << 11 and keyed
+// block << 8 | pixel.
+constexpr int kPixBits = 8;
+constexpr uint32_t kPixMask = (1u << kPixBits) - 1u;
+static_assert(kTilePixMax <= (1u << kPixBits), "tile pixel bits");
+__device__ __forceinline__ uint32_t c_pix(uint32_t c) { return c & kPixMask; }
+__device__ __forceinline__ uint32_t c_ka(uint32_t c) { return (c >> kPixBits) & 7u; }
+__device__ __forceinline__ uint32_t c_kb(uint32_t c) { return (c >> (kPixBits + 3)) & 7u; }
 // A slot is a block's contributions from one group of kSlotPix consecutive
-// pixels of a tile (keys are block << 7 | pixel), so a slot's list is at most
-// kSlotPix long and k_tiles' longest serial sum is bounded.
+// pixels of a tile, so a slot's list is at most kSlotPix long and k_tiles'
+// longest serial sum is bounded.
 #ifndef SH_SLOT_SHIFT
 #define SH_SLOT_SHIFT 6
 #endif
 constexpr int kSlotShift = SH_SLOT_SHIFT;
 constexpr int kSlotPix = 1 << kSlotShift;
-static_assert((1 << kSlotShift) == kSlotPix && kSlotPix <= kTilePix, "slot pixel groups");
+static_assert((1 << kSlotShift) == kSlotPix && kSlotPix <= int(kTilePixMin), "slot pixel groups");
 
-template <int K>
+// A tile of TP pixels for K nodes per pixel: two threads per pixel.
+template <int K, int TP>
 struct TileGeom
 {
-    static constexpr int kPairs = K * (K + 1) / 2;                  // (ka <= kb) slot pairs per pixel
-    static constexpr int kItems = kTilePix * kPairs / kTileThreads; // sort items per thread
-    static_assert(kTilePix * kPairs % kTileThreads == 0, "tile geometry");
-    static_assert(kTileThreads == 2 * kTilePix, "two threads per pixel");
+    static constexpr int kPairs = K * (K + 1) / 2; // (ka <= kb) slot pairs per pixel
+    static constexpr int kThreads = 2 * TP;
+    static constexpr int kItems = kPairs / 2;      // sort items per thread
+    static constexpr int kEnt = TP * kPairs;       // contributions a tile can hold
+    static_assert(kPairs % 2 == 0 && TP <= int(kTilePixMax), "tile geometry");
 };
 
 __device__ __forceinline__ int find_solve_t(const uint32_t *__restrict__ begin, int n, uint32_t wg)
@@ -106,6 +117,7 @@ __device__ __forceinline__ void slot_pair(int q, int &ka, int &kb)
 
 // Exclusive scan of one value per thread over the work-group, in thread order;
 // returns the thread's prefix, *total gets the sum.
+template <int NT>
 __device__ __forceinline__ uint32_t group_exclusive_scan(uint32_t v, uint32_t *sh, uint32_t *total)
 {
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -122,7 +134,7 @@ __device__ __forceinline__ uint32_t group_exclusive_scan(uint32_t v, uint32_t *s
     __syncthreads();
     uint32_t before = 0, all = 0;
 #pragma unroll
-    for (int i = 0; i < kTileThreads / 64; ++i)
+    for (int i = 0; i < NT / 64; ++i)
     {
         before += (uint32_t(i) < w) ? sh[i] : 0u;
         all += sh[i];
@@ -132,7 +144,9 @@ __device__ __forceinline__ uint32_t group_exclusive_scan(uint32_t v, uint32_t *s
     return before + x - v;
 }
 
-// Minimum of lo and maximum of hi over the work-group (every thread gets both).
+// Minimum of lo and maximum of hi over the work-group (every thread gets both);
+// sh holds 16 values.
+template <int NT>
 __device__ __forceinline__ void group_min_max(uint32_t &lo, uint32_t &hi, uint32_t *sh)
 {
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -145,38 +159,38 @@ __device__ __forceinline__ void group_min_max(uint32_t &lo, uint32_t &hi, uint32
     if (lane == 0)
     {
         sh[w] = lo;
-        sh[4 + w] = hi;
+        sh[8 + w] = hi;
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < kTileThreads / 64; ++i)
+    for (int i = 0; i < NT / 64; ++i)
     {
         lo = min(lo, sh[i]);
-        hi = max(hi, sh[4 + i]);
+        hi = max(hi, sh[8 + i]);
     }
     __syncthreads();
 }
 
 // ------------------------------------------------------------ index build
-// Per tile: contributions keyed (block << 7 | pixel), sorted in LDS.  Writes,
+// Per tile: contributions keyed (block << 8 | pixel), sorted in LDS.  Writes,
 // at the tile's fixed stride in the workspace, the packed contributions
-// (pixel | ka << 7 | kb << 10) in sorted order, each slot's block and first
+// (pixel | ka << 8 | kb << 11) in sorted order, each slot's block and first
 // contribution, and the tile's slot and contribution counts.
-template <int K>
-__global__ __launch_bounds__(kTileThreads) void k_tile_build(SolveDesc d, uint16_t *__restrict__ ws_contrib,
+template <int K, int TP>
+__global__ __launch_bounds__(2 * TP) void k_tile_build(SolveDesc d, uint16_t *__restrict__ ws_contrib,
                                                              uint32_t *__restrict__ ws_slot_block,
                                                              uint16_t *__restrict__ ws_slot_first,
                                                              uint32_t *__restrict__ tile_nslot,
                                                              uint32_t *__restrict__ tile_ncontrib)
 {
-    using G = TileGeom<K>;
-    using Sort = rocprim::block_radix_sort<uint32_t, kTileThreads, G::kItems, uint8_t>;
+    using G = TileGeom<K, TP>;
+    using Sort = rocprim::block_radix_sort<uint32_t, G::kThreads, G::kItems, uint8_t>;
     __shared__ typename Sort::storage_type sort_storage;
-    __shared__ uint32_t last_key[kTileThreads];
-    __shared__ uint32_t red[8];
+    __shared__ uint32_t last_key[G::kThreads];
+    __shared__ uint32_t red[16];
     const uint32_t t = blockIdx.x, tid = threadIdx.x;
     const uint32_t pl = tid >> 1, half = tid & 1;
-    const uint32_t p = t * kTilePix + pl;
+    const uint32_t p = t * TP + pl;
     // a row structure that overflowed its capacity is rejected by the host
     // after this launch; read none of it
     if (d.row_ptr[d.N] + uint32_t(d.N) > d.blk_cap)
@@ -239,12 +253,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_build(SolveDesc d, uint16
             if (on_a && on_b)
             {
                 const uint32_t lo = min(na, nb), hi = max(na, nb);
-                keys[i] = (block_of_pair(d, lo, hi) << 7) | pl;
+                keys[i] = (block_of_pair(d, lo, hi) << kPixBits) | pl;
                 vals[i] = uint8_t(ka | (kb << 3));
             }
         }
     }
-    // keys are (block << 7 | pixel), made relative to the tile's smallest
+    // keys are (block << 8 | pixel), made relative to the tile's smallest
     // block, and sorted on their block bits only: the items are generated in
     // pixel order and the radix sort is stable, so each block's items stay in
     // pixel order, as with the whole key, in fewer passes.  An invalid key
@@ -255,18 +269,18 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_build(SolveDesc d, uint16
     for (int i = 0; i < G::kItems; ++i)
         if (keys[i] != 0xffffffffu)
         {
-            bmin = min(bmin, keys[i] >> 7);
-            bmax = max(bmax, keys[i] >> 7);
+            bmin = min(bmin, keys[i] >> kPixBits);
+            bmax = max(bmax, keys[i] >> kPixBits);
         }
-    group_min_max(bmin, bmax, red);
+    group_min_max<G::kThreads>(bmin, bmax, red);
     if (bmin == 0xffffffffu)
         bmin = bmax = 0u; // a tile with no valid item
 #pragma unroll
     for (int i = 0; i < G::kItems; ++i)
         if (keys[i] != 0xffffffffu)
-            keys[i] -= bmin << 7;
+            keys[i] -= bmin << kPixBits;
     const unsigned block_bits = 32u - unsigned(__clz(bmax - bmin + 1u));
-    Sort().sort(keys, vals, sort_storage, 7u, min(32u, 7u + block_bits));
+    Sort().sort(keys, vals, sort_storage, unsigned(kPixBits), min(32u, unsigned(kPixBits) + block_bits));
 
     // slot heads: item gi = tid * kItems + i starts a slot when its block or
     // its pixel group (kSlotPix consecutive pixels of the tile) differs from
@@ -285,9 +299,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_build(SolveDesc d, uint16
         prev = keys[i];
     }
     uint32_t total_heads = 0, total_valid = 0;
-    uint32_t slot = group_exclusive_scan(nhead, red, &total_heads);
-    (void)group_exclusive_scan(nvalid, red, &total_valid);
-    const size_t stride = size_t(kTilePix) * G::kPairs;
+    uint32_t slot = group_exclusive_scan<G::kThreads>(nhead, red, &total_heads);
+    (void)group_exclusive_scan<G::kThreads>(nvalid, red, &total_valid);
+    const size_t stride = size_t(G::kEnt);
     uint16_t *__restrict__ cout = ws_contrib + size_t(t) * stride;
     prev = prev0;
 #pragma unroll
@@ -297,10 +311,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_build(SolveDesc d, uint16
         const bool valid = keys[i] != 0xffffffffu;
         const bool head = valid && (prev == 0xffffffffu || (keys[i] >> kSlotShift) != (prev >> kSlotShift));
         if (valid)
-            cout[gi] = uint16_t((keys[i] & 127u) | (uint32_t(vals[i]) << 7));
+            cout[gi] = uint16_t((keys[i] & kPixMask) | (uint32_t(vals[i]) << kPixBits));
         if (head)
         {
-            ws_slot_block[size_t(t) * stride + slot] = (keys[i] >> 7) + bmin;
+            ws_slot_block[size_t(t) * stride + slot] = (keys[i] >> kPixBits) + bmin;
             ws_slot_first[size_t(t) * stride + slot] = uint16_t(gi);
             ++slot;
         }
@@ -317,16 +331,16 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_build(SolveDesc d, uint16
 // Per tile: move the tile's contributions and slots to their compact places
 // (bases from the scans of the counts); slot_first becomes absolute, and
 // slot_block / slot id pairs are laid out for the block sort.
-template <int K>
+template <int K, int TP>
 __global__ __launch_bounds__(kTileThreads) void k_tile_compact(SolveDesc d, const uint16_t *__restrict__ ws_contrib,
                                                                const uint32_t *__restrict__ ws_slot_block,
                                                                const uint16_t *__restrict__ ws_slot_first,
                                                                uint32_t *__restrict__ sort_keys,
                                                                uint32_t *__restrict__ sort_vals)
 {
-    using G = TileGeom<K>;
+    using G = TileGeom<K, TP>;
     const uint32_t t = blockIdx.x, tid = threadIdx.x;
-    const size_t stride = size_t(kTilePix) * G::kPairs;
+    const size_t stride = size_t(G::kEnt);
     const uint32_t c0 = d.tile_cbase[t], nc = d.tile_cbase[t + 1] - c0;
     const uint32_t s0 = d.tile_sbase[t], ns = d.tile_sbase[t + 1] - s0;
     for (uint32_t i = tid; i < nc; i += kTileThreads)
@@ -386,16 +400,17 @@ __global__ void k_lower_bound_u32(const uint32_t *__restrict__ keys, uint32_t n_
 // same length; which lane sums a slot does not change its result.  The slot
 // lists take LDS for the launch's largest tile only.
 
-template <int K, class JT>
-__global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restrict__ sds,
+template <int K, class JT, int TP>
+__global__ __launch_bounds__(2 * TP) void k_tiles(const SolveDesc *__restrict__ sds,
                                                         const uint32_t *__restrict__ wg_begin, int n_solve,
                                                         uint32_t slot_cap)
 {
-    using G = TileGeom<K>;
-    constexpr int kEnt = kTilePix * G::kPairs;
-    constexpr uint32_t kNB = 2 * (kTilePix + 1); // visiting-order buckets
-    __shared__ float4 Hs[kTilePix * (kTileRec / 4)];
-    __shared__ float wS[kTilePix * K];
+    using G = TileGeom<K, TP>;
+    constexpr int NT = G::kThreads;
+    constexpr int kEnt = G::kEnt;
+    constexpr uint32_t kNB = 2 * (TP + 1); // visiting-order buckets
+    __shared__ float4 Hs[TP * (kTileRec / 4)];
+    __shared__ float wS[TP * K];
     __shared__ uint16_t ent[kEnt];
     __shared__ uint32_t hist[kNB];
     // slot list starts sst[slot_cap + 2], then the visiting order perm[slot_cap]
@@ -408,8 +423,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
     if (t >= uint32_t(d.n_tiles))
         return; // uniform over the group
     const uint32_t tid = threadIdx.x;
-    const uint32_t p0 = t * kTilePix;
-    const uint32_t np = min(uint32_t(kTilePix), uint32_t(d.P) - p0);
+    const uint32_t p0 = t * TP;
+    const uint32_t np = min(uint32_t(TP), uint32_t(d.P) - p0);
     // the tile's records and weights first: their loads need only the
     // descriptor, so they are in flight with the tile's list bounds
     float hg[kHG];
@@ -424,7 +439,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
     }
     const uint32_t s0 = d.tile_sbase[t], ns = d.tile_sbase[t + 1] - s0;
     const uint32_t c0 = d.tile_cbase[t], nc = d.tile_cbase[t + 1] - c0;
-    for (uint32_t i = tid; i < kNB; i += kTileThreads)
+    for (uint32_t i = tid; i < kNB; i += NT)
         hist[i] = 0u;
     if (tid < np)
     {
@@ -440,12 +455,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
     // the contribution list (at most kEnt entries) and the slot starts: every
     // load issued before the first LDS store, one round trip for all of them
     {
-        constexpr int kPer = kEnt / kTileThreads;
+        constexpr int kPer = kEnt / NT;
         uint16_t cv[kPer];
 #pragma unroll
         for (int j = 0; j < kPer; ++j)
         {
-            const uint32_t i = tid + uint32_t(j) * kTileThreads;
+            const uint32_t i = tid + uint32_t(j) * NT;
             cv[j] = i < nc ? d.tc_contrib[c0 + i] : uint16_t(0);
         }
         uint32_t sv = 0;
@@ -454,13 +469,13 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
 #pragma unroll
         for (int j = 0; j < kPer; ++j)
         {
-            const uint32_t i = tid + uint32_t(j) * kTileThreads;
+            const uint32_t i = tid + uint32_t(j) * NT;
             if (i < nc)
                 ent[i] = cv[j];
         }
         if (tid <= ns)
             sst[tid] = uint16_t(sv - c0);
-        for (uint32_t s = tid + kTileThreads; s <= ns; s += kTileThreads)
+        for (uint32_t s = tid + NT; s <= ns; s += NT)
             sst[s] = uint16_t(d.tc_slot_first[s0 + s] - c0);
     }
     __syncthreads();
@@ -469,10 +484,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
     // first -- a counting sort of (kind, length)
     auto bucket = [&](uint32_t s) {
         const uint32_t c = ent[sst[s]];
-        const uint32_t off = ((c >> 7) & 7u) != ((c >> 10) & 7u) ? uint32_t(kTilePix) + 1u : 0u;
-        return off + uint32_t(kTilePix) - uint32_t(sst[s + 1] - sst[s]);
+        const uint32_t off = c_ka(c) != c_kb(c) ? uint32_t(TP) + 1u : 0u;
+        return off + uint32_t(TP) - uint32_t(sst[s + 1] - sst[s]);
     };
-    for (uint32_t s = tid; s < ns; s += kTileThreads)
+    for (uint32_t s = tid; s < ns; s += NT)
         atomicAdd(&hist[bucket(s)], 1u);
     __syncthreads();
     if (tid < 64)
@@ -505,7 +520,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
             }
     }
     __syncthreads();
-    for (uint32_t s = tid; s < ns; s += kTileThreads)
+    for (uint32_t s = tid; s < ns; s += NT)
         perm[atomicAdd(&hist[bucket(s)], 1u)] = uint16_t(s);
     __syncthreads();
 
@@ -530,9 +545,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
         // products, so the second read's latency hides under them (the sums
         // stay in list order)
         const uint32_t cfirst = ent[e0];
-        const bool diag = ((cfirst >> 7) & 7u) == ((cfirst >> 10) & 7u);
+        const bool diag = c_ka(cfirst) == c_kb(cfirst);
         auto fetch = [&](uint32_t cc, float &coef, float4 (&v)[NG]) {
-            const uint32_t pl = cc & 127u, ka = (cc >> 7) & 7u, kb = (cc >> 10) & 7u;
+            const uint32_t pl = c_pix(cc), ka = c_ka(cc), kb = c_kb(cc);
             coef = wS[pl * K + ka] * wS[pl * K + kb];
             const float4 *__restrict__ h = Hs + pl * (kTileRec / 4);
 #pragma unroll
@@ -576,10 +591,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const SolveDesc *__restr
             if (g < kTileRec / 4 - 1 || diag)
                 out[size_t(dst) * (kPartStride / 4) + g] = acc[g];
     };
-    for (uint32_t k = tid; k < ns; k += kTileThreads)
+    for (uint32_t k = tid; k < ns; k += NT)
     {
         const uint32_t c = ent[sst[perm[k]]];
-        const bool diag = ((c >> 7) & 7u) == ((c >> 10) & 7u);
+        const bool diag = c_ka(c) == c_kb(c);
         if (__any(diag))
             sum_slots(k, std::integral_constant<int, kTileRec / 4>{});
         else
@@ -1351,16 +1366,17 @@ __global__ __launch_bounds__(64) void k_tiles_combine_wave(const SolveDesc *__re
 // i's diagonal-slot partials in tile order, the ARAP terms J_i^T (J_i x_i +
 // J_j x_j), the node-translation term and the damping.  x is the solve's
 // next PCG direction d_{k+1} (dv[(k+1)&1]) unless x_user is given.
-template <int K, class JT>
-__global__ __launch_bounds__(kTileThreads) void k_mf_tiles(const SolveDesc *__restrict__ sds,
+template <int K, class JT, int TP>
+__global__ __launch_bounds__(2 * TP) void k_mf_tiles(const SolveDesc *__restrict__ sds,
                                                            const uint32_t *__restrict__ wg_begin, int n_solve, int k_it,
                                                            const float *__restrict__ x_user)
 {
-    using G = TileGeom<K>;
-    __shared__ float uS[kTilePix * 6];
-    __shared__ float wS[kTilePix * K];
-    __shared__ uint16_t ent[kTilePix * G::kPairs];
-    __shared__ uint16_t sst[kTilePix * G::kPairs + 1];
+    using G = TileGeom<K, TP>;
+    constexpr int NT = G::kThreads;
+    __shared__ float uS[TP * 6];
+    __shared__ float wS[TP * K];
+    __shared__ uint16_t ent[G::kEnt];
+    __shared__ uint16_t sst[G::kEnt + 1];
     const int si = find_solve_t(wg_begin, n_solve, blockIdx.x);
     const SolveDesc d = global_desc(sds[si]);
     const uint32_t t = blockIdx.x - wg_begin[si];
@@ -1368,8 +1384,8 @@ __global__ __launch_bounds__(kTileThreads) void k_mf_tiles(const SolveDesc *__re
         return; // uniform over the group
     const float *__restrict__ x = x_user ? x_user : pick2(d.dv, (k_it + 1) & 1);
     const uint32_t tid = threadIdx.x;
-    const uint32_t p0 = t * kTilePix;
-    const uint32_t np = min(uint32_t(kTilePix), uint32_t(d.P) - p0);
+    const uint32_t p0 = t * TP;
+    const uint32_t np = min(uint32_t(TP), uint32_t(d.P) - p0);
     if (tid < np)
     {
         const uint32_t p = p0 + tid;
@@ -1419,23 +1435,23 @@ __global__ __launch_bounds__(kTileThreads) void k_mf_tiles(const SolveDesc *__re
     }
     const uint32_t c0 = d.tile_cbase[t], nc = d.tile_cbase[t + 1] - c0;
     const uint32_t s0 = d.tile_sbase[t], ns = d.tile_sbase[t + 1] - s0;
-    for (uint32_t i = tid; i < nc; i += kTileThreads)
+    for (uint32_t i = tid; i < nc; i += NT)
         ent[i] = d.tc_contrib[c0 + i];
-    for (uint32_t s = tid; s <= ns; s += kTileThreads)
+    for (uint32_t s = tid; s <= ns; s += NT)
         sst[s] = uint16_t(d.tc_slot_first[s0 + s] - c0);
     __syncthreads();
-    for (uint32_t task = tid; task < ns * 6; task += kTileThreads)
+    for (uint32_t task = tid; task < ns * 6; task += NT)
     {
         const uint32_t s = task / 6, comp = task % 6;
         const uint32_t e0 = sst[s], e1 = sst[s + 1];
         const uint32_t first = ent[e0];
-        if (((first >> 7) & 7u) != ((first >> 10) & 7u))
+        if (c_ka(first) != c_kb(first))
             continue; // off-diagonal slot: not needed
         float acc = 0.f;
         for (uint32_t e = e0; e < e1; ++e)
         {
             const uint32_t c = ent[e];
-            const uint32_t pl = c & 127u, ka = (c >> 7) & 7u;
+            const uint32_t pl = c_pix(c), ka = c_ka(c);
             acc = fmaf(wS[pl * K + ka], uS[pl * 6 + comp], acc);
         }
         d.tpart[size_t(s0 + s) * kPartStride + comp] = acc;
@@ -1585,18 +1601,41 @@ __global__ __launch_bounds__(64) void k_spmv_rows(const SolveDesc *__restrict__ 
 }
 
 // ------------------------------------------------------------ launchers
+// Calls f(std::integral_constant<int, TP>) for the tile size tp (64, 128 or 256).
+template <class F>
+static hipError_t with_tile_pix(uint32_t tp, F &&f)
+{
+    switch (tp)
+    {
+    case 64:
+        f(std::integral_constant<int, 64>{});
+        break;
+    case 128:
+        f(std::integral_constant<int, 128>{});
+        break;
+    case 256:
+        f(std::integral_constant<int, 256>{});
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_tile_build(const SolveDesc &d, uint16_t *ws_contrib, uint32_t *ws_slot_block,
                              uint16_t *ws_slot_first, uint32_t *tile_nslot, uint32_t *tile_ncontrib, hipStream_t st)
 {
     if (d.n_tiles == 0)
         return hipSuccess;
-    if (d.K == 8)
-        hipLaunchKernelGGL(k_tile_build<8>, dim3(d.n_tiles), dim3(kTileThreads), 0, st, d, ws_contrib, ws_slot_block,
-                           ws_slot_first, tile_nslot, tile_ncontrib);
-    else
-        hipLaunchKernelGGL(k_tile_build<4>, dim3(d.n_tiles), dim3(kTileThreads), 0, st, d, ws_contrib, ws_slot_block,
-                           ws_slot_first, tile_nslot, tile_ncontrib);
-    return hipGetLastError();
+    return with_tile_pix(uint32_t(d.tile_pix), [&](auto tp) {
+        constexpr int TP = decltype(tp)::value;
+        if (d.K == 8)
+            hipLaunchKernelGGL((k_tile_build<8, TP>), dim3(d.n_tiles), dim3(2 * TP), 0, st, d, ws_contrib,
+                               ws_slot_block, ws_slot_first, tile_nslot, tile_ncontrib);
+        else
+            hipLaunchKernelGGL((k_tile_build<4, TP>), dim3(d.n_tiles), dim3(2 * TP), 0, st, d, ws_contrib,
+                               ws_slot_block, ws_slot_first, tile_nslot, tile_ncontrib);
+    });
 }
 
 hipError_t launch_tile_compact(const SolveDesc &d, const uint16_t *ws_contrib, const uint32_t *ws_slot_block,
@@ -1605,13 +1644,15 @@ hipError_t launch_tile_compact(const SolveDesc &d, const uint16_t *ws_contrib, c
 {
     if (d.n_tiles == 0)
         return hipSuccess;
-    if (d.K == 8)
-        hipLaunchKernelGGL(k_tile_compact<8>, dim3(d.n_tiles), dim3(kTileThreads), 0, st, d, ws_contrib,
-                           ws_slot_block, ws_slot_first, sort_keys, sort_vals);
-    else
-        hipLaunchKernelGGL(k_tile_compact<4>, dim3(d.n_tiles), dim3(kTileThreads), 0, st, d, ws_contrib,
-                           ws_slot_block, ws_slot_first, sort_keys, sort_vals);
-    return hipGetLastError();
+    return with_tile_pix(uint32_t(d.tile_pix), [&](auto tp) {
+        constexpr int TP = decltype(tp)::value;
+        if (d.K == 8)
+            hipLaunchKernelGGL((k_tile_compact<8, TP>), dim3(d.n_tiles), dim3(kTileThreads), 0, st, d, ws_contrib,
+                               ws_slot_block, ws_slot_first, sort_keys, sort_vals);
+        else
+            hipLaunchKernelGGL((k_tile_compact<4, TP>), dim3(d.n_tiles), dim3(kTileThreads), 0, st, d, ws_contrib,
+                               ws_slot_block, ws_slot_first, sort_keys, sort_vals);
+    });
 }
 
 hipError_t launch_block_transpose(const SolveDesc &d, hipStream_t st)
@@ -1631,58 +1672,44 @@ hipError_t launch_lower_bound_u32(const uint32_t *keys, uint32_t n_keys, uint32_
     return hipGetLastError();
 }
 
-hipError_t launch_tiles(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
-                        uint32_t slot_cap, hipStream_t st)
+hipError_t launch_tiles(int K, bool fp16, uint32_t tile_pix, const SolveDesc *sds, const uint32_t *wmap,
+                        int n_solve, uint32_t n_wg, uint32_t slot_cap, hipStream_t st)
 {
     if (n_wg == 0)
         return hipSuccess;
     // the slot lists in LDS are sized for the launch's largest tile
     const size_t lds = ((size_t(slot_cap) * 2 + 2) * sizeof(uint16_t) + 3) & ~size_t(3);
-    if (K == 8)
-    {
-        if (fp16)
-            hipLaunchKernelGGL((k_tiles<8, __half>), dim3(n_wg), dim3(kTileThreads), lds, st, sds, wmap, n_solve,
-                               slot_cap);
+    return with_tile_pix(tile_pix, [&](auto tp) {
+        constexpr int TP = decltype(tp)::value;
+        const dim3 grid(n_wg), block(2 * TP);
+        if (K == 8 && fp16)
+            hipLaunchKernelGGL((k_tiles<8, __half, TP>), grid, block, lds, st, sds, wmap, n_solve, slot_cap);
+        else if (K == 8)
+            hipLaunchKernelGGL((k_tiles<8, float, TP>), grid, block, lds, st, sds, wmap, n_solve, slot_cap);
+        else if (fp16)
+            hipLaunchKernelGGL((k_tiles<4, __half, TP>), grid, block, lds, st, sds, wmap, n_solve, slot_cap);
         else
-            hipLaunchKernelGGL((k_tiles<8, float>), dim3(n_wg), dim3(kTileThreads), lds, st, sds, wmap, n_solve,
-                               slot_cap);
-    }
-    else
-    {
-        if (fp16)
-            hipLaunchKernelGGL((k_tiles<4, __half>), dim3(n_wg), dim3(kTileThreads), lds, st, sds, wmap, n_solve,
-                               slot_cap);
-        else
-            hipLaunchKernelGGL((k_tiles<4, float>), dim3(n_wg), dim3(kTileThreads), lds, st, sds, wmap, n_solve,
-                               slot_cap);
-    }
-    return hipGetLastError();
+            hipLaunchKernelGGL((k_tiles<4, float, TP>), grid, block, lds, st, sds, wmap, n_solve, slot_cap);
+    });
 }
 
-hipError_t launch_mf_tiles(int K, bool fp16, const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg,
-                           int k_it, const float *x_user, hipStream_t st)
+hipError_t launch_mf_tiles(int K, bool fp16, uint32_t tile_pix, const SolveDesc *sds, const uint32_t *wmap,
+                           int n_solve, uint32_t n_wg, int k_it, const float *x_user, hipStream_t st)
 {
     if (n_wg == 0)
         return hipSuccess;
-    if (K == 8)
-    {
-        if (fp16)
-            hipLaunchKernelGGL((k_mf_tiles<8, __half>), dim3(n_wg), dim3(kTileThreads), 0, st, sds, wmap, n_solve,
-                               k_it, x_user);
+    return with_tile_pix(tile_pix, [&](auto tp) {
+        constexpr int TP = decltype(tp)::value;
+        const dim3 grid(n_wg), block(2 * TP);
+        if (K == 8 && fp16)
+            hipLaunchKernelGGL((k_mf_tiles<8, __half, TP>), grid, block, 0, st, sds, wmap, n_solve, k_it, x_user);
+        else if (K == 8)
+            hipLaunchKernelGGL((k_mf_tiles<8, float, TP>), grid, block, 0, st, sds, wmap, n_solve, k_it, x_user);
+        else if (fp16)
+            hipLaunchKernelGGL((k_mf_tiles<4, __half, TP>), grid, block, 0, st, sds, wmap, n_solve, k_it, x_user);
         else
-            hipLaunchKernelGGL((k_mf_tiles<8, float>), dim3(n_wg), dim3(kTileThreads), 0, st, sds, wmap, n_solve,
-                               k_it, x_user);
-    }
-    else
-    {
-        if (fp16)
-            hipLaunchKernelGGL((k_mf_tiles<4, __half>), dim3(n_wg), dim3(kTileThreads), 0, st, sds, wmap, n_solve,
-                               k_it, x_user);
-        else
-            hipLaunchKernelGGL((k_mf_tiles<4, float>), dim3(n_wg), dim3(kTileThreads), 0, st, sds, wmap, n_solve,
-                               k_it, x_user);
-    }
-    return hipGetLastError();
+            hipLaunchKernelGGL((k_mf_tiles<4, float, TP>), grid, block, 0, st, sds, wmap, n_solve, k_it, x_user);
+    });
 }
 
 hipError_t launch_mf_rows(const SolveDesc *sds, const uint32_t *wmap, int n_solve, uint32_t n_wg, int k_it,

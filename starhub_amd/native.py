@@ -72,6 +72,8 @@ SH_SPMV_MATERIALIZED, SH_SPMV_MATRIX_FREE, SH_SPMV_AUTO = 0, 1, 2
 SH_FLAG_REFERENCE_INDEX, SH_FLAG_PAIR_MAJOR, SH_FLAG_NO_PERSISTENT_PCG, SH_FLAG_STREAMING_PCG = 1, 2, 4, 8
 SH_FLAG_PHASE_TIMING, SH_FLAG_ROW_SEGMENTS, SH_FLAG_COST_HISTORY, SH_FLAG_NO_GRAPH = 16, 32, 64, 128
 SH_FLAG_PIPELINED_PCG = 256
+SH_FLAG_TILE_PIX_64 = 512
+SH_FLAG_TILE_PIX_256 = 1024
 
 _VP = ctypes.c_void_p
 

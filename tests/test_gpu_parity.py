@@ -287,6 +287,58 @@ def test_alternative_paths(torch_dev, flags):
     say("default path warp error vs replayed fp64", err, "flips", flips)
 
 
+@pytest.mark.parametrize("tile_pix", [64, 256])
+@pytest.mark.parametrize("K", [8, 4])
+def test_tile_sizes(torch_dev, tile_pix, K):
+    """Tiles of 64 and 256 pixels, the autotuner's other tile sizes: the
+    full-frame solve meets the flat 1e-4 against the replayed FP64 oracle; the
+    materialised blocks of its last system agree with the 128-pixel path's (the
+    same sums, grouped into other slot partials) and the matrix-free product
+    (per tile, from the records) with the materialised one; a batch at that
+    size equals its members' solo solves bit for bit."""
+    import sho
+    from starhub_amd import native, scene, workload
+    torch = torch_dev
+    flag = {64: native.SH_FLAG_TILE_PIX_64, 256: native.SH_FLAG_TILE_PIX_256}[tile_pix]
+    cfg = replace(scene.config_c2(), K=K)
+    say("tile size", tile_pix, "K", K)
+    sc = scene.generate(cfg)
+    solver, frame, prod, _ = _solve(sc, flags=flag | native.SH_FLAG_PIPELINED_PCG)
+    assert native.get_tuning(solver)["flags"] & flag
+    err, _, flips = solver_check(solver, workload.host_inputs(sc, prod), cfg.solver, sho.max_threads())
+    say("tile", tile_pix, "warp error vs replayed fp64", err, "flips", flips)
+    base, _, _, _ = _solve(sc, flags=native.SH_FLAG_PIPELINED_PCG)
+    b0 = base.debug("blk", torch.float32).cpu().numpy()
+    b1 = solver.debug("blk", torch.float32).cpu().numpy()
+    # the last GN iteration's systems differ by the states the two solves reached
+    assert np.abs(b1 - b0).max() <= 1e-3 * np.abs(b0).max()
+    x = torch.from_numpy(np.random.default_rng(4).standard_normal(6 * cfg.num_nodes).astype(np.float32)).cuda()
+    ym = solver.apply_jtj(x, False).cpu().numpy()
+    yf = solver.apply_jtj(x, True).cpu().numpy()
+    assert np.abs(yf - ym).max() <= 2e-5 * np.abs(ym).max()
+    # a tuning change does not reach an index built before it
+    native.set_tuning(solver, native.SH_FLAG_PIPELINED_PCG, 0, 16)
+    assert np.array_equal(solver.apply_jtj(x, True).cpu().numpy(), yf)
+    solver.close()
+    base.close()
+    cfgs = [replace(scene.config_batch_member(n, 40 + n), K=K) for n in (300, 700, 1000)]
+    frames = [workload.device_frame(scene.generate(c))[0] for c in cfgs]
+    W = max(c.width for c in cfgs)
+    H = max(c.height for c in cfgs)
+    N = max(c.num_nodes for c in cfgs)
+    bcfg = native.make_config(cfgs[0].solver, W, H, N, K, flags=flag)
+    batch = native.BatchSolver(bcfg, len(cfgs))
+    for i, f in enumerate(frames):
+        batch.set_inputs(i, f)
+    batch.solve(len(cfgs))
+    for i, f in enumerate(frames):
+        solo = native.WarpSolver(bcfg)
+        solo.process_frame(f)
+        assert np.array_equal(batch.node_se3(i).cpu().numpy(), solo.node_se3().cpu().numpy()), i
+        solo.close()
+    batch.close()
+
+
 @pytest.mark.parametrize("name", ["tiny_k8", "tiny_k4"])
 def test_golden_fixture(torch_dev, name):
     import make_golden
