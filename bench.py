@@ -9,9 +9,12 @@ the K timed steps' requests are all queued when the timed region starts, so
 the server runs as fast as it can take them.  Every input is resident in HBM
 before the timed region.  Beside it, in the same run:
 
-* latency_ms / served_30hz: the serving workload as configured -- Poisson
-  arrivals at 30 Hz per stream (1,920 requests/s offered), 10,000 requests --
-  its p50/p99 request latency and served rate;
+* latency_ms / served_sustained: the same streams with Poisson arrivals at
+  80 % of the measured capacity (a load the server keeps up with), about four
+  seconds of them -- p50/p99 request latency and served rate;
+* served_30hz: the serving workload as configured -- Poisson arrivals at
+  30 Hz per stream (1,920 requests/s offered, above the capacity, so its
+  latency is the backlog's growth), 10,000 requests;
 * plain: the plain single-solve path on the full-frame workload (300k
   surfels, 1024 nodes, 640x480, 10 GN x 20 PCG), autotuned, with the
   roofline of its dominant kernel and the CPU baseline on the same frame;
@@ -437,21 +440,37 @@ def served_legs(args) -> dict:
     per_stream = np.bincount([i for _, i in order], minlength=len(frames))
     total_bytes = sum(per_stream[i] * bytes_of[i] for i in range(len(frames)))
 
-    # the same streams at the configured offered load
+    def paced(order):
+        """Submits `order` at its arrival times; returns the results."""
+        tickets = []
+        t0 = time.perf_counter()
+        for when, i in order:
+            d = when - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            tickets.append(submit(i))
+        return [srv.wait(t) for t in tickets]
+
+    def paced_stats(order, res):
+        ok = [r for r in res if r["status"] == 0]
+        lat = np.array([r["done_ms"] - r["submit_ms"] for r in ok])
+        span = (max(r["done_ms"] for r in ok) - min(r["submit_ms"] for r in ok)) / 1e3
+        return {"offered_per_s": len(res) / order[-1][0], "solves_per_s": len(ok) / span,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "failed": len(res) - len(ok)}
+
+    # the same streams at the configured offered load (30 Hz per stream,
+    # above the capacity: latency there is the backlog's growth)
     order30 = arrivals(args.served_requests, 30.0)
-    tickets = []
-    t0 = time.perf_counter()
-    for when, i in order30:
-        d = when - (time.perf_counter() - t0)
-        if d > 0:
-            time.sleep(d)
-        tickets.append(submit(i))
-    res30 = [srv.wait(t) for t in tickets]
+    res30 = paced(order30)
     st2 = srv.stats()
+    # and at a load the server sustains: 80 % of the measured capacity, about
+    # four seconds of arrivals
+    rate = 0.8 * capacity / len(frames)
+    order80 = arrivals(min(args.served_requests, int(4.0 * 0.8 * capacity)), rate)
+    res80 = paced(order80)
+    st3 = srv.stats()
     srv.close()
-    ok = [r for r in res30 if r["status"] == 0]
-    lat30 = np.array([r["done_ms"] - r["submit_ms"] for r in ok])
-    span = (max(r["done_ms"] for r in ok) - min(r["submit_ms"] for r in ok)) / 1e3
 
     # the roofline of the served run's dominant kernel: every solve of the
     # timed requests ran each GN phase once per GN iteration
@@ -478,9 +497,11 @@ def served_legs(args) -> dict:
                      "achieved_GBps": total_bytes / dt / 1e9},
         "served_30hz": {"workload": f"served as configured: the same 64 streams, Poisson arrivals at 30 Hz per "
                                     f"stream, {len(res30)} requests",
-                        "offered_per_s": len(res30) / order30[-1][0], "solves_per_s": len(ok) / span,
-                        "p50_ms": float(np.percentile(lat30, 50)), "p99_ms": float(np.percentile(lat30, 99)),
-                        "failed": len(res30) - len(ok), "max_batch": st2["max_batch"]},
+                        **paced_stats(order30, res30), "max_batch": st2["max_batch"]},
+        "served_sustained": {"workload": f"served below capacity: the same 64 streams, Poisson arrivals at "
+                                         f"{rate:.1f} Hz per stream (80 % of the measured capacity), "
+                                         f"{len(res80)} requests",
+                             **paced_stats(order80, res80), "max_batch": st3["max_batch"]},
         "plain_solves_per_s": plain, "plain_tunings": kinds,
         "roofline": roof,
         "_items": list(zip(scs, prods)),
@@ -564,8 +585,9 @@ def main():
             "dtype": "fp32 streams (fp64 PCG scalars, 6x6 inverse, cost)",
             "data": "synthetic",
             "config": {"workload": cap["workload"], "streams": 64, "requests_per_step": 64},
-            "latency_ms": {"p50": served["served_30hz"]["p50_ms"], "p99": served["served_30hz"]["p99_ms"],
-                           "at": "the configured load: Poisson 30 Hz per stream"},
+            "latency_ms": {"p50": served["served_sustained"]["p50_ms"], "p99": served["served_sustained"]["p99_ms"],
+                           "at": "a load the server sustains: Poisson arrivals at 80 % of the measured capacity "
+                                 "(the configured 30 Hz per stream is above it: served_30hz)"},
             "vs_plain": cap["solves_per_s"] / served["plain_solves_per_s"],
             "roofline": served["roofline"],
         }
@@ -574,6 +596,7 @@ def main():
             out["vs_cpu"] = cap["solves_per_s"] / out["cpu_baseline"]["value"]
         out["served"] = cap
         out["served_30hz"] = served["served_30hz"]
+        out["served_sustained"] = served["served_sustained"]
         out["plain_same_mix"] = {"solves_per_s": served["plain_solves_per_s"], "tunings": served["plain_tunings"]}
         out["plain"] = plain
         out["c3_fp16"] = c3

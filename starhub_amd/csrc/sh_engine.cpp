@@ -1412,10 +1412,12 @@ int Engine::autotune(uint32_t ns, hipStream_t st, uint32_t repeats, sh_tuning *b
     // then the tile size, for a winner on the tile path: 64 and 256 pixels
     // against the default 128 (the same 2% margin)
     if (rc == SH_OK && best.ms < 1e30 && !(best.flags & SH_FLAG_ROW_SEGMENTS))
+    {
+        const sh_tuning base = best;
         for (uint32_t tf : {uint32_t(SH_FLAG_TILE_PIX_64), uint32_t(SH_FLAG_TILE_PIX_256)})
         {
-            sh_tuning c = best;
-            c.flags |= tf;
+            sh_tuning c = base;
+            c.flags = (c.flags & ~uint32_t(SH_FLAG_TILE_PIX_64 | SH_FLAG_TILE_PIX_256)) | tf;
             double ms = 0.0;
             if ((rc = time_candidate(c, ns, st, repeats, e0, e1, &ms)) != SH_OK)
                 break;
@@ -1423,6 +1425,7 @@ int Engine::autotune(uint32_t ns, hipStream_t st, uint32_t repeats, sh_tuning *b
             if (ms > 0.0 && ms < best.ms * 0.98)
                 best = c;
         }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc != SH_OK)

@@ -721,6 +721,13 @@ def test_autotune_picks_a_valid_tuning(torch_dev):
     solver_check(solver, workload.host_inputs(sc, prod), sc.cfg.solver, sho.max_threads())
     bad = native.Tuning(native.SH_FLAG_REFERENCE_INDEX, 0, 85, 0.0)
     assert native.lib().sh_solver_set_tuning(solver._h, ctypes.byref(bad)) == 1  # SH_E_ARG
+    both = native.Tuning(native.SH_FLAG_TILE_PIX_64 | native.SH_FLAG_TILE_PIX_256, 0, 85, 0.0)
+    assert native.lib().sh_solver_set_tuning(solver._h, ctypes.byref(both)) == 1  # one tile size at most
+    # a later round starts from whatever tuning is in use, a tile size included
+    native.set_tuning(solver, native.SH_FLAG_PIPELINED_PCG | native.SH_FLAG_TILE_PIX_64, 0, 16)
+    again = native.autotune(solver, repeats=1)
+    tiles = again["flags"] & (native.SH_FLAG_TILE_PIX_64 | native.SH_FLAG_TILE_PIX_256)
+    assert tiles != (native.SH_FLAG_TILE_PIX_64 | native.SH_FLAG_TILE_PIX_256)
 
 
 def test_c3_fp16_full_size_per_iteration(torch_dev):
