@@ -62,13 +62,15 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=50, help="timed solves of the plain full-frame leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline legs")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64-oracle checks of the warp field")
+    ap.add_argument("--no-plain-mix", action="store_true",
+                    help="batched leg only: skip the plain path on the same mix (profiling)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline sample budget per leg")
     ap.add_argument("--served-requests", type=int, default=10000, help="requests of the 30 Hz serving run")
     ap.add_argument("--quick", action="store_true", help="plain full-frame leg only (profiling)")
     ap.add_argument("--only", default=None, choices=["plain", "c3", "batched", "served"],
                     help="run one leg only and print its object (profiling)")
     ap.add_argument("--tuning", default=None, metavar="FLAGS,SPMV,ROWS",
-                    help="plain leg: use this tuning instead of running the autotuner (profiled runs)")
+                    help="plain and batched legs: use this tuning instead of running the autotuner (profiled runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain device copies of 256 MiB (the reference reading for HBM counters)")
     return ap.parse_args()
@@ -354,7 +356,12 @@ def batched_leg(args) -> dict:
     b = native.BatchSolver(native.make_config(cfgs[0].solver, W, H, N, cfgs[0].K), len(cfgs))
     for i, f in enumerate(frames):
         b.set_inputs(i, f)
-    tuning = native.autotune(b, batch_solves=len(cfgs), repeats=2)
+    if args.tuning:
+        f, m, r = (int(v) for v in args.tuning.split(","))
+        native.set_batch_tuning(b, f, m, r)
+        tuning = {"flags": f, "spmv_mode": m, "pcg_rows_per_group": r, "ms": None}
+    else:
+        tuning = native.autotune(b, batch_solves=len(cfgs), repeats=2)
     b.solve(len(cfgs))
     reps = 5
     t0 = time.perf_counter()
@@ -363,6 +370,8 @@ def batched_leg(args) -> dict:
     dt = time.perf_counter() - t0
     b.close()
     batched = reps * len(cfgs) / dt
+    if args.no_plain_mix:
+        return {"solves_per_s": batched, "ms_per_batch": 1e3 * dt / reps, "tuning": tuning}
     plain, sizes, kinds = plain_rate_tuned(frames, cfgs, native)
     nbytes = sum(solve_bytes(c, P, U) for c, (P, U) in zip(cfgs, sizes))
     out = {"workload": "32 solves, N log-uniform [128, 4096], 300 surfels/node, 320x240..1280x720, "
