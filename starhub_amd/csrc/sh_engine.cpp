@@ -482,23 +482,23 @@ int Engine::index_rows(uint32_t slot, IndexLane &ln, hipStream_t st)
     d.P = int(P);
     const uint32_t nt = div_up(P, uint32_t(d.tile_pix));
     const IndexZero z{ln.err, ln.meta, ln.rowlen + d.N, ln.segcnt + d.N, ln.tnslot + nt, ln.tncontrib + nt};
-    SH_HIP(launch_compact_pixels(d, ln.flags, ln.pos, d.gpix != d.pix ? ln.tpos : nullptr, z, st));
-    // node -> term (Node2TermsIndex::BuildIndex, Node2TermsIndex.cu:191-199,312-360)
+    // the compaction also writes the node -> term pairs (Node2TermsIndex::BuildIndex,
+    // Node2TermsIndex.cu:191-199,312-360)
+    SH_HIP(launch_compact_pixels_kv(d, ln.flags, ln.pos, d.gpix != d.pix ? ln.tpos : nullptr, z, ln.k16a, ln.v32a,
+                                    st));
     const uint32_t n_kv = P * uint32_t(K_) + 2 * uint32_t(d.E) + uint32_t(d.N);
-    SH_HIP(launch_node2term_kv(d, ln.k16a, ln.v32a, st));
     size_t tb = sort_temp_bytes_;
     SH_HIP(sort_pairs_u16_u32(ln.sort_temp, tb, ln.k16a, ln.k16b, ln.v32a, d.n2t_terms, n_kv, 16, st));
     SH_HIP(launch_lower_bound_u16(ln.k16b, n_kv, uint32_t(d.N) + 1, d.n2t_off, st));
     SH_HIP(launch_row_bitmap(d, ln.bitmap, ln.rowlen, ln.segcnt, ln.meta, st));
-    tb = sort_temp_bytes_;
-    SH_HIP(exclusive_scan_u32(ln.sort_temp, tb, ln.segcnt, d.seg_ptr, uint32_t(d.N) + 1, st));
-    tb = sort_temp_bytes_;
-    SH_HIP(exclusive_scan_u32(ln.sort_temp, tb, ln.rowlen, d.row_ptr, uint32_t(d.N) + 1, st));
+    SH_HIP(launch_scan_small(ScanPair{{ln.segcnt, ln.rowlen}, {d.seg_ptr, d.row_ptr}, {uint32_t(d.N) + 1,
+                                                                                    uint32_t(d.N) + 1}, 2},
+                             st));
     SH_HIP(launch_row_cols(d, ln.bitmap, ln.meta, st));
-    const Readback rb{{d.seg_ptr + d.N, ln.meta, ln.meta + 1, nullptr}, {6, 4, 5, 0}, 3};
-    SH_HIP(launch_readback(rb, ln.h_meta, st));
-    // the row structure is complete here (blk_col bounded by blk_cap)
-    SH_HIP(launch_tile_build_async(d, ln, st));
+    // the row structure is complete here (blk_col bounded by blk_cap); its
+    // segment total, longest row and pair count go to the host with the tile
+    // totals
+    SH_HIP(launch_tile_build_async(d, ln, st, {d.seg_ptr + d.N, ln.meta, ln.meta + 1}, {6, 4, 5}));
     return SH_OK;
 }
 
@@ -680,13 +680,12 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
     const uint32_t nt = div_up(P, uint32_t(d.tile_pix));
     const IndexZero z{err, static_cast<uint32_t *>(ws_meta_), static_cast<uint32_t *>(ws_rowlen_) + d.N,
                       static_cast<uint32_t *>(ws_segcnt_) + d.N, lanes_[0].tnslot + nt, lanes_[0].tncontrib + nt};
-    SH_HIP(launch_compact_pixels(d, flags, pos, tiled ? tpos : nullptr, z, st));
-
-    // node -> term (Node2TermsIndex::BuildIndex, Node2TermsIndex.cu:191-199,312-360)
+    // the compaction also writes the node -> term pairs (Node2TermsIndex::BuildIndex,
+    // Node2TermsIndex.cu:191-199,312-360)
     const uint32_t n_kv = P * uint32_t(K_) + 2 * uint32_t(d.E) + uint32_t(d.N);
     uint16_t *k16a = static_cast<uint16_t *>(ws_k16a_), *k16b = static_cast<uint16_t *>(ws_k16b_);
     uint32_t *v32a = static_cast<uint32_t *>(ws_v32a_);
-    SH_HIP(launch_node2term_kv(d, k16a, v32a, st));
+    SH_HIP(launch_compact_pixels_kv(d, flags, pos, tiled ? tpos : nullptr, z, k16a, v32a, st));
     tb = sort_temp_bytes_;
     SH_HIP(sort_pairs_u16_u32(sort_temp_, tb, k16a, k16b, v32a, d.n2t_terms, n_kv, 16, st));
     SH_HIP(launch_lower_bound_u16(k16b, n_kv, uint32_t(d.N) + 1, d.n2t_off, st));
@@ -701,8 +700,7 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
     // block-row structure from the node->term lists (dense-term counts and
     // segments, row lengths)
     SH_HIP(launch_row_bitmap(d, bitmap, rowlen, segcnt, meta, st));
-    tb = sort_temp_bytes_;
-    SH_HIP(exclusive_scan_u32(sort_temp_, tb, segcnt, d.seg_ptr, uint32_t(d.N) + 1, st));
+    SH_HIP(launch_scan_small(ScanPair{{segcnt, nullptr}, {d.seg_ptr, nullptr}, {uint32_t(d.N) + 1, 0}, 1}, st));
     SH_HIP(hipMemcpyAsync(h_meta_ + 6, d.seg_ptr + d.N, 4, hipMemcpyDeviceToHost, st));
     // reference node pair -> term index, half index (NodePair2TermsIndex::BuildHalfIndex, :294-298,591-654)
     if (!d.pair_entries || !d.half_keys || !d.half_off || !d.upper_pos || !d.lower_pos)
@@ -742,7 +740,7 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
         SH_HIP(sort_pairs_u32_u32(sort_temp_, tb, k32a, k32b, p32a, p32b, 2 * U, 24, st));
     SH_HIP(launch_sym_scatter(d, k32b, p32b, st));
     SH_HIP(launch_row_ptr(d, k32b, st));
-    SH_HIP(launch_tile_build_async(d, lanes_[0], st));
+    SH_HIP(launch_tile_build_async(d, lanes_[0], st, {}, {}));
     SH_HIP(hipStreamSynchronize(st));
     if (int rc = build_tiles(sl, lanes_[0], st); rc != SH_OK)
         return rc;
@@ -753,7 +751,9 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
 // Tile index, first part (asynchronous): per-tile sorted contributions and
 // slots in the workspace, and the two count scans whose totals the host reads
 // (into h_meta_[10], h_meta_[11]) at the next synchronisation.
-hipError_t Engine::launch_tile_build_async(SolveDesc &d, IndexLane &ln, hipStream_t st)
+hipError_t Engine::launch_tile_build_async(SolveDesc &d, IndexLane &ln, hipStream_t st,
+                                           std::initializer_list<const uint32_t *> more_src,
+                                           std::initializer_list<uint32_t> more_dst)
 {
     d.n_tiles = int(div_up(uint32_t(d.P), uint32_t(d.tile_pix)));
     const uint32_t nt = uint32_t(d.n_tiles);
@@ -762,13 +762,15 @@ hipError_t Engine::launch_tile_build_async(SolveDesc &d, IndexLane &ln, hipStrea
     // ns[nt], ns[nt + 1] and nc[nt] were zeroed by the compaction kernel
     if ((e = launch_tile_build(d, ln.tcontrib, ln.tslot_block, ln.tslot_first, ns, nc, st)) != hipSuccess)
         return e;
-    size_t tb = sort_temp_bytes_;
-    if ((e = exclusive_scan_u32(ln.sort_temp, tb, ns, d.tile_sbase, nt + 1, st)) != hipSuccess)
+    if ((e = launch_scan_small(ScanPair{{ns, nc}, {d.tile_sbase, d.tile_cbase}, {nt + 1, nt + 1}, 2}, st)) !=
+        hipSuccess)
         return e;
-    tb = sort_temp_bytes_;
-    if ((e = exclusive_scan_u32(ln.sort_temp, tb, nc, d.tile_cbase, nt + 1, st)) != hipSuccess)
-        return e;
-    const Readback rb{{d.tile_sbase + nt, d.tile_cbase + nt, ns + nt + 1, nullptr}, {10, 11, 13, 0}, 3};
+    Readback rb{{d.tile_sbase + nt, d.tile_cbase + nt, ns + nt + 1}, {10, 11, 13}, 3};
+    for (size_t k = 0; k < more_src.size() && rb.n < 8; ++k, ++rb.n)
+    {
+        rb.src[rb.n] = more_src.begin()[k];
+        rb.dst[rb.n] = more_dst.begin()[k];
+    }
     return launch_readback(rb, ln.h_meta, st);
 }
 
@@ -793,8 +795,7 @@ int Engine::build_tiles(Slot &sl, IndexLane &ln, hipStream_t st)
     size_t tb = sort_temp_bytes_;
     if (S > 0)
         SH_HIP(sort_pairs_u32_u32(ln.sort_temp, tb, k32a, k32b, p32a, d.list_items, S, end_bit, st));
-    SH_HIP(launch_slot_pos(d.list_items, S, d.slot_pos, st));
-    SH_HIP(launch_lower_bound_u32(k32b, S, nb + 1, d.list_off, st));
+    SH_HIP(launch_slot_lists(d.list_items, k32b, S, nb + 1, d.slot_pos, d.list_off, st));
     SH_HIP(launch_block_transpose(d, st));
     return SH_OK;
 }

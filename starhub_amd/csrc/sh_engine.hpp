@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <initializer_list>
 #include <vector>
 
 #include "../../include/sh/warp_solver.h"
@@ -161,7 +162,11 @@ private:
     int index_tiles(uint32_t slot, IndexLane &ln, hipStream_t st);
     int size_partials(Slot &sl);
     int build_tiles(Slot &sl, IndexLane &ln, hipStream_t st);
-    hipError_t launch_tile_build_async(SolveDesc &d, IndexLane &ln, hipStream_t st);
+    // the tile build and its count scans; their totals, with up to five more
+    // device words (more_src to h_meta[more_dst]), go to the lane's host buffer
+    hipError_t launch_tile_build_async(SolveDesc &d, IndexLane &ln, hipStream_t st,
+                                       std::initializer_list<const uint32_t *> more_src,
+                                       std::initializer_list<uint32_t> more_dst);
     int size_tiles(Slot &sl, uint32_t n_slots, uint32_t n_contrib);
     hipError_t mark(int phase, hipStream_t st);
     void collect_marks();

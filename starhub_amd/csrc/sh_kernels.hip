@@ -118,8 +118,35 @@ __global__ void k_pixel_flags(SolveDesc d, uint32_t n, uint32_t nh, uint32_t nbx
 // ImageTermKNNFetcher.cu:56-88).
 template <int K>
 __global__ void k_compact_pixels(SolveDesc d, const uint32_t *__restrict__ flags, const uint32_t *__restrict__ pos,
-                                 const uint32_t *__restrict__ tpos, uint32_t nbx, IndexZero z)
+                                 const uint32_t *__restrict__ tpos, uint32_t nbx, IndexZero z,
+                                 uint16_t *__restrict__ kv_keys, uint32_t *__restrict__ kv_vals)
 {
+    const uint32_t n_pix_wg = (uint32_t(d.W) * uint32_t(d.H) + 255u) / 256u;
+    if (kv_keys && blockIdx.x >= max(1u, n_pix_wg))
+    {
+        // the ARAP and node-translation terms' (node key, term) pairs: 2 per
+        // edge (a self edge's keys are 0xffff), 1 per node (Node2TermsIndex.cu:6-76)
+        const uint32_t t = uint32_t(d.P) + (blockIdx.x - max(1u, n_pix_wg)) * blockDim.x + threadIdx.x;
+        const uint32_t P = d.P, E = d.E, N = d.N;
+        if (t < P + E)
+        {
+            const uint32_t e = t - P;
+            const uint16_t a = d.edges[3 * e], b = d.edges[3 * e + 1];
+            const size_t o = size_t(P) * K + 2 * size_t(e);
+            kv_keys[o] = (a != b) ? a : 0xffff;
+            kv_keys[o + 1] = (a != b) ? b : 0xffff;
+            kv_vals[o] = t;
+            kv_vals[o + 1] = t;
+        }
+        else if (t < P + E + N)
+        {
+            const uint32_t m = t - P - E;
+            const size_t o = size_t(P) * K + 2 * size_t(E) + m;
+            kv_keys[o] = uint16_t(m);
+            kv_vals[o] = t;
+        }
+        return;
+    }
     if (blockIdx.x == 0)
     {
         const uint32_t t = threadIdx.x;
@@ -168,6 +195,16 @@ __global__ void k_compact_pixels(SolveDesc d, const uint32_t *__restrict__ flags
         d.pknn[size_t(o) * K + k] = kn[k];
         d.pw[size_t(o) * K + k] = w[k];
     }
+    if (kv_keys)
+    {
+        // the pixel term's (node key, term) pairs, K of them (Node2TermsIndex.cu:6-76)
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+        {
+            kv_keys[size_t(o) * K + k] = kn[k];
+            kv_vals[size_t(o) * K + k] = o;
+        }
+    }
     if (tpos)
     {
         // the GN phase's copy, at the pixel's tile-order position
@@ -179,41 +216,6 @@ __global__ void k_compact_pixels(SolveDesc d, const uint32_t *__restrict__ flags
             d.gpknn[size_t(ot) * K + k] = kn[k];
             d.gpw[size_t(ot) * K + k] = w[k];
         }
-    }
-}
-
-// (node key, term) pairs: K per pixel, 2 per ARAP edge (self edge -> 0xffff),
-// 1 per node (node-translation term).  Node2TermsIndex.cu:6-76
-template <int K>
-__global__ void k_node2term_kv(SolveDesc d, uint16_t *__restrict__ keys, uint32_t *__restrict__ vals)
-{
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t P = d.P, E = d.E, N = d.N;
-    if (t < P)
-    {
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-        {
-            keys[size_t(t) * K + k] = d.pknn[size_t(t) * K + k];
-            vals[size_t(t) * K + k] = t;
-        }
-    }
-    else if (t < P + E)
-    {
-        const uint32_t e = t - P;
-        const uint16_t i = d.edges[3 * e], j = d.edges[3 * e + 1];
-        const size_t o = size_t(P) * K + 2 * size_t(e);
-        keys[o] = (i != j) ? i : 0xffff;
-        keys[o + 1] = (i != j) ? j : 0xffff;
-        vals[o] = t;
-        vals[o + 1] = t;
-    }
-    else if (t < P + E + N)
-    {
-        const uint32_t n = t - P - E;
-        const size_t o = size_t(P) * K + 2 * size_t(E) + n;
-        keys[o] = uint16_t(n);
-        vals[o] = t;
     }
 }
 
@@ -1451,16 +1453,68 @@ hipError_t launch_pixel_flags(const SolveDesc &d, uint32_t n, uint32_t *flags, u
     return hipGetLastError();
 }
 
+hipError_t launch_compact_pixels_kv(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos,
+                                    const uint32_t *tpos, const IndexZero &z, uint16_t *keys, uint32_t *vals,
+                                    hipStream_t st)
+{
+    const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
+    // block 0 always runs: it clears z; the pair work-groups follow the pixels'
+    const uint32_t n_pix_wg = max(1u, (n + 255u) / 256u);
+    const uint32_t n_kv_wg = keys ? (uint32_t(d.E) + uint32_t(d.N) + 255u) / 256u : 0u;
+    const dim3 grid(n_pix_wg + n_kv_wg);
+    const uint32_t nbx = tile_order_blocks_x(uint32_t(d.W));
+    if (d.K == 8)
+        hipLaunchKernelGGL(k_compact_pixels<8>, grid, dim3(256), 0, st, d, flags, pos, tpos, nbx, z, keys, vals);
+    else
+        hipLaunchKernelGGL(k_compact_pixels<4>, grid, dim3(256), 0, st, d, flags, pos, tpos, nbx, z, keys, vals);
+    return hipGetLastError();
+}
+
 hipError_t launch_compact_pixels(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos, const uint32_t *tpos,
                                  const IndexZero &z, hipStream_t st)
 {
-    const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
-    const dim3 grid(max(1u, (n + 255u) / 256u)); // block 0 always runs: it clears z
-    const uint32_t nbx = tile_order_blocks_x(uint32_t(d.W));
-    if (d.K == 8)
-        hipLaunchKernelGGL(k_compact_pixels<8>, grid, dim3(256), 0, st, d, flags, pos, tpos, nbx, z);
-    else
-        hipLaunchKernelGGL(k_compact_pixels<4>, grid, dim3(256), 0, st, d, flags, pos, tpos, nbx, z);
+    return launch_compact_pixels_kv(d, flags, pos, tpos, z, nullptr, nullptr, st);
+}
+
+__global__ __launch_bounds__(1024) void k_scan_small(ScanPair p)
+{
+    __shared__ uint32_t wsum[16];
+    const uint32_t a = blockIdx.x;
+    const uint32_t n = p.n[a];
+    const uint32_t *__restrict__ in = p.in[a];
+    uint32_t *__restrict__ out = p.out[a];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    // each thread a contiguous run of the array
+    const uint32_t per = (n + 1023u) / 1024u;
+    const uint32_t b = min(n, tid * per), e = min(n, b + per);
+    uint32_t own = 0;
+    for (uint32_t i = b; i < e; ++i)
+        own += in[i];
+    uint32_t x = own;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+    {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (int(lane) >= o)
+            x += y;
+    }
+    if (lane == 63)
+        wsum[w] = x;
+    __syncthreads();
+    uint32_t run = x - own;
+    for (uint32_t i = 0; i < w; ++i)
+        run += wsum[i];
+    for (uint32_t i = b; i < e; ++i)
+    {
+        const uint32_t v = in[i];
+        out[i] = run;
+        run += v;
+    }
+}
+
+hipError_t launch_scan_small(const ScanPair &p, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_scan_small, dim3(p.count), dim3(1024), 0, st, p);
     return hipGetLastError();
 }
 
@@ -1473,16 +1527,6 @@ __global__ void k_readback(Readback r, uint32_t *host)
 hipError_t launch_readback(const Readback &r, uint32_t *host, hipStream_t st)
 {
     hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, st, r, host);
-    return hipGetLastError();
-}
-
-hipError_t launch_node2term_kv(const SolveDesc &d, uint16_t *keys, uint32_t *vals, hipStream_t st)
-{
-    const uint32_t n = uint32_t(d.P + d.E + d.N);
-    if (d.K == 8)
-        hipLaunchKernelGGL(k_node2term_kv<8>, SH_GRID(n, 256), dim3(256), 0, st, d, keys, vals);
-    else
-        hipLaunchKernelGGL(k_node2term_kv<4>, SH_GRID(n, 256), dim3(256), 0, st, d, keys, vals);
     return hipGetLastError();
 }
 

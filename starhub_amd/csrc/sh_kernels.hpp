@@ -31,16 +31,32 @@ struct IndexZero
 // tile-order copies (gpix, gpknn, gpw) at their tile-order scan positions
 hipError_t launch_compact_pixels(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos, const uint32_t *tpos,
                                  const IndexZero &z, hipStream_t st);
+// The same compaction, also writing the Node2Term (node key, term) pairs that
+// the stable node sort takes: K per pixel term (from the compacted ids), 2 per
+// ARAP edge and 1 per node-translation term (extra work-groups; d.P is set).
+hipError_t launch_compact_pixels_kv(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos,
+                                    const uint32_t *tpos, const IndexZero &z, uint16_t *keys, uint32_t *vals,
+                                    hipStream_t st);
 // Up to four device words copied into a pinned host buffer by one small
 // kernel (host[dst[i]] = *src[i]), in place of one copy per word.
 struct Readback
 {
-    const uint32_t *src[4];
-    uint32_t dst[4];
+    const uint32_t *src[8];
+    uint32_t dst[8];
     uint32_t n;
 };
+// Exclusive scans of one or two short arrays (counts per node or per tile:
+// a few thousand to a few ten thousand words), one work-group each, in one
+// launch: out[a][i] = in[a][0] + .. + in[a][i - 1] for i < n[a].
+struct ScanPair
+{
+    const uint32_t *in[2];
+    uint32_t *out[2];
+    uint32_t n[2];
+    uint32_t count; // 1 or 2
+};
+hipError_t launch_scan_small(const ScanPair &p, hipStream_t st);
 hipError_t launch_readback(const Readback &r, uint32_t *host, hipStream_t st);
-hipError_t launch_node2term_kv(const SolveDesc &d, uint16_t *keys, uint32_t *vals, hipStream_t st);
 hipError_t launch_lower_bound_u16(const uint16_t *keys, uint32_t n_keys, uint32_t n_out, uint32_t *out,
                                   hipStream_t st);
 hipError_t launch_pair_kv(const SolveDesc &d, uint32_t *keys, uint32_t *vals, hipStream_t st);
@@ -199,14 +215,14 @@ hipError_t launch_tile_compact(const SolveDesc &d, const uint16_t *ws_contrib, c
                                const uint16_t *ws_slot_first, uint32_t *sort_keys, uint32_t *sort_vals,
                                hipStream_t st);
 hipError_t launch_block_transpose(const SolveDesc &d, hipStream_t st);
-hipError_t launch_lower_bound_u32(const uint32_t *keys, uint32_t n_keys, uint32_t n_out, uint32_t *out,
-                                  hipStream_t st);
 hipError_t launch_tiles(int K, bool fp16, uint32_t tile_pix, const SolveDesc *sds, const uint32_t *wmap,
                         int n_solve, uint32_t n_wg, uint32_t slot_cap, hipStream_t st);
 // LDS of the tile combine: one work-group per row (wave = false) or one wave per row
 size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len, bool wave);
-// slot_pos[list_items[l]] = l for l < n: the list position of every slot
-hipError_t launch_slot_pos(const uint32_t *list_items, uint32_t n, uint32_t *slot_pos, hipStream_t st);
+// slot_pos[list_items[l]] = l for l < n: the list position of every slot, and
+// list_off[b] (b < n_blocks): the first list position of block b in the sorted blocks
+hipError_t launch_slot_lists(const uint32_t *list_items, const uint32_t *sorted_blocks, uint32_t n,
+                             uint32_t n_blocks, uint32_t *slot_pos, uint32_t *list_off, hipStream_t st);
 // matrix-free J^T W J x (x = d_{k+1} of each solve unless x_user; y = d.q unless y_user)
 hipError_t launch_mf_tiles(int K, bool fp16, uint32_t tile_pix, const SolveDesc *sds, const uint32_t *wmap,
                            int n_solve, uint32_t n_wg, int k_it, const float *x_user, hipStream_t st);

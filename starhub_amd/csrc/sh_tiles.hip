@@ -368,25 +368,6 @@ __global__ __launch_bounds__(kTileThreads) void k_block_transpose(SolveDesc d)
     }
 }
 
-// list_off[b] = first position of block b among the sorted slot blocks.
-__global__ void k_lower_bound_u32(const uint32_t *__restrict__ keys, uint32_t n_keys, uint32_t n_out,
-                                  uint32_t *__restrict__ out)
-{
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n_out)
-        return;
-    uint32_t lo = 0, hi = n_keys;
-    while (lo < hi)
-    {
-        const uint32_t m = (lo + hi) >> 1;
-        if (keys[m] < b)
-            lo = m + 1;
-        else
-            hi = m;
-    }
-    out[b] = lo;
-}
-
 // ------------------------------------------------------------ GN phase
 // One work-group per tile.  The records (28 floats: 21 H, 6 g, pad) and the
 // folded weights of the tile's pixels are staged in LDS with the tile's
@@ -1663,15 +1644,6 @@ hipError_t launch_block_transpose(const SolveDesc &d, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_lower_bound_u32(const uint32_t *keys, uint32_t n_keys, uint32_t n_out, uint32_t *out,
-                                  hipStream_t st)
-{
-    if (n_out == 0)
-        return hipSuccess;
-    hipLaunchKernelGGL(k_lower_bound_u32, dim3((n_out + 255) / 256), dim3(256), 0, st, keys, n_keys, n_out, out);
-    return hipGetLastError();
-}
-
 hipError_t launch_tiles(int K, bool fp16, uint32_t tile_pix, const SolveDesc *sds, const uint32_t *wmap,
                         int n_solve, uint32_t n_wg, uint32_t slot_cap, hipStream_t st)
 {
@@ -1739,18 +1711,40 @@ size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len, bool wave)
            sizeof(uint32_t) * (3 * L1 + 1) + sizeof(uint16_t) * ((max_nodes + 1) & ~1u);
 }
 
-__global__ void k_slot_pos(const uint32_t *__restrict__ list_items, uint32_t n, uint32_t *__restrict__ slot_pos)
+// After the slot sort by block: slot_pos[list_items[l]] = l for l < n (each
+// slot's position in the block lists, where k_tiles puts its partial), and
+// list_off[b] = the first position of block b for b < n_blocks (a lower bound
+// in the sorted blocks).
+__global__ void k_slot_lists(const uint32_t *__restrict__ list_items, const uint32_t *__restrict__ sorted_blocks,
+                             uint32_t n, uint32_t n_blocks, uint32_t *__restrict__ slot_pos,
+                             uint32_t *__restrict__ list_off)
 {
     const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l < n)
         slot_pos[list_items[l]] = l;
+    if (l < n_blocks)
+    {
+        uint32_t lo = 0, hi = n;
+        while (lo < hi)
+        {
+            const uint32_t m = (lo + hi) >> 1;
+            if (sorted_blocks[m] < l)
+                lo = m + 1;
+            else
+                hi = m;
+        }
+        list_off[l] = lo;
+    }
 }
 
-hipError_t launch_slot_pos(const uint32_t *list_items, uint32_t n, uint32_t *slot_pos, hipStream_t st)
+hipError_t launch_slot_lists(const uint32_t *list_items, const uint32_t *sorted_blocks, uint32_t n,
+                             uint32_t n_blocks, uint32_t *slot_pos, uint32_t *list_off, hipStream_t st)
 {
-    if (n == 0)
+    const uint32_t m = max(n, n_blocks);
+    if (m == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_slot_pos, dim3((n + 255) / 256), dim3(256), 0, st, list_items, n, slot_pos);
+    hipLaunchKernelGGL(k_slot_lists, dim3((m + 255) / 256), dim3(256), 0, st, list_items, sorted_blocks, n, n_blocks,
+                       slot_pos, list_off);
     return hipGetLastError();
 }
 
