@@ -230,7 +230,9 @@ int sh_solver_sizes(const sh_solver *solver, uint32_t *num_pixels, uint32_t *num
 /* Device views of internal arrays for parity checks: returns the device
  * pointer and element count.  Names: pix, pknn, pw, n2t_off, n2t_terms,
  * half_keys, half_off, pair_entries, upper_pos, lower_pos (with
- * SH_FLAG_REFERENCE_INDEX), row_ptr, blk_col, hg, dense_cnt, seg_ptr,
+ * SH_FLAG_REFERENCE_INDEX), row_ptr, blk_col, hg (the per-pixel H/g records;
+ * on the tile path with FP32 streams the tile kernel builds them without
+ * storing them, and they are stored by the next matrix-free apply), dense_cnt, seg_ptr,
  * tile_cbase, tile_sbase, list_off, blk_transpose, reg, blk, dinv, rhs, x,
  * dq0, dq1, delta, rot_cut (uint8, gn_iterations x num_nodes: 1 where the
  * warp update dropped the node's rotation increment because |w|_1 < 1e-4,

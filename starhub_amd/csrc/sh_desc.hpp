@@ -94,7 +94,7 @@ struct SolveDesc
     double *mv[2];    // pipelined PCG: the exchanged vector m = M^-1 w, 6N each, by iteration parity
     double *delta;    // pcg_iters + 1 : r.s per iteration
     double *cost;     // weighted cost (dense + ARAP) at the start of each GN iteration, then the final one
-    double *cost_part; // work-group partials: ceil(P/256) dense, then ceil((E+N)/256) ARAP
+    double *cost_part; // partials: ceil(P/64) dense (one per 64-pixel group), then ceil((E+N)/256) ARAP
     uint8_t *rot_cut;  // gn_iterations * N: 1 where the update dropped a node's rotation (|w|_1 < 1e-4)
     float *twist_l1;   // gn_iterations * N: the FP32 |w|_1 that decision was taken on
     // matrix-free scratch

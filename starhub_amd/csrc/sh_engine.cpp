@@ -203,7 +203,7 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         A(d.mv[1], size_t(ncap_) * 6 * 8);
         A(d.delta, size_t(c.pcg_iterations + 1) * 8);
         A(d.cost, size_t(c.gn_iterations + 2) * 8);
-        A(d.cost_part, (size_t(div_up(pcap_, 256)) + div_up(ecap_ + ncap_, 256) + 1) * 8);
+        A(d.cost_part, (size_t(div_up(pcap_, 64)) + div_up(ecap_ + ncap_, 256) + 1) * 8);
         A(d.rot_cut, size_t(std::max(1u, c.gn_iterations)) * ncap_);
         A(d.twist_l1, size_t(std::max(1u, c.gn_iterations)) * ncap_ * 4);
     }
@@ -1020,12 +1020,18 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
     if (!(cfg_.flags & (SH_FLAG_ROW_SEGMENTS | SH_FLAG_PAIR_MAJOR)) && tiles_lds > 160u * 1024u)
         return SH_E_CAPACITY;
     const bool tiles_path = !(cfg_.flags & (SH_FLAG_ROW_SEGMENTS | SH_FLAG_PAIR_MAJOR));
+    // the tile kernel builds the FP32 per-pixel records itself, so they never
+    // go through HBM; FP16 records are streamed (that is their point), and the
+    // matrix-free operator reads the stored records every PCG iteration
+    const bool fused = tiles_path && !fp16 && !matrix_free;
+    for (uint32_t s = 0; s < ns; ++s)
+        slots_[batch_[s]].records_fused = fused;
     const uint32_t pipe_slots = use_coop && resident && pipelined ? map_total_[kMapCoop] : 0u;
     for (uint32_t it = 0; it < iters; ++it)
     {
         const int parity = int(it); // iteration offset: DQ buffer (gn_iter + it) & 1, cost slot gn_iter + it
         SH_HIP(mark(kPhDense, st));
-        SH_HIP(launch_terms(K_, fp16, sds, map(kMapPixel, ns), map_total_[kMapPixel], map(kMapReg, ns),
+        SH_HIP(launch_terms(K_, fp16, sds, map(kMapPixel, ns), fused ? 0u : map_total_[kMapPixel], map(kMapReg, ns),
                             map_total_[kMapReg], n, parity, st));
         if (cfg_.flags & SH_FLAG_PAIR_MAJOR)
         {
@@ -1044,7 +1050,8 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
         else
         {
             SH_HIP(mark(kPhTiles, st));
-            SH_HIP(launch_tiles(K_, fp16, tile_pix(), sds, map(kMapTile, ns), n, map_total_[kMapTile], slot_cap, st));
+            SH_HIP(launch_tiles(K_, fp16, tile_pix(), fused, parity, sds, map(kMapTile, ns), n, map_total_[kMapTile],
+                                slot_cap, st));
             SH_HIP(mark(kPhTilesCombine, st));
             SH_HIP(launch_tiles_combine(sds, map(kMapRow, ns), n, map_total_[kMapRow], tiles_lds, comb_wave,
                                         static_cast<PcgSlot2 *>(d_slots2_), pipe_slots, st));
@@ -1308,6 +1315,15 @@ int Engine::apply_jtj(uint32_t slot, const float *x, float *y, bool matrix_free,
     const SolveDesc *sds = static_cast<const SolveDesc *>(d_descs_);
     if (matrix_free)
     {
+        if (slots_[slot].records_fused)
+        {
+            // the records of the last GN iteration were built inside the tile
+            // kernel and never stored: store them from the warp field that
+            // iteration started from (the other buffer of the pair)
+            SH_HIP(launch_terms(K_, fp16_, sds, map(kMapPixel, 1), map_total_[kMapPixel], map(kMapReg, 1), 0, 1, -1,
+                                st));
+            slots_[slot].records_fused = false;
+        }
         SH_HIP(launch_mf_tiles(K_, fp16_, uint32_t(slots_[slot].d.tile_pix), sds, map(kMapTile, 1), 1,
                                map_total_[kMapTile], 0, x, st));
         SH_HIP(launch_mf_rows(sds, map(kMapRow, 1), 1, map_total_[kMapRow], 0, x, y, st));

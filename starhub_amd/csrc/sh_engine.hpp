@@ -109,6 +109,7 @@ private:
         float *tpw = nullptr;
         uint32_t max_row_len = 0;
         uint32_t tile_slot_max = 0; // largest slot count of one tile
+        bool records_fused = false;  // the last GN iteration built its records in the tile kernel (not stored)
         void *part_buf = nullptr; // segment partials, grown on demand
         size_t part_bytes = 0;
         void *tile_buf = nullptr; // tile contributions, slots, lists and partials, grown on demand

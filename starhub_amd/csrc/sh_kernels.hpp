@@ -215,8 +215,11 @@ hipError_t launch_tile_compact(const SolveDesc &d, const uint16_t *ws_contrib, c
                                const uint16_t *ws_slot_first, uint32_t *sort_keys, uint32_t *sort_vals,
                                hipStream_t st);
 hipError_t launch_block_transpose(const SolveDesc &d, hipStream_t st);
-hipError_t launch_tiles(int K, bool fp16, uint32_t tile_pix, const SolveDesc *sds, const uint32_t *wmap,
-                        int n_solve, uint32_t n_wg, uint32_t slot_cap, hipStream_t st);
+// fused: the tile kernel builds the FP32 per-pixel records itself (and the
+// dense cost partials) from the warp field of GN iteration offset parity,
+// instead of reading them; the term kernel then runs the ARAP part only.
+hipError_t launch_tiles(int K, bool fp16, uint32_t tile_pix, bool fused, int parity, const SolveDesc *sds,
+                        const uint32_t *wmap, int n_solve, uint32_t n_wg, uint32_t slot_cap, hipStream_t st);
 // LDS of the tile combine: one work-group per row (wave = false) or one wave per row
 size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len, bool wave);
 // slot_pos[list_items[l]] = l for l < n: the list position of every slot, and

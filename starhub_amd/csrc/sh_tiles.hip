@@ -29,6 +29,7 @@
 #include "sh_common.hpp"
 #include "sh_desc.hpp"
 #include "sh_kernels.hpp"
+#include "sh_terms.hpp"
 
 namespace sh
 {
@@ -381,10 +382,10 @@ __global__ __launch_bounds__(kTileThreads) void k_block_transpose(SolveDesc d)
 // same length; which lane sums a slot does not change its result.  The slot
 // lists take LDS for the launch's largest tile only.
 
-template <int K, class JT, int TP>
+template <int K, class JT, int TP, bool FUSED>
 __global__ __launch_bounds__(2 * TP) void k_tiles(const SolveDesc *__restrict__ sds,
                                                         const uint32_t *__restrict__ wg_begin, int n_solve,
-                                                        uint32_t slot_cap)
+                                                        uint32_t slot_cap, int parity)
 {
     using G = TileGeom<K, TP>;
     constexpr int NT = G::kThreads;
@@ -410,13 +411,27 @@ __global__ __launch_bounds__(2 * TP) void k_tiles(const SolveDesc *__restrict__ 
     // descriptor, so they are in flight with the tile's list bounds
     float hg[kHG];
     float wk[K];
+    double cost = 0.0;
     if (tid < np)
     {
         const uint32_t p = p0 + tid;
-        load_hg<JT>(d.jr, p, hg);
+        if constexpr (FUSED)
+            cost = dense_pixel_hg<K>(d, p, parity, hg); // the record, built here instead of read
+        else
+            load_hg<JT>(d.jr, p, hg);
 #pragma unroll
         for (int k = 0; k < K; ++k)
             wk[k] = d.gpw[size_t(p) * K + k];
+    }
+    if constexpr (FUSED)
+    {
+        // the cost partials of the tile's 64-pixel groups, as the term kernel writes them
+        if (tid < uint32_t(TP))
+        {
+            const double wc = wave_sum(cost);
+            if ((tid & 63u) == 0 && p0 + tid < uint32_t(d.P))
+                d.cost_part[(p0 + tid) / 64u] = wc;
+        }
     }
     const uint32_t s0 = d.tile_sbase[t], ns = d.tile_sbase[t + 1] - s0;
     const uint32_t c0 = d.tile_cbase[t], nc = d.tile_cbase[t + 1] - c0;
@@ -1644,24 +1659,36 @@ hipError_t launch_block_transpose(const SolveDesc &d, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_tiles(int K, bool fp16, uint32_t tile_pix, const SolveDesc *sds, const uint32_t *wmap,
-                        int n_solve, uint32_t n_wg, uint32_t slot_cap, hipStream_t st)
+hipError_t launch_tiles(int K, bool fp16, uint32_t tile_pix, bool fused, int parity, const SolveDesc *sds,
+                        const uint32_t *wmap, int n_solve, uint32_t n_wg, uint32_t slot_cap, hipStream_t st)
 {
     if (n_wg == 0)
         return hipSuccess;
+    if (fused && fp16)
+        return hipErrorInvalidValue; // FP16 records are streamed, never built in the tile kernel
     // the slot lists in LDS are sized for the launch's largest tile
     const size_t lds = ((size_t(slot_cap) * 2 + 2) * sizeof(uint16_t) + 3) & ~size_t(3);
     return with_tile_pix(tile_pix, [&](auto tp) {
         constexpr int TP = decltype(tp)::value;
         const dim3 grid(n_wg), block(2 * TP);
-        if (K == 8 && fp16)
-            hipLaunchKernelGGL((k_tiles<8, __half, TP>), grid, block, lds, st, sds, wmap, n_solve, slot_cap);
+        if (K == 8 && fused)
+            hipLaunchKernelGGL((k_tiles<8, float, TP, true>), grid, block, lds, st, sds, wmap, n_solve, slot_cap,
+                               parity);
+        else if (K == 8 && fp16)
+            hipLaunchKernelGGL((k_tiles<8, __half, TP, false>), grid, block, lds, st, sds, wmap, n_solve, slot_cap,
+                               parity);
         else if (K == 8)
-            hipLaunchKernelGGL((k_tiles<8, float, TP>), grid, block, lds, st, sds, wmap, n_solve, slot_cap);
+            hipLaunchKernelGGL((k_tiles<8, float, TP, false>), grid, block, lds, st, sds, wmap, n_solve, slot_cap,
+                               parity);
+        else if (fused)
+            hipLaunchKernelGGL((k_tiles<4, float, TP, true>), grid, block, lds, st, sds, wmap, n_solve, slot_cap,
+                               parity);
         else if (fp16)
-            hipLaunchKernelGGL((k_tiles<4, __half, TP>), grid, block, lds, st, sds, wmap, n_solve, slot_cap);
+            hipLaunchKernelGGL((k_tiles<4, __half, TP, false>), grid, block, lds, st, sds, wmap, n_solve, slot_cap,
+                               parity);
         else
-            hipLaunchKernelGGL((k_tiles<4, float, TP>), grid, block, lds, st, sds, wmap, n_solve, slot_cap);
+            hipLaunchKernelGGL((k_tiles<4, float, TP, false>), grid, block, lds, st, sds, wmap, n_solve, slot_cap,
+                               parity);
     });
 }
 

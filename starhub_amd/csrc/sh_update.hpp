@@ -19,7 +19,7 @@ __device__ __forceinline__ dq8 load_dq(const dq8 *__restrict__ a, uint32_t i)
 // the dense and ARAP parts of k_terms, FP64, fixed order; one wave.
 __device__ inline void reduce_cost(const SolveDesc &d, int slot)
 {
-    const uint32_t n = (uint32_t(d.P) + 255u) / 256u + (uint32_t(d.E + d.N) + 255u) / 256u;
+    const uint32_t n = (uint32_t(d.P) + 63u) / 64u + (uint32_t(d.E + d.N) + 255u) / 256u;
     double v = 0.0;
     for (uint32_t i = threadIdx.x % 64u; i < n; i += 64)
         v += d.cost_part[i];

@@ -181,6 +181,10 @@ def test_stage_values(c1_one_iter, torch_dev):
     P, U, _ = solver.sizes()
     N = sc.cfg.num_nodes
     params = replace(sc.cfg.solver, gn_iterations=1)
+    # the default path builds the per-pixel records inside the tile kernel and
+    # does not store them; a matrix-free product stores those of the last GN
+    # iteration (from the warp field it started from) before it reads them
+    solver.apply_jtj(torch.zeros(6 * N, dtype=torch.float32, device="cuda"), True)
     hg = solver.debug("hg", torch.float32).cpu().numpy().reshape(P, 28).astype(np.float64)
     # the records follow the GN phase's pixel order (tile order): back to raster
     perm = _gn_order_to_raster(solver, torch, sc.cfg.width)
