@@ -1011,9 +1011,9 @@ int Engine::enqueue_gn(uint32_t ns, hipStream_t st, uint32_t iters, bool identit
             return SH_E_STATE;
     // the tile combine: one wave per row when the launch has enough rows to
     // fill the CUs several times over (batches, server steps); one work-group
-    // per row otherwise, where a single solve's rows finish sooner.  Both give
-    // the same bits.
-    const bool comb_wave = map_total_[kMapRow] >= kCombineWaveRows;
+    // per row otherwise, where a single solve's rows finish sooner (and for
+    // rows too long for the wave form's registers).  Both give the same bits.
+    const bool comb_wave = map_total_[kMapRow] >= kCombineWaveRows && max_l + 1 <= kWaveCombineMaxBlocks;
     const size_t tiles_lds = tiles_combine_lds(max_n, max_l, comb_wave);
     if ((cfg_.flags & SH_FLAG_ROW_SEGMENTS) && std::max(plan.lds, plan.lds_combine) > 160u * 1024u)
         return SH_E_CAPACITY;

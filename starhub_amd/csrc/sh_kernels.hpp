@@ -220,6 +220,9 @@ hipError_t launch_block_transpose(const SolveDesc &d, hipStream_t st);
 // instead of reading them; the term kernel then runs the ARAP part only.
 hipError_t launch_tiles(int K, bool fp16, uint32_t tile_pix, bool fused, int parity, const SolveDesc *sds,
                         const uint32_t *wmap, int n_solve, uint32_t n_wg, uint32_t slot_cap, hipStream_t st);
+// Longest row (blocks, diagonal included) the one-wave-per-row combine takes:
+// it holds a row's block sums in registers.
+constexpr uint32_t kWaveCombineMaxBlocks = 73;
 // LDS of the tile combine: one work-group per row (wave = false) or one wave per row
 size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len, bool wave);
 // slot_pos[list_items[l]] = l for l < n: the list position of every slot, and

@@ -1154,6 +1154,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles_combine_wg(const SolveDe
 }
 
 
+// Block-group tasks a lane of the wave combine holds in registers: rows of up
+// to 64 * kWaveTasks / 7 summed blocks (the engine takes the work-group form
+// for longer rows).
+constexpr uint32_t kWaveTasks = 8;
+static_assert(kWaveCombineMaxBlocks * (kTileRec / 4) <= 64 * kWaveTasks, "wave combine tasks");
+
 // One wave per node row: the same sums as k_tiles_combine_wg, in the same
 // order and therefore to the same bits -- every block's slot partials summed
 // (chunks in list order, then the chunk sums in order), the ARAP terms
@@ -1187,9 +1193,10 @@ __global__ __launch_bounds__(64) void k_tiles_combine_wave(const SolveDesc *__re
     const uint32_t base = d.row_ptr[i] + i;
     const uint32_t L1 = d.row_ptr[i + 1] - d.row_ptr[i] + 1;
     constexpr uint32_t kG = kTileRec / 4;
-    // LDS: run[L1][7] | regblk[L1][36] | cols[L1] | ls[L1] | le[L1] | trb[L1]
-    float4 *run = smem4;
-    float *regblk = reinterpret_cast<float *>(run + size_t(L1) * kG);
+    // LDS: regblk[L1][36] | cols[L1] | ls[L1] | le[L1] | trb[L1]; the block
+    // sums stay in registers (a lane sums and stores the same tasks)
+    float *regblk = reinterpret_cast<float *>(smem4);
+    float4 run[kWaveTasks];
     uint32_t *cols = reinterpret_cast<uint32_t *>(regblk + size_t(L1) * 36);
     uint32_t *ls = cols + L1;
     uint32_t *le = ls + L1;
@@ -1230,8 +1237,13 @@ __global__ __launch_bounds__(64) void k_tiles_combine_wave(const SolveDesc *__re
     // task per (block, group), its chunks in sequence (a batch launch has rows
     // enough to hide the latency of the sequence)
     const float4 *__restrict__ part = reinterpret_cast<const float4 *>(d.tpart);
-    for (uint32_t task = lane; task < nq * kG; task += 64)
+#pragma unroll
+    for (uint32_t k = 0; k < kWaveTasks; ++k)
     {
+        const uint32_t task = lane + 64u * k;
+        run[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (task >= nq * kG)
+            continue;
         const uint32_t q = task / kG, g = task % kG;
         // group 6 (g[3..5], pad) is written for diagonal slots only
         const uint32_t l1 = (q > 0 && g == kG - 1) ? ls[q] : le[q];
@@ -1244,7 +1256,7 @@ __global__ __launch_bounds__(64) void k_tiles_combine_wave(const SolveDesc *__re
             a.z += v.z;
             a.w += v.w;
         }
-        run[task] = a;
+        run[k] = a;
     }
 
     // ARAP and node-translation terms of the row (after its dense terms in
@@ -1304,12 +1316,16 @@ __global__ __launch_bounds__(64) void k_tiles_combine_wave(const SolveDesc *__re
     // block q's float4 group g of sums, plus ARAP and damping, and its
     // transpose; the diagonal block also goes to dg and its g part to red
     const size_t out0 = size_t(base) * 36;
-    for (uint32_t task = lane; task < nq * kG; task += 64)
+#pragma unroll
+    for (uint32_t k = 0; k < kWaveTasks; ++k)
     {
+        const uint32_t task = lane + 64u * k;
+        if (task >= nq * kG)
+            continue;
         const uint32_t q = task / kG, g = task % kG;
         const uint32_t c = q == 0 ? 0u : c_up + q - 1;
         const size_t tr = size_t(trb[q]) * 36; // (j, i) for c > 0
-        const float4 acc = run[task];
+        const float4 acc = run[k];
         const float av[4] = {acc.x, acc.y, acc.z, acc.w};
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -1733,7 +1749,7 @@ size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len, bool wave)
 {
     const size_t L1 = size_t(max_row_len) + 1;
     if (wave)
-        return sizeof(float4) * L1 * (kTileRec / 4) + sizeof(float) * L1 * 36 + sizeof(uint32_t) * 4 * L1;
+        return sizeof(float) * L1 * 36 + sizeof(uint32_t) * 4 * L1;
     return sizeof(float4) * (kCombUnionF4 + L1 * (kTileRec / 4)) + sizeof(float) * (L1 * 36 + 16) +
            sizeof(uint32_t) * (3 * L1 + 1) + sizeof(uint16_t) * ((max_nodes + 1) & ~1u);
 }
