@@ -637,7 +637,8 @@ struct RegTerm
     float ws; // lambda^2 cw^2 (0: invalid edge); translation: lambda^2 c^2
     int col;  // block column of the other node, -1 for none (translation term)
 };
-constexpr uint32_t kTermSub = 32; // terms staged in LDS at a time
+constexpr uint32_t kTermSub = 32;     // terms staged in LDS at a time (work-group form)
+constexpr uint32_t kTermSubWave = 16; // the same in the wave form (less LDS per row; same sums in the same order)
 
 // What a term's features are made of, as loaded from its records.
 struct RawTerm
@@ -1179,7 +1180,7 @@ __global__ __launch_bounds__(64) void k_tiles_combine_wave(const SolveDesc *__re
     __shared__ float dg[36];     // the diagonal block, for its inverse
     __shared__ double gm[6][12]; // its augmented matrix during the inversion
     __shared__ float red[16];    // g of the diagonal slots (0..5), of the terms (8..13)
-    __shared__ RegTerm rt[kTermSub]; // one sub-round of ARAP / node-translation terms
+    __shared__ RegTerm rt[kTermSubWave]; // one sub-round of ARAP / node-translation terms
     // a pipelined PCG that follows publishes into zeroed slots (the row map
     // has a work-group for every one of its groups)
     if (slots2 && blockIdx.x < n_slots2 && threadIdx.x < 4)
@@ -1262,7 +1263,7 @@ __global__ __launch_bounds__(64) void k_tiles_combine_wave(const SolveDesc *__re
     // ARAP and node-translation terms of the row (after its dense terms in
     // the node's term list), in rounds of 256 as in the work-group form: every
     // entry summed over a round's terms in list order, the round's sum then
-    // added to the row.  The terms are staged kTermSub at a time (one per
+    // added to the row.  The terms are staged kTermSubWave at a time (one per
     // lane); a lane's two entry sums carry over the sub-rounds of a round.
     const uint32_t dend = d.n2t_off[i] + d.dense_cnt[i];
     const uint32_t tend = d.n2t_off[i + 1];
@@ -1271,9 +1272,9 @@ __global__ __launch_bounds__(64) void k_tiles_combine_wave(const SolveDesc *__re
     {
         const uint32_t r1 = min(r0 + uint32_t(kTileThreads), tend);
         float diag[2] = {0.f, 0.f}, gq[2] = {0.f, 0.f};
-        for (uint32_t s0 = r0; s0 < r1; s0 += kTermSub)
+        for (uint32_t s0 = r0; s0 < r1; s0 += kTermSubWave)
         {
-            const uint32_t n = min(kTermSub, r1 - s0);
+            const uint32_t n = min(kTermSubWave, r1 - s0);
             if (lane < n)
                 rt[lane] = make_reg_term(d, i, load_reg_term(d, i, d.n2t_terms[s0 + lane]), [&](uint32_t j) {
                     // the ascending neighbours cols[1..L1)
