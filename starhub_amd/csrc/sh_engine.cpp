@@ -57,19 +57,15 @@ void Engine::release()
         sl.tile_buf = nullptr;
         sl.tile_bytes = 0;
     }
-    for (size_t l = 0; l < lanes_.size(); ++l)
+    for (char *&h : gws_.h_tab)
     {
-        IndexLane &ln = lanes_[l];
-        if (ln.stream)
-            (void)hipStreamSynchronize(ln.stream);
-        if (ln.ev)
-            (void)hipEventDestroy(ln.ev);
-        if (ln.stream)
-            (void)hipStreamDestroy(ln.stream);
-        if (l > 0 && ln.h_meta)
-            (void)hipHostFree(ln.h_meta); // lane 0's is h_meta_
+        if (h)
+            (void)hipHostFree(h);
+        h = nullptr;
     }
-    lanes_.clear();
+    if (gws_.h_out)
+        (void)hipHostFree(gws_.h_out);
+    gws_ = GroupWs{};
     if (h_meta_)
         (void)hipHostFree(h_meta_);
     h_meta_ = nullptr;
@@ -207,43 +203,14 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
         A(d.rot_cut, size_t(std::max(1u, c.gn_iterations)) * ncap_);
         A(d.twist_l1, size_t(std::max(1u, c.gn_iterations)) * ncap_ * 4);
     }
-    // shared index-build workspace
-    const size_t npix = size_t(pcap_);
-    A(ws_flags_, npix * 4);
-    A(ws_pos_, npix * 4);
-    A(ws_tflags_, size_t(tcap_) * 4);
-    A(ws_tpos_, size_t(tcap_) * 4);
-    A(ws_k16a_, size_t(n2t_cap_) * 2);
-    A(ws_k16b_, size_t(n2t_cap_) * 2);
-    A(ws_v32a_, size_t(n2t_cap_) * 4);
-    A(ws_k32a_, size_t(pair_cap_) * 4);
-    A(ws_k32b_, size_t(pair_cap_) * 4);
-    A(ws_p32a_, size_t(pair_cap_) * 4);
-    A(ws_p32b_, size_t(pair_cap_) * 4);
-    A(ws_head_, size_t(pair_cap_) * 4);
-    A(ws_hpos_, size_t(pair_cap_) * 4);
-    A(ws_meta_, 64);
-    A(ws_bitmap_, size_t(ncap_) * kBitmapWords * 4);
-    A(ws_rowlen_, size_t(ncap_ + 1) * 4);
-    A(ws_segcnt_, size_t(ncap_ + 1) * 4);
     tiles_cap_ = div_up(pcap_, kTilePixMin); // the most tiles: the smallest tile size
+    // index-build workspace: groups of up to eight solves of the largest frame
+    // (or more, smaller ones); the reference index builds one solve at a time
     {
-        // contributions at each tile's fixed stride: the tiles' pixels rounded up to a whole largest tile
-        const size_t tentries = (size_t(pcap_) + kTilePixMax) * size_t(K_ * (K_ + 1) / 2);
-        A(ws_tcontrib_, tentries * 2);
-        A(ws_tslot_block_, tentries * 4);
-        A(ws_tslot_first_, tentries * 2);
-        A(ws_tnslot_, size_t(tiles_cap_ + 2) * 4); // counts, scan total, largest count
-        A(ws_tncontrib_, size_t(tiles_cap_ + 1) * 4);
-    }
-    {
-        size_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
-        SH_HIP(sort_pairs_u16_u32(nullptr, b0, nullptr, nullptr, nullptr, nullptr, n2t_cap_, 16, nullptr));
-        SH_HIP(sort_pairs_u32_u32(nullptr, b1, nullptr, nullptr, nullptr, nullptr, uint32_t(pair_cap_), 24, nullptr));
-        SH_HIP(exclusive_scan_u32(nullptr, b2, nullptr, nullptr, uint32_t(pair_cap_), nullptr));
-        SH_HIP(exclusive_scan_u32(nullptr, b3, nullptr, nullptr, std::max(uint32_t(npix), tcap_), nullptr));
-        sort_temp_bytes_ = std::max(std::max(b0, b1), std::max(b2, b3));
-        A(sort_temp_, sort_temp_bytes_);
+        constexpr uint32_t kGroupUnits = 8;
+        group_units_ = (cfg_.flags & SH_FLAG_REFERENCE_INDEX) ? 1u : std::min(max_slots, kGroupUnits);
+        if (int rc = alloc_group_ws(group_units_, max_slots); rc != SH_OK)
+            return rc;
     }
     // batch-wide launch state
     // the launch's descriptors followed by its work maps: one upload per launch
@@ -261,46 +228,6 @@ int Engine::init(const sh_solver_config &c, uint32_t max_slots)
 #undef A
     SH_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_meta_), 64, hipHostMallocDefault));
     SH_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_flag_), 64, hipHostMallocDefault));
-    // index lanes: lane 0 is the workspace above; a batch engine gets up to
-    // kIndexLanes - 1 more (tile path only) so index builds overlap
-    {
-        constexpr uint32_t kIndexLanes = 8;
-        const uint32_t n_lanes =
-            (cfg_.flags & SH_FLAG_REFERENCE_INDEX) ? 1u : std::min(max_slots, kIndexLanes);
-        lanes_.resize(n_lanes);
-        IndexLane &l0 = lanes_[0];
-        l0.flags = static_cast<uint32_t *>(ws_flags_);
-        l0.pos = static_cast<uint32_t *>(ws_pos_);
-        l0.tflags = static_cast<uint32_t *>(ws_tflags_);
-        l0.tpos = static_cast<uint32_t *>(ws_tpos_);
-        l0.k16a = static_cast<uint16_t *>(ws_k16a_);
-        l0.k16b = static_cast<uint16_t *>(ws_k16b_);
-        l0.v32a = static_cast<uint32_t *>(ws_v32a_);
-        l0.k32a = static_cast<uint32_t *>(ws_k32a_);
-        l0.k32b = static_cast<uint32_t *>(ws_k32b_);
-        l0.p32a = static_cast<uint32_t *>(ws_p32a_);
-        l0.meta = static_cast<uint32_t *>(ws_meta_);
-        l0.bitmap = static_cast<uint32_t *>(ws_bitmap_);
-        l0.rowlen = static_cast<uint32_t *>(ws_rowlen_);
-        l0.segcnt = static_cast<uint32_t *>(ws_segcnt_);
-        l0.tcontrib = static_cast<uint16_t *>(ws_tcontrib_);
-        l0.tslot_block = static_cast<uint32_t *>(ws_tslot_block_);
-        l0.tslot_first = static_cast<uint16_t *>(ws_tslot_first_);
-        l0.tnslot = static_cast<uint32_t *>(ws_tnslot_);
-        l0.tncontrib = static_cast<uint32_t *>(ws_tncontrib_);
-        l0.sort_temp = sort_temp_;
-        l0.err = static_cast<uint32_t *>(d_flags_) + 2;
-        l0.h_meta = h_meta_;
-        for (uint32_t l = 1; l < n_lanes; ++l)
-            if (int rc = alloc_lane(lanes_[l], npix); rc != SH_OK)
-                return rc;
-        if (n_lanes > 1)
-            for (IndexLane &ln : lanes_)
-            {
-                SH_HIP(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking));
-                SH_HIP(hipEventCreateWithFlags(&ln.ev, hipEventDisableTiming));
-            }
-    }
     SH_HIP(hipEventCreateWithFlags(&sync_ev_, hipEventDisableTiming));
     SH_HIP(hipEventCreateWithFlags(&index_ev_, hipEventDisableTiming));
     stage_bytes_ = size_t(max_slots) * sizeof(SolveDesc) + size_t(kNumMaps) * (max_slots + 1) * 4;
@@ -396,330 +323,466 @@ bool Engine::tile_order() const
     return !(cfg_.flags & (SH_FLAG_ROW_SEGMENTS | SH_FLAG_PAIR_MAJOR));
 }
 
-// Workspace of an extra index lane (the tile-path arrays only).
-int Engine::alloc_lane(IndexLane &ln, size_t npix)
+// Index-build workspace for groups of up to `units` solves of the largest
+// frame (and at most max_items solves).
+int Engine::alloc_group_ws(uint32_t units, uint32_t max_items)
 {
+    GroupWs &g = gws_;
+    const uint64_t kp = uint64_t(K_ * (K_ + 1) / 2);
+    const uint64_t kv_unit = std::max<uint64_t>(n2t_cap_, pair_cap_);
+    const uint64_t tent_unit = (uint64_t(pcap_) + kTilePixMax) * kp;
+    // every position of the concatenated arrays stays below 2^31
+    const uint64_t lim = (1ull << 31) - 1;
+    units = uint32_t(
+        std::max<uint64_t>(1, std::min<uint64_t>({units, lim / kv_unit, lim / tent_unit, lim / pcap_, lim / tcap_})));
+    group_units_ = units;
+    g.cap_m = uint64_t(units) * pcap_;
+    g.cap_h = uint64_t(units) * tcap_; // the tuning may switch the tile order on later
+    g.cap_kv = uint64_t(units) * kv_unit;
+    g.cap_nodes = uint64_t(units) * (uint64_t(ncap_) + 1);
+    g.cap_tcnt = uint64_t(units) * (uint64_t(tiles_cap_) + 2);
+    g.cap_tent = uint64_t(units) * tent_unit;
+    g.cap_items = max_items;
     void *p = nullptr;
     int rc;
-#define L(field, type, bytes)                                                                                         \
-    if ((rc = alloc(&p, (bytes))) != SH_OK)                                                                           \
+#define A(field, type, count)                                                                                         \
+    if ((rc = alloc(&p, size_t(count) * sizeof(type))) != SH_OK)                                                      \
         return rc;                                                                                                    \
-    ln.field = static_cast<type *>(p);
-    L(flags, uint32_t, npix * 4);
-    L(pos, uint32_t, npix * 4);
-    L(tflags, uint32_t, size_t(tcap_) * 4);
-    L(tpos, uint32_t, size_t(tcap_) * 4);
-    L(k16a, uint16_t, size_t(n2t_cap_) * 2);
-    L(k16b, uint16_t, size_t(n2t_cap_) * 2);
-    L(v32a, uint32_t, size_t(n2t_cap_) * 4);
-    L(k32a, uint32_t, size_t(pair_cap_) * 4);
-    L(k32b, uint32_t, size_t(pair_cap_) * 4);
-    L(p32a, uint32_t, size_t(pair_cap_) * 4);
-    L(meta, uint32_t, 64);
-    L(err, uint32_t, 64);
-    L(bitmap, uint32_t, size_t(ncap_) * kBitmapWords * 4);
-    L(rowlen, uint32_t, size_t(ncap_ + 1) * 4);
-    L(segcnt, uint32_t, size_t(ncap_ + 1) * 4);
-    const size_t tentries = (size_t(pcap_) + kTilePixMax) * size_t(K_ * (K_ + 1) / 2);
-    L(tcontrib, uint16_t, tentries * 2);
-    L(tslot_block, uint32_t, tentries * 4);
-    L(tslot_first, uint16_t, tentries * 2);
-    L(tnslot, uint32_t, size_t(tiles_cap_ + 2) * 4);
-    L(tncontrib, uint32_t, size_t(tiles_cap_ + 1) * 4);
-#undef L
-    if ((rc = alloc(&ln.sort_temp, sort_temp_bytes_)) != SH_OK)
-        return rc;
-    SH_HIP(hipMemset(ln.err, 0, 64)); // then kept zero by the compaction kernel of each build
-    SH_HIP(hipHostMalloc(reinterpret_cast<void **>(&ln.h_meta), 64, hipHostMallocDefault));
-    return SH_OK;
-}
-
-// Index build, stage 1 (asynchronous): valid-pixel flags with the id range
-// check and their scan; the pixel count and the check flag go to the lane's
-// host buffer.
-int Engine::index_pixels(uint32_t slot, IndexLane &ln, hipStream_t st)
-{
-    SolveDesc &d = slots_[slot].d;
-    const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
-    size_t tb = sort_temp_bytes_;
-    // the GN phase's pixel order: tile order on the tile path, raster on the others
-    Slot &sl = slots_[slot];
-    const bool tiled = tile_order();
-    d.gpix = tiled ? sl.tpix : d.pix;
-    d.gpknn = tiled ? sl.tpknn : d.pknn;
-    d.gpw = tiled ? sl.tpw : d.pw;
-    d.tile_pix = int(tile_pix());
-    // ln.err is zero here: zeroed at allocation, by every build's compaction
-    // and on the rejection path
-    SH_HIP(launch_pixel_flags(d, n, ln.flags, tiled ? ln.tflags : nullptr, ln.err, st));
-    SH_HIP(exclusive_scan_u32(ln.sort_temp, tb, ln.flags, ln.pos, n, st));
-    if (tiled)
+    field = static_cast<type *>(p);
+    A(g.ws.flags, uint32_t, g.cap_m);
+    A(g.ws.pos, uint32_t, g.cap_m);
+    A(g.ws.tflags, uint32_t, g.cap_h);
+    A(g.ws.tpos, uint32_t, g.cap_h);
+    A(g.ws.errs, uint32_t, max_items);
+    A(g.ws.meta, uint32_t, size_t(max_items) * 16);
+    A(g.ws.bitmap, uint32_t, g.cap_nodes * kBitmapWords);
+    A(g.ws.rowlen, uint32_t, g.cap_nodes);
+    A(g.ws.segcnt, uint32_t, g.cap_nodes);
+    A(g.ws.tcontrib, uint16_t, g.cap_tent);
+    A(g.ws.tslot_block, uint32_t, g.cap_tent);
+    A(g.ws.tslot_first, uint16_t, g.cap_tent);
+    A(g.ws.tnslot, uint32_t, g.cap_tcnt);
+    A(g.ws.tncontrib, uint32_t, g.cap_tcnt);
+    A(g.ka, uint32_t, g.cap_kv);
+    A(g.kb, uint32_t, g.cap_kv);
+    A(g.va, uint32_t, g.cap_kv);
+    A(g.vb, uint32_t, g.cap_kv);
+    if (cfg_.flags & SH_FLAG_REFERENCE_INDEX)
     {
-        tb = sort_temp_bytes_;
-        SH_HIP(exclusive_scan_u32(ln.sort_temp, tb, ln.tflags, ln.tpos, tile_order_size(uint32_t(d.W), uint32_t(d.H)),
-                                  st));
+        A(g.head, uint32_t, pair_cap_);
+        A(g.hpos, uint32_t, pair_cap_);
     }
-    const Readback rb{{ln.pos + (n - 1), ln.flags + (n - 1), ln.err, nullptr}, {0, 1, 12, 0}, 3};
-    SH_HIP(launch_readback(rb, ln.h_meta, st));
-    return SH_OK;
-}
-
-// Stage 2 (after stage 1's readback): compaction, Node2Term, the block-row
-// structure and the per-tile contribution sort; the row meta and the tile
-// totals go to the host buffer.
-int Engine::index_rows(uint32_t slot, IndexLane &ln, hipStream_t st)
-{
-    Slot &sl = slots_[slot];
-    SolveDesc &d = sl.d;
-    if (ln.h_meta[12] != 0)
+    SH_HIP(hipMemset(g.ws.errs, 0, size_t(max_items) * 4)); // then cleared by each build's count kernel
     {
-        SH_HIP(hipMemsetAsync(ln.err, 0, 4, st)); // for the next build on this lane
-        return SH_E_ARG;                          // a surfel or node id out of range
+        size_t b0 = 0, b1 = 0;
+        SH_HIP(sort_pairs_u32_u32(nullptr, b0, nullptr, nullptr, nullptr, nullptr, uint32_t(g.cap_kv), 32, nullptr));
+        SH_HIP(exclusive_scan_u32(nullptr, b1, nullptr, nullptr,
+                                  uint32_t(std::max(std::max(g.cap_m, g.cap_h), uint64_t(pair_cap_))), nullptr));
+        g.sort_temp_bytes = std::max(b0, b1);
+        if ((rc = alloc(&g.sort_temp, g.sort_temp_bytes)) != SH_OK)
+            return rc;
     }
-    const uint32_t P = ln.h_meta[0] + ln.h_meta[1];
-    if (P > pcap_)
-        return SH_E_CAPACITY;
-    d.P = int(P);
-    const uint32_t nt = div_up(P, uint32_t(d.tile_pix));
-    const IndexZero z{ln.err, ln.meta, ln.rowlen + d.N, ln.segcnt + d.N, ln.tnslot + nt, ln.tncontrib + nt};
-    // the compaction also writes the node -> term pairs (Node2TermsIndex::BuildIndex,
-    // Node2TermsIndex.cu:191-199,312-360)
-    SH_HIP(launch_compact_pixels_kv(d, ln.flags, ln.pos, d.gpix != d.pix ? ln.tpos : nullptr, z, ln.k16a, ln.v32a,
-                                    st));
-    const uint32_t n_kv = P * uint32_t(K_) + 2 * uint32_t(d.E) + uint32_t(d.N);
-    size_t tb = sort_temp_bytes_;
-    SH_HIP(sort_pairs_u16_u32(ln.sort_temp, tb, ln.k16a, ln.k16b, ln.v32a, d.n2t_terms, n_kv, 16, st));
-    SH_HIP(launch_lower_bound_u16(ln.k16b, n_kv, uint32_t(d.N) + 1, d.n2t_off, st));
-    SH_HIP(launch_row_bitmap(d, ln.bitmap, ln.rowlen, ln.segcnt, ln.meta, st));
-    SH_HIP(launch_scan_small(ScanPair{{ln.segcnt, ln.rowlen}, {d.seg_ptr, d.row_ptr}, {uint32_t(d.N) + 1,
-                                                                                    uint32_t(d.N) + 1}, 2},
-                             st));
-    SH_HIP(launch_row_cols(d, ln.bitmap, ln.meta, st));
-    // the row structure is complete here (blk_col bounded by blk_cap); its
-    // segment total, longest row and pair count go to the host with the tile
-    // totals
-    SH_HIP(launch_tile_build_async(d, ln, st, {d.seg_ptr + d.N, ln.meta, ln.meta + 1}, {6, 4, 5}));
+    // device tables: descriptors | items | work maps | scan jobs (row scans, tile scans)
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t C = max_items;
+    g.off_items = up(C * sizeof(SolveDesc));
+    g.off_maps = g.off_items + C * sizeof(IndexItem);
+    g.off_jobs = up(g.off_maps + size_t(kNumGroupMaps) * (C + 1) * 4);
+    g.tab_bytes = g.off_jobs + 2 * (2 * C) * sizeof(ScanJob);
+    A(g.d_tab, char, g.tab_bytes);
+#undef A
+    for (char *&h : g.h_tab)
+        SH_HIP(hipHostMalloc(reinterpret_cast<void **>(&h), g.tab_bytes, hipHostMallocDefault));
+    SH_HIP(hipHostMalloc(reinterpret_cast<void **>(&g.h_out), C * 8 * 4, hipHostMallocDefault));
+    g_maps_.assign(size_t(kNumGroupMaps) * (C + 1), 0u);
     return SH_OK;
 }
 
-// Stage 3 (after stage 2's readback): sizes, slot partial buffers, the slot
-// lists by block (asynchronous).
-int Engine::index_tiles(uint32_t slot, IndexLane &ln, hipStream_t st)
+IndexGroup Engine::group_view() const
 {
-    Slot &sl = slots_[slot];
-    SolveDesc &d = sl.d;
-    const uint32_t U = ln.h_meta[5];
-    if (U > ucap_)
-        return SH_E_CAPACITY;
-    d.U = int(U);
-    d.n_pair_entries = 0;
-    sl.max_row_len = ln.h_meta[4];
-    d.n_seg = int(ln.h_meta[6]);
-    int rc = size_partials(sl);
-    if (rc != SH_OK)
-        return rc;
-    if ((rc = build_tiles(sl, ln, st)) != SH_OK)
-        return rc;
-    sl.indexed = true;
+    IndexGroup v;
+    v.sds = reinterpret_cast<const SolveDesc *>(gws_.d_tab);
+    v.items = reinterpret_cast<const IndexItem *>(gws_.d_tab + gws_.off_items);
+    v.n = int(g_items_.size());
+    return v;
+}
+
+const uint32_t *Engine::group_map(int m) const
+{
+    return reinterpret_cast<const uint32_t *>(gws_.d_tab + gws_.off_maps) + size_t(m) * (gws_.cap_items + 1);
+}
+
+// One copy of the group's tables (the current descriptors of its slots, the
+// items, work maps and scan jobs) through the stage's own pinned buffer: a
+// stage's buffer is rewritten only after the host has waited for that
+// stage's copy in the previous build.
+int Engine::upload_group(int stage, hipStream_t st)
+{
+    GroupWs &g = gws_;
+    char *h = g.h_tab[stage];
+    const size_t n = g_items_.size(), C = g.cap_items;
+    for (size_t i = 0; i < n; ++i)
+        std::memcpy(h + i * sizeof(SolveDesc), &slots_[g_slots_[i]].d, sizeof(SolveDesc));
+    std::memcpy(h + g.off_items, g_items_.data(), n * sizeof(IndexItem));
+    std::memcpy(h + g.off_maps, g_maps_.data(), g_maps_.size() * 4);
+    std::memcpy(h + g.off_jobs, g_jobs_[0].data(), g_jobs_[0].size() * sizeof(ScanJob));
+    std::memcpy(h + g.off_jobs + 2 * C * sizeof(ScanJob), g_jobs_[1].data(), g_jobs_[1].size() * sizeof(ScanJob));
+    SH_HIP(hipMemcpyAsync(g.d_tab, h, g.tab_bytes, hipMemcpyHostToDevice, st));
     return SH_OK;
 }
 
-// The indexes of several slots on the lanes: each lane runs one slot's
-// stages on its own stream, and the host advances whichever lane's readback
-// has arrived.  Every lane starts after the work already on st and st waits
-// for every lane at the end.  With rcs, a slot whose inputs are rejected
-// (SH_E_ARG / SH_E_CAPACITY / SH_E_NOMEM) records its status there and the
-// others go on; without, the first failure ends the build.
+// The index builds of several slots, in groups: a group takes slots while the
+// sums of their upper bounds (frame pixels, node pairs, nodes, tiles, tile
+// contributions) fit the workspace; one solve of the largest frame always
+// fits.  With rcs, a slot whose inputs are rejected (SH_E_ARG /
+// SH_E_CAPACITY / SH_E_NOMEM) records its status there and the others go on;
+// without, the first failure ends the build.
 int Engine::build_indexes(const uint32_t *list, uint32_t n, hipStream_t st, int *rcs)
 {
-    auto per_slot = [&](int rc) { return rcs && rc != SH_OK && rc != SH_E_HIP; };
     if (rcs)
         for (uint32_t i = 0; i < n; ++i)
             rcs[i] = SH_OK;
-    if (lanes_.size() <= 1 || n <= 1 || (cfg_.flags & SH_FLAG_REFERENCE_INDEX))
+    const GroupWs &g = gws_;
+    const uint64_t TP = tile_pix(), kp = uint64_t(K_ * (K_ + 1) / 2);
+    const bool tiled = tile_order(), ref = (cfg_.flags & SH_FLAG_REFERENCE_INDEX) != 0;
+    uint32_t i = 0;
+    while (i < n)
     {
-        for (uint32_t i = 0; i < n; ++i)
-            if (int rc = build_index(list[i], st); rc != SH_OK)
-            {
-                if (!per_slot(rc))
-                    return rc;
-                rcs[i] = rc;
-            }
-        return SH_OK;
-    }
-    const uint32_t L = uint32_t(std::min<size_t>(lanes_.size(), n));
-    SH_HIP(hipEventRecord(lanes_[0].ev, st));
-    for (uint32_t l = 0; l < L; ++l)
-    {
-        if (l > 0)
-            SH_HIP(hipStreamWaitEvent(lanes_[l].stream, lanes_[0].ev, 0));
-        lanes_[l].slot = -1;
-    }
-    SH_HIP(hipStreamWaitEvent(lanes_[0].stream, lanes_[0].ev, 0));
-    int rc = SH_OK;
-    uint32_t next = 0, done = 0;
-    while (rc == SH_OK && done < n)
-    {
-        bool progress = false;
-        int waiting = -1;
-        for (uint32_t l = 0; l < L && rc == SH_OK; ++l)
+        uint64_t m = 0, h = 0, kv = 0, nodes = 0, tcnt = 0, tent = 0, sub = 0;
+        uint32_t j = i;
+        while (j < n && j - i < g.cap_items && !(ref && j > i))
         {
-            IndexLane &ln = lanes_[l];
-            if (ln.slot < 0)
-            {
-                if (next == n)
-                    continue;
-                ln.item = int(next);
-                ln.slot = int(list[next++]);
-                ln.stage = 1;
-                if ((rc = index_pixels(uint32_t(ln.slot), ln, ln.stream)) == SH_OK &&
-                    hipEventRecord(ln.ev, ln.stream) != hipSuccess)
-                    rc = SH_E_HIP;
-                progress = true;
-                continue;
-            }
-            const hipError_t q = hipEventQuery(ln.ev);
-            if (q == hipErrorNotReady)
-            {
-                if (waiting < 0)
-                    waiting = int(l);
-                continue;
-            }
-            if (q != hipSuccess)
-            {
-                last_hip_error = q;
-                rc = SH_E_HIP;
+            const SolveDesc &d = slots_[list[j]].d;
+            const uint64_t px = uint64_t(d.W) * uint64_t(d.H), nt = (px + TP - 1) / TP;
+            const uint64_t m1 = m + px;
+            const uint64_t h1 = h + (tiled ? tile_order_size(uint32_t(d.W), uint32_t(d.H)) : 0u);
+            const uint64_t kv1 = kv + px * uint64_t(K_) + 2 * uint64_t(d.E) + uint64_t(d.N);
+            const uint64_t n1 = nodes + uint64_t(d.N) + 1, tc1 = tcnt + nt + 2, te1 = tent + nt * TP * kp;
+            const uint64_t s1 = sub + std::min<uint64_t>(nt * TP * kp, pair_cap_);
+            if (j > i && (m1 > g.cap_m || h1 > g.cap_h || kv1 > g.cap_kv || n1 > g.cap_nodes || tc1 > g.cap_tcnt ||
+                          te1 > g.cap_tent || s1 > g.cap_kv))
                 break;
-            }
-            if (ln.stage == 1)
-            {
-                ln.stage = 2;
-                if ((rc = index_rows(uint32_t(ln.slot), ln, ln.stream)) == SH_OK &&
-                    hipEventRecord(ln.ev, ln.stream) != hipSuccess)
-                    rc = SH_E_HIP;
-            }
-            else
-            {
-                rc = index_tiles(uint32_t(ln.slot), ln, ln.stream);
-                ln.slot = -1;
-                ++done;
-            }
-            if (per_slot(rc))
-            {
-                // this slot's inputs were rejected: record it, free the lane
-                rcs[ln.item] = rc;
-                if (ln.slot >= 0)
-                    ++done;
-                ln.slot = -1;
-                rc = SH_OK;
-            }
-            progress = true;
+            m = m1, h = h1, kv = kv1, nodes = n1, tcnt = tc1, tent = te1, sub = s1;
+            ++j;
         }
-        if (!progress && waiting >= 0)
-            __builtin_ia32_pause(); // poll again: a blocking wait on one lane wakes tens of microseconds late
+        if (int rc = build_group(list + i, j - i, st, rcs ? rcs + i : nullptr); rc != SH_OK)
+            return rc;
+        i = j;
     }
-    // st continues after every lane's work (also after a failure: nothing may
-    // still be writing a slot's arrays when the caller reuses them)
-    for (uint32_t l = 0; l < L; ++l)
-    {
-        lanes_[l].slot = -1;
-        if (hipEventRecord(lanes_[l].ev, lanes_[l].stream) != hipSuccess ||
-            hipStreamWaitEvent(st, lanes_[l].ev, 0) != hipSuccess)
-            rc = rc == SH_OK ? SH_E_HIP : rc;
-        if (rc != SH_OK)
-            (void)hipStreamSynchronize(lanes_[l].stream);
-    }
-    return rc;
+    return SH_OK;
 }
 
 // Once-per-solve index build (WarpSolver::buildSolverIndexStreamed, WarpSolver.cu:194-223).
-int Engine::build_index(uint32_t slot, hipStream_t st)
-{
-    if (!(cfg_.flags & SH_FLAG_REFERENCE_INDEX))
-    {
-        IndexLane &ln = lanes_[0];
-        int rc;
-        // the index half's own event: a server admits on one thread while it
-        // steps (and finishes) on another
-        if ((rc = index_pixels(slot, ln, st)) != SH_OK || (rc = wait_stream(st, index_ev_)) != SH_OK)
-            return rc;
-        if ((rc = index_rows(slot, ln, st)) != SH_OK || (rc = wait_stream(st, index_ev_)) != SH_OK)
-            return rc;
-        return index_tiles(slot, ln, st);
-    }
-    Slot &sl = slots_[slot];
-    SolveDesc &d = sl.d;
-    const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
-    size_t tb = sort_temp_bytes_;
-    uint32_t *flags = static_cast<uint32_t *>(ws_flags_), *pos = static_cast<uint32_t *>(ws_pos_);
-    uint32_t *err = static_cast<uint32_t *>(d_flags_) + 2;
-    SH_HIP(hipMemsetAsync(err, 0, 4, st));
-    const bool tiled = tile_order(); // raster on the pair-major path, which reads records by term id
-    d.gpix = tiled ? sl.tpix : d.pix;
-    d.gpknn = tiled ? sl.tpknn : d.pknn;
-    d.gpw = tiled ? sl.tpw : d.pw;
-    d.tile_pix = int(tile_pix());
-    uint32_t *tflags = static_cast<uint32_t *>(ws_tflags_), *tpos = static_cast<uint32_t *>(ws_tpos_);
-    SH_HIP(launch_pixel_flags(d, n, flags, tiled ? tflags : nullptr, err, st));
-    SH_HIP(exclusive_scan_u32(sort_temp_, tb, flags, pos, n, st));
-    if (tiled)
-    {
-        tb = sort_temp_bytes_;
-        SH_HIP(exclusive_scan_u32(sort_temp_, tb, tflags, tpos, tile_order_size(uint32_t(d.W), uint32_t(d.H)), st));
-    }
-    SH_HIP(hipMemcpyAsync(h_meta_ + 0, pos + (n - 1), 4, hipMemcpyDeviceToHost, st));
-    SH_HIP(hipMemcpyAsync(h_meta_ + 1, flags + (n - 1), 4, hipMemcpyDeviceToHost, st));
-    SH_HIP(hipMemcpyAsync(h_meta_ + 12, err, 4, hipMemcpyDeviceToHost, st));
-    SH_HIP(hipStreamSynchronize(st));
-    if (h_meta_[12] != 0)
-        return SH_E_ARG; // a surfel or node id out of range
-    const uint32_t P = h_meta_[0] + h_meta_[1];
-    if (P > pcap_)
-        return SH_E_CAPACITY;
-    d.P = int(P);
-    const uint32_t nt = div_up(P, uint32_t(d.tile_pix));
-    const IndexZero z{err, static_cast<uint32_t *>(ws_meta_), static_cast<uint32_t *>(ws_rowlen_) + d.N,
-                      static_cast<uint32_t *>(ws_segcnt_) + d.N, lanes_[0].tnslot + nt, lanes_[0].tncontrib + nt};
-    // the compaction also writes the node -> term pairs (Node2TermsIndex::BuildIndex,
-    // Node2TermsIndex.cu:191-199,312-360)
-    const uint32_t n_kv = P * uint32_t(K_) + 2 * uint32_t(d.E) + uint32_t(d.N);
-    uint16_t *k16a = static_cast<uint16_t *>(ws_k16a_), *k16b = static_cast<uint16_t *>(ws_k16b_);
-    uint32_t *v32a = static_cast<uint32_t *>(ws_v32a_);
-    SH_HIP(launch_compact_pixels_kv(d, flags, pos, tiled ? tpos : nullptr, z, k16a, v32a, st));
-    tb = sort_temp_bytes_;
-    SH_HIP(sort_pairs_u16_u32(sort_temp_, tb, k16a, k16b, v32a, d.n2t_terms, n_kv, 16, st));
-    SH_HIP(launch_lower_bound_u16(k16b, n_kv, uint32_t(d.N) + 1, d.n2t_off, st));
+int Engine::build_index(uint32_t slot, hipStream_t st) { return build_indexes(&slot, 1, st, nullptr); }
 
-    uint32_t *meta = static_cast<uint32_t *>(ws_meta_);
-    uint32_t *rowlen = static_cast<uint32_t *>(ws_rowlen_);
-    uint32_t *bitmap = static_cast<uint32_t *>(ws_bitmap_);
-    uint32_t *segcnt = static_cast<uint32_t *>(ws_segcnt_);
-    SH_HIP(hipMemsetAsync(meta, 0, 64, st));
-    SH_HIP(hipMemsetAsync(rowlen + d.N, 0, 4, st));
-    SH_HIP(hipMemsetAsync(segcnt + d.N, 0, 4, st));
-    // block-row structure from the node->term lists (dense-term counts and
-    // segments, row lengths)
-    SH_HIP(launch_row_bitmap(d, bitmap, rowlen, segcnt, meta, st));
-    SH_HIP(launch_scan_small(ScanPair{{segcnt, nullptr}, {d.seg_ptr, nullptr}, {uint32_t(d.N) + 1, 0}, 1}, st));
-    SH_HIP(hipMemcpyAsync(h_meta_ + 6, d.seg_ptr + d.N, 4, hipMemcpyDeviceToHost, st));
-    // reference node pair -> term index, half index (NodePair2TermsIndex::BuildHalfIndex, :294-298,591-654)
-    if (!d.pair_entries || !d.half_keys || !d.half_off || !d.upper_pos || !d.lower_pos)
+// Smallest bit count that holds every key below n_keys.
+static int key_bits(uint64_t n_keys)
+{
+    int b = 1;
+    while (b < 32 && (1ull << b) < n_keys)
+        ++b;
+    return b;
+}
+
+// The index build of one group, in three stages with one host readback
+// between them:
+//  1. valid-pixel flags (raster and tile order) with the id range checks,
+//     their group-wide scans; the pixel counts and check flags to the host;
+//  2. compaction into each slot's pixel streams and the Node2Term pairs, one
+//     stable node sort over the group (keys node0 + node), each slot's
+//     n2t lists, row bitmaps and dense-term counts, the row and segment scans,
+//     the block columns (the reference index: its pair index instead), the
+//     per-tile contribution sort and the tile count scans; the row, pair and
+//     slot totals to the host;
+//  3. compaction of the tiles' contributions and slots, one stable block sort
+//     over the group (keys blk0 + block), each slot's block lists and the
+//     block transpose map.
+// A slot that fails a check leaves the group (rcs) or ends the build.
+int Engine::build_group(const uint32_t *list, uint32_t n, hipStream_t st, int *rcs)
+{
+    GroupWs &g = gws_;
+    const bool tiled = tile_order(), ref = (cfg_.flags & SH_FLAG_REFERENCE_INDEX) != 0;
+    const uint32_t TP = tile_pix(), C = g.cap_items;
+    if (n == 0)
+        return SH_OK;
+    if (n > C || (ref && n > 1))
+        return SH_E_STATE;
+    std::vector<uint32_t> where(n); // each item's position in list (for rcs)
+    auto keep = [&](uint32_t i, int rc) {
+        // false: the build ends with rc
+        if (rcs && rc != SH_E_HIP)
+        {
+            rcs[where[i]] = rc;
+            return true;
+        }
+        return false;
+    };
+    auto set_map = [&](int m, auto wg_of) {
+        uint32_t *w = g_maps_.data() + size_t(m) * (C + 1);
+        uint32_t t = 0;
+        for (size_t i = 0; i < g_items_.size(); ++i)
+        {
+            w[i] = t;
+            t += wg_of(i);
+        }
+        w[g_items_.size()] = t;
+        return t;
+    };
+    auto desc_of = [&](size_t i) -> SolveDesc & { return slots_[g_slots_[i]].d; };
+    // drops the items whose rc is set (or ends the build), renumbering the rest
+    auto compact_items = [&](const std::vector<int> &rc_of) -> int {
+        size_t k = 0;
+        for (size_t i = 0; i < g_items_.size(); ++i)
+        {
+            if (rc_of[i] != SH_OK)
+            {
+                if (!keep(uint32_t(i), rc_of[i]))
+                    return rc_of[i];
+                continue;
+            }
+            g_slots_[k] = g_slots_[i];
+            g_items_[k] = g_items_[i];
+            g_items_[k].slot = uint32_t(k);
+            where[k] = where[i];
+            ++k;
+        }
+        g_slots_.resize(k);
+        g_items_.resize(k);
+        return SH_OK;
+    };
+    g_jobs_[0].clear();
+    g_jobs_[1].clear();
+
+    // ---- stage 1
+    g_slots_.assign(list, list + n);
+    g_items_.assign(n, IndexItem{});
+    uint64_t m0 = 0, h0 = 0;
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        where[i] = i;
+        Slot &sl = slots_[list[i]];
+        SolveDesc &d = sl.d;
+        // the GN phase's pixel order: tile order on the tile path, raster on the others
+        d.gpix = tiled ? sl.tpix : d.pix;
+        d.gpknn = tiled ? sl.tpknn : d.pknn;
+        d.gpw = tiled ? sl.tpw : d.pw;
+        d.tile_pix = int(TP);
+        d.P = 0;
+        d.n_tiles = 0;
+        sl.indexed = false;
+        IndexItem &it = g_items_[i];
+        it.slot = i;
+        it.m0 = uint32_t(m0);
+        it.h0 = uint32_t(h0);
+        m0 += uint64_t(d.W) * uint64_t(d.H);
+        h0 += tiled ? tile_order_size(uint32_t(d.W), uint32_t(d.H)) : 0u;
+    }
+    if (m0 > g.cap_m || h0 > g.cap_h)
+        return SH_E_STATE; // never: build_indexes sized the group
+    uint32_t wg = set_map(kGmFlags, [&](size_t i) {
+        const SolveDesc &d = desc_of(i);
+        const uint32_t px = uint32_t(d.W) * uint32_t(d.H);
+        const uint32_t nh = tiled ? tile_order_size(uint32_t(d.W), uint32_t(d.H)) : 0u;
+        return div_up(std::max(std::max(px, nh), uint32_t(d.E)), 256u);
+    });
+    int rc;
+    if ((rc = upload_group(0, st)) != SH_OK)
+        return rc;
+    IndexGroup gv = group_view();
+    SH_HIP(launch_pixel_flags(gv, group_map(kGmFlags), wg, tiled, g.ws, st));
+    size_t tb = g.sort_temp_bytes;
+    SH_HIP(exclusive_scan_u32(g.sort_temp, tb, g.ws.flags, g.ws.pos, uint32_t(m0), st));
+    if (tiled && h0 > 0)
+    {
+        tb = g.sort_temp_bytes;
+        SH_HIP(exclusive_scan_u32(g.sort_temp, tb, g.ws.tflags, g.ws.tpos, uint32_t(h0), st));
+    }
+    SH_HIP(launch_index_counts(gv, 1, g.ws, g.h_out, st));
+    if ((rc = wait_stream(st, index_ev_)) != SH_OK)
+        return rc;
+    {
+        std::vector<int> rc_of(n, SH_OK);
+        for (uint32_t i = 0; i < n; ++i)
+        {
+            const uint32_t *o = g.h_out + size_t(i) * 8;
+            if (o[1] != 0)
+                rc_of[i] = SH_E_ARG; // a surfel or node id out of range
+            else if (o[0] > pcap_)
+                rc_of[i] = SH_E_CAPACITY;
+            else
+                desc_of(i).P = int(o[0]);
+        }
+        if ((rc = compact_items(rc_of)) != SH_OK)
+            return rc;
+    }
+    if (g_items_.empty())
+        return SH_OK;
+
+    // ---- stage 2
+    uint64_t kv0 = 0, node0 = 0, t0 = 0, tc0 = 0;
+    for (size_t i = 0; i < g_items_.size(); ++i)
+    {
+        SolveDesc &d = desc_of(i);
+        IndexItem &it = g_items_[i];
+        const uint32_t P = uint32_t(d.P), N = uint32_t(d.N);
+        d.n_tiles = int(div_up(P, TP));
+        d.n_pair_entries = 0;
+        it.kv0 = uint32_t(kv0);
+        it.node0 = uint32_t(node0);
+        it.t0 = uint32_t(t0);
+        it.tc0 = uint32_t(tc0);
+        kv0 += uint64_t(P) * uint64_t(K_) + 2 * uint64_t(d.E) + N;
+        node0 += uint64_t(N) + 1;
+        t0 += uint64_t(d.n_tiles);
+        tc0 += uint64_t(d.n_tiles) + 2;
+        g_jobs_[0].push_back(ScanJob{g.ws.segcnt + it.node0, d.seg_ptr, N + 1, 0});
+        if (!ref)
+            g_jobs_[0].push_back(ScanJob{g.ws.rowlen + it.node0, d.row_ptr, N + 1, 0});
+        const uint32_t nt = uint32_t(d.n_tiles);
+        g_jobs_[1].push_back(ScanJob{g.ws.tnslot + it.tc0, d.tile_sbase, nt + 1, 0});
+        g_jobs_[1].push_back(ScanJob{g.ws.tncontrib + it.tc0, d.tile_cbase, nt + 1, 0});
+    }
+    if (kv0 > g.cap_kv || node0 > g.cap_nodes || tc0 > g.cap_tcnt || t0 * TP * uint64_t(K_ * (K_ + 1) / 2) > g.cap_tent)
+        return SH_E_STATE; // never: build_indexes sized the group
+    const uint32_t wg_compact = set_map(kGmCompact, [&](size_t i) {
+        const SolveDesc &d = desc_of(i);
+        return std::max(1u, div_up(uint32_t(d.W) * uint32_t(d.H), 256u)) + div_up(uint32_t(d.E + d.N), 256u);
+    });
+    const uint32_t wg_split = set_map(kGmSplit, [&](size_t i) {
+        const SolveDesc &d = desc_of(i);
+        const uint32_t n_kv = uint32_t(d.P) * uint32_t(K_) + 2u * uint32_t(d.E) + uint32_t(d.N);
+        return div_up(std::max(n_kv, uint32_t(d.N) + 1), 256u);
+    });
+    const uint32_t wg_node = set_map(kGmNode, [&](size_t i) { return uint32_t(desc_of(i).N); });
+    const uint32_t wg_tile = set_map(kGmTile, [&](size_t i) { return uint32_t(desc_of(i).n_tiles); });
+    if ((rc = upload_group(1, st)) != SH_OK)
+        return rc;
+    gv = group_view();
+    const ScanJob *jobs = reinterpret_cast<const ScanJob *>(g.d_tab + g.off_jobs);
+    SH_HIP(launch_compact_pixels(K_, gv, group_map(kGmCompact), wg_compact, tiled, g.ws, g.ka, g.va, st));
+    // Node2TermsIndex::BuildIndex (Node2TermsIndex.cu:191-199,312-360): one
+    // stable sort of the group's (node key, term) pairs
+    if (kv0 > 0)
+    {
+        tb = g.sort_temp_bytes;
+        SH_HIP(sort_pairs_u32_u32(g.sort_temp, tb, g.ka, g.kb, g.va, g.vb, uint32_t(kv0), key_bits(node0), st));
+    }
+    SH_HIP(launch_n2t_split(gv, group_map(kGmSplit), wg_split, g.kb, g.vb, st));
+    SH_HIP(launch_row_bitmap(K_, gv, group_map(kGmNode), wg_node, g.ws, st));
+    SH_HIP(launch_scan_jobs(jobs, uint32_t(g_jobs_[0].size()), st));
+    if (!ref)
+        SH_HIP(launch_row_cols(gv, group_map(kGmNode), wg_node, g.ws, st));
+    else if ((rc = reference_pair_index(slots_[g_slots_[0]], st)) != SH_OK)
+        return keep(0, rc) ? SH_OK : rc;
+    // the row structure is complete here (blk_col bounded by blk_cap)
+    SH_HIP(launch_tile_build(K_, TP, gv, group_map(kGmTile), wg_tile, g.ws, st));
+    SH_HIP(launch_scan_jobs(jobs + 2 * C, uint32_t(g_jobs_[1].size()), st));
+    SH_HIP(launch_index_counts(gv, 2, g.ws, g.h_out, st));
+    if ((rc = wait_stream(st, index_ev_)) != SH_OK)
+        return rc;
+    std::vector<uint32_t> n_slots(g_items_.size());
+    {
+        std::vector<int> rc_of(g_items_.size(), SH_OK);
+        for (size_t i = 0; i < g_items_.size(); ++i)
+        {
+            const uint32_t *o = g.h_out + i * 8;
+            Slot &sl = slots_[g_slots_[i]];
+            SolveDesc &d = sl.d;
+            int r = SH_OK;
+            if (o[2] > ucap_)
+                r = SH_E_CAPACITY;
+            else
+            {
+                d.U = int(o[2]);
+                if (!ref)
+                {
+                    sl.max_row_len = o[1];
+                    d.n_seg = int(o[0]);
+                    r = size_partials(sl);
+                }
+            }
+            if (r == SH_OK && uint64_t(o[3]) > pair_cap_)
+                r = SH_E_CAPACITY;
+            if (r == SH_OK)
+            {
+                sl.tile_slot_max = d.n_tiles > 0 ? o[5] : 0u;
+                r = size_tiles(sl, o[3], o[4]);
+            }
+            n_slots[i] = o[3];
+            rc_of[i] = r;
+        }
+        // n_slots follows the items
+        size_t k = 0;
+        for (size_t i = 0; i < rc_of.size(); ++i)
+            if (rc_of[i] == SH_OK)
+                n_slots[k++] = n_slots[i];
+        if ((rc = compact_items(rc_of)) != SH_OK)
+            return rc;
+    }
+    if (g_items_.empty())
+        return SH_OK;
+
+    // ---- stage 3
+    uint64_t sb0 = 0, blk0 = 0;
+    for (size_t i = 0; i < g_items_.size(); ++i)
+    {
+        const SolveDesc &d = desc_of(i);
+        g_items_[i].sb0 = uint32_t(sb0);
+        g_items_[i].blk0 = uint32_t(blk0);
+        sb0 += n_slots[i];
+        blk0 += uint64_t(d.N) + 2 * uint64_t(d.U) + 1;
+    }
+    if (sb0 > g.cap_kv)
+        return SH_E_STATE; // never: each solve's slots are at most its upper bound
+    const uint32_t wg_tile3 = set_map(kGmTile, [&](size_t i) { return uint32_t(desc_of(i).n_tiles); });
+    const uint32_t wg_lists = set_map(kGmLists, [&](size_t i) {
+        const SolveDesc &d = desc_of(i);
+        return div_up(std::max(n_slots[i], uint32_t(d.N) + 2u * uint32_t(d.U) + 1u), 256u);
+    });
+    const uint32_t wg_node3 = set_map(kGmNode, [&](size_t i) { return uint32_t(desc_of(i).N); });
+    if ((rc = upload_group(2, st)) != SH_OK)
+        return rc;
+    gv = group_view();
+    SH_HIP(launch_tile_compact(K_, TP, gv, group_map(kGmTile), wg_tile3, g.ws, g.ka, g.va, st));
+    if (sb0 > 0)
+    {
+        // the slots of every block in tile order (stable)
+        tb = g.sort_temp_bytes;
+        SH_HIP(sort_pairs_u32_u32(g.sort_temp, tb, g.ka, g.kb, g.va, g.vb, uint32_t(sb0), key_bits(blk0), st));
+    }
+    SH_HIP(launch_slot_lists(gv, group_map(kGmLists), wg_lists, g.kb, g.vb, st));
+    SH_HIP(launch_block_transpose(gv, group_map(kGmNode), wg_node3, st));
+    for (uint32_t s : g_slots_)
+        slots_[s].indexed = true;
+    return SH_OK;
+}
+
+// The reference's node-pair index and symmetric row blocks for the group's
+// one solve (NodePair2TermsIndex::BuildHalfIndex, :294-298,591-654, and
+// BuildSymmetricAndRowBlocksIndex, :300-312), after its node lists, row
+// bitmaps and segment scan: half keys and offsets, pair entries, row_ptr and
+// blk_col; sizes the segment partials.
+int Engine::reference_pair_index(Slot &sl, hipStream_t st)
+{
+    SolveDesc &d = sl.d;
+    GroupWs &g = gws_;
+    if (!d.pair_entries || !d.half_keys || !d.half_off || !d.upper_pos || !d.lower_pos || !g.head)
         return SH_E_STATE; // never launch on unallocated pair arrays
     const uint32_t np = uint32_t(K_ * (K_ - 1) / 2);
-    const uint32_t n_pkv = P * np + uint32_t(d.E);
-    uint32_t *k32a = static_cast<uint32_t *>(ws_k32a_), *k32b = static_cast<uint32_t *>(ws_k32b_);
-    uint32_t *p32a = static_cast<uint32_t *>(ws_p32a_), *p32b = static_cast<uint32_t *>(ws_p32b_);
-    uint32_t *head = static_cast<uint32_t *>(ws_head_), *hpos = static_cast<uint32_t *>(ws_hpos_);
-    SH_HIP(launch_pair_kv(d, k32a, p32a, st));
-    tb = sort_temp_bytes_;
-    SH_HIP(sort_pairs_u32_u32(sort_temp_, tb, k32a, k32b, p32a, p32b, n_pkv, 24, st));
-    SH_HIP(launch_pair_heads(k32b, n_pkv, head, meta + 8, st));
-    tb = sort_temp_bytes_;
-    SH_HIP(exclusive_scan_u32(sort_temp_, tb, head, hpos, n_pkv, st));
+    const uint32_t n_pkv = uint32_t(d.P) * np + uint32_t(d.E);
+    uint32_t *meta = g.ws.meta; // the group's only item
+    SH_HIP(hipMemcpyAsync(h_meta_ + 6, d.seg_ptr + d.N, 4, hipMemcpyDeviceToHost, st));
+    SH_HIP(launch_pair_kv(d, g.ka, g.va, st));
+    size_t tb = g.sort_temp_bytes;
+    SH_HIP(sort_pairs_u32_u32(g.sort_temp, tb, g.ka, g.kb, g.va, g.vb, n_pkv, 24, st));
+    SH_HIP(launch_pair_heads(g.kb, n_pkv, g.head, meta + 8, st));
+    tb = g.sort_temp_bytes;
+    SH_HIP(exclusive_scan_u32(g.sort_temp, tb, g.head, g.hpos, n_pkv, st));
     // the half arrays go to the workspace first: U is checked against the capacity below
-    SH_HIP(launch_pair_compact(k32b, head, hpos, n_pkv, meta + 8, k32a, p32a, meta + 2, st));
+    SH_HIP(launch_pair_compact(g.kb, g.head, g.hpos, n_pkv, meta + 8, g.ka, g.va, meta + 2, st));
     SH_HIP(hipMemcpyAsync(h_meta_, meta, 16, hipMemcpyDeviceToHost, st));
-    SH_HIP(hipStreamSynchronize(st));
+    if (int rc = wait_stream(st, index_ev_); rc != SH_OK)
+        return rc;
     const uint32_t U = h_meta_[2];
     if (U > ucap_)
         return SH_E_CAPACITY;
@@ -729,74 +792,15 @@ int Engine::build_index(uint32_t slot, hipStream_t st)
     d.n_seg = int(h_meta_[6]);
     if (int rc = size_partials(sl); rc != SH_OK)
         return rc;
-    SH_HIP(hipMemcpyAsync(d.half_keys, k32a, size_t(U) * 4, hipMemcpyDeviceToDevice, st));
-    SH_HIP(hipMemcpyAsync(d.half_off, p32a, size_t(U + 1) * 4, hipMemcpyDeviceToDevice, st));
-    SH_HIP(launch_pair_entries(d, p32b, st));
-
-    // symmetric index + block rows (BuildSymmetricAndRowBlocksIndex, :300-312)
-    SH_HIP(launch_sym_kv(d, k32a, p32a, st));
-    tb = sort_temp_bytes_;
+    SH_HIP(hipMemcpyAsync(d.half_keys, g.ka, size_t(U) * 4, hipMemcpyDeviceToDevice, st));
+    SH_HIP(hipMemcpyAsync(d.half_off, g.va, size_t(U + 1) * 4, hipMemcpyDeviceToDevice, st));
+    SH_HIP(launch_pair_entries(d, g.vb, st));
+    SH_HIP(launch_sym_kv(d, g.ka, g.va, st));
+    tb = g.sort_temp_bytes;
     if (U > 0)
-        SH_HIP(sort_pairs_u32_u32(sort_temp_, tb, k32a, k32b, p32a, p32b, 2 * U, 24, st));
-    SH_HIP(launch_sym_scatter(d, k32b, p32b, st));
-    SH_HIP(launch_row_ptr(d, k32b, st));
-    SH_HIP(launch_tile_build_async(d, lanes_[0], st, {}, {}));
-    SH_HIP(hipStreamSynchronize(st));
-    if (int rc = build_tiles(sl, lanes_[0], st); rc != SH_OK)
-        return rc;
-    sl.indexed = true;
-    return SH_OK;
-}
-
-// Tile index, first part (asynchronous): per-tile sorted contributions and
-// slots in the workspace, and the two count scans whose totals the host reads
-// (into h_meta_[10], h_meta_[11]) at the next synchronisation.
-hipError_t Engine::launch_tile_build_async(SolveDesc &d, IndexLane &ln, hipStream_t st,
-                                           std::initializer_list<const uint32_t *> more_src,
-                                           std::initializer_list<uint32_t> more_dst)
-{
-    d.n_tiles = int(div_up(uint32_t(d.P), uint32_t(d.tile_pix)));
-    const uint32_t nt = uint32_t(d.n_tiles);
-    uint32_t *ns = ln.tnslot, *nc = ln.tncontrib;
-    hipError_t e;
-    // ns[nt], ns[nt + 1] and nc[nt] were zeroed by the compaction kernel
-    if ((e = launch_tile_build(d, ln.tcontrib, ln.tslot_block, ln.tslot_first, ns, nc, st)) != hipSuccess)
-        return e;
-    if ((e = launch_scan_small(ScanPair{{ns, nc}, {d.tile_sbase, d.tile_cbase}, {nt + 1, nt + 1}, 2}, st)) !=
-        hipSuccess)
-        return e;
-    Readback rb{{d.tile_sbase + nt, d.tile_cbase + nt, ns + nt + 1}, {10, 11, 13}, 3};
-    for (size_t k = 0; k < more_src.size() && rb.n < 8; ++k, ++rb.n)
-    {
-        rb.src[rb.n] = more_src.begin()[k];
-        rb.dst[rb.n] = more_dst.begin()[k];
-    }
-    return launch_readback(rb, ln.h_meta, st);
-}
-
-// Tile index, second part (after the synchronisation): size the slot arrays,
-// compact, sort the slots by block (stable, so each block's list is in tile
-// order), list offsets and the mirror map.
-int Engine::build_tiles(Slot &sl, IndexLane &ln, hipStream_t st)
-{
-    SolveDesc &d = sl.d;
-    const uint32_t S = ln.h_meta[10], C = ln.h_meta[11];
-    if (uint64_t(S) > pair_cap_)
-        return SH_E_CAPACITY;
-    sl.tile_slot_max = d.n_tiles > 0 ? ln.h_meta[13] : 0u;
-    if (int rc = size_tiles(sl, S, C); rc != SH_OK)
-        return rc;
-    uint32_t *k32a = ln.k32a, *k32b = ln.k32b, *p32a = ln.p32a;
-    SH_HIP(launch_tile_compact(d, ln.tcontrib, ln.tslot_block, ln.tslot_first, k32a, p32a, st));
-    const uint32_t nb = uint32_t(d.N) + 2u * uint32_t(d.U);
-    int end_bit = 1;
-    while (end_bit < 32 && (1u << end_bit) <= nb)
-        ++end_bit;
-    size_t tb = sort_temp_bytes_;
-    if (S > 0)
-        SH_HIP(sort_pairs_u32_u32(ln.sort_temp, tb, k32a, k32b, p32a, d.list_items, S, end_bit, st));
-    SH_HIP(launch_slot_lists(d.list_items, k32b, S, nb + 1, d.slot_pos, d.list_off, st));
-    SH_HIP(launch_block_transpose(d, st));
+        SH_HIP(sort_pairs_u32_u32(g.sort_temp, tb, g.ka, g.kb, g.va, g.vb, 2 * U, 24, st));
+    SH_HIP(launch_sym_scatter(d, g.kb, g.vb, st));
+    SH_HIP(launch_row_ptr(d, g.kb, st));
     return SH_OK;
 }
 
@@ -1541,8 +1545,7 @@ int Engine::admit(uint32_t slot, hipStream_t st)
         return rc;
     SH_HIP(launch_identity_dq(sl.d.dq[0], uint32_t(sl.d.N), st));
     sl.d.gn_iter = 0;
-    SH_HIP(hipStreamSynchronize(st));
-    return SH_OK;
+    return wait_stream(st, index_ev_);
 }
 
 int Engine::admit_many(const uint32_t *list, uint32_t n, hipStream_t st, int *rcs)
@@ -1562,8 +1565,7 @@ int Engine::admit_many(const uint32_t *list, uint32_t n, hipStream_t st, int *rc
             SH_HIP(launch_identity_dq(sl.d.dq[0], uint32_t(sl.d.N), st));
             sl.d.gn_iter = 0;
         }
-    SH_HIP(hipStreamSynchronize(st));
-    return SH_OK;
+    return wait_stream(st, index_ev_);
 }
 
 // One GN iteration for an arbitrary set of admitted slots, each at its own

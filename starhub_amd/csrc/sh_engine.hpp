@@ -9,6 +9,7 @@
 
 #include "../../include/sh/warp_solver.h"
 #include "sh_desc.hpp"
+#include "sh_kernels.hpp"
 
 namespace sh
 {
@@ -32,11 +33,12 @@ public:
     // (index workspace); step() runs one GN iteration over a set of admitted
     // slots (launch state).  The two may run on different threads and streams.
     int admit(uint32_t slot, hipStream_t st);
-    // admit several slots at once (their index builds share the lanes); a
+    // admit several slots at once (their index builds run as groups); a
     // slot whose inputs are rejected gets its status in rcs and the others
     // proceed; the return value is SH_OK or a HIP error that failed them all
     int admit_many(const uint32_t *slots, uint32_t n, hipStream_t st, int *rcs);
-    uint32_t num_lanes() const { return uint32_t(lanes_.size()); }
+    // solves of the largest frame one index-build group holds
+    uint32_t index_units() const { return group_units_; }
     int step(const uint32_t *slots, uint32_t n, hipStream_t st);
     int gn_iter(uint32_t slot) const { return slots_[slot].d.gn_iter; }
     int cost_history(uint32_t slot, double *out, uint32_t capacity, uint32_t *count) const;
@@ -130,44 +132,46 @@ private:
         kNumMaps
     };
 
-    // Index-build workspace of one lane: the arrays a slot's index build
-    // uses between its host readbacks.  Lane 0 is the ws_* set below; a batch
-    // builds the indexes of its slots on several lanes at once, each with its
-    // own stream, so one slot's host readback overlaps the others' kernels.
-    struct IndexLane
+    // Index-build workspace of a group of solves: concatenated arrays sized
+    // for group_units_ solves of the largest frame (sums of per-solve upper
+    // bounds decide how many solves a group takes), the group's device tables
+    // (descriptors, items, work maps, scan jobs) and their pinned staging.
+    struct GroupWs
     {
-        uint32_t *flags = nullptr, *pos = nullptr, *v32a = nullptr, *meta = nullptr;
-        uint32_t *tflags = nullptr, *tpos = nullptr; // tile-order flags and their scan
-        uint16_t *k16a = nullptr, *k16b = nullptr;
-        uint32_t *k32a = nullptr, *k32b = nullptr, *p32a = nullptr;
-        uint32_t *bitmap = nullptr, *rowlen = nullptr, *segcnt = nullptr;
-        uint16_t *tcontrib = nullptr, *tslot_first = nullptr;
-        uint32_t *tslot_block = nullptr, *tnslot = nullptr, *tncontrib = nullptr;
+        IndexWs ws{};                               // the kernels' view
+        uint32_t *ka = nullptr, *kb = nullptr;      // sort keys in / out
+        uint32_t *va = nullptr, *vb = nullptr;      // sort values in / out
+        uint32_t *head = nullptr, *hpos = nullptr;  // the reference pair index only
         void *sort_temp = nullptr;
-        uint32_t *err = nullptr;    // device: the range-check flag
-        uint32_t *h_meta = nullptr; // pinned host readbacks
-        hipStream_t stream = nullptr;
-        hipEvent_t ev = nullptr;
-        int slot = -1; // scheduler state: the slot in progress, its position in the list and next stage
-        int item = -1;
-        int stage = 0;
+        size_t sort_temp_bytes = 0;
+        uint64_t cap_m = 0, cap_h = 0, cap_kv = 0, cap_nodes = 0, cap_tcnt = 0, cap_tent = 0;
+        uint32_t cap_items = 0;
+        char *d_tab = nullptr;                      // device tables
+        char *h_tab[3] = {nullptr, nullptr, nullptr}; // pinned staging, one per stage
+        size_t off_items = 0, off_maps = 0, off_jobs = 0, tab_bytes = 0;
+        uint32_t *h_out = nullptr;                  // pinned counts, 8 words per item
     };
-
+    enum
+    {
+        kGmFlags = 0,
+        kGmCompact,
+        kGmSplit,
+        kGmNode,
+        kGmTile,
+        kGmLists,
+        kNumGroupMaps
+    };
     int alloc(void **p, size_t bytes);
     void release();
-    int alloc_lane(IndexLane &ln, size_t npix);
+    int alloc_group_ws(uint32_t units, uint32_t max_items);
     int build_index(uint32_t slot, hipStream_t st);
     int build_indexes(const uint32_t *slots, uint32_t n, hipStream_t st, int *rcs = nullptr);
-    int index_pixels(uint32_t slot, IndexLane &ln, hipStream_t st);
-    int index_rows(uint32_t slot, IndexLane &ln, hipStream_t st);
-    int index_tiles(uint32_t slot, IndexLane &ln, hipStream_t st);
+    int build_group(const uint32_t *slots, uint32_t n, hipStream_t st, int *rcs);
+    int reference_pair_index(Slot &sl, hipStream_t st);
+    int upload_group(int stage, hipStream_t st);
+    IndexGroup group_view() const;
+    const uint32_t *group_map(int m) const;
     int size_partials(Slot &sl);
-    int build_tiles(Slot &sl, IndexLane &ln, hipStream_t st);
-    // the tile build and its count scans; their totals, with up to five more
-    // device words (more_src to h_meta[more_dst]), go to the lane's host buffer
-    hipError_t launch_tile_build_async(SolveDesc &d, IndexLane &ln, hipStream_t st,
-                                       std::initializer_list<const uint32_t *> more_src,
-                                       std::initializer_list<uint32_t> more_dst);
     int size_tiles(Slot &sl, uint32_t n_slots, uint32_t n_contrib);
     hipError_t mark(int phase, hipStream_t st);
     void collect_marks();
@@ -188,17 +192,16 @@ private:
     std::vector<Slot> slots_;
     std::vector<void *> allocs_;
     size_t bytes_ = 0;
-    // index workspace
-    void *ws_flags_ = nullptr, *ws_pos_ = nullptr, *ws_k16a_ = nullptr, *ws_k16b_ = nullptr, *ws_v32a_ = nullptr;
-    void *ws_k32a_ = nullptr, *ws_k32b_ = nullptr, *ws_p32a_ = nullptr, *ws_p32b_ = nullptr;
-    void *ws_head_ = nullptr, *ws_hpos_ = nullptr, *ws_meta_ = nullptr;
-    void *ws_bitmap_ = nullptr, *ws_rowlen_ = nullptr, *ws_segcnt_ = nullptr;
-    void *ws_tcontrib_ = nullptr, *ws_tslot_block_ = nullptr, *ws_tslot_first_ = nullptr;
-    void *ws_tnslot_ = nullptr, *ws_tncontrib_ = nullptr;
-    void *ws_tflags_ = nullptr, *ws_tpos_ = nullptr;
+    // index workspace and the group being built: its slots, items, work maps
+    // (kNumGroupMaps of n + 1 entries) and scan jobs (row scans, tile scans)
+    GroupWs gws_;
+    uint32_t group_units_ = 1;
+    std::vector<uint32_t> g_slots_;
+    std::vector<IndexItem> g_items_;
+    std::vector<uint32_t> g_maps_;
+    std::vector<ScanJob> g_jobs_[2];
     bool tile_order() const; // the GN phase reads the tile-order pixel streams
     uint32_t tiles_cap_ = 0;
-    std::vector<IndexLane> lanes_;
     int coop_max_groups_ = 0;
     size_t coop_lds_ = 0;
     bool coop_resident_ = false; // every solve of the launch fits the register-resident PCG
@@ -219,8 +222,6 @@ private:
     int grow_buffer(void **buf, size_t *held, size_t bytes);
     uint32_t pcg_fallbacks_ = 0;
     bool barrier_timeout_ = false; // the last finish() saw the PCG barrier flag
-    void *sort_temp_ = nullptr;
-    size_t sort_temp_bytes_ = 0;
     uint32_t *h_meta_ = nullptr; // index-build readbacks
     uint32_t *h_flag_ = nullptr; // launch-side readback (PCG barrier flag)
     hipEvent_t sync_ev_ = nullptr;  // polled by finish (the launch half)

@@ -68,28 +68,35 @@ __device__ __forceinline__ uint32_t tile_order(uint32_t x, uint32_t y, uint32_t 
 
 // Valid-pixel flags, and the range checks every later index kernel relies on:
 // a valid pixel names a surfel below S whose K node ids are below N, and every
-// ARAP edge names nodes below N.  A pixel that fails is flagged 0 and sets
-// *err; the host rejects the frame before anything reads the ids.  With
-// tflags, the threads walk the tile-order index space (nh entries, nbx
-// blocks per block row) and also write each flag at its tile-order index
-// (0 for the padding outside the image), for the tile-order scan.
-__global__ void k_pixel_flags(SolveDesc d, uint32_t n, uint32_t nh, uint32_t nbx, uint32_t *__restrict__ flags,
-                              uint32_t *__restrict__ tflags, uint32_t *__restrict__ err)
+// ARAP edge names nodes below N.  A pixel that fails is flagged 0 and sets the
+// item's error word; the host rejects the frame before anything reads the
+// ids.  Tiled, the threads walk the tile-order index space (nbx blocks per
+// block row) and also write each flag at its tile-order index (0 for the
+// padding outside the image), for the tile-order scan.
+__global__ __launch_bounds__(256) void k_pixel_flags(IndexGroup g, const uint32_t *__restrict__ wmap, bool tiled,
+                                                     IndexWs ws)
 {
-    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t N = uint32_t(d.N);
+    const int it = find_solve(wmap, g.n, blockIdx.x);
+    const IndexItem item = g.items[it];
+    const SolveDesc &d = g.sds[item.slot];
+    const uint32_t W = uint32_t(d.W), H = uint32_t(d.H), N = uint32_t(d.N), E = uint32_t(d.E);
+    const uint32_t n = W * H, nbx = (W + 31u) / 32u;
+    const uint32_t nh = tiled ? nbx * ((H + 31u) / 32u) * 1024u : 0u;
+    uint32_t *__restrict__ flags = ws.flags + item.m0;
+    uint32_t *__restrict__ tflags = ws.tflags + item.h0;
+    const uint32_t h = (blockIdx.x - wmap[it]) * blockDim.x + threadIdx.x;
     bool bad = false;
     uint32_t i = h; // raster index of the pixel this thread decides
     bool inside = h < n;
-    if (tflags)
+    if (tiled)
     {
         inside = false;
         if (h < nh)
         {
             const uint32_t b = h >> 10, m = h & 1023u;
             const uint32_t x = (b % nbx) * 32u + squeeze5(m), y = (b / nbx) * 32u + squeeze5(m >> 1);
-            inside = x < uint32_t(d.W) && y < uint32_t(d.H);
-            i = y * uint32_t(d.W) + x;
+            inside = x < W && y < H;
+            i = y * W + x;
             if (!inside)
                 tflags[h] = 0u;
         }
@@ -104,72 +111,77 @@ __global__ void k_pixel_flags(SolveDesc d, uint32_t n, uint32_t nh, uint32_t nbx
             if (d.surfel_knn[size_t(s) * d.K + k] >= N)
                 valid = false, bad = true;
         flags[i] = valid ? 1u : 0u;
-        if (tflags)
+        if (tiled)
             tflags[h] = valid ? 1u : 0u;
     }
-    if (h < uint32_t(d.E) && (d.edges[3 * size_t(h)] >= N || d.edges[3 * size_t(h) + 1] >= N))
+    if (h < E && (d.edges[3 * size_t(h)] >= N || d.edges[3 * size_t(h) + 1] >= N))
         bad = true;
     if (bad)
-        atomicOr(err, 1u);
+        atomicOr(ws.errs + it, 1u);
 }
 
 // Stream-compact the valid pixels in row-major order; gathers surfel KNN and
 // folds spatial*connect into one weight per slot (queryPixelKNNKernel,
 // WarpSolver.cu:7-48, + compactPontentialImageTermPixelsKernel,
-// ImageTermKNNFetcher.cu:56-88).
+// ImageTermKNNFetcher.cu:56-88).  The positions are the group-wide scans less
+// the item's first entry.
 template <int K>
-__global__ void k_compact_pixels(SolveDesc d, const uint32_t *__restrict__ flags, const uint32_t *__restrict__ pos,
-                                 const uint32_t *__restrict__ tpos, uint32_t nbx, IndexZero z,
-                                 uint16_t *__restrict__ kv_keys, uint32_t *__restrict__ kv_vals)
+__global__ __launch_bounds__(256) void k_compact_pixels(IndexGroup g, const uint32_t *__restrict__ wmap, bool tiled,
+                                                        IndexWs ws, uint32_t *__restrict__ kv_keys,
+                                                        uint32_t *__restrict__ kv_vals)
 {
-    const uint32_t n_pix_wg = (uint32_t(d.W) * uint32_t(d.H) + 255u) / 256u;
-    if (kv_keys && blockIdx.x >= max(1u, n_pix_wg))
+    const int it = find_solve(wmap, g.n, blockIdx.x);
+    const IndexItem item = g.items[it];
+    const SolveDesc d = global_desc(g.sds[item.slot]);
+    const uint32_t blk = blockIdx.x - wmap[it];
+    const uint32_t W = uint32_t(d.W), n = W * uint32_t(d.H);
+    const uint32_t n_pix_wg = max(1u, (n + 255u) / 256u);
+    const uint32_t P = d.P, E = d.E, N = d.N;
+    if (blk >= n_pix_wg)
     {
         // the ARAP and node-translation terms' (node key, term) pairs: 2 per
-        // edge (a self edge's keys are 0xffff), 1 per node (Node2TermsIndex.cu:6-76)
-        const uint32_t t = uint32_t(d.P) + (blockIdx.x - max(1u, n_pix_wg)) * blockDim.x + threadIdx.x;
-        const uint32_t P = d.P, E = d.E, N = d.N;
+        // edge (a self edge's keys sort after the nodes), 1 per node
+        // (Node2TermsIndex.cu:6-76)
+        const uint32_t t = P + (blk - n_pix_wg) * blockDim.x + threadIdx.x;
         if (t < P + E)
         {
             const uint32_t e = t - P;
-            const uint16_t a = d.edges[3 * e], b = d.edges[3 * e + 1];
-            const size_t o = size_t(P) * K + 2 * size_t(e);
-            kv_keys[o] = (a != b) ? a : 0xffff;
-            kv_keys[o + 1] = (a != b) ? b : 0xffff;
+            const uint32_t a = d.edges[3 * e], b = d.edges[3 * e + 1];
+            const size_t o = item.kv0 + size_t(P) * K + 2 * size_t(e);
+            kv_keys[o] = item.node0 + ((a != b) ? a : N);
+            kv_keys[o + 1] = item.node0 + ((a != b) ? b : N);
             kv_vals[o] = t;
             kv_vals[o + 1] = t;
         }
         else if (t < P + E + N)
         {
             const uint32_t m = t - P - E;
-            const size_t o = size_t(P) * K + 2 * size_t(E) + m;
-            kv_keys[o] = uint16_t(m);
+            const size_t o = item.kv0 + size_t(P) * K + 2 * size_t(E) + m;
+            kv_keys[o] = item.node0 + m;
             kv_vals[o] = t;
         }
         return;
     }
-    if (blockIdx.x == 0)
+    if (blk == 0)
     {
-        const uint32_t t = threadIdx.x;
+        // the words the later stages accumulate into
+        const uint32_t t = threadIdx.x, nt = uint32_t(d.n_tiles);
         if (t < 16)
-            z.meta[t] = 0u;
-        else if (t == 16)
-            *z.err = 0u;
+            ws.meta[size_t(it) * 16 + t] = 0u;
         else if (t == 17)
-            *z.rowlen_end = 0u;
+            ws.rowlen[item.node0 + N] = 0u;
         else if (t == 18)
-            *z.segcnt_end = 0u;
+            ws.segcnt[item.node0 + N] = 0u;
         else if (t == 19 || t == 20)
-            z.tile_nslot_end[t - 19] = 0u;
+            ws.tnslot[item.tc0 + nt + (t - 19)] = 0u;
         else if (t == 21)
-            *z.tile_ncontrib_end = 0u;
+            ws.tncontrib[item.tc0 + nt] = 0u;
     }
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
-    if (i >= n || flags[i] == 0u)
+    const uint32_t i = blk * blockDim.x + threadIdx.x;
+    if (i >= n || ws.flags[item.m0 + i] == 0u)
         return;
-    const uint32_t o = pos[i];
-    const uint32_t x = i % uint32_t(d.W), y = i / uint32_t(d.W);
+    const uint32_t o = ws.pos[item.m0 + i] - ws.pos[item.m0];
+    const uint32_t x = i % W, y = i / W;
     const uint32_t s = d.index_map[i];
     float fx = 0.f, fy = 0.f;
     if (d.flow)
@@ -196,20 +208,17 @@ __global__ void k_compact_pixels(SolveDesc d, const uint32_t *__restrict__ flags
         d.pknn[size_t(o) * K + k] = kn[k];
         d.pw[size_t(o) * K + k] = w[k];
     }
-    if (kv_keys)
-    {
-        // the pixel term's (node key, term) pairs, K of them (Node2TermsIndex.cu:6-76)
+    // the pixel term's (node key, term) pairs, K of them (Node2TermsIndex.cu:6-76)
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-        {
-            kv_keys[size_t(o) * K + k] = kn[k];
-            kv_vals[size_t(o) * K + k] = o;
-        }
+    for (int k = 0; k < K; ++k)
+    {
+        kv_keys[item.kv0 + size_t(o) * K + k] = item.node0 + kn[k];
+        kv_vals[item.kv0 + size_t(o) * K + k] = o;
     }
-    if (tpos)
+    if (tiled)
     {
         // the GN phase's copy, at the pixel's tile-order position
-        const uint32_t ot = tpos[tile_order(x, y, nbx)];
+        const uint32_t ot = ws.tpos[item.h0 + tile_order(x, y, (W + 31u) / 32u)] - ws.tpos[item.h0];
         d.gpix[ot] = px;
 #pragma unroll
         for (int k = 0; k < K; ++k)
@@ -220,24 +229,64 @@ __global__ void k_compact_pixels(SolveDesc d, const uint32_t *__restrict__ flags
     }
 }
 
-// offset[n] = first sorted position whose key >= n  (computeTermOffsetKernel,
-// Node2TermsIndex.cu:78-122, written as a lower bound)
-__global__ void k_lower_bound_u16(const uint16_t *__restrict__ keys, uint32_t n_keys, uint32_t n_out,
-                                  uint32_t *__restrict__ out)
+// After the stable node sort of the group's pairs: each solve's sorted term
+// ids, and offset[v] = the first sorted position whose key is >= v
+// (computeTermOffsetKernel, Node2TermsIndex.cu:78-122, written as a lower
+// bound), v <= N.
+__global__ __launch_bounds__(256) void k_n2t_split(IndexGroup g, const uint32_t *__restrict__ wmap,
+                                                   const uint32_t *__restrict__ skeys,
+                                                   const uint32_t *__restrict__ svals)
 {
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n_out)
-        return;
-    uint32_t lo = 0, hi = n_keys;
-    while (lo < hi)
+    const int it = find_solve(wmap, g.n, blockIdx.x);
+    const IndexItem item = g.items[it];
+    const SolveDesc d = global_desc(g.sds[item.slot]);
+    const uint32_t N = uint32_t(d.N);
+    const uint32_t n_kv = uint32_t(d.P) * uint32_t(d.K) + 2u * uint32_t(d.E) + N;
+    const uint32_t v = (blockIdx.x - wmap[it]) * blockDim.x + threadIdx.x;
+    if (v < n_kv)
+        d.n2t_terms[v] = svals[item.kv0 + v];
+    if (v <= N)
     {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (uint32_t(keys[mid]) < v)
-            lo = mid + 1;
-        else
-            hi = mid;
+        const uint32_t key = item.node0 + v;
+        uint32_t lo = item.kv0, hi = item.kv0 + n_kv;
+        while (lo < hi)
+        {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (skeys[mid] < key)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        d.n2t_off[v] = lo - item.kv0;
     }
-    out[v] = lo;
+}
+
+// The counts the host reads between the stages (see launch_index_counts).
+__global__ void k_index_counts(IndexGroup g, int stage, IndexWs ws, uint32_t *host)
+{
+    const int it = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (it >= g.n)
+        return;
+    const IndexItem item = g.items[it];
+    const SolveDesc &d = g.sds[item.slot];
+    uint32_t *o = host + size_t(it) * 8;
+    if (stage == 1)
+    {
+        const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
+        o[0] = n ? ws.pos[item.m0 + n - 1] + ws.flags[item.m0 + n - 1] - ws.pos[item.m0] : 0u;
+        o[1] = ws.errs[it];
+        ws.errs[it] = 0u; // for the next build
+    }
+    else
+    {
+        const uint32_t N = uint32_t(d.N), nt = uint32_t(d.n_tiles);
+        o[0] = d.seg_ptr[N];
+        o[1] = ws.meta[size_t(it) * 16];
+        o[2] = d.row_ptr[N] / 2u;
+        o[3] = d.tile_sbase[nt];
+        o[4] = d.tile_cbase[nt];
+        o[5] = ws.tnslot[item.tc0 + nt + 1];
+    }
 }
 
 // (i<j) pair keys i*4096+j per term, padded with the uninitialised key:
@@ -1294,50 +1343,45 @@ uint32_t tile_order_size(uint32_t width, uint32_t height)
     return tile_order_blocks_x(width) * ((height + 31u) / 32u) * 1024u;
 }
 
-hipError_t launch_pixel_flags(const SolveDesc &d, uint32_t n, uint32_t *flags, uint32_t *tflags, uint32_t *err,
-                              hipStream_t st)
+hipError_t launch_pixel_flags(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, bool tiled,
+                              const IndexWs &ws, hipStream_t st)
 {
-    const uint32_t nh = tflags ? tile_order_size(uint32_t(d.W), uint32_t(d.H)) : 0u;
-    const uint32_t m = max(max(n, nh), uint32_t(d.E));
-    if (m == 0)
+    if (n_wg == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_pixel_flags, SH_GRID(m, 256), dim3(256), 0, st, d, n, nh,
-                       tile_order_blocks_x(uint32_t(d.W)), flags, tflags, err);
+    hipLaunchKernelGGL(k_pixel_flags, dim3(n_wg), dim3(256), 0, st, g, wmap, tiled, ws);
     return hipGetLastError();
 }
 
-hipError_t launch_compact_pixels_kv(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos,
-                                    const uint32_t *tpos, const IndexZero &z, uint16_t *keys, uint32_t *vals,
-                                    hipStream_t st)
+hipError_t launch_compact_pixels(int K, const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, bool tiled,
+                                 const IndexWs &ws, uint32_t *keys, uint32_t *vals, hipStream_t st)
 {
-    const uint32_t n = uint32_t(d.W) * uint32_t(d.H);
-    // block 0 always runs: it clears z; the pair work-groups follow the pixels'
-    const uint32_t n_pix_wg = max(1u, (n + 255u) / 256u);
-    const uint32_t n_kv_wg = keys ? (uint32_t(d.E) + uint32_t(d.N) + 255u) / 256u : 0u;
-    const dim3 grid(n_pix_wg + n_kv_wg);
-    const uint32_t nbx = tile_order_blocks_x(uint32_t(d.W));
-    if (d.K == 8)
-        hipLaunchKernelGGL(k_compact_pixels<8>, grid, dim3(256), 0, st, d, flags, pos, tpos, nbx, z, keys, vals);
+    if (n_wg == 0)
+        return hipSuccess;
+    if (K == 8)
+        hipLaunchKernelGGL(k_compact_pixels<8>, dim3(n_wg), dim3(256), 0, st, g, wmap, tiled, ws, keys, vals);
     else
-        hipLaunchKernelGGL(k_compact_pixels<4>, grid, dim3(256), 0, st, d, flags, pos, tpos, nbx, z, keys, vals);
+        hipLaunchKernelGGL(k_compact_pixels<4>, dim3(n_wg), dim3(256), 0, st, g, wmap, tiled, ws, keys, vals);
     return hipGetLastError();
 }
 
-hipError_t launch_compact_pixels(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos, const uint32_t *tpos,
-                                 const IndexZero &z, hipStream_t st)
+hipError_t launch_n2t_split(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const uint32_t *skeys,
+                            const uint32_t *svals, hipStream_t st)
 {
-    return launch_compact_pixels_kv(d, flags, pos, tpos, z, nullptr, nullptr, st);
+    if (n_wg == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_n2t_split, dim3(n_wg), dim3(256), 0, st, g, wmap, skeys, svals);
+    return hipGetLastError();
 }
 
-__global__ __launch_bounds__(1024) void k_scan_small(ScanPair p)
+// One work-group per job; each thread scans a contiguous run of the array.
+__global__ __launch_bounds__(1024) void k_scan_jobs(const ScanJob *__restrict__ jobs)
 {
     __shared__ uint32_t wsum[16];
-    const uint32_t a = blockIdx.x;
-    const uint32_t n = p.n[a];
-    const uint32_t *__restrict__ in = p.in[a];
-    uint32_t *__restrict__ out = p.out[a];
+    const ScanJob j = jobs[blockIdx.x];
+    const uint32_t n = j.n;
+    const uint32_t *__restrict__ in = j.in;
+    uint32_t *__restrict__ out = j.out;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-    // each thread a contiguous run of the array
     const uint32_t per = (n + 1023u) / 1024u;
     const uint32_t b = min(n, tid * per), e = min(n, b + per);
     uint32_t own = 0;
@@ -1365,28 +1409,19 @@ __global__ __launch_bounds__(1024) void k_scan_small(ScanPair p)
     }
 }
 
-hipError_t launch_scan_small(const ScanPair &p, hipStream_t st)
+hipError_t launch_scan_jobs(const ScanJob *jobs, uint32_t n_jobs, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_scan_small, dim3(p.count), dim3(1024), 0, st, p);
+    if (n_jobs == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_scan_jobs, dim3(n_jobs), dim3(1024), 0, st, jobs);
     return hipGetLastError();
 }
 
-__global__ void k_readback(Readback r, uint32_t *host)
+hipError_t launch_index_counts(const IndexGroup &g, int stage, const IndexWs &ws, uint32_t *host, hipStream_t st)
 {
-    if (threadIdx.x < r.n)
-        host[r.dst[threadIdx.x]] = *r.src[threadIdx.x];
-}
-
-hipError_t launch_readback(const Readback &r, uint32_t *host, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, st, r, host);
-    return hipGetLastError();
-}
-
-hipError_t launch_lower_bound_u16(const uint16_t *keys, uint32_t n_keys, uint32_t n_out, uint32_t *out,
-                                  hipStream_t st)
-{
-    hipLaunchKernelGGL(k_lower_bound_u16, SH_GRID(n_out, 256), dim3(256), 0, st, keys, n_keys, n_out, out);
+    if (g.n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_index_counts, dim3((uint32_t(g.n) + 63u) / 64u), dim3(64), 0, st, g, stage, ws, host);
     return hipGetLastError();
 }
 

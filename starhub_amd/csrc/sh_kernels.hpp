@@ -13,52 +13,75 @@ namespace sh
 // blocks in raster order, Morton order inside a block; index space size.
 uint32_t tile_order_blocks_x(uint32_t width);
 uint32_t tile_order_size(uint32_t width, uint32_t height);
-// tflags NULL: raster flags only
-hipError_t launch_pixel_flags(const SolveDesc &d, uint32_t n, uint32_t *flags, uint32_t *tflags, uint32_t *err,
-                              hipStream_t st);
-// Words an index build accumulates into, zeroed by the compaction kernel for
-// the stages after it (and err for the next build on the same workspace).
-struct IndexZero
+
+// ------------------------------------------------------------ index build
+// The index build runs over a group of solves: every stage is one launch
+// over all of them.  A solve's arrays in the concatenated workspace start at
+// its item's offsets; its own outputs go to the arrays its descriptor names.
+struct IndexItem
 {
-    uint32_t *err;               // 1 word
-    uint32_t *meta;              // 16 words
-    uint32_t *rowlen_end;        // 1 word
-    uint32_t *segcnt_end;        // 1 word
-    uint32_t *tile_nslot_end;    // 2 words: the scan total and the largest count
-    uint32_t *tile_ncontrib_end; // 1 word
+    uint32_t slot;  // the solve's descriptor in the group's descriptor array
+    uint32_t m0;    // first raster flag (W * H per solve)
+    uint32_t h0;    // first tile-order flag (tile_order_size per solve)
+    uint32_t kv0;   // first Node2Term pair (P K + 2 E + N per solve)
+    uint32_t node0; // first node key (N + 1 per solve): node sort keys, bitmap rows, row and segment counts
+    uint32_t t0;    // first tile of the tile workspace (n_tiles per solve, each at the full tile stride)
+    uint32_t tc0;   // first tile count (n_tiles + 2 per solve: counts, scan total, largest count)
+    uint32_t sb0;   // first slot of the block sort (the solve's slot count)
+    uint32_t blk0;  // first block key (N + 2 U + 1 per solve)
+    uint32_t pad[7];
 };
-// tpos NULL: raster arrays only (the GN phase reads them); else also the
-// tile-order copies (gpix, gpknn, gpw) at their tile-order scan positions
-hipError_t launch_compact_pixels(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos, const uint32_t *tpos,
-                                 const IndexZero &z, hipStream_t st);
-// The same compaction, also writing the Node2Term (node key, term) pairs that
-// the stable node sort takes: K per pixel term (from the compacted ids), 2 per
-// ARAP edge and 1 per node-translation term (extra work-groups; d.P is set).
-hipError_t launch_compact_pixels_kv(const SolveDesc &d, const uint32_t *flags, const uint32_t *pos,
-                                    const uint32_t *tpos, const IndexZero &z, uint16_t *keys, uint32_t *vals,
-                                    hipStream_t st);
-// Up to four device words copied into a pinned host buffer by one small
-// kernel (host[dst[i]] = *src[i]), in place of one copy per word.
-struct Readback
+static_assert(sizeof(IndexItem) == 64, "index item");
+// The group's device tables.
+struct IndexGroup
 {
-    const uint32_t *src[8];
-    uint32_t dst[8];
+    const SolveDesc *sds;
+    const IndexItem *items;
+    int n;
+};
+// The group's workspace.  errs and the per-item meta words are cleared by the
+// kernels that read them last, so they are zero when the next build starts.
+struct IndexWs
+{
+    uint32_t *flags, *pos;         // raster valid flags and their exclusive scan
+    uint32_t *tflags, *tpos;       // the same in tile order
+    uint32_t *errs;                // one range-check word per item
+    uint32_t *meta;                // 16 words per item: [0] longest row
+    uint32_t *bitmap;              // kBitmapWords per node key
+    uint32_t *rowlen, *segcnt;     // per node key
+    uint16_t *tcontrib;            // per tile, at its fixed stride: sorted contributions
+    uint32_t *tslot_block;         // per tile: each slot's block
+    uint16_t *tslot_first;         // per tile: each slot's first contribution
+    uint32_t *tnslot, *tncontrib;  // per tile count
+};
+// Exclusive scan of one short array (counts per node or per tile) by one
+// work-group: out[i] = in[0] + .. + in[i - 1] for i < n.
+struct ScanJob
+{
+    const uint32_t *in;
+    uint32_t *out;
     uint32_t n;
+    uint32_t pad;
 };
-// Exclusive scans of one or two short arrays (counts per node or per tile:
-// a few thousand to a few ten thousand words), one work-group each, in one
-// launch: out[a][i] = in[a][0] + .. + in[a][i - 1] for i < n[a].
-struct ScanPair
-{
-    const uint32_t *in[2];
-    uint32_t *out[2];
-    uint32_t n[2];
-    uint32_t count; // 1 or 2
-};
-hipError_t launch_scan_small(const ScanPair &p, hipStream_t st);
-hipError_t launch_readback(const Readback &r, uint32_t *host, hipStream_t st);
-hipError_t launch_lower_bound_u16(const uint16_t *keys, uint32_t n_keys, uint32_t n_out, uint32_t *out,
-                                  hipStream_t st);
+// Every launch takes a work map: wmap[i] is the first work-group of item i,
+// wmap[n] = n_wg.
+// Valid-pixel flags (tiled: also in tile order) and the range checks.
+hipError_t launch_pixel_flags(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, bool tiled,
+                              const IndexWs &ws, hipStream_t st);
+// Compaction of the valid pixels into each solve's streams (tiled: also the
+// tile-order copies), the Node2Term (node key, term) pairs at kv0 (keys
+// node0 + node, node0 + N for a self edge), and the clearing of the words
+// later stages accumulate into.
+hipError_t launch_compact_pixels(int K, const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, bool tiled,
+                                 const IndexWs &ws, uint32_t *keys, uint32_t *vals, hipStream_t st);
+// After the node sort: each solve's n2t_terms and n2t_off.
+hipError_t launch_n2t_split(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const uint32_t *skeys,
+                            const uint32_t *svals, hipStream_t st);
+hipError_t launch_scan_jobs(const ScanJob *jobs, uint32_t n_jobs, hipStream_t st);
+// Counts for the host, host[item * 8 + k]: stage 1 -> [0] valid pixels,
+// [1] range-check flag (then cleared); stage 2 -> [0] segments, [1] longest
+// row, [2] U, [3] slots, [4] contributions, [5] largest tile slot count.
+hipError_t launch_index_counts(const IndexGroup &g, int stage, const IndexWs &ws, uint32_t *host, hipStream_t st);
 hipError_t launch_pair_kv(const SolveDesc &d, uint32_t *keys, uint32_t *vals, hipStream_t st);
 hipError_t launch_pair_heads(const uint32_t *keys, uint32_t n, uint32_t *head, uint32_t *meta, hipStream_t st);
 hipError_t launch_pair_compact(const uint32_t *keys, const uint32_t *head, const uint32_t *hpos, uint32_t n,
@@ -104,9 +127,13 @@ hipError_t launch_skin(uint32_t K, const float4 *surfels, uint32_t S, const floa
 hipError_t launch_node_graph(uint32_t K, const float4 *nodes, uint32_t N, float r2, uint16_t *node_knn,
                              float *node_spatial, uint16_t *edges, float *node_connect, hipStream_t st);
 
-hipError_t launch_row_bitmap(const SolveDesc &d, uint32_t *bitmap, uint32_t *row_len, uint32_t *seg_cnt,
-                             uint32_t *meta, hipStream_t st);
-hipError_t launch_row_cols(const SolveDesc &d, const uint32_t *bitmap, uint32_t *meta, hipStream_t st);
+// One work-group per node: the row bitmap, row length, dense-term count and
+// segment count; the longest row into the item's meta[0].
+hipError_t launch_row_bitmap(int K, const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const IndexWs &ws,
+                             hipStream_t st);
+// One work-group per node: the block columns from the bitmap and row_ptr.
+hipError_t launch_row_cols(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const IndexWs &ws,
+                           hipStream_t st);
 // Dense terms per segment of a node row (one work-group of the segment kernel).
 constexpr uint32_t kSegTerms = 256;
 struct RowsPlan
@@ -209,12 +236,15 @@ hipError_t launch_forward_warp(uint32_t K, const float4 *warp, const float4 *ref
 
 // Pixel tiles (sh_tiles.hip): 64, 128 (default) or 256 pixels, per the tuning
 constexpr uint32_t kTilePixMin = 64, kTilePixDefault = 128, kTilePixMax = 256;
-hipError_t launch_tile_build(const SolveDesc &d, uint16_t *ws_contrib, uint32_t *ws_slot_block,
-                             uint16_t *ws_slot_first, uint32_t *tile_nslot, uint32_t *tile_ncontrib, hipStream_t st);
-hipError_t launch_tile_compact(const SolveDesc &d, const uint16_t *ws_contrib, const uint32_t *ws_slot_block,
-                               const uint16_t *ws_slot_first, uint32_t *sort_keys, uint32_t *sort_vals,
-                               hipStream_t st);
-hipError_t launch_block_transpose(const SolveDesc &d, hipStream_t st);
+// One work-group per tile: the tile's sorted contributions, slots and counts.
+hipError_t launch_tile_build(int K, uint32_t tile_pix, const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg,
+                             const IndexWs &ws, hipStream_t st);
+// One work-group per tile: contributions and slots to their compact places,
+// and the (blk0 + block, slot) pairs of the block sort at sb0.
+hipError_t launch_tile_compact(int K, uint32_t tile_pix, const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg,
+                               const IndexWs &ws, uint32_t *sort_keys, uint32_t *sort_vals, hipStream_t st);
+// One work-group per node: the transpose map of the row's blocks.
+hipError_t launch_block_transpose(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, hipStream_t st);
 // fused: the tile kernel builds the FP32 per-pixel records itself (and the
 // dense cost partials) from the warp field of GN iteration offset parity,
 // instead of reading them; the term kernel then runs the ARAP part only.
@@ -225,10 +255,11 @@ hipError_t launch_tiles(int K, bool fp16, uint32_t tile_pix, bool fused, int par
 constexpr uint32_t kWaveCombineMaxBlocks = 73;
 // LDS of the tile combine: one work-group per row (wave = false) or one wave per row
 size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len, bool wave);
-// slot_pos[list_items[l]] = l for l < n: the list position of every slot, and
-// list_off[b] (b < n_blocks): the first list position of block b in the sorted blocks
-hipError_t launch_slot_lists(const uint32_t *list_items, const uint32_t *sorted_blocks, uint32_t n,
-                             uint32_t n_blocks, uint32_t *slot_pos, uint32_t *list_off, hipStream_t st);
+// After the block sort: each solve's list_items (slot ids by block),
+// slot_pos[list_items[l]] = l, and list_off[b] = the first list position of
+// block b (b <= N + 2U).
+hipError_t launch_slot_lists(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const uint32_t *skeys,
+                             const uint32_t *svals, hipStream_t st);
 // matrix-free J^T W J x (x = d_{k+1} of each solve unless x_user; y = d.q unless y_user)
 hipError_t launch_mf_tiles(int K, bool fp16, uint32_t tile_pix, const SolveDesc *sds, const uint32_t *wmap,
                            int n_solve, uint32_t n_wg, int k_it, const float *x_user, hipStream_t st);
@@ -242,8 +273,6 @@ hipError_t launch_tiles_combine(const SolveDesc *sds, const uint32_t *wmap, int 
                                 bool wave, PcgSlot2 *slots2, uint32_t n_slots2, hipStream_t st);
 
 // rocPRIM wrappers (sh_sort.hip); temp == nullptr queries the temp size
-hipError_t sort_pairs_u16_u32(void *temp, size_t &temp_bytes, const uint16_t *kin, uint16_t *kout,
-                              const uint32_t *vin, uint32_t *vout, uint32_t n, int end_bit, hipStream_t st);
 hipError_t sort_pairs_u32_u32(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout,
                               const uint32_t *vin, uint32_t *vout, uint32_t n, int end_bit, hipStream_t st);
 hipError_t exclusive_scan_u32(void *temp, size_t &temp_bytes, const uint32_t *in, uint32_t *out, uint32_t n,

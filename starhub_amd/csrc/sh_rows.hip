@@ -53,14 +53,15 @@ constexpr int kMaxWords = kPairBase / 32; // bitmap words for N <= 4096
 // ARAP and node-translation terms in the sorted list) and the segments of
 // kSegTerms dense terms they split into.
 template <int K>
-__global__ __launch_bounds__(kRowThreads) void k_row_bitmap(SolveDesc d, uint32_t *__restrict__ bitmap,
-                                                            uint32_t *__restrict__ row_len,
-                                                            uint32_t *__restrict__ seg_cnt,
-                                                            uint32_t *__restrict__ meta)
+__global__ __launch_bounds__(kRowThreads) void k_row_bitmap(IndexGroup g, const uint32_t *__restrict__ wmap,
+                                                            IndexWs ws)
 {
     __shared__ uint32_t bits[kMaxWords];
     __shared__ uint32_t cnt[2];
-    const uint32_t i = blockIdx.x;
+    const int it = find_solve_r(wmap, g.n, blockIdx.x);
+    const IndexItem item = g.items[it];
+    const SolveDesc d = global_desc(g.sds[item.slot]);
+    const uint32_t i = blockIdx.x - wmap[it];
     const uint32_t words = (uint32_t(d.N) + 31) / 32;
     for (uint32_t w = threadIdx.x; w < words; w += blockDim.x)
         bits[w] = 0u;
@@ -70,9 +71,9 @@ __global__ __launch_bounds__(kRowThreads) void k_row_bitmap(SolveDesc d, uint32_
     const uint32_t beg = d.n2t_off[i], end = d.n2t_off[i + 1];
     const uint32_t P = d.P, E = d.E;
     uint32_t dense = 0;
-    for (uint32_t it = beg + threadIdx.x; it < end; it += blockDim.x)
+    for (uint32_t t_i = beg + threadIdx.x; t_i < end; t_i += blockDim.x)
     {
-        const uint32_t t = d.n2t_terms[it];
+        const uint32_t t = d.n2t_terms[t_i];
         if (t < P)
         {
             ++dense;
@@ -98,36 +99,41 @@ __global__ __launch_bounds__(kRowThreads) void k_row_bitmap(SolveDesc d, uint32_
         bits[i >> 5] &= ~(1u << (i & 31));
     __syncthreads();
     uint32_t pc = 0;
+    uint32_t *__restrict__ row_bits = ws.bitmap + size_t(item.node0 + i) * kMaxWords;
     for (uint32_t w = threadIdx.x; w < words; w += blockDim.x)
     {
         pc += __popc(bits[w]);
-        bitmap[size_t(i) * kMaxWords + w] = bits[w];
+        row_bits[w] = bits[w];
     }
     atomicAdd(&cnt[1], pc);
     __syncthreads();
     if (threadIdx.x == 0)
     {
-        row_len[i] = cnt[1];
+        ws.rowlen[item.node0 + i] = cnt[1];
         d.dense_cnt[i] = cnt[0];
-        seg_cnt[i] = (cnt[0] + kSegTerms - 1) / kSegTerms;
-        atomicMax(&meta[0], cnt[1]);
+        ws.segcnt[item.node0 + i] = (cnt[0] + kSegTerms - 1) / kSegTerms;
+        atomicMax(&ws.meta[size_t(it) * 16], cnt[1]);
     }
 }
 
 // Expand row i's bitmap into ascending block columns: diagonal block first,
 // then the neighbours (the order of the reference's sorted symmetric keys).
-__global__ __launch_bounds__(kRowThreads) void k_row_cols(SolveDesc d, const uint32_t *__restrict__ bitmap)
+__global__ __launch_bounds__(kRowThreads) void k_row_cols(IndexGroup g, const uint32_t *__restrict__ wmap, IndexWs ws)
 {
     __shared__ uint32_t pre[kMaxWords];
-    const uint32_t i = blockIdx.x;
+    const int it = find_solve_r(wmap, g.n, blockIdx.x);
+    const IndexItem item = g.items[it];
+    const SolveDesc d = global_desc(g.sds[item.slot]);
+    const uint32_t i = blockIdx.x - wmap[it];
     const uint32_t words = (uint32_t(d.N) + 31) / 32;
     const uint32_t base = d.row_ptr[i] + i;
+    const uint32_t *__restrict__ row_bits = ws.bitmap + size_t(item.node0 + i) * kMaxWords;
     // exclusive prefix of the word popcounts (words <= 128, one pass by wave 0)
     if (threadIdx.x < 64)
     {
         const uint32_t w0 = 2 * threadIdx.x, w1 = w0 + 1;
-        const uint32_t b0 = (w0 < words) ? bitmap[size_t(i) * kMaxWords + w0] : 0u;
-        const uint32_t b1 = (w1 < words) ? bitmap[size_t(i) * kMaxWords + w1] : 0u;
+        const uint32_t b0 = (w0 < words) ? row_bits[w0] : 0u;
+        const uint32_t b1 = (w1 < words) ? row_bits[w1] : 0u;
         const uint32_t c0 = __popc(b0), c1 = __popc(b1);
         uint32_t v = c0 + c1;
         // inclusive wave scan
@@ -150,7 +156,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row_cols(SolveDesc d, const uin
         d.blk_col[base] = i;
     for (uint32_t w = threadIdx.x; w < words; w += blockDim.x)
     {
-        uint32_t b = bitmap[size_t(i) * kMaxWords + w];
+        uint32_t b = row_bits[w];
         uint32_t o = base + 1 + pre[w];
         while (b)
         {
@@ -161,13 +167,6 @@ __global__ __launch_bounds__(kRowThreads) void k_row_cols(SolveDesc d, const uin
             b &= b - 1;
         }
     }
-}
-
-// U = row_ptr[N] / 2 and the largest row into meta.
-__global__ void k_row_meta(SolveDesc d, uint32_t *__restrict__ meta)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0)
-        meta[1] = d.row_ptr[d.N] / 2;
 }
 
 // (a, b) of packed upper-triangle entry e of a symmetric 6x6 block.
@@ -491,20 +490,24 @@ __global__ __launch_bounds__(kRowThreads) void k_rows_combine(const SolveDesc *_
     }
 }
 
-hipError_t launch_row_bitmap(const SolveDesc &d, uint32_t *bitmap, uint32_t *row_len, uint32_t *seg_cnt,
-                             uint32_t *meta, hipStream_t st)
+hipError_t launch_row_bitmap(int K, const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const IndexWs &ws,
+                             hipStream_t st)
 {
-    if (d.K == 8)
-        hipLaunchKernelGGL(k_row_bitmap<8>, dim3(d.N), dim3(kRowThreads), 0, st, d, bitmap, row_len, seg_cnt, meta);
+    if (n_wg == 0)
+        return hipSuccess;
+    if (K == 8)
+        hipLaunchKernelGGL(k_row_bitmap<8>, dim3(n_wg), dim3(kRowThreads), 0, st, g, wmap, ws);
     else
-        hipLaunchKernelGGL(k_row_bitmap<4>, dim3(d.N), dim3(kRowThreads), 0, st, d, bitmap, row_len, seg_cnt, meta);
+        hipLaunchKernelGGL(k_row_bitmap<4>, dim3(n_wg), dim3(kRowThreads), 0, st, g, wmap, ws);
     return hipGetLastError();
 }
 
-hipError_t launch_row_cols(const SolveDesc &d, const uint32_t *bitmap, uint32_t *meta, hipStream_t st)
+hipError_t launch_row_cols(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const IndexWs &ws,
+                           hipStream_t st)
 {
-    hipLaunchKernelGGL(k_row_cols, dim3(d.N), dim3(kRowThreads), 0, st, d, bitmap);
-    hipLaunchKernelGGL(k_row_meta, dim3(1), dim3(64), 0, st, d, meta);
+    if (n_wg == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_row_cols, dim3(n_wg), dim3(kRowThreads), 0, st, g, wmap, ws);
     return hipGetLastError();
 }
 

@@ -132,27 +132,26 @@ static int hip_status(hipError_t e)
     return SH_OK;
 }
 
-// Requests one admission round takes per index lane, at most: the lanes take
-// the next solve of the round as each finishes its last.  Under a backlog the
-// rounds are larger, so more solves share each GN step (measured on the
-// 64-stream mix: 1,900 solves/s at one per lane, 2,100 at two, 2,120 at four);
-// under a light load a round holds what is waiting.
-static constexpr uint32_t kAdmitPerLane = 4;
+// Requests one admission round takes per index-build unit (one solve of the
+// largest frame in an index group), at most: the round's index builds run as
+// few groups.  Under a backlog the rounds are larger, so more solves share
+// each GN step; under a light load a round holds what is waiting.
+static constexpr uint32_t kAdmitPerUnit = 4;
 
 void sh_server::admit_loop()
 {
     std::unique_lock<std::mutex> lk(mu);
-    const uint32_t lanes = std::max(1u, engine->num_lanes());
+    const uint32_t units = std::max(1u, engine->index_units());
     std::vector<uint64_t> picked;
     std::vector<uint32_t> pslots;
     std::vector<int> rcs;
     for (;;)
     {
         // the oldest waiting requests whose streams have nothing in flight, up
-        // to kAdmitPerLane per index lane and one per free slot (frames of one
+        // to kAdmitPerUnit per index unit and one per free slot (frames of one
         // stream stay in order)
         picked.clear();
-        for (auto it = waiting.begin(); it != waiting.end() && picked.size() < kAdmitPerLane * lanes &&
+        for (auto it = waiting.begin(); it != waiting.end() && picked.size() < kAdmitPerUnit * units &&
                                         picked.size() < free_slots.size();)
         {
             StreamState &ss = streams[reqs[*it].stream];

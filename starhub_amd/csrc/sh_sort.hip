@@ -1,4 +1,6 @@
-// Device-wide stable radix sorts and scans for the once-per-solve index build.
+// Device-wide stable radix sorts and scans for the index build (one over every
+// solve of a group: each solve's keys carry its own base, so its entries sort
+// into one contiguous run in the same relative order as alone).
 // rocPRIM's LSD radix sort is stable, which is exactly the semantics the
 // reference's CUB DeviceRadixSort::SortPairs calls rely on
 // (libs/star/common/algorithm_types.cuh:26-124), so the sorted node->term and
@@ -11,12 +13,6 @@
 
 namespace sh
 {
-hipError_t sort_pairs_u16_u32(void *temp, size_t &temp_bytes, const uint16_t *kin, uint16_t *kout,
-                              const uint32_t *vin, uint32_t *vout, uint32_t n, int end_bit, hipStream_t st)
-{
-    return rocprim::radix_sort_pairs(temp, temp_bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), st);
-}
-
 // Merge-sort limit 0: 32-bit keys always take the onesweep LSD path (stable;
 // for a few hundred thousand 15..24-bit keys it is a fraction of the
 // block-sort + merge-pass path rocPRIM picks below 1M items by default).

@@ -178,20 +178,21 @@ __device__ __forceinline__ void group_min_max(uint32_t &lo, uint32_t &hi, uint32
 // (pixel | ka << 8 | kb << 11) in sorted order, each slot's block and first
 // contribution, and the tile's slot and contribution counts.
 template <int K, int TP>
-__global__ __launch_bounds__(2 * TP) void k_tile_build(SolveDesc d, uint16_t *__restrict__ ws_contrib,
-                                                             uint32_t *__restrict__ ws_slot_block,
-                                                             uint16_t *__restrict__ ws_slot_first,
-                                                             uint32_t *__restrict__ tile_nslot,
-                                                             uint32_t *__restrict__ tile_ncontrib)
+__global__ __launch_bounds__(2 * TP) void k_tile_build(IndexGroup g, const uint32_t *__restrict__ wmap, IndexWs ws)
 {
     using G = TileGeom<K, TP>;
     using Sort = rocprim::block_radix_sort<uint32_t, G::kThreads, G::kItems, uint8_t>;
     __shared__ typename Sort::storage_type sort_storage;
     __shared__ uint32_t last_key[G::kThreads];
     __shared__ uint32_t red[16];
-    const uint32_t t = blockIdx.x, tid = threadIdx.x;
+    const int it = find_solve_t(wmap, g.n, blockIdx.x);
+    const IndexItem item = g.items[it];
+    const SolveDesc d = global_desc(g.sds[item.slot]);
+    const uint32_t t = blockIdx.x - wmap[it], tid = threadIdx.x;
     const uint32_t pl = tid >> 1, half = tid & 1;
     const uint32_t p = t * TP + pl;
+    uint32_t *__restrict__ tile_nslot = ws.tnslot + item.tc0;
+    uint32_t *__restrict__ tile_ncontrib = ws.tncontrib + item.tc0;
     // a row structure that overflowed its capacity is rejected by the host
     // after this launch; read none of it
     if (d.row_ptr[d.N] + uint32_t(d.N) > d.blk_cap)
@@ -303,7 +304,8 @@ __global__ __launch_bounds__(2 * TP) void k_tile_build(SolveDesc d, uint16_t *__
     uint32_t slot = group_exclusive_scan<G::kThreads>(nhead, red, &total_heads);
     (void)group_exclusive_scan<G::kThreads>(nvalid, red, &total_valid);
     const size_t stride = size_t(G::kEnt);
-    uint16_t *__restrict__ cout = ws_contrib + size_t(t) * stride;
+    const size_t tbase = size_t(item.t0 + t) * stride;
+    uint16_t *__restrict__ cout = ws.tcontrib + tbase;
     prev = prev0;
 #pragma unroll
     for (int i = 0; i < G::kItems; ++i)
@@ -315,8 +317,8 @@ __global__ __launch_bounds__(2 * TP) void k_tile_build(SolveDesc d, uint16_t *__
             cout[gi] = uint16_t((keys[i] & kPixMask) | (uint32_t(vals[i]) << kPixBits));
         if (head)
         {
-            ws_slot_block[size_t(t) * stride + slot] = (keys[i] >> kPixBits) + bmin;
-            ws_slot_first[size_t(t) * stride + slot] = uint16_t(gi);
+            ws.tslot_block[tbase + slot] = (keys[i] >> kPixBits) + bmin;
+            ws.tslot_first[tbase + slot] = uint16_t(gi);
             ++slot;
         }
         prev = keys[i];
@@ -325,7 +327,7 @@ __global__ __launch_bounds__(2 * TP) void k_tile_build(SolveDesc d, uint16_t *__
     {
         tile_nslot[t] = total_heads;
         tile_ncontrib[t] = total_valid; // valid items lead after the sort
-        atomicMax(tile_nslot + gridDim.x + 1, total_heads); // largest slot count of the solve's tiles
+        atomicMax(tile_nslot + uint32_t(d.n_tiles) + 1, total_heads); // largest slot count of the solve's tiles
     }
 }
 
@@ -333,34 +335,37 @@ __global__ __launch_bounds__(2 * TP) void k_tile_build(SolveDesc d, uint16_t *__
 // (bases from the scans of the counts); slot_first becomes absolute, and
 // slot_block / slot id pairs are laid out for the block sort.
 template <int K, int TP>
-__global__ __launch_bounds__(kTileThreads) void k_tile_compact(SolveDesc d, const uint16_t *__restrict__ ws_contrib,
-                                                               const uint32_t *__restrict__ ws_slot_block,
-                                                               const uint16_t *__restrict__ ws_slot_first,
-                                                               uint32_t *__restrict__ sort_keys,
+__global__ __launch_bounds__(kTileThreads) void k_tile_compact(IndexGroup g, const uint32_t *__restrict__ wmap,
+                                                               IndexWs ws, uint32_t *__restrict__ sort_keys,
                                                                uint32_t *__restrict__ sort_vals)
 {
     using G = TileGeom<K, TP>;
-    const uint32_t t = blockIdx.x, tid = threadIdx.x;
-    const size_t stride = size_t(G::kEnt);
+    const int it = find_solve_t(wmap, g.n, blockIdx.x);
+    const IndexItem item = g.items[it];
+    const SolveDesc d = global_desc(g.sds[item.slot]);
+    const uint32_t t = blockIdx.x - wmap[it], tid = threadIdx.x;
+    const size_t tbase = size_t(item.t0 + t) * size_t(G::kEnt);
     const uint32_t c0 = d.tile_cbase[t], nc = d.tile_cbase[t + 1] - c0;
     const uint32_t s0 = d.tile_sbase[t], ns = d.tile_sbase[t + 1] - s0;
     for (uint32_t i = tid; i < nc; i += kTileThreads)
-        d.tc_contrib[c0 + i] = ws_contrib[size_t(t) * stride + i];
+        d.tc_contrib[c0 + i] = ws.tcontrib[tbase + i];
     for (uint32_t s = tid; s < ns; s += kTileThreads)
     {
-        d.tc_slot_first[s0 + s] = c0 + ws_slot_first[size_t(t) * stride + s];
-        sort_keys[s0 + s] = ws_slot_block[size_t(t) * stride + s];
-        sort_vals[s0 + s] = s0 + s;
+        d.tc_slot_first[s0 + s] = c0 + ws.tslot_first[tbase + s];
+        sort_keys[item.sb0 + s0 + s] = item.blk0 + ws.tslot_block[tbase + s];
+        sort_vals[item.sb0 + s0 + s] = s0 + s;
     }
-    if (t + 1 == gridDim.x && tid == 0)
+    if (t + 1 == uint32_t(d.n_tiles) && tid == 0)
         d.tc_slot_first[d.tile_sbase[t + 1]] = d.tile_cbase[t + 1];
 }
 
 // For every block of row i: the block holding the same dense values on the
 // upper side ((i, c) itself for c >= i, (c, i) for c < i).
-__global__ __launch_bounds__(kTileThreads) void k_block_transpose(SolveDesc d)
+__global__ __launch_bounds__(kTileThreads) void k_block_transpose(IndexGroup g, const uint32_t *__restrict__ wmap)
 {
-    const uint32_t i = blockIdx.x;
+    const int it = find_solve_t(wmap, g.n, blockIdx.x);
+    const SolveDesc d = global_desc(g.sds[g.items[it].slot]);
+    const uint32_t i = blockIdx.x - wmap[it];
     const uint32_t base = d.row_ptr[i] + i, L1 = d.row_ptr[i + 1] - d.row_ptr[i] + 1;
     for (uint32_t c = threadIdx.x; c < L1; c += kTileThreads)
     {
@@ -1635,44 +1640,41 @@ static hipError_t with_tile_pix(uint32_t tp, F &&f)
     return hipGetLastError();
 }
 
-hipError_t launch_tile_build(const SolveDesc &d, uint16_t *ws_contrib, uint32_t *ws_slot_block,
-                             uint16_t *ws_slot_first, uint32_t *tile_nslot, uint32_t *tile_ncontrib, hipStream_t st)
+hipError_t launch_tile_build(int K, uint32_t tile_pix, const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg,
+                             const IndexWs &ws, hipStream_t st)
 {
-    if (d.n_tiles == 0)
+    if (n_wg == 0)
         return hipSuccess;
-    return with_tile_pix(uint32_t(d.tile_pix), [&](auto tp) {
+    return with_tile_pix(tile_pix, [&](auto tp) {
         constexpr int TP = decltype(tp)::value;
-        if (d.K == 8)
-            hipLaunchKernelGGL((k_tile_build<8, TP>), dim3(d.n_tiles), dim3(2 * TP), 0, st, d, ws_contrib,
-                               ws_slot_block, ws_slot_first, tile_nslot, tile_ncontrib);
+        if (K == 8)
+            hipLaunchKernelGGL((k_tile_build<8, TP>), dim3(n_wg), dim3(2 * TP), 0, st, g, wmap, ws);
         else
-            hipLaunchKernelGGL((k_tile_build<4, TP>), dim3(d.n_tiles), dim3(2 * TP), 0, st, d, ws_contrib,
-                               ws_slot_block, ws_slot_first, tile_nslot, tile_ncontrib);
+            hipLaunchKernelGGL((k_tile_build<4, TP>), dim3(n_wg), dim3(2 * TP), 0, st, g, wmap, ws);
     });
 }
 
-hipError_t launch_tile_compact(const SolveDesc &d, const uint16_t *ws_contrib, const uint32_t *ws_slot_block,
-                               const uint16_t *ws_slot_first, uint32_t *sort_keys, uint32_t *sort_vals,
-                               hipStream_t st)
+hipError_t launch_tile_compact(int K, uint32_t tile_pix, const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg,
+                               const IndexWs &ws, uint32_t *sort_keys, uint32_t *sort_vals, hipStream_t st)
 {
-    if (d.n_tiles == 0)
+    if (n_wg == 0)
         return hipSuccess;
-    return with_tile_pix(uint32_t(d.tile_pix), [&](auto tp) {
+    return with_tile_pix(tile_pix, [&](auto tp) {
         constexpr int TP = decltype(tp)::value;
-        if (d.K == 8)
-            hipLaunchKernelGGL((k_tile_compact<8, TP>), dim3(d.n_tiles), dim3(kTileThreads), 0, st, d, ws_contrib,
-                               ws_slot_block, ws_slot_first, sort_keys, sort_vals);
+        if (K == 8)
+            hipLaunchKernelGGL((k_tile_compact<8, TP>), dim3(n_wg), dim3(kTileThreads), 0, st, g, wmap, ws,
+                               sort_keys, sort_vals);
         else
-            hipLaunchKernelGGL((k_tile_compact<4, TP>), dim3(d.n_tiles), dim3(kTileThreads), 0, st, d, ws_contrib,
-                               ws_slot_block, ws_slot_first, sort_keys, sort_vals);
+            hipLaunchKernelGGL((k_tile_compact<4, TP>), dim3(n_wg), dim3(kTileThreads), 0, st, g, wmap, ws,
+                               sort_keys, sort_vals);
     });
 }
 
-hipError_t launch_block_transpose(const SolveDesc &d, hipStream_t st)
+hipError_t launch_block_transpose(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, hipStream_t st)
 {
-    if (d.N == 0)
+    if (n_wg == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_block_transpose, dim3(d.N), dim3(kTileThreads), 0, st, d);
+    hipLaunchKernelGGL(k_block_transpose, dim3(n_wg), dim3(kTileThreads), 0, st, g, wmap);
     return hipGetLastError();
 }
 
@@ -1755,40 +1757,49 @@ size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len, bool wave)
            sizeof(uint32_t) * (3 * L1 + 1) + sizeof(uint16_t) * ((max_nodes + 1) & ~1u);
 }
 
-// After the slot sort by block: slot_pos[list_items[l]] = l for l < n (each
-// slot's position in the block lists, where k_tiles puts its partial), and
-// list_off[b] = the first position of block b for b < n_blocks (a lower bound
-// in the sorted blocks).
-__global__ void k_slot_lists(const uint32_t *__restrict__ list_items, const uint32_t *__restrict__ sorted_blocks,
-                             uint32_t n, uint32_t n_blocks, uint32_t *__restrict__ slot_pos,
-                             uint32_t *__restrict__ list_off)
+// After the group's block sort (keys blk0 + block, values the solve's slot
+// ids, stable: tile order within a block): list_items[l] and
+// slot_pos[list_items[l]] = l for l < S (each slot's position in the block
+// lists, where k_tiles puts its partial), and list_off[b] = the first
+// position of block b for b <= N + 2U (a lower bound in the sorted blocks).
+__global__ __launch_bounds__(256) void k_slot_lists(IndexGroup g, const uint32_t *__restrict__ wmap,
+                                                    const uint32_t *__restrict__ skeys,
+                                                    const uint32_t *__restrict__ svals)
 {
-    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l < n)
-        slot_pos[list_items[l]] = l;
+    const int it = find_solve_t(wmap, g.n, blockIdx.x);
+    const IndexItem item = g.items[it];
+    const SolveDesc d = global_desc(g.sds[item.slot]);
+    const uint32_t S = d.tile_sbase[d.n_tiles];
+    const uint32_t n_blocks = uint32_t(d.N) + 2u * uint32_t(d.U) + 1u;
+    const uint32_t l = (blockIdx.x - wmap[it]) * blockDim.x + threadIdx.x;
+    if (l < S)
+    {
+        const uint32_t v = svals[item.sb0 + l];
+        d.list_items[l] = v;
+        d.slot_pos[v] = l;
+    }
     if (l < n_blocks)
     {
-        uint32_t lo = 0, hi = n;
+        const uint32_t key = item.blk0 + l;
+        uint32_t lo = item.sb0, hi = item.sb0 + S;
         while (lo < hi)
         {
             const uint32_t m = (lo + hi) >> 1;
-            if (sorted_blocks[m] < l)
+            if (skeys[m] < key)
                 lo = m + 1;
             else
                 hi = m;
         }
-        list_off[l] = lo;
+        d.list_off[l] = lo - item.sb0;
     }
 }
 
-hipError_t launch_slot_lists(const uint32_t *list_items, const uint32_t *sorted_blocks, uint32_t n,
-                             uint32_t n_blocks, uint32_t *slot_pos, uint32_t *list_off, hipStream_t st)
+hipError_t launch_slot_lists(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const uint32_t *skeys,
+                             const uint32_t *svals, hipStream_t st)
 {
-    const uint32_t m = max(n, n_blocks);
-    if (m == 0)
+    if (n_wg == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_slot_lists, dim3((m + 255) / 256), dim3(256), 0, st, list_items, sorted_blocks, n, n_blocks,
-                       slot_pos, list_off);
+    hipLaunchKernelGGL(k_slot_lists, dim3(n_wg), dim3(256), 0, st, g, wmap, skeys, svals);
     return hipGetLastError();
 }
 
