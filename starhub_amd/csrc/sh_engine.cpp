@@ -356,6 +356,7 @@ int Engine::alloc_group_ws(uint32_t units, uint32_t max_items)
     A(g.ws.errs, uint32_t, max_items);
     A(g.ws.meta, uint32_t, size_t(max_items) * 16);
     A(g.ws.bitmap, uint32_t, g.cap_nodes * kBitmapWords);
+    A(g.ws.bitpre, uint2, g.cap_nodes * kBitmapWords);
     A(g.ws.rowlen, uint32_t, g.cap_nodes);
     A(g.ws.segcnt, uint32_t, g.cap_nodes);
     A(g.ws.tcontrib, uint16_t, g.cap_tent);
@@ -670,9 +671,10 @@ int Engine::build_group(const uint32_t *list, uint32_t n, hipStream_t st, int *r
     SH_HIP(launch_n2t_split(gv, group_map(kGmSplit), wg_split, g.kb, g.vb, st));
     SH_HIP(launch_row_bitmap(K_, gv, group_map(kGmNode), wg_node, g.ws, st));
     SH_HIP(launch_scan_jobs(jobs, uint32_t(g_jobs_[0].size()), st));
-    if (!ref)
-        SH_HIP(launch_row_cols(gv, group_map(kGmNode), wg_node, g.ws, st));
-    else if ((rc = reference_pair_index(slots_[g_slots_[0]], st)) != SH_OK)
+    // the word prefix counts of the row bitmaps, and the block columns (the
+    // reference index builds them from its pair index)
+    SH_HIP(launch_row_cols(gv, group_map(kGmNode), wg_node, g.ws, !ref, st));
+    if (ref && (rc = reference_pair_index(slots_[g_slots_[0]], st)) != SH_OK)
         return keep(0, rc) ? SH_OK : rc;
     // the row structure is complete here (blk_col bounded by blk_cap)
     SH_HIP(launch_tile_build(K_, TP, gv, group_map(kGmTile), wg_tile, g.ws, st));
@@ -751,7 +753,7 @@ int Engine::build_group(const uint32_t *list, uint32_t n, hipStream_t st, int *r
         SH_HIP(sort_pairs_u32_u32(g.sort_temp, tb, g.ka, g.kb, g.va, g.vb, uint32_t(sb0), key_bits(blk0), st));
     }
     SH_HIP(launch_slot_lists(gv, group_map(kGmLists), wg_lists, g.kb, g.vb, st));
-    SH_HIP(launch_block_transpose(gv, group_map(kGmNode), wg_node3, st));
+    SH_HIP(launch_block_transpose(gv, group_map(kGmNode), wg_node3, g.ws, st));
     for (uint32_t s : g_slots_)
         slots_[s].indexed = true;
     return SH_OK;

@@ -48,6 +48,7 @@ struct IndexWs
     uint32_t *errs;                // one range-check word per item
     uint32_t *meta;                // 16 words per item: [0] longest row
     uint32_t *bitmap;              // kBitmapWords per node key
+    uint2 *bitpre;                 // the same words with the neighbour count of the words before each
     uint32_t *rowlen, *segcnt;     // per node key
     uint16_t *tcontrib;            // per tile, at its fixed stride: sorted contributions
     uint32_t *tslot_block;         // per tile: each slot's block
@@ -131,8 +132,9 @@ hipError_t launch_node_graph(uint32_t K, const float4 *nodes, uint32_t N, float 
 // segment count; the longest row into the item's meta[0].
 hipError_t launch_row_bitmap(int K, const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const IndexWs &ws,
                              hipStream_t st);
-// One work-group per node: the block columns from the bitmap and row_ptr.
-hipError_t launch_row_cols(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const IndexWs &ws,
+// One work-group per node: the bitmap words with their word prefix counts
+// (bitpre) and, with cols, the block columns from them and row_ptr.
+hipError_t launch_row_cols(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const IndexWs &ws, bool cols,
                            hipStream_t st);
 // Dense terms per segment of a node row (one work-group of the segment kernel).
 constexpr uint32_t kSegTerms = 256;
@@ -244,7 +246,8 @@ hipError_t launch_tile_build(int K, uint32_t tile_pix, const IndexGroup &g, cons
 hipError_t launch_tile_compact(int K, uint32_t tile_pix, const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg,
                                const IndexWs &ws, uint32_t *sort_keys, uint32_t *sort_vals, hipStream_t st);
 // One work-group per node: the transpose map of the row's blocks.
-hipError_t launch_block_transpose(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, hipStream_t st);
+hipError_t launch_block_transpose(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const IndexWs &ws,
+                                  hipStream_t st);
 // fused: the tile kernel builds the FP32 per-pixel records itself (and the
 // dense cost partials) from the warp field of GN iteration offset parity,
 // instead of reading them; the term kernel then runs the ARAP part only.

@@ -71,25 +71,60 @@ __global__ __launch_bounds__(kRowThreads) void k_row_bitmap(IndexGroup g, const 
     const uint32_t beg = d.n2t_off[i], end = d.n2t_off[i + 1];
     const uint32_t P = d.P, E = d.E;
     uint32_t dense = 0;
-    for (uint32_t t_i = beg + threadIdx.x; t_i < end; t_i += blockDim.x)
+    // kU terms per thread and round: their term ids, then their node lists,
+    // are loaded before any is used
+    constexpr int kU = 4;
+    for (uint32_t t0 = beg + threadIdx.x; t0 < end; t0 += kU * blockDim.x)
     {
-        const uint32_t t = d.n2t_terms[t_i];
-        if (t < P)
-        {
-            ++dense;
+        uint32_t tt[kU];
 #pragma unroll
-            for (int k = 0; k < K; ++k)
+        for (int u = 0; u < kU; ++u)
+        {
+            const uint32_t t_i = t0 + u * blockDim.x;
+            tt[u] = t_i < end ? d.n2t_terms[t_i] : 0xffffffffu;
+        }
+        uint32_t nw[kU][K / 2]; // the node ids of a dense term, two per word
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+        {
+            if (tt[u] < P)
             {
-                const uint32_t n = d.pknn[size_t(t) * K + k];
-                atomicOr(&bits[n >> 5], 1u << (n & 31));
+                if constexpr (K == 8)
+                {
+                    const uint4 v = *reinterpret_cast<const uint4 *>(d.pknn + size_t(tt[u]) * 8);
+                    nw[u][0] = v.x, nw[u][1] = v.y, nw[u][2] = v.z, nw[u][3] = v.w;
+                }
+                else
+                {
+                    const uint2 v = *reinterpret_cast<const uint2 *>(d.pknn + size_t(tt[u]) * 4);
+                    nw[u][0] = v.x, nw[u][1] = v.y;
+                }
+            }
+            else if (tt[u] < P + E)
+            {
+                const uint32_t e = tt[u] - P;
+                nw[u][0] = uint32_t(d.edges[3 * e]) | (uint32_t(d.edges[3 * e + 1]) << 16);
             }
         }
-        else if (t < P + E)
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
         {
-            const uint32_t e = t - P;
-            const uint32_t a = d.edges[3 * e], b = d.edges[3 * e + 1];
-            atomicOr(&bits[a >> 5], 1u << (a & 31));
-            atomicOr(&bits[b >> 5], 1u << (b & 31));
+            if (tt[u] < P)
+            {
+                ++dense;
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                {
+                    const uint32_t n = (nw[u][k >> 1] >> (16 * (k & 1))) & 0xffffu;
+                    atomicOr(&bits[n >> 5], 1u << (n & 31));
+                }
+            }
+            else if (tt[u] < P + E)
+            {
+                const uint32_t a = nw[u][0] & 0xffffu, b = nw[u][0] >> 16;
+                atomicOr(&bits[a >> 5], 1u << (a & 31));
+                atomicOr(&bits[b >> 5], 1u << (b & 31));
+            }
         }
     }
     // integer sums: order independent
@@ -116,9 +151,12 @@ __global__ __launch_bounds__(kRowThreads) void k_row_bitmap(IndexGroup g, const 
     }
 }
 
-// Expand row i's bitmap into ascending block columns: diagonal block first,
-// then the neighbours (the order of the reference's sorted symmetric keys).
-__global__ __launch_bounds__(kRowThreads) void k_row_cols(IndexGroup g, const uint32_t *__restrict__ wmap, IndexWs ws)
+// Row i's bitmap words with the neighbour count of the words before each
+// (bitpre, for the block lookups of the tile build and the transpose map)
+// and, with cols, its ascending block columns: diagonal block first, then
+// the neighbours (the order of the reference's sorted symmetric keys).
+__global__ __launch_bounds__(kRowThreads) void k_row_cols(IndexGroup g, const uint32_t *__restrict__ wmap, IndexWs ws,
+                                                          int cols)
 {
     __shared__ uint32_t pre[kMaxWords];
     const int it = find_solve_r(wmap, g.n, blockIdx.x);
@@ -126,8 +164,8 @@ __global__ __launch_bounds__(kRowThreads) void k_row_cols(IndexGroup g, const ui
     const SolveDesc d = global_desc(g.sds[item.slot]);
     const uint32_t i = blockIdx.x - wmap[it];
     const uint32_t words = (uint32_t(d.N) + 31) / 32;
-    const uint32_t base = d.row_ptr[i] + i;
     const uint32_t *__restrict__ row_bits = ws.bitmap + size_t(item.node0 + i) * kMaxWords;
+    uint2 *__restrict__ row_pre = ws.bitpre + size_t(item.node0 + i) * kMaxWords;
     // exclusive prefix of the word popcounts (words <= 128, one pass by wave 0)
     if (threadIdx.x < 64)
     {
@@ -150,8 +188,13 @@ __global__ __launch_bounds__(kRowThreads) void k_row_cols(IndexGroup g, const ui
             pre[w1] = ex + c0;
     }
     __syncthreads();
+    for (uint32_t w = threadIdx.x; w < words; w += blockDim.x)
+        row_pre[w] = make_uint2(row_bits[w], pre[w]);
+    if (!cols)
+        return; // uniform
     // rows beyond the block capacity are left unwritten; the host rejects
     // the solve (capacity) after reading U back
+    const uint32_t base = d.row_ptr[i] + i;
     if (threadIdx.x == 0 && base < d.blk_cap)
         d.blk_col[base] = i;
     for (uint32_t w = threadIdx.x; w < words; w += blockDim.x)
@@ -502,12 +545,12 @@ hipError_t launch_row_bitmap(int K, const IndexGroup &g, const uint32_t *wmap, u
     return hipGetLastError();
 }
 
-hipError_t launch_row_cols(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const IndexWs &ws,
+hipError_t launch_row_cols(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const IndexWs &ws, bool cols,
                            hipStream_t st)
 {
     if (n_wg == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_row_cols, dim3(n_wg), dim3(kRowThreads), 0, st, g, wmap, ws);
+    hipLaunchKernelGGL(k_row_cols, dim3(n_wg), dim3(kRowThreads), 0, st, g, wmap, ws, cols ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -79,41 +79,45 @@ __device__ __forceinline__ int find_solve_t(const uint32_t *__restrict__ begin, 
     return lo;
 }
 
-// Full-block index of (lo, hi), lo <= hi: the diagonal block of row lo or the
-// position of hi among row lo's ascending neighbours.
-__device__ __forceinline__ uint32_t block_of_pair(const SolveDesc &d, uint32_t lo, uint32_t hi)
+// Full-block index of (lo, hi): the diagonal block of row lo for lo == hi,
+// else the position of hi among row lo's ascending neighbours, base + 1 + the
+// neighbours below hi, counted from row lo's bitmap word that holds hi and
+// the neighbour count of the words before it (bitpre: the rows of the
+// solve, kBitmapWords words each).  Two independent loads.
+__device__ __forceinline__ uint32_t block_of_pair(const uint32_t *__restrict__ row_ptr,
+                                                  const uint2 *__restrict__ bitpre, uint32_t lo, uint32_t hi)
 {
-    const uint32_t base = d.row_ptr[lo] + lo;
-    if (lo == hi)
-        return base;
-    uint32_t a = base + 1, b = d.row_ptr[lo + 1] + lo + 1; // [a, b) ascending
-    while (a < b)
-    {
-        const uint32_t m = (a + b) >> 1;
-        if (d.blk_col[m] < hi)
-            a = m + 1;
-        else
-            b = m;
-    }
-    return a;
+    const uint32_t base = row_ptr[lo] + lo;
+    const uint2 w = bitpre[size_t(lo) * kBitmapWords + (hi >> 5)];
+    return lo == hi ? base : base + 1u + w.y + uint32_t(__popc(w.x & ((1u << (hi & 31u)) - 1u)));
 }
 
-// (ka, kb) of pair q in 0 .. K(K+1)/2 - 1, ka <= kb, lexicographic.
+// (ka, kb) of pair q in 0 .. K(K+1)/2 - 1, ka <= kb, lexicographic; folds to
+// constants for a constant q.
 template <int K>
-__device__ __forceinline__ void slot_pair(int q, int &ka, int &kb)
+__host__ __device__ constexpr int pair_first(int q)
 {
     int a = 0, rem = q;
-#pragma unroll
     for (int r = 0; r < K; ++r)
-    {
         if (rem >= K - a)
         {
             rem -= K - a;
             ++a;
         }
-    }
-    ka = a;
-    kb = a + rem;
+    return a;
+}
+
+template <int K>
+__host__ __device__ constexpr int pair_second(int q)
+{
+    int a = 0, rem = q;
+    for (int r = 0; r < K; ++r)
+        if (rem >= K - a)
+        {
+            rem -= K - a;
+            ++a;
+        }
+    return a + rem;
 }
 
 // Exclusive scan of one value per thread over the work-group, in thread order;
@@ -231,32 +235,20 @@ __global__ __launch_bounds__(2 * TP) void k_tile_build(IndexGroup g, const uint3
                 if (kn[k2] == kn[k])
                     act[k] = false;
         }
+        const uint2 *__restrict__ bitpre = ws.bitpre + size_t(item.node0) * kBitmapWords;
 #pragma unroll
         for (int i = 0; i < G::kItems; ++i)
         {
-            int ka, kb;
-            slot_pair<K>(int(half) * G::kItems + i, ka, kb);
-            bool on_a = false, on_b = false;
-            uint32_t na = 0, nb = 0;
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-            {
-                if (k == ka)
-                {
-                    on_a = act[k];
-                    na = kn[k];
-                }
-                if (k == kb)
-                {
-                    on_b = act[k];
-                    nb = kn[k];
-                }
-            }
+            // this thread's half of the pairs: q = i or kItems + i
+            const int ka0 = pair_first<K>(i), kb0 = pair_second<K>(i);
+            const int ka1 = pair_first<K>(G::kItems + i), kb1 = pair_second<K>(G::kItems + i);
+            const bool on_a = half ? act[ka1] : act[ka0], on_b = half ? act[kb1] : act[kb0];
+            const uint32_t na = half ? kn[ka1] : kn[ka0], nb = half ? kn[kb1] : kn[kb0];
             if (on_a && on_b)
             {
                 const uint32_t lo = min(na, nb), hi = max(na, nb);
-                keys[i] = (block_of_pair(d, lo, hi) << kPixBits) | pl;
-                vals[i] = uint8_t(ka | (kb << 3));
+                keys[i] = (block_of_pair(d.row_ptr, bitpre, lo, hi) << kPixBits) | pl;
+                vals[i] = uint8_t(half ? (ka1 | (kb1 << 3)) : (ka0 | (kb0 << 3)));
             }
         }
     }
@@ -359,18 +351,20 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_compact(IndexGroup g, con
         d.tc_slot_first[d.tile_sbase[t + 1]] = d.tile_cbase[t + 1];
 }
 
-// For every block of row i: the block holding the same dense values on the
-// upper side ((i, c) itself for c >= i, (c, i) for c < i).
-__global__ __launch_bounds__(kTileThreads) void k_block_transpose(IndexGroup g, const uint32_t *__restrict__ wmap)
+// For every block (i, c) of row i: the index of its transpose (c, i), in row c.
+__global__ __launch_bounds__(kTileThreads) void k_block_transpose(IndexGroup g, const uint32_t *__restrict__ wmap,
+                                                                  IndexWs ws)
 {
     const int it = find_solve_t(wmap, g.n, blockIdx.x);
-    const SolveDesc d = global_desc(g.sds[g.items[it].slot]);
+    const IndexItem item = g.items[it];
+    const SolveDesc d = global_desc(g.sds[item.slot]);
+    const uint2 *__restrict__ bitpre = ws.bitpre + size_t(item.node0) * kBitmapWords;
     const uint32_t i = blockIdx.x - wmap[it];
     const uint32_t base = d.row_ptr[i] + i, L1 = d.row_ptr[i + 1] - d.row_ptr[i] + 1;
     for (uint32_t c = threadIdx.x; c < L1; c += kTileThreads)
     {
         const uint32_t col = d.blk_col[base + c];
-        d.blk_transpose[base + c] = col == i ? base + c : block_of_pair(d, col, i); // block (col, i) of row col
+        d.blk_transpose[base + c] = col == i ? base + c : block_of_pair(d.row_ptr, bitpre, col, i); // block (col, i) of row col
     }
 }
 
@@ -1670,11 +1664,12 @@ hipError_t launch_tile_compact(int K, uint32_t tile_pix, const IndexGroup &g, co
     });
 }
 
-hipError_t launch_block_transpose(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, hipStream_t st)
+hipError_t launch_block_transpose(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const IndexWs &ws,
+                                  hipStream_t st)
 {
     if (n_wg == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_block_transpose, dim3(n_wg), dim3(kTileThreads), 0, st, g, wmap);
+    hipLaunchKernelGGL(k_block_transpose, dim3(n_wg), dim3(kTileThreads), 0, st, g, wmap, ws);
     return hipGetLastError();
 }
 
