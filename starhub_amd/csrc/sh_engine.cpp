@@ -324,7 +324,8 @@ bool Engine::tile_order() const
 }
 
 // Index-build workspace for groups of up to `units` solves of the largest
-// frame (and at most max_items solves).
+// frame with every pixel valid (and at most max_items solves).
+static constexpr uint32_t kFlagUnits = 4;
 int Engine::alloc_group_ws(uint32_t units, uint32_t max_items)
 {
     GroupWs &g = gws_;
@@ -333,11 +334,15 @@ int Engine::alloc_group_ws(uint32_t units, uint32_t max_items)
     const uint64_t tent_unit = (uint64_t(pcap_) + kTilePixMax) * kp;
     // every position of the concatenated arrays stays below 2^31
     const uint64_t lim = (1ull << 31) - 1;
-    units = uint32_t(
-        std::max<uint64_t>(1, std::min<uint64_t>({units, lim / kv_unit, lim / tent_unit, lim / pcap_, lim / tcap_})));
+    units = uint32_t(std::max<uint64_t>(
+        1, std::min<uint64_t>({units, lim / kv_unit, lim / tent_unit, lim / (pcap_ * uint64_t(kFlagUnits)),
+                               lim / (tcap_ * uint64_t(kFlagUnits))})));
     group_units_ = units;
-    g.cap_m = uint64_t(units) * pcap_;
-    g.cap_h = uint64_t(units) * tcap_; // the tuning may switch the tile order on later
+    // the flag stage takes kFlagUnits frames per unit (16 bytes per map
+    // pixel); the row and tile stages' groups are then formed by the valid
+    // pixels, a fraction of the frame
+    g.cap_m = uint64_t(units) * kFlagUnits * pcap_;
+    g.cap_h = uint64_t(units) * kFlagUnits * tcap_; // the tuning may switch the tile order on later
     g.cap_kv = uint64_t(units) * kv_unit;
     g.cap_nodes = uint64_t(units) * (uint64_t(ncap_) + 1);
     g.cap_tcnt = uint64_t(units) * (uint64_t(tiles_cap_) + 2);
@@ -432,10 +437,10 @@ int Engine::upload_group(int stage, hipStream_t st)
     return SH_OK;
 }
 
-// The index builds of several slots, in groups: a group takes slots while the
-// sums of their upper bounds (frame pixels, node pairs, nodes, tiles, tile
-// contributions) fit the workspace; one solve of the largest frame always
-// fits.  With rcs, a slot whose inputs are rejected (SH_E_ARG /
+// The index builds of several slots, in groups: a group takes slots while
+// their frames fit the flag stage's workspace (the row and tile stages then
+// run over sub-groups sized by the valid pixels); one solve of the largest
+// frame always fits.  With rcs, a slot whose inputs are rejected (SH_E_ARG /
 // SH_E_CAPACITY / SH_E_NOMEM) records its status there and the others go on;
 // without, the first failure ends the build.
 int Engine::build_indexes(const uint32_t *list, uint32_t n, hipStream_t st, int *rcs)
@@ -444,26 +449,20 @@ int Engine::build_indexes(const uint32_t *list, uint32_t n, hipStream_t st, int 
         for (uint32_t i = 0; i < n; ++i)
             rcs[i] = SH_OK;
     const GroupWs &g = gws_;
-    const uint64_t TP = tile_pix(), kp = uint64_t(K_ * (K_ + 1) / 2);
     const bool tiled = tile_order(), ref = (cfg_.flags & SH_FLAG_REFERENCE_INDEX) != 0;
     uint32_t i = 0;
     while (i < n)
     {
-        uint64_t m = 0, h = 0, kv = 0, nodes = 0, tcnt = 0, tent = 0, sub = 0;
+        uint64_t m = 0, h = 0;
         uint32_t j = i;
         while (j < n && j - i < g.cap_items && !(ref && j > i))
         {
             const SolveDesc &d = slots_[list[j]].d;
-            const uint64_t px = uint64_t(d.W) * uint64_t(d.H), nt = (px + TP - 1) / TP;
-            const uint64_t m1 = m + px;
+            const uint64_t m1 = m + uint64_t(d.W) * uint64_t(d.H);
             const uint64_t h1 = h + (tiled ? tile_order_size(uint32_t(d.W), uint32_t(d.H)) : 0u);
-            const uint64_t kv1 = kv + px * uint64_t(K_) + 2 * uint64_t(d.E) + uint64_t(d.N);
-            const uint64_t n1 = nodes + uint64_t(d.N) + 1, tc1 = tcnt + nt + 2, te1 = tent + nt * TP * kp;
-            const uint64_t s1 = sub + std::min<uint64_t>(nt * TP * kp, pair_cap_);
-            if (j > i && (m1 > g.cap_m || h1 > g.cap_h || kv1 > g.cap_kv || n1 > g.cap_nodes || tc1 > g.cap_tcnt ||
-                          te1 > g.cap_tent || s1 > g.cap_kv))
+            if (j > i && (m1 > g.cap_m || h1 > g.cap_h))
                 break;
-            m = m1, h = h1, kv = kv1, nodes = n1, tcnt = tc1, tent = te1, sub = s1;
+            m = m1, h = h1;
             ++j;
         }
         if (int rc = build_group(list + i, j - i, st, rcs ? rcs + i : nullptr); rc != SH_OK)
@@ -486,7 +485,8 @@ static int key_bits(uint64_t n_keys)
 }
 
 // The index build of one group, in three stages with one host readback
-// between them:
+// between them (stages 2 and 3 once per sub-group of the slots that pass
+// stage 1, sized by their valid pixels):
 //  1. valid-pixel flags (raster and tile order) with the id range checks,
 //     their group-wide scans; the pixel counts and check flags to the host;
 //  2. compaction into each slot's pixel streams and the Node2Term pairs, one
@@ -619,143 +619,178 @@ int Engine::build_group(const uint32_t *list, uint32_t n, hipStream_t st, int *r
     if (g_items_.empty())
         return SH_OK;
 
-    // ---- stage 2
-    uint64_t kv0 = 0, node0 = 0, t0 = 0, tc0 = 0;
-    for (size_t i = 0; i < g_items_.size(); ++i)
-    {
-        SolveDesc &d = desc_of(i);
-        IndexItem &it = g_items_[i];
-        const uint32_t P = uint32_t(d.P), N = uint32_t(d.N);
-        d.n_tiles = int(div_up(P, TP));
-        d.n_pair_entries = 0;
-        it.kv0 = uint32_t(kv0);
-        it.node0 = uint32_t(node0);
-        it.t0 = uint32_t(t0);
-        it.tc0 = uint32_t(tc0);
-        kv0 += uint64_t(P) * uint64_t(K_) + 2 * uint64_t(d.E) + N;
-        node0 += uint64_t(N) + 1;
-        t0 += uint64_t(d.n_tiles);
-        tc0 += uint64_t(d.n_tiles) + 2;
-        g_jobs_[0].push_back(ScanJob{g.ws.segcnt + it.node0, d.seg_ptr, N + 1, 0});
-        if (!ref)
-            g_jobs_[0].push_back(ScanJob{g.ws.rowlen + it.node0, d.row_ptr, N + 1, 0});
-        const uint32_t nt = uint32_t(d.n_tiles);
-        g_jobs_[1].push_back(ScanJob{g.ws.tnslot + it.tc0, d.tile_sbase, nt + 1, 0});
-        g_jobs_[1].push_back(ScanJob{g.ws.tncontrib + it.tc0, d.tile_cbase, nt + 1, 0});
-    }
-    if (kv0 > g.cap_kv || node0 > g.cap_nodes || tc0 > g.cap_tcnt || t0 * TP * uint64_t(K_ * (K_ + 1) / 2) > g.cap_tent)
-        return SH_E_STATE; // never: build_indexes sized the group
-    const uint32_t wg_compact = set_map(kGmCompact, [&](size_t i) {
-        const SolveDesc &d = desc_of(i);
-        return std::max(1u, div_up(uint32_t(d.W) * uint32_t(d.H), 256u)) + div_up(uint32_t(d.E + d.N), 256u);
-    });
-    const uint32_t wg_split = set_map(kGmSplit, [&](size_t i) {
-        const SolveDesc &d = desc_of(i);
-        const uint32_t n_kv = uint32_t(d.P) * uint32_t(K_) + 2u * uint32_t(d.E) + uint32_t(d.N);
-        return div_up(std::max(n_kv, uint32_t(d.N) + 1), 256u);
-    });
-    const uint32_t wg_node = set_map(kGmNode, [&](size_t i) { return uint32_t(desc_of(i).N); });
-    const uint32_t wg_tile = set_map(kGmTile, [&](size_t i) { return uint32_t(desc_of(i).n_tiles); });
-    if ((rc = upload_group(1, st)) != SH_OK)
-        return rc;
-    gv = group_view();
-    const ScanJob *jobs = reinterpret_cast<const ScanJob *>(g.d_tab + g.off_jobs);
-    SH_HIP(launch_compact_pixels(K_, gv, group_map(kGmCompact), wg_compact, tiled, g.ws, g.ka, g.va, st));
-    // Node2TermsIndex::BuildIndex (Node2TermsIndex.cu:191-199,312-360): one
-    // stable sort of the group's (node key, term) pairs
-    if (kv0 > 0)
-    {
-        tb = g.sort_temp_bytes;
-        SH_HIP(sort_pairs_u32_u32(g.sort_temp, tb, g.ka, g.kb, g.va, g.vb, uint32_t(kv0), key_bits(node0), st));
-    }
-    SH_HIP(launch_n2t_split(gv, group_map(kGmSplit), wg_split, g.kb, g.vb, st));
-    SH_HIP(launch_row_bitmap(K_, gv, group_map(kGmNode), wg_node, g.ws, st));
-    SH_HIP(launch_scan_jobs(jobs, uint32_t(g_jobs_[0].size()), st));
-    // the word prefix counts of the row bitmaps, and the block columns (the
-    // reference index builds them from its pair index)
-    SH_HIP(launch_row_cols(gv, group_map(kGmNode), wg_node, g.ws, !ref, st));
-    if (ref && (rc = reference_pair_index(slots_[g_slots_[0]], st)) != SH_OK)
-        return keep(0, rc) ? SH_OK : rc;
-    // the row structure is complete here (blk_col bounded by blk_cap)
-    SH_HIP(launch_tile_build(K_, TP, gv, group_map(kGmTile), wg_tile, g.ws, st));
-    SH_HIP(launch_scan_jobs(jobs + 2 * C, uint32_t(g_jobs_[1].size()), st));
-    SH_HIP(launch_index_counts(gv, 2, g.ws, g.h_out, st));
-    if ((rc = wait_stream(st, index_ev_)) != SH_OK)
-        return rc;
-    std::vector<uint32_t> n_slots(g_items_.size());
-    {
-        std::vector<int> rc_of(g_items_.size(), SH_OK);
+    // stages 2 and 3 over sub-groups whose valid pixels fit the row and
+    // tile workspace
+    auto rows_tiles = [&]() -> int {
+        g_jobs_[0].clear();
+        g_jobs_[1].clear();
+        // ---- stage 2
+        uint64_t kv0 = 0, node0 = 0, t0 = 0, tc0 = 0;
         for (size_t i = 0; i < g_items_.size(); ++i)
         {
-            const uint32_t *o = g.h_out + i * 8;
-            Slot &sl = slots_[g_slots_[i]];
-            SolveDesc &d = sl.d;
-            int r = SH_OK;
-            if (o[2] > ucap_)
-                r = SH_E_CAPACITY;
-            else
-            {
-                d.U = int(o[2]);
-                if (!ref)
-                {
-                    sl.max_row_len = o[1];
-                    d.n_seg = int(o[0]);
-                    r = size_partials(sl);
-                }
-            }
-            if (r == SH_OK && uint64_t(o[3]) > pair_cap_)
-                r = SH_E_CAPACITY;
-            if (r == SH_OK)
-            {
-                sl.tile_slot_max = d.n_tiles > 0 ? o[5] : 0u;
-                r = size_tiles(sl, o[3], o[4]);
-            }
-            n_slots[i] = o[3];
-            rc_of[i] = r;
+            SolveDesc &d = desc_of(i);
+            IndexItem &it = g_items_[i];
+            const uint32_t P = uint32_t(d.P), N = uint32_t(d.N);
+            d.n_tiles = int(div_up(P, TP));
+            d.n_pair_entries = 0;
+            it.kv0 = uint32_t(kv0);
+            it.node0 = uint32_t(node0);
+            it.t0 = uint32_t(t0);
+            it.tc0 = uint32_t(tc0);
+            kv0 += uint64_t(P) * uint64_t(K_) + 2 * uint64_t(d.E) + N;
+            node0 += uint64_t(N) + 1;
+            t0 += uint64_t(d.n_tiles);
+            tc0 += uint64_t(d.n_tiles) + 2;
+            g_jobs_[0].push_back(ScanJob{g.ws.segcnt + it.node0, d.seg_ptr, N + 1, 0});
+            if (!ref)
+                g_jobs_[0].push_back(ScanJob{g.ws.rowlen + it.node0, d.row_ptr, N + 1, 0});
+            const uint32_t nt = uint32_t(d.n_tiles);
+            g_jobs_[1].push_back(ScanJob{g.ws.tnslot + it.tc0, d.tile_sbase, nt + 1, 0});
+            g_jobs_[1].push_back(ScanJob{g.ws.tncontrib + it.tc0, d.tile_cbase, nt + 1, 0});
         }
-        // n_slots follows the items
-        size_t k = 0;
-        for (size_t i = 0; i < rc_of.size(); ++i)
-            if (rc_of[i] == SH_OK)
-                n_slots[k++] = n_slots[i];
-        if ((rc = compact_items(rc_of)) != SH_OK)
+        if (kv0 > g.cap_kv || node0 > g.cap_nodes || tc0 > g.cap_tcnt || t0 * TP * uint64_t(K_ * (K_ + 1) / 2) > g.cap_tent)
+            return SH_E_STATE; // never: build_indexes sized the group
+        const uint32_t wg_compact = set_map(kGmCompact, [&](size_t i) {
+            const SolveDesc &d = desc_of(i);
+            return std::max(1u, div_up(uint32_t(d.W) * uint32_t(d.H), 256u)) + div_up(uint32_t(d.E + d.N), 256u);
+        });
+        const uint32_t wg_split = set_map(kGmSplit, [&](size_t i) {
+            const SolveDesc &d = desc_of(i);
+            const uint32_t n_kv = uint32_t(d.P) * uint32_t(K_) + 2u * uint32_t(d.E) + uint32_t(d.N);
+            return div_up(std::max(n_kv, uint32_t(d.N) + 1), 256u);
+        });
+        const uint32_t wg_node = set_map(kGmNode, [&](size_t i) { return uint32_t(desc_of(i).N); });
+        const uint32_t wg_tile = set_map(kGmTile, [&](size_t i) { return uint32_t(desc_of(i).n_tiles); });
+        if ((rc = upload_group(1, st)) != SH_OK)
             return rc;
-    }
-    if (g_items_.empty())
-        return SH_OK;
+        gv = group_view();
+        const ScanJob *jobs = reinterpret_cast<const ScanJob *>(g.d_tab + g.off_jobs);
+        SH_HIP(launch_compact_pixels(K_, gv, group_map(kGmCompact), wg_compact, tiled, g.ws, g.ka, g.va, st));
+        // Node2TermsIndex::BuildIndex (Node2TermsIndex.cu:191-199,312-360): one
+        // stable sort of the group's (node key, term) pairs
+        if (kv0 > 0)
+        {
+            tb = g.sort_temp_bytes;
+            SH_HIP(sort_pairs_u32_u32(g.sort_temp, tb, g.ka, g.kb, g.va, g.vb, uint32_t(kv0), key_bits(node0), st));
+        }
+        SH_HIP(launch_n2t_split(gv, group_map(kGmSplit), wg_split, g.kb, g.vb, st));
+        SH_HIP(launch_row_bitmap(K_, gv, group_map(kGmNode), wg_node, g.ws, st));
+        SH_HIP(launch_scan_jobs(jobs, uint32_t(g_jobs_[0].size()), st));
+        // the word prefix counts of the row bitmaps, and the block columns (the
+        // reference index builds them from its pair index)
+        SH_HIP(launch_row_cols(gv, group_map(kGmNode), wg_node, g.ws, !ref, st));
+        if (ref && (rc = reference_pair_index(slots_[g_slots_[0]], st)) != SH_OK)
+            return keep(0, rc) ? SH_OK : rc;
+        // the row structure is complete here (blk_col bounded by blk_cap)
+        SH_HIP(launch_tile_build(K_, TP, gv, group_map(kGmTile), wg_tile, g.ws, st));
+        SH_HIP(launch_scan_jobs(jobs + 2 * C, uint32_t(g_jobs_[1].size()), st));
+        SH_HIP(launch_index_counts(gv, 2, g.ws, g.h_out, st));
+        if ((rc = wait_stream(st, index_ev_)) != SH_OK)
+            return rc;
+        std::vector<uint32_t> n_slots(g_items_.size());
+        {
+            std::vector<int> rc_of(g_items_.size(), SH_OK);
+            for (size_t i = 0; i < g_items_.size(); ++i)
+            {
+                const uint32_t *o = g.h_out + i * 8;
+                Slot &sl = slots_[g_slots_[i]];
+                SolveDesc &d = sl.d;
+                int r = SH_OK;
+                if (o[2] > ucap_)
+                    r = SH_E_CAPACITY;
+                else
+                {
+                    d.U = int(o[2]);
+                    if (!ref)
+                    {
+                        sl.max_row_len = o[1];
+                        d.n_seg = int(o[0]);
+                        r = size_partials(sl);
+                    }
+                }
+                if (r == SH_OK && uint64_t(o[3]) > pair_cap_)
+                    r = SH_E_CAPACITY;
+                if (r == SH_OK)
+                {
+                    sl.tile_slot_max = d.n_tiles > 0 ? o[5] : 0u;
+                    r = size_tiles(sl, o[3], o[4]);
+                }
+                n_slots[i] = o[3];
+                rc_of[i] = r;
+            }
+            // n_slots follows the items
+            size_t k = 0;
+            for (size_t i = 0; i < rc_of.size(); ++i)
+                if (rc_of[i] == SH_OK)
+                    n_slots[k++] = n_slots[i];
+            if ((rc = compact_items(rc_of)) != SH_OK)
+                return rc;
+        }
+        if (g_items_.empty())
+            return SH_OK;
 
-    // ---- stage 3
-    uint64_t sb0 = 0, blk0 = 0;
-    for (size_t i = 0; i < g_items_.size(); ++i)
+        // ---- stage 3
+        uint64_t sb0 = 0, blk0 = 0;
+        for (size_t i = 0; i < g_items_.size(); ++i)
+        {
+            const SolveDesc &d = desc_of(i);
+            g_items_[i].sb0 = uint32_t(sb0);
+            g_items_[i].blk0 = uint32_t(blk0);
+            sb0 += n_slots[i];
+            blk0 += uint64_t(d.N) + 2 * uint64_t(d.U) + 1;
+        }
+        if (sb0 > g.cap_kv)
+            return SH_E_STATE; // never: each solve's slots are at most its upper bound
+        const uint32_t wg_tile3 = set_map(kGmTile, [&](size_t i) { return uint32_t(desc_of(i).n_tiles); });
+        const uint32_t wg_lists = set_map(kGmLists, [&](size_t i) {
+            const SolveDesc &d = desc_of(i);
+            return div_up(std::max(n_slots[i], uint32_t(d.N) + 2u * uint32_t(d.U) + 1u), 256u);
+        });
+        const uint32_t wg_node3 = set_map(kGmNode, [&](size_t i) { return uint32_t(desc_of(i).N); });
+        if ((rc = upload_group(2, st)) != SH_OK)
+            return rc;
+        gv = group_view();
+        SH_HIP(launch_tile_compact(K_, TP, gv, group_map(kGmTile), wg_tile3, g.ws, g.ka, g.va, st));
+        if (sb0 > 0)
+        {
+            // the slots of every block in tile order (stable)
+            tb = g.sort_temp_bytes;
+            SH_HIP(sort_pairs_u32_u32(g.sort_temp, tb, g.ka, g.kb, g.va, g.vb, uint32_t(sb0), key_bits(blk0), st));
+        }
+        SH_HIP(launch_slot_lists(gv, group_map(kGmLists), wg_lists, g.kb, g.vb, st));
+        SH_HIP(launch_block_transpose(gv, group_map(kGmNode), wg_node3, g.ws, st));
+        for (uint32_t s : g_slots_)
+            slots_[s].indexed = true;
+        return SH_OK;
+    };
+    const std::vector<uint32_t> s1_slots = g_slots_;
+    const std::vector<IndexItem> s1_items = g_items_;
+    const std::vector<uint32_t> s1_where(where.begin(), where.begin() + s1_items.size());
+    const uint64_t kp = uint64_t(K_ * (K_ + 1) / 2);
+    for (size_t a = 0; a < s1_items.size();)
     {
-        const SolveDesc &d = desc_of(i);
-        g_items_[i].sb0 = uint32_t(sb0);
-        g_items_[i].blk0 = uint32_t(blk0);
-        sb0 += n_slots[i];
-        blk0 += uint64_t(d.N) + 2 * uint64_t(d.U) + 1;
+        uint64_t kv = 0, nodes = 0, tcnt = 0, tent = 0, sub = 0;
+        size_t b = a;
+        for (; b < s1_items.size(); ++b)
+        {
+            const SolveDesc &d = slots_[s1_slots[b]].d;
+            const uint64_t nt = div_up(uint32_t(d.P), TP);
+            const uint64_t kv1 = kv + uint64_t(d.P) * uint64_t(K_) + 2 * uint64_t(d.E) + uint64_t(d.N);
+            const uint64_t n1 = nodes + uint64_t(d.N) + 1, tc1 = tcnt + nt + 2, te1 = tent + nt * TP * kp;
+            const uint64_t s1 = sub + std::min<uint64_t>(nt * TP * kp, pair_cap_);
+            if (b > a && (kv1 > g.cap_kv || n1 > g.cap_nodes || tc1 > g.cap_tcnt || te1 > g.cap_tent || s1 > g.cap_kv))
+                break;
+            kv = kv1, nodes = n1, tcnt = tc1, tent = te1, sub = s1;
+        }
+        g_slots_.assign(s1_slots.begin() + a, s1_slots.begin() + b);
+        g_items_.assign(s1_items.begin() + a, s1_items.begin() + b);
+        for (size_t i = 0; i < g_items_.size(); ++i)
+            g_items_[i].slot = uint32_t(i);
+        where.assign(s1_where.begin() + a, s1_where.begin() + b);
+        if ((rc = rows_tiles()) != SH_OK)
+            return rc;
+        a = b;
     }
-    if (sb0 > g.cap_kv)
-        return SH_E_STATE; // never: each solve's slots are at most its upper bound
-    const uint32_t wg_tile3 = set_map(kGmTile, [&](size_t i) { return uint32_t(desc_of(i).n_tiles); });
-    const uint32_t wg_lists = set_map(kGmLists, [&](size_t i) {
-        const SolveDesc &d = desc_of(i);
-        return div_up(std::max(n_slots[i], uint32_t(d.N) + 2u * uint32_t(d.U) + 1u), 256u);
-    });
-    const uint32_t wg_node3 = set_map(kGmNode, [&](size_t i) { return uint32_t(desc_of(i).N); });
-    if ((rc = upload_group(2, st)) != SH_OK)
-        return rc;
-    gv = group_view();
-    SH_HIP(launch_tile_compact(K_, TP, gv, group_map(kGmTile), wg_tile3, g.ws, g.ka, g.va, st));
-    if (sb0 > 0)
-    {
-        // the slots of every block in tile order (stable)
-        tb = g.sort_temp_bytes;
-        SH_HIP(sort_pairs_u32_u32(g.sort_temp, tb, g.ka, g.kb, g.va, g.vb, uint32_t(sb0), key_bits(blk0), st));
-    }
-    SH_HIP(launch_slot_lists(gv, group_map(kGmLists), wg_lists, g.kb, g.vb, st));
-    SH_HIP(launch_block_transpose(gv, group_map(kGmNode), wg_node3, g.ws, st));
-    for (uint32_t s : g_slots_)
-        slots_[s].indexed = true;
     return SH_OK;
 }
 

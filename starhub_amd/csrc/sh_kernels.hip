@@ -107,9 +107,23 @@ __global__ __launch_bounds__(256) void k_pixel_flags(IndexGroup g, const uint32_
         bool valid = s != kInvalidIndex;
         if (valid && s >= d.S)
             valid = false, bad = true;
-        for (int k = 0; valid && k < d.K; ++k)
-            if (d.surfel_knn[size_t(s) * d.K + k] >= N)
+        if (valid)
+        {
+            // the largest of the surfel's K node ids (one load when aligned)
+            const uint16_t *kn = d.surfel_knn + size_t(s) * d.K;
+            uint32_t mx = 0;
+            if (d.K == 8 && (reinterpret_cast<uintptr_t>(d.surfel_knn) & 15u) == 0)
+            {
+                const uint4 v = *reinterpret_cast<const uint4 *>(kn);
+                mx = max(max(max(v.x & 0xffffu, v.x >> 16), max(v.y & 0xffffu, v.y >> 16)),
+                         max(max(v.z & 0xffffu, v.z >> 16), max(v.w & 0xffffu, v.w >> 16)));
+            }
+            else
+                for (int k = 0; k < d.K; ++k)
+                    mx = max(mx, uint32_t(kn[k]));
+            if (mx >= N)
                 valid = false, bad = true;
+        }
         flags[i] = valid ? 1u : 0u;
         if (tiled)
             tflags[h] = valid ? 1u : 0u;
@@ -193,21 +207,65 @@ __global__ __launch_bounds__(256) void k_compact_pixels(IndexGroup g, const uint
     const uint32_t tx = f2u_rn_sat(fx + float(x));
     const uint32_t ty = f2u_rn_sat(fy + float(y));
     const ushort4 px = make_ushort4(uint16_t(x), uint16_t(y), uint16_t(tx), uint16_t(ty));
-    uint16_t kn[K];
+    // the surfel's K node ids and folded weights: vector loads when the
+    // caller's arrays are aligned for them, vector stores into ours
+    uint32_t kw[K / 2]; // node ids, two per word
     float w[K];
+    const bool aligned = ((reinterpret_cast<uintptr_t>(d.surfel_knn) | reinterpret_cast<uintptr_t>(d.surfel_spatial) |
+                           reinterpret_cast<uintptr_t>(d.surfel_connect)) &
+                          15u) == 0;
+    if (aligned)
+    {
+        const uint32_t *kk = reinterpret_cast<const uint32_t *>(d.surfel_knn + size_t(s) * K);
+        const float4 *sp = reinterpret_cast<const float4 *>(d.surfel_spatial + size_t(s) * K);
+        const float4 *cn = reinterpret_cast<const float4 *>(d.surfel_connect + size_t(s) * K);
+        if constexpr (K == 8)
+        {
+            const uint4 v = *reinterpret_cast<const uint4 *>(kk);
+            kw[0] = v.x, kw[1] = v.y, kw[2] = v.z, kw[3] = v.w;
+        }
+        else
+        {
+            const uint2 v = *reinterpret_cast<const uint2 *>(kk);
+            kw[0] = v.x, kw[1] = v.y;
+        }
+#pragma unroll
+        for (int q = 0; q < K / 4; ++q)
+        {
+            const float4 a = sp[q], c = cn[q];
+            w[4 * q] = a.x * c.x;
+            w[4 * q + 1] = a.y * c.y;
+            w[4 * q + 2] = a.z * c.z;
+            w[4 * q + 3] = a.w * c.w;
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int k = 0; k < K; k += 2)
+            kw[k / 2] = uint32_t(d.surfel_knn[size_t(s) * K + k]) |
+                        (uint32_t(d.surfel_knn[size_t(s) * K + k + 1]) << 16);
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            w[k] = d.surfel_spatial[size_t(s) * K + k] * d.surfel_connect[size_t(s) * K + k];
+    }
+    uint16_t kn[K];
 #pragma unroll
     for (int k = 0; k < K; ++k)
-    {
-        kn[k] = d.surfel_knn[size_t(s) * K + k];
-        w[k] = d.surfel_spatial[size_t(s) * K + k] * d.surfel_connect[size_t(s) * K + k];
-    }
+        kn[k] = uint16_t((kw[k >> 1] >> (16 * (k & 1))) & 0xffffu);
+    // stores of K ids (2K bytes) and K weights (4K bytes) at o: aligned
+    auto put = [&](uint16_t *pk, float *pw, uint32_t at) {
+        if constexpr (K == 8)
+            *reinterpret_cast<uint4 *>(pk + size_t(at) * 8) = make_uint4(kw[0], kw[1], kw[2], kw[3]);
+        else
+            *reinterpret_cast<uint2 *>(pk + size_t(at) * 4) = make_uint2(kw[0], kw[1]);
+#pragma unroll
+        for (int q = 0; q < K / 4; ++q)
+            *reinterpret_cast<float4 *>(pw + size_t(at) * K + 4 * q) =
+                make_float4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+    };
     d.pix[o] = px;
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-    {
-        d.pknn[size_t(o) * K + k] = kn[k];
-        d.pw[size_t(o) * K + k] = w[k];
-    }
+    put(d.pknn, d.pw, o);
     // the pixel term's (node key, term) pairs, K of them (Node2TermsIndex.cu:6-76)
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -220,12 +278,7 @@ __global__ __launch_bounds__(256) void k_compact_pixels(IndexGroup g, const uint
         // the GN phase's copy, at the pixel's tile-order position
         const uint32_t ot = ws.tpos[item.h0 + tile_order(x, y, (W + 31u) / 32u)] - ws.tpos[item.h0];
         d.gpix[ot] = px;
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-        {
-            d.gpknn[size_t(ot) * K + k] = kn[k];
-            d.gpw[size_t(ot) * K + k] = w[k];
-        }
+        put(d.gpknn, d.gpw, ot);
     }
 }
 
