@@ -1015,7 +1015,36 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_spmv(const SolveDesc *__res
         const uint32_t b0 = d.row_ptr[node] + node;
         const uint32_t b1 = d.row_ptr[node + 1] + node + 1;
         float acc = 0.f;
-        for (uint32_t bb = b0; bb < b1; ++bb)
+        // four blocks per round: their loads are issued before their products,
+        // which are summed in block order (the same sums as one block a round)
+        constexpr int kU = 4;
+        uint32_t bb = b0;
+        for (; bb + kU <= b1; bb += kU)
+        {
+            uint32_t col[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                col[u] = d.blk_col[bb + u];
+            float av[kU][6], xv[kU][6];
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+            {
+                const float *A = d.blk + size_t(bb + u) * 36 + 6 * a;
+                const size_t cr = size_t(col[u]) * 6;
+#pragma unroll
+                for (int c = 0; c < 6; ++c)
+                {
+                    av[u][c] = A[c];
+                    xv[u][c] = d_next(beta, dold[cr + c], d.s[cr + c]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+#pragma unroll
+                for (int c = 0; c < 6; ++c)
+                    acc += av[u][c] * xv[u][c];
+        }
+        for (; bb < b1; ++bb)
         {
             const uint32_t col = d.blk_col[bb];
             const float *A = d.blk + size_t(bb) * 36 + 6 * a;
