@@ -71,22 +71,27 @@ __global__ __launch_bounds__(kRowThreads) void k_row_bitmap(IndexGroup g, const 
     const uint32_t beg = d.n2t_off[i], end = d.n2t_off[i + 1];
     const uint32_t P = d.P, E = d.E;
     uint32_t dense = 0;
-    // kU terms per thread and round: their term ids, then their node lists,
-    // are loaded before any is used
+    // kU consecutive terms of the list per thread and round: their term ids,
+    // then their node lists, are loaded before any is used, and a dense term
+    // whose node list equals the previous one's (neighbouring pixels mostly
+    // share their surfels' nodes) sets no bits
     constexpr int kU = 4;
-    for (uint32_t t0 = beg + threadIdx.x; t0 < end; t0 += kU * blockDim.x)
+    for (uint32_t t0 = beg + threadIdx.x * kU; t0 < end; t0 += kU * blockDim.x)
     {
         uint32_t tt[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u)
         {
-            const uint32_t t_i = t0 + u * blockDim.x;
+            const uint32_t t_i = t0 + u;
             tt[u] = t_i < end ? d.n2t_terms[t_i] : 0xffffffffu;
         }
         uint32_t nw[kU][K / 2]; // the node ids of a dense term, two per word
 #pragma unroll
         for (int u = 0; u < kU; ++u)
         {
+#pragma unroll
+            for (int q = 0; q < K / 2; ++q)
+                nw[u][q] = 0u;
             if (tt[u] < P)
             {
                 if constexpr (K == 8)
@@ -112,12 +117,17 @@ __global__ __launch_bounds__(kRowThreads) void k_row_bitmap(IndexGroup g, const 
             if (tt[u] < P)
             {
                 ++dense;
+                bool same = u > 0 && tt[u - 1] < P;
 #pragma unroll
-                for (int k = 0; k < K; ++k)
-                {
-                    const uint32_t n = (nw[u][k >> 1] >> (16 * (k & 1))) & 0xffffu;
-                    atomicOr(&bits[n >> 5], 1u << (n & 31));
-                }
+                for (int q = 0; q < K / 2; ++q)
+                    same = same && nw[u][q] == nw[u > 0 ? u - 1 : 0][q];
+                if (!same)
+#pragma unroll
+                    for (int k = 0; k < K; ++k)
+                    {
+                        const uint32_t n = (nw[u][k >> 1] >> (16 * (k & 1))) & 0xffffu;
+                        atomicOr(&bits[n >> 5], 1u << (n & 31));
+                    }
             }
             else if (tt[u] < P + E)
             {

@@ -177,16 +177,50 @@ __device__ __forceinline__ void group_min_max(uint32_t &lo, uint32_t &hi, uint32
 }
 
 // ------------------------------------------------------------ index build
-// Per tile: contributions keyed (block << 8 | pixel), sorted in LDS.  Writes,
-// at the tile's fixed stride in the workspace, the packed contributions
-// (pixel | ka << 8 | kb << 11) in sorted order, each slot's block and first
-// contribution, and the tile's slot and contribution counts.
+// LDS of the tile build's ranking path: the tile's nodes and their ranks,
+// its node pairs (keys la * Dn + lb of the ranks la <= lb, Dn distinct nodes)
+// and their ranks, and each pair's pixels.
+template <int TP>
+struct TileRankLds
+{
+    static constexpr int kMaxNodes = 256;   // distinct nodes of a tile on this path
+    static constexpr int kPairWords = kMaxNodes * kMaxNodes / 32;
+    static constexpr int kMaxPairs = 1024;  // distinct pairs of a tile on this path
+    static constexpr int kMaskWords = TP / 32;
+    uint32_t nbits[kBitmapWords];
+    uint16_t npre[kBitmapWords];
+    uint16_t node_of[kMaxNodes];
+    uint32_t pbits[kPairWords];
+    uint16_t ppre[kPairWords];
+    uint16_t pkey[kMaxPairs];
+    uint32_t mask[kMaxPairs][kMaskWords];
+    uint16_t coff[kMaxPairs], soff[kMaxPairs];
+    uint32_t total[2];
+};
+
+// Per tile: the contributions in block order, each block's in pixel order,
+// at the tile's fixed stride in the workspace: the packed contributions
+// (pixel | ka << 8 | kb << 11), each slot's block and first contribution (a
+// slot: the contributions of one block and one group of kSlotPix pixels),
+// and the tile's slot and contribution counts.  The order of the blocks of
+// pairs lo <= hi is the order of (lo, hi): rows in node order, a row's blocks
+// in column order.  So the tile ranks its nodes (an LDS bitmap of node ids
+// and its prefix counts), then its pairs by (rank lo, rank hi) the same way,
+// and a contribution's place is its pair's first place plus the pair's pixels
+// before it (a pixel mask per pair); the block of a pair is looked up once.
+// A tile with more nodes or pairs than that LDS holds sorts its
+// (block << 8 | pixel) keys instead (a stable LDS radix sort).
 template <int K, int TP>
 __global__ __launch_bounds__(2 * TP) void k_tile_build(IndexGroup g, const uint32_t *__restrict__ wmap, IndexWs ws)
 {
     using G = TileGeom<K, TP>;
+    using R = TileRankLds<TP>;
     using Sort = rocprim::block_radix_sort<uint32_t, G::kThreads, G::kItems, uint8_t>;
-    __shared__ typename Sort::storage_type sort_storage;
+    __shared__ union
+    {
+        typename Sort::storage_type sort;
+        R rank;
+    } sm;
     __shared__ uint32_t last_key[G::kThreads];
     __shared__ uint32_t red[16];
     const int it = find_solve_t(wmap, g.n, blockIdx.x);
@@ -208,24 +242,33 @@ __global__ __launch_bounds__(2 * TP) void k_tile_build(IndexGroup g, const uint3
         }
         return; // uniform over the group
     }
+    const uint2 *__restrict__ bitpre = ws.bitpre + size_t(item.node0) * kBitmapWords;
+    const size_t stride = size_t(G::kEnt);
+    const size_t tbase = size_t(item.t0 + t) * stride;
+    uint16_t *__restrict__ cout = ws.tcontrib + tbase;
 
-    uint32_t keys[G::kItems];
+    // this thread's half of the pixel's pairs: q = i or kItems + i, as
+    // (lo | hi << 16) and (ka | kb << 3); all ones: no contribution
+    uint32_t lohi[G::kItems];
     uint8_t vals[G::kItems];
 #pragma unroll
     for (int i = 0; i < G::kItems; ++i)
     {
-        keys[i] = 0xffffffffu;
+        lohi[i] = 0xffffffffu;
         vals[i] = 0;
     }
+    uint16_t kn[K];
+    bool act[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        act[k] = false, kn[k] = 0;
     if (p < uint32_t(d.P))
     {
-        uint16_t kn[K];
 #pragma unroll
         for (int k = 0; k < K; ++k)
             kn[k] = d.gpknn[size_t(p) * K + k];
         // slot k is active when no later slot names the same node (last match,
         // dense_image_jacobian.cuh:17-23)
-        bool act[K];
 #pragma unroll
         for (int k = 0; k < K; ++k)
         {
@@ -235,29 +278,199 @@ __global__ __launch_bounds__(2 * TP) void k_tile_build(IndexGroup g, const uint3
                 if (kn[k2] == kn[k])
                     act[k] = false;
         }
-        const uint2 *__restrict__ bitpre = ws.bitpre + size_t(item.node0) * kBitmapWords;
 #pragma unroll
         for (int i = 0; i < G::kItems; ++i)
         {
-            // this thread's half of the pairs: q = i or kItems + i
             const int ka0 = pair_first<K>(i), kb0 = pair_second<K>(i);
             const int ka1 = pair_first<K>(G::kItems + i), kb1 = pair_second<K>(G::kItems + i);
             const bool on_a = half ? act[ka1] : act[ka0], on_b = half ? act[kb1] : act[kb0];
             const uint32_t na = half ? kn[ka1] : kn[ka0], nb = half ? kn[kb1] : kn[kb0];
             if (on_a && on_b)
             {
-                const uint32_t lo = min(na, nb), hi = max(na, nb);
-                keys[i] = (block_of_pair(d.row_ptr, bitpre, lo, hi) << kPixBits) | pl;
+                lohi[i] = min(na, nb) | (max(na, nb) << 16);
                 vals[i] = uint8_t(half ? (ka1 | (kb1 << 3)) : (ka0 | (kb0 << 3)));
             }
         }
     }
-    // keys are (block << 8 | pixel), made relative to the tile's smallest
-    // block, and sorted on their block bits only: the items are generated in
-    // pixel order and the radix sort is stable, so each block's items stay in
-    // pixel order, as with the whole key, in fewer passes.  An invalid key
-    // (all ones) sorts after every valid one: the relative blocks stay below
-    // the largest value the sorted bits hold.
+
+    // ---- ranking path
+    R &r = sm.rank;
+    for (uint32_t w = tid; w < kBitmapWords; w += G::kThreads)
+        r.nbits[w] = 0u;
+    __syncthreads();
+    // the pixel's active nodes, half of them per thread
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (act[k] && (k < K / 2) == (half == 0))
+            atomicOr(&r.nbits[kn[k] >> 5], 1u << (kn[k] & 31u));
+    __syncthreads();
+    if (tid < 64)
+    {
+        // node ranks: exclusive prefix of the word popcounts (two words a lane)
+        const uint32_t w0 = 2 * tid, w1 = w0 + 1;
+        const uint32_t b0 = r.nbits[w0], b1 = r.nbits[w1];
+        const uint32_t c0 = __popc(b0), c1 = __popc(b1);
+        uint32_t v = c0 + c1;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            const uint32_t u = __shfl_up(v, o, 64);
+            if (int(tid) >= o)
+                v += u;
+        }
+        const uint32_t ex = v - c0 - c1;
+        r.npre[w0] = uint16_t(ex);
+        r.npre[w1] = uint16_t(ex + c0);
+        if (tid == 63)
+            r.total[0] = v;
+        // the node of each rank
+        uint32_t at = ex;
+        for (uint32_t bb = b0; bb; bb &= bb - 1, ++at)
+            if (at < uint32_t(R::kMaxNodes))
+                r.node_of[at] = uint16_t(w0 * 32 + __ffs(bb) - 1);
+        for (uint32_t bb = b1; bb; bb &= bb - 1, ++at)
+            if (at < uint32_t(R::kMaxNodes))
+                r.node_of[at] = uint16_t(w1 * 32 + __ffs(bb) - 1);
+    }
+    __syncthreads();
+    const uint32_t dn = r.total[0];
+    bool ranked = dn <= uint32_t(R::kMaxNodes);
+    uint32_t pk[G::kItems]; // pair keys, then pair ranks
+    if (ranked)
+    {
+        const uint32_t pwords = (dn * dn + 31u) / 32u;
+        for (uint32_t w = tid; w < pwords; w += G::kThreads)
+            r.pbits[w] = 0u;
+        __syncthreads();
+        auto nrank = [&](uint32_t n) {
+            return uint32_t(r.npre[n >> 5]) + uint32_t(__popc(r.nbits[n >> 5] & ((1u << (n & 31u)) - 1u)));
+        };
+#pragma unroll
+        for (int i = 0; i < G::kItems; ++i)
+            if (lohi[i] != 0xffffffffu)
+            {
+                pk[i] = nrank(lohi[i] & 0xffffu) * dn + nrank(lohi[i] >> 16);
+                atomicOr(&r.pbits[pk[i] >> 5], 1u << (pk[i] & 31u));
+            }
+        __syncthreads();
+        // pair ranks: exclusive prefix of the word popcounts, a run of words per thread
+        const uint32_t per = (pwords + G::kThreads - 1) / G::kThreads;
+        const uint32_t wb = min(pwords, tid * per), we = min(pwords, wb + per);
+        uint32_t own = 0;
+        for (uint32_t w = wb; w < we; ++w)
+            own += __popc(r.pbits[w]);
+        uint32_t np = 0;
+        uint32_t run = group_exclusive_scan<G::kThreads>(own, red, &np);
+        ranked = np <= uint32_t(R::kMaxPairs); // uniform
+        if (ranked)
+        {
+            for (uint32_t w = wb; w < we; ++w)
+            {
+                const uint32_t bw = r.pbits[w];
+                r.ppre[w] = uint16_t(run);
+                for (uint32_t bb = bw; bb; bb &= bb - 1, ++run)
+                    r.pkey[run] = uint16_t(w * 32 + __ffs(bb) - 1);
+            }
+            for (uint32_t e = tid; e < np * uint32_t(R::kMaskWords); e += G::kThreads)
+                (&r.mask[0][0])[e] = 0u;
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < G::kItems; ++i)
+                if (lohi[i] != 0xffffffffu)
+                {
+                    const uint32_t key = pk[i];
+                    pk[i] = uint32_t(r.ppre[key >> 5]) + uint32_t(__popc(r.pbits[key >> 5] & ((1u << (key & 31u)) - 1u)));
+                    atomicOr(&r.mask[pk[i]][pl >> 5], 1u << (pl & 31u));
+                }
+            __syncthreads();
+            // each pair's contribution count and slot count (non-empty
+            // kSlotPix-pixel groups), a run of pairs per thread, and their
+            // exclusive scans
+            const uint32_t per2 = (np + G::kThreads - 1) / G::kThreads;
+            const uint32_t qb = min(np, tid * per2), qe = min(np, qb + per2);
+            uint32_t nc = 0, ns = 0;
+            for (uint32_t q = qb; q < qe; ++q)
+#pragma unroll
+                for (int w = 0; w < R::kMaskWords; ++w)
+                {
+                    const uint32_t m = r.mask[q][w];
+                    nc += __popc(m);
+                    if (w % (kSlotPix / 32) == 0)
+                    {
+                        uint32_t grp = 0;
+#pragma unroll
+                        for (int w2 = 0; w2 < kSlotPix / 32; ++w2)
+                            grp |= r.mask[q][w + w2];
+                        ns += grp ? 1u : 0u;
+                    }
+                }
+            uint32_t total_c = 0, total_s = 0;
+            uint32_t c_run = group_exclusive_scan<G::kThreads>(nc, red, &total_c);
+            uint32_t s_run = group_exclusive_scan<G::kThreads>(ns, red, &total_s);
+            for (uint32_t q = qb; q < qe; ++q)
+            {
+                r.coff[q] = uint16_t(c_run);
+                r.soff[q] = uint16_t(s_run);
+                // the pair's slots: block and first contribution
+                const uint32_t key = r.pkey[q];
+                const uint32_t lo = r.node_of[key / dn], hi = r.node_of[key % dn];
+                const uint32_t blk = block_of_pair(d.row_ptr, bitpre, lo, hi);
+                uint32_t c = c_run;
+#pragma unroll
+                for (int w = 0; w < R::kMaskWords; w += kSlotPix / 32)
+                {
+                    uint32_t cnt = 0;
+#pragma unroll
+                    for (int w2 = 0; w2 < kSlotPix / 32; ++w2)
+                        cnt += __popc(r.mask[q][w + w2]);
+                    if (cnt)
+                    {
+                        ws.tslot_block[tbase + s_run] = blk;
+                        ws.tslot_first[tbase + s_run] = uint16_t(c);
+                        ++s_run;
+                        c += cnt;
+                    }
+                }
+                c_run = c;
+            }
+            __syncthreads();
+            // each contribution at its pair's first place plus the pair's pixels before it
+#pragma unroll
+            for (int i = 0; i < G::kItems; ++i)
+                if (lohi[i] != 0xffffffffu)
+                {
+                    const uint32_t q = pk[i], pw = pl >> 5;
+                    uint32_t at = r.coff[q] + uint32_t(__popc(r.mask[q][pw] & ((1u << (pl & 31u)) - 1u)));
+#pragma unroll
+                    for (int w = 0; w < R::kMaskWords; ++w)
+                        if (uint32_t(w) < pw)
+                            at += __popc(r.mask[q][w]);
+                    cout[at] = uint16_t(pl | (uint32_t(vals[i]) << kPixBits));
+                }
+            if (tid == 0)
+            {
+                tile_nslot[t] = total_s;
+                tile_ncontrib[t] = total_c;
+                atomicMax(tile_nslot + uint32_t(d.n_tiles) + 1, total_s); // largest slot count of the solve's tiles
+            }
+            return; // uniform
+        }
+    }
+    __syncthreads(); // the LDS goes to the sort
+
+    // ---- sorting path: keys (block << 8 | pixel)
+    uint32_t keys[G::kItems];
+#pragma unroll
+    for (int i = 0; i < G::kItems; ++i)
+        keys[i] = lohi[i] == 0xffffffffu
+                      ? 0xffffffffu
+                      : (block_of_pair(d.row_ptr, bitpre, lohi[i] & 0xffffu, lohi[i] >> 16) << kPixBits) | pl;
+    // keys are made relative to the tile's smallest block and sorted on their
+    // block bits only: the items are generated in pixel order and the radix
+    // sort is stable, so each block's items stay in pixel order, as with the
+    // whole key, in fewer passes.  An invalid key (all ones) sorts after every
+    // valid one: the relative blocks stay below the largest value the sorted
+    // bits hold.
     uint32_t bmin = 0xffffffffu, bmax = 0u;
 #pragma unroll
     for (int i = 0; i < G::kItems; ++i)
@@ -274,7 +487,7 @@ __global__ __launch_bounds__(2 * TP) void k_tile_build(IndexGroup g, const uint3
         if (keys[i] != 0xffffffffu)
             keys[i] -= bmin << kPixBits;
     const unsigned block_bits = 32u - unsigned(__clz(bmax - bmin + 1u));
-    Sort().sort(keys, vals, sort_storage, unsigned(kPixBits), min(32u, unsigned(kPixBits) + block_bits));
+    Sort().sort(keys, vals, sm.sort, unsigned(kPixBits), min(32u, unsigned(kPixBits) + block_bits));
 
     // slot heads: item gi = tid * kItems + i starts a slot when its block or
     // its pixel group (kSlotPix consecutive pixels of the tile) differs from
@@ -295,9 +508,6 @@ __global__ __launch_bounds__(2 * TP) void k_tile_build(IndexGroup g, const uint3
     uint32_t total_heads = 0, total_valid = 0;
     uint32_t slot = group_exclusive_scan<G::kThreads>(nhead, red, &total_heads);
     (void)group_exclusive_scan<G::kThreads>(nvalid, red, &total_valid);
-    const size_t stride = size_t(G::kEnt);
-    const size_t tbase = size_t(item.t0 + t) * stride;
-    uint16_t *__restrict__ cout = ws.tcontrib + tbase;
     prev = prev0;
 #pragma unroll
     for (int i = 0; i < G::kItems; ++i)
