@@ -13,8 +13,8 @@ before the timed region.  Beside it, in the same run:
   80 % of the measured capacity (a load the server keeps up with), about four
   seconds of them -- p50/p99 request latency and served rate;
 * served_30hz: the serving workload as configured -- Poisson arrivals at
-  30 Hz per stream (1,920 requests/s offered, above the capacity, so its
-  latency is the backlog's growth), 10,000 requests;
+  30 Hz per stream (about 1,950 requests/s offered; where that is above the
+  capacity, its latency is the backlog's growth), 10,000 requests;
 * plain: the plain single-solve path on the full-frame workload (300k
   surfels, 1024 nodes, 640x480, 10 GN x 20 PCG), autotuned, with the
   roofline of its dominant kernel and the CPU baseline on the same frame;
@@ -596,7 +596,9 @@ def main():
             "config": {"workload": cap["workload"], "streams": 64, "requests_per_step": 64},
             "latency_ms": {"p50": served["served_sustained"]["p50_ms"], "p99": served["served_sustained"]["p99_ms"],
                            "at": "a load the server sustains: Poisson arrivals at 80 % of the measured capacity "
-                                 "(the configured 30 Hz per stream is above it: served_30hz)"},
+                                 + ("(the configured 30 Hz per stream is above that: served_30hz)"
+                                    if served["served_30hz"]["offered_per_s"] > 0.8 * cap["solves_per_s"]
+                                    else "(the configured 30 Hz per stream is below it: served_30hz)")},
             "vs_plain": cap["solves_per_s"] / served["plain_solves_per_s"],
             "roofline": served["roofline"],
         }
