@@ -52,7 +52,7 @@ __device__ __forceinline__ uint32_t c_kb(uint32_t c) { return (c >> (kPixBits + 
 #endif
 constexpr int kSlotShift = SH_SLOT_SHIFT;
 constexpr int kSlotPix = 1 << kSlotShift;
-static_assert((1 << kSlotShift) == kSlotPix && kSlotPix <= int(kTilePixMin), "slot pixel groups");
+static_assert((1 << kSlotShift) == kSlotPix && kSlotPix >= 32, "slot pixel groups");
 
 // A tile of TP pixels for K nodes per pixel: two threads per pixel.
 template <int K, int TP>
@@ -187,6 +187,7 @@ struct TileRankLds
     static constexpr int kPairWords = kMaxNodes * kMaxNodes / 32;
     static constexpr int kMaxPairs = 1024;  // distinct pairs of a tile on this path
     static constexpr int kMaskWords = TP / 32;
+    static constexpr int kGroupWords = (kSlotPix < TP ? kSlotPix : TP) / 32; // mask words of a slot's pixel group
     uint32_t nbits[kBitmapWords];
     uint16_t npre[kBitmapWords];
     uint16_t node_of[kMaxNodes];
@@ -395,11 +396,11 @@ __global__ __launch_bounds__(2 * TP) void k_tile_build(IndexGroup g, const uint3
                 {
                     const uint32_t m = r.mask[q][w];
                     nc += __popc(m);
-                    if (w % (kSlotPix / 32) == 0)
+                    if (w % R::kGroupWords == 0)
                     {
                         uint32_t grp = 0;
 #pragma unroll
-                        for (int w2 = 0; w2 < kSlotPix / 32; ++w2)
+                        for (int w2 = 0; w2 < R::kGroupWords; ++w2)
                             grp |= r.mask[q][w + w2];
                         ns += grp ? 1u : 0u;
                     }
@@ -417,11 +418,11 @@ __global__ __launch_bounds__(2 * TP) void k_tile_build(IndexGroup g, const uint3
                 const uint32_t blk = block_of_pair(d.row_ptr, bitpre, lo, hi);
                 uint32_t c = c_run;
 #pragma unroll
-                for (int w = 0; w < R::kMaskWords; w += kSlotPix / 32)
+                for (int w = 0; w < R::kMaskWords; w += R::kGroupWords)
                 {
                     uint32_t cnt = 0;
 #pragma unroll
-                    for (int w2 = 0; w2 < kSlotPix / 32; ++w2)
+                    for (int w2 = 0; w2 < R::kGroupWords; ++w2)
                         cnt += __popc(r.mask[q][w + w2]);
                     if (cnt)
                     {

@@ -343,6 +343,48 @@ def test_tile_sizes(torch_dev, tile_pix, K):
     batch.close()
 
 
+@pytest.mark.parametrize("part", ["all", "left_half"])
+def test_tile_build_sort_path(torch_dev, part):
+    """Tiles with more distinct nodes than the tile build's ranking LDS holds
+    take its sorting path: each surfel's K nodes drawn from a random window of
+    64 node ids (hundreds of distinct nodes per 128-pixel tile of the
+    1024-node frame, rows of at most 126 neighbours) for every surfel ("all")
+    or for the surfels seen in the left half of the image ("left_half": both
+    paths in one launch).  The tile index holds one contribution per (pixel, node pair
+    lo <= hi of its distinct nodes), and the first GN iteration's blocks and
+    right-hand side equal the row-segment path's (no tiles) within the FP32
+    differences of another summation order."""
+    from starhub_amd import native, scene, workload
+    torch = torch_dev
+    sc = scene.generate(scene.config_c2())
+    params = replace(sc.cfg.solver, gn_iterations=1)
+    _, prod = workload.device_frame(sc)
+    K, N = sc.cfg.K, sc.cfg.num_nodes
+    shape = prod["surfel_knn"].shape
+    knn = prod["surfel_knn"].reshape(-1, K).copy()
+    idx = sc.index_map
+    seen = idx if part == "all" else idx[:, : sc.cfg.width // 2]
+    surfels = np.unique(seen[seen != 0xFFFFFFFF])
+    rng = np.random.default_rng(11)
+    for s in surfels:
+        knn[s] = rng.integers(0, N - 64) + rng.choice(64, K, replace=False)
+    prod = dict(prod, surfel_knn=knn.reshape(shape))
+    say("tile build sort path", part, "scrambled surfels", len(surfels))
+    tiled, _, _, _ = _solve(sc, producers=prod, params=params)
+    rows, _, _, _ = _solve(sc, producers=prod, params=params, flags=native.SH_FLAG_ROW_SEGMENTS)
+    vis = idx[idx != 0xFFFFFFFF]
+    distinct = np.array([len(np.unique(knn[s])) for s in vis])
+    cbase = _idx(tiled, "tile_cbase", torch)
+    assert int(cbase[-1]) == int((distinct * (distinct + 1) // 2).sum())
+    for name in ("blk", "rhs"):
+        a = tiled.debug(name, torch.float32).cpu().numpy()
+        b = rows.debug(name, torch.float32).cpu().numpy()
+        say(name, "max diff", np.abs(a - b).max(), "max", np.abs(b).max())
+        assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max()
+    tiled.close()
+    rows.close()
+
+
 @pytest.mark.parametrize("name", ["tiny_k8", "tiny_k4"])
 def test_golden_fixture(torch_dev, name):
     import make_golden
