@@ -675,6 +675,55 @@ def test_served_same_alone_and_in_mix(torch_dev):
     srv.close()
 
 
+def test_served_rejects_one_request_of_a_round(torch_dev):
+    """The requests an admission round takes are indexed as one group: a
+    request whose index map names a surfel beyond its stream's count is
+    rejected with SH_E_ARG by the group's range check, while the other
+    requests are solved, each equal to its frame solved alone, bit for bit;
+    the stream's next request is solved as before."""
+    import dataclasses
+    from starhub_amd import native, scene, workload
+    torch = torch_dev
+    cfgs = [scene.config_batch_member(n, 900 + i) for i, n in enumerate([200, 500, 320, 800])]
+    frames = [workload.device_frame(scene.generate(c))[0] for c in cfgs]
+    W = max(c.width for c in cfgs)
+    H = max(c.height for c in cfgs)
+    N = max(c.num_nodes for c in cfgs)
+    scfg = native.make_config(cfgs[0].solver, W, H, N, 8)
+    solo_cfg = native.make_config(cfgs[0].solver, W, H, N, 8, flags=native.SH_FLAG_NO_PERSISTENT_PCG)
+    alone = []
+    for f in frames:
+        s = native.WarpSolver(solo_cfg)
+        s.process_frame(f)
+        alone.append(s.node_se3().cpu().numpy())
+        s.close()
+    imap = frames[2].index_map.clone()
+    flat = imap.view(-1)
+    first = int(torch.nonzero(flat != -1)[0])
+    flat[first] = frames[2].num_surfels
+    bad = dataclasses.replace(frames[2], index_map=imap)
+    srv = native.Server(scfg, max_active=8)
+    for i, f in enumerate(frames):
+        assert srv.add_stream(i, f) == 0
+    order = [(0, frames[0]), (2, bad), (1, frames[1]), (3, frames[3])]
+    outs = [torch.zeros(cfgs[i].num_nodes, 8, dtype=torch.float32, device="cuda") for i, _ in order]
+    tickets = [srv.submit(i, f, o) for (i, f), o in zip(order, outs)]
+    results = [srv.wait(t) for t in tickets]
+    torch.cuda.synchronize()
+    say("statuses", [r["status"] for r in results])
+    assert [r["status"] for r in results] == [0, 1, 0, 0]  # SH_E_ARG = 1
+    for (i, _), o, r in zip(order, outs, results):
+        if r["status"] == 0:
+            assert np.array_equal(o.cpu().numpy(), alone[i]), i
+    out = torch.zeros(cfgs[2].num_nodes, 8, dtype=torch.float32, device="cuda")
+    assert srv.wait(srv.submit(2, frames[2], out))["status"] == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), alone[2])
+    st = srv.stats()
+    assert st["failed"] == 1
+    srv.close()
+
+
 def test_server_budget_admission(torch_dev):
     """A budget that holds the solve slots but not every stream refuses the
     streams beyond it with SH_E_CAPACITY (admission by HBM budget)."""
