@@ -658,30 +658,34 @@ __global__ __launch_bounds__(2 * TP) void k_tiles(const SolveDesc *__restrict__ 
     }
     if (ns > slot_cap)
         return; // never taken: slot_cap is the largest slot count of the launch's tiles
-    // the contribution list (at most kEnt entries) and the slot starts: every
-    // load issued before the first LDS store, one round trip for all of them
+    // the contribution list (at most kEnt entries) and the slot starts, staged
+    // by the upper TP threads (no record of their own) while the lower TP
+    // build or read the records: every load issued before the first LDS
+    // store, one round trip for all of them
+    if (tid >= uint32_t(TP))
     {
-        constexpr int kPer = kEnt / NT;
+        constexpr int kPer = kEnt / TP;
+        const uint32_t u = tid - uint32_t(TP);
         uint16_t cv[kPer];
 #pragma unroll
         for (int j = 0; j < kPer; ++j)
         {
-            const uint32_t i = tid + uint32_t(j) * NT;
+            const uint32_t i = u + uint32_t(j) * TP;
             cv[j] = i < nc ? d.tc_contrib[c0 + i] : uint16_t(0);
         }
         uint32_t sv = 0;
-        if (tid <= ns)
-            sv = d.tc_slot_first[s0 + tid];
+        if (u <= ns)
+            sv = d.tc_slot_first[s0 + u];
 #pragma unroll
         for (int j = 0; j < kPer; ++j)
         {
-            const uint32_t i = tid + uint32_t(j) * NT;
+            const uint32_t i = u + uint32_t(j) * TP;
             if (i < nc)
                 ent[i] = cv[j];
         }
-        if (tid <= ns)
-            sst[tid] = uint16_t(sv - c0);
-        for (uint32_t s = tid + NT; s <= ns; s += NT)
+        if (u <= ns)
+            sst[u] = uint16_t(sv - c0);
+        for (uint32_t s = u + TP; s <= ns; s += TP)
             sst[s] = uint16_t(d.tc_slot_first[s0 + s] - c0);
     }
     __syncthreads();
