@@ -339,12 +339,13 @@ int Engine::alloc_group_ws(uint32_t units, uint32_t max_items)
                                lim / (tcap_ * uint64_t(kFlagUnits))})));
     group_units_ = units;
     // the flag stage takes kFlagUnits frames per unit (16 bytes per map
-    // pixel); the row and tile stages' groups are then formed by the valid
-    // pixels, a fraction of the frame
+    // pixel), and the node arrays kFlagUnits node sets per unit (a batch's
+    // solves have far fewer nodes than the largest); the row and tile stages'
+    // groups are then formed by the valid pixels, a fraction of the frame
     g.cap_m = uint64_t(units) * kFlagUnits * pcap_;
     g.cap_h = uint64_t(units) * kFlagUnits * tcap_; // the tuning may switch the tile order on later
     g.cap_kv = uint64_t(units) * kv_unit;
-    g.cap_nodes = uint64_t(units) * (uint64_t(ncap_) + 1);
+    g.cap_nodes = uint64_t(units) * kFlagUnits * (uint64_t(ncap_) + 1); // about 1.6 KB a node
     g.cap_tcnt = uint64_t(units) * (uint64_t(tiles_cap_) + 2);
     g.cap_tent = uint64_t(units) * tent_unit;
     g.cap_items = max_items;
