@@ -134,9 +134,11 @@ static int hip_status(hipError_t e)
 
 // Requests one admission round takes per index-build unit (one solve of the
 // largest frame in an index group), at most: the round's index builds run as
-// few groups.  Under a backlog the rounds are larger, so more solves share
-// each GN step; under a light load a round holds what is waiting.
-static constexpr uint32_t kAdmitPerUnit = 4;
+// one group.  Smaller rounds reach the GN steps sooner, so more admitted
+// solves share each step (measured on the 64-stream mix with eight units:
+// 2,726 solves/s at one per unit, 2,720 at two, 2,717 at three, 2,681 at
+// four, 2,480 at eight); under a light load a round holds what is waiting.
+static constexpr uint32_t kAdmitPerUnit = 2;
 
 void sh_server::admit_loop()
 {
