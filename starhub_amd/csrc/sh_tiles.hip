@@ -10,11 +10,12 @@
 //
 // Once per solve (index build), one work-group per tile lists the tile's
 // contributions -- (node pair (a <= b) of distinct nodes of a pixel, pixel,
-// slot of a, slot of b) -- keys them by (upper block of the pair, pixel) and
-// radix-sorts them in LDS.  Equal blocks within a group of kSlotPix pixels
-// form a slot; slots are numbered across tiles in tile order, and a stable
-// sort of the slots by block gives every block the list of its slot partials
-// in tile order.
+// slot of a, slot of b) -- in (upper block of the pair, pixel) order: by the
+// ranks of its node pairs and a pixel mask per pair (or, for a tile with
+// too many nodes, an LDS radix sort of the keys).  Equal blocks within a
+// group of kSlotPix pixels form a slot; slots are numbered across tiles in
+// tile order, and a stable sort of the slots by block gives every block the
+// list of its slot partials in tile order.
 //
 // Per GN iteration, one work-group per tile stages its records, weights and
 // contribution list in LDS and sums every slot's contributions in pixel

@@ -107,7 +107,7 @@ def test_served_workload_full_size_budgeted(torch_dev):
     p1.close()
     slots, streams_fit = 32, 48
     p32 = native.Server(scfg, max_active=slots)
-    engine32 = p32.stats()["hbm_reserved"]  # 32 solve slots and their index lanes
+    engine32 = p32.stats()["hbm_reserved"]  # 32 solve slots and the index-build workspace
     p32.close()
     say("bytes of 32 slots", engine32, "largest slot growth", growth, "all streams", sum(stream_bytes))
     assert growth > 0 and engine32 > base1
