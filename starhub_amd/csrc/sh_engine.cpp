@@ -657,8 +657,8 @@ int Engine::build_group(const uint32_t *list, uint32_t n, hipStream_t st, int *r
         });
         const uint32_t wg_split = set_map(kGmSplit, [&](size_t i) {
             const SolveDesc &d = desc_of(i);
-            const uint32_t n_kv = uint32_t(d.P) * uint32_t(K_) + 2u * uint32_t(d.E) + uint32_t(d.N);
-            return div_up(std::max(n_kv, uint32_t(d.N) + 1), 256u);
+            // kSplitPer sorted pairs a thread (every node has its translation term)
+            return div_up(uint32_t(d.P) * uint32_t(K_) + 2u * uint32_t(d.E) + uint32_t(d.N), 256u * kSplitPer);
         });
         const uint32_t wg_node = set_map(kGmNode, [&](size_t i) { return uint32_t(desc_of(i).N); });
         const uint32_t wg_tile = set_map(kGmTile, [&](size_t i) { return uint32_t(desc_of(i).n_tiles); });

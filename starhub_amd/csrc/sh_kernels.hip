@@ -284,8 +284,10 @@ __global__ __launch_bounds__(256) void k_compact_pixels(IndexGroup g, const uint
 
 // After the stable node sort of the group's pairs: each solve's sorted term
 // ids, and offset[v] = the first sorted position whose key is >= v
-// (computeTermOffsetKernel, Node2TermsIndex.cu:78-122, written as a lower
-// bound), v <= N.
+// (computeTermOffsetKernel, Node2TermsIndex.cu:78-122), v <= N: a thread per
+// sorted position j writes offset[v] = j for the keys v after the one below
+// it up to its own (the first position of the solve starts at node 0, the
+// last also writes the ones after its key, up to N, as n_kv).
 __global__ __launch_bounds__(256) void k_n2t_split(IndexGroup g, const uint32_t *__restrict__ wmap,
                                                    const uint32_t *__restrict__ skeys,
                                                    const uint32_t *__restrict__ svals)
@@ -295,22 +297,24 @@ __global__ __launch_bounds__(256) void k_n2t_split(IndexGroup g, const uint32_t 
     const SolveDesc d = global_desc(g.sds[item.slot]);
     const uint32_t N = uint32_t(d.N);
     const uint32_t n_kv = uint32_t(d.P) * uint32_t(d.K) + 2u * uint32_t(d.E) + N;
-    const uint32_t v = (blockIdx.x - wmap[it]) * blockDim.x + threadIdx.x;
-    if (v < n_kv)
-        d.n2t_terms[v] = svals[item.kv0 + v];
-    if (v <= N)
+    // kSplitPer positions a thread, a work-group's positions contiguous (a
+    // work-group's fixed cost, finding its solve and descriptor, is spread
+    // over more work)
+    const uint32_t j0 = (blockIdx.x - wmap[it]) * blockDim.x * kSplitPer + threadIdx.x;
+#pragma unroll
+    for (uint32_t u = 0; u < kSplitPer; ++u)
     {
-        const uint32_t key = item.node0 + v;
-        uint32_t lo = item.kv0, hi = item.kv0 + n_kv;
-        while (lo < hi)
-        {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (skeys[mid] < key)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        d.n2t_off[v] = lo - item.kv0;
+        const uint32_t j = j0 + u * blockDim.x;
+        if (j >= n_kv)
+            break;
+        d.n2t_terms[j] = svals[item.kv0 + j];
+        const uint32_t key = skeys[item.kv0 + j] - item.node0; // <= N
+        const uint32_t first = j == 0 ? 0u : skeys[item.kv0 + j - 1] - item.node0 + 1u;
+        for (uint32_t v = first; v <= key; ++v)
+            d.n2t_off[v] = j;
+        if (j + 1 == n_kv)
+            for (uint32_t v = key + 1; v <= N; ++v)
+                d.n2t_off[v] = n_kv;
     }
 }
 

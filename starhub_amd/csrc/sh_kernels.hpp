@@ -75,7 +75,9 @@ hipError_t launch_pixel_flags(const IndexGroup &g, const uint32_t *wmap, uint32_
 // later stages accumulate into.
 hipError_t launch_compact_pixels(int K, const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, bool tiled,
                                  const IndexWs &ws, uint32_t *keys, uint32_t *vals, hipStream_t st);
-// After the node sort: each solve's n2t_terms and n2t_off.
+// After the node sort: each solve's n2t_terms and n2t_off, kSplitPer sorted
+// pairs a thread (work-groups of 256 threads).
+constexpr uint32_t kSplitPer = 8;
 hipError_t launch_n2t_split(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const uint32_t *skeys,
                             const uint32_t *svals, hipStream_t st);
 hipError_t launch_scan_jobs(const ScanJob *jobs, uint32_t n_jobs, hipStream_t st);
