@@ -585,7 +585,7 @@ int Engine::build_group(const uint32_t *list, uint32_t n, hipStream_t st, int *r
         const SolveDesc &d = desc_of(i);
         const uint32_t px = uint32_t(d.W) * uint32_t(d.H);
         const uint32_t nh = tiled ? tile_order_size(uint32_t(d.W), uint32_t(d.H)) : 0u;
-        return div_up(std::max(std::max(px, nh), uint32_t(d.E)), 256u);
+        return div_up(std::max(std::max(px, nh), uint32_t(d.E)), 256u * kPixPer);
     });
     int rc;
     if ((rc = upload_group(0, st)) != SH_OK)

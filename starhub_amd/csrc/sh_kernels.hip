@@ -84,52 +84,58 @@ __global__ __launch_bounds__(256) void k_pixel_flags(IndexGroup g, const uint32_
     const uint32_t nh = tiled ? nbx * ((H + 31u) / 32u) * 1024u : 0u;
     uint32_t *__restrict__ flags = ws.flags + item.m0;
     uint32_t *__restrict__ tflags = ws.tflags + item.h0;
-    const uint32_t h = (blockIdx.x - wmap[it]) * blockDim.x + threadIdx.x;
+    // kPixPer indices a thread, a work-group's indices contiguous
+    const uint32_t hb = (blockIdx.x - wmap[it]) * blockDim.x * kPixPer + threadIdx.x;
     bool bad = false;
-    uint32_t i = h; // raster index of the pixel this thread decides
-    bool inside = h < n;
-    if (tiled)
+#pragma unroll
+    for (uint32_t u = 0; u < kPixPer; ++u)
     {
-        inside = false;
-        if (h < nh)
-        {
-            const uint32_t b = h >> 10, m = h & 1023u;
-            const uint32_t x = (b % nbx) * 32u + squeeze5(m), y = (b / nbx) * 32u + squeeze5(m >> 1);
-            inside = x < W && y < H;
-            i = y * W + x;
-            if (!inside)
-                tflags[h] = 0u;
-        }
-    }
-    if (inside)
-    {
-        const uint32_t s = d.index_map[i];
-        bool valid = s != kInvalidIndex;
-        if (valid && s >= d.S)
-            valid = false, bad = true;
-        if (valid)
-        {
-            // the largest of the surfel's K node ids (one load when aligned)
-            const uint16_t *kn = d.surfel_knn + size_t(s) * d.K;
-            uint32_t mx = 0;
-            if (d.K == 8 && (reinterpret_cast<uintptr_t>(d.surfel_knn) & 15u) == 0)
-            {
-                const uint4 v = *reinterpret_cast<const uint4 *>(kn);
-                mx = max(max(max(v.x & 0xffffu, v.x >> 16), max(v.y & 0xffffu, v.y >> 16)),
-                         max(max(v.z & 0xffffu, v.z >> 16), max(v.w & 0xffffu, v.w >> 16)));
-            }
-            else
-                for (int k = 0; k < d.K; ++k)
-                    mx = max(mx, uint32_t(kn[k]));
-            if (mx >= N)
-                valid = false, bad = true;
-        }
-        flags[i] = valid ? 1u : 0u;
+        const uint32_t h = hb + u * blockDim.x;
+        uint32_t i = h; // raster index of the pixel this thread decides
+        bool inside = h < n;
         if (tiled)
-            tflags[h] = valid ? 1u : 0u;
+        {
+            inside = false;
+            if (h < nh)
+            {
+                const uint32_t b = h >> 10, m = h & 1023u;
+                const uint32_t x = (b % nbx) * 32u + squeeze5(m), y = (b / nbx) * 32u + squeeze5(m >> 1);
+                inside = x < W && y < H;
+                i = y * W + x;
+                if (!inside)
+                    tflags[h] = 0u;
+            }
+        }
+        if (inside)
+        {
+            const uint32_t s = d.index_map[i];
+            bool valid = s != kInvalidIndex;
+            if (valid && s >= d.S)
+                valid = false, bad = true;
+            if (valid)
+            {
+                // the largest of the surfel's K node ids (one load when aligned)
+                const uint16_t *kn = d.surfel_knn + size_t(s) * d.K;
+                uint32_t mx = 0;
+                if (d.K == 8 && (reinterpret_cast<uintptr_t>(d.surfel_knn) & 15u) == 0)
+                {
+                    const uint4 v = *reinterpret_cast<const uint4 *>(kn);
+                    mx = max(max(max(v.x & 0xffffu, v.x >> 16), max(v.y & 0xffffu, v.y >> 16)),
+                             max(max(v.z & 0xffffu, v.z >> 16), max(v.w & 0xffffu, v.w >> 16)));
+                }
+                else
+                    for (int k = 0; k < d.K; ++k)
+                        mx = max(mx, uint32_t(kn[k]));
+                if (mx >= N)
+                    valid = false, bad = true;
+            }
+            flags[i] = valid ? 1u : 0u;
+            if (tiled)
+                tflags[h] = valid ? 1u : 0u;
+        }
+        if (h < E && (d.edges[3 * size_t(h)] >= N || d.edges[3 * size_t(h) + 1] >= N))
+            bad = true;
     }
-    if (h < E && (d.edges[3 * size_t(h)] >= N || d.edges[3 * size_t(h) + 1] >= N))
-        bad = true;
     if (bad)
         atomicOr(ws.errs + it, 1u);
 }

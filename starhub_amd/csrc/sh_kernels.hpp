@@ -66,6 +66,9 @@ struct ScanJob
 };
 // Every launch takes a work map: wmap[i] is the first work-group of item i,
 // wmap[n] = n_wg.
+// Pixels (or tile-order indices) a thread of the flag kernel takes
+// (work-groups of 256 threads).
+constexpr uint32_t kPixPer = 4;
 // Valid-pixel flags (tiled: also in tile order) and the range checks.
 hipError_t launch_pixel_flags(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, bool tiled,
                               const IndexWs &ws, hipStream_t st);
