@@ -184,7 +184,7 @@ __device__ __forceinline__ void group_min_max(uint32_t &lo, uint32_t &hi, uint32
 template <int TP>
 struct TileRankLds
 {
-    static constexpr int kMaxNodes = 256;   // distinct nodes of a tile on this path
+    static constexpr int kMaxNodes = 128;   // distinct nodes of a tile on this path
     static constexpr int kPairWords = kMaxNodes * kMaxNodes / 32;
     static constexpr int kMaxPairs = 1024;  // distinct pairs of a tile on this path
     static constexpr int kMaskWords = TP / 32;
@@ -211,9 +211,12 @@ struct TileRankLds
 // and a contribution's place is its pair's first place plus the pair's pixels
 // before it (a pixel mask per pair); the block of a pair is looked up once.
 // A tile with more nodes or pairs than that LDS holds sorts its
-// (block << 8 | pixel) keys instead (a stable LDS radix sort).
+// (block << 8 | pixel) keys instead (a stable LDS radix sort).  Compiled for
+// five 128-pixel tiles a CU (at most 102 registers; the kernel waits mostly
+// on memory and barriers, so more tiles in flight pay).
 template <int K, int TP>
-__global__ __launch_bounds__(2 * TP) void k_tile_build(IndexGroup g, const uint32_t *__restrict__ wmap, IndexWs ws)
+__global__ __launch_bounds__(2 * TP, 5 * 128 / TP) void k_tile_build(IndexGroup g, const uint32_t *__restrict__ wmap,
+                                                                      IndexWs ws)
 {
     using G = TileGeom<K, TP>;
     using R = TileRankLds<TP>;
