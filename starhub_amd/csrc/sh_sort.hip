@@ -19,9 +19,21 @@ namespace sh
 using OnesweepOnly = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
                                                 rocprim::default_config, 0>;
 
+// Ten bits a pass for 17..20-bit keys (a batch's block keys): two passes
+// instead of three.  The sort kernel's shape is rocPRIM's own gfx950 choice
+// for 32-bit keys and values.
+using Onesweep10 = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 16>, 10,
+                                        rocprim::block_radix_rank_algorithm::match>,
+    0>;
+
 hipError_t sort_pairs_u32_u32(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout,
                               const uint32_t *vin, uint32_t *vout, uint32_t n, int end_bit, hipStream_t st)
 {
+    if (end_bit > 16 && end_bit <= 20)
+        return rocprim::radix_sort_pairs<Onesweep10>(temp, temp_bytes, kin, kout, vin, vout, n, 0u,
+                                                     unsigned(end_bit), st);
     return rocprim::radix_sort_pairs<OnesweepOnly>(temp, temp_bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit),
                                                    st);
 }
