@@ -157,6 +157,7 @@ private:
         kGmCompact,
         kGmSplit,
         kGmNode,
+        kGmRowBits,
         kGmTile,
         kGmLists,
         kNumGroupMaps

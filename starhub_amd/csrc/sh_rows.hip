@@ -32,6 +32,7 @@ __device__ __forceinline__ int find_solve_r(const uint32_t *__restrict__ begin, 
 }
 
 constexpr int kRowThreads = 256;
+static_assert(kRowThreads == 64 * kRowNodes, "k_row_bitmap: one wave per node");
 
 // Entry (c, q) of the x-side ARAP row [-[p]x, I]:  q < 3 -> -skew(p)[c][q],
 // q >= 3 -> delta(c, q - 3).  The y side is its negative with p = T_j x_j.
@@ -48,35 +49,38 @@ __device__ __forceinline__ float reg_elem(f3 p, int c, int q)
 }
 constexpr int kMaxWords = kPairBase / 32; // bitmap words for N <= 4096
 
-// One work-group per node: OR the nodes of every term of node i into an LDS
-// bitmap, drop i itself, count.  Also counts i's dense terms (they precede the
-// ARAP and node-translation terms in the sorted list) and the segments of
-// kSegTerms dense terms they split into.
+// One wave per node, kRowNodes nodes a work-group: OR the nodes of every term
+// of node i into the wave's LDS bitmap, drop i itself, count.  Also counts i's
+// dense terms (they precede the ARAP and node-translation terms in the sorted
+// list) and the segments of kSegTerms dense terms they split into.  A node's
+// list is a few hundred terms, so a wave per node keeps four times as many
+// nodes in flight as a work-group per node would.
 template <int K>
 __global__ __launch_bounds__(kRowThreads) void k_row_bitmap(IndexGroup g, const uint32_t *__restrict__ wmap,
                                                             IndexWs ws)
 {
-    __shared__ uint32_t bits[kMaxWords];
-    __shared__ uint32_t cnt[2];
+    __shared__ uint32_t bits_all[kRowNodes][kMaxWords];
     const int it = find_solve_r(wmap, g.n, blockIdx.x);
     const IndexItem item = g.items[it];
     const SolveDesc d = global_desc(g.sds[item.slot]);
-    const uint32_t i = blockIdx.x - wmap[it];
-    const uint32_t words = (uint32_t(d.N) + 31) / 32;
-    for (uint32_t w = threadIdx.x; w < words; w += blockDim.x)
+    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t i = (blockIdx.x - wmap[it]) * kRowNodes + wv;
+    const uint32_t N = uint32_t(d.N);
+    const bool live = i < N;
+    uint32_t *bits = bits_all[wv];
+    const uint32_t words = (N + 31) / 32;
+    for (uint32_t w = lane; w < words; w += 64)
         bits[w] = 0u;
-    if (threadIdx.x < 2)
-        cnt[threadIdx.x] = 0u;
     __syncthreads();
-    const uint32_t beg = d.n2t_off[i], end = d.n2t_off[i + 1];
+    const uint32_t beg = live ? d.n2t_off[i] : 0u, end = live ? d.n2t_off[i + 1] : 0u;
     const uint32_t P = d.P, E = d.E;
     uint32_t dense = 0;
-    // kU consecutive terms of the list per thread and round: their term ids,
+    // kU consecutive terms of the list per lane and round: their term ids,
     // then their node lists, are loaded before any is used, and a dense term
     // whose node list equals the previous one's (neighbouring pixels mostly
     // share their surfels' nodes) sets no bits
     constexpr int kU = 4;
-    for (uint32_t t0 = beg + threadIdx.x * kU; t0 < end; t0 += kU * blockDim.x)
+    for (uint32_t t0 = beg + lane * kU; t0 < end; t0 += kU * 64)
     {
         uint32_t tt[kU];
 #pragma unroll
@@ -137,27 +141,32 @@ __global__ __launch_bounds__(kRowThreads) void k_row_bitmap(IndexGroup g, const 
             }
         }
     }
-    // integer sums: order independent
-    atomicAdd(&cnt[0], dense);
     __syncthreads();
-    if (threadIdx.x == 0)
-        bits[i >> 5] &= ~(1u << (i & 31));
-    __syncthreads();
+    if (!live)
+        return;
     uint32_t pc = 0;
     uint32_t *__restrict__ row_bits = ws.bitmap + size_t(item.node0 + i) * kMaxWords;
-    for (uint32_t w = threadIdx.x; w < words; w += blockDim.x)
+    for (uint32_t w = lane; w < words; w += 64)
     {
-        pc += __popc(bits[w]);
-        row_bits[w] = bits[w];
+        uint32_t b = bits[w];
+        if (w == (i >> 5))
+            b &= ~(1u << (i & 31));
+        pc += __popc(b);
+        row_bits[w] = b;
     }
-    atomicAdd(&cnt[1], pc);
-    __syncthreads();
-    if (threadIdx.x == 0)
+    // integer sums over the wave: order independent
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
     {
-        ws.rowlen[item.node0 + i] = cnt[1];
-        d.dense_cnt[i] = cnt[0];
-        ws.segcnt[item.node0 + i] = (cnt[0] + kSegTerms - 1) / kSegTerms;
-        atomicMax(&ws.meta[size_t(it) * 16], cnt[1]);
+        pc += __shfl_xor(pc, o, 64);
+        dense += __shfl_xor(dense, o, 64);
+    }
+    if (lane == 0)
+    {
+        ws.rowlen[item.node0 + i] = pc;
+        d.dense_cnt[i] = dense;
+        ws.segcnt[item.node0 + i] = (dense + kSegTerms - 1) / kSegTerms;
+        atomicMax(&ws.meta[size_t(it) * 16], pc);
     }
 }
 
