@@ -38,9 +38,18 @@ hipError_t sort_pairs_u32_u32(void *temp, size_t &temp_bytes, const uint32_t *ki
                                                    st);
 }
 
+// The flag stage's scans cover every pixel of a group (16 M on a 32-solve
+// batch). rocPRIM has no tuned scan for gfx950 (its fallback is 256 threads x
+// 16 items); 32 items a thread halves the blocks the decoupled look-back
+// chains through: 72 -> 48 us a scan on that batch.
+using ScanConfig =
+    rocprim::scan_config<256, 32, rocprim::block_load_method::block_load_transpose,
+                         rocprim::block_store_method::block_store_transpose,
+                         rocprim::block_scan_algorithm::using_warp_scan>;
+
 hipError_t exclusive_scan_u32(void *temp, size_t &temp_bytes, const uint32_t *in, uint32_t *out, uint32_t n,
                               hipStream_t st)
 {
-    return rocprim::exclusive_scan(temp, temp_bytes, in, out, 0u, n, rocprim::plus<uint32_t>(), st);
+    return rocprim::exclusive_scan<ScanConfig>(temp, temp_bytes, in, out, 0u, n, rocprim::plus<uint32_t>(), st);
 }
 } // namespace sh
