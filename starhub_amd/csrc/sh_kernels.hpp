@@ -252,7 +252,8 @@ hipError_t launch_tile_build(int K, uint32_t tile_pix, const IndexGroup &g, cons
 // and the (blk0 + block, slot) pairs of the block sort at sb0.
 hipError_t launch_tile_compact(int K, uint32_t tile_pix, const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg,
                                const IndexWs &ws, uint32_t *sort_keys, uint32_t *sort_vals, hipStream_t st);
-// One work-group per node: the transpose map of the row's blocks.
+// One wave per node, kRowNodes nodes a work-group: the transpose map of the
+// row's blocks.
 hipError_t launch_block_transpose(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const IndexWs &ws,
                                   hipStream_t st);
 // fused: the tile kernel builds the FP32 per-pixel records itself (and the

@@ -19,9 +19,12 @@ namespace sh
 using OnesweepOnly = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
                                                 rocprim::default_config, 0>;
 
-// Ten bits a pass for 17..20-bit keys (a batch's block keys): two passes
-// instead of three.  The sort kernel's shape is rocPRIM's own gfx950 choice
-// for 32-bit keys and values.
+// Ten bits a pass for 9..20-bit keys: two passes (three for a batch's 17..20-bit
+// block keys at rocPRIM's eight), and for the 16-bit node keys the same two
+// passes run faster than rocPRIM's default 8-bit kernel, which keeps a
+// per-thread array in scratch (two passes 511 -> 439 us on a 32-solve batch).
+// The sort kernel's shape is rocPRIM's own gfx950 choice for 32-bit keys and
+// values.
 using Onesweep10 = rocprim::radix_sort_config<
     rocprim::default_config, rocprim::default_config,
     rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 16>, 10,
@@ -31,7 +34,7 @@ using Onesweep10 = rocprim::radix_sort_config<
 hipError_t sort_pairs_u32_u32(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout,
                               const uint32_t *vin, uint32_t *vout, uint32_t n, int end_bit, hipStream_t st)
 {
-    if (end_bit > 16 && end_bit <= 20)
+    if (end_bit > 8 && end_bit <= 20)
         return rocprim::radix_sort_pairs<Onesweep10>(temp, temp_bytes, kin, kout, vin, vout, n, 0u,
                                                      unsigned(end_bit), st);
     return rocprim::radix_sort_pairs<OnesweepOnly>(temp, temp_bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit),

@@ -566,7 +566,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_compact(IndexGroup g, con
         d.tc_slot_first[d.tile_sbase[t + 1]] = d.tile_cbase[t + 1];
 }
 
-// For every block (i, c) of row i: the index of its transpose (c, i), in row c.
+// One wave per node, kRowNodes nodes a work-group: for every block (i, c) of
+// row i, the index of its transpose (c, i), in row c.
+static_assert(kTileThreads == 64 * kRowNodes, "k_block_transpose: one wave per node");
 __global__ __launch_bounds__(kTileThreads) void k_block_transpose(IndexGroup g, const uint32_t *__restrict__ wmap,
                                                                   IndexWs ws)
 {
@@ -574,9 +576,11 @@ __global__ __launch_bounds__(kTileThreads) void k_block_transpose(IndexGroup g, 
     const IndexItem item = g.items[it];
     const SolveDesc d = global_desc(g.sds[item.slot]);
     const uint2 *__restrict__ bitpre = ws.bitpre + size_t(item.node0) * kBitmapWords;
-    const uint32_t i = blockIdx.x - wmap[it];
+    const uint32_t i = (blockIdx.x - wmap[it]) * kRowNodes + (threadIdx.x >> 6);
+    if (i >= uint32_t(d.N))
+        return;
     const uint32_t base = d.row_ptr[i] + i, L1 = d.row_ptr[i + 1] - d.row_ptr[i] + 1;
-    for (uint32_t c = threadIdx.x; c < L1; c += kTileThreads)
+    for (uint32_t c = threadIdx.x & 63u; c < L1; c += 64)
     {
         const uint32_t col = d.blk_col[base + c];
         d.blk_transpose[base + c] = col == i ? base + c : block_of_pair(d.row_ptr, bitpre, col, i); // block (col, i) of row col
