@@ -653,7 +653,9 @@ int Engine::build_group(const uint32_t *list, uint32_t n, hipStream_t st, int *r
             return SH_E_STATE; // never: build_indexes sized the group
         const uint32_t wg_compact = set_map(kGmCompact, [&](size_t i) {
             const SolveDesc &d = desc_of(i);
-            return std::max(1u, div_up(uint32_t(d.W) * uint32_t(d.H), 256u)) + div_up(uint32_t(d.E + d.N), 256u);
+            // the pixel work-groups walk the tile-order index space when tiled
+            const uint32_t walk = tiled ? tile_order_size(uint32_t(d.W), uint32_t(d.H)) : uint32_t(d.W) * uint32_t(d.H);
+            return std::max(1u, div_up(walk, 256u)) + div_up(uint32_t(d.E + d.N), 256u);
         });
         const uint32_t wg_split = set_map(kGmSplit, [&](size_t i) {
             const SolveDesc &d = desc_of(i);

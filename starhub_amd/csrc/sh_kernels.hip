@@ -154,8 +154,13 @@ __global__ __launch_bounds__(256) void k_compact_pixels(IndexGroup g, const uint
     const IndexItem item = g.items[it];
     const SolveDesc d = global_desc(g.sds[item.slot]);
     const uint32_t blk = blockIdx.x - wmap[it];
-    const uint32_t W = uint32_t(d.W), n = W * uint32_t(d.H);
-    const uint32_t n_pix_wg = max(1u, (n + 255u) / 256u);
+    const uint32_t W = uint32_t(d.W), H = uint32_t(d.H), n = W * H;
+    // tiled, the threads walk the tile-order index space: a work-group's 256
+    // indices are an 8-row band of one 32x32 block, so its tile-order records
+    // are one contiguous run and its raster records eight runs of a row each
+    const uint32_t nbx = (W + 31u) / 32u;
+    const uint32_t n_walk = tiled ? nbx * ((H + 31u) / 32u) * 1024u : n;
+    const uint32_t n_pix_wg = max(1u, (n_walk + 255u) / 256u);
     const uint32_t P = d.P, E = d.E, N = d.N;
     if (blk >= n_pix_wg)
     {
@@ -197,11 +202,29 @@ __global__ __launch_bounds__(256) void k_compact_pixels(IndexGroup g, const uint
         else if (t == 21)
             ws.tncontrib[item.tc0 + nt] = 0u;
     }
-    const uint32_t i = blk * blockDim.x + threadIdx.x;
-    if (i >= n || ws.flags[item.m0 + i] == 0u)
+    const uint32_t h = blk * blockDim.x + threadIdx.x; // walk index
+    uint32_t i = h, x, y;                              // raster index and pixel
+    if (tiled)
+    {
+        if (h >= n_walk)
+            return;
+        const uint32_t b = h >> 10, m = h & 1023u;
+        x = (b % nbx) * 32u + squeeze5(m);
+        y = (b / nbx) * 32u + squeeze5(m >> 1);
+        if (x >= W || y >= H)
+            return;
+        i = y * W + x;
+    }
+    else
+    {
+        if (h >= n)
+            return;
+        x = i % W;
+        y = i / W;
+    }
+    if (ws.flags[item.m0 + i] == 0u)
         return;
     const uint32_t o = ws.pos[item.m0 + i] - ws.pos[item.m0];
-    const uint32_t x = i % W, y = i / W;
     const uint32_t s = d.index_map[i];
     float fx = 0.f, fy = 0.f;
     if (d.flow)
@@ -282,7 +305,7 @@ __global__ __launch_bounds__(256) void k_compact_pixels(IndexGroup g, const uint
     if (tiled)
     {
         // the GN phase's copy, at the pixel's tile-order position
-        const uint32_t ot = ws.tpos[item.h0 + tile_order(x, y, (W + 31u) / 32u)] - ws.tpos[item.h0];
+        const uint32_t ot = ws.tpos[item.h0 + h] - ws.tpos[item.h0]; // h = tile_order(x, y, nbx)
         d.gpix[ot] = px;
         put(d.gpknn, d.gpw, ot);
     }
