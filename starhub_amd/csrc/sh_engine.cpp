@@ -663,8 +663,12 @@ int Engine::build_group(const uint32_t *list, uint32_t n, hipStream_t st, int *r
             return div_up(uint32_t(d.P) * uint32_t(K_) + 2u * uint32_t(d.E) + uint32_t(d.N), 256u * kSplitPer);
         });
         const uint32_t wg_node = set_map(kGmNode, [&](size_t i) { return uint32_t(desc_of(i).N); });
+        // a wave per node once the group's nodes fill the device four times over
+        // (256 CUs, eight 256-thread work-groups each); a work-group per node
+        // below that (a single solve of a thousand nodes)
+        const uint32_t row_npw = node0 >= 4u * 256u * 8u ? kRowNodes : 1u;
         const uint32_t wg_rowbits =
-            set_map(kGmRowBits, [&](size_t i) { return div_up(uint32_t(desc_of(i).N), kRowNodes); });
+            set_map(kGmRowBits, [&](size_t i) { return div_up(uint32_t(desc_of(i).N), row_npw); });
         const uint32_t wg_tile = set_map(kGmTile, [&](size_t i) { return uint32_t(desc_of(i).n_tiles); });
         if ((rc = upload_group(1, st)) != SH_OK)
             return rc;
@@ -679,7 +683,7 @@ int Engine::build_group(const uint32_t *list, uint32_t n, hipStream_t st, int *r
             SH_HIP(sort_pairs_u32_u32(g.sort_temp, tb, g.ka, g.kb, g.va, g.vb, uint32_t(kv0), key_bits(node0), st));
         }
         SH_HIP(launch_n2t_split(gv, group_map(kGmSplit), wg_split, g.kb, g.vb, st));
-        SH_HIP(launch_row_bitmap(K_, gv, group_map(kGmRowBits), wg_rowbits, g.ws, st));
+        SH_HIP(launch_row_bitmap(K_, row_npw, gv, group_map(kGmRowBits), wg_rowbits, g.ws, st));
         SH_HIP(launch_scan_jobs(jobs, uint32_t(g_jobs_[0].size()), st));
         // the word prefix counts of the row bitmaps, and the block columns (the
         // reference index builds them from its pair index)

@@ -133,12 +133,12 @@ hipError_t launch_skin(uint32_t K, const float4 *surfels, uint32_t S, const floa
 hipError_t launch_node_graph(uint32_t K, const float4 *nodes, uint32_t N, float r2, uint16_t *node_knn,
                              float *node_spatial, uint16_t *edges, float *node_connect, hipStream_t st);
 
-// One wave per node, kRowNodes nodes a work-group of 256 threads: the row
-// bitmap, row length, dense-term count and segment count; the longest row
-// into the item's meta[0].
+// nodes_per_wg (1, or kRowNodes: one wave each) nodes a work-group of 256
+// threads: the row bitmap, row length, dense-term count and segment count;
+// the longest row into the item's meta[0].
 constexpr uint32_t kRowNodes = 4;
-hipError_t launch_row_bitmap(int K, const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const IndexWs &ws,
-                             hipStream_t st);
+hipError_t launch_row_bitmap(int K, uint32_t nodes_per_wg, const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg,
+                             const IndexWs &ws, hipStream_t st);
 // One work-group per node: the bitmap words with their word prefix counts
 // (bitpre) and, with cols, the block columns from them and row_ptr.
 hipError_t launch_row_cols(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const IndexWs &ws, bool cols,

@@ -32,7 +32,6 @@ __device__ __forceinline__ int find_solve_r(const uint32_t *__restrict__ begin, 
 }
 
 constexpr int kRowThreads = 256;
-static_assert(kRowThreads == 64 * kRowNodes, "k_row_bitmap: one wave per node");
 
 // Entry (c, q) of the x-side ARAP row [-[p]x, I]:  q < 3 -> -skew(p)[c][q],
 // q >= 3 -> delta(c, q - 3).  The y side is its negative with p = T_j x_j.
@@ -49,38 +48,44 @@ __device__ __forceinline__ float reg_elem(f3 p, int c, int q)
 }
 constexpr int kMaxWords = kPairBase / 32; // bitmap words for N <= 4096
 
-// One wave per node, kRowNodes nodes a work-group: OR the nodes of every term
-// of node i into the wave's LDS bitmap, drop i itself, count.  Also counts i's
-// dense terms (they precede the ARAP and node-translation terms in the sorted
-// list) and the segments of kSegTerms dense terms they split into.  A node's
-// list is a few hundred terms, so a wave per node keeps four times as many
-// nodes in flight as a work-group per node would.
-template <int K>
+// TPN threads per node (a wave or the whole work-group), kRowThreads / TPN
+// nodes a work-group: OR the nodes of every term of node i into the node's LDS
+// bitmap, drop i itself, count.  Also counts i's dense terms (they precede the
+// ARAP and node-translation terms in the sorted list) and the segments of
+// kSegTerms dense terms they split into.  A node's list is a few hundred
+// terms: a wave per node keeps four times as many nodes in flight, which pays
+// when the group has enough nodes to fill the device.
+template <int K, int TPN>
 __global__ __launch_bounds__(kRowThreads) void k_row_bitmap(IndexGroup g, const uint32_t *__restrict__ wmap,
                                                             IndexWs ws)
 {
-    __shared__ uint32_t bits_all[kRowNodes][kMaxWords];
+    static_assert(TPN % 64 == 0 && kRowThreads % TPN == 0, "whole waves per node");
+    constexpr int kNodes = kRowThreads / TPN;
+    __shared__ uint32_t bits_all[kNodes][kMaxWords];
+    __shared__ uint32_t cnt[kNodes][2];
     const int it = find_solve_r(wmap, g.n, blockIdx.x);
     const IndexItem item = g.items[it];
     const SolveDesc d = global_desc(g.sds[item.slot]);
-    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    const uint32_t i = (blockIdx.x - wmap[it]) * kRowNodes + wv;
+    const uint32_t slot = threadIdx.x / TPN, lane = threadIdx.x % TPN;
+    const uint32_t i = (blockIdx.x - wmap[it]) * kNodes + slot;
     const uint32_t N = uint32_t(d.N);
     const bool live = i < N;
-    uint32_t *bits = bits_all[wv];
+    uint32_t *bits = bits_all[slot];
     const uint32_t words = (N + 31) / 32;
-    for (uint32_t w = lane; w < words; w += 64)
+    for (uint32_t w = lane; w < words; w += TPN)
         bits[w] = 0u;
+    if (lane < 2)
+        cnt[slot][lane] = 0u;
     __syncthreads();
     const uint32_t beg = live ? d.n2t_off[i] : 0u, end = live ? d.n2t_off[i + 1] : 0u;
     const uint32_t P = d.P, E = d.E;
     uint32_t dense = 0;
-    // kU consecutive terms of the list per lane and round: their term ids,
+    // kU consecutive terms of the list per thread and round: their term ids,
     // then their node lists, are loaded before any is used, and a dense term
     // whose node list equals the previous one's (neighbouring pixels mostly
     // share their surfels' nodes) sets no bits
     constexpr int kU = 4;
-    for (uint32_t t0 = beg + lane * kU; t0 < end; t0 += kU * 64)
+    for (uint32_t t0 = beg + lane * kU; t0 < end; t0 += kU * TPN)
     {
         uint32_t tt[kU];
 #pragma unroll
@@ -142,31 +147,39 @@ __global__ __launch_bounds__(kRowThreads) void k_row_bitmap(IndexGroup g, const 
         }
     }
     __syncthreads();
-    if (!live)
-        return;
     uint32_t pc = 0;
-    uint32_t *__restrict__ row_bits = ws.bitmap + size_t(item.node0 + i) * kMaxWords;
-    for (uint32_t w = lane; w < words; w += 64)
+    if (live)
     {
-        uint32_t b = bits[w];
-        if (w == (i >> 5))
-            b &= ~(1u << (i & 31));
-        pc += __popc(b);
-        row_bits[w] = b;
+        uint32_t *__restrict__ row_bits = ws.bitmap + size_t(item.node0 + i) * kMaxWords;
+        for (uint32_t w = lane; w < words; w += TPN)
+        {
+            uint32_t b = bits[w];
+            if (w == (i >> 5))
+                b &= ~(1u << (i & 31));
+            pc += __popc(b);
+            row_bits[w] = b;
+        }
     }
-    // integer sums over the wave: order independent
+    // integer sums over each wave, then over the node's waves: order independent
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
     {
         pc += __shfl_xor(pc, o, 64);
         dense += __shfl_xor(dense, o, 64);
     }
-    if (lane == 0)
+    if ((threadIdx.x & 63u) == 0)
     {
-        ws.rowlen[item.node0 + i] = pc;
-        d.dense_cnt[i] = dense;
-        ws.segcnt[item.node0 + i] = (dense + kSegTerms - 1) / kSegTerms;
-        atomicMax(&ws.meta[size_t(it) * 16], pc);
+        atomicAdd(&cnt[slot][0], dense);
+        atomicAdd(&cnt[slot][1], pc);
+    }
+    __syncthreads();
+    if (live && lane == 0)
+    {
+        const uint32_t n_dense = cnt[slot][0], len = cnt[slot][1];
+        ws.rowlen[item.node0 + i] = len;
+        d.dense_cnt[i] = n_dense;
+        ws.segcnt[item.node0 + i] = (n_dense + kSegTerms - 1) / kSegTerms;
+        atomicMax(&ws.meta[size_t(it) * 16], len);
     }
 }
 
@@ -552,15 +565,23 @@ __global__ __launch_bounds__(kRowThreads) void k_rows_combine(const SolveDesc *_
     }
 }
 
-hipError_t launch_row_bitmap(int K, const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const IndexWs &ws,
-                             hipStream_t st)
+hipError_t launch_row_bitmap(int K, uint32_t nodes_per_wg, const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg,
+                             const IndexWs &ws, hipStream_t st)
 {
+    static_assert(kRowThreads == 256 && kRowNodes == 4, "launch shapes");
     if (n_wg == 0)
         return hipSuccess;
-    if (K == 8)
-        hipLaunchKernelGGL(k_row_bitmap<8>, dim3(n_wg), dim3(kRowThreads), 0, st, g, wmap, ws);
+    if (nodes_per_wg != 1 && nodes_per_wg != kRowNodes)
+        return hipErrorInvalidValue;
+    const bool wave = nodes_per_wg == kRowNodes;
+    if (K == 8 && wave)
+        hipLaunchKernelGGL((k_row_bitmap<8, 64>), dim3(n_wg), dim3(kRowThreads), 0, st, g, wmap, ws);
+    else if (K == 8)
+        hipLaunchKernelGGL((k_row_bitmap<8, 256>), dim3(n_wg), dim3(kRowThreads), 0, st, g, wmap, ws);
+    else if (wave)
+        hipLaunchKernelGGL((k_row_bitmap<4, 64>), dim3(n_wg), dim3(kRowThreads), 0, st, g, wmap, ws);
     else
-        hipLaunchKernelGGL(k_row_bitmap<4>, dim3(n_wg), dim3(kRowThreads), 0, st, g, wmap, ws);
+        hipLaunchKernelGGL((k_row_bitmap<4, 256>), dim3(n_wg), dim3(kRowThreads), 0, st, g, wmap, ws);
     return hipGetLastError();
 }
 
