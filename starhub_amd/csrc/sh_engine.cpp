@@ -662,7 +662,7 @@ int Engine::build_group(const uint32_t *list, uint32_t n, hipStream_t st, int *r
             // kSplitPer sorted pairs a thread (every node has its translation term)
             return div_up(uint32_t(d.P) * uint32_t(K_) + 2u * uint32_t(d.E) + uint32_t(d.N), 256u * kSplitPer);
         });
-        const uint32_t wg_node = set_map(kGmNode, [&](size_t i) { return uint32_t(desc_of(i).N); });
+        const uint32_t wg_node = set_map(kGmNode, [&](size_t i) { return div_up(uint32_t(desc_of(i).N), kRowNodes); });
         // a wave per node once the group's nodes fill the device four times over
         // (256 CUs, eight 256-thread work-groups each); a work-group per node
         // below that (a single solve of a thousand nodes)
@@ -755,8 +755,7 @@ int Engine::build_group(const uint32_t *list, uint32_t n, hipStream_t st, int *r
             const SolveDesc &d = desc_of(i);
             return div_up(std::max(n_slots[i], uint32_t(d.N) + 2u * uint32_t(d.U) + 1u), 256u);
         });
-        const uint32_t wg_node3 =
-            set_map(kGmRowBits, [&](size_t i) { return div_up(uint32_t(desc_of(i).N), kRowNodes); });
+        const uint32_t wg_node3 = set_map(kGmNode, [&](size_t i) { return div_up(uint32_t(desc_of(i).N), kRowNodes); });
         if ((rc = upload_group(2, st)) != SH_OK)
             return rc;
         gv = group_view();
@@ -768,7 +767,7 @@ int Engine::build_group(const uint32_t *list, uint32_t n, hipStream_t st, int *r
             SH_HIP(sort_pairs_u32_u32(g.sort_temp, tb, g.ka, g.kb, g.va, g.vb, uint32_t(sb0), key_bits(blk0), st));
         }
         SH_HIP(launch_slot_lists(gv, group_map(kGmLists), wg_lists, g.kb, g.vb, st));
-        SH_HIP(launch_block_transpose(gv, group_map(kGmRowBits), wg_node3, g.ws, st));
+        SH_HIP(launch_block_transpose(gv, group_map(kGmNode), wg_node3, g.ws, st));
         for (uint32_t s : g_slots_)
             slots_[s].indexed = true;
         return SH_OK;

@@ -139,8 +139,9 @@ hipError_t launch_node_graph(uint32_t K, const float4 *nodes, uint32_t N, float 
 constexpr uint32_t kRowNodes = 4;
 hipError_t launch_row_bitmap(int K, uint32_t nodes_per_wg, const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg,
                              const IndexWs &ws, hipStream_t st);
-// One work-group per node: the bitmap words with their word prefix counts
-// (bitpre) and, with cols, the block columns from them and row_ptr.
+// One wave per node, kRowNodes nodes a work-group: the bitmap words with
+// their word prefix counts (bitpre) and, with cols, the block columns from
+// them and row_ptr.
 hipError_t launch_row_cols(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const IndexWs &ws, bool cols,
                            hipStream_t st);
 // Dense terms per segment of a node row (one work-group of the segment kernel).

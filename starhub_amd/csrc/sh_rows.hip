@@ -183,56 +183,52 @@ __global__ __launch_bounds__(kRowThreads) void k_row_bitmap(IndexGroup g, const 
     }
 }
 
-// Row i's bitmap words with the neighbour count of the words before each
-// (bitpre, for the block lookups of the tile build and the transpose map)
-// and, with cols, its ascending block columns: diagonal block first, then
-// the neighbours (the order of the reference's sorted symmetric keys).
+// One wave per node, kRowNodes nodes a work-group: row i's bitmap words with
+// the neighbour count of the words before each (bitpre, for the block lookups
+// of the tile build and the transpose map) and, with cols, its ascending block
+// columns: diagonal block first, then the neighbours (the order of the
+// reference's sorted symmetric keys).  A row has at most 128 words: two a lane.
 __global__ __launch_bounds__(kRowThreads) void k_row_cols(IndexGroup g, const uint32_t *__restrict__ wmap, IndexWs ws,
                                                           int cols)
 {
-    __shared__ uint32_t pre[kMaxWords];
+    static_assert(kMaxWords == 128 && kRowThreads == 64 * kRowNodes, "two bitmap words a lane, a wave per node");
     const int it = find_solve_r(wmap, g.n, blockIdx.x);
     const IndexItem item = g.items[it];
     const SolveDesc d = global_desc(g.sds[item.slot]);
-    const uint32_t i = blockIdx.x - wmap[it];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t i = (blockIdx.x - wmap[it]) * kRowNodes + (threadIdx.x >> 6);
+    if (i >= uint32_t(d.N))
+        return; // the whole wave
     const uint32_t words = (uint32_t(d.N) + 31) / 32;
     const uint32_t *__restrict__ row_bits = ws.bitmap + size_t(item.node0 + i) * kMaxWords;
     uint2 *__restrict__ row_pre = ws.bitpre + size_t(item.node0 + i) * kMaxWords;
-    // exclusive prefix of the word popcounts (words <= 128, one pass by wave 0)
-    if (threadIdx.x < 64)
-    {
-        const uint32_t w0 = 2 * threadIdx.x, w1 = w0 + 1;
-        const uint32_t b0 = (w0 < words) ? row_bits[w0] : 0u;
-        const uint32_t b1 = (w1 < words) ? row_bits[w1] : 0u;
-        const uint32_t c0 = __popc(b0), c1 = __popc(b1);
-        uint32_t v = c0 + c1;
-        // inclusive wave scan
+    const uint32_t w0 = 2 * lane, w1 = w0 + 1;
+    const uint32_t b0 = (w0 < words) ? row_bits[w0] : 0u;
+    const uint32_t b1 = (w1 < words) ? row_bits[w1] : 0u;
+    const uint32_t c0 = __popc(b0), c1 = __popc(b1);
+    // exclusive prefix of the word popcounts: an inclusive wave scan of the
+    // lane pairs' counts
+    uint32_t v = c0 + c1;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1)
-        {
-            const uint32_t u = __shfl_up(v, o, 64);
-            if (int(threadIdx.x) >= o)
-                v += u;
-        }
-        const uint32_t ex = v - c0 - c1;
-        pre[w0] = ex;
-        if (w1 < kMaxWords)
-            pre[w1] = ex + c0;
+    for (int o = 1; o < 64; o <<= 1)
+    {
+        const uint32_t u = __shfl_up(v, o, 64);
+        if (int(lane) >= o)
+            v += u;
     }
-    __syncthreads();
-    for (uint32_t w = threadIdx.x; w < words; w += blockDim.x)
-        row_pre[w] = make_uint2(row_bits[w], pre[w]);
+    const uint32_t ex = v - c0 - c1;
+    if (w0 < words)
+        row_pre[w0] = make_uint2(b0, ex);
+    if (w1 < words)
+        row_pre[w1] = make_uint2(b1, ex + c0);
     if (!cols)
         return; // uniform
     // rows beyond the block capacity are left unwritten; the host rejects
     // the solve (capacity) after reading U back
     const uint32_t base = d.row_ptr[i] + i;
-    if (threadIdx.x == 0 && base < d.blk_cap)
+    if (lane == 0 && base < d.blk_cap)
         d.blk_col[base] = i;
-    for (uint32_t w = threadIdx.x; w < words; w += blockDim.x)
-    {
-        uint32_t b = row_bits[w];
-        uint32_t o = base + 1 + pre[w];
+    auto emit = [&](uint32_t b, uint32_t w, uint32_t o) {
         while (b)
         {
             const uint32_t bit = __ffs(b) - 1;
@@ -241,7 +237,9 @@ __global__ __launch_bounds__(kRowThreads) void k_row_cols(IndexGroup g, const ui
             ++o;
             b &= b - 1;
         }
-    }
+    };
+    emit(b0, w0, base + 1 + ex);
+    emit(b1, w1, base + 1 + ex + c0);
 }
 
 // (a, b) of packed upper-triangle entry e of a symmetric 6x6 block.
