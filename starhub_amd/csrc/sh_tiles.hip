@@ -1979,7 +1979,9 @@ size_t tiles_combine_lds(uint32_t max_nodes, uint32_t max_row_len, bool wave)
 // ids, stable: tile order within a block): list_items[l] and
 // slot_pos[list_items[l]] = l for l < S (each slot's position in the block
 // lists, where k_tiles puts its partial), and list_off[b] = the first
-// position of block b for b <= N + 2U (a lower bound in the sorted blocks).
+// position of block b for b <= N + 2U (a lower bound in the sorted blocks):
+// sorted position l writes it for the blocks after the key below it up to its
+// own, the last position also for the blocks after its key (no searches).
 __global__ __launch_bounds__(256) void k_slot_lists(IndexGroup g, const uint32_t *__restrict__ wmap,
                                                     const uint32_t *__restrict__ skeys,
                                                     const uint32_t *__restrict__ svals)
@@ -1990,26 +1992,25 @@ __global__ __launch_bounds__(256) void k_slot_lists(IndexGroup g, const uint32_t
     const uint32_t S = d.tile_sbase[d.n_tiles];
     const uint32_t n_blocks = uint32_t(d.N) + 2u * uint32_t(d.U) + 1u;
     const uint32_t l = (blockIdx.x - wmap[it]) * blockDim.x + threadIdx.x;
-    if (l < S)
+    if (S == 0)
     {
-        const uint32_t v = svals[item.sb0 + l];
-        d.list_items[l] = v;
-        d.slot_pos[v] = l;
+        if (l < n_blocks)
+            d.list_off[l] = 0u;
+        return;
     }
-    if (l < n_blocks)
-    {
-        const uint32_t key = item.blk0 + l;
-        uint32_t lo = item.sb0, hi = item.sb0 + S;
-        while (lo < hi)
-        {
-            const uint32_t m = (lo + hi) >> 1;
-            if (skeys[m] < key)
-                lo = m + 1;
-            else
-                hi = m;
-        }
-        d.list_off[l] = lo - item.sb0;
-    }
+    if (l >= S)
+        return;
+    const uint32_t v = svals[item.sb0 + l];
+    d.list_items[l] = v;
+    d.slot_pos[v] = l;
+    // a slot's block is below N + 2U; the bound keeps every store in the array
+    const uint32_t key = min(skeys[item.sb0 + l] - item.blk0, n_blocks - 1u);
+    const uint32_t first = l == 0 ? 0u : skeys[item.sb0 + l - 1] - item.blk0 + 1u;
+    for (uint32_t b = first; b <= key; ++b)
+        d.list_off[b] = l;
+    if (l + 1 == S)
+        for (uint32_t b = key + 1; b < n_blocks; ++b)
+            d.list_off[b] = S;
 }
 
 hipError_t launch_slot_lists(const IndexGroup &g, const uint32_t *wmap, uint32_t n_wg, const uint32_t *skeys,
