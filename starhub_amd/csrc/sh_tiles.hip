@@ -554,28 +554,32 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_compact(IndexGroup g, con
     const size_t tbase = size_t(item.t0 + t) * size_t(G::kEnt);
     const uint32_t c0 = d.tile_cbase[t], nc = d.tile_cbase[t + 1] - c0;
     const uint32_t s0 = d.tile_sbase[t], ns = d.tile_sbase[t + 1] - s0;
-    // the contributions two a 32-bit word: the source is word aligned (even
-    // tile stride), the destination after one odd leading entry; with that
-    // entry, destination word j is the upper half of source word j and the
-    // lower half of word j + 1 (the last word read holds entry nc at most,
-    // inside the tile's stride when nc is odd)
-    static_assert(G::kEnt % 2 == 0, "word-aligned tile stride");
+    // the contributions four a 64-bit word: the source is word aligned (the
+    // tile stride is a multiple of four), the destination after h < 4 leading
+    // entries; destination word j then takes its entries from source words j
+    // and j + 1, and the last word it reads holds entry nc - 1, inside the
+    // tile's stride
+    static_assert(G::kEnt % 4 == 0, "word-aligned tile stride");
     const uint16_t *__restrict__ src = ws.tcontrib + tbase;
     uint16_t *__restrict__ dst = d.tc_contrib + c0;
-    const uint32_t h = (nc > 0 && (c0 & 1u)) ? 1u : 0u; // entries before the first aligned word
-    const uint32_t m = (nc - h) / 2;                   // destination words
-    const uint32_t *__restrict__ sw = reinterpret_cast<const uint32_t *>(src);
-    uint32_t *__restrict__ dw = reinterpret_cast<uint32_t *>(dst + h);
+    const uint32_t h = min(nc, (4u - (c0 & 3u)) & 3u); // entries before the first aligned word
+    const uint32_t m = (nc - h) / 4;                  // destination words
+    const uint32_t r = nc - h - 4 * m;                // entries after the last
+    const uint64_t *__restrict__ sw = reinterpret_cast<const uint64_t *>(src);
+    uint64_t *__restrict__ dw = reinterpret_cast<uint64_t *>(dst + h);
     if (h == 0)
         for (uint32_t j = tid; j < m; j += kTileThreads)
             dw[j] = sw[j];
     else
+    {
+        const uint32_t lo = 16u * h, hi = 64u - lo;
         for (uint32_t j = tid; j < m; j += kTileThreads)
-            dw[j] = (sw[j] >> 16) | (sw[j + 1] << 16);
-    if (tid == 0 && h)
-        dst[0] = src[0];
-    if (tid == 0 && ((nc - h) & 1u))
-        dst[nc - 1] = src[nc - 1];
+            dw[j] = (sw[j] >> lo) | (sw[j + 1] << hi);
+    }
+    if (tid < h)
+        dst[tid] = src[tid];
+    if (tid < r)
+        dst[h + 4 * m + tid] = src[h + 4 * m + tid];
     for (uint32_t s = tid; s < ns; s += kTileThreads)
     {
         d.tc_slot_first[s0 + s] = c0 + ws.tslot_first[tbase + s];
