@@ -663,9 +663,9 @@ int Engine::build_group(const uint32_t *list, uint32_t n, hipStream_t st, int *r
             return div_up(uint32_t(d.P) * uint32_t(K_) + 2u * uint32_t(d.E) + uint32_t(d.N), 256u * kSplitPer);
         });
         const uint32_t wg_node = set_map(kGmNode, [&](size_t i) { return div_up(uint32_t(desc_of(i).N), kRowNodes); });
-        // a wave per node once the group's nodes fill the device four times over
-        // (256 CUs, eight 256-thread work-groups each); a work-group per node
-        // below that (a single solve of a thousand nodes)
+        // a wave per node once that launch fills the device (256 CUs x eight
+        // 256-thread work-groups x four nodes = 8,192 nodes); a work-group per
+        // node below that (a single solve of a thousand nodes)
         const uint32_t row_npw = node0 >= 4u * 256u * 8u ? kRowNodes : 1u;
         const uint32_t wg_rowbits =
             set_map(kGmRowBits, [&](size_t i) { return div_up(uint32_t(desc_of(i).N), row_npw); });
