@@ -635,7 +635,7 @@ __global__ __launch_bounds__(2 * TP) void k_tiles(const SolveDesc *__restrict__ 
     constexpr uint32_t kNB = 2 * (TP + 1); // visiting-order buckets
     __shared__ float4 Hs[TP * (kTileRec / 4)];
     __shared__ float wS[TP * K];
-    __shared__ uint16_t ent[kEnt];
+    __shared__ __align__(8) uint16_t ent[kEnt];
     __shared__ uint32_t hist[kNB];
     // slot list starts sst[slot_cap + 2], then the visiting order perm[slot_cap]
     extern __shared__ uint16_t slot_lds[];
@@ -696,24 +696,38 @@ __global__ __launch_bounds__(2 * TP) void k_tiles(const SolveDesc *__restrict__ 
     // store, one round trip for all of them
     if (tid >= uint32_t(TP))
     {
-        constexpr int kPer = kEnt / TP;
+        // four entries a 64-bit word: word w of the list takes its entries
+        // from the aligned source words w and w + 1 when the list starts off a
+        // word boundary (h entries in); the second is read only when it holds
+        // an entry of the list, and the words hold no entry past nc - 1 that
+        // is ever read back
+        static_assert(kEnt % 4 == 0, "whole words");
+        constexpr int kPerW = (kEnt / 4 + TP - 1) / TP; // w < nw <= kEnt / 4 bounds each
         const uint32_t u = tid - uint32_t(TP);
-        uint16_t cv[kPer];
+        const uint32_t h = c0 & 3u, nw = (nc + 3u) / 4u;
+        const uint64_t *__restrict__ sw = reinterpret_cast<const uint64_t *>(d.tc_contrib + (c0 - h));
+        uint64_t cv[kPerW];
 #pragma unroll
-        for (int j = 0; j < kPer; ++j)
+        for (int j = 0; j < kPerW; ++j)
         {
-            const uint32_t i = u + uint32_t(j) * TP;
-            cv[j] = i < nc ? d.tc_contrib[c0 + i] : uint16_t(0);
+            const uint32_t w = u + uint32_t(j) * TP;
+            uint64_t lo = 0, hi = 0;
+            if (w < nw)
+                lo = sw[w];
+            if (w < nw && h != 0 && 4u * w + 4u - h < nc)
+                hi = sw[w + 1];
+            cv[j] = h == 0 ? lo : (lo >> (16u * h)) | (hi << (64u - 16u * h));
         }
         uint32_t sv = 0;
         if (u <= ns)
             sv = d.tc_slot_first[s0 + u];
+        uint64_t *__restrict__ ew = reinterpret_cast<uint64_t *>(ent);
 #pragma unroll
-        for (int j = 0; j < kPer; ++j)
+        for (int j = 0; j < kPerW; ++j)
         {
-            const uint32_t i = u + uint32_t(j) * TP;
-            if (i < nc)
-                ent[i] = cv[j];
+            const uint32_t w = u + uint32_t(j) * TP;
+            if (w < nw)
+                ew[w] = cv[j];
         }
         if (u <= ns)
             sst[u] = uint16_t(sv - c0);
