@@ -634,7 +634,7 @@ __global__ __launch_bounds__(2 * TP) void k_tiles(const SolveDesc *__restrict__ 
     constexpr int kEnt = G::kEnt;
     constexpr uint32_t kNB = 2 * (TP + 1); // visiting-order buckets
     __shared__ float4 Hs[TP * (kTileRec / 4)];
-    __shared__ float wS[TP * K];
+    __shared__ __align__(16) float wS[TP * K];
     __shared__ __align__(8) uint16_t ent[kEnt];
     __shared__ uint32_t hist[kNB];
     // slot list starts sst[slot_cap + 2], then the visiting order perm[slot_cap]
@@ -661,9 +661,14 @@ __global__ __launch_bounds__(2 * TP) void k_tiles(const SolveDesc *__restrict__ 
             cost = dense_pixel_hg<K>(d, p, parity, hg); // the record, built here instead of read
         else
             load_hg<JT>(d.jr, p, hg);
+        // K weights a pixel, 16-byte aligned (the compaction stores them as float4)
+        const float4 *__restrict__ w4 = reinterpret_cast<const float4 *>(d.gpw + size_t(p) * K);
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-            wk[k] = d.gpw[size_t(p) * K + k];
+        for (int q = 0; q < K / 4; ++q)
+        {
+            const float4 v = w4[q];
+            wk[4 * q] = v.x, wk[4 * q + 1] = v.y, wk[4 * q + 2] = v.z, wk[4 * q + 3] = v.w;
+        }
     }
     if constexpr (FUSED)
     {
@@ -685,8 +690,9 @@ __global__ __launch_bounds__(2 * TP) void k_tiles(const SolveDesc *__restrict__ 
         for (int v = 0; v < kTileRec / 4; ++v)
             Hs[tid * (kTileRec / 4) + v] = make_float4(hg[4 * v], hg[4 * v + 1], hg[4 * v + 2], hg[4 * v + 3]);
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-            wS[tid * K + k] = wk[k];
+        for (int q = 0; q < K / 4; ++q)
+            reinterpret_cast<float4 *>(wS)[tid * (K / 4) + q] =
+                make_float4(wk[4 * q], wk[4 * q + 1], wk[4 * q + 2], wk[4 * q + 3]);
     }
     if (ns > slot_cap)
         return; // never taken: slot_cap is the largest slot count of the launch's tiles
